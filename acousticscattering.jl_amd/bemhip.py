@@ -1,0 +1,604 @@
+"""Host-side mirror of the reference's full-3D BEM API over the libbemhip C ABI.
+
+Names, argument meaning and error behaviour follow brandynlucca/AcousticScattering.jl
+(paths relative to that repository):
+
+  bem(surface, boundary, k; incidence_angle=pi/2, ...)   src/solvers/bem/surface.jl:10-22
+  solve_full_bem(boundary, k, quad; ...)                  src/solvers/bem/full.jl:52-66,273-287
+  scattering_amplitude(sol; direction)                     src/solvers/bem/postprocessing.jl:1-13
+  target_strength(f) = 20 log10 |f|                        src/postprocessing/target_strength.jl:8
+  far_field(quad, xhat, k, p, dp)                          src/solvers/bem/full.jl:509-519
+  frequency_sweep(solve, frequencies, c)                   src/postprocessing/sweeps.jl:103-110
+  incidence_angle_sweep(surface, boundary, k, angles)      src/postprocessing/sweeps.jl:626-701
+  mesh(nodes, triangles)                                   src/geometry/surface.jl:271-298
+  Rigid / PressureRelease / FluidFilled(g, h)              src/core/materials.jl:16,23,62-72
+
+Every solve runs on the GPU through libbemhip.so; there is no CPU path.
+Importing this module fails loudly when the library has not been built.
+"""
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbemhip.so")
+
+BEM_OK, BEM_ERR_ARGUMENT, BEM_ERR_DIMENSION, BEM_ERR_DEVICE = 0, 1, 2, 3
+BEM_ERR_ADMISSION, BEM_ERR_NUMERIC, BEM_ERR_STATE = 4, 5, 6
+BEM_RIGID, BEM_PRESSURE_RELEASE, BEM_FLUID = 0, 1, 2
+BEM_BURTON_MILLER, BEM_CBIE, BEM_MULLER = 0, 1, 2
+BEM_DOUBLE, BEM_MIXED = 0, 1
+REASONS = ["none", "conversion", "single_singular", "condition", "nonfinite",
+           "iteration_limit", "stagnation"]
+
+#: Every symbol declared in include/bemhip.h.
+EXPORTED = [
+    "bem_version", "bem_problem_defaults", "bem_ctx_create", "bem_ctx_destroy", "bem_last_error",
+    "bem_ctx_memory", "bem_ctx_set_profiling", "bem_ctx_kernel_stats", "bem_ctx_reset_stats",
+    "bem_icosphere_size", "bem_icosphere", "bem_revolution_size", "bem_revolution",
+    "bem_mesh_create", "bem_mesh_validate", "bem_mesh_destroy", "bem_mesh_info", "bem_mesh_panels",
+    "bem_mesh_quadrature", "bem_mesh_near_count", "bem_operator_create", "bem_operator_destroy",
+    "bem_operator_matrix", "bem_operator_size", "bem_operator_solve", "bem_solve",
+    "bem_solution_destroy", "bem_solution_count", "bem_solution_traces", "bem_far_field",
+    "bem_solution_diagnostics", "bem_sweep",
+]
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("boundary", ctypes.c_int32), ("formulation", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("equilibrate", ctypes.c_int32),
+                ("k", ctypes.c_double), ("density_contrast", ctypes.c_double),
+                ("soundspeed_contrast", ctypes.c_double), ("refine_tolerance", ctypes.c_double),
+                ("refine_maxiter", ctypes.c_int32), ("force_promotion", ctypes.c_int32),
+                ("condition_guard", ctypes.c_double)]
+
+
+class Diagnostics(ctypes.Structure):
+    _fields_ = [("unknown_count", ctypes.c_int64), ("quadrature_nodes", ctypes.c_int64),
+                ("absolute_residual", ctypes.c_double), ("relative_residual", ctypes.c_double),
+                ("scaled_absolute_residual", ctypes.c_double),
+                ("scaled_relative_residual", ctypes.c_double),
+                ("factor_precision", ctypes.c_int32), ("refinement_reason", ctypes.c_int32),
+                ("refinement_iterations", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("normwise_backward_error", ctypes.c_double),
+                ("componentwise_backward_error", ctypes.c_double),
+                ("single_reciprocal_condition", ctypes.c_double),
+                ("assembly_ms", ctypes.c_double), ("factor_ms", ctypes.c_double),
+                ("solve_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["factor_precision"] = "double" if self.factor_precision == 0 else "single"
+        d["refinement_reason"] = REASONS[self.refinement_reason]
+        d["converged"] = bool(self.converged)
+        return d
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+_LIB = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libbemhip.so (raises when it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise ImportError(f"libbemhip.so not built at {path}; run __graft_entry__.build()")
+    L = ctypes.CDLL(path)
+    sig = {
+        "bem_version": (ctypes.c_char_p, []),
+        "bem_problem_defaults": (None, [ctypes.POINTER(Problem), ctypes.c_int32]),
+        "bem_ctx_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+        "bem_ctx_destroy": (ctypes.c_int, [_vp]),
+        "bem_last_error": (ctypes.c_char_p, [_vp]),
+        "bem_ctx_memory": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+        "bem_ctx_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int32]),
+        "bem_ctx_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip, _dp, _dp, _dp]),
+        "bem_ctx_reset_stats": (ctypes.c_int, [_vp]),
+        "bem_icosphere_size": (ctypes.c_int, [ctypes.c_int32, _ip, _ip]),
+        "bem_icosphere": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, _dp, _ip]),
+        "bem_revolution_size": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _ip, _ip]),
+        "bem_revolution": (ctypes.c_int, [ctypes.c_int32, _dp, _dp, ctypes.c_int32, _dp, _ip]),
+        "bem_mesh_create": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, ctypes.c_int64, _ip, ctypes.POINTER(_vp)]),
+        "bem_mesh_validate": (ctypes.c_int, [ctypes.c_int64, _dp, ctypes.c_int64, _ip, _dp,
+                                             ctypes.c_char_p, ctypes.c_int32]),
+        "bem_mesh_destroy": (ctypes.c_int, [_vp]),
+        "bem_mesh_info": (ctypes.c_int, [_vp, _ip, _ip, _dp, _dp]),
+        "bem_mesh_panels": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+        "bem_mesh_quadrature": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+        "bem_mesh_near_count": (ctypes.c_int, [_vp, _ip]),
+        "bem_operator_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Problem), ctypes.POINTER(_vp)]),
+        "bem_operator_destroy": (ctypes.c_int, [_vp]),
+        "bem_operator_matrix": (ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
+        "bem_operator_size": (ctypes.c_int, [_vp, _ip]),
+        "bem_operator_solve": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.POINTER(_vp)]),
+        "bem_solve": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Problem), ctypes.c_double,
+                                     ctypes.c_double, ctypes.POINTER(_vp)]),
+        "bem_solution_destroy": (ctypes.c_int, [_vp]),
+        "bem_solution_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), _ip]),
+        "bem_solution_traces": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
+        "bem_far_field": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _dp, _dp]),
+        "bem_solution_diagnostics": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(Diagnostics)]),
+        "bem_sweep": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(_vp), ctypes.POINTER(Problem),
+                                     _dp, _dp, _dp, ctypes.POINTER(Diagnostics)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class DimensionMismatch(ValueError):
+    pass
+
+
+class AdmissionError(RuntimeError):
+    pass
+
+
+def _raise(code, msg):
+    if code == BEM_ERR_ARGUMENT:
+        raise ValueError(msg)          # ArgumentError
+    if code == BEM_ERR_DIMENSION:
+        raise DimensionMismatch(msg)
+    if code == BEM_ERR_ADMISSION:
+        raise AdmissionError(msg)
+    raise RuntimeError(msg)            # ErrorException
+
+
+# ---------------------------------------------------------------------------
+# boundary conditions and bodies
+# ---------------------------------------------------------------------------
+@dataclass(frozen=True)
+class Rigid:
+    code: int = BEM_RIGID
+
+
+@dataclass(frozen=True)
+class PressureRelease:
+    code: int = BEM_PRESSURE_RELEASE
+
+
+@dataclass(frozen=True)
+class FluidFilled:
+    """g = rho_int / rho_ext, h = c_int / c_ext (src/core/materials.jl:62-72)."""
+    density_contrast: float
+    soundspeed_contrast: float
+    code: int = BEM_FLUID
+
+
+@dataclass(frozen=True)
+class Sphere:
+    radius: float
+
+
+@dataclass(frozen=True)
+class Spheroid:
+    """Semi-axis `a` along x, `b` along y and z (src/geometry/spheroid.jl:7-28)."""
+    a: float
+    b: float
+
+
+# ---------------------------------------------------------------------------
+# context
+# ---------------------------------------------------------------------------
+class Context:
+    """One GPU: a HIP stream, an HBM budget and the library's resident state."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        L = load_library()
+        h = _vp()
+        rc = L.bem_ctx_create(device, hbm_budget_bytes, ctypes.byref(h))
+        if rc != BEM_OK:
+            _raise(rc, f"bem_ctx_create failed on device {device} (code {rc})")
+        self._h = h
+        self.lib = L
+
+    def check(self, rc):
+        if rc != BEM_OK:
+            _raise(rc, self.lib.bem_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.bem_ctx_destroy(self._h)
+            self._h = None
+
+    def memory(self):
+        used, budget = ctypes.c_uint64(), ctypes.c_uint64()
+        self.check(self.lib.bem_ctx_memory(self._h, ctypes.byref(used), ctypes.byref(budget)))
+        return used.value, budget.value
+
+    def set_profiling(self, on=True):
+        self.check(self.lib.bem_ctx_set_profiling(self._h, 1 if on else 0))
+
+    def reset_stats(self):
+        self.check(self.lib.bem_ctx_reset_stats(self._h))
+
+    def kernel_stats(self, name):
+        n = ctypes.c_int64()
+        ms, fl, by = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self.check(self.lib.bem_ctx_kernel_stats(self._h, name.encode(), ctypes.byref(n),
+                                                 ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by)))
+        return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+
+
+_DEFAULT = None
+
+
+def default_context():
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = Context(0)
+    return _DEFAULT
+
+
+# ---------------------------------------------------------------------------
+# meshes
+# ---------------------------------------------------------------------------
+def icosphere_arrays(level, ax=1.0, ay=None, az=None):
+    """Level-`level` icosphere mapped by (ax, ay, az): nodes 3xNv, triangles 3xN (1-based)."""
+    L = load_library()
+    ay = ax if ay is None else ay
+    az = ay if az is None else az
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+    if L.bem_icosphere_size(level, ctypes.byref(nv), ctypes.byref(nt)) != BEM_OK:
+        raise ValueError("invalid icosphere level")
+    nodes = np.empty((nv.value, 3))
+    tri = np.empty((nt.value, 3), dtype=np.int64)
+    if L.bem_icosphere(level, ax, ay, az, _d(nodes), _i(tri)) != BEM_OK:
+        raise ValueError("invalid icosphere axes")
+    return nodes.T.copy(), tri.T.copy()
+
+
+def revolution_arrays(xs, rs, n_phi):
+    """Body of revolution about x from meridian samples (poles at both ends)."""
+    L = load_library()
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    rs = np.ascontiguousarray(rs, dtype=np.float64)
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+    if L.bem_revolution_size(len(xs), n_phi, ctypes.byref(nv), ctypes.byref(nt)) != BEM_OK:
+        raise ValueError("need at least 3 meridian samples and 3 azimuthal segments")
+    nodes = np.empty((nv.value, 3))
+    tri = np.empty((nt.value, 3), dtype=np.int64)
+    if L.bem_revolution(len(xs), _d(xs), _d(rs), n_phi, _d(nodes), _i(tri)) != BEM_OK:
+        raise ValueError("invalid meridian")
+    return nodes.T.copy(), tri.T.copy()
+
+
+def validate_surface(nodes, triangles):
+    """Topology checks of src/geometry/surface.jl:341-405 (host only); returns resolution."""
+    L = load_library()
+    nodes = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64).T)
+    tri = np.ascontiguousarray(np.asarray(triangles, dtype=np.int64).T)
+    res = ctypes.c_double()
+    buf = ctypes.create_string_buffer(256)
+    rc = L.bem_mesh_validate(nodes.shape[0], _d(nodes), tri.shape[0], _i(tri), ctypes.byref(res), buf, 256)
+    if rc != BEM_OK:
+        raise ValueError(buf.value.decode())
+    return res.value
+
+
+class Mesh:
+    """A closed triangulated surface resident on the device (`Mesh` of src/geometry/mesh.jl:39-44)."""
+
+    def __init__(self, nodes, triangles, ctx=None, body=None):
+        self.ctx = ctx or default_context()
+        nodes = np.asarray(nodes, dtype=np.float64)
+        triangles = np.asarray(triangles)
+        if nodes.ndim != 2 or nodes.shape[0] != 3:
+            raise ValueError("nodes must have three rows")
+        if triangles.ndim != 2 or triangles.shape[0] != 3:
+            raise ValueError("triangles must have 3 rows (flat panels)")
+        if not np.all(np.isfinite(nodes)):
+            raise ValueError("node coordinates must be finite")
+        self.nodes = nodes
+        self.triangles = triangles.astype(np.int64)
+        self.body = body
+        h = _vp()
+        nd = np.ascontiguousarray(nodes.T)
+        tr = np.ascontiguousarray(self.triangles.T)
+        self.ctx.check(self.ctx.lib.bem_mesh_create(self.ctx._h, nd.shape[0], _d(nd), tr.shape[0], _i(tr),
+                                                    ctypes.byref(h)))
+        self._h = h
+        nt, nv = ctypes.c_int64(), ctypes.c_int64()
+        res, vol = ctypes.c_double(), ctypes.c_double()
+        self.ctx.lib.bem_mesh_info(h, ctypes.byref(nt), ctypes.byref(nv), ctypes.byref(res), ctypes.byref(vol))
+        self.n, self.resolution, self.volume = nt.value, res.value, vol.value
+
+    def __len__(self):
+        return self.n
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self.ctx._h:
+            self.ctx.lib.bem_mesh_destroy(self._h)
+            self._h = None
+
+    def panels(self):
+        c = np.empty((self.n, 3)); nr = np.empty((self.n, 3)); a = np.empty(self.n)
+        self.ctx.check(self.ctx.lib.bem_mesh_panels(self._h, _d(c), _d(nr), _d(a)))
+        return c, nr, a
+
+    def quadrature(self):
+        """7-point Gauss nodes: list of (coords, normal, weight), the shape far_field consumes."""
+        xyz = np.empty((7 * self.n, 3)); nr = np.empty((7 * self.n, 3)); w = np.empty(7 * self.n)
+        self.ctx.check(self.ctx.lib.bem_mesh_quadrature(self._h, _d(xyz), _d(nr), _d(w)))
+        return xyz, nr, w
+
+    def near_count(self):
+        c = ctypes.c_int64()
+        self.ctx.check(self.ctx.lib.bem_mesh_near_count(self._h, ctypes.byref(c)))
+        return c.value
+
+
+def mesh(nodes_or_body, triangles=None, *, level=3, units="m", ctx=None):
+    """`mesh(nodes, triangles)` for a supplied surface (1-based triangles,
+    src/geometry/surface.jl:271-298) or `mesh(body; level)` for a Sphere /
+    Spheroid meshed as a mapped icosphere."""
+    scale = {"m": 1.0, "cm": 1e-2, "mm": 1e-3}.get(units)
+    if scale is None:
+        raise ValueError("units must be :m, :cm or :mm")
+    if triangles is not None:
+        return Mesh(np.asarray(nodes_or_body, dtype=np.float64) * scale, triangles, ctx=ctx)
+    body = nodes_or_body
+    if isinstance(body, Sphere):
+        nodes, tri = icosphere_arrays(level, body.radius * scale)
+    elif isinstance(body, Spheroid):
+        nodes, tri = icosphere_arrays(level, body.a * scale, body.b * scale, body.b * scale)
+    else:
+        raise ValueError("unsupported body")
+    return Mesh(nodes, tri, ctx=ctx, body=body)
+
+
+# ---------------------------------------------------------------------------
+# solves
+# ---------------------------------------------------------------------------
+def incidence_direction(beta, alpha=0.0):
+    """Polar angle from +x; azimuth from +y toward +z (src/solvers/bem/full.jl:15-18)."""
+    return np.array([math.cos(beta), math.sin(beta) * math.cos(alpha), math.sin(beta) * math.sin(alpha)])
+
+
+_FORMULATIONS = {"burton_miller": BEM_BURTON_MILLER, "cbie": BEM_CBIE, "muller": BEM_MULLER}
+
+
+def make_problem(boundary, k, *, formulation=None, precision="double", refinement=None,
+                 equilibrate=True, _force_promotion=False):
+    L = load_library()
+    p = Problem()
+    L.bem_problem_defaults(ctypes.byref(p), boundary.code)
+    p.k = float(k)
+    if formulation is not None:
+        if formulation not in _FORMULATIONS:
+            raise ValueError("formulation must be :cbie or :burton_miller" if boundary.code != BEM_FLUID
+                             else "fluid formulation must be :muller or :cbie")
+        p.formulation = _FORMULATIONS[formulation]
+    if isinstance(boundary, FluidFilled):
+        p.density_contrast = float(boundary.density_contrast)
+        p.soundspeed_contrast = float(boundary.soundspeed_contrast)
+    if precision not in ("double", "mixed"):
+        raise ValueError("precision must be :double or :mixed")
+    p.precision = BEM_MIXED if precision == "mixed" else BEM_DOUBLE
+    p.equilibrate = 1 if equilibrate else 0
+    refinement = dict(refinement or {})
+    if refinement and precision != "mixed":
+        raise ValueError("refinement options require precision=:mixed")
+    for key in refinement:
+        if key not in ("tolerance", "maxiter", "condition_guard"):
+            raise ValueError("unknown refinement option")
+    if "tolerance" in refinement:
+        p.refine_tolerance = float(refinement["tolerance"])
+    if "maxiter" in refinement:
+        mi = refinement["maxiter"]
+        if not isinstance(mi, int) or mi < 0:
+            raise ValueError("maxiter must be a nonnegative integer")
+        p.refine_maxiter = mi
+    if "condition_guard" in refinement:
+        p.condition_guard = float(refinement["condition_guard"])
+    p.force_promotion = 1 if _force_promotion else 0
+    return p
+
+
+class Operator:
+    """Assembled and factored system resident in HBM; solve() for many incidences."""
+
+    def __init__(self, surface, boundary, k, **kw):
+        self.mesh = surface
+        self.ctx = surface.ctx
+        self.boundary = boundary
+        self.k = float(k)
+        self.problem = make_problem(boundary, k, **kw)
+        h = _vp()
+        self.ctx.check(self.ctx.lib.bem_operator_create(self.ctx._h, surface._h, ctypes.byref(self.problem),
+                                                        ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_int64()
+        self.ctx.lib.bem_operator_size(h, ctypes.byref(n))
+        self.n = n.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self.ctx._h:
+            self.ctx.lib.bem_operator_destroy(self._h)
+            self._h = None
+
+    def matrix(self):
+        A = np.empty((self.n, self.n), dtype=np.complex128, order="F")
+        self.ctx.check(self.ctx.lib.bem_operator_matrix(self._h, A.ctypes.data_as(_dp), self.n))
+        return A
+
+    def solve(self, angles):
+        """angles: sequence of (incidence_angle, incidence_azimuth)."""
+        ang = np.ascontiguousarray(np.asarray(angles, dtype=np.float64).reshape(-1, 2))
+        h = _vp()
+        self.ctx.check(self.ctx.lib.bem_operator_solve(self._h, ang.shape[0], _d(ang), ctypes.byref(h)))
+        return _Solution(self.ctx, h, self.mesh, self.k, ang)
+
+
+class _Solution:
+    def __init__(self, ctx, h, mesh, k, angles):
+        self.ctx, self._h, self.mesh, self.k, self.angles = ctx, h, mesh, k, angles
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self.ctx._h:
+            self.ctx.lib.bem_solution_destroy(self._h)
+            self._h = None
+
+    def traces(self, r=0):
+        p = np.empty(self.mesh.n, dtype=np.complex128)
+        q = np.empty(self.mesh.n, dtype=np.complex128)
+        self.ctx.check(self.ctx.lib.bem_solution_traces(self._h, r, p.ctypes.data_as(_dp), q.ctypes.data_as(_dp)))
+        return p, q
+
+    def far_field(self, r=0, directions=None):
+        if directions is None:
+            out = np.empty(1, dtype=np.complex128)
+            self.ctx.check(self.ctx.lib.bem_far_field(self._h, r, 1, None, out.ctypes.data_as(_dp)))
+            return out
+        dirs = np.ascontiguousarray(np.atleast_2d(np.asarray(directions, dtype=np.float64)))
+        if dirs.shape[1] != 3:
+            raise ValueError("direction must be a finite unit vector with three components")
+        out = np.empty(dirs.shape[0], dtype=np.complex128)
+        self.ctx.check(self.ctx.lib.bem_far_field(self._h, r, dirs.shape[0], _d(dirs), out.ctypes.data_as(_dp)))
+        return out
+
+    def diagnostics(self, r=0):
+        d = Diagnostics()
+        self.ctx.check(self.ctx.lib.bem_solution_diagnostics(self._h, r, ctypes.byref(d)))
+        return d.as_dict()
+
+
+def solve_full_bem(boundary, k, surface, *, incidence_angle=0.0, incidence_azimuth=0.0, incident=None,
+                   formulation=None, compression=None, correction=None, gmres_kwargs=None,
+                   equilibrate=True, condition_limit=512, precision="double", refinement=None,
+                   return_diagnostics=False):
+    """Drop-in for solve_full_bem (src/solvers/bem/full.jl:52-66,273-287).
+
+    Returns (p_scat, dpdn_scat, quad[, diagnostics]) at the panel collocation
+    points.  The dense direct solve replaces the reference's GMRES for
+    rigid/soft (its converged limit); `compression`, `correction` and
+    `gmres_kwargs` are accepted for signature compatibility only.
+    """
+    if incident is not None:
+        raise ValueError("only the plain unit plane wave is supported on the device path")
+    if not (math.isfinite(incidence_angle) and math.isfinite(incidence_azimuth)):
+        raise ValueError("incidence angles must be finite")
+    if condition_limit < 0:
+        raise ValueError("condition_limit must be nonnegative")
+    op = Operator(surface, boundary, k, formulation=formulation, precision=precision,
+                  refinement=refinement, equilibrate=equilibrate)
+    sol = op.solve([(incidence_angle, incidence_azimuth)])
+    p, q = sol.traces(0)
+    if return_diagnostics:
+        d = sol.diagnostics(0)
+        d["formulation"] = {BEM_BURTON_MILLER: "burton_miller", BEM_CBIE: "cbie",
+                            BEM_MULLER: "muller"}[op.problem.formulation]
+        d["method"] = "mixed_refinement" if precision == "mixed" else "direct"
+        return p, q, surface, d
+    return p, q, surface
+
+
+@dataclass
+class BEMSolution:
+    """Result of bem(); fields follow src/solvers/bem/types.jl:2-36."""
+    body: object
+    boundary: object
+    k: float
+    method: str
+    surface: Mesh
+    p_scat: np.ndarray
+    dpdn_scat: np.ndarray
+    incidence_angle: float
+    incidence_azimuth: float
+    diagnostics: dict = field(default_factory=dict)
+    _solution: object = None
+
+
+def bem(surface, boundary, k, *, incidence_angle=math.pi / 2, incidence_azimuth=0.0, **kw):
+    """Drop-in for bem(surface::Mesh, boundary, k; ...) (src/solvers/bem/surface.jl:10-22)."""
+    if not (math.isfinite(k) and k > 0):
+        raise ValueError("k must be finite and positive")
+    op = Operator(surface, boundary, k, formulation=kw.pop("formulation", None),
+                  precision=kw.pop("precision", "double"), refinement=kw.pop("refinement", None),
+                  equilibrate=kw.pop("equilibrate", True),
+                  _force_promotion=kw.pop("_force_promotion", False))
+    for ignored in ("compression", "correction", "gmres_kwargs", "condition_limit"):
+        kw.pop(ignored, None)
+    if kw:
+        raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
+    sol = op.solve([(incidence_angle, incidence_azimuth)])
+    p, q = sol.traces(0)
+    d = sol.diagnostics(0)
+    d["unknown_count"] = op.n
+    return BEMSolution(surface.body, boundary, float(k), "full", surface, p, q, incidence_angle,
+                       incidence_azimuth, d, sol)
+
+
+def scattering_amplitude(sol, direction=None):
+    """Far-field amplitude [m] (src/solvers/bem/postprocessing.jl:1-13); default backscatter."""
+    if direction is None:
+        return complex(sol._solution.far_field(0)[0])
+    d = np.asarray(direction, dtype=np.float64)
+    if d.shape != (3,) or not np.all(np.isfinite(d)):
+        raise ValueError("direction must be a finite unit vector with three components")
+    return complex(sol._solution.far_field(0, d)[0])
+
+
+def target_strength(x, **kw):
+    """TS = 20 log10 |f| in dB re 1 m^2 (src/postprocessing/target_strength.jl:8,26-28)."""
+    if isinstance(x, BEMSolution):
+        x = scattering_amplitude(x, **kw)
+    return 20.0 * math.log10(abs(x))
+
+
+def diagnostics(sol):
+    return sol.diagnostics
+
+
+def incidence_angle_sweep(surface, boundary, k, angles, *, incidence_azimuth=0.0, **kw):
+    """Assemble and factor once, solve every incidence (src/postprocessing/sweeps.jl:626-701).
+    Returns (amplitudes, target strengths) at backscatter."""
+    op = Operator(surface, boundary, k, **kw)
+    sol = op.solve([(a, incidence_azimuth) for a in angles])
+    amps = np.array([sol.far_field(r)[0] for r in range(len(angles))])
+    return amps, 20.0 * np.log10(np.abs(amps))
+
+
+def frequency_sweep(solve, frequencies, sound_speed):
+    """Sequential map over k = 2 pi f / c (src/postprocessing/sweeps.jl:103-110)."""
+    if not (math.isfinite(sound_speed) and sound_speed > 0):
+        raise ValueError("sound_speed must be finite and positive")
+    freqs = np.asarray(frequencies, dtype=np.float64)
+    if np.any(freqs <= 0):
+        raise ValueError("frequencies must be positive")
+    ks = 2 * math.pi * freqs / sound_speed
+    amps = np.array([scattering_amplitude(solve(k)) for k in ks])
+    return {"frequencies": freqs, "k": ks, "ts": 20 * np.log10(np.abs(amps)), "amplitudes": amps}
+
+
+def batched_sweep(meshes, problems, angles, ctx=None):
+    """Run many independent solves through bem_sweep; returns (amplitudes, latency_ms, diagnostics)."""
+    ctx = ctx or meshes[0].ctx
+    n = len(meshes)
+    hs = (_vp * n)(*[m._h for m in meshes])
+    probs = (Problem * n)(*problems)
+    ang = np.ascontiguousarray(np.asarray(angles, dtype=np.float64).reshape(n, 2))
+    amps = np.empty(n, dtype=np.complex128)
+    lat = np.empty(n)
+    diags = (Diagnostics * n)()
+    ctx.check(ctx.lib.bem_sweep(ctx._h, n, hs, probs, _d(ang), amps.ctypes.data_as(_dp), _d(lat), diags))
+    return amps, lat, [d.as_dict() for d in diags]

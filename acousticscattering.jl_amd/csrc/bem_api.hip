@@ -1,0 +1,1172 @@
+// C ABI of the MI355X BEM library: contexts, meshes, operators (assembled +
+// factored systems resident in HBM), solutions, refinement and sweeps.
+//
+// Reference behaviour mirrored here (paths relative to the reference repo):
+//   solve_full_bem drivers           src/solvers/bem/full.jl:52-66,130-184,273-495
+//   equilibrated LU + unscaling      src/solvers/bem/full.jl:364-440
+//   MixedRefinement state machine    src/solvers/bem/fluid_system.jl:172-309
+//   residual report                  src/numerics/diagnostics.jl:9-18
+//   far field / target strength      src/solvers/bem/full.jl:509-531
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bemhip.h"
+#include "bem_device.h"
+#include "bem_internal.h"
+
+using namespace bem;
+
+namespace {
+
+struct BemError : std::runtime_error {
+    int code;
+    BemError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                        \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            throw BemError(BEM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+const char* kClassNames[] = {"assemble", "near", "lu_panel", "lu_gemm", "lu_trsm",
+                             "lu_swap",  "solve", "residual", "far_field"};
+constexpr int kNumClasses = 9;
+enum { C_ASSEMBLE = 0, C_NEAR = 1, C_RESIDUAL = 7, C_FAR = 8 };
+
+struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+    double flops, bytes;
+};
+
+struct Prof {
+    bool on = false;
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> pool;
+    int64_t launches[kNumClasses] = {};
+    double ms[kNumClasses] = {}, flops[kNumClasses] = {}, bytes[kNumClasses] = {};
+    hipEvent_t cur = nullptr;
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void drain() {
+        for (auto& r : recs) {
+            hipEventSynchronize(r.b);
+            float t = 0;
+            hipEventElapsedTime(&t, r.a, r.b);
+            launches[r.cls] += 1;
+            ms[r.cls] += t;
+            flops[r.cls] += r.flops;
+            bytes[r.cls] += r.bytes;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        recs.clear();
+    }
+};
+
+}  // namespace
+
+namespace bem {
+void prof_begin(void* prof, hipStream_t s, int cls) {
+    Prof* p = static_cast<Prof*>(prof);
+    if (!p || !p->on) return;
+    (void)cls;
+    p->cur = p->get();
+    if (p->cur) hipEventRecord(p->cur, s);
+}
+void prof_end(void* prof, hipStream_t s, int cls, double flops, double bytes) {
+    Prof* p = static_cast<Prof*>(prof);
+    if (!p || !p->on || !p->cur) return;
+    hipEvent_t e = p->get();
+    if (!e) return;
+    hipEventRecord(e, s);
+    p->recs.push_back({cls, p->cur, e, flops, bytes});
+    p->cur = nullptr;
+    if (p->recs.size() > 4096) p->drain();
+}
+}  // namespace bem
+
+struct bem_ctx_s {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0, used = 0;
+    std::string err;
+    std::recursive_mutex mu;
+    Prof prof;
+    PanelWS ws{};
+    void* ws_mem = nullptr;
+
+    void* alloc(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        if (used + bytes > budget)
+            throw BemError(BEM_ERR_ADMISSION, "request of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
+                                                  std::to_string(used) + " of " + std::to_string(budget) + " in use)");
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) throw BemError(BEM_ERR_ADMISSION, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        used += bytes;
+        sizes[p] = bytes;
+        return p;
+    }
+    void release(void* p) {
+        if (!p) return;
+        auto it = sizes.find(p);
+        if (it != sizes.end()) {
+            used -= it->second;
+            sizes.erase(it);
+        }
+        hipFree(p);
+    }
+    std::map<void*, size_t> sizes;
+};
+
+struct bem_mesh_s {
+    bem_ctx ctx = nullptr;
+    HostMesh h;
+    DevMesh d;
+    std::vector<void*> bufs;
+    int refs = 1;
+};
+
+struct bem_operator_s {
+    bem_ctx ctx = nullptr;
+    bem_mesh mesh = nullptr;
+    bem_problem prob{};
+    int n = 0, nt = 0;
+    bool single = false;          // FP32 storage / factor
+    void* A = nullptr;            // original system, row-major, R storage
+    void* B = nullptr;            // RHS operator (rigid/soft)
+    void* LU = nullptr;           // factor in R
+    void* LU64 = nullptr;         // FP64 factor after promotion
+    int* ipiv = nullptr;
+    int* perm = nullptr;
+    void* inv = nullptr;
+    int* ipiv64 = nullptr;
+    int* perm64 = nullptr;
+    void* inv64 = nullptr;
+    double* rows = nullptr;       // equilibration (null when off)
+    double* cols = nullptr;
+    double matrix_norm = 0;
+    int precision = 1;            // 1 single factor active, 0 double
+    int reason = BEM_REASON_NONE;
+    double rcond = NAN;
+    double assembly_ms = 0, factor_ms = 0;
+    // scratch vectors
+    z64 *vb = nullptr, *vx = nullptr, *vr = nullptr, *vt = nullptr;
+    double* vden = nullptr;
+    void *wv = nullptr, *wv2 = nullptr, *wv3 = nullptr;  // R vectors
+    double* red = nullptr;
+    int* flag = nullptr;
+};
+
+struct bem_solution_s {
+    bem_ctx ctx = nullptr;
+    bem_mesh mesh = nullptr;
+    double k = 0;
+    int nrhs = 0, nt = 0;
+    std::vector<double> angles;
+    z64 *p = nullptr, *q = nullptr;  // device, nt x nrhs
+    std::vector<std::complex<double>> hp, hq;
+    std::vector<bem_diagnostics> diag;
+};
+
+namespace {
+
+template <typename F>
+int guarded(bem_ctx ctx, F&& f) {
+    try {
+        f();
+        if (ctx) ctx->err.clear();
+        return BEM_OK;
+    } catch (const BemError& e) {
+        if (ctx) ctx->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        return BEM_ERR_DEVICE;
+    }
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void mesh_release(bem_mesh m) {
+    if (!m) return;
+    if (--m->refs > 0) return;
+    for (void* p : m->bufs) m->ctx->release(p);
+    delete m;
+}
+
+void validate_problem(const bem_problem& p) {
+    if (!(std::isfinite(p.k) && p.k > 0)) throw BemError(BEM_ERR_ARGUMENT, "k must be finite and positive");
+    if (p.boundary == BEM_FLUID) {
+        if (p.formulation != BEM_MULLER && p.formulation != BEM_CBIE)
+            throw BemError(BEM_ERR_ARGUMENT, "fluid formulation must be :muller or :cbie");
+        const double g = p.density_contrast, h = p.soundspeed_contrast;
+        if (!(std::isfinite(g) && g > 0 && std::isfinite(h) && h > 0))
+            throw BemError(BEM_ERR_ARGUMENT, "fluid density and soundspeed contrasts must be finite and positive");
+    } else if (p.boundary == BEM_RIGID || p.boundary == BEM_PRESSURE_RELEASE) {
+        if (p.formulation != BEM_BURTON_MILLER && p.formulation != BEM_CBIE)
+            throw BemError(BEM_ERR_ARGUMENT, "formulation must be :cbie or :burton_miller");
+    } else {
+        throw BemError(BEM_ERR_ARGUMENT, "unknown boundary condition");
+    }
+    if (p.precision != BEM_DOUBLE && p.precision != BEM_MIXED)
+        throw BemError(BEM_ERR_ARGUMENT, "precision must be :double or :mixed");
+    if (p.precision == BEM_MIXED) {
+        if (!p.equilibrate) throw BemError(BEM_ERR_ARGUMENT, "mixed precision requires equilibrate=true");
+        if (!(std::isfinite(p.refine_tolerance) && p.refine_tolerance > 0 && p.refine_tolerance < 1))
+            throw BemError(BEM_ERR_ARGUMENT, "invalid refinement tolerance");
+        if (p.refine_maxiter < 0) throw BemError(BEM_ERR_ARGUMENT, "maxiter must be a nonnegative integer");
+    }
+}
+
+void direction(double beta, double alpha, double* d) {
+    d[0] = std::cos(beta);
+    d[1] = std::sin(beta) * std::cos(alpha);
+    d[2] = std::sin(beta) * std::sin(alpha);
+}
+
+template <typename R>
+void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) {
+    bem_ctx ctx = op->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = op->n;
+    launch_fill_int(s, ctx->ws.status, 1, INT_MAX);
+    launch_fill_int(s, ctx->ws.status + 1, 1, 0);
+    launch_fill_int(s, (int*)ctx->ws.bar, 2, 0);
+    int rc = lu_factor<R>(s, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, ctx->ws, &ctx->prof);
+    if (rc == 2) throw BemError(BEM_ERR_ADMISSION, "system too large for one cooperative panel launch");
+    if (rc != 0) throw BemError(BEM_ERR_DEVICE, "LU panel launch failed");
+    int status[2];
+    std::vector<int> hp(n);
+    HIPCHECK(hipMemcpyAsync(status, ctx->ws.status, sizeof status, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(hp.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (status[1] != 0) throw BemError(BEM_ERR_DEVICE, "LU panel barrier timed out");
+    std::vector<int> pm(n);
+    for (int i = 0; i < n; ++i) pm[i] = i;
+    for (int i = 0; i < n; ++i) {
+        int p = hp[i];
+        if (p < i || p >= n) throw BemError(BEM_ERR_DEVICE, "LU produced an invalid pivot");
+        std::swap(pm[i], pm[p]);
+    }
+    HIPCHECK(hipMemcpyAsync(perm, pm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (status[0] != INT_MAX) throw BemError(BEM_ERR_NUMERIC, "singular:" + std::to_string(status[0]));
+}
+
+void promote(bem_operator op, int reason) {
+    bem_ctx ctx = op->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = op->n;
+    if (op->LU) { ctx->release(op->LU); op->LU = nullptr; }
+    if (!op->LU64) op->LU64 = ctx->alloc(sizeof(z64) * (size_t)n * n);
+    if (!op->ipiv64) op->ipiv64 = (int*)ctx->alloc(sizeof(int) * n);
+    if (!op->perm64) op->perm64 = (int*)ctx->alloc(sizeof(int) * n);
+    if (!op->inv64) op->inv64 = ctx->alloc(sizeof(z64) * 2 * 4096 * (size_t)((n + 63) / 64));
+    if (op->single)
+        launch_scale_copy<float, double>(s, (const z32*)op->A, n, n, op->rows, op->cols, (z64*)op->LU64, n);
+    else
+        launch_scale_copy<double, double>(s, (const z64*)op->A, n, n, op->rows, op->cols, (z64*)op->LU64, n);
+    try {
+        factor_into<double>(op, op->LU64, op->ipiv64, op->perm64, op->inv64);
+    } catch (const BemError& e) {
+        if (e.code == BEM_ERR_NUMERIC) throw BemError(BEM_ERR_NUMERIC, "double-precision factorization is singular");
+        throw;
+    }
+    op->precision = 0;
+    op->reason = reason;
+}
+
+// max over the kRedBlocks partial results in op->red (stride `stride`, slot `slot`)
+double host_max(bem_operator op, int count, int stride, int slot, double* all = nullptr) {
+    std::vector<double> h((size_t)count * stride);
+    HIPCHECK(hipMemcpyAsync(h.data(), op->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, op->ctx->stream));
+    HIPCHECK(hipStreamSynchronize(op->ctx->stream));
+    if (all) {
+        for (int q = 0; q < stride; ++q) {
+            double m = 0;
+            for (int b = 0; b < count; ++b) m = std::max(m, h[(size_t)b * stride + q]);
+            all[q] = m;
+        }
+        return all[slot];
+    }
+    double m = 0;
+    for (int b = 0; b < count; ++b) {
+        double v = h[(size_t)b * stride + slot];
+        if (!(v <= m)) m = v;  // propagates inf/nan
+    }
+    return m;
+}
+
+// delta = correction(F, rhs) (fluid_system.jl:229-239).  Returns false when
+// the scaled right-hand side is non-finite (delta filled with NaN).
+bool correction(bem_operator op, const z64* rhs, z64* out, int accumulate, int* changed) {
+    bem_ctx ctx = op->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = op->n;
+    launch_max_abs_scaled(s, n, rhs, op->rows, op->red);
+    const double mag = host_max(op, kRedBlocks, 1, 0);
+    if (!std::isfinite(mag)) return false;
+    if (mag == 0) {
+        if (!accumulate) HIPCHECK(hipMemsetAsync(out, 0, sizeof(z64) * n, s));
+        return true;
+    }
+    if (op->precision == 1) {
+        launch_prep_rhs<float>(s, n, rhs, op->rows, 1.0 / mag, (z32*)op->wv);
+        lu_solve<float>(s, (const z32*)op->LU, n, n, op->perm, (const z32*)op->inv, (const z32*)op->wv,
+                        (z32*)op->wv2, (z32*)op->wv3, &ctx->prof);
+        launch_finish<float>(s, n, (const z32*)op->wv3, mag, op->cols, out, accumulate, changed);
+    } else {
+        const z64* LU = (const z64*)(op->LU64 ? op->LU64 : op->LU);
+        const int* perm = op->LU64 ? op->perm64 : op->perm;
+        const z64* inv = (const z64*)(op->LU64 ? op->inv64 : op->inv);
+        launch_prep_rhs<double>(s, n, rhs, op->rows, 1.0 / mag, (z64*)op->wv);
+        lu_solve<double>(s, LU, n, n, perm, inv, (const z64*)op->wv, (z64*)op->wv2, (z64*)op->wv3, &ctx->prof);
+        launch_finish<double>(s, n, (const z64*)op->wv3, mag, op->cols, out, accumulate, changed);
+    }
+    return true;
+}
+
+void residual(bem_operator op, const z64* x, const z64* b, z64* r, double* den) {
+    bem_ctx ctx = op->ctx;
+    prof_begin(&ctx->prof, ctx->stream, C_RESIDUAL);
+    if (op->single)
+        launch_matvec<float>(ctx->stream, (const z32*)op->A, op->n, op->n, op->n, x, r, b, den);
+    else
+        launch_matvec<double>(ctx->stream, (const z64*)op->A, op->n, op->n, op->n, x, r, b, den);
+    prof_end(&ctx->prof, ctx->stream, C_RESIDUAL, 8.0 * (double)op->n * op->n,
+             (double)op->n * op->n * (op->single ? 8.0 : 16.0));
+}
+
+struct BwdErr {
+    double nw, cw;
+};
+
+BwdErr backward_errors(bem_operator op, const z64* x, const z64* b) {
+    const int n = op->n;
+    residual(op, x, b, op->vr, op->vden);
+    launch_bwd_stats(op->ctx->stream, n, op->vr, op->vden, x, b, op->red);
+    double all[5];
+    host_max(op, kRedBlocks, 5, 0, all);
+    if (all[4] != 0) return {INFINITY, INFINITY};
+    const double rn = all[0], cw = all[1], xm = all[2], bm = all[3];
+    const double scale = op->matrix_norm * xm + bm;
+    double nw;
+    if (!std::isfinite(scale)) nw = INFINITY;
+    else if (scale == 0) nw = (rn == 0) ? 0.0 : INFINITY;
+    else nw = rn / scale;
+    return {nw, cw};
+}
+
+// solve_refined (fluid_system.jl:264-307); x in op->vx, b in op->vb
+void solve_refined(bem_operator op, bem_diagnostics& dg) {
+    bem_ctx ctx = op->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = op->n;
+    const double tol = op->prob.refine_tolerance;
+    const int maxiter = op->prob.refine_maxiter;
+    auto fresh = [&]() {
+        if (!correction(op, op->vb, op->vx, 0, nullptr)) launch_fill_double(s, (double*)op->vx, 2 * (int64_t)n, NAN);
+    };
+    fresh();
+    int corrections = 0, attempts = 0, stalls = 0;
+    double previous = INFINITY;
+    for (int guard = 0; guard < 1000; ++guard) {
+        BwdErr e = backward_errors(op, op->vx, op->vb);
+        const double metric = std::max(e.nw, e.cw);
+        if (std::isfinite(metric) && metric <= tol) {
+            dg.normwise_backward_error = e.nw;
+            dg.componentwise_backward_error = e.cw;
+            dg.refinement_iterations = corrections;
+            dg.converged = 1;
+            return;
+        }
+        stalls = (metric >= previous * 0.9) ? stalls + 1 : 0;
+        int reason = BEM_REASON_NONE;
+        if (!std::isfinite(metric)) reason = BEM_REASON_NONFINITE;
+        else if (attempts >= maxiter) reason = BEM_REASON_ITERATION_LIMIT;
+        else if (stalls >= 2) reason = BEM_REASON_STAGNATION;
+        if (reason != BEM_REASON_NONE) {
+            if (op->precision == 0)
+                throw BemError(BEM_ERR_NUMERIC, "double-precision fallback failed backward-error acceptance");
+            promote(op, reason);
+            fresh();
+            previous = INFINITY;
+            attempts = stalls = 0;
+            continue;
+        }
+        // delta = correction(F, residual); x += delta
+        launch_fill_int(s, op->flag, 1, 0);
+        if (!correction(op, op->vr, op->vx, 1, op->flag)) {
+            launch_fill_double(s, (double*)op->vx, 2 * (int64_t)n, NAN);
+        } else {
+            int changed = 0;
+            HIPCHECK(hipMemcpyAsync(&changed, op->flag, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            if (!changed) {
+                if (op->precision == 0) throw BemError(BEM_ERR_NUMERIC, "double-precision correction stagnated");
+                promote(op, BEM_REASON_STAGNATION);
+                fresh();
+                previous = INFINITY;
+                attempts = stalls = 0;
+                continue;
+            }
+        }
+        previous = metric;
+        ++corrections;
+        ++attempts;
+    }
+    throw BemError(BEM_ERR_NUMERIC, "refinement did not terminate");
+}
+
+void op_free(bem_operator op) {
+    bem_ctx c = op->ctx;
+    for (void* p : {op->A, op->B, op->LU, op->LU64, (void*)op->ipiv, (void*)op->perm, op->inv, (void*)op->ipiv64,
+                    (void*)op->perm64, op->inv64, (void*)op->rows, (void*)op->cols, (void*)op->vb, (void*)op->vx,
+                    (void*)op->vr, (void*)op->vt, (void*)op->vden, op->wv, op->wv2, op->wv3, (void*)op->red,
+                    (void*)op->flag})
+        c->release(p);
+    mesh_release(op->mesh);
+    delete op;
+}
+
+void operator_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* prob, bem_operator* out) {
+    if (!mesh || !prob || !out) throw BemError(BEM_ERR_ARGUMENT, "null argument");
+    validate_problem(*prob);
+    hipStream_t s = ctx->stream;
+    auto* op = new bem_operator_s();
+    op->ctx = ctx;
+    op->mesh = mesh;
+    mesh->refs++;
+    op->prob = *prob;
+    op->nt = mesh->d.nt;
+    const bool fluid = prob->boundary == BEM_FLUID;
+    op->n = fluid ? 2 * op->nt : op->nt;
+    op->single = prob->precision == BEM_MIXED;
+    op->precision = op->single ? 1 : 0;
+    const int n = op->n;
+    const size_t esz = op->single ? sizeof(z32) : sizeof(z64);
+    try {
+        const double t0 = now_ms();
+        op->A = ctx->alloc(esz * (size_t)n * n);
+        if (!fluid) op->B = ctx->alloc(esz * (size_t)n * n);
+        AsmParams ap;
+        ap.k = prob->k;
+        ap.g = fluid ? prob->density_contrast : 1.0;
+        ap.kint = fluid ? prob->k / prob->soundspeed_contrast : prob->k;
+        ap.ld = n;
+        void *o0 = op->A, *o1 = op->B;
+        if (fluid) {
+            ap.kind = prob->formulation == BEM_MULLER ? ASM_MULLER : ASM_FLUID_CBIE;
+        } else {
+            ap.kind = prob->formulation == BEM_BURTON_MILLER ? ASM_BM : ASM_CBIE_RS;
+            // rigid solves out0 (1/2 I - D [- aH]); soft solves out1 (S [+ a(1/2 I + K')])
+            if (prob->boundary == BEM_PRESSURE_RELEASE) std::swap(o0, o1);
+        }
+        const double evals = (double)op->nt * op->nt * 7.0 * (fluid ? 2.0 : 1.0);
+        prof_begin(&ctx->prof, s, C_ASSEMBLE);
+        if (op->single) launch_assemble<float>(s, mesh->d, ap, (z32*)o0, (z32*)o1);
+        else launch_assemble<double>(s, mesh->d, ap, (z64*)o0, (z64*)o1);
+        prof_end(&ctx->prof, s, C_ASSEMBLE, evals, (double)n * n * esz * (fluid ? 1.0 : 2.0));
+        prof_begin(&ctx->prof, s, C_NEAR);
+        if (op->single) launch_near_self<float>(s, mesh->d, ap, (z32*)o0, (z32*)o1);
+        else launch_near_self<double>(s, mesh->d, ap, (z64*)o0, (z64*)o1);
+        prof_end(&ctx->prof, s, C_NEAR, (double)mesh->d.nnear * kNearPts * (fluid ? 2.0 : 1.0), 0.0);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(s));
+        const double t1 = now_ms();
+        op->assembly_ms = t1 - t0;
+
+        // scratch
+        op->vb = (z64*)ctx->alloc(sizeof(z64) * n);
+        op->vx = (z64*)ctx->alloc(sizeof(z64) * n);
+        op->vr = (z64*)ctx->alloc(sizeof(z64) * n);
+        op->vt = (z64*)ctx->alloc(sizeof(z64) * 2 * (size_t)n);
+        op->vden = (double*)ctx->alloc(sizeof(double) * n);
+        op->wv = ctx->alloc(sizeof(z64) * n);
+        op->wv2 = ctx->alloc(sizeof(z64) * n);
+        op->wv3 = ctx->alloc(sizeof(z64) * n);
+        op->red = (double*)ctx->alloc(sizeof(double) * 5 * kRedBlocks);
+        op->flag = (int*)ctx->alloc(sizeof(int) * 4);
+        op->ipiv = (int*)ctx->alloc(sizeof(int) * n);
+        op->perm = (int*)ctx->alloc(sizeof(int) * n);
+        op->inv = ctx->alloc(esz * 2 * 4096 * (size_t)((n + 63) / 64));
+        op->LU = ctx->alloc(esz * (size_t)n * n);
+
+        // equilibration + finiteness + matrix norm
+        double* rowsum = (double*)op->vden;
+        if (prob->equilibrate) {
+            op->rows = (double*)ctx->alloc(sizeof(double) * n);
+            op->cols = (double*)ctx->alloc(sizeof(double) * n);
+        }
+        double* rows = op->rows ? op->rows : (double*)op->vt;
+        if (op->single) launch_row_stats<float>(s, (const z32*)op->A, n, n, rows, rowsum);
+        else launch_row_stats<double>(s, (const z64*)op->A, n, n, rows, rowsum);
+        {
+            std::vector<double> hr(n), hs(n);
+            HIPCHECK(hipMemcpyAsync(hr.data(), rows, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipMemcpyAsync(hs.data(), rowsum, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            double mn = 0;
+            for (int i = 0; i < n; ++i) {
+                if (!std::isfinite(hr[i])) throw BemError(BEM_ERR_ARGUMENT, "matrix must be finite");
+                if (!std::isfinite(hs[i])) throw BemError(BEM_ERR_ARGUMENT, "matrix norm overflow");
+                mn = std::max(mn, hs[i]);
+            }
+            op->matrix_norm = mn;
+        }
+        if (op->rows) {
+            launch_fix_scale(s, n, op->rows);
+            if (op->single) launch_col_max<float>(s, (const z32*)op->A, n, n, op->rows, op->cols);
+            else launch_col_max<double>(s, (const z64*)op->A, n, n, op->rows, op->cols);
+            launch_fix_scale(s, n, op->cols);
+        }
+        if (op->single) launch_scale_copy<float, float>(s, (const z32*)op->A, n, n, op->rows, op->cols, (z32*)op->LU, n);
+        else launch_scale_copy<double, double>(s, (const z64*)op->A, n, n, op->rows, op->cols, (z64*)op->LU, n);
+        try {
+            if (op->single) factor_into<float>(op, op->LU, op->ipiv, op->perm, op->inv);
+            else factor_into<double>(op, op->LU, op->ipiv, op->perm, op->inv);
+        } catch (const BemError& e) {
+            if (e.code != BEM_ERR_NUMERIC) throw;
+            if (!op->single) throw BemError(BEM_ERR_NUMERIC, "matrix is singular (zero pivot)");
+            promote(op, BEM_REASON_SINGLE_SINGULAR);
+        }
+        if (op->single && op->precision == 1 && prob->force_promotion) promote(op, BEM_REASON_CONDITION);
+        HIPCHECK(hipStreamSynchronize(s));
+        op->factor_ms = now_ms() - t1;
+    } catch (...) {
+        op_free(op);
+        throw;
+    }
+    *out = op;
+}
+
+void operator_solve(bem_operator op, int nrhs, const double* angles, bem_solution* out) {
+    if (nrhs <= 0 || !angles || !out) throw BemError(BEM_ERR_ARGUMENT, "need at least one incidence");
+    for (int r = 0; r < 2 * nrhs; ++r)
+        if (!std::isfinite(angles[r])) throw BemError(BEM_ERR_ARGUMENT, "incidence angles must be finite");
+    bem_ctx ctx = op->ctx;
+    hipStream_t s = ctx->stream;
+    const int nt = op->nt, n = op->n;
+    const bem_problem& pr = op->prob;
+    const bool fluid = pr.boundary == BEM_FLUID;
+    auto* sol = new bem_solution_s();
+    sol->ctx = ctx;
+    sol->mesh = op->mesh;
+    op->mesh->refs++;
+    sol->k = pr.k;
+    sol->nrhs = nrhs;
+    sol->nt = nt;
+    sol->angles.assign(angles, angles + 2 * nrhs);
+    sol->diag.resize(nrhs);
+    double* ddir = nullptr;
+    try {
+        sol->p = (z64*)ctx->alloc(sizeof(z64) * (size_t)nt * nrhs);
+        sol->q = (z64*)ctx->alloc(sizeof(z64) * (size_t)nt * nrhs);
+        std::vector<double> dirs(3 * nrhs);
+        for (int r = 0; r < nrhs; ++r) direction(angles[2 * r], angles[2 * r + 1], &dirs[3 * r]);
+        ddir = (double*)ctx->alloc(sizeof(double) * 3 * nrhs);
+        HIPCHECK(hipMemcpyAsync(ddir, dirs.data(), sizeof(double) * 3 * nrhs, hipMemcpyHostToDevice, s));
+        z64* pinc = op->vt;        // nt
+        z64* qinc = op->vt + nt;   // nt
+        for (int r = 0; r < nrhs; ++r) {
+            const double t0 = now_ms();
+            bem_diagnostics& dg = sol->diag[r];
+            std::memset(&dg, 0, sizeof dg);
+            dg.unknown_count = n;
+            dg.quadrature_nodes = nt;
+            launch_incident(s, op->mesh->d, pr.k, 1, ddir + 3 * r, pinc, qinc);
+            // right-hand side b (n) in op->vb
+            if (fluid && pr.formulation == BEM_MULLER) {
+                HIPCHECK(hipMemcpyAsync(op->vb, pinc, sizeof(z64) * nt, hipMemcpyDeviceToDevice, s));
+                HIPCHECK(hipMemcpyAsync(op->vb + nt, qinc, sizeof(z64) * nt, hipMemcpyDeviceToDevice, s));
+            } else if (fluid) {
+                // [0; -A[N+1:2N, :] * [p_inc; q_inc]]   (full.jl:448-453)
+                HIPCHECK(hipMemsetAsync(op->vb, 0, sizeof(z64) * nt, s));
+                if (op->single)
+                    launch_matvec<float>(s, (const z32*)op->A + (size_t)nt * n, n, nt, n, op->vt, op->vr, nullptr, nullptr);
+                else
+                    launch_matvec<double>(s, (const z64*)op->A + (size_t)nt * n, n, nt, n, op->vt, op->vr, nullptr, nullptr);
+                launch_copy_z(s, nt, op->vr, op->vb + nt, -1.0, nullptr);
+            } else {
+                const z64* src = pr.boundary == BEM_RIGID ? qinc : pinc;
+                if (op->single) launch_matvec<float>(s, (const z32*)op->B, n, n, n, src, op->vb, nullptr, nullptr);
+                else launch_matvec<double>(s, (const z64*)op->B, n, n, n, src, op->vb, nullptr, nullptr);
+            }
+            if (pr.precision == BEM_MIXED) {
+                solve_refined(op, dg);
+            } else {
+                if (!correction(op, op->vb, op->vx, 0, nullptr))
+                    throw BemError(BEM_ERR_ARGUMENT, "right-hand side must be finite");
+                dg.converged = 1;
+            }
+            dg.factor_precision = op->precision;
+            dg.refinement_reason = op->reason;
+            dg.single_reciprocal_condition = op->rcond;
+            // residual report on the original equations: r = b - A x
+            residual(op, op->vx, op->vb, op->vr, nullptr);
+            launch_norm_stats(s, n, op->vr, op->vb, op->rows, op->red);
+            {
+                std::vector<double> h(4 * kRedBlocks);
+                HIPCHECK(hipMemcpyAsync(h.data(), op->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
+                double ss[4] = {0, 0, 0, 0};
+                for (int b = 0; b < kRedBlocks; ++b)
+                    for (int q = 0; q < 4; ++q) ss[q] += h[4 * b + q];
+                dg.absolute_residual = std::sqrt(ss[0]);
+                const double bn = std::sqrt(ss[1]);
+                dg.relative_residual = bn == 0 ? (dg.absolute_residual == 0 ? 0.0 : INFINITY) : dg.absolute_residual / bn;
+                dg.scaled_absolute_residual = std::sqrt(ss[2]);
+                const double sbn = std::sqrt(ss[3]);
+                dg.scaled_relative_residual =
+                    sbn == 0 ? (dg.scaled_absolute_residual == 0 ? 0.0 : INFINITY) : dg.scaled_absolute_residual / sbn;
+            }
+            // traces (full.jl:160-161,472-477)
+            z64* P = sol->p + (size_t)r * nt;
+            z64* Q = sol->q + (size_t)r * nt;
+            if (fluid) {
+                const bool mul = pr.formulation == BEM_MULLER;
+                launch_copy_z(s, nt, op->vx, P, 1.0, mul ? pinc : nullptr);
+                launch_copy_z(s, nt, op->vx + nt, Q, 1.0, mul ? qinc : nullptr);
+            } else if (pr.boundary == BEM_RIGID) {
+                launch_copy_z(s, nt, op->vx, P, 1.0, nullptr);
+                launch_copy_z(s, nt, qinc, Q, -1.0, nullptr);
+            } else {
+                launch_copy_z(s, nt, pinc, P, -1.0, nullptr);
+                launch_copy_z(s, nt, op->vx, Q, 1.0, nullptr);
+            }
+            HIPCHECK(hipStreamSynchronize(s));
+            dg.assembly_ms = op->assembly_ms;
+            dg.factor_ms = op->factor_ms;
+            dg.solve_ms = now_ms() - t0;
+        }
+        sol->hp.resize((size_t)nt * nrhs);
+        sol->hq.resize((size_t)nt * nrhs);
+        HIPCHECK(hipMemcpyAsync(sol->hp.data(), sol->p, sizeof(z64) * nt * nrhs, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(sol->hq.data(), sol->q, sizeof(z64) * nt * nrhs, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        ctx->release(ddir);
+    } catch (...) {
+        ctx->release(ddir);
+        ctx->release(sol->p);
+        ctx->release(sol->q);
+        mesh_release(sol->mesh);
+        delete sol;
+        throw;
+    }
+    *out = sol;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* bem_version(void) { return "bemhip 0.1.0 (gfx950)"; }
+
+void bem_problem_defaults(bem_problem* p, int32_t bc) {
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->boundary = bc;
+    p->formulation = bc == BEM_FLUID ? BEM_MULLER : BEM_BURTON_MILLER;
+    p->precision = BEM_DOUBLE;
+    p->equilibrate = 1;
+    p->k = 1.0;
+    p->density_contrast = 1.0;
+    p->soundspeed_contrast = 1.0;
+    p->refine_tolerance = 1e-13;
+    p->refine_maxiter = 8;
+    p->force_promotion = 0;
+    p->condition_guard = 10.0 * 1.1920928955078125e-07;
+}
+
+int bem_ctx_create(int32_t device, uint64_t budget, bem_ctx* out) {
+    if (!out) return BEM_ERR_ARGUMENT;
+    *out = nullptr;
+    auto* c = new bem_ctx_s();
+    int rc = guarded(c, [&] {
+        int count = 0;
+        HIPCHECK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw BemError(BEM_ERR_ARGUMENT, "invalid device");
+        HIPCHECK(hipSetDevice(device));
+        c->device = device;
+        hipDeviceProp_t prop;
+        HIPCHECK(hipGetDeviceProperties(&prop, device));
+        c->cu_count = prop.multiProcessorCount;
+        size_t fr = 0, tot = 0;
+        HIPCHECK(hipMemGetInfo(&fr, &tot));
+        c->budget = budget ? budget : (uint64_t)(0.9 * (double)fr);
+        HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        if (upload_rules() != 0) throw BemError(BEM_ERR_DEVICE, "constant table upload failed");
+        const int G = panel_capacity(c->cu_count);
+        if (G <= 0) throw BemError(BEM_ERR_DEVICE, "cooperative panel kernel cannot be resident");
+        c->ws.max_blocks = G;
+        size_t bytes = sizeof(double) * 2 * G + sizeof(int) * 2 * G + sizeof(z64) * 2 * (size_t)G * 64 +
+                       sizeof(z64) * 2 * 64 + 64 + 64;
+        char* m = (char*)c->alloc(bytes + 1024);
+        c->ws_mem = m;
+        HIPCHECK(hipMemset(m, 0, bytes + 1024));
+        size_t off = 0;
+        auto take = [&](size_t b) { void* p = m + off; off += (b + 255) & ~(size_t)255; return p; };
+        c->ws.cand_val = (double*)take(sizeof(double) * 2 * G);
+        c->ws.cand_idx = (int*)take(sizeof(int) * 2 * G);
+        c->ws.cand_row = take(sizeof(z64) * 2 * (size_t)G * 64);
+        c->ws.diag_row = take(sizeof(z64) * 2 * 64);
+        c->ws.bar = (unsigned*)take(64);
+        c->ws.status = (int*)take(64);
+        if (off > bytes + 1024) throw BemError(BEM_ERR_DEVICE, "workspace layout");
+        HIPCHECK(hipDeviceSynchronize());
+    });
+    if (rc != BEM_OK) {
+        std::fprintf(stderr, "bem_ctx_create: %s\n", c->err.c_str());
+        if (c->stream) hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return BEM_OK;
+}
+
+int bem_ctx_destroy(bem_ctx ctx) {
+    if (!ctx) return BEM_ERR_ARGUMENT;
+    hipStreamSynchronize(ctx->stream);
+    ctx->prof.drain();
+    for (auto e : ctx->prof.pool) hipEventDestroy(e);
+    ctx->release(ctx->ws_mem);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return BEM_OK;
+}
+
+const char* bem_last_error(bem_ctx ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int bem_ctx_memory(bem_ctx ctx, uint64_t* used, uint64_t* budget) {
+    if (!ctx) return BEM_ERR_ARGUMENT;
+    if (used) *used = ctx->used;
+    if (budget) *budget = ctx->budget;
+    return BEM_OK;
+}
+
+int bem_ctx_set_profiling(bem_ctx ctx, int32_t enable) {
+    if (!ctx) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    ctx->prof.on = enable != 0;
+    return BEM_OK;
+}
+
+int bem_ctx_kernel_stats(bem_ctx ctx, const char* name, int64_t* launches, double* total_ms, double* flops,
+                         double* bytes) {
+    if (!ctx || !name) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    ctx->prof.drain();
+    for (int c = 0; c < kNumClasses; ++c)
+        if (std::strcmp(name, kClassNames[c]) == 0) {
+            if (launches) *launches = ctx->prof.launches[c];
+            if (total_ms) *total_ms = ctx->prof.ms[c];
+            if (flops) *flops = ctx->prof.flops[c];
+            if (bytes) *bytes = ctx->prof.bytes[c];
+            return BEM_OK;
+        }
+    ctx->err = std::string("unknown kernel class ") + name;
+    return BEM_ERR_ARGUMENT;
+}
+
+int bem_ctx_reset_stats(bem_ctx ctx) {
+    if (!ctx) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    ctx->prof.drain();
+    for (int c = 0; c < kNumClasses; ++c) {
+        ctx->prof.launches[c] = 0;
+        ctx->prof.ms[c] = ctx->prof.flops[c] = ctx->prof.bytes[c] = 0;
+    }
+    return BEM_OK;
+}
+
+int bem_icosphere_size(int32_t level, int64_t* nv, int64_t* nt) {
+    if (level < 0 || level > 9) return BEM_ERR_ARGUMENT;
+    int64_t f = 20;
+    for (int i = 0; i < level; ++i) f *= 4;
+    if (nt) *nt = f;
+    if (nv) *nv = f / 2 + 2;
+    return BEM_OK;
+}
+
+int bem_icosphere(int32_t level, double ax, double ay, double az, double* nodes, int64_t* tri) {
+    if (level < 0 || level > 9 || !nodes || !tri) return BEM_ERR_ARGUMENT;
+    if (!(ax > 0 && ay > 0 && az > 0)) return BEM_ERR_ARGUMENT;
+    std::vector<double> v;
+    std::vector<int64_t> f;
+    icosphere(level, ax, ay, az, v, f);
+    std::memcpy(nodes, v.data(), sizeof(double) * v.size());
+    for (size_t i = 0; i < f.size(); ++i) tri[i] = f[i] + 1;
+    return BEM_OK;
+}
+
+int bem_revolution_size(int32_t ns, int32_t nphi, int64_t* nv, int64_t* nt) {
+    if (ns < 3 || nphi < 3) return BEM_ERR_ARGUMENT;
+    if (nv) *nv = (int64_t)(ns - 2) * nphi + 2;
+    if (nt) *nt = 2 * (int64_t)(ns - 2) * nphi;
+    return BEM_OK;
+}
+
+// Rings s = 1..ns-2 with n_phi nodes each (phi_m = 2 pi m / n_phi, node at
+// (x_s, r_s cos phi, r_s sin phi)), pole nodes at s = 0 (first) and s = ns-1
+// (last).  Faces are ordered outward for a meridian running from -x to +x.
+int bem_revolution(int32_t ns, const double* xs, const double* rs, int32_t nphi, double* nodes, int64_t* tri) {
+    if (ns < 3 || nphi < 3 || !xs || !rs || !nodes || !tri) return BEM_ERR_ARGUMENT;
+    const int rings = ns - 2;
+    int64_t v = 0;
+    nodes[3 * v] = xs[0]; nodes[3 * v + 1] = 0; nodes[3 * v + 2] = 0; ++v;
+    for (int s = 1; s <= rings; ++s)
+        for (int m = 0; m < nphi; ++m) {
+            const double ph = 2.0 * M_PI * m / nphi;
+            nodes[3 * v] = xs[s];
+            nodes[3 * v + 1] = rs[s] * std::cos(ph);
+            nodes[3 * v + 2] = rs[s] * std::sin(ph);
+            ++v;
+        }
+    nodes[3 * v] = xs[ns - 1]; nodes[3 * v + 1] = 0; nodes[3 * v + 2] = 0;
+    const int64_t last = v;
+    auto ring = [&](int s, int m) -> int64_t { return 1 + (int64_t)(s - 1) * nphi + ((m % nphi) + nphi) % nphi; };
+    int64_t t = 0;
+    auto put = [&](int64_t a, int64_t b, int64_t c) {
+        tri[3 * t] = a + 1; tri[3 * t + 1] = b + 1; tri[3 * t + 2] = c + 1; ++t;
+    };
+    for (int m = 0; m < nphi; ++m) put(0, ring(1, m + 1), ring(1, m));
+    for (int s = 1; s < rings; ++s)
+        for (int m = 0; m < nphi; ++m) {
+            put(ring(s, m), ring(s, m + 1), ring(s + 1, m + 1));
+            put(ring(s, m), ring(s + 1, m + 1), ring(s + 1, m));
+        }
+    for (int m = 0; m < nphi; ++m) put(last, ring(rings, m), ring(rings, m + 1));
+    return BEM_OK;
+}
+
+int bem_mesh_validate(int64_t nv, const double* nodes, int64_t nt, const int64_t* tri, double* resolution,
+                      char* message, int32_t mlen) {
+    if (!nodes || !tri || nv <= 0) {
+        if (message && mlen > 0) std::snprintf(message, mlen, "invalid mesh arrays");
+        return BEM_ERR_ARGUMENT;
+    }
+    std::vector<int64_t> t0(3 * (size_t)std::max<int64_t>(nt, 0));
+    for (size_t i = 0; i < t0.size(); ++i) t0[i] = tri[i] - 1;
+    std::string msg;
+    double vol = 0;
+    int code = validate_topology(nv, nodes, nt, t0.data(), resolution, &vol, msg);
+    if (message && mlen > 0) std::snprintf(message, mlen, "%s", msg.c_str());
+    return code ? BEM_ERR_ARGUMENT : BEM_OK;
+}
+
+int bem_mesh_create(bem_ctx ctx, int64_t nv, const double* nodes, int64_t nt, const int64_t* tri, bem_mesh* out) {
+    if (!ctx || !out) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        if (!nodes || !tri || nv <= 0 || nt <= 0) throw BemError(BEM_ERR_ARGUMENT, "surface must contain triangles");
+        if (nt > (int64_t)INT_MAX / 2) throw BemError(BEM_ERR_ARGUMENT, "too many triangles");
+        auto* m = new bem_mesh_s();
+        m->ctx = ctx;
+        HostMesh& h = m->h;
+        h.nv = nv;
+        h.nt = nt;
+        h.nodes.assign(nodes, nodes + 3 * nv);
+        h.tri.resize(3 * nt);
+        for (int64_t i = 0; i < 3 * nt; ++i) h.tri[i] = tri[i] - 1;
+        std::string msg;
+        int code = validate_topology(nv, h.nodes.data(), nt, h.tri.data(), &h.resolution, &h.volume, msg);
+        if (code) {
+            delete m;
+            throw BemError(BEM_ERR_ARGUMENT, msg);
+        }
+        build_geometry(h);
+        build_near_list(h);
+        try {
+            hipStream_t s = ctx->stream;
+            DevMesh& d = m->d;
+            d.nt = (int)nt;
+            auto up = [&](const void* src, size_t bytes) {
+                void* p = ctx->alloc(bytes);
+                m->bufs.push_back(p);
+                HIPCHECK(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s));
+                return p;
+            };
+            std::vector<double> cen(3 * nt), nrm(3 * nt), gp(21 * nt);
+            for (int64_t j = 0; j < nt; ++j)
+                for (int c = 0; c < 3; ++c) {
+                    cen[c * nt + j] = h.cen[3 * j + c];
+                    nrm[c * nt + j] = h.nrm[3 * j + c];
+                }
+            for (int64_t j = 0; j < nt; ++j)
+                for (int q = 0; q < 7; ++q)
+                    for (int c = 0; c < 3; ++c) gp[(3 * q + c) * nt + j] = h.gauss[3 * (7 * j + q) + c];
+            std::vector<float> cen32(cen.begin(), cen.end()), nrm32(nrm.begin(), nrm.end()),
+                gp32(gp.begin(), gp.end()), area32(h.area.begin(), h.area.end());
+            d.cen = (double*)up(cen.data(), sizeof(double) * cen.size());
+            d.nrm = (double*)up(nrm.data(), sizeof(double) * nrm.size());
+            d.area = (double*)up(h.area.data(), sizeof(double) * nt);
+            d.gp = (double*)up(gp.data(), sizeof(double) * gp.size());
+            d.vert = (double*)up(h.vert.data(), sizeof(double) * h.vert.size());
+            d.cen32 = (float*)up(cen32.data(), sizeof(float) * cen32.size());
+            d.nrm32 = (float*)up(nrm32.data(), sizeof(float) * nrm32.size());
+            d.area32 = (float*)up(area32.data(), sizeof(float) * nt);
+            d.gp32 = (float*)up(gp32.data(), sizeof(float) * gp32.size());
+            std::vector<int32_t> rowof(h.near_col.size());
+            for (int64_t i = 0; i < nt; ++i)
+                for (int32_t e = h.near_ptr[i]; e < h.near_ptr[i + 1]; ++e) rowof[e] = (int32_t)i;
+            d.nnear = (int64_t)h.near_col.size();
+            d.near_ptr = (int*)up(h.near_ptr.data(), sizeof(int32_t) * h.near_ptr.size());
+            d.near_col = (int*)up(h.near_col.data(), sizeof(int32_t) * std::max<size_t>(1, h.near_col.size()));
+            d.near_row = (int*)up(rowof.data(), sizeof(int32_t) * std::max<size_t>(1, rowof.size()));
+            HIPCHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            for (void* p : m->bufs) ctx->release(p);
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int bem_mesh_destroy(bem_mesh mesh) {
+    if (!mesh) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(mesh->ctx->mu);
+    mesh_release(mesh);
+    return BEM_OK;
+}
+
+int bem_mesh_info(bem_mesh m, int64_t* nt, int64_t* nv, double* res, double* vol) {
+    if (!m) return BEM_ERR_ARGUMENT;
+    if (nt) *nt = m->h.nt;
+    if (nv) *nv = m->h.nv;
+    if (res) *res = m->h.resolution;
+    if (vol) *vol = m->h.volume;
+    return BEM_OK;
+}
+
+int bem_mesh_panels(bem_mesh m, double* c, double* n, double* a) {
+    if (!m) return BEM_ERR_ARGUMENT;
+    if (c) std::memcpy(c, m->h.cen.data(), sizeof(double) * m->h.cen.size());
+    if (n) std::memcpy(n, m->h.nrm.data(), sizeof(double) * m->h.nrm.size());
+    if (a) std::memcpy(a, m->h.area.data(), sizeof(double) * m->h.area.size());
+    return BEM_OK;
+}
+
+int bem_mesh_quadrature(bem_mesh m, double* coords, double* normals, double* weights) {
+    if (!m) return BEM_ERR_ARGUMENT;
+    double l[7][3], w[7];
+    q7_rule(l, w);
+    const int64_t nt = m->h.nt;
+    for (int64_t j = 0; j < nt; ++j)
+        for (int q = 0; q < 7; ++q) {
+            const int64_t p = 7 * j + q;
+            for (int c = 0; c < 3; ++c) {
+                if (coords) coords[3 * p + c] = m->h.gauss[3 * p + c];
+                if (normals) normals[3 * p + c] = m->h.nrm[3 * j + c];
+            }
+            if (weights) weights[p] = w[q] * m->h.area[j];
+        }
+    return BEM_OK;
+}
+
+int bem_mesh_near_count(bem_mesh m, int64_t* count) {
+    if (!m || !count) return BEM_ERR_ARGUMENT;
+    *count = (int64_t)m->h.near_col.size();
+    return BEM_OK;
+}
+
+int bem_operator_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, bem_operator* out) {
+    if (!ctx) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] { operator_create(ctx, mesh, problem, out); });
+}
+
+int bem_operator_destroy(bem_operator op) {
+    if (!op) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(op->ctx->mu);
+    op_free(op);
+    return BEM_OK;
+}
+
+int bem_operator_size(bem_operator op, int64_t* n) {
+    if (!op || !n) return BEM_ERR_ARGUMENT;
+    *n = op->n;
+    return BEM_OK;
+}
+
+int bem_operator_matrix(bem_operator op, double* Aout, int64_t ld) {
+    if (!op || !Aout || ld < op->n) return BEM_ERR_ARGUMENT;
+    bem_ctx ctx = op->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        const int n = op->n;
+        std::vector<z64> h((size_t)n * n);
+        if (op->single) {
+            std::vector<z32> f((size_t)n * n);
+            HIPCHECK(hipMemcpy(f.data(), op->A, sizeof(z32) * f.size(), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < f.size(); ++i) h[i] = z64{f[i].x, f[i].y};
+        } else {
+            HIPCHECK(hipMemcpy(h.data(), op->A, sizeof(z64) * h.size(), hipMemcpyDeviceToHost));
+        }
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                Aout[2 * (i + j * ld)] = h[(size_t)i * n + j].x;
+                Aout[2 * (i + j * ld) + 1] = h[(size_t)i * n + j].y;
+            }
+    });
+}
+
+int bem_operator_solve(bem_operator op, int32_t nrhs, const double* angles, bem_solution* out) {
+    if (!op) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(op->ctx->mu);
+    return guarded(op->ctx, [&] { operator_solve(op, nrhs, angles, out); });
+}
+
+int bem_solve(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, double beta, double alpha, bem_solution* out) {
+    if (!ctx) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        bem_operator op = nullptr;
+        operator_create(ctx, mesh, problem, &op);
+        const double ang[2] = {beta, alpha};
+        try {
+            operator_solve(op, 1, ang, out);
+        } catch (...) {
+            op_free(op);
+            throw;
+        }
+        op_free(op);
+    });
+}
+
+int bem_solution_destroy(bem_solution sol) {
+    if (!sol) return BEM_ERR_ARGUMENT;
+    bem_ctx ctx = sol->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    ctx->release(sol->p);
+    ctx->release(sol->q);
+    mesh_release(sol->mesh);
+    delete sol;
+    return BEM_OK;
+}
+
+int bem_solution_count(bem_solution sol, int32_t* nrhs, int64_t* np) {
+    if (!sol) return BEM_ERR_ARGUMENT;
+    if (nrhs) *nrhs = sol->nrhs;
+    if (np) *np = sol->nt;
+    return BEM_OK;
+}
+
+int bem_solution_traces(bem_solution sol, int32_t r, double* p, double* q) {
+    if (!sol || r < 0 || r >= sol->nrhs) return BEM_ERR_ARGUMENT;
+    const size_t nt = sol->nt;
+    if (p) std::memcpy(p, sol->hp.data() + r * nt, sizeof(double) * 2 * nt);
+    if (q) std::memcpy(q, sol->hq.data() + r * nt, sizeof(double) * 2 * nt);
+    return BEM_OK;
+}
+
+int bem_solution_diagnostics(bem_solution sol, int32_t r, bem_diagnostics* out) {
+    if (!sol || !out || r < 0 || r >= sol->nrhs) return BEM_ERR_ARGUMENT;
+    *out = sol->diag[r];
+    return BEM_OK;
+}
+
+int bem_far_field(bem_solution sol, int32_t r, int64_t ndir, const double* dirs, double* out) {
+    if (!sol || !out || r < 0 || r >= sol->nrhs || ndir <= 0 || ndir > (1 << 20)) return BEM_ERR_ARGUMENT;
+    bem_ctx ctx = sol->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        std::vector<double> hd(3 * ndir);
+        if (dirs) {
+            for (int64_t d = 0; d < ndir; ++d) {
+                const double* x = dirs + 3 * d;
+                if (!(std::isfinite(x[0]) && std::isfinite(x[1]) && std::isfinite(x[2])))
+                    throw BemError(BEM_ERR_ARGUMENT, "direction must be finite");
+            }
+            std::memcpy(hd.data(), dirs, sizeof(double) * 3 * ndir);
+        } else {
+            double d[3];
+            direction(sol->angles[2 * r], sol->angles[2 * r + 1], d);
+            for (int64_t q = 0; q < ndir; ++q)
+                for (int c = 0; c < 3; ++c) hd[3 * q + c] = -d[c];
+        }
+        hipStream_t s = ctx->stream;
+        double* dd = (double*)ctx->alloc(sizeof(double) * hd.size());
+        z64* part = nullptr;
+        z64* res = nullptr;
+        try {
+            part = (z64*)ctx->alloc(sizeof(z64) * ndir * kFarBlocks);
+            res = (z64*)ctx->alloc(sizeof(z64) * ndir);
+            HIPCHECK(hipMemcpyAsync(dd, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice, s));
+            prof_begin(&ctx->prof, s, C_FAR);
+            launch_far_field(s, sol->mesh->d, sol->k, 1, (int)ndir, dd, sol->p + (size_t)r * sol->nt,
+                             sol->q + (size_t)r * sol->nt, part, res);
+            prof_end(&ctx->prof, s, C_FAR, (double)sol->nt * 7 * ndir, (double)sol->nt * ndir * 7 * 8);
+            HIPCHECK(hipMemcpyAsync(out, res, sizeof(z64) * ndir, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            ctx->release(dd); ctx->release(part); ctx->release(res);
+            throw;
+        }
+        ctx->release(dd); ctx->release(part); ctx->release(res);
+    });
+}
+
+int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh* meshes, const bem_problem* problems, const double* angles,
+              double* amplitudes, double* latency_ms, bem_diagnostics* diagnostics) {
+    if (!ctx || n <= 0 || !meshes || !problems || !angles || !amplitudes) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        std::vector<int> order(n);
+        for (int i = 0; i < n; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            const double ca = (double)meshes[a]->h.nt * (problems[a].boundary == BEM_FLUID ? 2 : 1);
+            const double cb = (double)meshes[b]->h.nt * (problems[b].boundary == BEM_FLUID ? 2 : 1);
+            return ca > cb;
+        });
+        for (int i : order) {
+            const double t0 = now_ms();
+            bem_operator op = nullptr;
+            operator_create(ctx, meshes[i], &problems[i], &op);
+            bem_solution sol = nullptr;
+            try {
+                operator_solve(op, 1, angles + 2 * i, &sol);
+            } catch (...) {
+                op_free(op);
+                throw;
+            }
+            op_free(op);
+            int rc = bem_far_field(sol, 0, 1, nullptr, amplitudes + 2 * i);
+            if (diagnostics) diagnostics[i] = sol->diag[0];
+            bem_solution_destroy(sol);
+            if (rc != BEM_OK) throw BemError(rc, ctx->err);
+            if (latency_ms) latency_ms[i] = now_ms() - t0;
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,432 @@
+// Helmholtz operator assembly on flat triangle panels (CDNA4 / gfx950).
+//
+// Kernel convention G = e^{ikR}/(4 pi R) (reference docs/src/models/
+// boundary_methods.md:9-13).  With r = x - y, R = |r|, a = r.n_x/R,
+// b = r.n_y/R, c = n_x.n_y:
+//   S: G          D: G (1/R - ik) b          K': G (ik - 1/R) a
+//   H: G [ k^2 ab + (c - 3ab)/R^2 + i k (3ab - c)/R ]
+// Regular pairs use the 7-point rule (one unknown per panel, collocated at
+// the centroid).  Near pairs (|c_i - c_j| < 2 hmax_j) use the same rule on
+// 64 sub-triangles in FP64; self panels use closed-form radial integration
+// with Gauss-Legendre in the polar angle (single layer, and the Hadamard
+// finite part of the hypersingular operator).  Systems assembled:
+//   Burton-Miller rigid/soft  (reference src/solvers/bem/full.jl:103-121)
+//   Mueller and CBIE fluid    (reference src/solvers/bem/full.jl:341,344)
+// Output matrices are row-major (row = target panel) so that the LU's row
+// interchanges and panel reads are contiguous.
+#include <cmath>
+#include <vector>
+
+#include "bem_device.h"
+#include "bem_internal.h"
+
+namespace bem {
+
+__constant__ double c_q7[7][4];          // l0, l1, l2, w
+__constant__ double c_near[kNearPts][4]; // l0, l1, l2, w
+__constant__ double c_gl[kSelfGL][2];    // node, weight on [-1, 1]
+
+int upload_rules() {
+    double l[7][3], w[7];
+    q7_rule(l, w);
+    double q7[7][4];
+    for (int q = 0; q < 7; ++q) {
+        for (int c = 0; c < 3; ++c) q7[q][c] = l[q][c];
+        q7[q][3] = w[q];
+    }
+    std::vector<double> l3, nw;
+    near_rule(l3, nw);
+    std::vector<double> nr(4 * kNearPts);
+    for (int p = 0; p < kNearPts; ++p) {
+        for (int c = 0; c < 3; ++c) nr[4 * p + c] = l3[3 * p + c];
+        nr[4 * p + 3] = nw[p];
+    }
+    double gx[kSelfGL], gw[kSelfGL], gl[kSelfGL][2];
+    gauss_legendre(kSelfGL, gx, gw);
+    for (int g = 0; g < kSelfGL; ++g) { gl[g][0] = gx[g]; gl[g][1] = gw[g]; }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_q7), q7, sizeof q7) != hipSuccess) return 1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_near), nr.data(), sizeof(double) * nr.size()) != hipSuccess) return 1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_gl), gl, sizeof gl) != hipSuccess) return 1;
+    return 0;
+}
+
+__device__ inline void fsincos(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ inline void fsincos(float x, float* s, float* c) { sincosf(x, s, c); }
+__device__ inline double frsqrt(double x) { return rsqrt(x); }
+__device__ inline float frsqrt(float x) { return rsqrtf(x); }
+
+template <typename R>
+struct Acc4 {
+    C2<R> S, D, K, H;
+};
+
+// Add one quadrature point's contribution to the accumulators selected by NEED
+// (bit 0: S, 1: D, 2: K', 3: H).
+template <typename R, int NEED>
+__device__ __forceinline__ void kpoint(R k, R rx, R ry, R rz, R nx0, R nx1, R nx2, R ny0, R ny1,
+                                       R ny2, R c, R w, Acc4<R>& acc) {
+    const R R2 = rx * rx + ry * ry + rz * rz;
+    const R iR = frsqrt(R2);
+    const R Rr = R2 * iR;
+    R sn, cs;
+    fsincos(k * Rr, &sn, &cs);
+    const R gr = w * iR * R(0.07957747154594767);  // 1/(4 pi)
+    const R Gx = gr * cs, Gy = gr * sn;
+    if (NEED & 1) { acc.S.x += Gx; acc.S.y += Gy; }
+    if (NEED & 2) {
+        const R b = (rx * ny0 + ry * ny1 + rz * ny2) * iR;
+        const R tx = b * iR, ty = -k * b;
+        acc.D.x += Gx * tx - Gy * ty;
+        acc.D.y += Gx * ty + Gy * tx;
+    }
+    if (NEED & 4) {
+        const R a = (rx * nx0 + ry * nx1 + rz * nx2) * iR;
+        const R tx = -a * iR, ty = k * a;
+        acc.K.x += Gx * tx - Gy * ty;
+        acc.K.y += Gx * ty + Gy * tx;
+    }
+    if (NEED & 8) {
+        const R a = (rx * nx0 + ry * nx1 + rz * nx2) * iR;
+        const R b = (rx * ny0 + ry * ny1 + rz * ny2) * iR;
+        const R ab = a * b;
+        const R tx = k * k * ab + (c - R(3) * ab) * iR * iR;
+        const R ty = k * iR * (R(3) * ab - c);
+        acc.H.x += Gx * tx - Gy * ty;
+        acc.H.y += Gx * ty + Gy * tx;
+    }
+}
+
+template <int KIND>
+struct Need {
+    static constexpr int v = (KIND == ASM_BM || KIND == ASM_MULLER) ? 15 : 3;
+    static constexpr bool two_media = (KIND == ASM_MULLER || KIND == ASM_FLUID_CBIE);
+};
+
+// Combine operator values into the off-diagonal system entries.
+template <typename R, int KIND>
+__device__ __forceinline__ void store_entries(C2<R>* __restrict__ out0, C2<R>* __restrict__ out1,
+                                              int64_t ld, int nt, int i, int j, double k, double g,
+                                              const Acc4<double>& e, const Acc4<double>& f,
+                                              bool diag) {
+    const double ik = 1.0 / k;
+    if (KIND == ASM_BM) {
+        // 1/2 d - D - (i/k) H  ;  S + (i/k)(1/2 d + K')
+        double h = diag ? 0.5 : 0.0;
+        out0[(int64_t)i * ld + j] = C2<R>{R(h - e.D.x + e.H.y * ik), R(-e.D.y - e.H.x * ik)};
+        out1[(int64_t)i * ld + j] = C2<R>{R(e.S.x - e.K.y * ik), R(e.S.y + (e.K.x + h) * ik)};
+    } else if (KIND == ASM_CBIE_RS) {
+        double h = diag ? 0.5 : 0.0;
+        out0[(int64_t)i * ld + j] = C2<R>{R(h - e.D.x), R(-e.D.y)};
+        out1[(int64_t)i * ld + j] = C2<R>{R(e.S.x), R(e.S.y)};
+    } else if (KIND == ASM_MULLER) {
+        // [d - De + Di, Se - g Si; -He + Hi/g, d + K'e - K'i]
+        double d1 = diag ? 1.0 : 0.0, ig = 1.0 / g;
+        out0[(int64_t)i * ld + j] = C2<R>{R(d1 - e.D.x + f.D.x), R(-e.D.y + f.D.y)};
+        out0[(int64_t)i * ld + j + nt] = C2<R>{R(e.S.x - g * f.S.x), R(e.S.y - g * f.S.y)};
+        out0[(int64_t)(i + nt) * ld + j] = C2<R>{R(-e.H.x + f.H.x * ig), R(-e.H.y + f.H.y * ig)};
+        out0[(int64_t)(i + nt) * ld + j + nt] = C2<R>{R(d1 + e.K.x - f.K.x), R(e.K.y - f.K.y)};
+    } else {
+        // [1/2 d - De, Se; 1/2 d + Di, -g Si]
+        double h = diag ? 0.5 : 0.0;
+        out0[(int64_t)i * ld + j] = C2<R>{R(h - e.D.x), R(-e.D.y)};
+        out0[(int64_t)i * ld + j + nt] = C2<R>{R(e.S.x), R(e.S.y)};
+        out0[(int64_t)(i + nt) * ld + j] = C2<R>{R(h + f.D.x), R(f.D.y)};
+        out0[(int64_t)(i + nt) * ld + j + nt] = C2<R>{R(-g * f.S.x), R(-g * f.S.y)};
+    }
+}
+
+template <typename R>
+__device__ inline Acc4<double> widen(const Acc4<R>& a) {
+    return Acc4<double>{{a.S.x, a.S.y}, {a.D.x, a.D.y}, {a.K.x, a.K.y}, {a.H.x, a.H.y}};
+}
+
+// ---------------------------------------------------------------------------
+// Regular pairs.  One lane per source panel j (so the row-major stores of a
+// row are coalesced); each wave walks ROWS target panels, whose centroid and
+// normal are wave-uniform.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct MeshView {
+    const R *cen, *nrm, *area, *gp;
+};
+__device__ inline MeshView<double> view_of(const DevMesh& m, double) { return {m.cen, m.nrm, m.area, m.gp}; }
+__device__ inline MeshView<float> view_of(const DevMesh& m, float) { return {m.cen32, m.nrm32, m.area32, m.gp32}; }
+
+constexpr int kAsmRows = 16;
+
+template <typename R, int KIND>
+__global__ __launch_bounds__(256) void k_assemble(DevMesh m, AsmParams p, C2<R>* __restrict__ out0,
+                                                  C2<R>* __restrict__ out1) {
+    const MeshView<R> v = view_of(m, R(0));
+    const int nt = m.nt;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + lane;
+    const bool vj = j < nt;
+    const int jj = vj ? j : nt - 1;
+    R gx[7], gy[7], gz[7], gw[7];
+    const R aj = v.area[jj];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        gx[q] = v.gp[(size_t)(3 * q) * nt + jj];
+        gy[q] = v.gp[(size_t)(3 * q + 1) * nt + jj];
+        gz[q] = v.gp[(size_t)(3 * q + 2) * nt + jj];
+        gw[q] = R(c_q7[q][3]) * aj;
+    }
+    const R ny0 = v.nrm[jj], ny1 = v.nrm[nt + jj], ny2 = v.nrm[2 * nt + jj];
+    const R k = R(p.k), ki = R(p.kint);
+    const int i0 = (blockIdx.y * 4 + wv) * kAsmRows;
+    for (int r = 0; r < kAsmRows; ++r) {
+        const int i = i0 + r;
+        if (i >= nt) break;
+        const R x0 = v.cen[i], x1 = v.cen[nt + i], x2 = v.cen[2 * nt + i];
+        const R nx0 = v.nrm[i], nx1 = v.nrm[nt + i], nx2 = v.nrm[2 * nt + i];
+        const R c = nx0 * ny0 + nx1 * ny1 + nx2 * ny2;
+        Acc4<R> e{}, f{};
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            const R rx = x0 - gx[q], ry = x1 - gy[q], rz = x2 - gz[q];
+            kpoint<R, Need<KIND>::v>(k, rx, ry, rz, nx0, nx1, nx2, ny0, ny1, ny2, c, gw[q], e);
+            if (Need<KIND>::two_media)
+                kpoint<R, Need<KIND>::v>(ki, rx, ry, rz, nx0, nx1, nx2, ny0, ny1, ny2, c, gw[q], f);
+        }
+        if (vj && i != j)
+            store_entries<R, KIND>(out0, out1, p.ld, nt, i, j, p.k, p.g, widen(e), widen(f), false);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Near pairs: one wave per pair, one lane per sub-triangle (7 points each),
+// FP64, fixed-order wave reduction.
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ inline void wave_sum_acc(Acc4<double>& a) {
+    a.S.x = wave_sum(a.S.x); a.S.y = wave_sum(a.S.y);
+    a.D.x = wave_sum(a.D.x); a.D.y = wave_sum(a.D.y);
+    a.K.x = wave_sum(a.K.x); a.K.y = wave_sum(a.K.y);
+    a.H.x = wave_sum(a.H.x); a.H.y = wave_sum(a.H.y);
+}
+
+template <typename R, int KIND>
+__global__ __launch_bounds__(256) void k_near(DevMesh m, AsmParams p, C2<R>* __restrict__ out0,
+                                              C2<R>* __restrict__ out1) {
+    const int lane = threadIdx.x & 63;
+    const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= m.nnear) return;
+    const int nt = m.nt;
+    const int i = m.near_row[e], j = m.near_col[e];
+    const double x0 = m.cen[i], x1 = m.cen[nt + i], x2 = m.cen[2 * nt + i];
+    const double nx0 = m.nrm[i], nx1 = m.nrm[nt + i], nx2 = m.nrm[2 * nt + i];
+    const double ny0 = m.nrm[j], ny1 = m.nrm[nt + j], ny2 = m.nrm[2 * nt + j];
+    const double c = nx0 * ny0 + nx1 * ny1 + nx2 * ny2;
+    const double* V = m.vert + 9 * (size_t)j;
+    const double aj = m.area[j];
+    Acc4<double> a{}, b{};
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const int pt = lane * 7 + q;
+        const double l0 = c_near[pt][0], l1 = c_near[pt][1], l2 = c_near[pt][2];
+        const double w = c_near[pt][3] * aj;
+        const double y0 = l0 * V[0] + l1 * V[3] + l2 * V[6];
+        const double y1 = l0 * V[1] + l1 * V[4] + l2 * V[7];
+        const double y2 = l0 * V[2] + l1 * V[5] + l2 * V[8];
+        const double rx = x0 - y0, ry = x1 - y1, rz = x2 - y2;
+        kpoint<double, Need<KIND>::v>(p.k, rx, ry, rz, nx0, nx1, nx2, ny0, ny1, ny2, c, w, a);
+        if (Need<KIND>::two_media)
+            kpoint<double, Need<KIND>::v>(p.kint, rx, ry, rz, nx0, nx1, nx2, ny0, ny1, ny2, c, w, b);
+    }
+    wave_sum_acc(a);
+    if (Need<KIND>::two_media) wave_sum_acc(b);
+    if (lane == 0) store_entries<R, KIND>(out0, out1, p.ld, nt, i, j, p.k, p.g, a, b, false);
+}
+
+// Self panel (x at the centroid, in the panel's plane).  Per edge, with d the
+// distance to the edge line and theta the polar angle from its foot:
+//   S = (1/4pi) sum_e int (e^{ikR}-1)/(ik) dtheta,  R = d/cos(theta)
+//   H = (1/4pi) [2 pi i k - sum_e int e^{ikR}/R dtheta]   (Hadamard f.p.)
+__device__ void self_terms(const double* V, const double* x, double k, C2<double>& S, C2<double>& H) {
+    double sSx = 0, sSy = 0, sHx = 0, sHy = 0;
+    for (int e = 0; e < 3; ++e) {
+        const double* P = V + 3 * e;
+        const double* Q = V + 3 * ((e + 1) % 3);
+        double t0 = Q[0] - P[0], t1 = Q[1] - P[1], t2 = Q[2] - P[2];
+        const double L = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+        t0 /= L; t1 /= L; t2 /= L;
+        const double p0 = P[0] - x[0], p1 = P[1] - x[1], p2 = P[2] - x[2];
+        const double sP = p0 * t0 + p1 * t1 + p2 * t2;
+        const double f0 = p0 - sP * t0, f1 = p1 - sP * t1, f2 = p2 - sP * t2;
+        const double dist = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+        const double th0 = atan2(sP, dist), th1 = atan2(sP + L, dist);
+        const double half = 0.5 * (th1 - th0), mid = 0.5 * (th1 + th0);
+        for (int gq = 0; gq < kSelfGL; ++gq) {
+            const double th = mid + half * c_gl[gq][0];
+            const double wg = half * c_gl[gq][1];
+            const double R = dist / cos(th);
+            const double xr = k * R;
+            double sn, cs;
+            sincos(xr, &sn, &cs);
+            const double sh = sin(0.5 * xr);
+            if (k > 0) {
+                sSx += wg * sn / k;
+                sSy += wg * 2.0 * sh * sh / k;
+            } else {
+                sSx += wg * R;
+            }
+            sHx += wg * cs / R;
+            sHy += wg * sn / R;
+        }
+    }
+    const double i4p = 0.07957747154594767;
+    S = C2<double>{sSx * i4p, sSy * i4p};
+    H = C2<double>{-sHx * i4p, (2.0 * M_PI * k - sHy) * i4p};
+}
+
+template <typename R, int KIND>
+__global__ __launch_bounds__(128) void k_self(DevMesh m, AsmParams p, C2<R>* __restrict__ out0,
+                                              C2<R>* __restrict__ out1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = m.nt;
+    if (i >= nt) return;
+    double V[9], x[3];
+    for (int c = 0; c < 9; ++c) V[c] = m.vert[9 * (size_t)i + c];
+    x[0] = m.cen[i]; x[1] = m.cen[nt + i]; x[2] = m.cen[2 * nt + i];
+    Acc4<double> a{}, b{};
+    self_terms(V, x, p.k, a.S, a.H);
+    if (Need<KIND>::two_media) self_terms(V, x, p.kint, b.S, b.H);
+    store_entries<R, KIND>(out0, out1, p.ld, nt, i, i, p.k, p.g, a, b, true);
+}
+
+template <typename R, int KIND>
+static void launch_assemble_k(hipStream_t s, const DevMesh& m, const AsmParams& p, C2<R>* o0, C2<R>* o1) {
+    dim3 grid((m.nt + 63) / 64, (m.nt + 4 * kAsmRows - 1) / (4 * kAsmRows));
+    hipLaunchKernelGGL((k_assemble<R, KIND>), grid, dim3(256), 0, s, m, p, o0, o1);
+}
+
+template <typename R>
+void launch_assemble(hipStream_t s, const DevMesh& m, const AsmParams& p, C2<R>* o0, C2<R>* o1) {
+    switch (p.kind) {
+    case ASM_BM: launch_assemble_k<R, ASM_BM>(s, m, p, o0, o1); break;
+    case ASM_CBIE_RS: launch_assemble_k<R, ASM_CBIE_RS>(s, m, p, o0, o1); break;
+    case ASM_MULLER: launch_assemble_k<R, ASM_MULLER>(s, m, p, o0, o1); break;
+    default: launch_assemble_k<R, ASM_FLUID_CBIE>(s, m, p, o0, o1); break;
+    }
+}
+
+template <typename R, int KIND>
+static void launch_near_self_k(hipStream_t s, const DevMesh& m, const AsmParams& p, C2<R>* o0, C2<R>* o1) {
+    if (m.nnear > 0) {
+        dim3 grid((unsigned)((m.nnear + 3) / 4));
+        hipLaunchKernelGGL((k_near<R, KIND>), grid, dim3(256), 0, s, m, p, o0, o1);
+    }
+    hipLaunchKernelGGL((k_self<R, KIND>), dim3((m.nt + 127) / 128), dim3(128), 0, s, m, p, o0, o1);
+}
+
+template <typename R>
+void launch_near_self(hipStream_t s, const DevMesh& m, const AsmParams& p, C2<R>* o0, C2<R>* o1) {
+    switch (p.kind) {
+    case ASM_BM: launch_near_self_k<R, ASM_BM>(s, m, p, o0, o1); break;
+    case ASM_CBIE_RS: launch_near_self_k<R, ASM_CBIE_RS>(s, m, p, o0, o1); break;
+    case ASM_MULLER: launch_near_self_k<R, ASM_MULLER>(s, m, p, o0, o1); break;
+    default: launch_near_self_k<R, ASM_FLUID_CBIE>(s, m, p, o0, o1); break;
+    }
+}
+
+template void launch_assemble<double>(hipStream_t, const DevMesh&, const AsmParams&, z64*, z64*);
+template void launch_assemble<float>(hipStream_t, const DevMesh&, const AsmParams&, z32*, z32*);
+template void launch_near_self<double>(hipStream_t, const DevMesh&, const AsmParams&, z64*, z64*);
+template void launch_near_self<float>(hipStream_t, const DevMesh&, const AsmParams&, z32*, z32*);
+
+// ---------------------------------------------------------------------------
+// Plane-wave traces (reference src/core/incident.jl:64-76) at the centroids.
+// ---------------------------------------------------------------------------
+__global__ void k_incident(DevMesh m, double k, int nrhs, const double* __restrict__ dirs,
+                           z64* __restrict__ p, z64* __restrict__ q) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    const int nt = m.nt;
+    if (i >= nt || r >= nrhs) return;
+    const double d0 = dirs[3 * r], d1 = dirs[3 * r + 1], d2 = dirs[3 * r + 2];
+    const double ph = k * (d0 * m.cen[i] + d1 * m.cen[nt + i] + d2 * m.cen[2 * nt + i]);
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    const double dn = k * (d0 * m.nrm[i] + d1 * m.nrm[nt + i] + d2 * m.nrm[2 * nt + i]);
+    p[(size_t)r * nt + i] = z64{cs, sn};
+    q[(size_t)r * nt + i] = z64{-dn * sn, dn * cs};  // i k (d.n) e^{i k d.c}
+}
+
+void launch_incident(hipStream_t s, const DevMesh& m, double k, int nrhs, const double* dirs, z64* p, z64* q) {
+    hipLaunchKernelGGL(k_incident, dim3((m.nt + 255) / 256, nrhs), dim3(256), 0, s, m, k, nrhs, dirs, p, q);
+}
+
+// ---------------------------------------------------------------------------
+// Far field f(xhat) = -(1/4pi) sum_j sum_g w_jg (i k (xhat.n_j) p_j + q_j)
+// e^{-ik xhat.y_jg}  (reference src/solvers/bem/full.jl:509-519), FP64,
+// deterministic two-pass reduction.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_far_partial(DevMesh m, double k, int ndir,
+                                                     const double* __restrict__ dirs,
+                                                     const z64* __restrict__ p,
+                                                     const z64* __restrict__ q, z64* partial) {
+    const int rd = blockIdx.y;  // r * ndir + d
+    const int r = rd / ndir, d = rd % ndir;
+    const int nt = m.nt;
+    const double x0 = dirs[3 * d], x1 = dirs[3 * d + 1], x2 = dirs[3 * d + 2];
+    double sx = 0, sy = 0;
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < nt; j += gridDim.x * 256) {
+        const double xn = x0 * m.nrm[j] + x1 * m.nrm[nt + j] + x2 * m.nrm[2 * nt + j];
+        const z64 pj = p[(size_t)r * nt + j], qj = q[(size_t)r * nt + j];
+        // coef = i k xn p + q
+        const double cx = -k * xn * pj.y + qj.x, cy = k * xn * pj.x + qj.y;
+        double ex = 0, ey = 0;
+#pragma unroll
+        for (int g = 0; g < 7; ++g) {
+            const double ph = -k * (x0 * m.gp[(size_t)(3 * g) * nt + j] + x1 * m.gp[(size_t)(3 * g + 1) * nt + j] +
+                                    x2 * m.gp[(size_t)(3 * g + 2) * nt + j]);
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            ex += c_q7[g][3] * cs;
+            ey += c_q7[g][3] * sn;
+        }
+        const double a = m.area[j];
+        sx += a * (cx * ex - cy * ey);
+        sy += a * (cx * ey + cy * ex);
+    }
+    __shared__ double bx[256], by[256];
+    bx[threadIdx.x] = sx;
+    by[threadIdx.x] = sy;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (threadIdx.x < w) {
+            bx[threadIdx.x] += bx[threadIdx.x + w];
+            by[threadIdx.x] += by[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)rd * gridDim.x + blockIdx.x] = z64{bx[0], by[0]};
+}
+
+__global__ void k_far_final(int count, int nblk, const z64* partial, z64* out) {
+    const int rd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rd >= count) return;
+    double sx = 0, sy = 0;
+    for (int b = 0; b < nblk; ++b) {
+        sx += partial[(size_t)rd * nblk + b].x;
+        sy += partial[(size_t)rd * nblk + b].y;
+    }
+    const double i4p = 0.07957747154594767;
+    out[rd] = z64{-sx * i4p, -sy * i4p};
+}
+
+void launch_far_field(hipStream_t s, const DevMesh& m, double k, int nrhs, int ndir, const double* dirs,
+                      const z64* p, const z64* q, z64* partial, z64* out) {
+    int nblk = (m.nt + 255) / 256;
+    if (nblk > kFarBlocks) nblk = kFarBlocks;
+    hipLaunchKernelGGL(k_far_partial, dim3(nblk, nrhs * ndir), dim3(256), 0, s, m, k, ndir, dirs, p, q, partial);
+    hipLaunchKernelGGL(k_far_final, dim3((nrhs * ndir + 63) / 64), dim3(64), 0, s, nrhs * ndir, nblk, partial, out);
+}
+
+}  // namespace bem

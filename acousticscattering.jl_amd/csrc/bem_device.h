@@ -1,0 +1,136 @@
+// Device-side types and launcher declarations shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bem {
+
+// Interleaved complex value (re, im); same layout as Julia ComplexF64/ComplexF32.
+template <typename R>
+struct alignas(2 * sizeof(R)) C2 {
+    R x, y;
+};
+using z64 = C2<double>;
+using z32 = C2<float>;
+
+template <typename R>
+__host__ __device__ inline C2<R> cmk(R a, R b) { return C2<R>{a, b}; }
+template <typename R>
+__host__ __device__ inline C2<R> cadd(C2<R> a, C2<R> b) { return C2<R>{a.x + b.x, a.y + b.y}; }
+template <typename R>
+__host__ __device__ inline C2<R> csub(C2<R> a, C2<R> b) { return C2<R>{a.x - b.x, a.y - b.y}; }
+template <typename R>
+__host__ __device__ inline C2<R> cmul(C2<R> a, C2<R> b) {
+    return C2<R>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__host__ __device__ inline C2<R> cscale(C2<R> a, R s) { return C2<R>{a.x * s, a.y * s}; }
+// 1/z, scaled to avoid overflow in |z|^2 (Smith's method, 1962).
+template <typename R>
+__host__ __device__ inline C2<R> cinv(C2<R> z) {
+    R a = z.x, b = z.y;
+    if ((a < 0 ? -a : a) >= (b < 0 ? -b : b)) {
+        R r = b / a, d = a + b * r;
+        return C2<R>{R(1) / d, -r / d};
+    }
+    R r = a / b, d = b + a * r;
+    return C2<R>{r / d, R(-1) / d};
+}
+template <typename R>
+__host__ __device__ inline R cabs1(C2<R> z) {
+    return (z.x < 0 ? -z.x : z.x) + (z.y < 0 ? -z.y : z.y);
+}
+
+// Panel geometry resident on the device.  SoA arrays: component d of panel j
+// at [d*nt + j]; Gauss point q, component d of panel j at [(3q+d)*nt + j].
+struct DevMesh {
+    int nt = 0;
+    double *cen = nullptr, *nrm = nullptr, *area = nullptr, *gp = nullptr, *vert = nullptr;
+    float *cen32 = nullptr, *nrm32 = nullptr, *area32 = nullptr, *gp32 = nullptr;
+    int *near_ptr = nullptr, *near_col = nullptr, *near_row = nullptr;
+    int64_t nnear = 0;
+};
+
+// System kinds written by the assembly kernels.
+enum AsmKind : int {
+    ASM_BM = 0,          // out0 = 1/2 I - D - a H ; out1 = S + a (1/2 I + K'), a = i/k
+    ASM_CBIE_RS = 1,     // out0 = 1/2 I - D       ; out1 = S
+    ASM_MULLER = 2,      // out0 = 2N x 2N Mueller system
+    ASM_FLUID_CBIE = 3,  // out0 = 2N x 2N CBIE fluid system
+};
+
+struct AsmParams {
+    int kind;
+    double k, kint, g;
+    int64_t ld;  // leading dimension (row-major) of the outputs
+};
+
+// ---- bem_assembly.hip ----
+int upload_rules();  // constant tables (7-point rule, near rule, Gauss-Legendre)
+// Regular 7-point pairs for every (i, j); precision R; row-major outputs.
+template <typename R>
+void launch_assemble(hipStream_t s, const DevMesh& m, const AsmParams& p, C2<R>* out0, C2<R>* out1);
+// FP64 near pairs (subdivided) and self panels, overwriting those entries.
+template <typename R>
+void launch_near_self(hipStream_t s, const DevMesh& m, const AsmParams& p, C2<R>* out0, C2<R>* out1);
+// Plane-wave traces at the centroids for nrhs directions (3 x nrhs doubles).
+void launch_incident(hipStream_t s, const DevMesh& m, double k, int nrhs, const double* dirs,
+                     z64* p, z64* q);
+// Far-field amplitudes: out[r*ndir + d] for nrhs x ndir; partial buffer of
+// size nrhs*ndir*kFarBlocks.
+constexpr int kFarBlocks = 256;
+void launch_far_field(hipStream_t s, const DevMesh& m, double k, int nrhs, int ndir,
+                      const double* dirs, const z64* p, const z64* q, z64* partial, z64* out);
+
+// ---- bem_linalg.hip ----
+template <typename R>
+void launch_row_stats(hipStream_t s, const C2<R>* A, int64_t lda, int n, double* rows, double* rowsum);
+template <typename R>
+void launch_col_max(hipStream_t s, const C2<R>* A, int64_t lda, int n, const double* rows, double* cols);
+void launch_fix_scale(hipStream_t s, int n, double* v);
+template <typename Ri, typename Ro>
+void launch_scale_copy(hipStream_t s, const C2<Ri>* A, int64_t lda, int n, const double* rows,
+                       const double* cols, C2<Ro>* out, int64_t ldo);
+// y = A x (FP64 arithmetic); when b != null also r = b - A x and den = |b| + |A||x|.
+template <typename R>
+void launch_matvec(hipStream_t s, const C2<R>* A, int64_t lda, int rows, int cols, const z64* x, z64* y,
+                   const z64* b, double* den);
+
+struct PanelWS {
+    double* cand_val;  // [2][G]
+    int* cand_idx;     // [2][G]
+    void* cand_row;    // [2][G][64] complex R
+    void* diag_row;    // [2][64] complex R
+    unsigned* bar;     // [2] barrier counter, generation
+    int* status;       // [0] first zero pivot (INT_MAX if none), [1] barrier timeout
+    int max_blocks;
+};
+int panel_capacity(int device_cu_count);  // largest cooperative grid for the panel kernel
+template <typename R>
+int lu_factor(hipStream_t s, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv, PanelWS& ws,
+              void* prof);
+template <typename R>
+void lu_solve(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm,
+              const C2<R>* inv, const C2<R>* b, C2<R>* work, C2<R>* x, void* prof);
+
+// Refinement helpers (FP64 vectors).
+template <typename R>
+void launch_prep_rhs(hipStream_t s, int n, const z64* b, const double* rows, double inv_mag, C2<R>* out);
+template <typename R>
+void launch_finish(hipStream_t s, int n, const C2<R>* v, double mag, const double* cols, z64* x,
+                   int accumulate, int* changed);
+// Per-block maxima for backward errors: writes nblk x 5 doubles
+// {max|r|, max ratio, max|x|, max|b|, nonfinite flag}.
+constexpr int kRedBlocks = 128;
+void launch_bwd_stats(hipStream_t s, int n, const z64* r, const double* den, const z64* x,
+                      const z64* b, double* out);
+void launch_max_abs_scaled(hipStream_t s, int n, const z64* b, const double* rows, double* out);
+// Per-block sums of |r|^2, |b|^2, |r/rows|^2, |b/rows|^2 (nblk x 4).
+void launch_norm_stats(hipStream_t s, int n, const z64* r, const z64* b, const double* rows,
+                       double* out);
+void launch_fill_int(hipStream_t s, int* p, int n, int v);
+void launch_fill_double(hipStream_t s, double* p, int64_t n, double v);
+void launch_copy_z(hipStream_t s, int n, const z64* src, z64* dst, double scale, const z64* sub);
+
+}  // namespace bem

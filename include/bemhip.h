@@ -1,0 +1,209 @@
+/*
+ * bemhip.h -- C ABI of the MI355X full-3D boundary-element library.
+ *
+ * This is the drop-in boundary for the reference's full-3D BEM solve
+ * (brandynlucca/AcousticScattering.jl v0.1.2).  Each entry point names the
+ * reference function it replaces (paths relative to the reference repo).
+ * Plain C types only: pointers, sizes, doubles.  Complex arrays are
+ * interleaved (re, im) doubles, which is Julia's ComplexF64 layout.  Matrices
+ * passed in are column-major (Julia order); triangles are 1-based.
+ *
+ * Every function returns an int status (BEM_OK on success).  The message of
+ * the last failure on a context is returned by bem_last_error().  No C++
+ * exception crosses this boundary.  The library never keeps a caller's host
+ * pointer after a call returns; device state lives behind opaque handles.
+ */
+#ifndef BEMHIP_H
+#define BEMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mapped by the host shim to the reference's errors) ---- */
+#define BEM_OK 0
+#define BEM_ERR_ARGUMENT 1  /* ArgumentError (e.g. src/solvers/bem/full.jl:71,297-298) */
+#define BEM_ERR_DIMENSION 2 /* DimensionMismatch                                       */
+#define BEM_ERR_DEVICE 3    /* HIP failure or library unusable on this device            */
+#define BEM_ERR_ADMISSION 4 /* request exceeds the context's HBM budget                  */
+#define BEM_ERR_NUMERIC 5   /* double-precision fallback rejected (fluid_system.jl:286)  */
+#define BEM_ERR_STATE 6     /* wrong handle or request not finished                      */
+
+/* ---- boundary conditions (src/core/materials.jl:16,23,62-72) ---- */
+#define BEM_RIGID 0
+#define BEM_PRESSURE_RELEASE 1
+#define BEM_FLUID 2
+
+/* ---- formulations (src/solvers/bem/full.jl:52-59,273-280) ---- */
+#define BEM_BURTON_MILLER 0 /* rigid / pressure-release default                   */
+#define BEM_CBIE 1          /* conventional equation (rigid, soft or fluid)        */
+#define BEM_MULLER 2        /* fluid default                                       */
+
+/* ---- precision (src/solvers/bem/full.jl:349-385) ---- */
+#define BEM_DOUBLE 0 /* FP64 assembly, FP64 LU                                      */
+#define BEM_MIXED 1  /* FP32 regular pairs, complex-FP32 LU, FP64 refinement         */
+
+/* ---- refinement reasons (src/solvers/bem/fluid_system.jl:186-227,282-300) ---- */
+#define BEM_REASON_NONE 0
+#define BEM_REASON_CONVERSION 1
+#define BEM_REASON_SINGLE_SINGULAR 2
+#define BEM_REASON_CONDITION 3
+#define BEM_REASON_NONFINITE 4
+#define BEM_REASON_ITERATION_LIMIT 5
+#define BEM_REASON_STAGNATION 6
+
+typedef struct bem_ctx_s *bem_ctx;
+typedef struct bem_mesh_s *bem_mesh;
+typedef struct bem_operator_s *bem_operator;
+typedef struct bem_solution_s *bem_solution;
+
+/* Solver options; mirrors the keyword arguments of solve_full_bem
+ * (src/solvers/bem/full.jl:52-59 rigid/soft, :273-280 fluid).  Fill with
+ * bem_problem_defaults() and override fields. */
+typedef struct {
+    int32_t boundary;            /* BEM_RIGID / BEM_PRESSURE_RELEASE / BEM_FLUID        */
+    int32_t formulation;         /* BEM_BURTON_MILLER / BEM_CBIE / BEM_MULLER           */
+    int32_t precision;           /* BEM_DOUBLE / BEM_MIXED                              */
+    int32_t equilibrate;         /* row/column max scaling before LU (full.jl:388-394)  */
+    double k;                    /* exterior wavenumber [rad/m], finite and > 0        */
+    double density_contrast;     /* g = rho_int / rho_ext (fluid only)                 */
+    double soundspeed_contrast;  /* h = c_int / c_ext (fluid only); k_int = k / h      */
+    double refine_tolerance;     /* backward-error acceptance, default 1e-13           */
+    int32_t refine_maxiter;      /* default 8                                          */
+    int32_t force_promotion;     /* test hook: 1 = promote to FP64 before refining     */
+    double condition_guard;      /* default 10 eps(Float32); <0 disables the estimate  */
+} bem_problem;
+
+/* Per right-hand side report; field names follow the reference's diagnostics
+ * (src/solvers/bem/full.jl:170-181,479-491; fluid_system.jl:172-309). */
+typedef struct {
+    int64_t unknown_count;
+    int64_t quadrature_nodes;
+    double absolute_residual;
+    double relative_residual;
+    double scaled_absolute_residual;
+    double scaled_relative_residual;
+    int32_t factor_precision;    /* 0 double, 1 single                                  */
+    int32_t refinement_reason;   /* BEM_REASON_*                                        */
+    int32_t refinement_iterations;
+    int32_t converged;           /* 1 when the refinement accepted the solution        */
+    double normwise_backward_error;
+    double componentwise_backward_error;
+    double single_reciprocal_condition;
+    double assembly_ms;
+    double factor_ms;
+    double solve_ms;
+} bem_diagnostics;
+
+/* Library version string. */
+const char *bem_version(void);
+
+/* Fill `p` with the reference defaults for boundary `bc`. */
+void bem_problem_defaults(bem_problem *p, int32_t bc);
+
+/* ---- context ---------------------------------------------------------- */
+/* Create a context on HIP device `device`.  `hbm_budget_bytes` bounds the
+ * device memory the context holds for meshes, operators and factors; 0 means
+ * 90% of the device's free memory.  Requests beyond the budget fail with
+ * BEM_ERR_ADMISSION. */
+int bem_ctx_create(int32_t device, uint64_t hbm_budget_bytes, bem_ctx *out);
+int bem_ctx_destroy(bem_ctx ctx);
+const char *bem_last_error(bem_ctx ctx);
+/* Device bytes currently held by the context. */
+int bem_ctx_memory(bem_ctx ctx, uint64_t *used, uint64_t *budget);
+/* Kernel timing with HIP events on the library's stream: 1 enables. */
+int bem_ctx_set_profiling(bem_ctx ctx, int32_t enable);
+/* Accumulated event time and algorithmic work of one kernel class since the
+ * last reset.  Classes: "assemble", "near", "lu_panel", "lu_gemm", "lu_trsm",
+ * "lu_swap", "solve", "residual", "far_field". */
+int bem_ctx_kernel_stats(bem_ctx ctx, const char *name, int64_t *launches, double *total_ms,
+                         double *flops, double *bytes);
+int bem_ctx_reset_stats(bem_ctx ctx);
+
+/* ---- mesh generators (host only, no device needed) --------------------- */
+/* Sizes of the level-`level` icosphere: 10*4^L+2 nodes, 20*4^L triangles. */
+int bem_icosphere_size(int32_t level, int64_t *n_nodes, int64_t *n_tri);
+/* Icosphere mapped by (x,y,z) -> (ax x, ay y, az z): a sphere (ax=ay=az=a) or
+ * a spheroid with its axis along x (src/geometry/spheroid.jl:7-28;
+ * src/geometry/canonical_mesh.jl:24-27).  nodes 3 x Nv column-major,
+ * tri 3 x N, 1-based, outward. */
+int bem_icosphere(int32_t level, double ax, double ay, double az, double *nodes, int64_t *tri);
+/* Body of revolution about x: meridian samples (x_s, r_s), s = 0..ns-1, with
+ * r_0 = r_{ns-1} = 0 (poles), revolved with `n_phi` azimuthal segments
+ * (src/geometry/bodies.jl:73-89).  Sizes: nodes = (ns-2)*n_phi + 2,
+ * triangles = 2*(ns-2)*n_phi. */
+int bem_revolution_size(int32_t ns, int32_t n_phi, int64_t *n_nodes, int64_t *n_tri);
+int bem_revolution(int32_t ns, const double *xs, const double *rs, int32_t n_phi, double *nodes,
+                   int64_t *tri);
+
+/* ---- mesh (src/geometry/surface.jl:271-298 supplied-mesh contract) ----- */
+/* Validates the topology like src/geometry/surface.jl:341-405 (closed,
+ * manifold, connected, consistently oriented, positive volume) and uploads
+ * the panel geometry to the device. */
+int bem_mesh_create(bem_ctx ctx, int64_t n_nodes, const double *nodes, int64_t n_tri,
+                    const int64_t *tri, bem_mesh *out);
+/* Validation only (host), with the reference's messages. */
+int bem_mesh_validate(int64_t n_nodes, const double *nodes, int64_t n_tri, const int64_t *tri,
+                      double *resolution, char *message, int32_t message_len);
+int bem_mesh_destroy(bem_mesh mesh);
+/* N, Nv, resolution (longest edge, surface.jl:363) and enclosed volume. */
+int bem_mesh_info(bem_mesh mesh, int64_t *n_tri, int64_t *n_nodes, double *resolution,
+                  double *volume);
+/* Panel geometry: centroids 3 x N, unit normals 3 x N, areas N. */
+int bem_mesh_panels(bem_mesh mesh, double *centroids, double *normals, double *areas);
+/* 7-point Gauss nodes of every panel (3 x 7N coords, 3 x 7N normals, 7N
+ * weights).  The Julia shim hands these to the reference's far_field
+ * (src/solvers/bem/full.jl:509-519) as its `quad`. */
+int bem_mesh_quadrature(bem_mesh mesh, double *coords, double *normals, double *weights);
+/* Number of near (subdivided-quadrature) panel pairs. */
+int bem_mesh_near_count(bem_mesh mesh, int64_t *count);
+
+/* ---- operators: assemble + factor once, solve many ---------------------
+ * Replaces _assemble_full_boundary/_assemble_full_fluid followed by
+ * _factor_fluid_system (src/solvers/bem/full.jl:68-128,289-347,364-409);
+ * used by incidence_angle_sweep (src/postprocessing/sweeps.jl:626-701). */
+int bem_operator_create(bem_ctx ctx, bem_mesh mesh, const bem_problem *problem,
+                        bem_operator *out);
+int bem_operator_destroy(bem_operator op);
+/* Download the assembled (unscaled) system matrix, column-major n x n
+ * complex, n = N or 2N.  Test and diagnostics use. */
+int bem_operator_matrix(bem_operator op, double *A, int64_t ld);
+int bem_operator_size(bem_operator op, int64_t *n);
+
+/* Solve for `n_rhs` plane waves with incidence angles (beta_r, alpha_r)
+ * stored as angles[2r], angles[2r+1] (polar from +x, azimuth from +y toward
+ * +z, full.jl:15-18).  Replaces _solve_full_boundary / _solve_full_fluid
+ * (full.jl:130-184,456-495). */
+int bem_operator_solve(bem_operator op, int32_t n_rhs, const double *angles, bem_solution *out);
+
+/* One-shot solve_full_bem (full.jl:52-66,273-287): assemble, factor, solve. */
+int bem_solve(bem_ctx ctx, bem_mesh mesh, const bem_problem *problem, double incidence_angle,
+              double incidence_azimuth, bem_solution *out);
+
+/* ---- solutions --------------------------------------------------------- */
+int bem_solution_destroy(bem_solution sol);
+int bem_solution_count(bem_solution sol, int32_t *n_rhs, int64_t *n_panels);
+/* Scattered surface traces p_scat, dpdn_scat for right-hand side `r`
+ * (N complex each), as returned by solve_full_bem. */
+int bem_solution_traces(bem_solution sol, int32_t r, double *p_scat, double *dpdn_scat);
+/* Far-field amplitude f(xhat) for `n_dir` unit directions (3 x n_dir) of
+ * right-hand side `r` (full.jl:509-519), FP64 on the device.  A null `dirs`
+ * means backscatter xhat = -d (src/solvers/bem/postprocessing.jl:4-6). */
+int bem_far_field(bem_solution sol, int32_t r, int64_t n_dir, const double *dirs, double *out);
+int bem_solution_diagnostics(bem_solution sol, int32_t r, bem_diagnostics *out);
+
+/* ---- batched frequency sweep (src/postprocessing/sweeps.jl:103-110) ----
+ * Solves `n` independent problems (mesh_i, problem_i) at incidence
+ * (beta_i, alpha_i) and writes the backscatter amplitude of each (complex,
+ * 2 doubles) and its wall latency [ms] from submission to completion.
+ * Work is ordered largest-first and overlapped across device streams. */
+int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh *meshes, const bem_problem *problems,
+              const double *angles, double *amplitudes, double *latency_ms,
+              bem_diagnostics *diagnostics);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BEMHIP_H */

@@ -1,0 +1,216 @@
+"""GPU parity: libbemhip (through its C ABI) against the CPU FP64 oracle.
+
+Tolerances: FP64 assembly entries 1e-12 relative to the matrix max; FP64
+solutions 1e-9 relative in the scattering amplitude (the plain GPU form's
+parity bar); the mixed form |dTS| <= 0.01 dB and amplitude rtol 1e-8 against
+the FP64 form (src/solvers/bem/fluid_system.jl acceptance, test/basic/bem/
+formulations.jl:431-436); physics against the sphere modal series at the
+reference's 0.1 dB / 1 % bar (test/extended/bem/sphere/full3d.jl:139-140).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import bemhip as bh
+from oracle import meshes, modal
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+K_CONFIG1 = 2 * math.pi * 38000.0 / 1477.4
+A_CONFIG1 = 0.01
+FLUID_CONFIG1 = (1028.9 / 1026.8, 1480.3 / 1477.4)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    say("[gpu] creating context on device 0")
+    c = bh.Context(0)
+    yield c
+    c.close()
+
+
+def _sphere(ctx, level, radius=A_CONFIG1):
+    nodes, tri = meshes.sphere(radius, level)
+    return bh.Mesh(nodes, tri + 1, ctx=ctx), O.Mesh(nodes, tri)
+
+
+CASES = [
+    ("rigid_bm", bh.Rigid(), O.RIGID_BM, None),
+    ("soft_bm", bh.PressureRelease(), O.SOFT_BM, None),
+    ("rigid_cbie", bh.Rigid(), O.RIGID_CBIE, "cbie"),
+    ("soft_cbie", bh.PressureRelease(), O.SOFT_CBIE, "cbie"),
+    ("fluid_muller", bh.FluidFilled(*FLUID_CONFIG1), O.FLUID_MULLER, None),
+    ("fluid_cbie", bh.FluidFilled(1.2, 1.1), O.FLUID_CBIE, "cbie"),
+]
+
+
+@pytest.mark.parametrize("name,bc,kind,form", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_assembled_matrix_matches_oracle(ctx, name, bc, kind, form, precision):
+    say(f"[gpu] assembly {name} {precision} L2 sphere")
+    gm, om = _sphere(ctx, 2)
+    op = bh.Operator(gm, bc, K_CONFIG1, formulation=form, precision=precision)
+    A = op.matrix()
+    g, h = (bc.density_contrast, bc.soundspeed_contrast) if isinstance(bc, bh.FluidFilled) else (1.0, 1.0)
+    R = om.assemble(kind, K_CONFIG1, g, h)
+    err = np.abs(A - R).max() / np.abs(R).max()
+    tol = 1e-12 if precision == "double" else 2e-6
+    assert err < tol, f"{name} {precision}: rel err {err:.3e}"
+
+
+@pytest.mark.parametrize("bcname", ["rigid", "soft", "fluid"])
+def test_config1_solution_matches_oracle_fp64(ctx, bcname):
+    say(f"[gpu] config-1 {bcname} FP64 vs oracle")
+    gm, om = _sphere(ctx, 3)
+    if bcname == "fluid":
+        bc = bh.FluidFilled(*FLUID_CONFIG1)
+        p0, q0, _ = O.solve_fluid(om, *FLUID_CONFIG1, K_CONFIG1, beta=0.0)
+    else:
+        bc = bh.Rigid() if bcname == "rigid" else bh.PressureRelease()
+        p0, q0, _ = O.solve_boundary(om, bcname, K_CONFIG1, beta=0.0)
+    p, q, _, diag = bh.solve_full_bem(bc, K_CONFIG1, gm, incidence_angle=0.0, return_diagnostics=True)
+    assert np.abs(p - p0).max() / np.abs(p0).max() < 1e-9
+    assert np.abs(q - q0).max() / np.abs(q0).max() < 1e-9
+    f0 = om.far_field(K_CONFIG1, -O.direction(0.0), p0, q0)[0]
+    sol = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0)
+    f = bh.scattering_amplitude(sol)
+    assert abs(f - f0) / abs(f0) < 1e-9
+    assert diag["relative_residual"] < 1e-12
+
+
+@pytest.mark.parametrize("bcname,level", [("rigid", 3), ("fluid", 3), ("rigid", 4), ("soft", 4), ("fluid", 4)])
+def test_sphere_against_modal_series(ctx, bcname, level):
+    say(f"[gpu] modal {bcname} L{level}")
+    gm = bh.mesh(bh.Sphere(A_CONFIG1), level=level, ctx=ctx)
+    bc = {"rigid": bh.Rigid(), "soft": bh.PressureRelease(), "fluid": bh.FluidFilled(*FLUID_CONFIG1)}[bcname]
+    mbc = {"rigid": ("rigid",), "soft": ("soft",), "fluid": ("fluid",) + FLUID_CONFIG1}[bcname]
+    sol = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0)
+    f = bh.scattering_amplitude(sol)
+    fm = modal.form_function(mbc, K_CONFIG1, A_CONFIG1)
+    assert abs(bh.target_strength(f) - modal.target_strength(fm)) < 0.1
+    assert abs(f - fm) / abs(fm) < 0.01
+
+
+def test_mixed_matches_double_config1_fluid(ctx):
+    say("[gpu] mixed vs double, config-1 fluid L3")
+    gm, _ = _sphere(ctx, 3)
+    bc = bh.FluidFilled(*FLUID_CONFIG1)
+    sd = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0)
+    sm = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0, precision="mixed")
+    fd, fm = bh.scattering_amplitude(sd), bh.scattering_amplitude(sm)
+    d = sm.diagnostics
+    assert d["factor_precision"] == "single"
+    assert d["converged"]
+    assert d["normwise_backward_error"] <= 1e-13 and d["componentwise_backward_error"] <= 1e-13
+    assert abs(bh.target_strength(fm) - bh.target_strength(fd)) <= 0.01
+    # the mixed form solves the FP32-assembled system; its distance to the
+    # FP64 system bounds the amplitude difference
+    assert abs(fm - fd) / abs(fd) < 1e-5
+
+
+def test_mixed_refinement_reaches_fp32_system_solution(ctx):
+    say("[gpu] mixed refinement vs oracle solve of the same FP32-assembled matrix")
+    gm, om = _sphere(ctx, 2)
+    bc = bh.FluidFilled(*FLUID_CONFIG1)
+    op = bh.Operator(gm, bc, K_CONFIG1, precision="mixed")
+    A = op.matrix()  # FP32-assembled system, widened to FP64
+    p0, q0, dd = O.solve_fluid(om, *FLUID_CONFIG1, K_CONFIG1, A=A)
+    sol = op.solve([(0.0, 0.0)])
+    p, q = sol.traces(0)
+    assert np.abs(p - p0).max() / np.abs(p0).max() < 1e-9
+    assert np.abs(q - q0).max() / np.abs(q0).max() < 1e-9
+
+
+def test_forced_promotion(ctx):
+    say("[gpu] forced promotion to FP64")
+    gm, _ = _sphere(ctx, 2)
+    bc = bh.FluidFilled(*FLUID_CONFIG1)
+    s = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0, precision="mixed", _force_promotion=True)
+    d = s.diagnostics
+    assert d["factor_precision"] == "double" and d["refinement_reason"] == "condition"
+    sd = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0)
+    assert abs(bh.scattering_amplitude(s) - bh.scattering_amplitude(sd)) / abs(bh.scattering_amplitude(sd)) < 1e-5
+
+
+def test_iteration_limit_promotes(ctx):
+    say("[gpu] maxiter=0 forces iteration_limit promotion")
+    gm, _ = _sphere(ctx, 2)
+    s = bh.bem(gm, bh.FluidFilled(*FLUID_CONFIG1), K_CONFIG1, incidence_angle=0.0, precision="mixed",
+               refinement={"maxiter": 0})
+    d = s.diagnostics
+    assert d["factor_precision"] == "double"
+    assert d["refinement_reason"] == "iteration_limit"
+
+
+def test_matched_medium_is_invisible(ctx):
+    say("[gpu] matched medium g=h=1, L4")
+    gm = bh.mesh(bh.Sphere(1.0), level=4, ctx=ctx)
+    s = bh.bem(gm, bh.FluidFilled(1.0, 1.0), 1.0)
+    assert abs(bh.scattering_amplitude(s)) < 1e-12
+
+
+def test_spheroid_L4_fluid_matches_oracle(ctx):
+    say("[gpu] spheroid L4 fluid Mueller (2N=10240) vs oracle")
+    nodes, tri = meshes.spheroid(0.02, 0.01, 4)
+    gm = bh.Mesh(nodes, tri + 1, ctx=ctx)
+    om = O.Mesh(nodes, tri)
+    k = 500.0
+    g, h = 1.0357, 1.0279
+    p0, q0, _ = O.solve_fluid(om, g, h, k, beta=0.0)
+    f0 = om.far_field(k, -O.direction(0.0), p0, q0)[0]
+    s = bh.bem(gm, bh.FluidFilled(g, h), k, incidence_angle=0.0)
+    f = bh.scattering_amplitude(s)
+    assert abs(f - f0) / abs(f0) < 1e-9
+    sm = bh.bem(gm, bh.FluidFilled(g, h), k, incidence_angle=0.0, precision="mixed")
+    assert abs(bh.target_strength(bh.scattering_amplitude(sm)) - bh.target_strength(f)) <= 0.01
+
+
+def test_incidence_sweep_matches_single_solves(ctx):
+    say("[gpu] incidence-angle sweep vs single solves")
+    gm, _ = _sphere(ctx, 2)
+    angles = [0.0, 0.7, 1.9, math.pi]
+    amps, ts = bh.incidence_angle_sweep(gm, bh.Rigid(), K_CONFIG1, angles)
+    for a, f in zip(angles, amps):
+        s = bh.bem(gm, bh.Rigid(), K_CONFIG1, incidence_angle=a)
+        assert abs(bh.scattering_amplitude(s) - f) <= 1e-12 * abs(f)
+
+
+def test_batched_sweep_matches_single_solves(ctx):
+    say("[gpu] bem_sweep vs single solves")
+    m2 = bh.mesh(bh.Spheroid(0.02, 0.01), level=2, ctx=ctx)
+    m3 = bh.mesh(bh.Spheroid(0.02, 0.01), level=3, ctx=ctx)
+    fl = bh.FluidFilled(1.0357, 1.0279)
+    ks = [50.0, 200.0, 400.0]
+    ms = [m2, m3, m2]
+    probs = [bh.make_problem(fl, k, precision="mixed") for k in ks]
+    amps, lat, diags = bh.batched_sweep(ms, probs, [(0.0, 0.0)] * 3)
+    for m, k, f in zip(ms, ks, amps):
+        s = bh.bem(m, fl, k, incidence_angle=0.0, precision="mixed")
+        assert abs(bh.scattering_amplitude(s) - f) <= 1e-12 * abs(f)
+    assert np.all(lat > 0)
+
+
+def test_argument_errors(ctx):
+    say("[gpu] argument errors")
+    gm, _ = _sphere(ctx, 1, 1.0)
+    for k in (0.0, -1.0, math.inf, math.nan):
+        with pytest.raises(ValueError):
+            bh.solve_full_bem(bh.Rigid(), k, gm)
+    for bad in (bh.FluidFilled(0.0, 1.0), bh.FluidFilled(1.0, -1.0), bh.FluidFilled(math.inf, 1.0),
+                bh.FluidFilled(1.0, math.nan)):
+        with pytest.raises(ValueError):
+            bh.solve_full_bem(bad, 1.0, gm)
+    with pytest.raises(ValueError):
+        bh.solve_full_bem(bh.FluidFilled(1.2, 1.1), 1.0, gm, formulation="burton_miller")
+    with pytest.raises(ValueError):
+        bh.solve_full_bem(bh.Rigid(), 1.0, gm, formulation="muller")
+    with pytest.raises(ValueError):
+        bh.solve_full_bem(bh.FluidFilled(1.2, 1.1), 1.0, gm, precision="mixed", equilibrate=False)
