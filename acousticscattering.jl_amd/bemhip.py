@@ -43,7 +43,7 @@ EXPORTED = [
     "bem_mesh_quadrature", "bem_mesh_near_count", "bem_operator_create", "bem_operator_destroy",
     "bem_operator_matrix", "bem_operator_size", "bem_operator_solve", "bem_solve",
     "bem_solution_destroy", "bem_solution_count", "bem_solution_traces", "bem_far_field",
-    "bem_solution_diagnostics", "bem_sweep",
+    "bem_solution_diagnostics", "bem_sweep", "bem_debug_lu",
 ]
 
 
@@ -128,6 +128,7 @@ def load_library(path=LIB_PATH):
         "bem_solution_diagnostics": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(Diagnostics)]),
         "bem_sweep": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(_vp), ctypes.POINTER(Problem),
                                      _dp, _dp, _dp, ctypes.POINTER(Diagnostics)]),
+        "bem_debug_lu": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, ctypes.c_int32, _dp, _ip, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -588,6 +589,24 @@ def frequency_sweep(solve, frequencies, sound_speed):
     ks = 2 * math.pi * freqs / sound_speed
     amps = np.array([scattering_amplitude(solve(k)) for k in ks])
     return {"frequencies": freqs, "k": ks, "ts": 20 * np.log10(np.abs(amps)), "amplitudes": amps}
+
+
+def debug_lu(A, single=False, b=None, ctx=None):
+    """Device LU of a dense complex matrix (test hook): returns (LU, 1-based ipiv[, x])."""
+    ctx = ctx or default_context()
+    A = np.asfortranarray(np.asarray(A, dtype=np.complex128))
+    n = A.shape[0]
+    LU = np.empty((n, n), dtype=np.complex128, order="F")
+    piv = np.empty(n, dtype=np.int64)
+    if b is not None:
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.empty(n, dtype=np.complex128)
+        ctx.check(ctx.lib.bem_debug_lu(ctx._h, n, A.ctypes.data_as(_dp), 1 if single else 0, LU.ctypes.data_as(_dp),
+                                       _i(piv), b.ctypes.data_as(_dp), x.ctypes.data_as(_dp)))
+        return LU, piv, x
+    ctx.check(ctx.lib.bem_debug_lu(ctx._h, n, A.ctypes.data_as(_dp), 1 if single else 0, LU.ctypes.data_as(_dp),
+                                   _i(piv), None, None))
+    return LU, piv
 
 
 def batched_sweep(meshes, problems, angles, ctx=None):

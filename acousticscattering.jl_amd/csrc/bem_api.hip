@@ -1169,4 +1169,73 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh* meshes, const bem_problem*
     });
 }
 
+int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t single, double* LUout, int64_t* ipiv_out,
+                 const double* b, double* x) {
+    if (!ctx || !A || n64 <= 0 || n64 > 65536) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        const int n = (int)n64;
+        hipStream_t s = ctx->stream;
+        const size_t esz = single ? sizeof(z32) : sizeof(z64);
+        std::vector<char> host((size_t)n * n * esz);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                const double re = A[2 * (i + (size_t)j * n)], im = A[2 * (i + (size_t)j * n) + 1];
+                if (single) ((z32*)host.data())[(size_t)i * n + j] = z32{(float)re, (float)im};
+                else ((z64*)host.data())[(size_t)i * n + j] = z64{re, im};
+            }
+        bem_operator_s op;
+        op.ctx = ctx;
+        op.n = n;
+        op.single = single != 0;
+        std::vector<void*> bufs;
+        auto al = [&](size_t bytes) { void* p = ctx->alloc(bytes); bufs.push_back(p); return p; };
+        try {
+            op.LU = al(esz * (size_t)n * n);
+            op.ipiv = (int*)al(sizeof(int) * n);
+            op.perm = (int*)al(sizeof(int) * n);
+            op.inv = al(esz * 2 * 4096 * (size_t)((n + 63) / 64));
+            HIPCHECK(hipMemcpyAsync(op.LU, host.data(), host.size(), hipMemcpyHostToDevice, s));
+            if (single) factor_into<float>(&op, op.LU, op.ipiv, op.perm, op.inv);
+            else factor_into<double>(&op, op.LU, op.ipiv, op.perm, op.inv);
+            HIPCHECK(hipMemcpyAsync(host.data(), op.LU, host.size(), hipMemcpyDeviceToHost, s));
+            std::vector<int> hp(n);
+            HIPCHECK(hipMemcpyAsync(hp.data(), op.ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            if (LUout)
+                for (int i = 0; i < n; ++i)
+                    for (int j = 0; j < n; ++j) {
+                        double re, im;
+                        if (single) { z32 v = ((z32*)host.data())[(size_t)i * n + j]; re = v.x; im = v.y; }
+                        else { z64 v = ((z64*)host.data())[(size_t)i * n + j]; re = v.x; im = v.y; }
+                        LUout[2 * (i + (size_t)j * n)] = re;
+                        LUout[2 * (i + (size_t)j * n) + 1] = im;
+                    }
+            if (ipiv_out)
+                for (int i = 0; i < n; ++i) ipiv_out[i] = hp[i] + 1;
+            if (b && x) {
+                op.vb = (z64*)al(sizeof(z64) * n);
+                op.vx = (z64*)al(sizeof(z64) * n);
+                op.wv = al(sizeof(z64) * n);
+                op.wv2 = al(sizeof(z64) * n);
+                op.wv3 = al(sizeof(z64) * n);
+                op.red = (double*)al(sizeof(double) * 5 * kRedBlocks);
+                op.precision = single ? 1 : 0;
+                HIPCHECK(hipMemcpyAsync(op.vb, b, sizeof(z64) * n, hipMemcpyHostToDevice, s));
+                if (!correction(&op, op.vb, op.vx, 0, nullptr)) throw BemError(BEM_ERR_ARGUMENT, "nonfinite b");
+                HIPCHECK(hipMemcpyAsync(x, op.vx, sizeof(z64) * n, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
+            }
+        } catch (...) {
+            for (void* p : bufs) ctx->release(p);
+            op.LU = op.inv = op.wv = op.wv2 = op.wv3 = nullptr;
+            op.ipiv = op.perm = nullptr;
+            op.vb = op.vx = nullptr;
+            op.red = nullptr;
+            throw;
+        }
+        for (void* p : bufs) ctx->release(p);
+    });
+}
+
 }  // extern "C"

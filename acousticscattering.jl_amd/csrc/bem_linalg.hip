@@ -579,12 +579,17 @@ __global__ __launch_bounds__(256) void k_trsm(C2<R>* __restrict__ A, int64_t lda
 // C -= A B, complex, row-major; MFMA 16x16x4 with four real products.
 template <typename R>
 struct Mfma;
+// Operand layout (both): A[i][k] from lane i + 16k, B[k][j] from lane j + 16k.
+// Result slot r of lane l holds D[row(l, r)][l % 16]; the row rule differs
+// between the two instructions on gfx950 (measured with a one-wave probe):
+//   f64 16x16x4: row = l/16 + 4r        f32 16x16x4: row = 4(l/16) + r
 template <>
 struct Mfma<double> {
     typedef double v4 __attribute__((ext_vector_type(4)));
     __device__ static inline v4 op(double a, double b, v4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -592,13 +597,13 @@ struct Mfma<float> {
     __device__ static inline v4 op(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 constexpr int kGM = 128, kGN = 128, kGK = 16;
 
 // 512 threads: 8 waves as 2 (rows) x 4 (columns), each wave a 64 x 32 tile of
-// 4 x 2 MFMA blocks.  Lane layout of v_mfma_*_16x16x4: A[i][k] from lane
-// i + 16k, B[k][j] from lane j + 16k, D[i][j] in lane j + 16(i/4), slot i%4.
+// 4 x 2 MFMA blocks (lane layouts in Mfma<R>).
 template <typename R>
 __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A, int64_t lda,
                                                      const C2<R>* __restrict__ B, int64_t ldb,
@@ -679,7 +684,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm + x * 16 + (lane >> 4) * 4 + r;
+                const int gm = m0 + wm + x * 16 + Mfma<R>::row(lane, r);
                 const int gn = n0 + wn + y * 16 + (lane & 15);
                 if (gm < M && gn < N) {
                     C2<R> c = C[(int64_t)gm * ldc + gn];

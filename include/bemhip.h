@@ -203,6 +203,17 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh *meshes, const bem_problem 
               const double *angles, double *amplitudes, double *latency_ms,
               bem_diagnostics *diagnostics);
 
+/* ---- test hook: dense LU of a caller matrix ------------------------------
+ * Factors the column-major n x n complex matrix A on the device with the
+ * library's LU (no equilibration; FP32 when single != 0) and returns the
+ * factors column-major in LU (unit-lower L below the diagonal, U on and above)
+ * and the 1-based pivot rows in ipiv, the LAPACK zgetrf/cgetrf convention the
+ * reference relies on (src/solvers/bem/full.jl:404).  When b and x are
+ * non-null it also solves A x = b with those factors.  No counterpart in the
+ * reference: it exists so tests can check the factorization by itself. */
+int bem_debug_lu(bem_ctx ctx, int64_t n, const double *A, int32_t single, double *LU, int64_t *ipiv,
+                 const double *b, double *x);
+
 #ifdef __cplusplus
 }
 #endif
