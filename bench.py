@@ -53,12 +53,24 @@ def sweep_plan(bh, ctx, n_points, max_level):
     return plan, meshes
 
 
-def run_sweep(bh, ctx, plan, meshes, precision):
+def run_sweep(bh, ctx, plan, meshes, precision, chunk=16):
+    """One pass over the sweep, largest meshes first, in synchronous chunks of
+    `chunk` solves (so progress can be reported); returns per-solve results in
+    plan order and the wall time of the pass."""
     fl = bh.FluidFilled(G_RHO, H_C)
-    ms = [meshes[L] for (_, _, L) in plan]
-    probs = [bh.make_problem(fl, k, precision=precision) for (_, k, _) in plan]
+    order = sorted(range(len(plan)), key=lambda i: -plan[i][2])
+    amps = np.empty(len(plan), dtype=np.complex128)
+    lat = np.empty(len(plan))
+    diags = [None] * len(plan)
     t0 = time.perf_counter()
-    amps, lat, diags = bh.batched_sweep(ms, probs, [(0.0, 0.0)] * len(plan), ctx=ctx)
+    for c0 in range(0, len(order), chunk):
+        idx = order[c0:c0 + chunk]
+        ms = [meshes[plan[i][2]] for i in idx]
+        probs = [bh.make_problem(fl, plan[i][1], precision=precision) for i in idx]
+        a, l, d = bh.batched_sweep(ms, probs, [(0.0, 0.0)] * len(idx), ctx=ctx)
+        for j, i in enumerate(idx):
+            amps[i], lat[i], diags[i] = a[j], l[j], d[j]
+        log(f"  {precision}: {c0 + len(idx)}/{len(plan)} solves, {time.perf_counter() - t0:.1f}s")
     return amps, lat, diags, time.perf_counter() - t0
 
 
@@ -85,6 +97,7 @@ def cpu_baseline(plan, budget_s=30.0):
         t1 = time.perf_counter()
         O.solve_fluid(m, G_RHO, H_C, k, A=A)
         t2 = time.perf_counter()
+        log(f"  cpu baseline N={m.n}: assembly {t1 - t0:.2f}s, LU+solve {t2 - t1:.2f}s")
         per[L] = (t1 - t0, t2 - t1, m.n)
         sample.append(f"N={m.n}: assembly {t1 - t0:.2f}s, LU+solve {t2 - t1:.2f}s")
     if not per:
@@ -109,7 +122,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--points", type=int, default=256)
     ap.add_argument("--max-level", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
