@@ -256,7 +256,6 @@ void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) 
     const int n = op->n;
     launch_fill_int(s, ctx->ws.status, 1, INT_MAX);
     launch_fill_int(s, ctx->ws.status + 1, 1, 0);
-    launch_fill_int(s, (int*)ctx->ws.bar, 2, 0);
     int rc = lu_factor<R>(s, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, ctx->ws, &ctx->prof);
     if (rc == 2) throw BemError(BEM_ERR_ADMISSION, "system too large for one cooperative panel launch");
     if (rc != 0) throw BemError(BEM_ERR_DEVICE, "LU panel launch failed");
@@ -728,7 +727,7 @@ int bem_ctx_create(int32_t device, uint64_t budget, bem_ctx* out) {
         if (G <= 0) throw BemError(BEM_ERR_DEVICE, "cooperative panel kernel cannot be resident");
         c->ws.max_blocks = G;
         size_t bytes = sizeof(double) * 2 * G + sizeof(int) * 2 * G + sizeof(z64) * 2 * (size_t)G * 64 +
-                       sizeof(z64) * 2 * 64 + 64 + 64;
+                       sizeof(z64) * 2 * 64 + sizeof(unsigned) * 2 * G + 64 + 6 * 256;
         char* m = (char*)c->alloc(bytes + 1024);
         c->ws_mem = m;
         HIPCHECK(hipMemset(m, 0, bytes + 1024));
@@ -738,7 +737,8 @@ int bem_ctx_create(int32_t device, uint64_t budget, bem_ctx* out) {
         c->ws.cand_idx = (int*)take(sizeof(int) * 2 * G);
         c->ws.cand_row = take(sizeof(z64) * 2 * (size_t)G * 64);
         c->ws.diag_row = take(sizeof(z64) * 2 * 64);
-        c->ws.bar = (unsigned*)take(64);
+        c->ws.bar = (unsigned*)take(sizeof(unsigned) * 2 * G);
+        c->ws.epoch = 1;
         c->ws.status = (int*)take(64);
         if (off > bytes + 1024) throw BemError(BEM_ERR_DEVICE, "workspace layout");
         HIPCHECK(hipDeviceSynchronize());

@@ -102,9 +102,10 @@ struct PanelWS {
     int* cand_idx;     // [2][G]
     void* cand_row;    // [2][G][64] complex R
     void* diag_row;    // [2][64] complex R
-    unsigned* bar;     // [2] barrier counter, generation
+    unsigned* bar;     // [2][G] per-workgroup stamps of the panel exchange
     int* status;       // [0] first zero pivot (INT_MAX if none), [1] barrier timeout
     int max_blocks;
+    unsigned epoch;    // stamp base of the next panel launch (advances by 64)
 };
 int panel_capacity(int device_cu_count);  // largest cooperative grid for the panel kernel
 template <typename R>

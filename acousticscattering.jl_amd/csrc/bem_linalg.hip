@@ -307,36 +307,64 @@ void launch_norm_stats(hipStream_t s, int n, const z64* r, const z64* b, const d
 // ---------------------------------------------------------------------------
 constexpr int kNB = 64;
 
-// Generation barrier over all workgroups of a cooperative launch.
-// Agent-scope release/acquire makes the candidate rows written before it
-// visible to every XCD after it.  The spin is bounded: a timeout sets the
-// sticky flag status[1], after which every later barrier returns at once, so
-// a failed launch drains quickly and the host reports the error.
+// Cross-workgroup traffic of the cooperative panel uses relaxed agent-scope
+// atomic loads and stores only.  On gfx950 these are plain global loads and
+// stores with the sc1 bit, coherent across XCDs without the L2-wide
+// writeback/invalidate (buffer_wbl2 / buffer_inv) that agent-scope
+// release/acquire fences emit.  Ordering comes from s_waitcnt: a wave's
+// communication stores have completed before its workgroup arrives.
+template <typename T>
+__device__ __forceinline__ void st_ag(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_ag(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename R>
+__device__ __forceinline__ void st_ag(C2<R>* p, C2<R> v) {
+    st_ag(&p->x, v.x);
+    st_ag(&p->y, v.y);
+}
+template <typename R>
+__device__ __forceinline__ C2<R> ld_ag(const C2<R>* p) {
+    return C2<R>{ld_ag(&p->x), ld_ag(&p->y)};
+}
+__device__ __forceinline__ void wait_mem() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Generation barrier over all workgroups of a cooperative launch.  The spin
+// is bounded: a timeout sets the sticky flag status[1], after which every
+// later barrier returns at once, so a failed launch drains quickly and the
+// host reports the error.
 __device__ inline void grid_barrier(unsigned* bar, unsigned G, int* status) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    wait_mem();
     __syncthreads();
-    if (threadIdx.x == 0 && __hip_atomic_load(&status[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0 && ld_ag(&status[1]) == 0) {
+        const unsigned gen = ld_ag(&bar[1]);
+        wait_mem();
+        const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == G - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            st_ag(&bar[0], 0u);
+            wait_mem();
+            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             unsigned spins = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(2);
-                if ((++spins & 1023u) == 0 &&
-                    __hip_atomic_load(&status[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-                    break;
-                if (spins > (1u << 22)) {
-                    __hip_atomic_store(&status[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while (ld_ag(&bar[1]) == gen) {
+                __builtin_amdgcn_s_sleep(1);
+                if ((++spins & 1023u) == 0 && ld_ag(&status[1]) != 0) break;
+                if (spins > (1u << 23)) {
+                    st_ag(&status[1], 1);
                     break;
                 }
             }
         }
+        wait_mem();
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // argmax of (value, index): larger value wins, ties go to the smaller index.
@@ -371,9 +399,16 @@ __device__ __forceinline__ void place(C2<R> (&a)[kPanelRPT], int r, C2<R> v) {
         if (q == r) a[q] = v;
 }
 
+// Per-column exchange: every workgroup publishes its best candidate (value,
+// row index, the row's 64 values) in slot [j & 1][b] and then stamps it with
+// seq = epoch + j + 1; wave 0 of every workgroup polls the stamps, reduces the
+// candidates, and all lanes read the winning row and the diagonal row.  The
+// stamps are unique per launch and column, so no counter needs resetting; the
+// two slot parities keep a slow reader's column-j data intact while a fast
+// workgroup publishes column j + 1.
 template <typename R>
 __global__ __launch_bounds__(256) void k_lu_panel(C2<R>* __restrict__ A, int64_t lda, int n, int k0, int w,
-                                                  int* __restrict__ ipiv, PanelWS ws) {
+                                                  int* __restrict__ ipiv, PanelWS ws, unsigned epoch) {
     constexpr int RPT = kPanelRPT;
     const int G = gridDim.x, b = blockIdx.x;
     const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
@@ -381,6 +416,7 @@ __global__ __launch_bounds__(256) void k_lu_panel(C2<R>* __restrict__ A, int64_t
     const int base = wg0 + g * RPT;       // first row of this lane
     C2<R>* cand_row = (C2<R>*)ws.cand_row;
     C2<R>* diag_row = (C2<R>*)ws.diag_row;
+    unsigned* seqs = ws.bar;              // [2][G] stamps
     __shared__ double s_v[4];
     __shared__ int s_i[4];
     __shared__ int s_win, s_blk;
@@ -393,98 +429,107 @@ __global__ __launch_bounds__(256) void k_lu_panel(C2<R>* __restrict__ A, int64_t
         const int row = base + r;
         a[r] = (row < n && c < w) ? A[(int64_t)row * lda + k0 + c] : C2<R>{0, 0};
     }
+#pragma unroll 1
+    for (int j = 0; j < w; ++j) {
+        const int par = j & 1;
+        const unsigned stamp = epoch + (unsigned)j + 1u;
+        // -- candidates in column j among rows >= k0 + j (first maximum wins)
+        if (c == j) {
+            double bv = -1.0;
+            int bi = INT_MAX;
 #pragma unroll
-    for (int j = 0; j < kNB; ++j) {
-        if (j < w) {
-            const int par = j & 1;
-            // -- candidates in column j among rows >= k0 + j (first maximum wins)
-            if (c == j) {
-                double bv = -1.0;
-                int bi = INT_MAX;
-#pragma unroll
-                for (int r = 0; r < RPT; ++r) {
-                    const int row = base + r;
-                    if (row < n && row >= k0 + j) {
-                        const double v = (double)cabs1(a[r]);
-                        if (v > bv) { bv = v; bi = row; }
+            for (int r = 0; r < RPT; ++r) {
+                const int row = base + r;
+                if (row < n && row >= k0 + j) {
+                    const double v = (double)cabs1(a[r]);
+                    if (v > bv) { bv = v; bi = row; }
+                }
+            }
+            s_v[g] = bv;
+            s_i[g] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double bv = s_v[0];
+            int bi = s_i[0];
+            for (int q = 1; q < 4; ++q)
+                if (s_v[q] > bv || (s_v[q] == bv && s_i[q] < bi)) { bv = s_v[q]; bi = s_i[q]; }
+            st_ag(&ws.cand_val[par * G + b], bv);
+            st_ag(&ws.cand_idx[par * G + b], bi);
+            s_win = bi;
+        }
+        __syncthreads();
+        {
+            const int wrow = s_win;
+            if (wrow != INT_MAX) {
+                const int rel = wrow - wg0;
+                if (g == rel / RPT) st_ag(&cand_row[(size_t)(par * G + b) * kNB + c], pick(a, rel % RPT));
+            }
+        }
+        if (b == 0 && g == j / RPT) st_ag(&diag_row[par * kNB + c], pick(a, j % RPT));
+        wait_mem();
+        __syncthreads();
+        if (tid == 0) st_ag(&seqs[par * G + b], stamp);
+        // -- global pivot: poll every stamp, then reduce the candidates
+        if (g == 0) {
+            double bv = -2.0;
+            int bi = INT_MAX, bb = 0;
+            for (int q = c; q < G; q += 64) {
+                unsigned spins = 0;
+                while (ld_ag(&seqs[par * G + q]) != stamp) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > (1u << 23)) {
+                        st_ag(&ws.status[1], 1);
+                        break;
                     }
                 }
-                s_v[g] = bv;
-                s_i[g] = bi;
+                const double vv = ld_ag(&ws.cand_val[par * G + q]);
+                const int ii = ld_ag(&ws.cand_idx[par * G + q]);
+                if (vv > bv || (vv == bv && ii < bi)) { bv = vv; bi = ii; bb = q; }
             }
-            __syncthreads();
-            if (tid == 0) {
-                double bv = s_v[0];
-                int bi = s_i[0];
-                for (int q = 1; q < 4; ++q)
-                    if (s_v[q] > bv || (s_v[q] == bv && s_i[q] < bi)) { bv = s_v[q]; bi = s_i[q]; }
-                ws.cand_val[par * G + b] = bv;
-                ws.cand_idx[par * G + b] = bi;
-                s_win = bi;
-            }
-            __syncthreads();
-            {
-                const int wrow = s_win;
-                if (wrow != INT_MAX) {
-                    const int rel = wrow - wg0;
-                    if (g == rel / RPT) cand_row[(size_t)(par * G + b) * kNB + c] = pick(a, rel % RPT);
-                }
-            }
-            if (b == 0 && g == j / RPT) diag_row[par * kNB + c] = a[j % RPT];
-            grid_barrier(ws.bar, (unsigned)G, ws.status);
-            // -- global pivot: every workgroup reduces the same candidates
-            if (g == 0) {
-                double bv = -2.0;
-                int bi = INT_MAX, bb = 0;
-                for (int q = c; q < G; q += 64) {
-                    const double vv = ws.cand_val[par * G + q];
-                    const int ii = ws.cand_idx[par * G + q];
-                    if (vv > bv || (vv == bv && ii < bi)) { bv = vv; bi = ii; bb = q; }
-                }
-                wave_argmax(bv, bi, bb);
-                if (c == 0) { s_win = bi; s_blk = bb; }
-            }
-            __syncthreads();
-            const int p = s_win, pb = s_blk;
-            if (tid < kNB) {
-                s_prow[tid] = cand_row[(size_t)(par * G + pb) * kNB + tid];
-                s_diag[tid] = diag_row[par * kNB + tid];
-            }
-            __syncthreads();
-            const C2<R> pv = s_prow[j];
-            const C2<R> prc = s_prow[c];
-            // -- interchange rows k0 + j and p
-            if (p != k0 + j) {
-                if (b == 0 && g == j / RPT) a[j % RPT] = prc;
-                const int rel = p - wg0;
-                if (rel >= 0 && rel < kPanelRows && g == rel / RPT) place(a, rel % RPT, s_diag[c]);
-            }
-            if (b == 0 && tid == 0) {
-                ipiv[k0 + j] = p;
-                if (pv.x == 0 && pv.y == 0) atomicMin(&ws.status[0], k0 + j);
-            }
-            // -- multipliers (lanes of column j), then the rank-1 update
-            if (c == j) {
-                const bool nz = pv.x != 0 || pv.y != 0;
-                const C2<R> ip = nz ? cinv(pv) : C2<R>{0, 0};
+            wave_argmax(bv, bi, bb);
+            if (c == 0) { s_win = bi; s_blk = bb; }
+        }
+        __syncthreads();
+        const int p = s_win, pb = s_blk;
+        if (tid < kNB) {
+            s_prow[tid] = ld_ag(&cand_row[(size_t)(par * G + pb) * kNB + tid]);
+            s_diag[tid] = ld_ag(&diag_row[par * kNB + tid]);
+        }
+        __syncthreads();
+        const C2<R> pv = s_prow[j];
+        const C2<R> prc = s_prow[c];
+        // -- interchange rows k0 + j and p
+        if (p != k0 + j) {
+            if (b == 0 && g == j / RPT) place(a, j % RPT, prc);
+            const int rel = p - wg0;
+            if (rel >= 0 && rel < kPanelRows && g == rel / RPT) place(a, rel % RPT, s_diag[c]);
+        }
+        if (b == 0 && tid == 0) {
+            ipiv[k0 + j] = p;
+            if (pv.x == 0 && pv.y == 0) atomicMin(&ws.status[0], k0 + j);
+        }
+        // -- multipliers (lanes of column j), then the rank-1 update
+        if (c == j) {
+            const bool nz = pv.x != 0 || pv.y != 0;
+            const C2<R> ip = nz ? cinv(pv) : C2<R>{0, 0};
 #pragma unroll
-                for (int r = 0; r < RPT; ++r) {
-                    const int row = base + r;
-                    C2<R> l{0, 0};
-                    if (row < n && row > k0 + j) {
-                        l = cmul(a[r], ip);
-                        a[r] = l;
-                    }
-                    s_l[g * RPT + r] = l;
+            for (int r = 0; r < RPT; ++r) {
+                const int row = base + r;
+                C2<R> l{0, 0};
+                if (row < n && row > k0 + j) {
+                    l = cmul(a[r], ip);
+                    a[r] = l;
                 }
+                s_l[g * RPT + r] = l;
             }
-            __syncthreads();
-            if (c > j && c < w) {
+        }
+        __syncthreads();
+        if (c > j && c < w) {
 #pragma unroll
-                for (int r = 0; r < RPT; ++r) {
-                    const int row = base + r;
-                    if (row < n && row > k0 + j) a[r] = csub(a[r], cmul(s_l[g * RPT + r], prc));
-                }
+            for (int r = 0; r < RPT; ++r) {
+                const int row = base + r;
+                if (row < n && row > k0 + j) a[r] = csub(a[r], cmul(s_l[g * RPT + r], prc));
             }
         }
     }
@@ -495,13 +540,18 @@ __global__ __launch_bounds__(256) void k_lu_panel(C2<R>* __restrict__ A, int64_t
     }
 }
 
-// Row interchanges of the panel's pivots on columns outside the panel.
+// Row interchanges of pivots ipiv[k0 .. k0+w) applied, in order, to the
+// columns [cb0, ce0) and [cb1, ce1); each thread owns one column, so every
+// interchange touches two contiguous row segments across the grid.
 template <typename R>
-__global__ __launch_bounds__(256) void k_laswp(C2<R>* __restrict__ A, int64_t lda, int n, int k0, int w,
-                                               const int* __restrict__ ipiv) {
-    int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= n - w) return;
-    const int col = t < k0 ? t : t + w;
+__global__ __launch_bounds__(256) void k_laswp(C2<R>* __restrict__ A, int64_t lda, int k0, int w,
+                                               const int* __restrict__ ipiv, int cb0, int ce0, int cb1, int ce1) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int n0 = ce0 - cb0;
+    int col;
+    if (t < n0) col = cb0 + t;
+    else if (t - n0 < ce1 - cb1) col = cb1 + (t - n0);
+    else return;
     for (int s = 0; s < w; ++s) {
         const int r = k0 + s, p = ipiv[r];
         if (p != r) {
@@ -514,64 +564,95 @@ __global__ __launch_bounds__(256) void k_laswp(C2<R>* __restrict__ A, int64_t ld
 
 // Inverses of the diagonal block's unit-lower L and upper U (bs <= 64),
 // stored row-major as inv[(2 kb) * 4096] (L) and inv[(2 kb + 1) * 4096] (U).
+// Four lanes own one column and split each substitution dot product by the
+// summation index mod 4 (combined with two xor shuffles), so no block-wide
+// barrier is needed inside a substitution.
 template <typename R>
-__global__ __launch_bounds__(64) void k_trtri(const C2<R>* __restrict__ A, int64_t lda, int k0, int bs,
-                                              C2<R>* __restrict__ inv) {
-    __shared__ C2<R> D[kNB][kNB];
-    __shared__ C2<R> X[kNB][kNB];
-    const int c = threadIdx.x;
-    for (int r = 0; r < kNB; ++r)
+__global__ __launch_bounds__(256) void k_trtri(const C2<R>* __restrict__ A, int64_t lda, int k0, int bs,
+                                               C2<R>* __restrict__ inv) {
+    __shared__ C2<R> D[kNB][kNB + 1];
+    __shared__ C2<R> X[kNB][kNB + 1];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kNB * kNB; e += 256) {
+        const int r = e / kNB, c = e % kNB;
         D[r][c] = (r < bs && c < bs) ? A[(int64_t)(k0 + r) * lda + k0 + c] : C2<R>{0, 0};
+    }
     __syncthreads();
-    // column c of inv(L): x_r = delta_rc - sum_{s<r} L_rs x_s
+    const int c = tid >> 2;  // column
+    const int h = tid & 3;   // summation index mod 4
+    C2<R>* out = inv + (size_t)(2 * (k0 / kNB)) * kNB * kNB;
+    // inv(L): x_r = delta_rc - sum_{c <= s < r} L_rs x_s
     for (int r = 0; r < kNB; ++r) {
-        C2<R> v{0, 0};
-        if (r >= c && r < bs) {
-            v = C2<R>{r == c ? R(1) : R(0), R(0)};
-            for (int s = c; s < r; ++s) v = csub(v, cmul(D[r][s], X[s][c]));
+        double sx = 0, sy = 0;
+        if (r > c && r < bs) {
+            for (int s2 = c + h; s2 < r; s2 += 4) {
+                const C2<R> d = D[r][s2], x = X[s2][c];
+                sx += (double)d.x * x.x - (double)d.y * x.y;
+                sy += (double)d.x * x.y + (double)d.y * x.x;
+            }
         }
-        X[r][c] = v;
+        sx += __shfl_xor(sx, 1, 64);
+        sy += __shfl_xor(sy, 1, 64);
+        sx += __shfl_xor(sx, 2, 64);
+        sy += __shfl_xor(sy, 2, 64);
+        if (h == 0) {
+            C2<R> v{0, 0};
+            if (r < bs && r >= c) v = (r == c) ? C2<R>{1, 0} : C2<R>{(R)-sx, (R)-sy};
+            X[r][c] = v;
+        }
     }
     __syncthreads();
-    C2<R>* Li = inv + (size_t)(2 * (k0 / kNB)) * kNB * kNB;
-    for (int r = 0; r < kNB; ++r) Li[r * kNB + c] = X[r][c];
+    for (int e = tid; e < kNB * kNB; e += 256) out[e] = X[e / kNB][e % kNB];
     __syncthreads();
-    // column c of inv(U): back substitution
+    // inv(U): x_r = (delta_rc - sum_{r < s <= c} U_rs x_s) / U_rr
     for (int r = kNB - 1; r >= 0; --r) {
-        C2<R> v{0, 0};
-        if (r <= c && c < bs) {
-            v = C2<R>{r == c ? R(1) : R(0), R(0)};
-            for (int s = r + 1; s <= c; ++s) v = csub(v, cmul(D[r][s], X[s][c]));
-            v = cmul(v, cinv(D[r][r]));
+        double sx = 0, sy = 0;
+        if (r < c && c < bs) {
+            for (int s2 = r + 1 + h; s2 <= c; s2 += 4) {
+                const C2<R> d = D[r][s2], x = X[s2][c];
+                sx += (double)d.x * x.x - (double)d.y * x.y;
+                sy += (double)d.x * x.y + (double)d.y * x.x;
+            }
         }
-        X[r][c] = v;
+        sx += __shfl_xor(sx, 1, 64);
+        sy += __shfl_xor(sy, 1, 64);
+        sx += __shfl_xor(sx, 2, 64);
+        sy += __shfl_xor(sy, 2, 64);
+        if (h == 0) {
+            C2<R> v{0, 0};
+            if (r <= c && c < bs) {
+                const C2<R> num = (r == c) ? C2<R>{1, 0} : C2<R>{(R)-sx, (R)-sy};
+                v = cmul(num, cinv(D[r][r]));
+            }
+            X[r][c] = v;
+        }
     }
     __syncthreads();
-    C2<R>* Ui = Li + kNB * kNB;
-    for (int r = 0; r < kNB; ++r) Ui[r * kNB + c] = X[r][c];
+    for (int e = tid; e < kNB * kNB; e += 256) out[kNB * kNB + e] = X[e / kNB][e % kNB];
 }
 
-// U12 = inv(L11) A12 on 64-column strips, in place.
+// Rows k0 .. k0+bs of columns [cb, ce): X = inv(L_kb) X on 64-column strips,
+// in place (L_kb is the unit-lower diagonal block of block k0 / 64).
 template <typename R>
-__global__ __launch_bounds__(256) void k_trsm(C2<R>* __restrict__ A, int64_t lda, int n, int k0, int bs,
+__global__ __launch_bounds__(256) void k_trsm(C2<R>* __restrict__ A, int64_t lda, int k0, int bs, int cb, int ce,
                                               const C2<R>* __restrict__ inv) {
     __shared__ C2<R> Li[kNB][kNB];
     __shared__ C2<R> S[kNB][kNB];
-    const int c0 = k0 + bs + blockIdx.x * kNB;
+    const int c0 = cb + blockIdx.x * kNB;
     const int tid = threadIdx.x;
     const C2<R>* Lg = inv + (size_t)(2 * (k0 / kNB)) * kNB * kNB;
     for (int e = tid; e < kNB * kNB; e += 256) {
         const int r = e / kNB, c = e % kNB;
         Li[r][c] = Lg[e];
-        S[r][c] = (r < bs && c0 + c < n) ? A[(int64_t)(k0 + r) * lda + c0 + c] : C2<R>{0, 0};
+        S[r][c] = (r < bs && c0 + c < ce) ? A[(int64_t)(k0 + r) * lda + c0 + c] : C2<R>{0, 0};
     }
     __syncthreads();
     const int c = tid & 63, rg = tid >> 6;
     for (int rr = 0; rr < 16; ++rr) {
         const int r = rg * 16 + rr;
-        if (r >= bs || c0 + c >= n) continue;
+        if (r >= bs || c0 + c >= ce) continue;
         C2<R> v{0, 0};
-        for (int s = 0; s <= r; ++s) v = cadd(v, cmul(Li[r][s], S[s][c]));
+        for (int s2 = 0; s2 <= r; ++s2) v = cadd(v, cmul(Li[r][s2], S[s2][c]));
         A[(int64_t)(k0 + r) * lda + c0 + c] = v;
     }
 }
@@ -714,41 +795,81 @@ int panel_capacity(int cu_count) {
 }
 
 template <typename R>
+static void launch_laswp(hipStream_t s, C2<R>* A, int64_t lda, int k0, int w, const int* ipiv, int cb0, int ce0,
+                         int cb1, int ce1, void* prof) {
+    const int cols = (ce0 - cb0) + (ce1 - cb1);
+    if (cols <= 0) return;
+    prof_begin(prof, s, CLS_SWAP);
+    hipLaunchKernelGGL((k_laswp<R>), dim3((cols + 255) / 256), dim3(256), 0, s, A, lda, k0, w, ipiv, cb0, ce0, cb1, ce1);
+    prof_end(prof, s, CLS_SWAP, 0.0, 4.0 * (double)cols * w * sizeof(C2<R>));
+}
+
+template <typename R>
+static void launch_trsm(hipStream_t s, C2<R>* A, int64_t lda, int k0, int bs, int cb, int ce, const C2<R>* inv,
+                        void* prof) {
+    if (ce <= cb) return;
+    prof_begin(prof, s, CLS_TRSM);
+    hipLaunchKernelGGL((k_trsm<R>), dim3((ce - cb + kNB - 1) / kNB), dim3(256), 0, s, A, lda, k0, bs, cb, ce, inv);
+    prof_end(prof, s, CLS_TRSM, 8.0 * (double)bs * bs * (ce - cb) / 2.0, 2.0 * (double)bs * (ce - cb) * sizeof(C2<R>));
+}
+
+// C[M x N] -= A[M x K] B[K x N] with C at (r0, c0), A at (r0, ka), B at (ka, c0).
+template <typename R>
+static void launch_update(hipStream_t s, C2<R>* A, int64_t lda, int r0, int c0, int ka, int M, int N, int K,
+                          void* prof) {
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    prof_begin(prof, s, CLS_GEMM);
+    dim3 grid((N + kGN - 1) / kGN, (M + kGM - 1) / kGM);
+    hipLaunchKernelGGL((k_gemm_sub<R>), grid, dim3(512), 0, s, A + (int64_t)r0 * lda + ka, lda,
+                       A + (int64_t)ka * lda + c0, lda, A + (int64_t)r0 * lda + c0, lda, M, N, K);
+    prof_end(prof, s, CLS_GEMM, 8.0 * (double)M * N * K, (2.0 * (double)M * N + (double)(M + N) * K) * sizeof(C2<R>));
+}
+
+// Two-level right-looking LU: 64-wide panels inside 256-wide outer blocks, so
+// the trailing update of each outer block is one K = 256 MFMA GEMM.
+constexpr int kOuter = 256;
+
+template <typename R>
 int lu_factor(hipStream_t s, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv, PanelWS& ws, void* prof) {
-    for (int k0 = 0; k0 < n; k0 += kNB) {
-        const int w = (n - k0) < kNB ? (n - k0) : kNB;
-        const int m = n - k0;
-        const int G = (m + kPanelRows - 1) / kPanelRows;
-        if (G > ws.max_blocks) return 2;
-        int k0v = k0, wv = w, nv = n;
-        int64_t ldv = lda;
-        C2<R>* Av = A;
-        int* ipv = ipiv;
-        PanelWS wsv = ws;
-        void* args[] = {&Av, &ldv, &nv, &k0v, &wv, &ipv, &wsv};
-        prof_begin(prof, s, CLS_PANEL);
-        if (hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_lu_panel<R>), dim3(G), dim3(256), args, 0, s) !=
-            hipSuccess)
-            return 3;
-        prof_end(prof, s, CLS_PANEL, 8.0 * (double)m * w * w / 2.0, 2.0 * (double)m * w * sizeof(C2<R>));
-        if (n - w > 0) {
-            prof_begin(prof, s, CLS_SWAP);
-            hipLaunchKernelGGL((k_laswp<R>), dim3((n - w + 255) / 256), dim3(256), 0, s, A, lda, n, k0, w, ipiv);
-            prof_end(prof, s, CLS_SWAP, 0.0, 4.0 * (double)(n - w) * w * sizeof(C2<R>));
+    for (int K0 = 0; K0 < n; K0 += kOuter) {
+        const int KW = (n - K0) < kOuter ? (n - K0) : kOuter;
+        const int Kend = K0 + KW;
+        for (int k0 = K0; k0 < Kend; k0 += kNB) {
+            const int w = (Kend - k0) < kNB ? (Kend - k0) : kNB;
+            const int m = n - k0;
+            const int G = (m + kPanelRows - 1) / kPanelRows;
+            if (G > ws.max_blocks) return 2;
+            int k0v = k0, wv = w, nv = n;
+            int64_t ldv = lda;
+            C2<R>* Av = A;
+            int* ipv = ipiv;
+            PanelWS wsv = ws;
+            unsigned epoch = ws.epoch;
+            ws.epoch += kNB;
+            void* args[] = {&Av, &ldv, &nv, &k0v, &wv, &ipv, &wsv, &epoch};
+            prof_begin(prof, s, CLS_PANEL);
+            if (hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_lu_panel<R>), dim3(G), dim3(256), args, 0,
+                                           s) != hipSuccess)
+                return 3;
+            prof_end(prof, s, CLS_PANEL, 8.0 * (double)m * w * w / 2.0, 2.0 * (double)m * w * sizeof(C2<R>));
+            // this panel's interchanges inside the outer block
+            launch_laswp<R>(s, A, lda, k0, w, ipiv, K0, k0, k0 + w, Kend, prof);
+            hipLaunchKernelGGL((k_trtri<R>), dim3(1), dim3(256), 0, s, A, lda, k0, w, inv);
+            if (k0 + w < Kend) {
+                launch_trsm<R>(s, A, lda, k0, w, k0 + w, Kend, inv, prof);
+                launch_update<R>(s, A, lda, k0 + w, k0 + w, k0, n - k0 - w, Kend - k0 - w, w, prof);
+            }
         }
-        hipLaunchKernelGGL((k_trtri<R>), dim3(1), dim3(kNB), 0, s, A, lda, k0, w, inv);
-        const int rest = n - k0 - w;
-        if (rest > 0) {
-            prof_begin(prof, s, CLS_TRSM);
-            hipLaunchKernelGGL((k_trsm<R>), dim3((rest + kNB - 1) / kNB), dim3(256), 0, s, A, lda, n, k0, w, inv);
-            prof_end(prof, s, CLS_TRSM, 8.0 * (double)w * w * rest / 2.0, 2.0 * (double)w * rest * sizeof(C2<R>));
-            prof_begin(prof, s, CLS_GEMM);
-            dim3 grid((rest + kGN - 1) / kGN, (rest + kGM - 1) / kGM);
-            hipLaunchKernelGGL((k_gemm_sub<R>), grid, dim3(512), 0, s, A + (int64_t)(k0 + w) * lda + k0, lda,
-                               A + (int64_t)k0 * lda + k0 + w, lda, A + (int64_t)(k0 + w) * lda + k0 + w, lda, rest,
-                               rest, w);
-            prof_end(prof, s, CLS_GEMM, 8.0 * (double)rest * rest * w,
-                     (2.0 * (double)rest * rest + 2.0 * (double)rest * w) * sizeof(C2<R>));
+        // the outer block's interchanges on every other column
+        launch_laswp<R>(s, A, lda, K0, KW, ipiv, 0, K0, Kend, n, prof);
+        if (Kend < n) {
+            // U12 = inv(L11) A12 by 64-row blocks, then the K = KW trailing update
+            for (int i0 = K0; i0 < Kend; i0 += kNB) {
+                const int wi = (Kend - i0) < kNB ? (Kend - i0) : kNB;
+                launch_trsm<R>(s, A, lda, i0, wi, Kend, n, inv, prof);
+                launch_update<R>(s, A, lda, i0 + wi, Kend, i0, Kend - i0 - wi, n - Kend, wi, prof);
+            }
+            launch_update<R>(s, A, lda, Kend, Kend, K0, n - Kend, n - Kend, KW, prof);
         }
     }
     return 0;
