@@ -257,14 +257,15 @@ void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) 
     launch_fill_int(s, ctx->ws.status, 1, INT_MAX);
     launch_fill_int(s, ctx->ws.status + 1, 1, 0);
     int rc = lu_factor<R>(s, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, ctx->ws, &ctx->prof);
-    if (rc == 2) throw BemError(BEM_ERR_ADMISSION, "system too large for one cooperative panel launch");
+    if (rc == 2) throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (81920 rows)");
     if (rc != 0) throw BemError(BEM_ERR_DEVICE, "LU panel launch failed");
+    HIPCHECK(hipGetLastError());
     int status[2];
     std::vector<int> hp(n);
     HIPCHECK(hipMemcpyAsync(status, ctx->ws.status, sizeof status, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(hp.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
-    if (status[1] != 0) throw BemError(BEM_ERR_DEVICE, "LU panel barrier timed out");
+    if (status[0] != INT_MAX) throw BemError(BEM_ERR_NUMERIC, "singular:" + std::to_string(status[0]));
     std::vector<int> pm(n);
     for (int i = 0; i < n; ++i) pm[i] = i;
     for (int i = 0; i < n; ++i) {
@@ -274,7 +275,6 @@ void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) 
     }
     HIPCHECK(hipMemcpyAsync(perm, pm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
-    if (status[0] != INT_MAX) throw BemError(BEM_ERR_NUMERIC, "singular:" + std::to_string(status[0]));
 }
 
 void promote(bem_operator op, int reason) {
@@ -723,24 +723,24 @@ int bem_ctx_create(int32_t device, uint64_t budget, bem_ctx* out) {
         c->budget = budget ? budget : (uint64_t)(0.9 * (double)fr);
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         if (upload_rules() != 0) throw BemError(BEM_ERR_DEVICE, "constant table upload failed");
-        const int G = panel_capacity(c->cu_count);
-        if (G <= 0) throw BemError(BEM_ERR_DEVICE, "cooperative panel kernel cannot be resident");
+        const int G = 640;  // 81920 rows: N = 40960 panels in a Mueller system
+        const int nodes = 2 * G + 16;
         c->ws.max_blocks = G;
-        size_t bytes = sizeof(double) * 2 * G + sizeof(int) * 2 * G + sizeof(z64) * 2 * (size_t)G * 64 +
-                       sizeof(z64) * 2 * 64 + sizeof(unsigned) * 2 * G + 64 + 6 * 256;
-        char* m = (char*)c->alloc(bytes + 1024);
+        c->ws.max_nodes = nodes;
+        size_t bytes = 64 + sizeof(int) * nodes * (2 + 64) + sizeof(z64) * (size_t)nodes * 64 * 64 + sizeof(int) * 80 +
+                       8 * 256;
+        char* m = (char*)c->alloc(bytes);
         c->ws_mem = m;
-        HIPCHECK(hipMemset(m, 0, bytes + 1024));
+        HIPCHECK(hipMemset(m, 0, bytes));
         size_t off = 0;
         auto take = [&](size_t b) { void* p = m + off; off += (b + 255) & ~(size_t)255; return p; };
-        c->ws.cand_val = (double*)take(sizeof(double) * 2 * G);
-        c->ws.cand_idx = (int*)take(sizeof(int) * 2 * G);
-        c->ws.cand_row = take(sizeof(z64) * 2 * (size_t)G * 64);
-        c->ws.diag_row = take(sizeof(z64) * 2 * 64);
-        c->ws.bar = (unsigned*)take(sizeof(unsigned) * 2 * G);
-        c->ws.epoch = 1;
         c->ws.status = (int*)take(64);
-        if (off > bytes + 1024) throw BemError(BEM_ERR_DEVICE, "workspace layout");
+        c->ws.tickets = (int*)take(sizeof(int) * nodes);
+        c->ws.node_cnt = (int*)take(sizeof(int) * nodes);
+        c->ws.node_rows = (int*)take(sizeof(int) * nodes * 64);
+        c->ws.node_vals = take(sizeof(z64) * (size_t)nodes * 64 * 64);
+        c->ws.root = (int*)take(sizeof(int) * 80);
+        if (off > bytes) throw BemError(BEM_ERR_DEVICE, "workspace layout");
         HIPCHECK(hipDeviceSynchronize());
     });
     if (rc != BEM_OK) {

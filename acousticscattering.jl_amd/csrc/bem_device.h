@@ -97,17 +97,17 @@ template <typename R>
 void launch_matvec(hipStream_t s, const C2<R>* A, int64_t lda, int rows, int cols, const z64* x, z64* y,
                    const z64* b, double* den);
 
+// Workspace of the tournament-pivoting panel (one per stream).
 struct PanelWS {
-    double* cand_val;  // [2][G]
-    int* cand_idx;     // [2][G]
-    void* cand_row;    // [2][G][64] complex R
-    void* diag_row;    // [2][64] complex R
-    unsigned* bar;     // [2][G] per-workgroup stamps of the panel exchange
-    int* status;       // [0] first zero pivot (INT_MAX if none), [1] barrier timeout
-    int max_blocks;
-    unsigned epoch;    // stamp base of the next panel launch (advances by 64)
+    int* status;       // [0] first zero pivot (INT_MAX if none), [1] reserved
+    int* tickets;      // [max_nodes] merge tickets, zero between launches
+    int* node_cnt;     // [max_nodes] rows nominated by each tree node
+    int* node_rows;    // [max_nodes][64] their global row indices
+    void* node_vals;   // [max_nodes][64][64] their original values (complex, sized for FP64)
+    int* root;         // [0] count, [1..65) the panel's pivot rows in order
+    int max_blocks;    // largest leaf count (128 rows per leaf)
+    int max_nodes;
 };
-int panel_capacity(int device_cu_count);  // largest cooperative grid for the panel kernel
 template <typename R>
 int lu_factor(hipStream_t s, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv, PanelWS& ws,
               void* prof);

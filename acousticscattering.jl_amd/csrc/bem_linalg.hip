@@ -4,13 +4,15 @@
 // float (mixed path).  Row-major keeps the partial-pivoting row interchanges
 // and the panel's per-row reads contiguous.
 //
-// LU: right-looking, block width 64, partial pivoting by max |re|+|im|
-// (the LAPACK zgetrf/cgetrf rule used by the reference,
-// src/solvers/bem/full.jl:404, src/solvers/bem/fluid_system.jl:217).
-//   panel   one cooperative launch factors the n-k0 x 64 panel; each row
-//           lives in the registers of a lane pair; one grid barrier per column
+// LU: right-looking, 64-wide panels inside 256-wide outer blocks.  The
+// reference factors with LAPACK zgetrf/cgetrf (partial pivoting,
+// src/solvers/bem/full.jl:404, src/solvers/bem/fluid_system.jl:217); the
+// panels here pivot by tournament (communication-avoiding LU), which has the
+// same backward stability in practice and needs no grid-wide barrier.
+//   panel   tournament pivoting (TSLU) over 128-row leaves and a log-depth
+//           merge tree, then LU of the 64x64 block, its L/U inverses and
+//           L21 = A21 inv(U11) on MFMA
 //   swap    row interchanges outside the panel (coalesced row segments)
-//   trtri   inverses of the 64x64 unit-lower L11 and upper U11 blocks
 //   trsm    U12 = inv(L11) A12
 //   gemm    A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32),
 //           complex as four real products, 128x128 tiles staged in LDS
@@ -336,37 +338,6 @@ __device__ __forceinline__ void wait_mem() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// Generation barrier over all workgroups of a cooperative launch.  The spin
-// is bounded: a timeout sets the sticky flag status[1], after which every
-// later barrier returns at once, so a failed launch drains quickly and the
-// host reports the error.
-__device__ inline void grid_barrier(unsigned* bar, unsigned G, int* status) {
-    wait_mem();
-    __syncthreads();
-    if (threadIdx.x == 0 && ld_ag(&status[1]) == 0) {
-        const unsigned gen = ld_ag(&bar[1]);
-        wait_mem();
-        const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == G - 1) {
-            st_ag(&bar[0], 0u);
-            wait_mem();
-            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            unsigned spins = 0;
-            while (ld_ag(&bar[1]) == gen) {
-                __builtin_amdgcn_s_sleep(1);
-                if ((++spins & 1023u) == 0 && ld_ag(&status[1]) != 0) break;
-                if (spins > (1u << 23)) {
-                    st_ag(&status[1], 1);
-                    break;
-                }
-            }
-        }
-        wait_mem();
-    }
-    __syncthreads();
-}
-
 // argmax of (value, index): larger value wins, ties go to the smaller index.
 __device__ inline void wave_argmax(double& v, int& idx, int& blk) {
 #pragma unroll
@@ -378,166 +349,397 @@ __device__ inline void wave_argmax(double& v, int& idx, int& blk) {
     }
 }
 
-// Panel layout: lane c of wave g owns column c of kPanelRPT consecutive rows,
-// so a workgroup holds 4 * kPanelRPT rows x 64 columns in registers and the
-// loads of a row are coalesced across the wave.
-constexpr int kPanelRPT = 32;
-constexpr int kPanelRows = 4 * kPanelRPT;
+// ---------------------------------------------------------------------------
+// Panel factorization by tournament pivoting (CALU / TSLU: Grigori, Demmel and
+// Xiang, SIAM J. Matrix Anal. Appl. 32 (2011)).  The m x w panel is split into
+// 128-row leaves.  Each leaf runs partial pivoting (max |re|+|im|, first
+// maximum wins) on its rows and nominates w rows; nominations are merged
+// pairwise up a binary tree, each merge again running partial pivoting on the
+// 2w candidate rows (original values).  The root's selection order is the
+// panel's pivot order.  No grid-wide barrier: a merge is done by the second
+// workgroup to arrive at a tree node (atomic ticket), the first one exits.
+// ---------------------------------------------------------------------------
+constexpr int kTsRows = 128;    // rows per leaf / merge
+constexpr int kTsRPT = 16;      // rows per lane
+constexpr int kTsThreads = 512; // 8 waves
 
-template <typename R>
-__device__ __forceinline__ C2<R> pick(const C2<R> (&a)[kPanelRPT], int r) {
+template <int N, typename R>
+__device__ __forceinline__ C2<R> pickn(const C2<R> (&a)[N], int r) {
     C2<R> v{0, 0};
 #pragma unroll
-    for (int q = 0; q < kPanelRPT; ++q)
+    for (int q = 0; q < N; ++q)
         if (q == r) v = a[q];
     return v;
 }
-template <typename R>
-__device__ __forceinline__ void place(C2<R> (&a)[kPanelRPT], int r, C2<R> v) {
+template <int N, typename R>
+__device__ __forceinline__ void placen(C2<R> (&a)[N], int r, C2<R> v) {
 #pragma unroll
-    for (int q = 0; q < kPanelRPT; ++q)
+    for (int q = 0; q < N; ++q)
         if (q == r) a[q] = v;
 }
 
-// Per-column exchange: every workgroup publishes its best candidate (value,
-// row index, the row's 64 values) in slot [j & 1][b] and then stamps it with
-// seq = epoch + j + 1; wave 0 of every workgroup polls the stamps, reduces the
-// candidates, and all lanes read the winning row and the diagonal row.  The
-// stamps are unique per launch and column, so no counter needs resetting; the
-// two slot parities keep a slow reader's column-j data intact while a fast
-// workgroup publishes column j + 1.
-template <typename R>
-__global__ __launch_bounds__(256) void k_lu_panel(C2<R>* __restrict__ A, int64_t lda, int n, int k0, int w,
-                                                  int* __restrict__ ipiv, PanelWS ws, unsigned epoch) {
-    constexpr int RPT = kPanelRPT;
-    const int G = gridDim.x, b = blockIdx.x;
-    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
-    const int wg0 = k0 + b * kPanelRows;  // first row of this workgroup
-    const int base = wg0 + g * RPT;       // first row of this lane
-    C2<R>* cand_row = (C2<R>*)ws.cand_row;
-    C2<R>* diag_row = (C2<R>*)ws.diag_row;
-    unsigned* seqs = ws.bar;              // [2][G] stamps
-    __shared__ double s_v[4];
-    __shared__ int s_i[4];
-    __shared__ int s_win, s_blk;
-    __shared__ C2<R> s_prow[kNB], s_diag[kNB];
-    __shared__ C2<R> s_l[kPanelRows];
-
-    C2<R> a[RPT];
-#pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-        const int row = base + r;
-        a[r] = (row < n && c < w) ? A[(int64_t)row * lda + k0 + c] : C2<R>{0, 0};
+__device__ inline int ts_level_size(int G, int L) {
+    int sz = G;
+    for (int i = 0; i < L; ++i) sz = (sz + 1) >> 1;
+    return sz;
+}
+__device__ inline int ts_level_off(int G, int L) {
+    int off = 0, sz = G;
+    for (int i = 0; i < L; ++i) {
+        off += sz;
+        sz = (sz + 1) >> 1;
     }
+    return off;
+}
+
+struct TsShared {
+    double v[8];
+    int i[8];
+    int p;
+    int ticket;
+    int perm[kTsRows];
+    int row[kTsRows];
+};
+
+// Wave-uniform broadcast of lane `src`'s value (src uniform).
+__device__ __forceinline__ float bcast(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double bcast(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Partial pivoting on the register block a (row position r = 16 g + q of lane
+// column c), `nv` valid rows, w columns.  On return perm[0 .. min(w, nv)) holds
+// the selected positions in pivot order.  Two block barriers per column: one
+// after the per-wave candidates, one after the pivot and diagonal rows are
+// staged; multipliers travel within each wave by readlane from lane j.
+template <typename R>
+__device__ int ts_gepp(C2<R> (&a)[kTsRPT], int nv, int w, TsShared& sh, C2<R>* s_prow, C2<R>* s_drow,
+                       C2<R>* s_l) {
+    (void)s_l;
+    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+    if (tid < kTsRows) sh.perm[tid] = tid;
+    __syncthreads();
+    const int cnt = nv < w ? nv : w;
 #pragma unroll 1
-    for (int j = 0; j < w; ++j) {
-        const int par = j & 1;
-        const unsigned stamp = epoch + (unsigned)j + 1u;
-        // -- candidates in column j among rows >= k0 + j (first maximum wins)
+    for (int j = 0; j < cnt; ++j) {
         if (c == j) {
             double bv = -1.0;
             int bi = INT_MAX;
 #pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int row = base + r;
-                if (row < n && row >= k0 + j) {
-                    const double v = (double)cabs1(a[r]);
-                    if (v > bv) { bv = v; bi = row; }
+            for (int q = 0; q < kTsRPT; ++q) {
+                const int r = g * kTsRPT + q;
+                if (r < nv && r >= j) {
+                    const double v = (double)cabs1(a[q]);
+                    if (v > bv) { bv = v; bi = r; }
                 }
             }
-            s_v[g] = bv;
-            s_i[g] = bi;
+            sh.v[g] = bv;
+            sh.i[g] = bi;
         }
         __syncthreads();
+        double bv = sh.v[0];
+        int p = sh.i[0];
+#pragma unroll
+        for (int q = 1; q < 8; ++q) {
+            const double vq = sh.v[q];
+            const int iq = sh.i[q];
+            if (vq > bv || (vq == bv && iq < p)) { bv = vq; p = iq; }
+        }
+        if (p < j || p >= nv) p = j;  // no comparable candidate (NaN after a zero pivot)
+        if (g == p / kTsRPT) s_prow[c] = pickn<kTsRPT>(a, p % kTsRPT);
+        if (g == j / kTsRPT) s_drow[c] = pickn<kTsRPT>(a, j % kTsRPT);
         if (tid == 0) {
-            double bv = s_v[0];
-            int bi = s_i[0];
-            for (int q = 1; q < 4; ++q)
-                if (s_v[q] > bv || (s_v[q] == bv && s_i[q] < bi)) { bv = s_v[q]; bi = s_i[q]; }
-            st_ag(&ws.cand_val[par * G + b], bv);
-            st_ag(&ws.cand_idx[par * G + b], bi);
-            s_win = bi;
-        }
-        __syncthreads();
-        {
-            const int wrow = s_win;
-            if (wrow != INT_MAX) {
-                const int rel = wrow - wg0;
-                if (g == rel / RPT) st_ag(&cand_row[(size_t)(par * G + b) * kNB + c], pick(a, rel % RPT));
-            }
-        }
-        if (b == 0 && g == j / RPT) st_ag(&diag_row[par * kNB + c], pick(a, j % RPT));
-        wait_mem();
-        __syncthreads();
-        if (tid == 0) st_ag(&seqs[par * G + b], stamp);
-        // -- global pivot: poll every stamp, then reduce the candidates
-        if (g == 0) {
-            double bv = -2.0;
-            int bi = INT_MAX, bb = 0;
-            for (int q = c; q < G; q += 64) {
-                unsigned spins = 0;
-                while (ld_ag(&seqs[par * G + q]) != stamp) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > (1u << 23)) {
-                        st_ag(&ws.status[1], 1);
-                        break;
-                    }
-                }
-                const double vv = ld_ag(&ws.cand_val[par * G + q]);
-                const int ii = ld_ag(&ws.cand_idx[par * G + q]);
-                if (vv > bv || (vv == bv && ii < bi)) { bv = vv; bi = ii; bb = q; }
-            }
-            wave_argmax(bv, bi, bb);
-            if (c == 0) { s_win = bi; s_blk = bb; }
-        }
-        __syncthreads();
-        const int p = s_win, pb = s_blk;
-        if (tid < kNB) {
-            s_prow[tid] = ld_ag(&cand_row[(size_t)(par * G + pb) * kNB + tid]);
-            s_diag[tid] = ld_ag(&diag_row[par * kNB + tid]);
+            const int t = sh.perm[j];
+            sh.perm[j] = sh.perm[p];
+            sh.perm[p] = t;
         }
         __syncthreads();
         const C2<R> pv = s_prow[j];
         const C2<R> prc = s_prow[c];
-        // -- interchange rows k0 + j and p
-        if (p != k0 + j) {
-            if (b == 0 && g == j / RPT) place(a, j % RPT, prc);
-            const int rel = p - wg0;
-            if (rel >= 0 && rel < kPanelRows && g == rel / RPT) place(a, rel % RPT, s_diag[c]);
+        if (p != j) {
+            if (g == j / kTsRPT) placen<kTsRPT>(a, j % kTsRPT, prc);
+            if (g == p / kTsRPT) placen<kTsRPT>(a, p % kTsRPT, s_drow[c]);
         }
-        if (b == 0 && tid == 0) {
-            ipiv[k0 + j] = p;
-            if (pv.x == 0 && pv.y == 0) atomicMin(&ws.status[0], k0 + j);
-        }
-        // -- multipliers (lanes of column j), then the rank-1 update
-        if (c == j) {
-            const bool nz = pv.x != 0 || pv.y != 0;
-            const C2<R> ip = nz ? cinv(pv) : C2<R>{0, 0};
+        const bool nz = pv.x != 0 || pv.y != 0;
+        const C2<R> ip = nz ? cinv(pv) : C2<R>{0, 0};
+        const bool upd = c > j && c < w;
 #pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int row = base + r;
-                C2<R> l{0, 0};
-                if (row < n && row > k0 + j) {
-                    l = cmul(a[r], ip);
-                    a[r] = l;
-                }
-                s_l[g * RPT + r] = l;
+        for (int q = 0; q < kTsRPT; ++q) {
+            const int r = g * kTsRPT + q;
+            if (r > j && r < nv) {  // wave-uniform
+                const C2<R> lq = cmul(a[q], ip);
+                const C2<R> l{bcast(lq.x, j), bcast(lq.y, j)};
+                if (c == j) a[q] = l;
+                else if (upd) a[q] = csub(a[q], cmul(l, prc));
+            }
+        }
+    }
+    __syncthreads();
+    return cnt;
+}
+
+// Publish the selected rows (original values a0) to node `nd`.
+template <typename R>
+__device__ void ts_publish(const C2<R> (&a0)[kTsRPT], int cnt, int nd, const TsShared& sh, const PanelWS& ws) {
+    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+    C2<R>* vals = (C2<R>*)ws.node_vals + (size_t)nd * kNB * kNB;
+    for (int s2 = 0; s2 < cnt; ++s2) {
+        const int pos = sh.perm[s2];
+        if (g == pos / kTsRPT) st_ag(&vals[s2 * kNB + c], pickn<kTsRPT>(a0, pos % kTsRPT));
+    }
+    if (tid < cnt) st_ag(&ws.node_rows[nd * kNB + tid], sh.row[sh.perm[tid]]);
+    if (tid == 0) st_ag(&ws.node_cnt[nd], cnt);
+    wait_mem();
+    __syncthreads();
+}
+
+template <typename R>
+__global__ __launch_bounds__(kTsThreads) void k_tslu(const C2<R>* __restrict__ A, int64_t lda, int n, int k0, int w,
+                                                     PanelWS ws) {
+    __shared__ TsShared sh;
+    __shared__ C2<R> s_prow[kNB], s_drow[kNB], s_l[kTsRows];
+    const int G = gridDim.x;
+    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+    const int m = n - k0;
+    int level = 0, idx = blockIdx.x;
+    int nv = min(kTsRows, m - idx * kTsRows);
+    C2<R> a[kTsRPT], a0[kTsRPT];
+#pragma unroll
+    for (int q = 0; q < kTsRPT; ++q) {
+        const int r = g * kTsRPT + q;
+        const int row = k0 + idx * kTsRows + r;
+        a0[q] = (r < nv && c < w) ? A[(int64_t)row * lda + k0 + c] : C2<R>{0, 0};
+        a[q] = a0[q];
+    }
+    if (tid < kTsRows) sh.row[tid] = (tid < nv) ? k0 + idx * kTsRows + tid : -1;
+    __syncthreads();
+    int cnt = ts_gepp(a, nv, w, sh, s_prow, s_drow, s_l);
+    for (;;) {
+        const int size = ts_level_size(G, level);
+        if (size == 1) {
+            // root: the panel's pivot rows in order
+            if (tid < cnt) st_ag(&ws.root[1 + tid], sh.row[sh.perm[tid]]);
+            if (tid == 0) st_ag(&ws.root[0], cnt);
+            return;
+        }
+        const int nd = ts_level_off(G, level) + idx;
+        ts_publish(a0, cnt, nd, sh, ws);
+        const int parent = ts_level_off(G, level + 1) + (idx >> 1);
+        const int sib = idx ^ 1;
+        if (sib < size) {
+            if (tid == 0) {
+                const int t = __hip_atomic_fetch_add(&ws.tickets[parent], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (t != 0) st_ag(&ws.tickets[parent], 0);
+                sh.ticket = t;
+            }
+            __syncthreads();
+            if (sh.ticket == 0) return;  // the sibling will merge
+        }
+        // merge the children of `parent` (one or two nodes)
+        const int left = ts_level_off(G, level) + (idx & ~1);
+        const int nl = ld_ag(&ws.node_cnt[left]);
+        const int nr = (sib < size) ? ld_ag(&ws.node_cnt[left + 1]) : 0;
+        nv = nl + nr;
+        const C2<R>* vl = (const C2<R>*)ws.node_vals + (size_t)left * kNB * kNB;
+        const C2<R>* vr = vl + kNB * kNB;
+#pragma unroll
+        for (int q = 0; q < kTsRPT; ++q) {
+            const int r = g * kTsRPT + q;
+            C2<R> v{0, 0};
+            if (c < w) {
+                if (r < nl) v = ld_ag(&vl[r * kNB + c]);
+                else if (r < nv) v = ld_ag(&vr[(r - nl) * kNB + c]);
+            }
+            a0[q] = v;
+            a[q] = v;
+        }
+        if (tid < kTsRows)
+            sh.row[tid] = tid < nl ? ld_ag(&ws.node_rows[left * kNB + tid])
+                                   : (tid < nv ? ld_ag(&ws.node_rows[(left + 1) * kNB + tid - nl]) : -1);
+        __syncthreads();
+        cnt = ts_gepp(a, nv, w, sh, s_prow, s_drow, s_l);
+        level += 1;
+        idx >>= 1;
+    }
+}
+
+// Invert the 64 x 64 triangular matrix T (LDS, leading dimension kNB + 1) in
+// place: unit lower (upper = 0) or upper with its diagonal (upper = 1).
+// Recursive doubling: 8 x 8 diagonal blocks by substitution, then
+// X21 = -inv(T22) T21 inv(T11) (lower) or X12 = -inv(T11) T12 inv(T22)
+// (upper) at block sizes 8, 16, 32.  W is LDS scratch of the same shape.
+template <typename R>
+__device__ void tri_inverse(C2<R> (*T)[kNB + 1], C2<R> (*W)[kNB + 1], int bs, int upper) {
+    const int tid = threadIdx.x;  // 256 threads
+    // pad beyond bs with the identity
+    for (int e = tid; e < kNB * kNB; e += 256) {
+        const int r = e / kNB, c = e % kNB;
+        if (r >= bs || c >= bs) T[r][c] = (r == c) ? C2<R>{1, 0} : C2<R>{0, 0};
+    }
+    __syncthreads();
+    // 8 x 8 diagonal blocks: thread (block, column) substitutes one column
+    C2<R> x[8];
+    const int blk = (tid & 63) >> 3, cc8 = tid & 7, o8 = blk * 8;
+    if (tid < 64) {
+        if (!upper) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                C2<R> v = (r == cc8) ? C2<R>{1, 0} : C2<R>{0, 0};
+#pragma unroll
+                for (int s2 = 0; s2 < 8; ++s2)
+                    if (s2 < r) v = csub(v, cmul(T[o8 + r][o8 + s2], x[s2]));
+                x[r] = (r >= cc8) ? v : C2<R>{0, 0};
+            }
+        } else {
+#pragma unroll
+            for (int r = 7; r >= 0; --r) {
+                C2<R> v = (r == cc8) ? C2<R>{1, 0} : C2<R>{0, 0};
+#pragma unroll
+                for (int s2 = 0; s2 < 8; ++s2)
+                    if (s2 > r) v = csub(v, cmul(T[o8 + r][o8 + s2], x[s2]));
+                x[r] = (r <= cc8) ? cmul(v, cinv(T[o8 + r][o8 + r])) : C2<R>{0, 0};
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < 64) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) T[o8 + r][o8 + cc8] = x[r];
+    }
+    __syncthreads();
+    for (int h = 8; h < kNB; h *= 2) {
+        // pairs of h x h inverse blocks at offsets o, o + h (o = 0, 2h, ...)
+        const int pairs = kNB / (2 * h);
+        // step 1: W = T_off * inv(first)  (lower: T21 inv(T11); upper: T12 inv(T22))
+        for (int e = tid; e < pairs * h * h; e += 256) {
+            const int pr = e / (h * h), r = (e / h) % h, cc = e % h, o = pr * 2 * h;
+            C2<R> v{0, 0};
+            if (!upper) {
+                for (int k = 0; k < h; ++k) v = cadd(v, cmul(T[o + h + r][o + k], T[o + k][o + cc]));
+            } else {
+                for (int k = 0; k < h; ++k) v = cadd(v, cmul(T[o + r][o + h + k], T[o + h + k][o + h + cc]));
+            }
+            W[o + r][cc] = v;
+        }
+        __syncthreads();
+        // step 2: off-diagonal block = -inv(other) * W
+        for (int e = tid; e < pairs * h * h; e += 256) {
+            const int pr = e / (h * h), r = (e / h) % h, cc = e % h, o = pr * 2 * h;
+            C2<R> v{0, 0};
+            if (!upper) {
+                for (int k = 0; k < h; ++k) v = cadd(v, cmul(T[o + h + r][o + h + k], W[o + k][cc]));
+                T[o + h + r][o + cc] = C2<R>{-v.x, -v.y};
+            } else {
+                for (int k = 0; k < h; ++k) v = cadd(v, cmul(T[o + r][o + k], W[o + k][cc]));
+                T[o + r][o + h + cc] = C2<R>{-v.x, -v.y};
             }
         }
         __syncthreads();
-        if (c > j && c < w) {
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int row = base + r;
-                if (row < n && row > k0 + j) a[r] = csub(a[r], cmul(s_l[g * RPT + r], prc));
+    }
+}
+
+// From the root's pivot rows: LAPACK-style ipiv, the row permutation of the
+// panel columns, LU without pivoting of the 64 x 64 diagonal block, and the
+// inverses of its L and U (inv[2 kb], inv[2 kb + 1]).
+template <typename R>
+__global__ __launch_bounds__(256) void k_tslu_apply(C2<R>* __restrict__ A, int64_t lda, int n, int k0, int w,
+                                                    PanelWS ws, int* __restrict__ ipiv, C2<R>* __restrict__ inv) {
+    __shared__ C2<R> D[kNB][kNB + 1];
+    __shared__ C2<R> B[kNB][kNB + 1];
+    __shared__ int s_pos[kNB], s_row[kNB], s_cnt, s_root[kNB];
+    __shared__ int s_dst[kNB], s_src[kNB], s_nd;
+    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+    if (tid <= kNB) {
+        const int v = ld_ag(&ws.root[tid]);
+        if (tid == 0) s_cnt = v;
+        else s_root[tid - 1] = v;
+    }
+    __syncthreads();
+    const int cnt = s_cnt;
+    if (tid == 0) {
+        // Sequential interchanges (the LAPACK ipiv convention).  Rows waiting
+        // at unprocessed top-block positions are always original top-block
+        // rows, and rows from below only move when selected, so the state is
+        // two arrays over top-block offsets: s_row (row at position k0 + o)
+        // and s_pos (position of original row k0 + o, possibly below the block).
+        for (int o = 0; o < w; ++o) {
+            s_row[o] = k0 + o;
+            s_pos[o] = k0 + o;
+        }
+        for (int s2 = 0; s2 < cnt; ++s2) {
+            const int t = s_root[s2];
+            const bool top = t >= k0 && t < k0 + w;
+            const int pt = top ? s_pos[t - k0] : t;
+            ipiv[k0 + s2] = pt;
+            if (pt != k0 + s2) {
+                const int u = s_row[s2];
+                s_row[s2] = t;
+                s_pos[u - k0] = pt;
+                if (pt < k0 + w) s_row[pt - k0] = u;
+                if (top) s_pos[t - k0] = k0 + s2;
             }
         }
+        // top-block rows that end below the block
+        int nd = 0;
+        for (int o = 0; o < w; ++o)
+            if (s_pos[o] >= k0 + w) {
+                s_dst[nd] = s_pos[o];
+                s_src[nd] = k0 + o;
+                ++nd;
+            }
+        s_nd = nd;
     }
-#pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-        const int row = base + r;
-        if (row < n && c < w) A[(int64_t)row * lda + k0 + c] = a[r];
+    __syncthreads();
+    const int nd = s_nd;
+    // read every source row (original values) before anything is written:
+    // the selected rows into D, the rows leaving the top block into B
+    for (int r = g; r < kNB; r += 4)
+        D[r][c] = (r < cnt && c < w) ? A[(int64_t)s_root[r] * lda + k0 + c] : C2<R>{0, 0};
+    for (int t = g; t < nd; t += 4) B[t][c] = (c < w) ? A[(int64_t)s_src[t] * lda + k0 + c] : C2<R>{0, 0};
+    __syncthreads();
+    for (int t = g; t < nd; t += 4)
+        if (c < w) A[(int64_t)s_dst[t] * lda + k0 + c] = B[t][c];
+    __syncthreads();
+    // LU without pivoting of the diagonal block
+    for (int j = 0; j < w; ++j) {
+        const C2<R> pv = D[j][j];
+        const bool nz = pv.x != 0 || pv.y != 0;
+        if (tid == 0 && !nz) atomicMin(&ws.status[0], k0 + j);
+        const C2<R> ip = nz ? cinv(pv) : C2<R>{0, 0};
+        __syncthreads();
+        if (tid > j && tid < w) D[tid][j] = cmul(D[tid][j], ip);
+        __syncthreads();
+        for (int e = tid; e < kNB * kNB; e += 256) {
+            const int r = e / kNB, cc = e % kNB;
+            if (r > j && r < w && cc > j && cc < w) D[r][cc] = csub(D[r][cc], cmul(D[r][j], D[j][cc]));
+        }
+        __syncthreads();
     }
+    for (int r = g; r < w; r += 4)
+        if (c < w) A[(int64_t)(k0 + r) * lda + k0 + c] = D[r][c];
+    __syncthreads();
+    // inv(L): copy the strict lower part with a unit diagonal
+    for (int e = tid; e < kNB * kNB; e += 256) {
+        const int r = e / kNB, cc = e % kNB;
+        B[r][cc] = (r == cc) ? C2<R>{1, 0} : (r > cc ? D[r][cc] : C2<R>{0, 0});
+    }
+    __syncthreads();
+    tri_inverse<R>(B, D, w, 0);  // D is scratch now; reload U afterwards
+    C2<R>* out = inv + (size_t)(2 * (k0 / kNB)) * kNB * kNB;
+    for (int e = tid; e < kNB * kNB; e += 256) out[e] = B[e / kNB][e % kNB];
+    __syncthreads();
+    for (int e = tid; e < kNB * kNB; e += 256) {
+        const int r = e / kNB, cc = e % kNB;
+        B[r][cc] = (r <= cc && r < w && cc < w) ? A[(int64_t)(k0 + r) * lda + k0 + cc] : C2<R>{0, 0};
+    }
+    __syncthreads();
+    tri_inverse<R>(B, D, w, 1);
+    for (int e = tid; e < kNB * kNB; e += 256) out[kNB * kNB + e] = B[e / kNB][e % kNB];
 }
 
 // Row interchanges of pivots ipiv[k0 .. k0+w) applied, in order, to the
@@ -560,75 +762,6 @@ __global__ __launch_bounds__(256) void k_laswp(C2<R>* __restrict__ A, int64_t ld
             A[(int64_t)p * lda + col] = x;
         }
     }
-}
-
-// Inverses of the diagonal block's unit-lower L and upper U (bs <= 64),
-// stored row-major as inv[(2 kb) * 4096] (L) and inv[(2 kb + 1) * 4096] (U).
-// Four lanes own one column and split each substitution dot product by the
-// summation index mod 4 (combined with two xor shuffles), so no block-wide
-// barrier is needed inside a substitution.
-template <typename R>
-__global__ __launch_bounds__(256) void k_trtri(const C2<R>* __restrict__ A, int64_t lda, int k0, int bs,
-                                               C2<R>* __restrict__ inv) {
-    __shared__ C2<R> D[kNB][kNB + 1];
-    __shared__ C2<R> X[kNB][kNB + 1];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < kNB * kNB; e += 256) {
-        const int r = e / kNB, c = e % kNB;
-        D[r][c] = (r < bs && c < bs) ? A[(int64_t)(k0 + r) * lda + k0 + c] : C2<R>{0, 0};
-    }
-    __syncthreads();
-    const int c = tid >> 2;  // column
-    const int h = tid & 3;   // summation index mod 4
-    C2<R>* out = inv + (size_t)(2 * (k0 / kNB)) * kNB * kNB;
-    // inv(L): x_r = delta_rc - sum_{c <= s < r} L_rs x_s
-    for (int r = 0; r < kNB; ++r) {
-        double sx = 0, sy = 0;
-        if (r > c && r < bs) {
-            for (int s2 = c + h; s2 < r; s2 += 4) {
-                const C2<R> d = D[r][s2], x = X[s2][c];
-                sx += (double)d.x * x.x - (double)d.y * x.y;
-                sy += (double)d.x * x.y + (double)d.y * x.x;
-            }
-        }
-        sx += __shfl_xor(sx, 1, 64);
-        sy += __shfl_xor(sy, 1, 64);
-        sx += __shfl_xor(sx, 2, 64);
-        sy += __shfl_xor(sy, 2, 64);
-        if (h == 0) {
-            C2<R> v{0, 0};
-            if (r < bs && r >= c) v = (r == c) ? C2<R>{1, 0} : C2<R>{(R)-sx, (R)-sy};
-            X[r][c] = v;
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < kNB * kNB; e += 256) out[e] = X[e / kNB][e % kNB];
-    __syncthreads();
-    // inv(U): x_r = (delta_rc - sum_{r < s <= c} U_rs x_s) / U_rr
-    for (int r = kNB - 1; r >= 0; --r) {
-        double sx = 0, sy = 0;
-        if (r < c && c < bs) {
-            for (int s2 = r + 1 + h; s2 <= c; s2 += 4) {
-                const C2<R> d = D[r][s2], x = X[s2][c];
-                sx += (double)d.x * x.x - (double)d.y * x.y;
-                sy += (double)d.x * x.y + (double)d.y * x.x;
-            }
-        }
-        sx += __shfl_xor(sx, 1, 64);
-        sy += __shfl_xor(sy, 1, 64);
-        sx += __shfl_xor(sx, 2, 64);
-        sy += __shfl_xor(sy, 2, 64);
-        if (h == 0) {
-            C2<R> v{0, 0};
-            if (r <= c && c < bs) {
-                const C2<R> num = (r == c) ? C2<R>{1, 0} : C2<R>{(R)-sx, (R)-sy};
-                v = cmul(num, cinv(D[r][r]));
-            }
-            X[r][c] = v;
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < kNB * kNB; e += 256) out[kNB * kNB + e] = X[e / kNB][e % kNB];
 }
 
 // Rows k0 .. k0+bs of columns [cb, ce): X = inv(L_kb) X on 64-column strips,
@@ -685,7 +818,7 @@ constexpr int kGM = 128, kGN = 128, kGK = 16;
 
 // 512 threads: 8 waves as 2 (rows) x 4 (columns), each wave a 64 x 32 tile of
 // 4 x 2 MFMA blocks (lane layouts in Mfma<R>).
-template <typename R>
+template <typename R, int ASSIGN>
 __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A, int64_t lda,
                                                      const C2<R>* __restrict__ B, int64_t ldb,
                                                      C2<R>* __restrict__ C, int64_t ldc, int M, int N, int K) {
@@ -768,10 +901,14 @@ __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A
                 const int gm = m0 + wm + x * 16 + Mfma<R>::row(lane, r);
                 const int gn = n0 + wn + y * 16 + (lane & 15);
                 if (gm < M && gn < N) {
-                    C2<R> c = C[(int64_t)gm * ldc + gn];
-                    c.x -= accR[x][y][r];
-                    c.y -= accI[x][y][r];
-                    C[(int64_t)gm * ldc + gn] = c;
+                    if (ASSIGN) {
+                        C[(int64_t)gm * ldc + gn] = C2<R>{accR[x][y][r], accI[x][y][r]};
+                    } else {
+                        C2<R> c = C[(int64_t)gm * ldc + gn];
+                        c.x -= accR[x][y][r];
+                        c.y -= accI[x][y][r];
+                        C[(int64_t)gm * ldc + gn] = c;
+                    }
                 }
             }
 }
@@ -784,15 +921,6 @@ void prof_begin(void* prof, hipStream_t s, int cls);
 void prof_end(void* prof, hipStream_t s, int cls, double flops, double bytes);
 enum { CLS_PANEL = 2, CLS_GEMM = 3, CLS_TRSM = 4, CLS_SWAP = 5, CLS_SOLVE = 6 };
 
-int panel_capacity(int cu_count) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_lu_panel<double>), 256, 0) != hipSuccess)
-        return 0;
-    int per_cu32 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu32, reinterpret_cast<const void*>(&k_lu_panel<float>), 256, 0) != hipSuccess)
-        return 0;
-    return (per_cu < per_cu32 ? per_cu : per_cu32) * cu_count;
-}
 
 template <typename R>
 static void launch_laswp(hipStream_t s, C2<R>* A, int64_t lda, int k0, int w, const int* ipiv, int cb0, int ce0,
@@ -820,7 +948,7 @@ static void launch_update(hipStream_t s, C2<R>* A, int64_t lda, int r0, int c0, 
     if (M <= 0 || N <= 0 || K <= 0) return;
     prof_begin(prof, s, CLS_GEMM);
     dim3 grid((N + kGN - 1) / kGN, (M + kGM - 1) / kGM);
-    hipLaunchKernelGGL((k_gemm_sub<R>), grid, dim3(512), 0, s, A + (int64_t)r0 * lda + ka, lda,
+    hipLaunchKernelGGL((k_gemm_sub<R, 0>), grid, dim3(512), 0, s, A + (int64_t)r0 * lda + ka, lda,
                        A + (int64_t)ka * lda + c0, lda, A + (int64_t)r0 * lda + c0, lda, M, N, K);
     prof_end(prof, s, CLS_GEMM, 8.0 * (double)M * N * K, (2.0 * (double)M * N + (double)(M + N) * K) * sizeof(C2<R>));
 }
@@ -837,24 +965,21 @@ int lu_factor(hipStream_t s, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv
         for (int k0 = K0; k0 < Kend; k0 += kNB) {
             const int w = (Kend - k0) < kNB ? (Kend - k0) : kNB;
             const int m = n - k0;
-            const int G = (m + kPanelRows - 1) / kPanelRows;
+            const int G = (m + kTsRows - 1) / kTsRows;
             if (G > ws.max_blocks) return 2;
-            int k0v = k0, wv = w, nv = n;
-            int64_t ldv = lda;
-            C2<R>* Av = A;
-            int* ipv = ipiv;
-            PanelWS wsv = ws;
-            unsigned epoch = ws.epoch;
-            ws.epoch += kNB;
-            void* args[] = {&Av, &ldv, &nv, &k0v, &wv, &ipv, &wsv, &epoch};
             prof_begin(prof, s, CLS_PANEL);
-            if (hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_lu_panel<R>), dim3(G), dim3(256), args, 0,
-                                           s) != hipSuccess)
-                return 3;
+            hipLaunchKernelGGL((k_tslu<R>), dim3(G), dim3(kTsThreads), 0, s, A, lda, n, k0, w, ws);
+            hipLaunchKernelGGL((k_tslu_apply<R>), dim3(1), dim3(256), 0, s, A, lda, n, k0, w, ws, ipiv, inv);
+            if (m > w) {
+                // L21 = A21 inv(U11), in place
+                const C2<R>* Ui = inv + (size_t)(2 * (k0 / kNB) + 1) * kNB * kNB;
+                dim3 grid(1, (m - w + kGM - 1) / kGM);
+                hipLaunchKernelGGL((k_gemm_sub<R, 1>), grid, dim3(512), 0, s, A + (int64_t)(k0 + w) * lda + k0, lda,
+                                   Ui, (int64_t)kNB, A + (int64_t)(k0 + w) * lda + k0, lda, m - w, w, w);
+            }
             prof_end(prof, s, CLS_PANEL, 8.0 * (double)m * w * w / 2.0, 2.0 * (double)m * w * sizeof(C2<R>));
             // this panel's interchanges inside the outer block
             launch_laswp<R>(s, A, lda, k0, w, ipiv, K0, k0, k0 + w, Kend, prof);
-            hipLaunchKernelGGL((k_trtri<R>), dim3(1), dim3(256), 0, s, A, lda, k0, w, inv);
             if (k0 + w < Kend) {
                 launch_trsm<R>(s, A, lda, k0, w, k0 + w, Kend, inv, prof);
                 launch_update<R>(s, A, lda, k0 + w, k0 + w, k0, n - k0 - w, Kend - k0 - w, w, prof);
