@@ -14,7 +14,9 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <map>
+#include <thread>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -52,6 +54,7 @@ struct ProfRec {
 };
 
 struct Prof {
+    std::mutex mu;
     bool on = false;
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> pool;
@@ -87,38 +90,61 @@ struct Prof {
 }  // namespace
 
 namespace bem {
+// Event timing of one launch (or launch group) on stream s.  The begin event
+// is kept per stream so several lanes can time concurrently.
+struct ProfLane {
+    Prof* prof;
+    hipEvent_t cur = nullptr;
+};
 void prof_begin(void* prof, hipStream_t s, int cls) {
-    Prof* p = static_cast<Prof*>(prof);
-    if (!p || !p->on) return;
+    ProfLane* pl = static_cast<ProfLane*>(prof);
+    if (!pl || !pl->prof->on) return;
     (void)cls;
-    p->cur = p->get();
-    if (p->cur) hipEventRecord(p->cur, s);
+    Prof* p = pl->prof;
+    std::lock_guard<std::mutex> g(p->mu);
+    pl->cur = p->get();
+    if (pl->cur) hipEventRecord(pl->cur, s);
 }
 void prof_end(void* prof, hipStream_t s, int cls, double flops, double bytes) {
-    Prof* p = static_cast<Prof*>(prof);
-    if (!p || !p->on || !p->cur) return;
+    ProfLane* pl = static_cast<ProfLane*>(prof);
+    if (!pl || !pl->prof->on || !pl->cur) return;
+    Prof* p = pl->prof;
+    std::lock_guard<std::mutex> g(p->mu);
     hipEvent_t e = p->get();
     if (!e) return;
     hipEventRecord(e, s);
-    p->recs.push_back({cls, p->cur, e, flops, bytes});
-    p->cur = nullptr;
+    p->recs.push_back({cls, pl->cur, e, flops, bytes});
+    pl->cur = nullptr;
     if (p->recs.size() > 4096) p->drain();
 }
 }  // namespace bem
 
+// One execution lane: a HIP stream, the panel workspace of the LU on that
+// stream, and (for batched sweeps) a bump-allocated device arena that holds a
+// whole solve, so concurrent lanes never call hipMalloc/hipFree mid-sweep.
+struct Lane {
+    hipStream_t s = nullptr;
+    PanelWS ws{};
+    void* ws_mem = nullptr;
+    char* arena = nullptr;
+    size_t cap = 0, off = 0;
+    ProfLane pl{nullptr};
+};
+
 struct bem_ctx_s {
     int device = 0;
     int cu_count = 0;
-    hipStream_t stream = nullptr;
     uint64_t budget = 0, used = 0;
     std::string err;
     std::recursive_mutex mu;
+    std::mutex amu;  // allocator bookkeeping
     Prof prof;
-    PanelWS ws{};
-    void* ws_mem = nullptr;
+    std::vector<Lane*> lanes;  // lanes[0] serves the single-solve entry points
 
+    Lane* lane0() { return lanes[0]; }
     void* alloc(size_t bytes) {
         if (bytes == 0) bytes = 16;
+        std::lock_guard<std::mutex> g(amu);
         if (used + bytes > budget)
             throw BemError(BEM_ERR_ADMISSION, "request of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
                                                   std::to_string(used) + " of " + std::to_string(budget) + " in use)");
@@ -131,10 +157,13 @@ struct bem_ctx_s {
     }
     void release(void* p) {
         if (!p) return;
-        auto it = sizes.find(p);
-        if (it != sizes.end()) {
-            used -= it->second;
-            sizes.erase(it);
+        {
+            std::lock_guard<std::mutex> g(amu);
+            auto it = sizes.find(p);
+            if (it != sizes.end()) {
+                used -= it->second;
+                sizes.erase(it);
+            }
         }
         hipFree(p);
     }
@@ -146,11 +175,15 @@ struct bem_mesh_s {
     HostMesh h;
     DevMesh d;
     std::vector<void*> bufs;
-    int refs = 1;
+    std::atomic<int> refs{1};
 };
 
 struct bem_operator_s {
     bem_ctx ctx = nullptr;
+    Lane* lane = nullptr;
+    bool use_arena = false;
+    size_t arena_mark = 0;
+    std::vector<void*> heap;      // allocations outside the lane arena
     bem_mesh mesh = nullptr;
     bem_problem prob{};
     int n = 0, nt = 0;
@@ -192,6 +225,42 @@ struct bem_solution_s {
 };
 
 namespace {
+
+// A lane: its own stream and tournament-pivoting workspace (and no arena).
+Lane* make_lane(bem_ctx c) {
+    auto* L = new Lane();
+    L->pl.prof = &c->prof;
+    HIPCHECK(hipStreamCreateWithFlags(&L->s, hipStreamNonBlocking));
+    const int G = 640;  // 81920 rows: N = 40960 panels in a Mueller system
+    const int nodes = 2 * G + 16;
+    L->ws.max_blocks = G;
+    L->ws.max_nodes = nodes;
+    const size_t bytes = 64 + sizeof(int) * nodes * (2 + 64) + sizeof(z64) * (size_t)nodes * 64 * 64 +
+                         sizeof(int) * 80 + 8 * 256;
+    char* m = (char*)c->alloc(bytes);
+    L->ws_mem = m;
+    HIPCHECK(hipMemset(m, 0, bytes));
+    size_t off = 0;
+    auto take = [&](size_t b) { void* p = m + off; off += (b + 255) & ~(size_t)255; return p; };
+    L->ws.status = (int*)take(64);
+    L->ws.tickets = (int*)take(sizeof(int) * nodes);
+    L->ws.node_cnt = (int*)take(sizeof(int) * nodes);
+    L->ws.node_rows = (int*)take(sizeof(int) * nodes * 64);
+    L->ws.node_vals = take(sizeof(z64) * (size_t)nodes * 64 * 64);
+    L->ws.root = (int*)take(sizeof(int) * 80);
+    if (off > bytes) throw BemError(BEM_ERR_DEVICE, "workspace layout");
+    HIPCHECK(hipDeviceSynchronize());
+    return L;
+}
+
+void free_lane(bem_ctx c, Lane* L) {
+    if (!L) return;
+    if (L->s) hipStreamSynchronize(L->s);
+    c->release(L->ws_mem);
+    c->release(L->arena);
+    if (L->s) hipStreamDestroy(L->s);
+    delete L;
+}
 
 template <typename F>
 int guarded(bem_ctx ctx, F&& f) {
@@ -249,20 +318,36 @@ void direction(double beta, double alpha, double* d) {
     d[2] = std::sin(beta) * std::sin(alpha);
 }
 
+// Device memory for an operator: from its lane's arena when the lane has one
+// (batched sweeps), otherwise from the context heap.
+void* op_alloc(bem_operator op, size_t bytes) {
+    Lane* L = op->lane;
+    if (L && L->arena && op->use_arena) {
+        const size_t a = (L->off + 255) & ~(size_t)255;
+        if (a + bytes <= L->cap) {
+            L->off = a + bytes;
+            return L->arena + a;
+        }
+    }
+    void* p = op->ctx->alloc(bytes);
+    op->heap.push_back(p);
+    return p;
+}
+
 template <typename R>
 void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) {
     bem_ctx ctx = op->ctx;
-    hipStream_t s = ctx->stream;
+    hipStream_t s = op->lane->s;
     const int n = op->n;
-    launch_fill_int(s, ctx->ws.status, 1, INT_MAX);
-    launch_fill_int(s, ctx->ws.status + 1, 1, 0);
-    int rc = lu_factor<R>(s, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, ctx->ws, &ctx->prof);
+    launch_fill_int(s, op->lane->ws.status, 1, INT_MAX);
+    launch_fill_int(s, op->lane->ws.status + 1, 1, 0);
+    int rc = lu_factor<R>(s, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, op->lane->ws, &op->lane->pl);
     if (rc == 2) throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (81920 rows)");
     if (rc != 0) throw BemError(BEM_ERR_DEVICE, "LU panel launch failed");
     HIPCHECK(hipGetLastError());
     int status[2];
     std::vector<int> hp(n);
-    HIPCHECK(hipMemcpyAsync(status, ctx->ws.status, sizeof status, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(status, op->lane->ws.status, sizeof status, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(hp.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     if (status[0] != INT_MAX) throw BemError(BEM_ERR_NUMERIC, "singular:" + std::to_string(status[0]));
@@ -279,13 +364,12 @@ void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) 
 
 void promote(bem_operator op, int reason) {
     bem_ctx ctx = op->ctx;
-    hipStream_t s = ctx->stream;
+    hipStream_t s = op->lane->s;
     const int n = op->n;
-    if (op->LU) { ctx->release(op->LU); op->LU = nullptr; }
-    if (!op->LU64) op->LU64 = ctx->alloc(sizeof(z64) * (size_t)n * n);
-    if (!op->ipiv64) op->ipiv64 = (int*)ctx->alloc(sizeof(int) * n);
-    if (!op->perm64) op->perm64 = (int*)ctx->alloc(sizeof(int) * n);
-    if (!op->inv64) op->inv64 = ctx->alloc(sizeof(z64) * 2 * 4096 * (size_t)((n + 63) / 64));
+    if (!op->LU64) op->LU64 = op_alloc(op, sizeof(z64) * (size_t)n * n);
+    if (!op->ipiv64) op->ipiv64 = (int*)op_alloc(op, sizeof(int) * n);
+    if (!op->perm64) op->perm64 = (int*)op_alloc(op, sizeof(int) * n);
+    if (!op->inv64) op->inv64 = op_alloc(op, sizeof(z64) * 2 * 4096 * (size_t)((n + 63) / 64));
     if (op->single)
         launch_scale_copy<float, double>(s, (const z32*)op->A, n, n, op->rows, op->cols, (z64*)op->LU64, n);
     else
@@ -303,8 +387,8 @@ void promote(bem_operator op, int reason) {
 // max over the kRedBlocks partial results in op->red (stride `stride`, slot `slot`)
 double host_max(bem_operator op, int count, int stride, int slot, double* all = nullptr) {
     std::vector<double> h((size_t)count * stride);
-    HIPCHECK(hipMemcpyAsync(h.data(), op->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, op->ctx->stream));
-    HIPCHECK(hipStreamSynchronize(op->ctx->stream));
+    HIPCHECK(hipMemcpyAsync(h.data(), op->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, op->lane->s));
+    HIPCHECK(hipStreamSynchronize(op->lane->s));
     if (all) {
         for (int q = 0; q < stride; ++q) {
             double m = 0;
@@ -325,7 +409,7 @@ double host_max(bem_operator op, int count, int stride, int slot, double* all = 
 // the scaled right-hand side is non-finite (delta filled with NaN).
 bool correction(bem_operator op, const z64* rhs, z64* out, int accumulate, int* changed) {
     bem_ctx ctx = op->ctx;
-    hipStream_t s = ctx->stream;
+    hipStream_t s = op->lane->s;
     const int n = op->n;
     launch_max_abs_scaled(s, n, rhs, op->rows, op->red);
     const double mag = host_max(op, kRedBlocks, 1, 0);
@@ -337,14 +421,14 @@ bool correction(bem_operator op, const z64* rhs, z64* out, int accumulate, int* 
     if (op->precision == 1) {
         launch_prep_rhs<float>(s, n, rhs, op->rows, 1.0 / mag, (z32*)op->wv);
         lu_solve<float>(s, (const z32*)op->LU, n, n, op->perm, (const z32*)op->inv, (const z32*)op->wv,
-                        (z32*)op->wv2, (z32*)op->wv3, &ctx->prof);
+                        (z32*)op->wv2, (z32*)op->wv3, &op->lane->pl);
         launch_finish<float>(s, n, (const z32*)op->wv3, mag, op->cols, out, accumulate, changed);
     } else {
         const z64* LU = (const z64*)(op->LU64 ? op->LU64 : op->LU);
         const int* perm = op->LU64 ? op->perm64 : op->perm;
         const z64* inv = (const z64*)(op->LU64 ? op->inv64 : op->inv);
         launch_prep_rhs<double>(s, n, rhs, op->rows, 1.0 / mag, (z64*)op->wv);
-        lu_solve<double>(s, LU, n, n, perm, inv, (const z64*)op->wv, (z64*)op->wv2, (z64*)op->wv3, &ctx->prof);
+        lu_solve<double>(s, LU, n, n, perm, inv, (const z64*)op->wv, (z64*)op->wv2, (z64*)op->wv3, &op->lane->pl);
         launch_finish<double>(s, n, (const z64*)op->wv3, mag, op->cols, out, accumulate, changed);
     }
     return true;
@@ -352,12 +436,12 @@ bool correction(bem_operator op, const z64* rhs, z64* out, int accumulate, int* 
 
 void residual(bem_operator op, const z64* x, const z64* b, z64* r, double* den) {
     bem_ctx ctx = op->ctx;
-    prof_begin(&ctx->prof, ctx->stream, C_RESIDUAL);
+    prof_begin(&op->lane->pl, op->lane->s, C_RESIDUAL);
     if (op->single)
-        launch_matvec<float>(ctx->stream, (const z32*)op->A, op->n, op->n, op->n, x, r, b, den);
+        launch_matvec<float>(op->lane->s, (const z32*)op->A, op->n, op->n, op->n, x, r, b, den);
     else
-        launch_matvec<double>(ctx->stream, (const z64*)op->A, op->n, op->n, op->n, x, r, b, den);
-    prof_end(&ctx->prof, ctx->stream, C_RESIDUAL, 8.0 * (double)op->n * op->n,
+        launch_matvec<double>(op->lane->s, (const z64*)op->A, op->n, op->n, op->n, x, r, b, den);
+    prof_end(&op->lane->pl, op->lane->s, C_RESIDUAL, 8.0 * (double)op->n * op->n,
              (double)op->n * op->n * (op->single ? 8.0 : 16.0));
 }
 
@@ -368,7 +452,7 @@ struct BwdErr {
 BwdErr backward_errors(bem_operator op, const z64* x, const z64* b) {
     const int n = op->n;
     residual(op, x, b, op->vr, op->vden);
-    launch_bwd_stats(op->ctx->stream, n, op->vr, op->vden, x, b, op->red);
+    launch_bwd_stats(op->lane->s, n, op->vr, op->vden, x, b, op->red);
     double all[5];
     host_max(op, kRedBlocks, 5, 0, all);
     if (all[4] != 0) return {INFINITY, INFINITY};
@@ -384,7 +468,7 @@ BwdErr backward_errors(bem_operator op, const z64* x, const z64* b) {
 // solve_refined (fluid_system.jl:264-307); x in op->vx, b in op->vb
 void solve_refined(bem_operator op, bem_diagnostics& dg) {
     bem_ctx ctx = op->ctx;
-    hipStream_t s = ctx->stream;
+    hipStream_t s = op->lane->s;
     const int n = op->n;
     const double tol = op->prob.refine_tolerance;
     const int maxiter = op->prob.refine_maxiter;
@@ -444,21 +528,22 @@ void solve_refined(bem_operator op, bem_diagnostics& dg) {
 
 void op_free(bem_operator op) {
     bem_ctx c = op->ctx;
-    for (void* p : {op->A, op->B, op->LU, op->LU64, (void*)op->ipiv, (void*)op->perm, op->inv, (void*)op->ipiv64,
-                    (void*)op->perm64, op->inv64, (void*)op->rows, (void*)op->cols, (void*)op->vb, (void*)op->vx,
-                    (void*)op->vr, (void*)op->vt, (void*)op->vden, op->wv, op->wv2, op->wv3, (void*)op->red,
-                    (void*)op->flag})
-        c->release(p);
+    for (void* p : op->heap) c->release(p);
+    if (op->use_arena) op->lane->off = op->arena_mark;
     mesh_release(op->mesh);
     delete op;
 }
 
-void operator_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* prob, bem_operator* out) {
+void operator_create(bem_ctx ctx, Lane* lane, bem_mesh mesh, const bem_problem* prob, bem_operator* out,
+                     bool use_arena = false) {
     if (!mesh || !prob || !out) throw BemError(BEM_ERR_ARGUMENT, "null argument");
     validate_problem(*prob);
-    hipStream_t s = ctx->stream;
     auto* op = new bem_operator_s();
     op->ctx = ctx;
+    op->lane = lane;
+    op->use_arena = use_arena && lane->arena;
+    op->arena_mark = lane->off;
+    hipStream_t s = lane->s;
     op->mesh = mesh;
     mesh->refs++;
     op->prob = *prob;
@@ -471,8 +556,8 @@ void operator_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* prob, bem_op
     const size_t esz = op->single ? sizeof(z32) : sizeof(z64);
     try {
         const double t0 = now_ms();
-        op->A = ctx->alloc(esz * (size_t)n * n);
-        if (!fluid) op->B = ctx->alloc(esz * (size_t)n * n);
+        op->A = op_alloc(op, esz * (size_t)n * n);
+        if (!fluid) op->B = op_alloc(op, esz * (size_t)n * n);
         AsmParams ap;
         ap.k = prob->k;
         ap.g = fluid ? prob->density_contrast : 1.0;
@@ -487,40 +572,40 @@ void operator_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* prob, bem_op
             if (prob->boundary == BEM_PRESSURE_RELEASE) std::swap(o0, o1);
         }
         const double evals = (double)op->nt * op->nt * 7.0 * (fluid ? 2.0 : 1.0);
-        prof_begin(&ctx->prof, s, C_ASSEMBLE);
+        prof_begin(&op->lane->pl, s, C_ASSEMBLE);
         if (op->single) launch_assemble<float>(s, mesh->d, ap, (z32*)o0, (z32*)o1);
         else launch_assemble<double>(s, mesh->d, ap, (z64*)o0, (z64*)o1);
-        prof_end(&ctx->prof, s, C_ASSEMBLE, evals, (double)n * n * esz * (fluid ? 1.0 : 2.0));
-        prof_begin(&ctx->prof, s, C_NEAR);
+        prof_end(&op->lane->pl, s, C_ASSEMBLE, evals, (double)n * n * esz * (fluid ? 1.0 : 2.0));
+        prof_begin(&op->lane->pl, s, C_NEAR);
         if (op->single) launch_near_self<float>(s, mesh->d, ap, (z32*)o0, (z32*)o1);
         else launch_near_self<double>(s, mesh->d, ap, (z64*)o0, (z64*)o1);
-        prof_end(&ctx->prof, s, C_NEAR, (double)mesh->d.nnear * kNearPts * (fluid ? 2.0 : 1.0), 0.0);
+        prof_end(&op->lane->pl, s, C_NEAR, (double)mesh->d.nnear * kNearPts * (fluid ? 2.0 : 1.0), 0.0);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipStreamSynchronize(s));
         const double t1 = now_ms();
         op->assembly_ms = t1 - t0;
 
         // scratch
-        op->vb = (z64*)ctx->alloc(sizeof(z64) * n);
-        op->vx = (z64*)ctx->alloc(sizeof(z64) * n);
-        op->vr = (z64*)ctx->alloc(sizeof(z64) * n);
-        op->vt = (z64*)ctx->alloc(sizeof(z64) * 2 * (size_t)n);
-        op->vden = (double*)ctx->alloc(sizeof(double) * n);
-        op->wv = ctx->alloc(sizeof(z64) * n);
-        op->wv2 = ctx->alloc(sizeof(z64) * n);
-        op->wv3 = ctx->alloc(sizeof(z64) * n);
-        op->red = (double*)ctx->alloc(sizeof(double) * 5 * kRedBlocks);
-        op->flag = (int*)ctx->alloc(sizeof(int) * 4);
-        op->ipiv = (int*)ctx->alloc(sizeof(int) * n);
-        op->perm = (int*)ctx->alloc(sizeof(int) * n);
-        op->inv = ctx->alloc(esz * 2 * 4096 * (size_t)((n + 63) / 64));
-        op->LU = ctx->alloc(esz * (size_t)n * n);
+        op->vb = (z64*)op_alloc(op, sizeof(z64) * n);
+        op->vx = (z64*)op_alloc(op, sizeof(z64) * n);
+        op->vr = (z64*)op_alloc(op, sizeof(z64) * n);
+        op->vt = (z64*)op_alloc(op, sizeof(z64) * 2 * (size_t)n);
+        op->vden = (double*)op_alloc(op, sizeof(double) * n);
+        op->wv = op_alloc(op, sizeof(z64) * n);
+        op->wv2 = op_alloc(op, sizeof(z64) * n);
+        op->wv3 = op_alloc(op, sizeof(z64) * n);
+        op->red = (double*)op_alloc(op, sizeof(double) * 5 * kRedBlocks);
+        op->flag = (int*)op_alloc(op, sizeof(int) * 4);
+        op->ipiv = (int*)op_alloc(op, sizeof(int) * n);
+        op->perm = (int*)op_alloc(op, sizeof(int) * n);
+        op->inv = op_alloc(op, esz * 2 * 4096 * (size_t)((n + 63) / 64));
+        op->LU = op_alloc(op, esz * (size_t)n * n);
 
         // equilibration + finiteness + matrix norm
         double* rowsum = (double*)op->vden;
         if (prob->equilibrate) {
-            op->rows = (double*)ctx->alloc(sizeof(double) * n);
-            op->cols = (double*)ctx->alloc(sizeof(double) * n);
+            op->rows = (double*)op_alloc(op, sizeof(double) * n);
+            op->cols = (double*)op_alloc(op, sizeof(double) * n);
         }
         double* rows = op->rows ? op->rows : (double*)op->vt;
         if (op->single) launch_row_stats<float>(s, (const z32*)op->A, n, n, rows, rowsum);
@@ -564,20 +649,95 @@ void operator_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* prob, bem_op
     *out = op;
 }
 
+// One right-hand side: plane wave with device direction ddir (3 doubles);
+// scattered traces into P, Q (nt each) and the report into dg.
+void solve_rhs(bem_operator op, const double* ddir, z64* P, z64* Q, bem_diagnostics& dg) {
+    hipStream_t s = op->lane->s;
+    const int nt = op->nt, n = op->n;
+    const bem_problem& pr = op->prob;
+    const bool fluid = pr.boundary == BEM_FLUID;
+    const double t0 = now_ms();
+    std::memset(&dg, 0, sizeof dg);
+    dg.unknown_count = n;
+    dg.quadrature_nodes = nt;
+    z64* pinc = op->vt;        // nt
+    z64* qinc = op->vt + nt;   // nt
+    launch_incident(s, op->mesh->d, pr.k, 1, ddir, pinc, qinc);
+    // right-hand side b (n) in op->vb
+    if (fluid && pr.formulation == BEM_MULLER) {
+        HIPCHECK(hipMemcpyAsync(op->vb, pinc, sizeof(z64) * nt, hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(op->vb + nt, qinc, sizeof(z64) * nt, hipMemcpyDeviceToDevice, s));
+    } else if (fluid) {
+        // [0; -A[N+1:2N, :] * [p_inc; q_inc]]   (full.jl:448-453)
+        launch_fill_double(s, (double*)op->vb, 2 * (int64_t)nt, 0.0);
+        if (op->single)
+            launch_matvec<float>(s, (const z32*)op->A + (size_t)nt * n, n, nt, n, op->vt, op->vr, nullptr, nullptr);
+        else
+            launch_matvec<double>(s, (const z64*)op->A + (size_t)nt * n, n, nt, n, op->vt, op->vr, nullptr, nullptr);
+        launch_copy_z(s, nt, op->vr, op->vb + nt, -1.0, nullptr);
+    } else {
+        const z64* src = pr.boundary == BEM_RIGID ? qinc : pinc;
+        if (op->single) launch_matvec<float>(s, (const z32*)op->B, n, n, n, src, op->vb, nullptr, nullptr);
+        else launch_matvec<double>(s, (const z64*)op->B, n, n, n, src, op->vb, nullptr, nullptr);
+    }
+    if (pr.precision == BEM_MIXED) {
+        solve_refined(op, dg);
+    } else {
+        if (!correction(op, op->vb, op->vx, 0, nullptr))
+            throw BemError(BEM_ERR_ARGUMENT, "right-hand side must be finite");
+        dg.converged = 1;
+    }
+    dg.factor_precision = op->precision;
+    dg.refinement_reason = op->reason;
+    dg.single_reciprocal_condition = op->rcond;
+    // residual report on the original equations: r = b - A x
+    residual(op, op->vx, op->vb, op->vr, nullptr);
+    launch_norm_stats(s, n, op->vr, op->vb, op->rows, op->red);
+    {
+        std::vector<double> h(4 * kRedBlocks);
+        HIPCHECK(hipMemcpyAsync(h.data(), op->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        double ss[4] = {0, 0, 0, 0};
+        for (int b = 0; b < kRedBlocks; ++b)
+            for (int q = 0; q < 4; ++q) ss[q] += h[4 * b + q];
+        dg.absolute_residual = std::sqrt(ss[0]);
+        const double bn = std::sqrt(ss[1]);
+        dg.relative_residual = bn == 0 ? (dg.absolute_residual == 0 ? 0.0 : INFINITY) : dg.absolute_residual / bn;
+        dg.scaled_absolute_residual = std::sqrt(ss[2]);
+        const double sbn = std::sqrt(ss[3]);
+        dg.scaled_relative_residual =
+            sbn == 0 ? (dg.scaled_absolute_residual == 0 ? 0.0 : INFINITY) : dg.scaled_absolute_residual / sbn;
+    }
+    // traces (full.jl:160-161,472-477)
+    if (fluid) {
+        const bool mul = pr.formulation == BEM_MULLER;
+        launch_copy_z(s, nt, op->vx, P, 1.0, mul ? pinc : nullptr);
+        launch_copy_z(s, nt, op->vx + nt, Q, 1.0, mul ? qinc : nullptr);
+    } else if (pr.boundary == BEM_RIGID) {
+        launch_copy_z(s, nt, op->vx, P, 1.0, nullptr);
+        launch_copy_z(s, nt, qinc, Q, -1.0, nullptr);
+    } else {
+        launch_copy_z(s, nt, pinc, P, -1.0, nullptr);
+        launch_copy_z(s, nt, op->vx, Q, 1.0, nullptr);
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    dg.assembly_ms = op->assembly_ms;
+    dg.factor_ms = op->factor_ms;
+    dg.solve_ms = now_ms() - t0;
+}
+
 void operator_solve(bem_operator op, int nrhs, const double* angles, bem_solution* out) {
     if (nrhs <= 0 || !angles || !out) throw BemError(BEM_ERR_ARGUMENT, "need at least one incidence");
     for (int r = 0; r < 2 * nrhs; ++r)
         if (!std::isfinite(angles[r])) throw BemError(BEM_ERR_ARGUMENT, "incidence angles must be finite");
     bem_ctx ctx = op->ctx;
-    hipStream_t s = ctx->stream;
-    const int nt = op->nt, n = op->n;
-    const bem_problem& pr = op->prob;
-    const bool fluid = pr.boundary == BEM_FLUID;
+    hipStream_t s = op->lane->s;
+    const int nt = op->nt;
     auto* sol = new bem_solution_s();
     sol->ctx = ctx;
     sol->mesh = op->mesh;
     op->mesh->refs++;
-    sol->k = pr.k;
+    sol->k = op->prob.k;
     sol->nrhs = nrhs;
     sol->nt = nt;
     sol->angles.assign(angles, angles + 2 * nrhs);
@@ -590,79 +750,8 @@ void operator_solve(bem_operator op, int nrhs, const double* angles, bem_solutio
         for (int r = 0; r < nrhs; ++r) direction(angles[2 * r], angles[2 * r + 1], &dirs[3 * r]);
         ddir = (double*)ctx->alloc(sizeof(double) * 3 * nrhs);
         HIPCHECK(hipMemcpyAsync(ddir, dirs.data(), sizeof(double) * 3 * nrhs, hipMemcpyHostToDevice, s));
-        z64* pinc = op->vt;        // nt
-        z64* qinc = op->vt + nt;   // nt
-        for (int r = 0; r < nrhs; ++r) {
-            const double t0 = now_ms();
-            bem_diagnostics& dg = sol->diag[r];
-            std::memset(&dg, 0, sizeof dg);
-            dg.unknown_count = n;
-            dg.quadrature_nodes = nt;
-            launch_incident(s, op->mesh->d, pr.k, 1, ddir + 3 * r, pinc, qinc);
-            // right-hand side b (n) in op->vb
-            if (fluid && pr.formulation == BEM_MULLER) {
-                HIPCHECK(hipMemcpyAsync(op->vb, pinc, sizeof(z64) * nt, hipMemcpyDeviceToDevice, s));
-                HIPCHECK(hipMemcpyAsync(op->vb + nt, qinc, sizeof(z64) * nt, hipMemcpyDeviceToDevice, s));
-            } else if (fluid) {
-                // [0; -A[N+1:2N, :] * [p_inc; q_inc]]   (full.jl:448-453)
-                HIPCHECK(hipMemsetAsync(op->vb, 0, sizeof(z64) * nt, s));
-                if (op->single)
-                    launch_matvec<float>(s, (const z32*)op->A + (size_t)nt * n, n, nt, n, op->vt, op->vr, nullptr, nullptr);
-                else
-                    launch_matvec<double>(s, (const z64*)op->A + (size_t)nt * n, n, nt, n, op->vt, op->vr, nullptr, nullptr);
-                launch_copy_z(s, nt, op->vr, op->vb + nt, -1.0, nullptr);
-            } else {
-                const z64* src = pr.boundary == BEM_RIGID ? qinc : pinc;
-                if (op->single) launch_matvec<float>(s, (const z32*)op->B, n, n, n, src, op->vb, nullptr, nullptr);
-                else launch_matvec<double>(s, (const z64*)op->B, n, n, n, src, op->vb, nullptr, nullptr);
-            }
-            if (pr.precision == BEM_MIXED) {
-                solve_refined(op, dg);
-            } else {
-                if (!correction(op, op->vb, op->vx, 0, nullptr))
-                    throw BemError(BEM_ERR_ARGUMENT, "right-hand side must be finite");
-                dg.converged = 1;
-            }
-            dg.factor_precision = op->precision;
-            dg.refinement_reason = op->reason;
-            dg.single_reciprocal_condition = op->rcond;
-            // residual report on the original equations: r = b - A x
-            residual(op, op->vx, op->vb, op->vr, nullptr);
-            launch_norm_stats(s, n, op->vr, op->vb, op->rows, op->red);
-            {
-                std::vector<double> h(4 * kRedBlocks);
-                HIPCHECK(hipMemcpyAsync(h.data(), op->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
-                HIPCHECK(hipStreamSynchronize(s));
-                double ss[4] = {0, 0, 0, 0};
-                for (int b = 0; b < kRedBlocks; ++b)
-                    for (int q = 0; q < 4; ++q) ss[q] += h[4 * b + q];
-                dg.absolute_residual = std::sqrt(ss[0]);
-                const double bn = std::sqrt(ss[1]);
-                dg.relative_residual = bn == 0 ? (dg.absolute_residual == 0 ? 0.0 : INFINITY) : dg.absolute_residual / bn;
-                dg.scaled_absolute_residual = std::sqrt(ss[2]);
-                const double sbn = std::sqrt(ss[3]);
-                dg.scaled_relative_residual =
-                    sbn == 0 ? (dg.scaled_absolute_residual == 0 ? 0.0 : INFINITY) : dg.scaled_absolute_residual / sbn;
-            }
-            // traces (full.jl:160-161,472-477)
-            z64* P = sol->p + (size_t)r * nt;
-            z64* Q = sol->q + (size_t)r * nt;
-            if (fluid) {
-                const bool mul = pr.formulation == BEM_MULLER;
-                launch_copy_z(s, nt, op->vx, P, 1.0, mul ? pinc : nullptr);
-                launch_copy_z(s, nt, op->vx + nt, Q, 1.0, mul ? qinc : nullptr);
-            } else if (pr.boundary == BEM_RIGID) {
-                launch_copy_z(s, nt, op->vx, P, 1.0, nullptr);
-                launch_copy_z(s, nt, qinc, Q, -1.0, nullptr);
-            } else {
-                launch_copy_z(s, nt, pinc, P, -1.0, nullptr);
-                launch_copy_z(s, nt, op->vx, Q, 1.0, nullptr);
-            }
-            HIPCHECK(hipStreamSynchronize(s));
-            dg.assembly_ms = op->assembly_ms;
-            dg.factor_ms = op->factor_ms;
-            dg.solve_ms = now_ms() - t0;
-        }
+        for (int r = 0; r < nrhs; ++r)
+            solve_rhs(op, ddir + 3 * r, sol->p + (size_t)r * nt, sol->q + (size_t)r * nt, sol->diag[r]);
         sol->hp.resize((size_t)nt * nrhs);
         sol->hq.resize((size_t)nt * nrhs);
         HIPCHECK(hipMemcpyAsync(sol->hp.data(), sol->p, sizeof(z64) * nt * nrhs, hipMemcpyDeviceToHost, s));
@@ -678,6 +767,50 @@ void operator_solve(bem_operator op, int nrhs, const double* angles, bem_solutio
         throw;
     }
     *out = sol;
+}
+
+// Device bytes one sweep item holds in a lane arena.
+size_t sweep_need(const bem_mesh m, const bem_problem& p) {
+    const size_t nt = (size_t)m->d.nt;
+    const size_t n = p.boundary == BEM_FLUID ? 2 * nt : nt;
+    const size_t esz = p.precision == BEM_MIXED ? sizeof(z32) : sizeof(z64);
+    size_t b = esz * n * n * (p.boundary == BEM_FLUID ? 2 : 3);    // A, LU (+ B)
+    b += esz * 2 * 4096 * ((n + 63) / 64);                           // block inverses
+    b += sizeof(z64) * n * 8 + sizeof(double) * n * 3 + sizeof(int) * n * 2;
+    b += sizeof(z64) * (2 * nt + kFarBlocks + 4) + 64 * 1024;
+    return b;
+}
+
+// One sweep item on lane L: assemble, factor, solve, backscatter amplitude.
+void sweep_item(bem_ctx ctx, Lane* L, bem_mesh mesh, const bem_problem& prob, const double* ang, double* amp,
+                bem_diagnostics* dgo) {
+    L->off = 0;
+    bem_operator op = nullptr;
+    operator_create(ctx, L, mesh, &prob, &op, true);
+    try {
+        const int nt = op->nt;
+        z64* P = (z64*)op_alloc(op, sizeof(z64) * nt);
+        z64* Q = (z64*)op_alloc(op, sizeof(z64) * nt);
+        z64* part = (z64*)op_alloc(op, sizeof(z64) * kFarBlocks);
+        z64* res = (z64*)op_alloc(op, sizeof(z64) * 2);
+        double* dd = (double*)op_alloc(op, sizeof(double) * 8);
+        double hd[6];
+        direction(ang[0], ang[1], hd);
+        for (int c = 0; c < 3; ++c) hd[3 + c] = -hd[c];
+        HIPCHECK(hipMemcpyAsync(dd, hd, sizeof hd, hipMemcpyHostToDevice, L->s));
+        bem_diagnostics dg;
+        solve_rhs(op, dd, P, Q, dg);
+        prof_begin(&L->pl, L->s, C_FAR);
+        launch_far_field(L->s, mesh->d, prob.k, 1, 1, dd + 3, P, Q, part, res);
+        prof_end(&L->pl, L->s, C_FAR, (double)nt * 7, (double)nt * 7 * 8);
+        HIPCHECK(hipMemcpyAsync(amp, res, sizeof(z64), hipMemcpyDeviceToHost, L->s));
+        HIPCHECK(hipStreamSynchronize(L->s));
+        if (dgo) *dgo = dg;
+    } catch (...) {
+        op_free(op);
+        throw;
+    }
+    op_free(op);
 }
 
 }  // namespace
@@ -721,31 +854,13 @@ int bem_ctx_create(int32_t device, uint64_t budget, bem_ctx* out) {
         size_t fr = 0, tot = 0;
         HIPCHECK(hipMemGetInfo(&fr, &tot));
         c->budget = budget ? budget : (uint64_t)(0.9 * (double)fr);
-        HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         if (upload_rules() != 0) throw BemError(BEM_ERR_DEVICE, "constant table upload failed");
-        const int G = 640;  // 81920 rows: N = 40960 panels in a Mueller system
-        const int nodes = 2 * G + 16;
-        c->ws.max_blocks = G;
-        c->ws.max_nodes = nodes;
-        size_t bytes = 64 + sizeof(int) * nodes * (2 + 64) + sizeof(z64) * (size_t)nodes * 64 * 64 + sizeof(int) * 80 +
-                       8 * 256;
-        char* m = (char*)c->alloc(bytes);
-        c->ws_mem = m;
-        HIPCHECK(hipMemset(m, 0, bytes));
-        size_t off = 0;
-        auto take = [&](size_t b) { void* p = m + off; off += (b + 255) & ~(size_t)255; return p; };
-        c->ws.status = (int*)take(64);
-        c->ws.tickets = (int*)take(sizeof(int) * nodes);
-        c->ws.node_cnt = (int*)take(sizeof(int) * nodes);
-        c->ws.node_rows = (int*)take(sizeof(int) * nodes * 64);
-        c->ws.node_vals = take(sizeof(z64) * (size_t)nodes * 64 * 64);
-        c->ws.root = (int*)take(sizeof(int) * 80);
-        if (off > bytes) throw BemError(BEM_ERR_DEVICE, "workspace layout");
+        c->lanes.push_back(make_lane(c));
         HIPCHECK(hipDeviceSynchronize());
     });
     if (rc != BEM_OK) {
         std::fprintf(stderr, "bem_ctx_create: %s\n", c->err.c_str());
-        if (c->stream) hipStreamDestroy(c->stream);
+        for (Lane* L : c->lanes) free_lane(c, L);
         delete c;
         return rc;
     }
@@ -755,11 +870,10 @@ int bem_ctx_create(int32_t device, uint64_t budget, bem_ctx* out) {
 
 int bem_ctx_destroy(bem_ctx ctx) {
     if (!ctx) return BEM_ERR_ARGUMENT;
-    hipStreamSynchronize(ctx->stream);
+    for (Lane* L : ctx->lanes) hipStreamSynchronize(L->s);
     ctx->prof.drain();
     for (auto e : ctx->prof.pool) hipEventDestroy(e);
-    ctx->release(ctx->ws_mem);
-    hipStreamDestroy(ctx->stream);
+    for (Lane* L : ctx->lanes) free_lane(ctx, L);
     delete ctx;
     return BEM_OK;
 }
@@ -906,7 +1020,7 @@ int bem_mesh_create(bem_ctx ctx, int64_t nv, const double* nodes, int64_t nt, co
         build_geometry(h);
         build_near_list(h);
         try {
-            hipStream_t s = ctx->stream;
+            hipStream_t s = ctx->lane0()->s;
             DevMesh& d = m->d;
             d.nt = (int)nt;
             auto up = [&](const void* src, size_t bytes) {
@@ -1002,7 +1116,7 @@ int bem_mesh_near_count(bem_mesh m, int64_t* count) {
 int bem_operator_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, bem_operator* out) {
     if (!ctx) return BEM_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    return guarded(ctx, [&] { operator_create(ctx, mesh, problem, out); });
+    return guarded(ctx, [&] { operator_create(ctx, ctx->lane0(), mesh, problem, out); });
 }
 
 int bem_operator_destroy(bem_operator op) {
@@ -1051,7 +1165,7 @@ int bem_solve(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, double bet
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     return guarded(ctx, [&] {
         bem_operator op = nullptr;
-        operator_create(ctx, mesh, problem, &op);
+        operator_create(ctx, ctx->lane0(), mesh, problem, &op);
         const double ang[2] = {beta, alpha};
         try {
             operator_solve(op, 1, ang, out);
@@ -1114,7 +1228,7 @@ int bem_far_field(bem_solution sol, int32_t r, int64_t ndir, const double* dirs,
             for (int64_t q = 0; q < ndir; ++q)
                 for (int c = 0; c < 3; ++c) hd[3 * q + c] = -d[c];
         }
-        hipStream_t s = ctx->stream;
+        hipStream_t s = ctx->lane0()->s;
         double* dd = (double*)ctx->alloc(sizeof(double) * hd.size());
         z64* part = nullptr;
         z64* res = nullptr;
@@ -1122,10 +1236,10 @@ int bem_far_field(bem_solution sol, int32_t r, int64_t ndir, const double* dirs,
             part = (z64*)ctx->alloc(sizeof(z64) * ndir * kFarBlocks);
             res = (z64*)ctx->alloc(sizeof(z64) * ndir);
             HIPCHECK(hipMemcpyAsync(dd, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice, s));
-            prof_begin(&ctx->prof, s, C_FAR);
+            prof_begin(&ctx->lane0()->pl, s, C_FAR);
             launch_far_field(s, sol->mesh->d, sol->k, 1, (int)ndir, dd, sol->p + (size_t)r * sol->nt,
                              sol->q + (size_t)r * sol->nt, part, res);
-            prof_end(&ctx->prof, s, C_FAR, (double)sol->nt * 7 * ndir, (double)sol->nt * ndir * 7 * 8);
+            prof_end(&ctx->lane0()->pl, s, C_FAR, (double)sol->nt * 7 * ndir, (double)sol->nt * ndir * 7 * 8);
             HIPCHECK(hipMemcpyAsync(out, res, sizeof(z64) * ndir, hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));
         } catch (...) {
@@ -1141,31 +1255,78 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh* meshes, const bem_problem*
     if (!ctx || n <= 0 || !meshes || !problems || !angles || !amplitudes) return BEM_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     return guarded(ctx, [&] {
-        std::vector<int> order(n);
-        for (int i = 0; i < n; ++i) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-            const double ca = (double)meshes[a]->h.nt * (problems[a].boundary == BEM_FLUID ? 2 : 1);
-            const double cb = (double)meshes[b]->h.nt * (problems[b].boundary == BEM_FLUID ? 2 : 1);
-            return ca > cb;
-        });
-        for (int i : order) {
-            const double t0 = now_ms();
-            bem_operator op = nullptr;
-            operator_create(ctx, meshes[i], &problems[i], &op);
-            bem_solution sol = nullptr;
-            try {
-                operator_solve(op, 1, angles + 2 * i, &sol);
-            } catch (...) {
-                op_free(op);
-                throw;
-            }
-            op_free(op);
-            int rc = bem_far_field(sol, 0, 1, nullptr, amplitudes + 2 * i);
-            if (diagnostics) diagnostics[i] = sol->diag[0];
-            bem_solution_destroy(sol);
-            if (rc != BEM_OK) throw BemError(rc, ctx->err);
-            if (latency_ms) latency_ms[i] = now_ms() - t0;
+        for (int i = 0; i < n; ++i) {
+            if (!meshes[i]) throw BemError(BEM_ERR_ARGUMENT, "null mesh");
+            validate_problem(problems[i]);
+            if (!std::isfinite(angles[2 * i]) || !std::isfinite(angles[2 * i + 1]))
+                throw BemError(BEM_ERR_ARGUMENT, "incidence angles must be finite");
         }
+        // largest first, so lanes finish together
+        std::vector<int> order(n);
+        std::vector<size_t> need(n);
+        for (int i = 0; i < n; ++i) {
+            order[i] = i;
+            need[i] = sweep_need(meshes[i], problems[i]);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return need[a] > need[b]; });
+        const size_t max_need = need[order[0]];
+        // lanes: as many as the HBM budget holds arenas for (at most 8)
+        uint64_t arena_now = 0;
+        for (Lane* L : ctx->lanes) arena_now += L->cap;
+        const uint64_t avail = ctx->budget - (ctx->used - arena_now);
+        int S = (int)std::min<uint64_t>(8, (uint64_t)(0.9 * (double)avail) / max_need);
+        S = std::max(1, std::min(S, n));
+        while ((int)ctx->lanes.size() < S) ctx->lanes.push_back(make_lane(ctx));
+        // release arenas that are too small or unused, then (re)allocate
+        for (int l = 0; l < (int)ctx->lanes.size(); ++l) {
+            Lane* L = ctx->lanes[l];
+            if (L->arena && (l >= S || L->cap < max_need)) {
+                ctx->release(L->arena);
+                L->arena = nullptr;
+                L->cap = 0;
+            }
+        }
+        for (int l = 0; l < S; ++l) {
+            Lane* L = ctx->lanes[l];
+            if (!L->arena) {
+                L->arena = (char*)ctx->alloc(max_need);
+                L->cap = max_need;
+            }
+            L->off = 0;
+        }
+        std::atomic<int> next{0};
+        std::atomic<bool> failed{false};
+        std::mutex emu;
+        int err_code = BEM_OK;
+        std::string err_msg;
+        auto worker = [&](int li) {
+            (void)hipSetDevice(ctx->device);
+            Lane* L = ctx->lanes[li];
+            for (;;) {
+                const int k = next.fetch_add(1);
+                if (k >= n || failed.load()) break;
+                const int i = order[k];
+                const double t0 = now_ms();
+                try {
+                    sweep_item(ctx, L, meshes[i], problems[i], angles + 2 * i, amplitudes + 2 * i,
+                               diagnostics ? diagnostics + i : nullptr);
+                } catch (const BemError& e) {
+                    std::lock_guard<std::mutex> lk(emu);
+                    if (!failed.exchange(true)) { err_code = e.code; err_msg = e.what(); }
+                    break;
+                } catch (const std::exception& e) {
+                    std::lock_guard<std::mutex> lk(emu);
+                    if (!failed.exchange(true)) { err_code = BEM_ERR_DEVICE; err_msg = e.what(); }
+                    break;
+                }
+                if (latency_ms) latency_ms[i] = now_ms() - t0;
+            }
+        };
+        std::vector<std::thread> th;
+        for (int l = 1; l < S; ++l) th.emplace_back(worker, l);
+        worker(0);
+        for (auto& t : th) t.join();
+        if (failed.load()) throw BemError(err_code, err_msg);
     });
 }
 
@@ -1175,7 +1336,7 @@ int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t single, doub
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     return guarded(ctx, [&] {
         const int n = (int)n64;
-        hipStream_t s = ctx->stream;
+        hipStream_t s = ctx->lane0()->s;
         const size_t esz = single ? sizeof(z32) : sizeof(z64);
         std::vector<char> host((size_t)n * n * esz);
         for (int i = 0; i < n; ++i)
@@ -1186,6 +1347,7 @@ int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t single, doub
             }
         bem_operator_s op;
         op.ctx = ctx;
+        op.lane = ctx->lane0();
         op.n = n;
         op.single = single != 0;
         std::vector<void*> bufs;

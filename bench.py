@@ -53,7 +53,7 @@ def sweep_plan(bh, ctx, n_points, max_level):
     return plan, meshes
 
 
-def run_sweep(bh, ctx, plan, meshes, precision, chunk=16):
+def run_sweep(bh, ctx, plan, meshes, precision, chunk=64):
     """One pass over the sweep, largest meshes first, in synchronous chunks of
     `chunk` solves (so progress can be reported); returns per-solve results in
     plan order and the wall time of the pass."""
