@@ -362,6 +362,88 @@ void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) 
     HIPCHECK(hipStreamSynchronize(s));
 }
 
+// Reciprocal condition number in the infinity norm of the equilibrated
+// single-precision matrix, as LAPACK gecon('I') (fluid_system.jl:218-222):
+// rcond = 1 / (||Ahat||_inf * est), est ~ ||inv(Ahat)||_inf = ||inv(Ahat)^H||_1
+// from the 1-norm estimator of Hager (1984) in Higham's form (ACM TOMS 14,
+// 1988): products with B = inv(Ahat)^H and B^H = inv(Ahat) are LU solves.
+double estimate_rcond(bem_operator op, double anorm) {
+    const int n = op->n;
+    hipStream_t s = op->lane->s;
+    if (anorm == 0) return 0.0;
+    typedef std::complex<double> cd;
+    std::vector<cd> v(n);
+    std::vector<z32> buf(n);
+    auto apply = [&](bool adjoint_of_B) {
+        for (int i = 0; i < n; ++i) buf[i] = z32{(float)v[i].real(), (float)v[i].imag()};
+        HIPCHECK(hipMemcpyAsync(op->wv, buf.data(), sizeof(z32) * n, hipMemcpyHostToDevice, s));
+        if (adjoint_of_B)
+            lu_solve<float>(s, (const z32*)op->LU, n, n, op->perm, (const z32*)op->inv, (const z32*)op->wv,
+                            (z32*)op->wv2, (z32*)op->wv3, &op->lane->pl);
+        else
+            lu_solve_h<float>(s, (const z32*)op->LU, n, n, op->perm, (const z32*)op->inv, (const z32*)op->wv,
+                              (z32*)op->wv2, (z32*)op->wv3, &op->lane->pl);
+        HIPCHECK(hipMemcpyAsync(buf.data(), op->wv3, sizeof(z32) * n, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        for (int i = 0; i < n; ++i) v[i] = cd(buf[i].x, buf[i].y);
+    };
+    auto norm1 = [&]() {
+        double t = 0;
+        for (const cd& z : v) t += std::abs(z);
+        return t;
+    };
+    auto sign = [&]() {
+        for (cd& z : v) {
+            const double a = std::abs(z);
+            z = a > 0 ? z / a : cd(1, 0);
+        }
+    };
+    auto argmax = [&]() {
+        int j = 0;
+        double m = -1;
+        for (int i = 0; i < n; ++i)
+            if (std::abs(v[i]) > m) { m = std::abs(v[i]); j = i; }
+        return j;
+    };
+    for (int i = 0; i < n; ++i) v[i] = cd(1.0 / n, 0);
+    apply(false);
+    if (n == 1) return 1.0 / (anorm * std::abs(v[0]));
+    double est = norm1();
+    if (!std::isfinite(est)) return NAN;
+    sign();
+    apply(true);
+    int j = argmax();
+    for (int iter = 2; iter <= 5; ++iter) {
+        std::fill(v.begin(), v.end(), cd(0, 0));
+        v[j] = cd(1, 0);
+        apply(false);
+        const double old = est;
+        est = norm1();
+        if (!std::isfinite(est)) return NAN;
+        if (est <= old) {
+            est = old;
+            break;
+        }
+        sign();
+        apply(true);
+        const int jlast = j;
+        const double zlast = std::abs(v[jlast]);
+        j = argmax();
+        if (zlast == std::abs(v[j])) break;
+    }
+    // alternating-sign test vector (Higham's final safeguard)
+    double alt = 1.0;
+    for (int i = 0; i < n; ++i) {
+        v[i] = cd(alt * (1.0 + (double)i / (n - 1)), 0);
+        alt = -alt;
+    }
+    apply(false);
+    const double temp = 2.0 * norm1() / (3.0 * n);
+    if (temp > est) est = temp;
+    if (!(est > 0)) return NAN;
+    return (1.0 / est) / anorm;
+}
+
 void promote(bem_operator op, int reason) {
     bem_ctx ctx = op->ctx;
     hipStream_t s = op->lane->s;
@@ -631,6 +713,16 @@ void operator_create(bem_ctx ctx, Lane* lane, bem_mesh mesh, const bem_problem* 
         }
         if (op->single) launch_scale_copy<float, float>(s, (const z32*)op->A, n, n, op->rows, op->cols, (z32*)op->LU, n);
         else launch_scale_copy<double, double>(s, (const z64*)op->A, n, n, op->rows, op->cols, (z64*)op->LU, n);
+        double anorm_low = 0;
+        const bool guard = op->single && prob->condition_guard >= 0;
+        if (guard) {
+            // ||Ahat||_inf of the equilibrated single-precision matrix
+            launch_row_stats<float>(s, (const z32*)op->LU, n, n, (double*)op->vt, rowsum);
+            std::vector<double> hs(n);
+            HIPCHECK(hipMemcpyAsync(hs.data(), rowsum, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            for (double v : hs) anorm_low = std::max(anorm_low, v);
+        }
         try {
             if (op->single) factor_into<float>(op, op->LU, op->ipiv, op->perm, op->inv);
             else factor_into<double>(op, op->LU, op->ipiv, op->perm, op->inv);
@@ -638,6 +730,10 @@ void operator_create(bem_ctx ctx, Lane* lane, bem_mesh mesh, const bem_problem* 
             if (e.code != BEM_ERR_NUMERIC) throw;
             if (!op->single) throw BemError(BEM_ERR_NUMERIC, "matrix is singular (zero pivot)");
             promote(op, BEM_REASON_SINGLE_SINGULAR);
+        }
+        if (guard && op->precision == 1) {
+            op->rcond = estimate_rcond(op, anorm_low);
+            if (!std::isfinite(op->rcond) || op->rcond <= prob->condition_guard) promote(op, BEM_REASON_CONDITION);
         }
         if (op->single && op->precision == 1 && prob->force_promotion) promote(op, BEM_REASON_CONDITION);
         HIPCHECK(hipStreamSynchronize(s));

@@ -115,6 +115,11 @@ template <typename R>
 void lu_solve(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm,
               const C2<R>* inv, const C2<R>* b, C2<R>* work, C2<R>* x, void* prof);
 
+// x = inv(A)^H b with the same factors (conjugate-transpose solves).
+template <typename R>
+void lu_solve_h(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm, const C2<R>* inv,
+                const C2<R>* b, C2<R>* work, C2<R>* x, void* prof);
+
 // Refinement helpers (FP64 vectors).
 template <typename R>
 void launch_prep_rhs(hipStream_t s, int n, const z64* b, const double* rows, double inv_mag, C2<R>* out);

@@ -1062,6 +1062,82 @@ __global__ __launch_bounds__(256) void k_trsv_block(const C2<R>* __restrict__ T,
     }
 }
 
+// Block kb of a solve with the conjugate transpose of a triangular factor.
+// UPPER = 1: forward substitution with U^H (lower), z = inv(U_kb)^H y[block],
+//            then y[r] -= sum_s conj(U[r0+s][r]) z[s] for rows after the block.
+// UPPER = 0: backward substitution with L^H (unit upper), z = inv(L_kb)^H y[block],
+//            then y[r] -= sum_s conj(L[r0+s][r]) z[s] for rows before the block.
+// One thread per row r: the reads U[r0+s][r] are coalesced across threads.
+template <typename R, int UPPER>
+__global__ __launch_bounds__(256) void k_trsvH_block(const C2<R>* __restrict__ T, int64_t lda, int n, int kb,
+                                                     const C2<R>* __restrict__ inv, C2<R>* __restrict__ y,
+                                                     C2<R>* __restrict__ out) {
+    __shared__ C2<R> z[kNB];
+    const int r0 = kb * kNB;
+    const int bs = (n - r0) < kNB ? (n - r0) : kNB;
+    const int tid = threadIdx.x;
+    const C2<R>* Di = inv + (size_t)(2 * kb + (UPPER ? 1 : 0)) * kNB * kNB;
+    if (tid < kNB) {
+        C2<R> v{0, 0};
+        if (tid < bs)
+            for (int c = 0; c < bs; ++c) {
+                const C2<R> d = Di[c * kNB + tid];  // (inv^H)[tid][c] = conj(inv[c][tid])
+                const C2<R> yc = y[r0 + c];
+                v.x += d.x * yc.x + d.y * yc.y;
+                v.y += d.x * yc.y - d.y * yc.x;
+            }
+        z[tid] = v;
+        if (blockIdx.x == 0 && tid < bs) out[r0 + tid] = v;
+    }
+    __syncthreads();
+    const int rbeg = UPPER ? r0 + bs : 0;
+    const int rend = UPPER ? n : r0;
+    for (int r = rbeg + blockIdx.x * 256 + tid; r < rend; r += gridDim.x * 256) {
+        double sx = 0, sy = 0;
+        for (int s2 = 0; s2 < bs; ++s2) {
+            const C2<R> t = T[(int64_t)(r0 + s2) * lda + r];
+            const C2<R> zc = z[s2];
+            sx += (double)t.x * zc.x + (double)t.y * zc.y;
+            sy += (double)t.x * zc.y - (double)t.y * zc.x;
+        }
+        C2<R> v = y[r];
+        v.x -= (R)sx;
+        v.y -= (R)sy;
+        y[r] = v;
+    }
+}
+
+template <typename R>
+__global__ void k_scatter(int n, const int* perm, const C2<R>* w, C2<R>* y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[perm[i]] = w[i];
+}
+
+// x = inv(A)^H b for P A = L U:  A^H = U^H L^H P, so x = P^T inv(L^H) inv(U^H) b.
+template <typename R>
+void lu_solve_h(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm, const C2<R>* inv,
+                const C2<R>* b, C2<R>* work, C2<R>* x, void* prof) {
+    const int nblk = (n + kNB - 1) / kNB;
+    prof_begin(prof, s, CLS_SOLVE);
+    (void)hipMemcpyAsync(work, b, sizeof(C2<R>) * n, hipMemcpyDeviceToDevice, s);
+    for (int kb = 0; kb < nblk; ++kb) {
+        const int rows = n - (kb + 1) * kNB;
+        int grid = rows > 0 ? (rows + 255) / 256 : 1;
+        if (grid > 1024) grid = 1024;
+        hipLaunchKernelGGL((k_trsvH_block<R, 1>), dim3(grid), dim3(256), 0, s, LU, lda, n, kb, inv, work, x);
+    }
+    (void)hipMemcpyAsync(work, x, sizeof(C2<R>) * n, hipMemcpyDeviceToDevice, s);
+    for (int kb = nblk - 1; kb >= 0; --kb) {
+        const int rows = kb * kNB;
+        int grid = rows > 0 ? (rows + 255) / 256 : 1;
+        if (grid > 1024) grid = 1024;
+        hipLaunchKernelGGL((k_trsvH_block<R, 0>), dim3(grid), dim3(256), 0, s, LU, lda, n, kb, inv, work, x);
+    }
+    hipLaunchKernelGGL((k_scatter<R>), dim3((n + 255) / 256), dim3(256), 0, s, n, perm, x, work);
+    (void)hipMemcpyAsync(x, work, sizeof(C2<R>) * n, hipMemcpyDeviceToDevice, s);
+    prof_end(prof, s, CLS_SOLVE, 8.0 * (double)n * n, (double)n * n * sizeof(C2<R>));
+}
+
 template <typename R>
 void lu_solve(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm, const C2<R>* inv,
               const C2<R>* b, C2<R>* work, C2<R>* x, void* prof) {
@@ -1104,5 +1180,7 @@ template int lu_factor<double>(hipStream_t, z64*, int64_t, int, int*, z64*, Pane
 template int lu_factor<float>(hipStream_t, z32*, int64_t, int, int*, z32*, PanelWS&, void*);
 template void lu_solve<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, const z64*, z64*, z64*, void*);
 template void lu_solve<float>(hipStream_t, const z32*, int64_t, int, const int*, const z32*, const z32*, z32*, z32*, void*);
+template void lu_solve_h<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, const z64*, z64*, z64*, void*);
+template void lu_solve_h<float>(hipStream_t, const z32*, int64_t, int, const int*, const z32*, const z32*, z32*, z32*, void*);
 
 }  // namespace bem

@@ -189,6 +189,37 @@ def test_forced_promotion(ctx):
     assert abs(bh.scattering_amplitude(s) - bh.scattering_amplitude(sd)) / abs(bh.scattering_amplitude(sd)) < 1e-5
 
 
+def test_condition_guard_forces_promotion(ctx):
+    """refinement=(condition_guard=1.0,) promotes with reason :condition
+    (test/basic/bem/formulations.jl:424-425)."""
+    say("[gpu] condition guard 1.0")
+    gm, _ = _sphere(ctx, 2)
+    s = bh.bem(gm, bh.FluidFilled(*FLUID_CONFIG1), K_CONFIG1, incidence_angle=0.0, precision="mixed",
+               refinement={"condition_guard": 1.0})
+    d = s.diagnostics
+    assert d["factor_precision"] == "double" and d["refinement_reason"] == "condition"
+
+
+def test_reciprocal_condition_estimate_matches_lapack(ctx):
+    """The single-precision rcond estimate against LAPACK cgecon('I') on the
+    same equilibrated FP32 matrix (fluid_system.jl:218-222)."""
+    say("[gpu] rcond estimate vs cgecon")
+    from scipy.linalg import lapack
+    gm, _ = _sphere(ctx, 2)
+    bc = bh.FluidFilled(*FLUID_CONFIG1)
+    op = bh.Operator(gm, bc, K_CONFIG1, precision="mixed")
+    A = op.matrix()
+    rows, cols = O.equilibrate(A)
+    low = (A / rows[:, None] / cols[None, :]).astype(np.complex64)
+    anorm = float(np.abs(low).sum(axis=1).max())
+    lu, piv, info = lapack.cgetrf(low)
+    rc_ref, _ = lapack.cgecon(lu, anorm, norm="I")
+    s = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0, precision="mixed")
+    rc = s.diagnostics["single_reciprocal_condition"]
+    assert np.isfinite(rc) and 0 < rc <= 1
+    assert 0.1 < rc / rc_ref < 10
+
+
 def test_iteration_limit_promotes(ctx):
     say("[gpu] maxiter=0 forces iteration_limit promotion")
     gm, _ = _sphere(ctx, 2)
