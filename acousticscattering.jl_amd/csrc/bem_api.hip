@@ -1376,7 +1376,7 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh* meshes, const bem_problem*
         // release arenas that are too small or unused, then (re)allocate
         for (int l = 0; l < (int)ctx->lanes.size(); ++l) {
             Lane* L = ctx->lanes[l];
-            if (L->arena && (l >= S || L->cap < max_need)) {
+            if (L->arena && (l >= S || L->cap < max_need || L->cap > max_need + max_need / 4)) {
                 ctx->release(L->arena);
                 L->arena = nullptr;
                 L->cap = 0;
