@@ -231,12 +231,11 @@ Lane* make_lane(bem_ctx c) {
     auto* L = new Lane();
     L->pl.prof = &c->prof;
     HIPCHECK(hipStreamCreateWithFlags(&L->s, hipStreamNonBlocking));
-    const int G = 640;  // 81920 rows: N = 40960 panels in a Mueller system
+    const int G = 2048;  // 262144 rows (128 per leaf)
     const int nodes = 2 * G + 16;
     L->ws.max_blocks = G;
     L->ws.max_nodes = nodes;
-    const size_t bytes = 64 + sizeof(int) * nodes * (2 + 64) + sizeof(z64) * (size_t)nodes * 64 * 64 +
-                         sizeof(int) * 80 + 8 * 256;
+    const size_t bytes = 64 + sizeof(int) * nodes * (2 + 64) + sizeof(int) * 80 + 8 * 256;
     char* m = (char*)c->alloc(bytes);
     L->ws_mem = m;
     HIPCHECK(hipMemset(m, 0, bytes));
@@ -246,7 +245,6 @@ Lane* make_lane(bem_ctx c) {
     L->ws.tickets = (int*)take(sizeof(int) * nodes);
     L->ws.node_cnt = (int*)take(sizeof(int) * nodes);
     L->ws.node_rows = (int*)take(sizeof(int) * nodes * 64);
-    L->ws.node_vals = take(sizeof(z64) * (size_t)nodes * 64 * 64);
     L->ws.root = (int*)take(sizeof(int) * 80);
     if (off > bytes) throw BemError(BEM_ERR_DEVICE, "workspace layout");
     HIPCHECK(hipDeviceSynchronize());
@@ -342,7 +340,7 @@ void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) 
     launch_fill_int(s, op->lane->ws.status, 1, INT_MAX);
     launch_fill_int(s, op->lane->ws.status + 1, 1, 0);
     int rc = lu_factor<R>(s, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, op->lane->ws, &op->lane->pl);
-    if (rc == 2) throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (81920 rows)");
+    if (rc == 2) throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (262144 rows)");
     if (rc != 0) throw BemError(BEM_ERR_DEVICE, "LU panel launch failed");
     HIPCHECK(hipGetLastError());
     int status[2];
@@ -350,6 +348,7 @@ void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) 
     HIPCHECK(hipMemcpyAsync(status, op->lane->ws.status, sizeof status, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(hp.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
+    if (status[1] != 0) throw BemError(BEM_ERR_DEVICE, "LU panel pivot selection was inconsistent");
     if (status[0] != INT_MAX) throw BemError(BEM_ERR_NUMERIC, "singular:" + std::to_string(status[0]));
     std::vector<int> pm(n);
     for (int i = 0; i < n; ++i) pm[i] = i;

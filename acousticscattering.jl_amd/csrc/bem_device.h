@@ -103,7 +103,6 @@ struct PanelWS {
     int* tickets;      // [max_nodes] merge tickets, zero between launches
     int* node_cnt;     // [max_nodes] rows nominated by each tree node
     int* node_rows;    // [max_nodes][64] their global row indices
-    void* node_vals;   // [max_nodes][64][64] their original values (complex, sized for FP64)
     int* root;         // [0] count, [1..65) the panel's pivot rows in order
     int max_blocks;    // largest leaf count (128 rows per leaf)
     int max_nodes;

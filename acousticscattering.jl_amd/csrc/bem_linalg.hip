@@ -304,8 +304,8 @@ void launch_norm_stats(hipStream_t s, int n, const z64* r, const z64* b, const d
 }
 
 // ---------------------------------------------------------------------------
-// LU panel: cooperative kernel, 128 rows per workgroup, each row held by a
-// lane pair (columns 0-31 and 32-63 of the panel) in registers.
+// LU: 64-column panels (kNB), tournament-pivoted panel factorization, MFMA
+// trailing updates.
 // ---------------------------------------------------------------------------
 constexpr int kNB = 64;
 
@@ -338,44 +338,221 @@ __device__ __forceinline__ void wait_mem() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// argmax of (value, index): larger value wins, ties go to the smaller index.
-__device__ inline void wave_argmax(double& v, int& idx, int& blk) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        double ov = __shfl_xor(v, off, 64);
-        int oi = __shfl_xor(idx, off, 64);
-        int ob = __shfl_xor(blk, off, 64);
-        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; blk = ob; }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Panel factorization by tournament pivoting (CALU / TSLU: Grigori, Demmel and
 // Xiang, SIAM J. Matrix Anal. Appl. 32 (2011)).  The m x w panel is split into
-// 128-row leaves.  Each leaf runs partial pivoting (max |re|+|im|, first
-// maximum wins) on its rows and nominates w rows; nominations are merged
-// pairwise up a binary tree, each merge again running partial pivoting on the
-// 2w candidate rows (original values).  The root's selection order is the
-// panel's pivot order.  No grid-wide barrier: a merge is done by the second
-// workgroup to arrive at a tree node (atomic ticket), the first one exits.
+// 128-row leaves.  Each leaf runs partial pivoting on its rows (max |re|+|im|,
+// ties to the lowest current position, as LAPACK's i?amax) and nominates w
+// rows; nominations are merged pairwise up a binary tree, each merge again
+// running partial pivoting on the 2w candidate rows, re-read from A by row
+// index (A is not written while the tree runs).  The root's selection order
+// is the panel's pivot order.  No grid-wide barrier: a merge is done by the
+// second workgroup to arrive at a tree node (atomic ticket), the first exits.
+//
+// One 128-row elimination (256 threads, 4 waves of 32 rows): lanes r and
+// r + 32 of a wave hold columns 0-31 and 32-63 of row r's active window.
+// Rows never move: a selected row retires and its LAPACK position is swapped
+// in bookkeeping only.  After each step the window shifts left by one column,
+// so the pivot column is always register 0 and every register index is
+// static.  One block barrier per step: each wave stages its best row in LDS
+// (double-buffered by step parity) before it, every wave picks the global
+// winner after it.
 // ---------------------------------------------------------------------------
-constexpr int kTsRows = 128;    // rows per leaf / merge
-constexpr int kTsRPT = 16;      // rows per lane
-constexpr int kTsThreads = 512; // 8 waves
+constexpr int kTsRows = 128;     // rows per leaf / merge
+constexpr int kTsThreads = 256;  // 4 waves x 32 rows, two lanes per row
+constexpr int kTsPS = 34;        // LDS stride of a half-row: 16-byte aligned, off the other half's banks
 
-template <int N, typename R>
-__device__ __forceinline__ C2<R> pickn(const C2<R> (&a)[N], int r) {
-    C2<R> v{0, 0};
-#pragma unroll
-    for (int q = 0; q < N; ++q)
-        if (q == r) v = a[q];
-    return v;
+template <typename R>
+struct TsCand {
+    R v;
+    int pos;
+};
+
+template <typename R>
+struct TsLds {
+    TsCand<R> cand[2][4];
+    __attribute__((aligned(16))) C2<R> prow[2][4][2 * kTsPS];
+    int sel[kNB];
+    int ticket;
+};
+
+// Pivot key of a live row: |re| + |im|; NaN sorts below every number (it is
+// picked only when nothing comparable is left); retired rows never win.
+template <typename R>
+__device__ __forceinline__ R ts_key(C2<R> z, bool live) {
+    R v = cabs1(z);
+    if (v != v) v = R(-0.5);
+    return live ? v : R(-1);
 }
-template <int N, typename R>
-__device__ __forceinline__ void placen(C2<R> (&a)[N], int r, C2<R> v) {
+
+// Cross-lane moves without LDS: DPP within 16-lane rows, readlane across rows,
+// and the gfx950 half-wave swap (lane l <-> lane l ^ 32).
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_r(float x) {
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_r(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_i<CTRL>((int)(b & 0xffffffffll)), hi = dpp_i<CTRL>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float rdl(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double rdl(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ int rdl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+// value of lane l ^ 32
+__device__ __forceinline__ int xhalf_i(int v, bool upper) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return upper ? (int)r[0] : (int)r[1];
+}
+__device__ __forceinline__ float xhalf(float v, bool upper) { return __int_as_float(xhalf_i(__float_as_int(v), upper)); }
+__device__ __forceinline__ double xhalf(double v, bool upper) {
+    const long long b = __double_as_longlong(v);
+    const int lo = xhalf_i((int)(b & 0xffffffffll), upper), hi = xhalf_i((int)(b >> 32), upper);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Wave-uniform max / min: butterflies inside each 16-lane row (quad swaps,
+// half-row and row mirrors), then the four row results by readlane.
+template <typename T>
+__device__ __forceinline__ T ts_max(T a, T b) { return a > b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T wave_max_u(T x) {
+    x = ts_max(x, dpp_r<0xB1>(x));
+    x = ts_max(x, dpp_r<0x4E>(x));
+    x = ts_max(x, dpp_r<0x141>(x));
+    x = ts_max(x, dpp_r<0x140>(x));
+    return ts_max(ts_max(rdl(x, 0), rdl(x, 16)), ts_max(rdl(x, 32), rdl(x, 48)));
+}
+__device__ __forceinline__ int wave_min_u(int x) {
+    x = min(x, dpp_i<0xB1>(x));
+    x = min(x, dpp_i<0x4E>(x));
+    x = min(x, dpp_i<0x141>(x));
+    x = min(x, dpp_i<0x140>(x));
+    return min(min(rdl(x, 0), rdl(x, 16)), min(rdl(x, 32), rdl(x, 48)));
+}
+
+// 1 / z for z != 0, scaled by |re| + |im| against overflow; only steers the
+// pivot choice (the factors are recomputed from original values).
+__device__ __forceinline__ C2<float> ts_inv(C2<float> z) {
+    const float is = __builtin_amdgcn_rcpf(fabsf(z.x) + fabsf(z.y));
+    const float x = z.x * is, y = z.y * is;
+    const float id = __builtin_amdgcn_rcpf(fmaf(x, x, y * y)) * is;
+    return C2<float>{x * id, -y * id};
+}
+__device__ __forceinline__ C2<double> ts_inv(C2<double> z) {
+    const double is = 1.0 / (fabs(z.x) + fabs(z.y));
+    const double x = z.x * is, y = z.y * is;
+    const double id = is / fma(x, x, y * y);
+    return C2<double>{x * id, -y * id};
+}
+// a - l p with four fused multiply-adds
+template <typename R>
+__device__ __forceinline__ C2<R> cfms(C2<R> a, C2<R> l, C2<R> p) {
+    return C2<R>{fma(-l.x, p.x, fma(l.y, p.y, a.x)), fma(-l.x, p.y, fma(-l.y, p.x, a.y))};
+}
+
+// Elimination steps [j0, j1) with a live window of W columns per lane (32
+// while the upper half-row still holds live columns, 16 once fewer than 16
+// remain).  The winner of step j writes its global row index to sh.sel[j].
+template <typename R, int W>
+__device__ __forceinline__ void ts_steps(C2<R> (&a)[32], int j0, int j1, bool& done, int& mypos, int myrow,
+                                         TsLds<R>& sh) {
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const bool upper = lane >= 32;
+#pragma unroll 1
+    for (int j = j0; j < j1; ++j) {
+        const int b = j & 1;
+        // this wave's best live row: max key, then lowest position
+        const R v = upper ? R(-1) : ts_key(a[0], !done);
+        const R wv = wave_max_u(v);
+        const int wp = wave_min_u((!upper && v == wv) ? mypos : INT_MAX);
+        if (lane == 0) {
+            sh.cand[b][g].v = wv;
+            sh.cand[b][g].pos = wp;
+        }
+        if (!done && mypos == wp) {  // both lanes of the wave's best row
 #pragma unroll
-    for (int q = 0; q < N; ++q)
-        if (q == r) a[q] = v;
+            for (int c = 0; c < W; ++c) sh.prow[b][g][(upper ? kTsPS : 0) + c] = a[c];
+        }
+        __syncthreads();
+        // the block's winner
+        TsCand<R> cq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cq[q] = sh.cand[b][q];
+        R gv = cq[0].v;
+        int gp = cq[0].pos, gw = 0;
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+            const bool t = cq[q].v > gv || (cq[q].v == gv && cq[q].pos < gp);
+            gv = t ? cq[q].v : gv;
+            gp = t ? cq[q].pos : gp;
+            gw = t ? q : gw;
+        }
+        const C2<R> pv = sh.prow[b][gw][0];
+        const C2<R>* pr = sh.prow[b][gw] + (upper ? kTsPS : 0);
+        const bool nz = pv.x != 0 || pv.y != 0;
+        const C2<R> ip = nz ? ts_inv(pv) : C2<R>{0, 0};
+        const C2<R> l0 = cmul(a[0], ip);
+        if (g == gw && !done && mypos == gp) {
+            done = true;
+            if (!upper) sh.sel[j] = myrow;
+        } else if (mypos == j) {
+            mypos = gp;  // the row at position j takes the winner's old position
+        }
+        // eliminate column j and shift the window left by one column; retired
+        // rows compute values nobody reads
+        if (W == 32) {
+            // one half-wave swap: the lower lane sends the multiplier, the
+            // upper lane its window column 32 (which moves to the lower lane)
+            const C2<R> s = upper ? a[0] : l0;
+            const C2<R> o{xhalf(s.x, upper), xhalf(s.y, upper)};
+            const C2<R> l = upper ? o : l0;
+            const C2<R> c32 = sh.prow[b][gw][kTsPS];
+#pragma unroll
+            for (int c = 0; c < W - 1; ++c) a[c] = cfms(a[c + 1], l, pr[c + 1]);
+            a[W - 1] = upper ? C2<R>{0, 0} : cfms(o, l, c32);
+        } else {
+#pragma unroll
+            for (int c = 0; c < W - 1; ++c) a[c] = cfms(a[c + 1], l0, pr[c + 1]);
+        }
+    }
+}
+
+// Partial pivoting over the block's nv rows (w columns); on return
+// sh.sel[0 .. cnt) holds the selected global rows in pivot order.
+template <typename R>
+__device__ int ts_eliminate(C2<R> (&a)[32], int nv, int w, int myrow, TsLds<R>& sh) {
+    const int slot = (threadIdx.x >> 6) * 32 + (threadIdx.x & 31);
+    const int cnt = nv < w ? nv : w;
+    bool done = myrow < 0;
+    int mypos = done ? 1024 + slot : slot;
+    ts_steps<R, 32>(a, 0, cnt < 48 ? cnt : 48, done, mypos, myrow, sh);
+    ts_steps<R, 16>(a, 48, cnt, done, mypos, myrow, sh);
+    __syncthreads();
+    return cnt;
+}
+
+// Window columns [32 part, 32 part + 32) of `row` (zero beyond w or for row < 0).
+template <typename R>
+__device__ __forceinline__ void ts_load_row(C2<R> (&a)[32], const C2<R>* __restrict__ A, int64_t lda, int row,
+                                            int col0, int w, int part) {
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        const int cc = part * 32 + c;
+        a[c] = (row >= 0 && cc < w) ? A[(int64_t)row * lda + col0 + cc] : C2<R>{0, 0};
+    }
 }
 
 __device__ inline int ts_level_size(int G, int L) {
@@ -392,143 +569,34 @@ __device__ inline int ts_level_off(int G, int L) {
     return off;
 }
 
-struct TsShared {
-    double v[8];
-    int i[8];
-    int p;
-    int ticket;
-    int perm[kTsRows];
-    int row[kTsRows];
-};
-
-// Wave-uniform broadcast of lane `src`'s value (src uniform).
-__device__ __forceinline__ float bcast(float v, int src) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
-}
-__device__ __forceinline__ double bcast(double v, int src) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-// Partial pivoting on the register block a (row position r = 16 g + q of lane
-// column c), `nv` valid rows, w columns.  On return perm[0 .. min(w, nv)) holds
-// the selected positions in pivot order.  Two block barriers per column: one
-// after the per-wave candidates, one after the pivot and diagonal rows are
-// staged; multipliers travel within each wave by readlane from lane j.
-template <typename R>
-__device__ int ts_gepp(C2<R> (&a)[kTsRPT], int nv, int w, TsShared& sh, C2<R>* s_prow, C2<R>* s_drow,
-                       C2<R>* s_l) {
-    (void)s_l;
-    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
-    if (tid < kTsRows) sh.perm[tid] = tid;
-    __syncthreads();
-    const int cnt = nv < w ? nv : w;
-#pragma unroll 1
-    for (int j = 0; j < cnt; ++j) {
-        if (c == j) {
-            double bv = -1.0;
-            int bi = INT_MAX;
-#pragma unroll
-            for (int q = 0; q < kTsRPT; ++q) {
-                const int r = g * kTsRPT + q;
-                if (r < nv && r >= j) {
-                    const double v = (double)cabs1(a[q]);
-                    if (v > bv) { bv = v; bi = r; }
-                }
-            }
-            sh.v[g] = bv;
-            sh.i[g] = bi;
-        }
-        __syncthreads();
-        double bv = sh.v[0];
-        int p = sh.i[0];
-#pragma unroll
-        for (int q = 1; q < 8; ++q) {
-            const double vq = sh.v[q];
-            const int iq = sh.i[q];
-            if (vq > bv || (vq == bv && iq < p)) { bv = vq; p = iq; }
-        }
-        if (p < j || p >= nv) p = j;  // no comparable candidate (NaN after a zero pivot)
-        if (g == p / kTsRPT) s_prow[c] = pickn<kTsRPT>(a, p % kTsRPT);
-        if (g == j / kTsRPT) s_drow[c] = pickn<kTsRPT>(a, j % kTsRPT);
-        if (tid == 0) {
-            const int t = sh.perm[j];
-            sh.perm[j] = sh.perm[p];
-            sh.perm[p] = t;
-        }
-        __syncthreads();
-        const C2<R> pv = s_prow[j];
-        const C2<R> prc = s_prow[c];
-        if (p != j) {
-            if (g == j / kTsRPT) placen<kTsRPT>(a, j % kTsRPT, prc);
-            if (g == p / kTsRPT) placen<kTsRPT>(a, p % kTsRPT, s_drow[c]);
-        }
-        const bool nz = pv.x != 0 || pv.y != 0;
-        const C2<R> ip = nz ? cinv(pv) : C2<R>{0, 0};
-        const bool upd = c > j && c < w;
-#pragma unroll
-        for (int q = 0; q < kTsRPT; ++q) {
-            const int r = g * kTsRPT + q;
-            if (r > j && r < nv) {  // wave-uniform
-                const C2<R> lq = cmul(a[q], ip);
-                const C2<R> l{bcast(lq.x, j), bcast(lq.y, j)};
-                if (c == j) a[q] = l;
-                else if (upd) a[q] = csub(a[q], cmul(l, prc));
-            }
-        }
-    }
-    __syncthreads();
-    return cnt;
-}
-
-// Publish the selected rows (original values a0) to node `nd`.
-template <typename R>
-__device__ void ts_publish(const C2<R> (&a0)[kTsRPT], int cnt, int nd, const TsShared& sh, const PanelWS& ws) {
-    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
-    C2<R>* vals = (C2<R>*)ws.node_vals + (size_t)nd * kNB * kNB;
-    for (int s2 = 0; s2 < cnt; ++s2) {
-        const int pos = sh.perm[s2];
-        if (g == pos / kTsRPT) st_ag(&vals[s2 * kNB + c], pickn<kTsRPT>(a0, pos % kTsRPT));
-    }
-    if (tid < cnt) st_ag(&ws.node_rows[nd * kNB + tid], sh.row[sh.perm[tid]]);
-    if (tid == 0) st_ag(&ws.node_cnt[nd], cnt);
-    wait_mem();
-    __syncthreads();
-}
-
 template <typename R>
 __global__ __launch_bounds__(kTsThreads) void k_tslu(const C2<R>* __restrict__ A, int64_t lda, int n, int k0, int w,
                                                      PanelWS ws) {
-    __shared__ TsShared sh;
-    __shared__ C2<R> s_prow[kNB], s_drow[kNB], s_l[kTsRows];
+    __shared__ TsLds<R> sh;
     const int G = gridDim.x;
-    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+    const int tid = threadIdx.x, part = (tid & 63) >> 5;
+    const int slot = (tid >> 6) * 32 + (tid & 31);
     const int m = n - k0;
     int level = 0, idx = blockIdx.x;
     int nv = min(kTsRows, m - idx * kTsRows);
-    C2<R> a[kTsRPT], a0[kTsRPT];
-#pragma unroll
-    for (int q = 0; q < kTsRPT; ++q) {
-        const int r = g * kTsRPT + q;
-        const int row = k0 + idx * kTsRows + r;
-        a0[q] = (r < nv && c < w) ? A[(int64_t)row * lda + k0 + c] : C2<R>{0, 0};
-        a[q] = a0[q];
-    }
-    if (tid < kTsRows) sh.row[tid] = (tid < nv) ? k0 + idx * kTsRows + tid : -1;
-    __syncthreads();
-    int cnt = ts_gepp(a, nv, w, sh, s_prow, s_drow, s_l);
+    int myrow = slot < nv ? k0 + idx * kTsRows + slot : -1;
+    if (tid < kNB) sh.sel[tid] = -1;
+    C2<R> a[32];
+    ts_load_row(a, A, lda, myrow, k0, w, part);
+    int cnt = ts_eliminate(a, nv, w, myrow, sh);
     for (;;) {
         const int size = ts_level_size(G, level);
         if (size == 1) {
             // root: the panel's pivot rows in order
-            if (tid < cnt) st_ag(&ws.root[1 + tid], sh.row[sh.perm[tid]]);
+            if (tid < cnt) st_ag(&ws.root[1 + tid], sh.sel[tid]);
             if (tid == 0) st_ag(&ws.root[0], cnt);
             return;
         }
         const int nd = ts_level_off(G, level) + idx;
-        ts_publish(a0, cnt, nd, sh, ws);
+        if (tid < cnt) st_ag(&ws.node_rows[nd * kNB + tid], sh.sel[tid]);
+        if (tid == 0) st_ag(&ws.node_cnt[nd], cnt);
+        wait_mem();
+        __syncthreads();
         const int parent = ts_level_off(G, level + 1) + (idx >> 1);
         const int sib = idx ^ 1;
         if (sib < size) {
@@ -545,24 +613,14 @@ __global__ __launch_bounds__(kTsThreads) void k_tslu(const C2<R>* __restrict__ A
         const int nl = ld_ag(&ws.node_cnt[left]);
         const int nr = (sib < size) ? ld_ag(&ws.node_cnt[left + 1]) : 0;
         nv = nl + nr;
-        const C2<R>* vl = (const C2<R>*)ws.node_vals + (size_t)left * kNB * kNB;
-        const C2<R>* vr = vl + kNB * kNB;
-#pragma unroll
-        for (int q = 0; q < kTsRPT; ++q) {
-            const int r = g * kTsRPT + q;
-            C2<R> v{0, 0};
-            if (c < w) {
-                if (r < nl) v = ld_ag(&vl[r * kNB + c]);
-                else if (r < nv) v = ld_ag(&vr[(r - nl) * kNB + c]);
-            }
-            a0[q] = v;
-            a[q] = v;
+        myrow = slot < nl ? ld_ag(&ws.node_rows[left * kNB + slot])
+                          : (slot < nv ? ld_ag(&ws.node_rows[(left + 1) * kNB + slot - nl]) : -1);
+        if (slot < nv && (myrow < k0 || myrow >= n)) {  // never expected: flag it, keep every access in bounds
+            st_ag(&ws.status[1], 1);
+            myrow = -1;
         }
-        if (tid < kTsRows)
-            sh.row[tid] = tid < nl ? ld_ag(&ws.node_rows[left * kNB + tid])
-                                   : (tid < nv ? ld_ag(&ws.node_rows[(left + 1) * kNB + tid - nl]) : -1);
-        __syncthreads();
-        cnt = ts_gepp(a, nv, w, sh, s_prow, s_drow, s_l);
+        ts_load_row(a, A, lda, myrow, k0, w, part);
+        cnt = ts_eliminate(a, nv, w, myrow, sh);
         level += 1;
         idx >>= 1;
     }
@@ -655,7 +713,17 @@ __global__ __launch_bounds__(256) void k_tslu_apply(C2<R>* __restrict__ A, int64
     __shared__ int s_dst[kNB], s_src[kNB], s_nd;
     const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
     if (tid <= kNB) {
-        const int v = ld_ag(&ws.root[tid]);
+        int v = ld_ag(&ws.root[tid]);
+        // the root's rows must be w distinct rows of [k0, n); out-of-range
+        // values are flagged and replaced so that every access stays in bounds
+        if (tid == 0 && v != w) {
+            st_ag(&ws.status[1], 1);
+            v = w;
+        }
+        if (tid > 0 && tid <= w && (v < k0 || v >= n)) {
+            st_ag(&ws.status[1], 1);
+            v = k0 + tid - 1 < n ? k0 + tid - 1 : k0;
+        }
         if (tid == 0) s_cnt = v;
         else s_root[tid - 1] = v;
     }

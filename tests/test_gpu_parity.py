@@ -83,6 +83,18 @@ def test_device_lu(ctx, n):
     assert np.abs(x32 - xs).max() < 20 * ref32 + 1e-6 * np.abs(xs).max()
 
 
+@pytest.mark.parametrize("n", [300, 1000, 2600])
+def test_device_lu_tournament_pivots(ctx, n):
+    """Multi-leaf panels: the device's pivot sequence is the tournament's
+    (tests/tslu_model.py), index for index, in FP64."""
+    import tslu_model as tm
+    say(f"[gpu] device LU tournament pivots n={n}")
+    rng = np.random.default_rng(100 + n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    _, piv = bh.debug_lu(A, ctx=ctx)
+    assert np.array_equal(piv - 1, tm.blocked_lu_pivots(A))
+
+
 def test_device_lu_singular_column_reports_zero_pivot(ctx):
     say("[gpu] device LU on a singular matrix")
     A = np.eye(100, dtype=np.complex128)
