@@ -43,7 +43,9 @@ EXPORTED = [
     "bem_mesh_quadrature", "bem_mesh_near_count", "bem_operator_create", "bem_operator_destroy",
     "bem_operator_matrix", "bem_operator_size", "bem_operator_solve", "bem_solve",
     "bem_solution_destroy", "bem_solution_count", "bem_solution_traces", "bem_far_field",
-    "bem_solution_diagnostics", "bem_sweep", "bem_debug_lu",
+    "bem_solution_diagnostics", "bem_sweep", "bem_server_create", "bem_server_destroy",
+    "bem_server_submit", "bem_server_poll", "bem_server_wait", "bem_server_error",
+    "bem_server_stats_get", "bem_debug_lu",
 ]
 
 
@@ -81,6 +83,17 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int64)
 _vp = ctypes.c_void_p
 _LIB = None
+
+
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("lanes", ctypes.c_int32), ("keep_factors", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
+                ("queued", ctypes.c_int64), ("running", ctypes.c_int64),
+                ("resident_operators", ctypes.c_int64), ("factor_reuses", ctypes.c_int64),
+                ("resident_bytes", ctypes.c_uint64)]
 
 
 def load_library(path=LIB_PATH):
@@ -129,6 +142,15 @@ def load_library(path=LIB_PATH):
         "bem_sweep": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(_vp), ctypes.POINTER(Problem),
                                      _dp, _dp, _dp, ctypes.POINTER(Diagnostics)]),
         "bem_debug_lu": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, ctypes.c_int32, _dp, _ip, _dp, _dp]),
+        "bem_server_create": (ctypes.c_int, [_vp, ctypes.POINTER(ServerOptions), ctypes.POINTER(_vp)]),
+        "bem_server_destroy": (ctypes.c_int, [_vp]),
+        "bem_server_submit": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Problem), ctypes.c_double,
+                                             ctypes.c_double, _ip]),
+        "bem_server_poll": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)]),
+        "bem_server_wait": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_double, _dp,
+                                           ctypes.POINTER(Diagnostics), _dp]),
+        "bem_server_error": (ctypes.c_char_p, [_vp, ctypes.c_int64]),
+        "bem_server_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -621,3 +643,91 @@ def batched_sweep(meshes, problems, angles, ctx=None):
     diags = (Diagnostics * n)()
     ctx.check(ctx.lib.bem_sweep(ctx._h, n, hs, probs, _d(ang), amps.ctypes.data_as(_dp), _d(lat), diags))
     return amps, lat, [d.as_dict() for d in diags]
+
+
+# ---------------------------------------------------------------------------
+# serving
+# ---------------------------------------------------------------------------
+@dataclass
+class ServeResult:
+    """One served request: backscatter amplitude, solve report and latencies [ms]."""
+    amplitude: complex
+    diagnostics: dict
+    queue_ms: float
+    run_ms: float
+    total_ms: float
+
+
+class Server:
+    """Continuous request loop on one GPU (the bem_server_* C ABI).
+
+    `submit` queues the unit of work of bem(surface, boundary, k;
+    incidence_angle, incidence_azimuth) followed by scattering_amplitude
+    (src/solvers/bem/surface.jl:10-22, src/solvers/bem/postprocessing.jl:1-13)
+    and returns a ticket at once; `lanes` requests run concurrently, each
+    admitted when its device memory fits the context's HBM budget.  With
+    keep_factors, assembled operators and LU factors stay resident and serve
+    later requests on the same mesh and problem without reassembly."""
+
+    def __init__(self, ctx=None, lanes=4, keep_factors=True):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        opt = ServerOptions(lanes, 1 if keep_factors else 0)
+        h = _vp()
+        self.ctx.check(self.lib.bem_server_create(self.ctx._h, ctypes.byref(opt), ctypes.byref(h)))
+        self._h = h
+
+    def _check(self, rc, ticket=0):
+        if rc != BEM_OK:
+            _raise(rc, self.lib.bem_server_error(self._h, ticket).decode())
+
+    def submit(self, surface, boundary, k, *, incidence_angle=math.pi / 2, incidence_azimuth=0.0, **kw):
+        """Queue a backscatter request; keyword arguments as for bem()."""
+        if not (math.isfinite(k) and k > 0):
+            raise ValueError("k must be finite and positive")
+        prob = make_problem(boundary, k, formulation=kw.pop("formulation", None),
+                            precision=kw.pop("precision", "mixed"), refinement=kw.pop("refinement", None),
+                            equilibrate=kw.pop("equilibrate", True))
+        if kw:
+            raise ValueError(f"unknown options {sorted(kw)}")
+        t = ctypes.c_int64()
+        self._check(self.lib.bem_server_submit(self._h, surface._h, ctypes.byref(prob), float(incidence_angle),
+                                               float(incidence_azimuth), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        done = ctypes.c_int32()
+        self._check(self.lib.bem_server_poll(self._h, ticket, ctypes.byref(done)))
+        return bool(done.value)
+
+    def wait(self, ticket, timeout=None):
+        """Block until the request ends; returns a ServeResult or raises its error
+        (TimeoutError when `timeout` seconds pass first)."""
+        amp = np.empty(2)
+        d = Diagnostics()
+        tm = np.empty(3)
+        rc = self.lib.bem_server_wait(self._h, ticket, -1.0 if timeout is None else 1e3 * timeout, _d(amp),
+                                      ctypes.byref(d), _d(tm))
+        if rc == BEM_ERR_STATE and timeout is not None and self.poll(ticket) is False:
+            raise TimeoutError(f"request {ticket} not finished")
+        self._check(rc, ticket)
+        return ServeResult(complex(amp[0], amp[1]), d.as_dict(), tm[0], tm[1], tm[2])
+
+    def stats(self):
+        st = ServerStats()
+        self._check(self.lib.bem_server_stats_get(self._h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.bem_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

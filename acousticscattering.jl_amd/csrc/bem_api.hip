@@ -15,7 +15,11 @@
 #include <cstdio>
 #include <cstring>
 #include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <list>
 #include <map>
+#include <unordered_map>
 #include <thread>
 #include <mutex>
 #include <stdexcept>
@@ -142,14 +146,41 @@ struct bem_ctx_s {
     std::vector<Lane*> lanes;  // lanes[0] serves the single-solve entry points
 
     Lane* lane0() { return lanes[0]; }
+    // Device memory: a caching allocator.  Released blocks go to a free pool
+    // instead of hipFree (which waits on the whole device and would serialize
+    // concurrent lanes); an allocation reuses the smallest pooled block of at
+    // least its size and at most a quarter larger.  `used` counts live bytes
+    // against the budget; pooled bytes are freed when the budget needs them.
+    // Callers release a block only once the work using it has completed.
+    static size_t granule(size_t bytes) {
+        const size_t g = bytes >= ((size_t)1 << 20) ? ((size_t)2 << 20) : 256;
+        return (bytes + g - 1) / g * g;
+    }
     void* alloc(size_t bytes) {
         if (bytes == 0) bytes = 16;
+        bytes = granule(bytes);
         std::lock_guard<std::mutex> g(amu);
+        auto it = pool.lower_bound(bytes);
+        if (it != pool.end() && it->first <= bytes + bytes / 4) {
+            void* p = it->second;
+            const size_t b = it->first;
+            pool.erase(it);
+            pooled -= b;
+            used += b;
+            sizes[p] = b;
+            return p;
+        }
         if (used + bytes > budget)
             throw BemError(BEM_ERR_ADMISSION, "request of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
                                                   std::to_string(used) + " of " + std::to_string(budget) + " in use)");
+        while (used + pooled + bytes > budget && !pool.empty()) trim_one();
         void* p = nullptr;
         hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            while (!pool.empty()) trim_one();
+            e = hipMalloc(&p, bytes);
+        }
         if (e != hipSuccess) throw BemError(BEM_ERR_ADMISSION, std::string("hipMalloc failed: ") + hipGetErrorString(e));
         used += bytes;
         sizes[p] = bytes;
@@ -157,17 +188,28 @@ struct bem_ctx_s {
     }
     void release(void* p) {
         if (!p) return;
-        {
-            std::lock_guard<std::mutex> g(amu);
-            auto it = sizes.find(p);
-            if (it != sizes.end()) {
-                used -= it->second;
-                sizes.erase(it);
-            }
-        }
-        hipFree(p);
+        std::lock_guard<std::mutex> g(amu);
+        auto it = sizes.find(p);
+        if (it == sizes.end()) return;
+        used -= it->second;
+        pool.emplace(it->second, p);
+        pooled += it->second;
+        sizes.erase(it);
+    }
+    // largest pooled block back to the device (amu held)
+    void trim_one() {
+        auto it = std::prev(pool.end());
+        hipFree(it->second);
+        pooled -= it->first;
+        pool.erase(it);
+    }
+    void trim_all() {
+        std::lock_guard<std::mutex> g(amu);
+        while (!pool.empty()) trim_one();
     }
     std::map<void*, size_t> sizes;
+    std::multimap<size_t, void*> pool;
+    uint64_t pooled = 0;
 };
 
 struct bem_mesh_s {
@@ -609,6 +651,7 @@ void solve_refined(bem_operator op, bem_diagnostics& dg) {
 
 void op_free(bem_operator op) {
     bem_ctx c = op->ctx;
+    if (op->lane) hipStreamSynchronize(op->lane->s);  // its buffers may return to the pool
     for (void* p : op->heap) c->release(p);
     if (op->use_arena) op->lane->off = op->arena_mark;
     mesh_release(op->mesh);
@@ -854,6 +897,7 @@ void operator_solve(bem_operator op, int nrhs, const double* angles, bem_solutio
         HIPCHECK(hipStreamSynchronize(s));
         ctx->release(ddir);
     } catch (...) {
+        hipStreamSynchronize(s);  // nothing pending may touch blocks that return to the pool
         ctx->release(ddir);
         ctx->release(sol->p);
         ctx->release(sol->q);
@@ -876,6 +920,41 @@ size_t sweep_need(const bem_mesh m, const bem_problem& p) {
     return b;
 }
 
+// Device bytes of the buffers backscatter() needs for nt panels.
+size_t backscatter_scratch(int nt) {
+    return sizeof(z64) * ((size_t)2 * nt + kFarBlocks + 2) + sizeof(double) * 8 + 4 * 256;
+}
+
+// Backscatter amplitude of one plane wave (incidence ang[0], ang[1]) through
+// op on its lane: traces by solve_rhs, then the FP64 far field at -d.
+void backscatter(bem_operator op, char* scratch, const double* ang, double* amp, bem_diagnostics* dgo) {
+    Lane* L = op->lane;
+    const int nt = op->nt;
+    char* cur = scratch;
+    auto take = [&](size_t b) {
+        char* q = cur;
+        cur += (b + 255) & ~(size_t)255;
+        return q;
+    };
+    z64* P = (z64*)take(sizeof(z64) * nt);
+    z64* Q = (z64*)take(sizeof(z64) * nt);
+    z64* part = (z64*)take(sizeof(z64) * kFarBlocks);
+    z64* res = (z64*)take(sizeof(z64) * 2);
+    double* dd = (double*)take(sizeof(double) * 8);
+    double hd[6];
+    direction(ang[0], ang[1], hd);
+    for (int c = 0; c < 3; ++c) hd[3 + c] = -hd[c];
+    HIPCHECK(hipMemcpyAsync(dd, hd, sizeof hd, hipMemcpyHostToDevice, L->s));
+    bem_diagnostics dg;
+    solve_rhs(op, dd, P, Q, dg);
+    prof_begin(&L->pl, L->s, C_FAR);
+    launch_far_field(L->s, op->mesh->d, op->prob.k, 1, 1, dd + 3, P, Q, part, res);
+    prof_end(&L->pl, L->s, C_FAR, (double)nt * 7, (double)nt * 7 * 8);
+    HIPCHECK(hipMemcpyAsync(amp, res, sizeof(z64), hipMemcpyDeviceToHost, L->s));
+    HIPCHECK(hipStreamSynchronize(L->s));
+    if (dgo) *dgo = dg;
+}
+
 // One sweep item on lane L: assemble, factor, solve, backscatter amplitude.
 void sweep_item(bem_ctx ctx, Lane* L, bem_mesh mesh, const bem_problem& prob, const double* ang, double* amp,
                 bem_diagnostics* dgo) {
@@ -883,29 +962,257 @@ void sweep_item(bem_ctx ctx, Lane* L, bem_mesh mesh, const bem_problem& prob, co
     bem_operator op = nullptr;
     operator_create(ctx, L, mesh, &prob, &op, true);
     try {
-        const int nt = op->nt;
-        z64* P = (z64*)op_alloc(op, sizeof(z64) * nt);
-        z64* Q = (z64*)op_alloc(op, sizeof(z64) * nt);
-        z64* part = (z64*)op_alloc(op, sizeof(z64) * kFarBlocks);
-        z64* res = (z64*)op_alloc(op, sizeof(z64) * 2);
-        double* dd = (double*)op_alloc(op, sizeof(double) * 8);
-        double hd[6];
-        direction(ang[0], ang[1], hd);
-        for (int c = 0; c < 3; ++c) hd[3 + c] = -hd[c];
-        HIPCHECK(hipMemcpyAsync(dd, hd, sizeof hd, hipMemcpyHostToDevice, L->s));
-        bem_diagnostics dg;
-        solve_rhs(op, dd, P, Q, dg);
-        prof_begin(&L->pl, L->s, C_FAR);
-        launch_far_field(L->s, mesh->d, prob.k, 1, 1, dd + 3, P, Q, part, res);
-        prof_end(&L->pl, L->s, C_FAR, (double)nt * 7, (double)nt * 7 * 8);
-        HIPCHECK(hipMemcpyAsync(amp, res, sizeof(z64), hipMemcpyDeviceToHost, L->s));
-        HIPCHECK(hipStreamSynchronize(L->s));
-        if (dgo) *dgo = dg;
+        char* scratch = (char*)op_alloc(op, backscatter_scratch(op->nt));
+        backscatter(op, scratch, ang, amp, dgo);
     } catch (...) {
         op_free(op);
         throw;
     }
     op_free(op);
+}
+
+// ---------------------------------------------------------------------------
+// serving (bem_server_*): worker lanes, HBM admission, resident operators
+// ---------------------------------------------------------------------------
+struct ServeReq {
+    int64_t id = 0;
+    bem_mesh mesh = nullptr;
+    bem_problem prob{};
+    double ang[2] = {0, 0};
+    size_t need = 0;
+    double t_submit = 0, t_start = 0, t_end = 0;
+    int status = BEM_OK;
+    std::string err;
+    double amp[2] = {0, 0};
+    bem_diagnostics diag{};
+    bool done = false;
+};
+
+// An operator kept resident after its request: it lives in `block`.
+struct Resident {
+    bem_mesh mesh = nullptr;
+    bem_problem prob{};
+    bem_operator op = nullptr;
+    char* block = nullptr;
+    size_t bytes = 0;
+    bool busy = false;
+    uint64_t stamp = 0;
+};
+
+struct ServeLane {
+    Lane* L = nullptr;
+    char* scratch = nullptr;
+    size_t scap = 0;
+};
+
+bool same_problem(const bem_problem& a, const bem_problem& b) {
+    return a.boundary == b.boundary && a.formulation == b.formulation && a.precision == b.precision &&
+           a.equilibrate == b.equilibrate && a.k == b.k && a.density_contrast == b.density_contrast &&
+           a.soundspeed_contrast == b.soundspeed_contrast && a.refine_tolerance == b.refine_tolerance &&
+           a.refine_maxiter == b.refine_maxiter && a.force_promotion == b.force_promotion &&
+           a.condition_guard == b.condition_guard;
+}
+
+}  // namespace
+
+struct bem_server_s {
+    bem_ctx ctx = nullptr;
+    bool keep = false;
+    std::vector<ServeLane> lanes;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServeReq*> queue;
+    std::unordered_map<int64_t, ServeReq*> reqs;
+    std::list<Resident> resident;
+    int64_t next_id = 1, submitted = 0, completed = 0, failed = 0, running = 0, reuses = 0;
+    uint64_t stamp = 0;
+    bool stop = false;
+    std::string msg;
+};
+
+namespace {
+
+void evict(bem_server srv, std::list<Resident>::iterator it) {
+    op_free(it->op);  // resident operators own no arena: this frees heap buffers only
+    srv->ctx->release(it->block);
+    mesh_release(it->mesh);
+    srv->resident.erase(it);
+}
+
+void finish(bem_server srv, ServeReq* r, int status, const std::string& err) {
+    r->status = status;
+    r->err = err;
+    r->done = true;
+    if (r->t_end == 0) r->t_end = now_ms();
+    if (status == BEM_OK) ++srv->completed;
+    else ++srv->failed;
+    mesh_release(r->mesh);
+    r->mesh = nullptr;
+}
+
+// Admission, with srv->mu held.  Oldest first: a request whose operator is
+// resident and idle runs on it; otherwise it needs a device block of its
+// estimated size, made from free budget plus idle resident operators evicted
+// least recently used.  The oldest request that does not fit keeps a claim on
+// memory: later requests are admitted only from memory beyond its need.
+bool admit(bem_server srv, ServeReq*& out, std::list<Resident>::iterator& hit, char*& block) {
+    bem_ctx ctx = srv->ctx;
+    size_t claim = 0;
+    hit = srv->resident.end();
+    for (auto qi = srv->queue.begin(); qi != srv->queue.end(); ++qi) {
+        ServeReq* r = *qi;
+        auto res = srv->resident.end();
+        for (auto it = srv->resident.begin(); it != srv->resident.end(); ++it)
+            if (it->mesh == r->mesh && same_problem(it->prob, r->prob)) res = it;
+        if (res != srv->resident.end()) {
+            if (res->busy) continue;  // in use on another lane: wait for it
+            res->busy = true;
+            hit = res;
+            out = r;
+            srv->queue.erase(qi);
+            return true;
+        }
+        uint64_t used, budget;
+        {
+            std::lock_guard<std::mutex> g(ctx->amu);
+            used = ctx->used;
+            budget = ctx->budget;
+        }
+        uint64_t freeb = budget > used ? budget - used : 0, idle = 0;
+        for (auto& e : srv->resident)
+            if (!e.busy) idle += e.bytes;
+        if (r->need + claim <= freeb + idle) {
+            while (freeb < r->need + claim) {
+                auto lru = srv->resident.end();
+                for (auto it = srv->resident.begin(); it != srv->resident.end(); ++it)
+                    if (!it->busy && (lru == srv->resident.end() || it->stamp < lru->stamp)) lru = it;
+                if (lru == srv->resident.end()) break;
+                freeb += lru->bytes;
+                evict(srv, lru);
+            }
+            try {
+                block = (char*)ctx->alloc(r->need);
+            } catch (const BemError&) {
+                block = nullptr;
+            }
+            if (block) {
+                out = r;
+                srv->queue.erase(qi);
+                return true;
+            }
+        }
+        if (claim == 0) claim = r->need;
+    }
+    // nothing runs, so nothing will free memory: the oldest request cannot fit
+    if (srv->running == 0 && !srv->queue.empty()) {
+        bool busy_wait = false;
+        for (auto& e : srv->resident) busy_wait |= e.busy;
+        if (!busy_wait) {
+            ServeReq* r = srv->queue.front();
+            srv->queue.pop_front();
+            finish(srv, r, BEM_ERR_ADMISSION, "request of " + std::to_string(r->need) +
+                                                  " bytes does not fit in the free HBM budget");
+            srv->cv_done.notify_all();
+            return admit(srv, out, hit, block);  // the rest of the queue may fit now
+        }
+    }
+    return false;
+}
+
+void serve_worker(bem_server srv, int li) {
+    (void)hipSetDevice(srv->ctx->device);
+    bem_ctx ctx = srv->ctx;
+    ServeLane& SL = srv->lanes[li];
+    Lane* L = SL.L;
+    std::unique_lock<std::mutex> lk(srv->mu);
+    for (;;) {
+        ServeReq* r = nullptr;
+        std::list<Resident>::iterator hit;
+        char* block = nullptr;
+        while (!srv->stop && !admit(srv, r, hit, block)) srv->cv_work.wait(lk);
+        if (srv->stop && !r) return;
+        const bool reuse = hit != srv->resident.end();
+        ++srv->running;
+        lk.unlock();
+        bem_operator built = nullptr;
+        int status = BEM_OK;
+        std::string err;
+        r->t_start = now_ms();
+        try {
+            bem_operator op;
+            if (reuse) {
+                op = hit->op;
+                op->lane = L;
+            } else {
+                L->arena = block;
+                L->cap = r->need;
+                L->off = 0;
+                operator_create(ctx, L, r->mesh, &r->prob, &op, true);
+                built = op;
+            }
+            const size_t sb = backscatter_scratch(op->nt);
+            if (SL.scap < sb) {
+                ctx->release(SL.scratch);
+                SL.scratch = nullptr;
+                SL.scap = 0;
+                SL.scratch = (char*)ctx->alloc(sb);
+                SL.scap = sb;
+            }
+            backscatter(op, SL.scratch, r->ang, r->amp, &r->diag);
+        } catch (const BemError& e) {
+            status = e.code;
+            err = e.what();
+        } catch (const std::exception& e) {
+            status = BEM_ERR_DEVICE;
+            err = e.what();
+        }
+        r->t_end = now_ms();
+        const bool keep = !reuse && built && srv->keep && status == BEM_OK;
+        if (!reuse && !keep) {  // back to the context's pool, outside the server lock
+            if (built) op_free(built);
+            ctx->release(block);
+        }
+        lk.lock();
+        if (reuse) {
+            hit->busy = false;
+            hit->stamp = ++srv->stamp;
+            ++srv->reuses;
+        } else if (keep) {
+            built->use_arena = false;  // owned by the resident block from now on
+            Resident e;
+            e.mesh = r->mesh;
+            e.mesh->refs++;
+            e.prob = r->prob;
+            e.op = built;
+            e.block = block;
+            e.bytes = r->need;
+            e.stamp = ++srv->stamp;
+            srv->resident.push_back(e);
+        }
+        L->arena = nullptr;
+        L->cap = L->off = 0;
+        finish(srv, r, status, err);
+        --srv->running;
+        srv->cv_work.notify_all();
+        srv->cv_done.notify_all();
+    }
+}
+
+template <typename F>
+int srv_guard(bem_server srv, F&& f) {
+    try {
+        f();
+        return BEM_OK;
+    } catch (const BemError& e) {
+        std::lock_guard<std::mutex> g(srv->mu);
+        srv->msg = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> g(srv->mu);
+        srv->msg = e.what();
+        return BEM_ERR_DEVICE;
+    }
 }
 
 }  // namespace
@@ -969,6 +1276,7 @@ int bem_ctx_destroy(bem_ctx ctx) {
     ctx->prof.drain();
     for (auto e : ctx->prof.pool) hipEventDestroy(e);
     for (Lane* L : ctx->lanes) free_lane(ctx, L);
+    ctx->trim_all();
     delete ctx;
     return BEM_OK;
 }
@@ -1153,6 +1461,7 @@ int bem_mesh_create(bem_ctx ctx, int64_t nv, const double* nodes, int64_t nt, co
             d.near_row = (int*)up(rowof.data(), sizeof(int32_t) * std::max<size_t>(1, rowof.size()));
             HIPCHECK(hipStreamSynchronize(s));
         } catch (...) {
+            hipStreamSynchronize(ctx->lane0()->s);
             for (void* p : m->bufs) ctx->release(p);
             delete m;
             throw;
@@ -1338,6 +1647,7 @@ int bem_far_field(bem_solution sol, int32_t r, int64_t ndir, const double* dirs,
             HIPCHECK(hipMemcpyAsync(out, res, sizeof(z64) * ndir, hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));
         } catch (...) {
+            hipStreamSynchronize(s);
             ctx->release(dd); ctx->release(part); ctx->release(res);
             throw;
         }
@@ -1425,6 +1735,169 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh* meshes, const bem_problem*
     });
 }
 
+int bem_server_create(bem_ctx ctx, const bem_server_options* options, bem_server* out) {
+    if (!ctx || !out) return BEM_ERR_ARGUMENT;
+    const int nl = options ? options->lanes : 4;
+    if (nl < 1 || nl > 16) {
+        ctx->err = "server lanes must be in 1..16";
+        return BEM_ERR_ARGUMENT;
+    }
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        auto* srv = new bem_server_s();
+        srv->ctx = ctx;
+        srv->keep = options ? options->keep_factors != 0 : true;
+        try {
+            for (int l = 0; l < nl; ++l) {
+                ServeLane sl;
+                sl.L = make_lane(ctx);
+                srv->lanes.push_back(sl);
+            }
+            for (int l = 0; l < nl; ++l) srv->workers.emplace_back(serve_worker, srv, l);
+        } catch (...) {
+            {
+                std::lock_guard<std::mutex> lk(srv->mu);
+                srv->stop = true;
+            }
+            srv->cv_work.notify_all();
+            for (auto& t : srv->workers) t.join();
+            for (auto& sl : srv->lanes) free_lane(ctx, sl.L);
+            delete srv;
+            throw;
+        }
+        *out = srv;
+    });
+}
+
+int bem_server_destroy(bem_server srv) {
+    if (!srv) return BEM_ERR_ARGUMENT;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+        for (ServeReq* r : srv->queue) finish(srv, r, BEM_ERR_STATE, "server destroyed before the request ran");
+        srv->queue.clear();
+    }
+    srv->cv_work.notify_all();
+    srv->cv_done.notify_all();
+    for (auto& t : srv->workers) t.join();
+    while (!srv->resident.empty()) evict(srv, srv->resident.begin());
+    for (auto& sl : srv->lanes) {
+        srv->ctx->release(sl.scratch);
+        free_lane(srv->ctx, sl.L);
+    }
+    for (auto& kv : srv->reqs) {
+        if (kv.second->mesh) mesh_release(kv.second->mesh);
+        delete kv.second;
+    }
+    delete srv;
+    return BEM_OK;
+}
+
+int bem_server_submit(bem_server srv, bem_mesh mesh, const bem_problem* problem, double beta, double alpha,
+                      int64_t* ticket) {
+    if (!srv || !mesh || !problem || !ticket) return BEM_ERR_ARGUMENT;
+    return srv_guard(srv, [&] {
+        if (mesh->ctx != srv->ctx) throw BemError(BEM_ERR_ARGUMENT, "mesh belongs to another context");
+        validate_problem(*problem);
+        if (!std::isfinite(beta) || !std::isfinite(alpha))
+            throw BemError(BEM_ERR_ARGUMENT, "incidence angles must be finite");
+        auto* r = new ServeReq();
+        r->mesh = mesh;
+        r->prob = *problem;
+        r->ang[0] = beta;
+        r->ang[1] = alpha;
+        r->need = sweep_need(mesh, *problem);
+        if (r->need > srv->ctx->budget) {
+            delete r;
+            throw BemError(BEM_ERR_ADMISSION, "request exceeds the context's HBM budget");
+        }
+        mesh->refs++;
+        r->t_submit = now_ms();
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            r->id = srv->next_id++;
+            srv->queue.push_back(r);
+            srv->reqs[r->id] = r;
+            ++srv->submitted;
+        }
+        srv->cv_work.notify_all();
+        *ticket = r->id;
+    });
+}
+
+int bem_server_poll(bem_server srv, int64_t ticket, int32_t* done) {
+    if (!srv || !done) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(ticket);
+    if (it == srv->reqs.end()) {
+        srv->msg = "unknown ticket";
+        return BEM_ERR_STATE;
+    }
+    *done = it->second->done ? 1 : 0;
+    return BEM_OK;
+}
+
+int bem_server_wait(bem_server srv, int64_t ticket, double timeout_ms, double* amplitude, bem_diagnostics* diag,
+                    double* timing) {
+    if (!srv) return BEM_ERR_ARGUMENT;
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(ticket);
+    if (it == srv->reqs.end()) {
+        srv->msg = "unknown ticket";
+        return BEM_ERR_STATE;
+    }
+    ServeReq* r = it->second;
+    auto ready = [&] { return r->done; };
+    if (timeout_ms < 0) {
+        srv->cv_done.wait(lk, ready);
+    } else if (!srv->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready)) {
+        srv->msg = "request not finished";
+        return BEM_ERR_STATE;
+    }
+    if (r->status != BEM_OK) return r->status;  // kept for bem_server_error
+    if (amplitude) {
+        amplitude[0] = r->amp[0];
+        amplitude[1] = r->amp[1];
+    }
+    if (diag) *diag = r->diag;
+    if (timing) {
+        timing[0] = r->t_start - r->t_submit;
+        timing[1] = r->t_end - r->t_start;
+        timing[2] = r->t_end - r->t_submit;
+    }
+    srv->reqs.erase(it);
+    delete r;
+    return BEM_OK;
+}
+
+const char* bem_server_error(bem_server srv, int64_t ticket) {
+    if (!srv) return "null server";
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(ticket);
+    if (it != srv->reqs.end() && it->second->done && it->second->status != BEM_OK) {
+        srv->msg = it->second->err;
+        delete it->second;
+        srv->reqs.erase(it);
+    }
+    return srv->msg.c_str();
+}
+
+int bem_server_stats_get(bem_server srv, bem_server_stats* out) {
+    if (!srv || !out) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    out->submitted = srv->submitted;
+    out->completed = srv->completed;
+    out->failed = srv->failed;
+    out->queued = (int64_t)srv->queue.size();
+    out->running = srv->running;
+    out->resident_operators = (int64_t)srv->resident.size();
+    out->factor_reuses = srv->reuses;
+    uint64_t b = 0;
+    for (auto& e : srv->resident) b += e.bytes;
+    out->resident_bytes = b;
+    return BEM_OK;
+}
+
 int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t single, double* LUout, int64_t* ipiv_out,
                  const double* b, double* x) {
     if (!ctx || !A || n64 <= 0 || n64 > 65536) return BEM_ERR_ARGUMENT;
@@ -1484,6 +1957,7 @@ int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t single, doub
                 HIPCHECK(hipStreamSynchronize(s));
             }
         } catch (...) {
+            hipStreamSynchronize(s);
             for (void* p : bufs) ctx->release(p);
             op.LU = op.inv = op.wv = op.wv2 = op.wv3 = nullptr;
             op.ipiv = op.perm = nullptr;

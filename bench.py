@@ -9,6 +9,11 @@ complex-FP32 MFMA LU, FP64 iterative refinement to backward error 1e-13,
 FP64 far field.  |dTS| is checked against the same sweep solved all-FP64.
 
 Prints one JSON line (value = solves/s of the timed sweep).
+
+`--workload serve` measures the served form instead: seeded Poisson streams of
+random bodies of revolution (rigid / pressure-release / fluid, N in [2000,
+40000]) at 50, 200 and 800 requests/s through the server, reporting solves/s
+and p50/p99 latency (submission to result) per rate.
 """
 import argparse
 import json
@@ -118,8 +123,72 @@ def cpu_baseline(plan, budget_s=30.0):
                       f"; sweep time extrapolated per class from N={nref} (assembly ~N^2, LU ~N^3)"}
 
 
+def serve_rate(bh, ctx, rate, count, seed, lanes):
+    """Replay one seeded stream through a server; latency = submission to result."""
+    import workloads as W
+
+    reqs = W.serving_stream(rate, count, seed)
+    meshes = [bh.Mesh(r["nodes"], r["triangles"], ctx=ctx) for r in reqs]
+    with bh.Server(ctx, lanes=lanes, keep_factors=True) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        for r, m in zip(reqs, meshes):
+            delay = r["t"] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(m, r["boundary"], r["k"]))
+        res = [srv.wait(t) for t in tickets]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([x.total_ms for x in res])
+    unknowns = [m.n * (2 if isinstance(r["boundary"], bh.FluidFilled) else 1) for r, m in zip(reqs, meshes)]
+    return {"rate": rate, "requests": count, "seed": seed, "solves_per_s": count / wall, "wall_s": wall,
+            "p50_latency_ms": float(np.percentile(lat, 50)), "p99_latency_ms": float(np.percentile(lat, 99)),
+            "mean_queue_ms": float(np.mean([x.queue_ms for x in res])),
+            "mean_run_ms": float(np.mean([x.run_ms for x in res])),
+            "unknowns_min_max": [min(unknowns), max(unknowns)],
+            "promoted_to_fp64": sum(1 for x in res if x.diagnostics["factor_precision"] == "double"),
+            "failed": st["failed"]}
+
+
+def serve_main(args):
+    import bemhip as bh
+
+    ctx = bh.Context(0)
+    # warm-up: load every kernel once
+    warm = bh.mesh(bh.Sphere(0.01), level=3, ctx=ctx)
+    for bc in (bh.Rigid(), bh.PressureRelease(), bh.FluidFilled(1.03, 1.02)):
+        with bh.Server(ctx, lanes=1) as srv:
+            srv.wait(srv.submit(warm, bc, 100.0))
+    rates = {}
+    for i, rate in enumerate(args.rates):
+        log(f"serving {args.requests} requests at {rate} req/s on {args.lanes} lanes ...")
+        rates[str(rate)] = serve_rate(bh, ctx, rate, args.requests, 1000 + i, args.lanes)
+        log(f"  {rates[str(rate)]['solves_per_s']:.1f} solves/s, p50 {rates[str(rate)]['p50_latency_ms']:.0f} ms, "
+            f"p99 {rates[str(rate)]['p99_latency_ms']:.0f} ms")
+    top = rates[str(args.rates[-1])]
+    out = {
+        "metric": "served BEM solves/s (Poisson stream of random bodies of revolution); p50/p99 latency",
+        "value": top["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": top["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c32 LU / f64 refinement", "data": "synthetic",
+        "config": {"workload": "served stream: bodies of revolution r=a sin(pi s)(1+eps cos 3 pi s), "
+                               "a~U[5,50] mm, aspect~U[1,8], eps~U[0,0.2], rigid/soft/fluid, ka~U[0.5,15], "
+                               f"N in [2000,40000], mixed precision, {args.lanes} lanes, factors resident",
+                   "rates": args.rates, "requests_per_rate": args.requests},
+        "p50_latency_ms": top["p50_latency_ms"], "p99_latency_ms": top["p99_latency_ms"],
+        "per_rate": rates,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["sweep", "serve"], default="sweep")
+    ap.add_argument("--rates", type=float, nargs="+", default=[50.0, 200.0, 800.0])
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--lanes", type=int, default=8)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
@@ -132,6 +201,9 @@ def main():
         print("bench.py: this build runs on one GPU only; the frequency sweep would shard "
               "frequencies across GPUs with no exchange (see the design notes).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "serve":
+        serve_main(args)
+        return
 
     import bemhip as bh
 

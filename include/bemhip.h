@@ -203,6 +203,54 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh *meshes, const bem_problem 
               const double *angles, double *amplitudes, double *latency_ms,
               bem_diagnostics *diagnostics);
 
+/* ---- serving: continuous request loop -------------------------------------
+ * The reference has no server; its unit of work is the blocking solve that
+ * `bem(surface, bc, k; incidence_angle, incidence_azimuth)` performs
+ * (src/solvers/bem/surface.jl:10-22) followed by the backscatter amplitude
+ * (src/postprocessing/bem/postprocessing.jl:1-13).  A server runs that unit
+ * for requests submitted at any time, from any thread:
+ *  - `lanes` worker streams run admitted requests concurrently on the device;
+ *  - a request is admitted when a lane is free and its device block (the
+ *    operator, factors and solve buffers, sized up front) fits in the
+ *    context's HBM budget; the oldest waiting request has first claim on
+ *    memory, later ones may pass it only with memory it cannot use;
+ *  - with keep_factors, assembled operators and their LU factors stay resident
+ *    after a request and serve later requests with the same mesh and problem
+ *    (any incidence) without reassembly; resident operators are evicted least
+ *    recently used when admission needs their memory.
+ * Results are bit-identical to a solo bem_solve of the same request. */
+typedef struct bem_server_s *bem_server;
+
+typedef struct {
+    int32_t lanes;        /* concurrent requests on the device, 1..16            */
+    int32_t keep_factors; /* keep operators + factors resident for reuse (0/1)    */
+} bem_server_options;
+
+typedef struct {
+    int64_t submitted, completed, failed, queued, running;
+    int64_t resident_operators, factor_reuses;
+    uint64_t resident_bytes; /* device bytes held by resident operators          */
+} bem_server_stats;
+
+int bem_server_create(bem_ctx ctx, const bem_server_options *options, bem_server *out);
+/* Waits for running requests; queued requests end with BEM_ERR_STATE. */
+int bem_server_destroy(bem_server srv);
+/* Queues a backscatter request (incidence angles as in bem_solve); the
+ * server holds a reference to the mesh until the request ends. */
+int bem_server_submit(bem_server srv, bem_mesh mesh, const bem_problem *problem, double incidence_angle,
+                      double incidence_azimuth, int64_t *ticket);
+int bem_server_poll(bem_server srv, int64_t ticket, int32_t *done);
+/* Blocks until the request ends or timeout_ms passes (< 0: no limit); returns
+ * BEM_ERR_STATE on timeout, else the request's own status, and on success the
+ * backscatter amplitude (2 doubles), the diagnostics and timing[3] = queue,
+ * run and total latency in ms.  A finished request is forgotten after a
+ * successful wait. */
+int bem_server_wait(bem_server srv, int64_t ticket, double timeout_ms, double *amplitude,
+                    bem_diagnostics *diagnostics, double *timing);
+/* Message of a failed request (valid until the next call on the server). */
+const char *bem_server_error(bem_server srv, int64_t ticket);
+int bem_server_stats_get(bem_server srv, bem_server_stats *out);
+
 /* ---- test hook: dense LU of a caller matrix ------------------------------
  * Factors the column-major n x n complex matrix A on the device with the
  * library's LU (no equilibration; FP32 when single != 0) and returns the
