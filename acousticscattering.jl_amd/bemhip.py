@@ -45,7 +45,8 @@ EXPORTED = [
     "bem_solution_destroy", "bem_solution_count", "bem_solution_traces", "bem_far_field",
     "bem_solution_diagnostics", "bem_sweep", "bem_server_create", "bem_server_destroy",
     "bem_server_submit", "bem_server_poll", "bem_server_wait", "bem_server_error",
-    "bem_server_stats_get", "bem_debug_lu",
+    "bem_server_stats_get", "bem_hmatrix_defaults", "bem_hmatrix_plan", "bem_hmatrix_create",
+    "bem_hmatrix_destroy", "bem_hmatrix_get_info", "bem_hmatrix_matvec", "bem_hmatrix_solve", "bem_debug_lu",
 ]
 
 
@@ -69,11 +70,12 @@ class Diagnostics(ctypes.Structure):
                 ("componentwise_backward_error", ctypes.c_double),
                 ("single_reciprocal_condition", ctypes.c_double),
                 ("assembly_ms", ctypes.c_double), ("factor_ms", ctypes.c_double),
-                ("solve_ms", ctypes.c_double)]
+                ("solve_ms", ctypes.c_double), ("iterations", ctypes.c_int32), ("method", ctypes.c_int32)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["factor_precision"] = "double" if self.factor_precision == 0 else "single"
+        d["method"] = ["direct", "mixed_refinement", "gmres"][self.method]
         d["refinement_reason"] = REASONS[self.refinement_reason]
         d["converged"] = bool(self.converged)
         return d
@@ -94,6 +96,20 @@ class ServerStats(ctypes.Structure):
                 ("queued", ctypes.c_int64), ("running", ctypes.c_int64),
                 ("resident_operators", ctypes.c_int64), ("factor_reuses", ctypes.c_int64),
                 ("resident_bytes", ctypes.c_uint64)]
+
+
+class HMatrixOptions(ctypes.Structure):
+    _fields_ = [("leaf_size", ctypes.c_int32), ("eta", ctypes.c_double), ("aca_tolerance", ctypes.c_double),
+                ("max_rank", ctypes.c_int32), ("gmres_restart", ctypes.c_int32), ("gmres_maxiter", ctypes.c_int32),
+                ("gmres_tolerance", ctypes.c_double), ("preconditioner", ctypes.c_int32)]
+
+
+class HMatrixInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("nodes", ctypes.c_int64), ("leaves", ctypes.c_int64),
+                ("dense_blocks", ctypes.c_int64), ("lowrank_blocks", ctypes.c_int64), ("near_pairs", ctypes.c_int64),
+                ("dense_bytes", ctypes.c_uint64), ("lowrank_bytes", ctypes.c_uint64), ("max_rank", ctypes.c_int32),
+                ("unconverged_blocks", ctypes.c_int32), ("mean_rank", ctypes.c_double), ("plan_ms", ctypes.c_double),
+                ("assembly_ms", ctypes.c_double), ("aca_ms", ctypes.c_double)]
 
 
 def load_library(path=LIB_PATH):
@@ -151,6 +167,15 @@ def load_library(path=LIB_PATH):
                                            ctypes.POINTER(Diagnostics), _dp]),
         "bem_server_error": (ctypes.c_char_p, [_vp, ctypes.c_int64]),
         "bem_server_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
+        "bem_hmatrix_defaults": (None, [ctypes.POINTER(HMatrixOptions)]),
+        "bem_hmatrix_plan": (ctypes.c_int, [ctypes.c_int64, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                            _ip, _ip, _ip, _ip, _ip, ctypes.c_int64]),
+        "bem_hmatrix_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Problem), ctypes.POINTER(HMatrixOptions),
+                                              ctypes.POINTER(_vp)]),
+        "bem_hmatrix_destroy": (ctypes.c_int, [_vp]),
+        "bem_hmatrix_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(HMatrixInfo)]),
+        "bem_hmatrix_matvec": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
+        "bem_hmatrix_solve": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.POINTER(_vp)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -370,10 +395,11 @@ class Mesh:
         return c.value
 
 
-def mesh(nodes_or_body, triangles=None, *, level=3, units="m", ctx=None):
+def mesh(nodes_or_body, triangles=None, *, level=3, n_phi=64, units="m", ctx=None):
     """`mesh(nodes, triangles)` for a supplied surface (1-based triangles,
-    src/geometry/surface.jl:271-298) or `mesh(body; level)` for a Sphere /
-    Spheroid meshed as a mapped icosphere."""
+    src/geometry/surface.jl:271-298), `mesh(body; level)` for a Sphere /
+    Spheroid meshed as a mapped icosphere, or `mesh(cylinder; n_phi)` for a
+    (bent) capped Cylinder with n_phi azimuthal segments."""
     scale = {"m": 1.0, "cm": 1e-2, "mm": 1e-3}.get(units)
     if scale is None:
         raise ValueError("units must be :m, :cm or :mm")
@@ -384,6 +410,9 @@ def mesh(nodes_or_body, triangles=None, *, level=3, units="m", ctx=None):
         nodes, tri = icosphere_arrays(level, body.radius * scale)
     elif isinstance(body, Spheroid):
         nodes, tri = icosphere_arrays(level, body.a * scale, body.b * scale, body.b * scale)
+    elif isinstance(body, Cylinder):
+        nodes, tri = cylinder_arrays(Cylinder(body.radius * scale, body.length * scale,
+                                              body.radius_curvature * scale, body.endcap_depth * scale), n_phi)
     else:
         raise ValueError("unsupported body")
     return Mesh(nodes, tri, ctx=ctx, body=body)
@@ -731,3 +760,128 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+# ---------------------------------------------------------------------------
+# hierarchical-matrix path
+# ---------------------------------------------------------------------------
+def hmatrix_plan(points, leaf_size=64, eta=2.0, near_radius=0.0):
+    """Cluster tree and block tree of the hierarchical path (host only).
+
+    points: 3 x n.  Returns (perm, n_nodes, blocks) with perm the tree order
+    (0-based point indices) and blocks an int64 array of rows (lowrank,
+    row_lo, m, col_lo, n), dense blocks first."""
+    L = load_library()
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).T)
+    n = pts.shape[0]
+    perm = np.empty(n, dtype=np.int64)
+    nn, nd, nl = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    args = (n, _d(pts), int(leaf_size), float(eta), float(near_radius), _i(perm), ctypes.byref(nn),
+            ctypes.byref(nd), ctypes.byref(nl))
+    if L.bem_hmatrix_plan(*args, None, 0) != BEM_OK:
+        raise ValueError("invalid cluster-tree arguments")
+    blocks = np.empty((nd.value + nl.value, 5), dtype=np.int64)
+    if L.bem_hmatrix_plan(*args, _i(blocks), blocks.size) != BEM_OK:
+        raise ValueError("invalid cluster-tree arguments")
+    return perm, nn.value, blocks
+
+
+class HMatrix:
+    """Compressed rigid-surface system (the compression=(method=:hmatrix,)
+    path of solve_full_bem, src/solvers/bem/full.jl:68-184 with
+    src/solvers/bem/compression/): cluster tree, dense leaves, ACA blocks in
+    complex FP32; solve() runs batched FP64 GMRES for many incidences."""
+
+    def __init__(self, surface, boundary, k, *, formulation=None, leaf_size=64, eta=2.0, aca_tolerance=1e-6,
+                 max_rank=256, gmres_restart=50, gmres_maxiter=1000, gmres_tolerance=1e-8, preconditioner=True):
+        if not isinstance(boundary, Rigid):
+            raise ValueError("the hierarchical path solves rigid surfaces")
+        if not (math.isfinite(k) and k > 0):
+            raise ValueError("k must be finite and positive")
+        self.mesh = surface
+        self.ctx = surface.ctx
+        self.k = float(k)
+        self.problem = make_problem(boundary, k, formulation=formulation)
+        o = HMatrixOptions(int(leaf_size), float(eta), float(aca_tolerance), int(max_rank), int(gmres_restart),
+                           int(gmres_maxiter), float(gmres_tolerance), 1 if preconditioner else 0)
+        h = _vp()
+        self.ctx.check(self.ctx.lib.bem_hmatrix_create(self.ctx._h, surface._h, ctypes.byref(self.problem),
+                                                       ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self.n = surface.n
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self.ctx._h:
+            self.ctx.lib.bem_hmatrix_destroy(self._h)
+            self._h = None
+
+    def info(self):
+        st = HMatrixInfo()
+        self.ctx.check(self.ctx.lib.bem_hmatrix_get_info(self._h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+    def matvec(self, x):
+        """A x for x of shape (n,) or (n, nrhs), panel order."""
+        x = np.asarray(x, dtype=np.complex128)
+        one = x.ndim == 1
+        X = np.asfortranarray(x.reshape(self.n, -1))
+        Y = np.empty_like(X, order="F")
+        self.ctx.check(self.ctx.lib.bem_hmatrix_matvec(self._h, X.shape[1], X.ctypes.data_as(_dp),
+                                                       Y.ctypes.data_as(_dp)))
+        return Y[:, 0] if one else Y
+
+    def solve(self, angles):
+        """angles: sequence of (incidence_angle, incidence_azimuth)."""
+        ang = np.ascontiguousarray(np.asarray(angles, dtype=np.float64).reshape(-1, 2))
+        h = _vp()
+        self.ctx.check(self.ctx.lib.bem_hmatrix_solve(self._h, ang.shape[0], _d(ang), ctypes.byref(h)))
+        return _Solution(self.ctx, h, self.mesh, self.k, ang)
+
+
+@dataclass(frozen=True)
+class Cylinder:
+    """Closed cylinder along x, optionally bent (src/geometry/cylinder.jl): radius,
+    length of the centerline, radius_curvature (inf: straight) and the depth
+    of the ellipsoidal end caps."""
+    radius: float
+    length: float
+    radius_curvature: float = math.inf
+    endcap_depth: float = 0.0
+
+
+def cylinder_arrays(body, n_phi, n_s=None):
+    """Mesh of a (bent) capped cylinder: a body of revolution about its axis,
+    then the reference's bend map and axis rotation (src/geometry/cylinder.jl:
+    15-25,64-70).  Returns nodes 3 x Nv, triangles 3 x N (1-based)."""
+    a, L, R, d = body.radius, body.length, body.radius_curvature, body.endcap_depth
+    if not (a > 0 and L > 0 and d > 0):
+        raise ValueError("need radius, length and endcap_depth > 0")
+    if not R > a:
+        raise ValueError("radius_curvature must exceed the cylinder radius")
+    if math.isfinite(R) and not L / R < 2 * math.pi:
+        raise ValueError("the cylinder centerline must span less than a full circle")
+    # meridian of the straight template (axis = template z): cap, tube, cap
+    th = np.linspace(0.0, np.pi / 2, 4001)
+    cap_z, cap_r = -L / 2 - d * np.cos(th), a * np.sin(th)
+    tube_z = np.linspace(-L / 2, L / 2, 4001)[1:]
+    z = np.concatenate([cap_z, tube_z, (L / 2 + d * np.cos(th[::-1]))[1:]])
+    r = np.concatenate([cap_r, np.full(tube_z.size, a), (a * np.sin(th[::-1]))[1:]])
+    arc = np.concatenate([[0.0], np.cumsum(np.hypot(np.diff(z), np.diff(r)))])
+    if n_s is None:
+        n_s = max(3, int(round(arc[-1] / (2 * np.pi * a / n_phi))) + 1)
+    t = np.linspace(0.0, arc[-1], n_s)
+    zs, rs = np.interp(t, arc, z), np.interp(t, arc, r)
+    rs[0] = rs[-1] = 0.0
+    nodes, tri = revolution_arrays(zs, rs, n_phi)
+    # template point (x, y, z) = (node y, node z, node x); bend; rotate to x
+    x, y, zt = nodes[1], nodes[2], nodes[0]
+    if math.isfinite(R):
+        s = np.clip(zt, -L / 2, L / 2)
+        tt = zt - s
+        g = s / R
+        bx = 2 * R * np.sin(g / 2) ** 2 + x * np.cos(g) + tt * np.sin(g)
+        by = y
+        bz = (R - x) * np.sin(g) + tt * np.cos(g)
+    else:
+        bx, by, bz = x, y, zt
+    return np.vstack([bz, bx, by]), tri

@@ -28,6 +28,7 @@
 
 #include "../../include/bemhip.h"
 #include "bem_device.h"
+#include "bem_hsolve.h"
 #include "bem_internal.h"
 
 using namespace bem;
@@ -538,7 +539,7 @@ bool correction(bem_operator op, const z64* rhs, z64* out, int accumulate, int* 
     const double mag = host_max(op, kRedBlocks, 1, 0);
     if (!std::isfinite(mag)) return false;
     if (mag == 0) {
-        if (!accumulate) HIPCHECK(hipMemsetAsync(out, 0, sizeof(z64) * n, s));
+        if (!accumulate) launch_fill_double(s, (double*)out, 2 * (int64_t)n, 0.0);
         return true;
     }
     if (op->precision == 1) {
@@ -826,6 +827,7 @@ void solve_rhs(bem_operator op, const double* ddir, z64* P, z64* Q, bem_diagnost
         dg.converged = 1;
     }
     dg.factor_precision = op->precision;
+    dg.method = pr.precision == BEM_MIXED ? 1 : 0;
     dg.refinement_reason = op->reason;
     dg.single_reciprocal_condition = op->rcond;
     // residual report on the original equations: r = b - A x
@@ -1213,6 +1215,507 @@ int srv_guard(bem_server srv, F&& f) {
         srv->msg = e.what();
         return BEM_ERR_DEVICE;
     }
+}
+
+}  // namespace
+
+// ===========================================================================
+// Hierarchical-matrix path: build (plan, dense leaves, ACA), apply, GMRES
+// ===========================================================================
+// One compressed operator over the shared plan: dense leaves (same layout for
+// every operator of the plan), compact ACA factors, projection tiles and the
+// per-leaf block lists of the apply.
+struct HOp {
+    int kind = HK_BM;
+    std::vector<HBlk> blk;  // dense [0, nd), low-rank [nd, nd + nl)
+    int ntiles = 0;
+    HBlk* dblk = nullptr;
+    z32* dval = nullptr;
+    z32* uv = nullptr;
+    int64_t* toff = nullptr;
+    HTile* tiles = nullptr;
+    z64* T = nullptr;
+    HLeafs L;
+    int max_rank = 0, unconverged = 0;
+    double mean_rank = 0, aca_ms = 0, assembly_ms = 0;
+    uint64_t lowrank_bytes = 0;
+};
+
+struct bem_hmatrix_s {
+    bem_ctx ctx = nullptr;
+    bem_mesh mesh = nullptr;
+    bem_problem prob{};
+    bem_hmatrix_options opt{};
+    Lane* lane = nullptr;
+    int64_t n = 0;
+    int nd = 0, nl = 0, nleaf = 0;
+    std::vector<void*> bufs;
+    int* perm = nullptr;
+    std::vector<int> hperm, leaf_lo, leaf_m, leaf_of_pos;
+    std::vector<HBlk> plan_blk;   // first-pass layout (rank capacity 64)
+    int64_t dense_count = 0, first_cap = 0;
+    // near-pair and self slots of the dense leaves
+    int *npi = nullptr, *npj = nullptr;
+    int64_t *ndst = nullptr, *sdst = nullptr;
+    int64_t nnear = 0;
+    HOp A, B;        // system and right-hand-side operators
+    z64* minv = nullptr;
+    bem_hmatrix_info info{};
+};
+
+namespace {
+
+void* hm_alloc(bem_hmatrix h, size_t bytes) {
+    void* p = h->ctx->alloc(bytes);
+    h->bufs.push_back(p);
+    return p;
+}
+template <typename T>
+T* hm_upload(bem_hmatrix h, const std::vector<T>& v) {
+    T* p = (T*)hm_alloc(h, sizeof(T) * std::max<size_t>(1, v.size()));
+    if (!v.empty()) HIPCHECK(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, h->lane->s));
+    return p;
+}
+
+void hm_free(bem_hmatrix h) {
+    if (h->lane) hipStreamSynchronize(h->lane->s);
+    for (void* p : h->bufs) h->ctx->release(p);
+    h->bufs.clear();
+    if (h->lane) free_lane(h->ctx, h->lane);
+    h->lane = nullptr;
+    if (h->mesh) mesh_release(h->mesh);
+    h->mesh = nullptr;
+}
+
+// Plan: cluster tree, block tree, dense-leaf layout and near/self slots.
+void hm_plan(bem_hmatrix h) {
+    const HostMesh& hm = h->mesh->h;
+    const int64_t n = hm.nt;
+    h->n = n;
+    const auto& o = h->opt;
+    const double t0 = now_ms();
+    // near pairs never fall in low-rank blocks (box distance > 2 hmax)
+    double hmax = 0;
+    for (double v : hm.hmax) hmax = std::max(hmax, v);
+    HPlan P;
+    hmatrix_plan(hm.cen.data(), n, o.leaf_size, o.eta, kNearEta * hmax * (1.0 + 1e-9), P);
+    std::vector<int> leaf_of_node(P.nodes.size(), -1);
+    std::vector<std::pair<int64_t, int32_t>> lv;
+    for (size_t id = 0; id < P.nodes.size(); ++id)
+        if (P.nodes[id].child[0] < 0) lv.push_back({P.nodes[id].lo, (int32_t)id});
+    std::sort(lv.begin(), lv.end());
+    h->nleaf = (int)lv.size();
+    h->leaf_of_pos.assign(n, 0);
+    for (int l = 0; l < h->nleaf; ++l) {
+        const HNode& nd = P.nodes[lv[l].second];
+        leaf_of_node[lv[l].second] = l;
+        h->leaf_lo.push_back((int)nd.lo);
+        h->leaf_m.push_back((int)(nd.hi - nd.lo));
+        for (int64_t q = nd.lo; q < nd.hi; ++q) h->leaf_of_pos[q] = l;
+    }
+    h->nd = (int)P.dense.size();
+    h->nl = (int)P.lowrank.size();
+    std::unordered_map<int64_t, int> dmap;
+    int64_t doff = 0;
+    for (int b = 0; b < h->nd; ++b) {
+        const HNode &r = P.nodes[P.dense[b].row], &c = P.nodes[P.dense[b].col];
+        HBlk x{(int)r.lo, (int)(r.hi - r.lo), (int)c.lo, (int)(c.hi - c.lo), 0, 0, doff};
+        doff += (int64_t)x.m * x.n;
+        h->plan_blk.push_back(x);
+        dmap[(int64_t)leaf_of_node[P.dense[b].row] * h->nleaf + leaf_of_node[P.dense[b].col]] = b;
+    }
+    h->dense_count = doff;
+    std::vector<int> npi, npj;
+    std::vector<int64_t> ndst, sdst(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t pi = P.iperm[i];
+        const int li = h->leaf_of_pos[pi];
+        for (int32_t e = hm.near_ptr[i]; e < hm.near_ptr[i + 1]; ++e) {
+            const int64_t j = hm.near_col[e];
+            const int64_t pj = P.iperm[j];
+            auto it = dmap.find((int64_t)li * h->nleaf + h->leaf_of_pos[pj]);
+            if (it == dmap.end()) throw BemError(BEM_ERR_DEVICE, "near pair outside the dense leaves");
+            const HBlk& b = h->plan_blk[it->second];
+            npi.push_back((int)i);
+            npj.push_back((int)j);
+            ndst.push_back(b.off + (pi - b.row_lo) * b.n + (pj - b.col_lo));
+        }
+        auto it = dmap.find((int64_t)li * h->nleaf + li);
+        if (it == dmap.end()) throw BemError(BEM_ERR_DEVICE, "diagonal leaf block missing");
+        const HBlk& b = h->plan_blk[it->second];
+        sdst[i] = b.off + (pi - b.row_lo) * b.n + (pi - b.col_lo);
+    }
+    const int cap1 = std::min(64, o.max_rank);
+    int64_t uoff = 0;
+    for (int b = 0; b < h->nl; ++b) {
+        const HNode &r = P.nodes[P.lowrank[b].row], &c = P.nodes[P.lowrank[b].col];
+        HBlk x{(int)r.lo, (int)(r.hi - r.lo), (int)c.lo, (int)(c.hi - c.lo), 0, 0, uoff};
+        x.rmax = std::min({x.m, x.n, cap1});
+        uoff += (int64_t)x.rmax * (x.m + x.n);
+        h->plan_blk.push_back(x);
+    }
+    h->first_cap = uoff;
+    h->hperm.assign(P.perm.begin(), P.perm.end());
+    h->perm = hm_upload(h, h->hperm);
+    h->npi = hm_upload(h, npi);
+    h->npj = hm_upload(h, npj);
+    h->ndst = hm_upload(h, ndst);
+    h->sdst = hm_upload(h, sdst);
+    h->nnear = (int64_t)npi.size();
+    h->info.n = n;
+    h->info.nodes = (int64_t)P.nodes.size();
+    h->info.leaves = h->nleaf;
+    h->info.dense_blocks = h->nd;
+    h->info.lowrank_blocks = h->nl;
+    h->info.near_pairs = h->nnear;
+    h->info.dense_bytes = sizeof(z32) * (uint64_t)doff;
+    h->info.plan_ms = now_ms() - t0;
+}
+
+// Assemble one operator over the plan: dense leaves, ACA (two passes),
+// compaction, projection tiles, per-leaf lists.
+void hm_operator(bem_hmatrix h, HOp& op, int kind) {
+    bem_ctx ctx = h->ctx;
+    const DevMesh& dm = h->mesh->d;
+    hipStream_t s = h->lane->s;
+    const auto& o = h->opt;
+    const int nd = h->nd, nl = h->nl;
+    op.kind = kind;
+    op.blk = h->plan_blk;
+    const double t1 = now_ms();
+    op.dblk = hm_upload(h, op.blk);
+    op.dval = (z32*)hm_alloc(h, sizeof(z32) * std::max<int64_t>(1, h->dense_count));
+    launch_hdense(s, dm, kind, h->prob.k, h->perm, nd, op.dblk, op.dval);
+    launch_hnear_self(s, dm, kind, h->prob.k, h->nnear, h->npi, h->npj, h->ndst, h->sdst, op.dval);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s));
+    const double t2 = now_ms();
+    op.assembly_ms = t2 - t1;
+    std::vector<int> list(nl);
+    for (int b = 0; b < nl; ++b) list[b] = nd + b;
+    std::stable_sort(list.begin(), list.end(), [&](int a, int b) {
+        return (int64_t)op.blk[a].m * op.blk[a].n > (int64_t)op.blk[b].m * op.blk[b].n;
+    });
+    int* dlist = hm_upload(h, list);
+    int* conv = (int*)hm_alloc(h, sizeof(int) * (nd + nl + 1));
+    z32* uv1 = (z32*)ctx->alloc(sizeof(z32) * std::max<int64_t>(1, h->first_cap));
+    z32* uv2 = nullptr;
+    std::vector<int> hconv(nd + nl);
+    try {
+        launch_aca(s, dm, kind, h->prob.k, h->perm, nl, dlist, op.dblk, uv1, conv, o.aca_tolerance);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(op.blk.data(), op.dblk, sizeof(HBlk) * op.blk.size(), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(hconv.data(), conv, sizeof(int) * hconv.size(), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        // pass 2: blocks that hit the first capacity, with the full cap
+        std::vector<int> redo;
+        int64_t uoff2 = 0;
+        for (int b = nd; b < nd + nl; ++b) {
+            HBlk& x = op.blk[b];
+            const int cap2 = std::min({x.m, x.n, o.max_rank});
+            if (!hconv[b] && cap2 > x.rmax) {
+                x.off = uoff2;
+                x.rmax = cap2;
+                x.rank = 0;
+                uoff2 += (int64_t)x.rmax * (x.m + x.n);
+                redo.push_back(b);
+            }
+        }
+        if (!redo.empty()) {
+            uv2 = (z32*)ctx->alloc(sizeof(z32) * uoff2);
+            HIPCHECK(hipMemcpyAsync(op.dblk, op.blk.data(), sizeof(HBlk) * op.blk.size(), hipMemcpyHostToDevice, s));
+            int* dredo = hm_upload(h, redo);
+            launch_aca(s, dm, kind, h->prob.k, h->perm, (int)redo.size(), dredo, op.dblk, uv2, conv,
+                       o.aca_tolerance);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(op.blk.data(), op.dblk, sizeof(HBlk) * op.blk.size(), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipMemcpyAsync(hconv.data(), conv, sizeof(int) * hconv.size(), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+        }
+        // compact factors: U (m x rank) then V (n x rank) per block
+        std::vector<HBlk> compact = op.blk, o1, n1, o2, n2;
+        std::vector<char> in2(nd + nl, 0);
+        for (int b : redo) in2[b] = 1;
+        int64_t coff = 0;
+        int rmx = 0, unconv = 0;
+        double rsum = 0;
+        for (int b = nd; b < nd + nl; ++b) {
+            HBlk& x = compact[b];
+            x.off = coff;
+            x.rmax = x.rank;
+            coff += (int64_t)x.rank * (x.m + x.n);
+            (in2[b] ? o2 : o1).push_back(op.blk[b]);
+            (in2[b] ? n2 : n1).push_back(x);
+            rmx = std::max(rmx, x.rank);
+            rsum += x.rank;
+            unconv += hconv[b] ? 0 : 1;
+        }
+        op.uv = (z32*)hm_alloc(h, sizeof(z32) * std::max<int64_t>(1, coff));
+        HBlk* do1 = hm_upload(h, o1);
+        HBlk* dn1 = hm_upload(h, n1);
+        HBlk* do2 = hm_upload(h, o2);
+        HBlk* dn2 = hm_upload(h, n2);
+        launch_hcompact(s, (int)o1.size(), do1, dn1, uv1, op.uv);
+        if (uv2) launch_hcompact(s, (int)o2.size(), do2, dn2, uv2, op.uv);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(s));
+        ctx->release(uv1);
+        ctx->release(uv2);
+        uv1 = uv2 = nullptr;
+        op.blk = compact;
+        HIPCHECK(hipMemcpyAsync(op.dblk, op.blk.data(), sizeof(HBlk) * op.blk.size(), hipMemcpyHostToDevice, s));
+        op.max_rank = rmx;
+        op.unconverged = unconv;
+        op.mean_rank = nl ? rsum / nl : 0.0;
+        op.lowrank_bytes = sizeof(z32) * (uint64_t)coff;
+        std::vector<HTile> tiles;
+        std::vector<int64_t> toff(nd + nl, 0);
+        int64_t tsum = 0;
+        for (int b = nd; b < nd + nl; ++b) {
+            toff[b] = tsum;
+            tsum += (int64_t)op.blk[b].rank * kHRhs;
+            for (int l0 = 0; l0 < op.blk[b].rank; l0 += 16) tiles.push_back({b, l0});
+        }
+        op.ntiles = (int)tiles.size();
+        op.tiles = hm_upload(h, tiles);
+        op.toff = hm_upload(h, toff);
+        op.T = (z64*)hm_alloc(h, sizeof(z64) * std::max<int64_t>(1, tsum));
+    } catch (...) {
+        hipStreamSynchronize(s);
+        ctx->release(uv1);
+        ctx->release(uv2);
+        throw;
+    }
+    op.aca_ms = now_ms() - t2;
+    // leaf row lists: dense blocks of the row, low-rank blocks covering it
+    const int nleaf = h->nleaf;
+    std::vector<std::vector<int>> dl(nleaf);
+    std::vector<std::vector<HRef>> ll(nleaf);
+    std::vector<int> diag(nleaf, -1);
+    for (int b = 0; b < nd; ++b) {
+        const int lr = h->leaf_of_pos[op.blk[b].row_lo], lc = h->leaf_of_pos[op.blk[b].col_lo];
+        dl[lr].push_back(b);
+        if (lr == lc) diag[lr] = b;
+    }
+    for (int b = nd; b < nd + nl; ++b) {
+        const HBlk& x = op.blk[b];
+        if (x.rank == 0) continue;
+        for (int l = h->leaf_of_pos[x.row_lo]; l < nleaf && h->leaf_lo[l] < x.row_lo + x.m; ++l)
+            ll[l].push_back({b, h->leaf_lo[l] - x.row_lo});
+    }
+    std::vector<int> dptr{0}, dlst, lptr{0};
+    std::vector<HRef> llst;
+    for (int l = 0; l < nleaf; ++l) {
+        dlst.insert(dlst.end(), dl[l].begin(), dl[l].end());
+        dptr.push_back((int)dlst.size());
+        llst.insert(llst.end(), ll[l].begin(), ll[l].end());
+        lptr.push_back((int)llst.size());
+    }
+    op.L.count = nleaf;
+    op.L.lo = hm_upload(h, h->leaf_lo);
+    op.L.m = hm_upload(h, h->leaf_m);
+    op.L.diag = hm_upload(h, diag);
+    op.L.dptr = hm_upload(h, dptr);
+    op.L.dlist = hm_upload(h, dlst);
+    op.L.lptr = hm_upload(h, lptr);
+    op.L.llist = hm_upload(h, llst);
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
+void hm_build(bem_hmatrix h) {
+    const bool bm = h->prob.formulation == BEM_BURTON_MILLER;
+    hm_plan(h);
+    hm_operator(h, h->A, bm ? HK_BM : HK_CBIE);
+    hm_operator(h, h->B, bm ? HK_BM_RHS : HK_CBIE_RHS);
+    if (h->opt.preconditioner) {
+        h->minv = (z64*)hm_alloc(h, sizeof(z64) * 4096 * (size_t)h->nleaf);
+        int* st = (int*)hm_alloc(h, sizeof(int));
+        launch_fill_int(h->lane->s, st, 1, 0);
+        launch_hjacobi_inv(h->lane->s, h->A.L, h->A.dblk, h->A.dval, h->minv, st);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(h->lane->s));
+    }
+    h->info.assembly_ms = h->A.assembly_ms + h->B.assembly_ms;
+    h->info.aca_ms = h->A.aca_ms + h->B.aca_ms;
+    h->info.max_rank = std::max(h->A.max_rank, h->B.max_rank);
+    h->info.unconverged_blocks = h->A.unconverged + h->B.unconverged;
+    h->info.mean_rank = h->A.mean_rank;
+    h->info.lowrank_bytes = h->A.lowrank_bytes + h->B.lowrank_bytes;
+    h->info.dense_bytes *= 2;
+}
+
+// Y = op X on the device (tree-position order, nr <= kHRhs)
+void hm_apply(bem_hmatrix h, HOp& op, const z64* X, int nr, z64* Y) {
+    launch_happly(h->lane->s, op.L, op.ntiles, op.tiles, op.dblk, op.dval, op.uv, op.toff, op.T, X, nr, Y);
+}
+
+// A M^{-1} X (M^{-1} = block Jacobi or identity); Z is scratch
+void hm_apply_prec(bem_hmatrix h, const z64* X, int nr, z64* Z, z64* Y) {
+    if (h->minv) {
+        launch_hjacobi_apply(h->lane->s, h->A.L, h->minv, X, nr, Z);
+        hm_apply(h, h->A, Z, nr, Y);
+    } else {
+        hm_apply(h, h->A, X, nr, Y);
+    }
+}
+
+__global__ void k_inv_norm(int nr, const z64* sq, double* scale, double* norm) {
+    const int c = threadIdx.x;
+    if (c >= nr) return;
+    const double v = sqrt(fmax(sq[c].x, 0.0));
+    norm[c] = v;
+    scale[c] = v > 0 ? 1.0 / v : 0.0;
+}
+
+// Restarted GMRES (right-preconditioned, CGS2 Arnoldi, complex Givens) for nr
+// right-hand sides B at once, all in FP64.  X receives the solution.
+void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagnostics>& dg) {
+    hipStream_t s = h->lane->s;
+    const int64_t n = h->n;
+    const int m = std::max(1, h->opt.gmres_restart);
+    const double tol = h->opt.gmres_tolerance;
+    const int64_t vs = n * nr;
+    z64* Vb = (z64*)h->ctx->alloc(sizeof(z64) * vs * (m + 1));
+    std::vector<void*> tmp{Vb};
+    auto al = [&](size_t b) { void* p = h->ctx->alloc(b); tmp.push_back(p); return p; };
+    try {
+        z64* U = (z64*)al(sizeof(z64) * vs);
+        z64* W = (z64*)al(sizeof(z64) * vs);
+        z64* Z = (z64*)al(sizeof(z64) * vs);
+        z64* R = (z64*)al(sizeof(z64) * vs);
+        z64* part = (z64*)al(sizeof(z64) * 512 * (m + 1) * nr);
+        z64* h1 = (z64*)al(sizeof(z64) * (m + 1) * nr);
+        z64* h2 = (z64*)al(sizeof(z64) * (m + 1) * nr);
+        z64* sq = (z64*)al(sizeof(z64) * nr);
+        double* scale = (double*)al(sizeof(double) * nr);
+        double* norm = (double*)al(sizeof(double) * nr);
+        z64* ycoef = (z64*)al(sizeof(z64) * m * nr);
+        launch_fill_double(s, (double*)U, 2 * vs, 0.0);
+        // ||b||
+        std::vector<double> bn(nr), rn(nr);
+        launch_gm_dots(s, n, nr, 1, B, vs, B, part, sq);
+        std::vector<z64> hs(nr);
+        HIPCHECK(hipMemcpyAsync(hs.data(), sq, sizeof(z64) * nr, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        for (int c = 0; c < nr; ++c) bn[c] = std::sqrt(std::max(hs[c].x, 0.0));
+        HIPCHECK(hipMemcpyAsync(R, B, sizeof(z64) * vs, hipMemcpyDeviceToDevice, s));
+        int iters = 0;
+        bool done = false;
+        std::vector<std::complex<double>> H((size_t)(m + 1) * m * nr), g((size_t)(m + 1) * nr), sn((size_t)m * nr);
+        std::vector<double> cs((size_t)m * nr);
+        std::vector<z64> hh1((m + 1) * nr), hh2((m + 1) * nr);
+        for (;;) {
+            // residual norms of this cycle's start
+            launch_gm_dots(s, n, nr, 1, R, vs, R, part, sq);
+            hipLaunchKernelGGL(k_inv_norm, dim3(1), dim3(64), 0, s, nr, sq, scale, norm);
+            HIPCHECK(hipMemcpyAsync(rn.data(), norm, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            bool all = true;
+            for (int c = 0; c < nr; ++c) all &= !(rn[c] > tol * bn[c]);
+            if (all || iters >= h->opt.gmres_maxiter) {
+                done = all;
+                break;
+            }
+            launch_gm_scale(s, n, nr, R, scale, Vb);
+            std::fill(g.begin(), g.end(), std::complex<double>(0));
+            for (int c = 0; c < nr; ++c) g[(size_t)c * (m + 1)] = rn[c];
+            int jend = 0;
+            for (int j = 0; j < m; ++j) {
+                z64* Vj = Vb + (int64_t)j * vs;
+                hm_apply_prec(h, Vj, nr, Z, W);
+                // classical Gram-Schmidt, twice
+                launch_gm_dots(s, n, nr, j + 1, Vb, vs, W, part, h1);
+                launch_gm_combine(s, n, nr, j + 1, Vb, vs, h1, -1.0, W);
+                launch_gm_dots(s, n, nr, j + 1, Vb, vs, W, part, h2);
+                launch_gm_combine(s, n, nr, j + 1, Vb, vs, h2, -1.0, W);
+                launch_gm_dots(s, n, nr, 1, W, vs, W, part, sq);
+                hipLaunchKernelGGL(k_inv_norm, dim3(1), dim3(64), 0, s, nr, sq, scale, norm);
+                launch_gm_scale(s, n, nr, W, scale, Vb + (int64_t)(j + 1) * vs);
+                HIPCHECK(hipMemcpyAsync(hh1.data(), h1, sizeof(z64) * (j + 1) * nr, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipMemcpyAsync(hh2.data(), h2, sizeof(z64) * (j + 1) * nr, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipMemcpyAsync(rn.data(), norm, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipStreamSynchronize(s));
+                ++iters;
+                jend = j + 1;
+                bool conv = true;
+                for (int c = 0; c < nr; ++c) {
+                    auto col = [&](int i) -> std::complex<double>& { return H[((size_t)c * (m + 1) + i) * m + j]; };
+                    for (int i = 0; i <= j; ++i)
+                        col(i) = std::complex<double>(hh1[i * nr + c].x + hh2[i * nr + c].x,
+                                                      hh1[i * nr + c].y + hh2[i * nr + c].y);
+                    col(j + 1) = rn[c];
+                    for (int i = 0; i < j; ++i) {  // earlier rotations
+                        const std::complex<double> a = col(i), b = col(i + 1);
+                        col(i) = cs[(size_t)c * m + i] * a + sn[(size_t)c * m + i] * b;
+                        col(i + 1) = -std::conj(sn[(size_t)c * m + i]) * a + cs[(size_t)c * m + i] * b;
+                    }
+                    const std::complex<double> a = col(j), b = col(j + 1);
+                    const double na = std::abs(a), rho = std::hypot(na, std::abs(b));
+                    double cc;
+                    std::complex<double> ss;
+                    if (rho == 0) {
+                        cc = 1;
+                        ss = 0;
+                    } else if (na == 0) {
+                        cc = 0;
+                        ss = std::conj(b) / std::abs(b);
+                        col(j) = std::abs(b);
+                    } else {
+                        cc = na / rho;
+                        ss = (a / na) * std::conj(b) / rho;
+                        col(j) = (a / na) * rho;
+                    }
+                    col(j + 1) = 0;
+                    cs[(size_t)c * m + j] = cc;
+                    sn[(size_t)c * m + j] = ss;
+                    std::complex<double>& gj = g[(size_t)c * (m + 1) + j];
+                    g[(size_t)c * (m + 1) + j + 1] = -std::conj(ss) * gj;
+                    gj = cc * gj;
+                    conv &= !(std::abs(g[(size_t)c * (m + 1) + j + 1]) > tol * bn[c]);
+                }
+                if (conv || iters >= h->opt.gmres_maxiter) break;
+            }
+            // y = H^{-1} g per column; U += V y; X = M^{-1} U; R = B - A X
+            std::vector<z64> y((size_t)jend * nr);
+            for (int c = 0; c < nr; ++c) {
+                std::vector<std::complex<double>> yc(jend);
+                for (int i = jend - 1; i >= 0; --i) {
+                    std::complex<double> v = g[(size_t)c * (m + 1) + i];
+                    for (int q = i + 1; q < jend; ++q) v -= H[((size_t)c * (m + 1) + i) * m + q] * yc[q];
+                    const std::complex<double> d = H[((size_t)c * (m + 1) + i) * m + i];
+                    yc[i] = d == 0.0 ? 0.0 : v / d;
+                }
+                for (int i = 0; i < jend; ++i) y[(size_t)i * nr + c] = z64{yc[i].real(), yc[i].imag()};
+            }
+            HIPCHECK(hipMemcpyAsync(ycoef, y.data(), sizeof(z64) * y.size(), hipMemcpyHostToDevice, s));
+            launch_gm_combine(s, n, nr, jend, Vb, vs, ycoef, 1.0, U);
+            if (h->minv) launch_hjacobi_apply(s, h->A.L, h->minv, U, nr, X);
+            else HIPCHECK(hipMemcpyAsync(X, U, sizeof(z64) * vs, hipMemcpyDeviceToDevice, s));
+            hm_apply(h, h->A, X, nr, W);
+            launch_gm_sub(s, vs, B, W, R);
+            HIPCHECK(hipGetLastError());
+        }
+        for (int c = 0; c < nr; ++c) {
+            bem_diagnostics& d = dg[c];
+            std::memset(&d, 0, sizeof d);
+            d.unknown_count = n;
+            d.quadrature_nodes = n;
+            d.absolute_residual = rn[c];
+            d.relative_residual = bn[c] > 0 ? rn[c] / bn[c] : rn[c];
+            d.iterations = iters;
+            d.method = 2;
+            d.factor_precision = 1;
+            d.converged = !(rn[c] > tol * bn[c]) ? 1 : 0;
+        }
+        (void)done;
+    } catch (...) {
+        hipStreamSynchronize(s);
+        for (void* p : tmp) h->ctx->release(p);
+        throw;
+    }
+    hipStreamSynchronize(s);
+    for (void* p : tmp) h->ctx->release(p);
 }
 
 }  // namespace
@@ -1896,6 +2399,216 @@ int bem_server_stats_get(bem_server srv, bem_server_stats* out) {
     for (auto& e : srv->resident) b += e.bytes;
     out->resident_bytes = b;
     return BEM_OK;
+}
+
+void bem_hmatrix_defaults(bem_hmatrix_options* o) {
+    if (!o) return;
+    o->leaf_size = 64;
+    o->eta = 2.0;
+    o->aca_tolerance = 1e-6;
+    o->max_rank = 256;
+    o->gmres_restart = 50;
+    o->gmres_maxiter = 1000;
+    o->gmres_tolerance = 1e-8;
+    o->preconditioner = 1;
+}
+
+int bem_hmatrix_plan(int64_t n, const double* points, int32_t leaf, double eta, double near_radius, int64_t* perm,
+                     int64_t* n_nodes, int64_t* n_dense, int64_t* n_lowrank, int64_t* blocks, int64_t cap) {
+    if (n <= 0 || !points || leaf < 1 || !(eta > 0) || !(near_radius >= 0)) return BEM_ERR_ARGUMENT;
+    try {
+        std::vector<double> aos(3 * (size_t)n);
+        for (int64_t i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) aos[3 * i + c] = points[3 * i + c];
+        HPlan P;
+        hmatrix_plan(aos.data(), n, leaf, eta, near_radius, P);
+        if (perm) std::copy(P.perm.begin(), P.perm.end(), perm);
+        if (n_nodes) *n_nodes = (int64_t)P.nodes.size();
+        if (n_dense) *n_dense = (int64_t)P.dense.size();
+        if (n_lowrank) *n_lowrank = (int64_t)P.lowrank.size();
+        const int64_t tot = (int64_t)(P.dense.size() + P.lowrank.size());
+        if (blocks && cap >= 5 * tot) {
+            int64_t e = 0;
+            auto put = [&](int lr, const HBlock& b) {
+                const HNode &r = P.nodes[b.row], &c = P.nodes[b.col];
+                blocks[e++] = lr;
+                blocks[e++] = r.lo;
+                blocks[e++] = r.hi - r.lo;
+                blocks[e++] = c.lo;
+                blocks[e++] = c.hi - c.lo;
+            };
+            for (auto& b : P.dense) put(0, b);
+            for (auto& b : P.lowrank) put(1, b);
+        }
+    } catch (...) {
+        return BEM_ERR_DEVICE;
+    }
+    return BEM_OK;
+}
+
+int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, const bem_hmatrix_options* options,
+                       bem_hmatrix* out) {
+    if (!ctx || !mesh || !problem || !out) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        validate_problem(*problem);
+        if (problem->boundary != BEM_RIGID)
+            throw BemError(BEM_ERR_ARGUMENT, "the hierarchical path solves rigid surfaces");
+        if (problem->formulation != BEM_BURTON_MILLER && problem->formulation != BEM_CBIE)
+            throw BemError(BEM_ERR_ARGUMENT, "formulation must be :burton_miller or :cbie");
+        bem_hmatrix_options o;
+        bem_hmatrix_defaults(&o);
+        if (options) o = *options;
+        if (o.leaf_size < 1 || o.leaf_size > 64) throw BemError(BEM_ERR_ARGUMENT, "leaf_size must be in 1..64");
+        if (!(o.eta > 0)) throw BemError(BEM_ERR_ARGUMENT, "eta must be positive");
+        if (!(o.aca_tolerance > 0 && o.aca_tolerance < 1)) throw BemError(BEM_ERR_ARGUMENT, "aca_tolerance must be in (0, 1)");
+        if (o.max_rank < 1 || o.max_rank > kAcaCap) throw BemError(BEM_ERR_ARGUMENT, "max_rank must be in 1..256");
+        if (o.gmres_restart < 1 || o.gmres_maxiter < 1 || !(o.gmres_tolerance > 0))
+            throw BemError(BEM_ERR_ARGUMENT, "invalid GMRES options");
+        auto* h = new bem_hmatrix_s();
+        h->ctx = ctx;
+        h->mesh = mesh;
+        mesh->refs++;
+        h->prob = *problem;
+        h->opt = o;
+        try {
+            h->lane = make_lane(ctx);
+            hm_build(h);
+        } catch (...) {
+            hm_free(h);
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int bem_hmatrix_destroy(bem_hmatrix h) {
+    if (!h) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(h->ctx->mu);
+    hm_free(h);
+    delete h;
+    return BEM_OK;
+}
+
+int bem_hmatrix_get_info(bem_hmatrix h, bem_hmatrix_info* out) {
+    if (!h || !out) return BEM_ERR_ARGUMENT;
+    *out = h->info;
+    return BEM_OK;
+}
+
+int bem_hmatrix_matvec(bem_hmatrix h, int32_t nrhs, const double* x, double* y) {
+    if (!h || nrhs <= 0 || !x || !y) return BEM_ERR_ARGUMENT;
+    bem_ctx ctx = h->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        const int64_t n = h->n;
+        hipStream_t s = h->lane->s;
+        z64* X = (z64*)ctx->alloc(sizeof(z64) * n * kHRhs);
+        z64* Y = nullptr;
+        try {
+            Y = (z64*)ctx->alloc(sizeof(z64) * n * kHRhs);
+            std::vector<z64> hx((size_t)n * kHRhs), hy((size_t)n * kHRhs);
+            for (int c0 = 0; c0 < nrhs; c0 += kHRhs) {
+                const int nr = std::min(kHRhs, nrhs - c0);
+                for (int64_t pos = 0; pos < n; ++pos)
+                    for (int c = 0; c < nr; ++c) {
+                        const double* src = x + 2 * ((size_t)(c0 + c) * n + h->hperm[pos]);
+                        hx[pos * nr + c] = z64{src[0], src[1]};
+                    }
+                HIPCHECK(hipMemcpyAsync(X, hx.data(), sizeof(z64) * n * nr, hipMemcpyHostToDevice, s));
+                hm_apply(h, h->A, X, nr, Y);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipMemcpyAsync(hy.data(), Y, sizeof(z64) * n * nr, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
+                for (int64_t pos = 0; pos < n; ++pos)
+                    for (int c = 0; c < nr; ++c) {
+                        double* dst = y + 2 * ((size_t)(c0 + c) * n + h->hperm[pos]);
+                        dst[0] = hy[pos * nr + c].x;
+                        dst[1] = hy[pos * nr + c].y;
+                    }
+            }
+        } catch (...) {
+            hipStreamSynchronize(s);
+            ctx->release(X);
+            ctx->release(Y);
+            throw;
+        }
+        ctx->release(X);
+        ctx->release(Y);
+    });
+}
+
+int bem_hmatrix_solve(bem_hmatrix h, int32_t nrhs, const double* angles, bem_solution* out) {
+    if (!h || nrhs <= 0 || !angles || !out) return BEM_ERR_ARGUMENT;
+    bem_ctx ctx = h->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        for (int r = 0; r < 2 * nrhs; ++r)
+            if (!std::isfinite(angles[r])) throw BemError(BEM_ERR_ARGUMENT, "incidence angles must be finite");
+        const int64_t n = h->n;
+        const int nt = (int)n;
+        hipStream_t s = h->lane->s;
+        auto* sol = new bem_solution_s();
+        sol->ctx = ctx;
+        sol->mesh = h->mesh;
+        h->mesh->refs++;
+        sol->k = h->prob.k;
+        sol->nrhs = nrhs;
+        sol->nt = nt;
+        sol->angles.assign(angles, angles + 2 * nrhs);
+        sol->diag.resize(nrhs);
+        std::vector<void*> tmp;
+        try {
+            const double t0 = now_ms();
+            sol->p = (z64*)ctx->alloc(sizeof(z64) * (size_t)nt * nrhs);
+            sol->q = (z64*)ctx->alloc(sizeof(z64) * (size_t)nt * nrhs);
+            std::vector<double> dirs(3 * nrhs);
+            for (int r = 0; r < nrhs; ++r) direction(angles[2 * r], angles[2 * r + 1], &dirs[3 * r]);
+            double* dd = (double*)ctx->alloc(sizeof(double) * 3 * nrhs);
+            tmp.push_back(dd);
+            HIPCHECK(hipMemcpyAsync(dd, dirs.data(), sizeof(double) * 3 * nrhs, hipMemcpyHostToDevice, s));
+            z64* B = (z64*)ctx->alloc(sizeof(z64) * n * kHRhs);
+            tmp.push_back(B);
+            z64* X = (z64*)ctx->alloc(sizeof(z64) * n * kHRhs);
+            tmp.push_back(X);
+            std::vector<bem_diagnostics> dg;
+            for (int c0 = 0; c0 < nrhs; c0 += kHRhs) {
+                const int nr = std::min(kHRhs, nrhs - c0);
+                // b = B q_inc (the right-hand side of the scattered-field equation)
+                launch_hqinc(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, X);
+                hm_apply(h, h->B, X, nr, B);
+                dg.assign(nr, bem_diagnostics{});
+                hm_gmres(h, nr, B, X, dg);
+                // traces of this chunk: columns c0 .. c0 + nr of P, Q
+                launch_htraces(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, X, sol->p + (size_t)c0 * nt,
+                               sol->q + (size_t)c0 * nt);
+                HIPCHECK(hipGetLastError());
+                for (int c = 0; c < nr; ++c) sol->diag[c0 + c] = dg[c];
+            }
+            sol->hp.resize((size_t)nt * nrhs);
+            sol->hq.resize((size_t)nt * nrhs);
+            HIPCHECK(hipMemcpyAsync(sol->hp.data(), sol->p, sizeof(z64) * nt * nrhs, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipMemcpyAsync(sol->hq.data(), sol->q, sizeof(z64) * nt * nrhs, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            const double ms = now_ms() - t0;
+            for (auto& d : sol->diag) {
+                d.assembly_ms = h->info.assembly_ms;
+                d.factor_ms = h->info.aca_ms;
+                d.solve_ms = ms;
+            }
+        } catch (...) {
+            hipStreamSynchronize(s);
+            for (void* p : tmp) ctx->release(p);
+            ctx->release(sol->p);
+            ctx->release(sol->q);
+            mesh_release(sol->mesh);
+            delete sol;
+            throw;
+        }
+        for (void* p : tmp) ctx->release(p);
+        *out = sol;
+    });
 }
 
 int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t single, double* LUout, int64_t* ipiv_out,

@@ -429,4 +429,320 @@ void launch_far_field(hipStream_t s, const DevMesh& m, double k, int nrhs, int n
     hipLaunchKernelGGL(k_far_final, dim3((nrhs * ndir + 63) / 64), dim3(64), 0, s, nrhs * ndir, nblk, partial, out);
 }
 
+// ---------------------------------------------------------------------------
+// Hierarchical path: entries of single-medium rigid systems by tree position.
+// Regular pairs in FP32 (7-point rule); near pairs and self panels in FP64
+// into the dense-block slots the host resolved.
+// ---------------------------------------------------------------------------
+template <int KIND>
+struct HNeed {
+    static constexpr int v = KIND == HK_BM ? 10 : KIND == HK_CBIE ? 2 : KIND == HK_BM_RHS ? 5 : 1;
+};
+// off-diagonal entry from the operator values (diag adds the 1/2 I terms)
+template <int KIND, typename R>
+__device__ __forceinline__ z32 hcombine(const Acc4<R>& e, R ik, bool diag) {
+    const R h = diag ? R(0.5) : R(0);
+    if (KIND == HK_BM) return z32{float(h - e.D.x + e.H.y * ik), float(-e.D.y - e.H.x * ik)};
+    if (KIND == HK_CBIE) return z32{float(h - e.D.x), float(-e.D.y)};
+    if (KIND == HK_BM_RHS) return z32{float(e.S.x - e.K.y * ik), float(e.S.y + (e.K.x + h) * ik)};
+    return z32{float(e.S.x), float(e.S.y)};
+}
+
+template <int KIND>
+__device__ __forceinline__ z32 hentry(const MeshView<float>& v, int nt, int i, int j, float k, float ik) {
+    const float x0 = v.cen[i], x1 = v.cen[nt + i], x2 = v.cen[2 * nt + i];
+    const float nx0 = v.nrm[i], nx1 = v.nrm[nt + i], nx2 = v.nrm[2 * nt + i];
+    const float ny0 = v.nrm[j], ny1 = v.nrm[nt + j], ny2 = v.nrm[2 * nt + j];
+    const float c = nx0 * ny0 + nx1 * ny1 + nx2 * ny2;
+    const float aj = v.area[j];
+    Acc4<float> e{};
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const float rx = x0 - v.gp[(size_t)(3 * q) * nt + j];
+        const float ry = x1 - v.gp[(size_t)(3 * q + 1) * nt + j];
+        const float rz = x2 - v.gp[(size_t)(3 * q + 2) * nt + j];
+        kpoint<float, HNeed<KIND>::v>(k, rx, ry, rz, nx0, nx1, nx2, ny0, ny1, ny2, c, float(c_q7[q][3]) * aj, e);
+    }
+    return hcombine<KIND>(e, ik, false);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_hdense(DevMesh mesh, float k, const int* __restrict__ perm,
+                                                const HBlk* __restrict__ blk, z32* __restrict__ val) {
+    const HBlk b = blk[blockIdx.x];
+    const MeshView<float> v = view_of(mesh, 0.f);
+    const float ik = 1.f / k;
+    for (int e = threadIdx.x; e < b.m * b.n; e += 256) {
+        const int r = e / b.n, c = e - r * b.n;
+        const int i = perm[b.row_lo + r], j = perm[b.col_lo + c];
+        val[b.off + e] = i == j ? z32{0.f, 0.f} : hentry<KIND>(v, mesh.nt, i, j, k, ik);
+    }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_hnear(DevMesh m, double k, int64_t npairs, const int* __restrict__ pi,
+                                               const int* __restrict__ pj, const int64_t* __restrict__ dst,
+                                               z32* __restrict__ val) {
+    const int lane = threadIdx.x & 63;
+    const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= npairs) return;
+    const int nt = m.nt;
+    const int i = pi[e], j = pj[e];
+    const double x0 = m.cen[i], x1 = m.cen[nt + i], x2 = m.cen[2 * nt + i];
+    const double nx0 = m.nrm[i], nx1 = m.nrm[nt + i], nx2 = m.nrm[2 * nt + i];
+    const double ny0 = m.nrm[j], ny1 = m.nrm[nt + j], ny2 = m.nrm[2 * nt + j];
+    const double c = nx0 * ny0 + nx1 * ny1 + nx2 * ny2;
+    const double* V = m.vert + 9 * (size_t)j;
+    const double aj = m.area[j];
+    Acc4<double> a{};
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const int pt = lane * 7 + q;
+        const double l0 = c_near[pt][0], l1 = c_near[pt][1], l2 = c_near[pt][2];
+        const double w = c_near[pt][3] * aj;
+        const double y0 = l0 * V[0] + l1 * V[3] + l2 * V[6];
+        const double y1 = l0 * V[1] + l1 * V[4] + l2 * V[7];
+        const double y2 = l0 * V[2] + l1 * V[5] + l2 * V[8];
+        kpoint<double, HNeed<KIND>::v>(k, x0 - y0, x1 - y1, x2 - y2, nx0, nx1, nx2, ny0, ny1, ny2, c, w, a);
+    }
+    wave_sum_acc(a);
+    if (lane == 0) val[dst[e]] = hcombine<KIND>(a, 1.0 / k, false);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(128) void k_hself(DevMesh m, double k, const int64_t* __restrict__ dst,
+                                               z32* __restrict__ val) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = m.nt;
+    if (i >= nt) return;
+    // flat panel: D_ii = K'_ii = 0; S_ii and the finite part H_ii in closed form
+    Acc4<double> a{};
+    if (KIND != HK_CBIE) {
+        double V[9], x[3];
+        for (int c = 0; c < 9; ++c) V[c] = m.vert[9 * (size_t)i + c];
+        x[0] = m.cen[i];
+        x[1] = m.cen[nt + i];
+        x[2] = m.cen[2 * nt + i];
+        self_terms(V, x, k, a.S, a.H);
+    }
+    val[dst[i]] = hcombine<KIND>(a, 1.0 / k, true);
+}
+
+template <int KIND>
+static void launch_hdense_k(hipStream_t s, const DevMesh& m, double k, const int* perm, int nblk, const HBlk* blk,
+                            z32* val) {
+    hipLaunchKernelGGL((k_hdense<KIND>), dim3(nblk), dim3(256), 0, s, m, (float)k, perm, blk, val);
+}
+void launch_hdense(hipStream_t s, const DevMesh& m, int kind, double k, const int* perm, int nblk, const HBlk* blk,
+                   z32* val) {
+    if (nblk <= 0) return;
+    switch (kind) {
+    case HK_BM: launch_hdense_k<HK_BM>(s, m, k, perm, nblk, blk, val); break;
+    case HK_CBIE: launch_hdense_k<HK_CBIE>(s, m, k, perm, nblk, blk, val); break;
+    case HK_BM_RHS: launch_hdense_k<HK_BM_RHS>(s, m, k, perm, nblk, blk, val); break;
+    default: launch_hdense_k<HK_CBIE_RHS>(s, m, k, perm, nblk, blk, val); break;
+    }
+}
+
+template <int KIND>
+static void launch_hnear_self_k(hipStream_t s, const DevMesh& m, double k, int64_t npairs, const int* pi,
+                                const int* pj, const int64_t* dst, const int64_t* self_dst, z32* val) {
+    if (npairs > 0)
+        hipLaunchKernelGGL((k_hnear<KIND>), dim3((unsigned)((npairs + 3) / 4)), dim3(256), 0, s, m, k, npairs, pi, pj,
+                           dst, val);
+    hipLaunchKernelGGL((k_hself<KIND>), dim3((m.nt + 127) / 128), dim3(128), 0, s, m, k, self_dst, val);
+}
+void launch_hnear_self(hipStream_t s, const DevMesh& m, int kind, double k, int64_t npairs, const int* pi,
+                       const int* pj, const int64_t* dst, const int64_t* self_dst, z32* val) {
+    switch (kind) {
+    case HK_BM: launch_hnear_self_k<HK_BM>(s, m, k, npairs, pi, pj, dst, self_dst, val); break;
+    case HK_CBIE: launch_hnear_self_k<HK_CBIE>(s, m, k, npairs, pi, pj, dst, self_dst, val); break;
+    case HK_BM_RHS: launch_hnear_self_k<HK_BM_RHS>(s, m, k, npairs, pi, pj, dst, self_dst, val); break;
+    default: launch_hnear_self_k<HK_CBIE_RHS>(s, m, k, npairs, pi, pj, dst, self_dst, val); break;
+    }
+}
+
+// Block-wide reductions of the ACA kernel (256 threads = 4 waves).
+// argmax of v >= 0 with ties to the smaller index; result in every thread.
+__device__ inline void aca_argmax(float& v, int& idx, float* sv, int* si) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(idx, off, 64);
+        if (ov > v || (ov == v && oi < idx)) {
+            v = ov;
+            idx = oi;
+        }
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        sv[threadIdx.x >> 6] = v;
+        si[threadIdx.x >> 6] = idx;
+    }
+    __syncthreads();
+    v = sv[0];
+    idx = si[0];
+#pragma unroll
+    for (int w = 1; w < 4; ++w)
+        if (sv[w] > v || (sv[w] == v && si[w] < idx)) {
+            v = sv[w];
+            idx = si[w];
+        }
+}
+// sums of NV doubles per thread, fixed order; results in every thread
+template <int NV>
+__device__ inline void aca_sums(double (&x)[NV], double* sd) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) x[q] += __shfl_xor(x[q], off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) sd[(threadIdx.x >> 6) * NV + q] = x[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) x[q] = (sd[q] + sd[NV + q]) + (sd[2 * NV + q] + sd[3 * NV + q]);
+}
+
+// Partial-pivot adaptive cross approximation (Bebendorf, Numer. Math. 86
+// (2000)) of one admissible block per workgroup, A ~ sum_l u_l v_l^T:
+// residual row at the pivot row -> pivot column = its largest entry, v = row /
+// pivot; residual column -> u; next pivot row = the largest |u| off the used
+// row.  Stops when |u| |v| <= tol ||S||_F, with the Frobenius norm of the
+// approximation updated exactly (cross terms included), or at the block's
+// rank capacity (conv = 0).
+template <int KIND>
+__global__ __launch_bounds__(256) void k_aca(DevMesh mesh, float k, const int* __restrict__ perm, const int* list,
+                                             HBlk* __restrict__ blks, z32* __restrict__ uv, int* __restrict__ conv,
+                                             float tol) {
+    const int bid = list ? list[blockIdx.x] : blockIdx.x;
+    const HBlk b = blks[bid];
+    const MeshView<float> v = view_of(mesh, 0.f);
+    const int nt = mesh.nt, tid = threadIdx.x;
+    const float ik = 1.f / k;
+    z32* U = uv + b.off;
+    z32* V = uv + b.off + (int64_t)b.rmax * b.m;
+    __shared__ z32 coef[kAcaCap];
+    __shared__ float sv[4];
+    __shared__ int si[4];
+    __shared__ double sd[4 * 32];
+    double normS2 = 0;
+    int r = 0, istar = 0, retries = 0, ok = 0;
+    while (r < b.rmax) {
+        // residual row istar -> V[:, r]
+        if (tid < r) coef[tid] = U[(int64_t)tid * b.m + istar];
+        __syncthreads();
+        const int gi = perm[b.row_lo + istar];
+        float bv = -1.f;
+        int bj = 0;
+        for (int j = tid; j < b.n; j += 256) {
+            z32 a = hentry<KIND>(v, nt, gi, perm[b.col_lo + j], k, ik);
+            for (int l = 0; l < r; ++l) a = csub(a, cmul(coef[l], V[(int64_t)l * b.n + j]));
+            V[(int64_t)r * b.n + j] = a;
+            const float m2 = a.x * a.x + a.y * a.y;
+            if (m2 > bv) {
+                bv = m2;
+                bj = j;
+            }
+        }
+        aca_argmax(bv, bj, sv, si);
+        if (!(bv > 0.f)) {
+            // the residual vanishes on this row: try a few other rows first
+            if (r == 0 && ++retries < 8 && retries < b.m) {
+                istar = retries;
+                continue;
+            }
+            ok = 1;
+            break;
+        }
+        const z32 pivot = V[(int64_t)r * b.n + bj];
+        const z32 pinv = cinv(pivot);
+        // residual column bj -> U[:, r]
+        if (tid < r) coef[tid] = V[(int64_t)tid * b.n + bj];
+        __syncthreads();
+        const int gj = perm[b.col_lo + bj];
+        float bu = -1.f;
+        int bi = 0;
+        double acc[2] = {0.0, 0.0};  // |u|^2, |v|^2
+        for (int i = tid; i < b.m; i += 256) {
+            z32 a = hentry<KIND>(v, nt, perm[b.row_lo + i], gj, k, ik);
+            for (int l = 0; l < r; ++l) a = csub(a, cmul(U[(int64_t)l * b.m + i], coef[l]));
+            U[(int64_t)r * b.m + i] = a;
+            const float m2 = a.x * a.x + a.y * a.y;
+            acc[0] += (double)m2;
+            if (i != istar && m2 > bu) {
+                bu = m2;
+                bi = i;
+            }
+        }
+        for (int j = tid; j < b.n; j += 256) {
+            const z32 a = cmul(V[(int64_t)r * b.n + j], pinv);
+            V[(int64_t)r * b.n + j] = a;
+            acc[1] += (double)a.x * a.x + (double)a.y * a.y;
+        }
+        __syncthreads();
+        aca_sums<2>(acc, sd);
+        // cross terms 2 Re sum_{l<r} (u_l^H u_r)(v_l^H v_r), eight l at a time
+        double cross = 0;
+        for (int l0 = 0; l0 < r; l0 += 8) {
+            double d[32];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) d[q] = 0;
+            for (int i = tid; i < b.m; i += 256) {
+                const z32 ur = U[(int64_t)r * b.m + i];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (l0 + q < r) {
+                        const z32 ul = U[(int64_t)(l0 + q) * b.m + i];
+                        d[4 * q] += (double)ul.x * ur.x + (double)ul.y * ur.y;
+                        d[4 * q + 1] += (double)ul.x * ur.y - (double)ul.y * ur.x;
+                    }
+            }
+            for (int j = tid; j < b.n; j += 256) {
+                const z32 vr = V[(int64_t)r * b.n + j];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (l0 + q < r) {
+                        const z32 vl = V[(int64_t)(l0 + q) * b.n + j];
+                        d[4 * q + 2] += (double)vl.x * vr.x + (double)vl.y * vr.y;
+                        d[4 * q + 3] += (double)vl.x * vr.y - (double)vl.y * vr.x;
+                    }
+            }
+            aca_sums<32>(d, sd);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) cross += d[4 * q] * d[4 * q + 2] - d[4 * q + 1] * d[4 * q + 3];
+        }
+        normS2 += acc[0] * acc[1] + 2.0 * cross;
+        r += 1;
+        if (acc[0] * acc[1] <= (double)tol * tol * normS2) {
+            ok = 1;
+            break;
+        }
+        aca_argmax(bu, bi, sv, si);
+        if (!(bu > 0.f)) {
+            ok = 1;
+            break;
+        }
+        istar = bi;
+    }
+    if (tid == 0) {
+        blks[bid].rank = r;
+        conv[bid] = ok;
+    }
+}
+
+void launch_aca(hipStream_t s, const DevMesh& m, int kind, double k, const int* perm, int nlist, const int* list,
+                HBlk* blk, z32* uv, int* conv, double tol) {
+    if (nlist <= 0) return;
+#define BEM_ACA(K) hipLaunchKernelGGL((k_aca<K>), dim3(nlist), dim3(256), 0, s, m, (float)k, perm, list, blk, uv, conv, (float)tol)
+    switch (kind) {
+    case HK_BM: BEM_ACA(HK_BM); break;
+    case HK_CBIE: BEM_ACA(HK_CBIE); break;
+    case HK_BM_RHS: BEM_ACA(HK_BM_RHS); break;
+    default: BEM_ACA(HK_CBIE_RHS); break;
+    }
+#undef BEM_ACA
+}
+
 }  // namespace bem

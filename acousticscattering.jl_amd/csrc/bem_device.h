@@ -83,6 +83,34 @@ constexpr int kFarBlocks = 256;
 void launch_far_field(hipStream_t s, const DevMesh& m, double k, int nrhs, int ndir,
                       const double* dirs, const z64* p, const z64* q, z64* partial, z64* out);
 
+// ---- hierarchical matrices (bem_assembly.hip: entries and ACA; bem_hsolve.hip: apply, GMRES) ----
+// A block of the compressed operator, in cluster-tree positions.  Dense: m x n
+// row-major complex FP32 values at `off`.  Low-rank A ~ U V^T: U (m x rmax) and
+// then V (n x rmax), both column-major, at `off`.
+struct HBlk {
+    int row_lo, m, col_lo, n;
+    int rmax, rank;
+    int64_t off;
+};
+// Operators of the hierarchical path (single medium, rigid surface): the
+// system matrix and the right-hand-side operator of the scattered-field
+// equations (reference src/solvers/bem/full.jl:103-111,144-148).
+enum HKind : int {
+    HK_BM = 0,        // 1/2 I - D - (i/k) H
+    HK_CBIE = 1,      // 1/2 I - D
+    HK_BM_RHS = 2,    // S + (i/k)(1/2 I + K')
+    HK_CBIE_RHS = 3,  // S
+};
+constexpr int kAcaCap = 256;  // largest rank of one ACA block
+void launch_hdense(hipStream_t s, const DevMesh& m, int kind, double k, const int* perm, int nblk, const HBlk* blk,
+                   z32* val);
+// near pairs (original panel indices pi, pj) and self panels into dense-block slots
+void launch_hnear_self(hipStream_t s, const DevMesh& m, int kind, double k, int64_t npairs, const int* pi,
+                       const int* pj, const int64_t* dst, const int64_t* self_dst, z32* val);
+// partial-pivot ACA of the listed low-rank blocks (all when list is null)
+void launch_aca(hipStream_t s, const DevMesh& m, int kind, double k, const int* perm, int nlist, const int* list,
+                HBlk* blk, z32* uv, int* conv, double tol);
+
 // ---- bem_linalg.hip ----
 template <typename R>
 void launch_row_stats(hipStream_t s, const C2<R>* A, int64_t lda, int n, double* rows, double* rowsum);

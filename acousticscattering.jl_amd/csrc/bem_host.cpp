@@ -390,3 +390,131 @@ void build_near_list(HostMesh& m) {
 }
 
 }  // namespace bem
+
+// ---------------------------------------------------------------------------
+// Hierarchical-matrix plan (cluster tree + block tree); rules in bem_internal.h.
+// ---------------------------------------------------------------------------
+namespace bem {
+
+namespace {
+
+void tight_box(const double* pts, const std::vector<int64_t>& perm, int64_t lo, int64_t hi, HNode& nd) {
+    for (int c = 0; c < 3; ++c) {
+        nd.bmin[c] = INFINITY;
+        nd.bmax[c] = -INFINITY;
+    }
+    for (int64_t q = lo; q < hi; ++q) {
+        const double* x = pts + 3 * perm[q];
+        for (int c = 0; c < 3; ++c) {
+            nd.bmin[c] = std::min(nd.bmin[c], x[c]);
+            nd.bmax[c] = std::max(nd.bmax[c], x[c]);
+        }
+    }
+}
+
+double box_diam(const HNode& a) {
+    double s = 0;
+    for (int c = 0; c < 3; ++c) s += (a.bmax[c] - a.bmin[c]) * (a.bmax[c] - a.bmin[c]);
+    return std::sqrt(s);
+}
+
+double box_dist(const HNode& a, const HNode& b) {
+    double s = 0;
+    for (int c = 0; c < 3; ++c) {
+        const double g = std::max({0.0, b.bmin[c] - a.bmax[c], a.bmin[c] - b.bmax[c]});
+        s += g * g;
+    }
+    return std::sqrt(s);
+}
+
+}  // namespace
+
+void hmatrix_plan(const double* pts, int64_t n, int leaf, double eta, double near_radius, HPlan& P) {
+    P.n = n;
+    P.leaf = leaf;
+    P.eta = eta;
+    P.near_radius = near_radius;
+    P.perm.resize(n);
+    for (int64_t i = 0; i < n; ++i) P.perm[i] = i;
+    P.nodes.clear();
+    P.dense.clear();
+    P.lowrank.clear();
+    HNode root;
+    root.lo = 0;
+    root.hi = n;
+    tight_box(pts, P.perm, 0, n, root);
+    P.nodes.push_back(root);
+    std::vector<int32_t> stack{0};
+    std::vector<int64_t> tmp;
+    while (!stack.empty()) {
+        const int32_t id = stack.back();
+        stack.pop_back();
+        const int64_t lo = P.nodes[id].lo, hi = P.nodes[id].hi;
+        if (hi - lo <= leaf) continue;
+        int axis = 0;
+        double ext = -1;
+        for (int c = 0; c < 3; ++c) {
+            const double e = P.nodes[id].bmax[c] - P.nodes[id].bmin[c];
+            if (e > ext) {
+                ext = e;
+                axis = c;
+            }
+        }
+        const double mid = 0.5 * (P.nodes[id].bmin[axis] + P.nodes[id].bmax[axis]);
+        tmp.clear();
+        for (int64_t q = lo; q < hi; ++q)
+            if (pts[3 * P.perm[q] + axis] < mid) tmp.push_back(P.perm[q]);
+        int64_t split = lo + (int64_t)tmp.size();
+        for (int64_t q = lo; q < hi; ++q)
+            if (!(pts[3 * P.perm[q] + axis] < mid)) tmp.push_back(P.perm[q]);
+        std::copy(tmp.begin(), tmp.end(), P.perm.begin() + lo);
+        if (split == lo || split == hi) split = lo + (hi - lo) / 2;
+        HNode L, R;
+        L.lo = lo;
+        L.hi = split;
+        R.lo = split;
+        R.hi = hi;
+        tight_box(pts, P.perm, L.lo, L.hi, L);
+        tight_box(pts, P.perm, R.lo, R.hi, R);
+        const int32_t li = (int32_t)P.nodes.size();
+        P.nodes.push_back(L);
+        P.nodes.push_back(R);
+        P.nodes[id].child[0] = li;
+        P.nodes[id].child[1] = li + 1;
+        // depth first, left before right
+        stack.push_back(li + 1);
+        stack.push_back(li);
+    }
+    P.iperm.resize(n);
+    for (int64_t q = 0; q < n; ++q) P.iperm[P.perm[q]] = q;
+    // block tree
+    std::vector<std::pair<int32_t, int32_t>> bs{{0, 0}};
+    while (!bs.empty()) {
+        const auto pr = bs.back();
+        bs.pop_back();
+        const HNode& a = P.nodes[pr.first];
+        const HNode& b = P.nodes[pr.second];
+        const double dist = box_dist(a, b);
+        if (dist > near_radius && std::min(box_diam(a), box_diam(b)) < eta * dist) {
+            P.lowrank.push_back({pr.first, pr.second});
+            continue;
+        }
+        const bool la = a.child[0] < 0, lb = b.child[0] < 0;
+        if (la && lb) {
+            P.dense.push_back({pr.first, pr.second});
+        } else if (la) {
+            bs.push_back({pr.first, b.child[1]});
+            bs.push_back({pr.first, b.child[0]});
+        } else if (lb) {
+            bs.push_back({a.child[1], pr.second});
+            bs.push_back({a.child[0], pr.second});
+        } else {
+            bs.push_back({a.child[1], b.child[1]});
+            bs.push_back({a.child[1], b.child[0]});
+            bs.push_back({a.child[0], b.child[1]});
+            bs.push_back({a.child[0], b.child[0]});
+        }
+    }
+}
+
+}  // namespace bem

@@ -1,0 +1,52 @@
+// Hierarchical-matrix apply and batched GMRES: device structures and launchers.
+#pragma once
+#include "bem_device.h"
+
+namespace bem {
+
+constexpr int kHRhs = 64;  // right-hand sides per batched apply (stride of T)
+
+// Tile of the low-rank projection: ranks [l0, l0 + 16) of block `blk`.
+struct HTile {
+    int blk, l0;
+};
+// A low-rank block covering a leaf row cluster, at row offset o in the block.
+struct HRef {
+    int blk, o;
+};
+// Leaf row clusters and, per leaf, its dense blocks and the low-rank blocks
+// covering it (CSR), plus its diagonal dense block.
+struct HLeafs {
+    int count = 0;
+    const int* lo = nullptr;
+    const int* m = nullptr;
+    const int* diag = nullptr;
+    const int* dptr = nullptr;
+    const int* dlist = nullptr;
+    const int* lptr = nullptr;
+    const HRef* llist = nullptr;
+};
+
+// Y = A X for nr <= kHRhs right-hand sides (tree-position order); T is scratch
+// of kHRhs * (sum of ranks) complex doubles at offsets toff.
+void launch_happly(hipStream_t s, const HLeafs& L, int ntiles, const HTile* tiles, const HBlk* blk, const z32* dval,
+                   const z32* uv, const int64_t* toff, z64* T, const z64* X, int nr, z64* Y);
+void launch_hjacobi_inv(hipStream_t s, const HLeafs& L, const HBlk* blk, const z32* dval, z64* minv, int* status);
+void launch_hjacobi_apply(hipStream_t s, const HLeafs& L, const z64* minv, const z64* X, int nr, z64* Y);
+
+// out[i][c] = sum_p conj(V_i[p][c]) W[p][c] for i < nb (partial: 512 * nb * nr)
+void launch_gm_dots(hipStream_t s, int64_t n, int nr, int nb, const z64* V, int64_t vstride, const z64* W,
+                    z64* partial, z64* out);
+// W += sign * sum_i V_i coef[i]
+void launch_gm_combine(hipStream_t s, int64_t n, int nr, int nb, const z64* V, int64_t vstride, const z64* coef,
+                       double sign, z64* W);
+void launch_gm_scale(hipStream_t s, int64_t n, int nr, const z64* W, const double* scale, z64* out);
+void launch_gm_sub(hipStream_t s, int64_t tot, const z64* a, const z64* b, z64* out);
+
+// q_inc in tree order (nr directions)
+void launch_hqinc(hipStream_t s, const DevMesh& m, double k, int nr, const double* dirs, const int* perm, z64* Q);
+void launch_htraces(hipStream_t s, const DevMesh& m, double k, int nr, const double* dirs, const int* perm,
+                    const z64* X, z64* P, z64* Q);
+void launch_hcompact(hipStream_t s, int nblk, const HBlk* oldb, const HBlk* newb, const z32* src, z32* dst);
+
+}  // namespace bem

@@ -1,0 +1,432 @@
+// Hierarchical-matrix apply and batched GMRES kernels (CDNA4 / gfx950).
+//
+// The compressed operator (reference: HMatrices.jl blocks built in
+// src/solvers/bem/compression/assembly.jl, applied by the batched workspace of
+// compression/workspace.jl:3-122) is stored in complex FP32: dense leaf blocks
+// and low-rank blocks U V^T.  It is applied to FP64 vectors with FP64
+// arithmetic, so the GMRES residual is an FP64 residual of the FP32-stored
+// operator.  Vectors hold nr right-hand sides per tree position:
+// X[pos * nr + c].  Apply = (1) T_b = V_b^T X_cols(b) for every low-rank
+// block, then (2) for every leaf row cluster, the sum of its dense blocks
+// times X and of U_b (its rows) times T_b for the low-rank blocks covering it,
+// in a fixed order (no atomics: the result is deterministic).
+#include "bem_device.h"
+#include "bem_hsolve.h"
+
+namespace bem {
+
+// ---------------------------------------------------------------------------
+// (1) T_b[l][c] = sum_j V_b[l][j] X[(col_lo + j) nr + c], 16 ranks per tile.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_hlr_t(const HBlk* __restrict__ blk, const z32* __restrict__ uv,
+                                               const HTile* __restrict__ tiles, const int64_t* __restrict__ toff,
+                                               const z64* __restrict__ X, int nr, z64* __restrict__ T) {
+    const HTile t = tiles[blockIdx.x];
+    const HBlk b = blk[t.blk];
+    const z32* V = uv + b.off + (int64_t)b.rank * b.m;  // compact: U is m x rank
+    const int tid = threadIdx.x, c = tid & 63, lg = tid >> 6;
+    const int lcnt = min(16, b.rank - t.l0);
+    __shared__ z32 sV[16][64];
+    z64 acc[4] = {};
+    for (int j0 = 0; j0 < b.n; j0 += 64) {
+        __syncthreads();
+        for (int e = tid; e < 16 * 64; e += 256) {
+            const int l = e >> 6, jj = e & 63;
+            sV[l][jj] = (l < lcnt && j0 + jj < b.n) ? V[(int64_t)(t.l0 + l) * b.n + j0 + jj] : z32{0.f, 0.f};
+        }
+        __syncthreads();
+        if (c < nr) {
+            const int jn = min(64, b.n - j0);
+            for (int jj = 0; jj < jn; ++jj) {
+                const z64 x = X[(int64_t)(b.col_lo + j0 + jj) * nr + c];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const z32 v = sV[lg * 4 + q][jj];
+                    acc[q].x += (double)v.x * x.x - (double)v.y * x.y;
+                    acc[q].y += (double)v.x * x.y + (double)v.y * x.x;
+                }
+            }
+        }
+    }
+    if (c < nr)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int l = lg * 4 + q;
+            if (l < lcnt) T[toff[t.blk] + (int64_t)(t.l0 + l) * kHRhs + c] = acc[q];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// (2) one workgroup per leaf row cluster: thread (c, row group of 16).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_hrows(const HLeafs L, const HBlk* __restrict__ blk,
+                                               const z32* __restrict__ dval, const z32* __restrict__ uv,
+                                               const int64_t* __restrict__ toff, const z64* __restrict__ T,
+                                               const z64* __restrict__ X, int nr, z64* __restrict__ Y) {
+    const int leaf = blockIdx.x;
+    const int lo = L.lo[leaf], m = L.m[leaf];
+    const int tid = threadIdx.x, c = tid & 63, rg = tid >> 6;
+    __shared__ z32 sD[64][64];
+    z64 acc[16] = {};
+    // dense blocks of this row
+    for (int e = L.dptr[leaf]; e < L.dptr[leaf + 1]; ++e) {
+        const HBlk b = blk[L.dlist[e]];
+        __syncthreads();
+        for (int q = tid; q < b.m * b.n; q += 256) sD[q / b.n][q % b.n] = dval[b.off + q];
+        __syncthreads();
+        if (c < nr)
+            for (int j = 0; j < b.n; ++j) {
+                const z64 x = X[(int64_t)(b.col_lo + j) * nr + c];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const z32 d = sD[rg * 16 + q][j];  // rows beyond m read stale values, never stored
+                    acc[q].x += (double)d.x * x.x - (double)d.y * x.y;
+                    acc[q].y += (double)d.x * x.y + (double)d.y * x.x;
+                }
+            }
+    }
+    // low-rank blocks covering this row: U_b[o + i][l] T_b[l][c]
+    for (int e = L.lptr[leaf]; e < L.lptr[leaf + 1]; ++e) {
+        const HRef ref = L.llist[e];
+        const HBlk b = blk[ref.blk];
+        const z32* U = uv + b.off;
+        for (int l0 = 0; l0 < b.rank; l0 += 64) {
+            const int lc = min(64, b.rank - l0);
+            __syncthreads();
+            for (int q = tid; q < 64 * 64; q += 256) {
+                const int l = q >> 6, i = q & 63;
+                sD[l][i] = (l < lc && i < m) ? U[(int64_t)(l0 + l) * b.m + ref.o + i] : z32{0.f, 0.f};
+            }
+            __syncthreads();
+            if (c < nr)
+                for (int l = 0; l < lc; ++l) {
+                    const z64 t = T[toff[ref.blk] + (int64_t)(l0 + l) * kHRhs + c];
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        const z32 u = sD[l][rg * 16 + q];
+                        acc[q].x += (double)u.x * t.x - (double)u.y * t.y;
+                        acc[q].y += (double)u.x * t.y + (double)u.y * t.x;
+                    }
+                }
+        }
+    }
+    if (c < nr)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int i = rg * 16 + q;
+            if (i < m) Y[(int64_t)(lo + i) * nr + c] = acc[q];
+        }
+}
+
+void launch_happly(hipStream_t s, const HLeafs& L, int ntiles, const HTile* tiles, const HBlk* blk, const z32* dval,
+                   const z32* uv, const int64_t* toff, z64* T, const z64* X, int nr, z64* Y) {
+    if (ntiles > 0) hipLaunchKernelGGL(k_hlr_t, dim3(ntiles), dim3(256), 0, s, blk, uv, tiles, toff, X, nr, T);
+    hipLaunchKernelGGL(k_hrows, dim3(L.count), dim3(256), 0, s, L, blk, dval, uv, toff, T, X, nr, Y);
+}
+
+// ---------------------------------------------------------------------------
+// Block-Jacobi preconditioner: inverses of the diagonal leaf blocks (FP64,
+// Gauss-Jordan with row pivoting; a singular block keeps the identity and
+// raises status[0]).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_hjacobi_inv(const HLeafs L, const HBlk* __restrict__ blk,
+                                                     const z32* __restrict__ dval, z64* __restrict__ minv,
+                                                     int* __restrict__ status) {
+    const int leaf = blockIdx.x;
+    const int m = L.m[leaf];
+    const HBlk b = blk[L.diag[leaf]];
+    const int tid = threadIdx.x;
+    __shared__ z64 A[64][65];
+    __shared__ int piv[64];
+    __shared__ double sv[4];
+    __shared__ int si[4];
+    __shared__ int s_bad;
+    for (int q = tid; q < 64 * 64; q += 256) {
+        const int i = q >> 6, j = q & 63;
+        z64 v{i == j ? 1.0 : 0.0, 0.0};
+        if (i < m && j < m) {
+            const z32 d = dval[b.off + i * m + j];
+            v = z64{d.x, d.y};
+        }
+        A[i][j] = v;
+    }
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        // pivot row: largest |A[i][k]|, i >= k (wave 0)
+        if (tid < 64) {
+            double v = -1.0;
+            int idx = k;
+            if (tid >= k && tid < m) {
+                v = cabs1(A[tid][k]);
+                idx = tid;
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const double ov = __shfl_xor(v, off, 64);
+                const int oi = __shfl_xor(idx, off, 64);
+                if (ov > v || (ov == v && oi < idx)) {
+                    v = ov;
+                    idx = oi;
+                }
+            }
+            if (tid == 0) {
+                piv[k] = idx;
+                sv[0] = v;
+            }
+        }
+        __syncthreads();
+        const int p = piv[k];
+        if (!(sv[0] > 0.0)) {
+            if (tid == 0) s_bad = 1;
+            break;
+        }
+        if (p != k)
+            for (int j = tid; j < 64; j += 256) {
+                const z64 t = A[k][j];
+                A[k][j] = A[p][j];
+                A[p][j] = t;
+            }
+        __syncthreads();
+        const z64 ip = cinv(A[k][k]);
+        __syncthreads();
+        if (tid == 0) A[k][k] = z64{1.0, 0.0};
+        __syncthreads();
+        for (int j = tid; j < 64; j += 256) A[k][j] = cmul(A[k][j], ip);
+        __syncthreads();
+        // eliminate column k from every other row
+        for (int q = tid; q < 64 * 64; q += 256) {
+            const int i = q >> 6, j = q & 63;
+            if (i != k && j != k) A[i][j] = csub(A[i][j], cmul(A[i][k], A[k][j]));
+        }
+        __syncthreads();
+        for (int i = tid; i < 64; i += 256)
+            if (i != k) A[i][k] = cmul(z64{-A[i][k].x, -A[i][k].y}, A[k][k]);
+        __syncthreads();
+    }
+    if (s_bad) {
+        if (tid == 0) atomicAdd(status, 1);
+        for (int q = tid; q < 64 * 64; q += 256) {
+            const int i = q >> 6, j = q & 63;
+            minv[(int64_t)leaf * 4096 + q] = z64{i == j ? 1.0 : 0.0, 0.0};
+        }
+        return;
+    }
+    // undo the row interchanges on the columns, last first
+    for (int k = m - 1; k >= 0; --k) {
+        const int p = piv[k];
+        if (p != k)
+            for (int i = tid; i < 64; i += 256) {
+                const z64 t = A[i][k];
+                A[i][k] = A[i][p];
+                A[i][p] = t;
+            }
+        __syncthreads();
+    }
+    for (int q = tid; q < 64 * 64; q += 256) minv[(int64_t)leaf * 4096 + q] = A[q >> 6][q & 63];
+}
+
+// Y[leaf rows] = Minv_leaf X[leaf rows]
+__global__ __launch_bounds__(256) void k_hjacobi_apply(const HLeafs L, const z64* __restrict__ minv,
+                                                       const z64* __restrict__ X, int nr, z64* __restrict__ Y) {
+    const int leaf = blockIdx.x;
+    const int lo = L.lo[leaf], m = L.m[leaf];
+    const int tid = threadIdx.x, c = tid & 63, rg = tid >> 6;
+    __shared__ z64 M[64][64];
+    for (int q = tid; q < 64 * 64; q += 256) M[q >> 6][q & 63] = minv[(int64_t)leaf * 4096 + q];
+    __syncthreads();
+    if (c >= nr) return;
+    z64 acc[16] = {};
+    for (int j = 0; j < m; ++j) {
+        const z64 x = X[(int64_t)(lo + j) * nr + c];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const z64 a = M[rg * 16 + q][j];
+            acc[q].x += a.x * x.x - a.y * x.y;
+            acc[q].y += a.x * x.y + a.y * x.x;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int i = rg * 16 + q;
+        if (i < m) Y[(int64_t)(lo + i) * nr + c] = acc[q];
+    }
+}
+
+void launch_hjacobi_inv(hipStream_t s, const HLeafs& L, const HBlk* blk, const z32* dval, z64* minv, int* status) {
+    hipLaunchKernelGGL(k_hjacobi_inv, dim3(L.count), dim3(256), 0, s, L, blk, dval, minv, status);
+}
+void launch_hjacobi_apply(hipStream_t s, const HLeafs& L, const z64* minv, const z64* X, int nr, z64* Y) {
+    hipLaunchKernelGGL(k_hjacobi_apply, dim3(L.count), dim3(256), 0, s, L, minv, X, nr, Y);
+}
+
+// ---------------------------------------------------------------------------
+// Batched GMRES vector kernels.  Basis vector i at V + i * vstride.
+// ---------------------------------------------------------------------------
+// partial[blk][i][c] = sum over this workgroup's positions of conj(V_i) W
+__global__ __launch_bounds__(256) void k_gm_dots(int64_t n, int nr, int nb, const z64* __restrict__ V, int64_t vstride,
+                                                 const z64* __restrict__ W, z64* __restrict__ partial) {
+    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+    __shared__ z64 red[4][8][64];
+    for (int i0 = 0; i0 < nb; i0 += 8) {
+        z64 acc[8] = {};
+        if (c < nr)
+            for (int64_t p = (int64_t)blockIdx.x * 4 + g; p < n; p += (int64_t)gridDim.x * 4) {
+                const z64 w = W[p * nr + c];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (i0 + q < nb) {
+                        const z64 v = V[(int64_t)(i0 + q) * vstride + p * nr + c];
+                        acc[q].x += v.x * w.x + v.y * w.y;
+                        acc[q].y += v.x * w.y - v.y * w.x;
+                    }
+            }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) red[g][q][c] = acc[q];
+        __syncthreads();
+        for (int e = tid; e < 8 * 64; e += 256) {
+            const int q = e >> 6, cc = e & 63;
+            if (i0 + q < nb && cc < nr) {
+                z64 sum = red[0][q][cc];
+                for (int gg = 1; gg < 4; ++gg) {
+                    sum.x += red[gg][q][cc].x;
+                    sum.y += red[gg][q][cc].y;
+                }
+                partial[((int64_t)blockIdx.x * nb + i0 + q) * nr + cc] = sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// out[i][c] = sum over workgroups of partial (fixed order)
+__global__ void k_gm_reduce(int nblk, int nb, int nr, const z64* __restrict__ partial, z64* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * nr) return;
+    double sx = 0, sy = 0;
+    for (int b = 0; b < nblk; ++b) {
+        sx += partial[(int64_t)b * nb * nr + e].x;
+        sy += partial[(int64_t)b * nb * nr + e].y;
+    }
+    out[e] = z64{sx, sy};
+}
+
+constexpr int kGmBlocks = 512;
+
+void launch_gm_dots(hipStream_t s, int64_t n, int nr, int nb, const z64* V, int64_t vstride, const z64* W,
+                    z64* partial, z64* out) {
+    hipLaunchKernelGGL(k_gm_dots, dim3(kGmBlocks), dim3(256), 0, s, n, nr, nb, V, vstride, W, partial);
+    hipLaunchKernelGGL(k_gm_reduce, dim3((nb * nr + 255) / 256), dim3(256), 0, s, kGmBlocks, nb, nr, partial, out);
+}
+
+// W[p][c] += sign * sum_i V_i[p][c] coef[i][c]
+__global__ __launch_bounds__(256) void k_gm_combine(int64_t n, int nr, int nb, const z64* __restrict__ V,
+                                                    int64_t vstride, const z64* __restrict__ coef, double sign,
+                                                    z64* __restrict__ W) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * nr) return;
+    const int c = (int)(e % nr);
+    double sx = 0, sy = 0;
+    for (int i = 0; i < nb; ++i) {
+        const z64 v = V[(int64_t)i * vstride + e], h = coef[i * nr + c];
+        sx += v.x * h.x - v.y * h.y;
+        sy += v.x * h.y + v.y * h.x;
+    }
+    W[e] = z64{W[e].x + sign * sx, W[e].y + sign * sy};
+}
+
+void launch_gm_combine(hipStream_t s, int64_t n, int nr, int nb, const z64* V, int64_t vstride, const z64* coef,
+                       double sign, z64* W) {
+    const int64_t tot = n * nr;
+    hipLaunchKernelGGL(k_gm_combine, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nr, nb, V, vstride, coef,
+                       sign, W);
+}
+
+// out[p][c] = W[p][c] * scale[c]
+__global__ void k_gm_scale(int64_t n, int nr, const z64* __restrict__ W, const double* __restrict__ scale,
+                           z64* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * nr) return;
+    const double f = scale[e % nr];
+    out[e] = z64{W[e].x * f, W[e].y * f};
+}
+
+void launch_gm_scale(hipStream_t s, int64_t n, int nr, const z64* W, const double* scale, z64* out) {
+    const int64_t tot = n * nr;
+    hipLaunchKernelGGL(k_gm_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nr, W, scale, out);
+}
+
+// out[p][c] = a[p][c] - b[p][c]
+__global__ void k_gm_sub(int64_t tot, const z64* __restrict__ a, const z64* __restrict__ b, z64* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= tot) return;
+    out[e] = z64{a[e].x - b[e].x, a[e].y - b[e].y};
+}
+
+void launch_gm_sub(hipStream_t s, int64_t tot, const z64* a, const z64* b, z64* out) {
+    hipLaunchKernelGGL(k_gm_sub, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, tot, a, b, out);
+}
+
+// ---------------------------------------------------------------------------
+// Incident traces (reference src/core/incident.jl:64-76): q_inc in tree
+// order for the right-hand side b = B q_inc of the scattered-field equation
+// (B = S + (i/k)(1/2 I + K') or S, full.jl:144-148), and the scattered traces
+// p_scat = x, q_scat = -q_inc of a rigid surface (full.jl:160-161).
+// ---------------------------------------------------------------------------
+__device__ inline void inc_at(const DevMesh& m, double k, const double* d, int i, z64& p, z64& q) {
+    const int nt = m.nt;
+    const double ph = k * (d[0] * m.cen[i] + d[1] * m.cen[nt + i] + d[2] * m.cen[2 * nt + i]);
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    const double dn = k * (d[0] * m.nrm[i] + d[1] * m.nrm[nt + i] + d[2] * m.nrm[2 * nt + i]);
+    p = z64{cs, sn};
+    q = z64{-dn * sn, dn * cs};
+}
+
+__global__ void k_hqinc(DevMesh m, double k, int nr, const double* __restrict__ dirs, const int* __restrict__ perm,
+                        z64* __restrict__ Q) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)m.nt * nr) return;
+    const int pos = (int)(e / nr), c = (int)(e % nr);
+    z64 p, q;
+    inc_at(m, k, dirs + 3 * c, perm[pos], p, q);
+    Q[e] = q;
+}
+
+__global__ void k_htraces(DevMesh m, double k, int nr, const double* __restrict__ dirs, const int* __restrict__ perm,
+                          const z64* __restrict__ X, z64* __restrict__ P, z64* __restrict__ Q) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)m.nt * nr) return;
+    const int pos = (int)(e / nr), c = (int)(e % nr);
+    const int i = perm[pos];
+    z64 p, q;
+    inc_at(m, k, dirs + 3 * c, i, p, q);
+    P[(int64_t)c * m.nt + i] = X[e];
+    Q[(int64_t)c * m.nt + i] = z64{-q.x, -q.y};
+}
+
+void launch_hqinc(hipStream_t s, const DevMesh& m, double k, int nr, const double* dirs, const int* perm, z64* Q) {
+    const int64_t tot = (int64_t)m.nt * nr;
+    hipLaunchKernelGGL(k_hqinc, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, k, nr, dirs, perm, Q);
+}
+void launch_htraces(hipStream_t s, const DevMesh& m, double k, int nr, const double* dirs, const int* perm,
+                    const z64* X, z64* P, z64* Q) {
+    const int64_t tot = (int64_t)m.nt * nr;
+    hipLaunchKernelGGL(k_htraces, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, k, nr, dirs, perm, X, P,
+                       Q);
+}
+
+// compact copy of low-rank factors: dst[i] for i < len from src
+__global__ void k_hcompact(int nblk, const HBlk* __restrict__ oldb, const HBlk* __restrict__ newb,
+                           const z32* __restrict__ src, z32* __restrict__ dst) {
+    const HBlk a = oldb[blockIdx.x], b = newb[blockIdx.x];
+    const int64_t nu = (int64_t)b.rank * b.m, nv = (int64_t)b.rank * b.n;
+    for (int64_t e = threadIdx.x; e < nu; e += 256) dst[b.off + e] = src[a.off + e];
+    for (int64_t e = threadIdx.x; e < nv; e += 256) dst[b.off + nu + e] = src[a.off + (int64_t)a.rmax * a.m + e];
+}
+
+void launch_hcompact(hipStream_t s, int nblk, const HBlk* oldb, const HBlk* newb, const z32* src, z32* dst) {
+    if (nblk > 0) hipLaunchKernelGGL(k_hcompact, dim3(nblk), dim3(256), 0, s, nblk, oldb, newb, src, dst);
+}
+
+}  // namespace bem

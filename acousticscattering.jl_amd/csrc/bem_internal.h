@@ -47,4 +47,33 @@ void build_near_list(HostMesh& m);
 void icosphere(int level, double ax, double ay, double az, std::vector<double>& nodes,
                std::vector<int64_t>& tri0);
 
+// ---- hierarchical-matrix plan ---------------------------------------------
+// Cluster tree: a node covers tree positions [lo, hi); it is split when it
+// holds more than `leaf` points, along the longest side of its tight bounding
+// box at the box midpoint (stable partition: coordinate < midpoint goes left;
+// an empty side falls back to splitting the count in half).  Nodes are
+// numbered in creation order (root 0, then depth-first, left before right).
+struct HNode {
+    int64_t lo = 0, hi = 0;
+    int32_t child[2] = {-1, -1};
+    double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+};
+// Block tree over (row node, column node) pairs, depth first from (root,
+// root): admissible pairs, min(diam) < eta * dist and dist > near_radius,
+// become low-rank blocks; pairs of two leaves become dense blocks; otherwise
+// the non-leaf side(s) are split (both when both are non-leaves).
+struct HBlock {
+    int32_t row = 0, col = 0;
+};
+struct HPlan {
+    int64_t n = 0;
+    int leaf = 64;
+    double eta = 2.0, near_radius = 0.0;
+    std::vector<int64_t> perm;   // tree position -> point
+    std::vector<int64_t> iperm;  // point -> tree position
+    std::vector<HNode> nodes;
+    std::vector<HBlock> dense, lowrank;
+};
+void hmatrix_plan(const double* points, int64_t n, int leaf, double eta, double near_radius, HPlan& plan);
+
 }  // namespace bem

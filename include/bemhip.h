@@ -95,6 +95,8 @@ typedef struct {
     double assembly_ms;
     double factor_ms;
     double solve_ms;
+    int32_t iterations;          /* GMRES iterations (hierarchical path), else 0      */
+    int32_t method;              /* 0 direct LU, 1 mixed refinement, 2 GMRES           */
 } bem_diagnostics;
 
 /* Library version string. */
@@ -250,6 +252,59 @@ int bem_server_wait(bem_server srv, int64_t ticket, double timeout_ms, double *a
 /* Message of a failed request (valid until the next call on the server). */
 const char *bem_server_error(bem_server srv, int64_t ticket);
 int bem_server_stats_get(bem_server srv, bem_server_stats *out);
+
+/* ---- hierarchical-matrix path ---------------------------------------------
+ * Reference: the compressed operators of src/solvers/bem/compression/
+ * (assembly.jl:3-129, ACA blocks through HMatrices.jl; workspace.jl:3-122,
+ * the batched H-matvec) solved by GMRES (full.jl:156-157; compression/
+ * fluid.jl:421-449).  Rigid surfaces, Burton-Miller or CBIE, with the dense
+ * path's scattered-field equation (full.jl:103-111,144-148): the system
+ * operator 1/2 I - D [- (i/k) H] and the right-hand-side operator
+ * S [+ (i/k)(1/2 I + K')] are both compressed over one cluster tree; the
+ * scattered traces are x and -dp_inc/dn.  Blocks are complex FP32 (dense
+ * leaves and ACA factors U V^T); GMRES runs in FP64 with FP64 residuals of
+ * that operator, right-preconditioned by block Jacobi on the diagonal leaf
+ * blocks. */
+typedef struct bem_hmatrix_s *bem_hmatrix;
+
+typedef struct {
+    int32_t leaf_size;       /* cluster-tree leaf size (64)                          */
+    double eta;              /* admissibility min(diam) < eta * dist (2)              */
+    double aca_tolerance;    /* ACA relative Frobenius tolerance per block (1e-6)     */
+    int32_t max_rank;        /* rank cap of one block, <= 256 (256)                   */
+    int32_t gmres_restart;   /* Krylov dimension per cycle (50)                       */
+    int32_t gmres_maxiter;   /* total iterations (1000)                               */
+    double gmres_tolerance;  /* relative FP64 residual ||b - A x|| / ||b|| (1e-8)      */
+    int32_t preconditioner;  /* 1 block Jacobi on the diagonal leaf blocks, 0 none (1) */
+} bem_hmatrix_options;
+
+typedef struct {
+    int64_t n, nodes, leaves, dense_blocks, lowrank_blocks, near_pairs;
+    uint64_t dense_bytes, lowrank_bytes;
+    int32_t max_rank, unconverged_blocks;
+    double mean_rank;
+    double plan_ms, assembly_ms, aca_ms;
+} bem_hmatrix_info;
+
+void bem_hmatrix_defaults(bem_hmatrix_options *options);
+/* Cluster tree and block tree only (host, no device): `points` 3 x n
+ * column-major.  perm (n, 0-based) receives the tree order; blocks, when
+ * non-null with room for 5 * (n_dense + n_lowrank) values, receives rows
+ * (lowrank 0/1, row_lo, m, col_lo, n), dense blocks first, each list in
+ * block-tree order.  The rules are stated in the library's sources. */
+int bem_hmatrix_plan(int64_t n, const double *points, int32_t leaf_size, double eta, double near_radius,
+                     int64_t *perm, int64_t *n_nodes, int64_t *n_dense, int64_t *n_lowrank, int64_t *blocks,
+                     int64_t blocks_cap);
+int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem *problem,
+                       const bem_hmatrix_options *options, bem_hmatrix *out);
+int bem_hmatrix_destroy(bem_hmatrix h);
+int bem_hmatrix_get_info(bem_hmatrix h, bem_hmatrix_info *out);
+/* y = A x for n_rhs columns (panel order, n x n_rhs column-major complex). */
+int bem_hmatrix_matvec(bem_hmatrix h, int32_t n_rhs, const double *x, double *y);
+/* GMRES for n_rhs plane-wave incidences (angles as in bem_operator_solve);
+ * the solution carries traces, far field and per-incidence diagnostics
+ * (method 2, iterations, relative_residual = ||b - A x|| / ||b||). */
+int bem_hmatrix_solve(bem_hmatrix h, int32_t n_rhs, const double *angles, bem_solution *out);
 
 /* ---- test hook: dense LU of a caller matrix ------------------------------
  * Factors the column-major n x n complex matrix A on the device with the
