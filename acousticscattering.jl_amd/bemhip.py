@@ -37,7 +37,7 @@ REASONS = ["none", "conversion", "single_singular", "condition", "nonfinite",
 #: Every symbol declared in include/bemhip.h.
 EXPORTED = [
     "bem_version", "bem_problem_defaults", "bem_ctx_create", "bem_ctx_destroy", "bem_last_error",
-    "bem_ctx_memory", "bem_ctx_set_profiling", "bem_ctx_kernel_stats", "bem_ctx_reset_stats",
+    "bem_ctx_memory", "bem_ctx_set_profiling", "bem_ctx_kernel_stats", "bem_ctx_reset_stats", "bem_ctx_set_lanes",
     "bem_icosphere_size", "bem_icosphere", "bem_revolution_size", "bem_revolution",
     "bem_mesh_create", "bem_mesh_validate", "bem_mesh_destroy", "bem_mesh_info", "bem_mesh_panels",
     "bem_mesh_quadrature", "bem_mesh_near_count", "bem_operator_create", "bem_operator_destroy",
@@ -130,6 +130,7 @@ def load_library(path=LIB_PATH):
         "bem_ctx_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int32]),
         "bem_ctx_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip, _dp, _dp, _dp]),
         "bem_ctx_reset_stats": (ctypes.c_int, [_vp]),
+        "bem_ctx_set_lanes": (ctypes.c_int, [_vp, ctypes.c_int32]),
         "bem_icosphere_size": (ctypes.c_int, [ctypes.c_int32, _ip, _ip]),
         "bem_icosphere": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, _dp, _ip]),
@@ -278,6 +279,10 @@ class Context:
 
     def reset_stats(self):
         self.check(self.lib.bem_ctx_reset_stats(self._h))
+
+    def set_lanes(self, max_lanes):
+        """Largest number of concurrent solves of batched sweeps (1..8)."""
+        self.check(self.lib.bem_ctx_set_lanes(self._h, int(max_lanes)))
 
     def kernel_stats(self, name):
         n = ctypes.c_int64()
@@ -541,9 +546,12 @@ def solve_full_bem(boundary, k, surface, *, incidence_angle=0.0, incidence_azimu
     """Drop-in for solve_full_bem (src/solvers/bem/full.jl:52-66,273-287).
 
     Returns (p_scat, dpdn_scat, quad[, diagnostics]) at the panel collocation
-    points.  The dense direct solve replaces the reference's GMRES for
-    rigid/soft (its converged limit); `compression`, `correction` and
-    `gmres_kwargs` are accepted for signature compatibility only.
+    points.  Without compression the dense direct solve replaces the
+    reference's GMRES for rigid/soft (its converged limit).  With
+    compression={"method": "hmatrix", ...} a rigid surface is solved by the
+    compressed operator and GMRES (HMatrix; the remaining compression keys
+    are HMatrix options, gmres_kwargs may set reltol / restart / maxiter).
+    `correction` is accepted for signature compatibility only.
     """
     if incident is not None:
         raise ValueError("only the plain unit plane wave is supported on the device path")
@@ -551,6 +559,27 @@ def solve_full_bem(boundary, k, surface, *, incidence_angle=0.0, incidence_azimu
         raise ValueError("incidence angles must be finite")
     if condition_limit < 0:
         raise ValueError("condition_limit must be nonnegative")
+    method = (compression or {}).get("method", "none")
+    if method not in ("none", "hmatrix"):
+        raise ValueError("compression method must be :none or :hmatrix")
+    if method == "hmatrix":
+        opts = {key: v for key, v in compression.items() if key != "method"}
+        g = dict(gmres_kwargs or {})
+        if "reltol" in g:
+            opts["gmres_tolerance"] = g.pop("reltol")
+        if "restart" in g:
+            opts["gmres_restart"] = g.pop("restart")
+        if "maxiter" in g:
+            opts["gmres_maxiter"] = g.pop("maxiter")
+        H = HMatrix(surface, boundary, k, formulation=formulation, **opts)
+        sol = H.solve([(incidence_angle, incidence_azimuth)])
+        p, q = sol.traces(0)
+        if return_diagnostics:
+            d = sol.diagnostics(0)
+            d["formulation"] = {BEM_BURTON_MILLER: "burton_miller", BEM_CBIE: "cbie"}[H.problem.formulation]
+            d["compression"] = dict(compression)
+            return p, q, surface, d
+        return p, q, surface
     op = Operator(surface, boundary, k, formulation=formulation, precision=precision,
                   refinement=refinement, equilibrate=equilibrate)
     sol = op.solve([(incidence_angle, incidence_azimuth)])

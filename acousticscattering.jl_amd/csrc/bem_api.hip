@@ -47,10 +47,10 @@ struct BemError : std::runtime_error {
             throw BemError(BEM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
-const char* kClassNames[] = {"assemble", "near", "lu_panel", "lu_gemm", "lu_trsm",
-                             "lu_swap",  "solve", "residual", "far_field"};
-constexpr int kNumClasses = 9;
-enum { C_ASSEMBLE = 0, C_NEAR = 1, C_RESIDUAL = 7, C_FAR = 8 };
+const char* kClassNames[] = {"assemble", "near", "lu_panel", "lu_gemm", "lu_trsm", "lu_swap",
+                             "solve",    "residual", "far_field", "h_apply", "h_aca", "h_ortho"};
+constexpr int kNumClasses = 12;
+enum { C_ASSEMBLE = 0, C_NEAR = 1, C_RESIDUAL = 7, C_FAR = 8, C_HAPPLY = 9, C_HACA = 10, C_HORTHO = 11 };
 
 struct ProfRec {
     int cls;
@@ -129,6 +129,8 @@ void prof_end(void* prof, hipStream_t s, int cls, double flops, double bytes) {
 // whole solve, so concurrent lanes never call hipMalloc/hipFree mid-sweep.
 struct Lane {
     hipStream_t s = nullptr;
+    hipStream_t s2 = nullptr;          // LU panel look-ahead
+    hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
     PanelWS ws{};
     void* ws_mem = nullptr;
     char* arena = nullptr;
@@ -145,6 +147,7 @@ struct bem_ctx_s {
     std::mutex amu;  // allocator bookkeeping
     Prof prof;
     std::vector<Lane*> lanes;  // lanes[0] serves the single-solve entry points
+    int max_lanes = 8;         // concurrent solves of bem_sweep
 
     Lane* lane0() { return lanes[0]; }
     // Device memory: a caching allocator.  Released blocks go to a free pool
@@ -274,6 +277,9 @@ Lane* make_lane(bem_ctx c) {
     auto* L = new Lane();
     L->pl.prof = &c->prof;
     HIPCHECK(hipStreamCreateWithFlags(&L->s, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&L->s2, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&L->ev_ready, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&L->ev_panel, hipEventDisableTiming));
     const int G = 2048;  // 262144 rows (128 per leaf)
     const int nodes = 2 * G + 16;
     L->ws.max_blocks = G;
@@ -297,9 +303,13 @@ Lane* make_lane(bem_ctx c) {
 void free_lane(bem_ctx c, Lane* L) {
     if (!L) return;
     if (L->s) hipStreamSynchronize(L->s);
+    if (L->s2) hipStreamSynchronize(L->s2);
     c->release(L->ws_mem);
     c->release(L->arena);
     if (L->s) hipStreamDestroy(L->s);
+    if (L->s2) hipStreamDestroy(L->s2);
+    if (L->ev_ready) hipEventDestroy(L->ev_ready);
+    if (L->ev_panel) hipEventDestroy(L->ev_panel);
     delete L;
 }
 
@@ -382,7 +392,12 @@ void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) 
     const int n = op->n;
     launch_fill_int(s, op->lane->ws.status, 1, INT_MAX);
     launch_fill_int(s, op->lane->ws.status + 1, 1, 0);
-    int rc = lu_factor<R>(s, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, op->lane->ws, &op->lane->pl);
+    LuStreams st;
+    st.main = s;
+    st.side = op->lane->s2;
+    st.ready = op->lane->ev_ready;
+    st.panel = op->lane->ev_panel;
+    int rc = lu_factor<R>(st, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, op->lane->ws, &op->lane->pl);
     if (rc == 2) throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (262144 rows)");
     if (rc != 0) throw BemError(BEM_ERR_DEVICE, "LU panel launch failed");
     HIPCHECK(hipGetLastError());
@@ -1402,7 +1417,9 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
     z32* uv2 = nullptr;
     std::vector<int> hconv(nd + nl);
     try {
+        prof_begin(&h->lane->pl, s, C_HACA);
         launch_aca(s, dm, kind, h->prob.k, h->perm, nl, dlist, op.dblk, uv1, conv, o.aca_tolerance);
+        prof_end(&h->lane->pl, s, C_HACA, 0.0, 0.0);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(op.blk.data(), op.dblk, sizeof(HBlk) * op.blk.size(), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipMemcpyAsync(hconv.data(), conv, sizeof(int) * hconv.size(), hipMemcpyDeviceToHost, s));
@@ -1546,7 +1563,13 @@ void hm_build(bem_hmatrix h) {
 
 // Y = op X on the device (tree-position order, nr <= kHRhs)
 void hm_apply(bem_hmatrix h, HOp& op, const z64* X, int nr, z64* Y) {
+    // algorithmic traffic: every block once, plus the input and output vectors
+    const double bytes = (double)h->dense_count * sizeof(z32) + (double)op.lowrank_bytes +
+                         2.0 * (double)h->n * nr * sizeof(z64);
+    const double flops = 8.0 * nr * ((double)h->dense_count + (double)op.lowrank_bytes / sizeof(z32));
+    prof_begin(&h->lane->pl, h->lane->s, C_HAPPLY);
     launch_happly(h->lane->s, op.L, op.ntiles, op.tiles, op.dblk, op.dval, op.uv, op.toff, op.T, X, nr, Y);
+    prof_end(&h->lane->pl, h->lane->s, C_HAPPLY, flops, bytes);
 }
 
 // A M^{-1} X (M^{-1} = block Jacobi or identity); Z is scratch
@@ -1624,6 +1647,7 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
                 z64* Vj = Vb + (int64_t)j * vs;
                 hm_apply_prec(h, Vj, nr, Z, W);
                 // classical Gram-Schmidt, twice
+                prof_begin(&h->lane->pl, s, C_HORTHO);
                 launch_gm_dots(s, n, nr, j + 1, Vb, vs, W, part, h1);
                 launch_gm_combine(s, n, nr, j + 1, Vb, vs, h1, -1.0, W);
                 launch_gm_dots(s, n, nr, j + 1, Vb, vs, W, part, h2);
@@ -1631,6 +1655,8 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
                 launch_gm_dots(s, n, nr, 1, W, vs, W, part, sq);
                 hipLaunchKernelGGL(k_inv_norm, dim3(1), dim3(64), 0, s, nr, sq, scale, norm);
                 launch_gm_scale(s, n, nr, W, scale, Vb + (int64_t)(j + 1) * vs);
+                prof_end(&h->lane->pl, s, C_HORTHO, 16.0 * 4 * (j + 1) * (double)vs,
+                         (4.0 * (j + 1) + 6.0) * (double)vs * sizeof(z64));
                 HIPCHECK(hipMemcpyAsync(hh1.data(), h1, sizeof(z64) * (j + 1) * nr, hipMemcpyDeviceToHost, s));
                 HIPCHECK(hipMemcpyAsync(hh2.data(), h2, sizeof(z64) * (j + 1) * nr, hipMemcpyDeviceToHost, s));
                 HIPCHECK(hipMemcpyAsync(rn.data(), norm, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
@@ -1815,6 +1841,13 @@ int bem_ctx_kernel_stats(bem_ctx ctx, const char* name, int64_t* launches, doubl
         }
     ctx->err = std::string("unknown kernel class ") + name;
     return BEM_ERR_ARGUMENT;
+}
+
+int bem_ctx_set_lanes(bem_ctx ctx, int32_t max_lanes) {
+    if (!ctx || max_lanes < 1 || max_lanes > 8) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    ctx->max_lanes = max_lanes;
+    return BEM_OK;
 }
 
 int bem_ctx_reset_stats(bem_ctx ctx) {
@@ -2182,7 +2215,7 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh* meshes, const bem_problem*
         uint64_t arena_now = 0;
         for (Lane* L : ctx->lanes) arena_now += L->cap;
         const uint64_t avail = ctx->budget - (ctx->used - arena_now);
-        int S = (int)std::min<uint64_t>(8, (uint64_t)(0.9 * (double)avail) / max_need);
+        int S = (int)std::min<uint64_t>((uint64_t)ctx->max_lanes, (uint64_t)(0.9 * (double)avail) / max_need);
         S = std::max(1, std::min(S, n));
         while ((int)ctx->lanes.size() < S) ctx->lanes.push_back(make_lane(ctx));
         // release arenas that are too small or unused, then (re)allocate

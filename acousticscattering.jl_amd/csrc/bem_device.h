@@ -135,8 +135,15 @@ struct PanelWS {
     int max_blocks;    // largest leaf count (128 rows per leaf)
     int max_nodes;
 };
+// Streams of one LU: main, and an optional side stream with two events for
+// the panel look-ahead (side = null: everything on main).
+struct LuStreams {
+    hipStream_t main = nullptr, side = nullptr;
+    hipEvent_t ready = nullptr, panel = nullptr;
+};
+// Returns 0, 2 (panel workspace too small) or 3 (HIP stream/event failure).
 template <typename R>
-int lu_factor(hipStream_t s, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv, PanelWS& ws,
+int lu_factor(const LuStreams& st, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv, PanelWS& ws,
               void* prof);
 template <typename R>
 void lu_solve(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm,

@@ -981,14 +981,150 @@ __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A
             }
 }
 
+// C -= A B (or C = A B), complex, row-major; 3M products on MFMA (Gauss:
+// T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi); Re = T1 - T2,
+// Im = T3 - T1 - T2), 25 % fewer matrix-core products than four real GEMMs.
+// 512 threads = 8 waves as 2 (rows) x 4 (columns), each wave 64 x 32 (4 x 2
+// MFMA tiles); K staged in chunks of KC through double-buffered LDS planes
+// (re, im, re + im; rows padded to 144 so an operand read hits distinct banks),
+// one barrier per chunk, the next chunk's global loads overlapping the MFMAs.
+template <typename R>
+struct G3;
+template <>
+struct G3<float> {
+    static constexpr int KC = 16;
+};
+template <>
+struct G3<double> {
+    static constexpr int KC = 8;
+};
+constexpr int kG3Pad = 144;
+
+template <typename R, int ASSIGN>
+__global__ __launch_bounds__(512, 1) void k_gemm3m(const C2<R>* __restrict__ A, int64_t lda,
+                                                   const C2<R>* __restrict__ B, int64_t ldb, C2<R>* __restrict__ C,
+                                                   int64_t ldc, int M, int N, int K) {
+    typedef typename Mfma<R>::v4 v4;
+    constexpr int KC = G3<R>::KC;
+    constexpr int PER = KC * 128 / 512;  // elements of each operand a thread stages per chunk
+    __shared__ R sA[2][3][KC][kG3Pad];
+    __shared__ R sB[2][3][KC][kG3Pad];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = (wv >> 2) * 64, wn = (wv & 3) * 32;
+    const int m0 = blockIdx.y * kGM, n0 = blockIdx.x * kGN;
+    v4 t1[4][2], t2[4][2], t3[4][2];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) t1[x][y] = t2[x][y] = t3[x][y] = v4{0, 0, 0, 0};
+    const int lm = tid & 127, lk = (tid >> 7) * PER;
+    C2<R> ra[PER], rb[PER];
+    auto gload = [&](int kc) {
+        const int gm = m0 + lm, gn = n0 + lm;
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int gk = kc + lk + t;
+            ra[t] = (gm < M && gk < K) ? A[(int64_t)gm * lda + gk] : C2<R>{0, 0};
+            rb[t] = (gn < N && gk < K) ? B[(int64_t)gk * ldb + gn] : C2<R>{0, 0};
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            sA[buf][0][lk + t][lm] = ra[t].x;
+            sA[buf][1][lk + t][lm] = ra[t].y;
+            sA[buf][2][lk + t][lm] = ra[t].x + ra[t].y;
+            sB[buf][0][lk + t][lm] = rb[t].x;
+            sB[buf][1][lk + t][lm] = rb[t].y;
+            sB[buf][2][lk + t][lm] = rb[t].x + rb[t].y;
+        }
+    };
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int kc = 0; kc < K; kc += KC) {
+        const bool more = kc + KC < K;
+        if (more) gload(kc + KC);
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4) {
+            const int kr = kk + (lane >> 4), cl = lane & 15;
+            R a0[4], a1[4], a2[4], b0[2], b1[2], b2[2];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a0[q] = sA[buf][0][kr][wm + q * 16 + cl];
+                a1[q] = sA[buf][1][kr][wm + q * 16 + cl];
+                a2[q] = sA[buf][2][kr][wm + q * 16 + cl];
+            }
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                b0[q] = sB[buf][0][kr][wn + q * 16 + cl];
+                b1[q] = sB[buf][1][kr][wn + q * 16 + cl];
+                b2[q] = sB[buf][2][kr][wn + q * 16 + cl];
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) t1[x][y] = Mfma<R>::op(a0[x], b0[y], t1[x][y]);
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) t2[x][y] = Mfma<R>::op(a1[x], b1[y], t2[x][y]);
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) t3[x][y] = Mfma<R>::op(a2[x], b2[y], t3[x][y]);
+        }
+        if (more) lstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm + x * 16 + Mfma<R>::row(lane, r);
+                const int gn = n0 + wn + y * 16 + (lane & 15);
+                if (gm < M && gn < N) {
+                    const R re = t1[x][y][r] - t2[x][y][r];
+                    const R im = t3[x][y][r] - t1[x][y][r] - t2[x][y][r];
+                    if (ASSIGN) {
+                        C[(int64_t)gm * ldc + gn] = C2<R>{re, im};
+                    } else {
+                        C2<R> c = C[(int64_t)gm * ldc + gn];
+                        c.x -= re;
+                        c.y -= im;
+                        C[(int64_t)gm * ldc + gn] = c;
+                    }
+                }
+            }
+}
+
 // ---------------------------------------------------------------------------
 // drivers
 // ---------------------------------------------------------------------------
 struct ProfHook;  // defined in the API unit; null when profiling is off
+#define HIPCHECK_LU(x)                       \
+    do {                                     \
+        if ((x) != hipSuccess) return 3;     \
+    } while (0)
 void prof_begin(void* prof, hipStream_t s, int cls);
 void prof_end(void* prof, hipStream_t s, int cls, double flops, double bytes);
 enum { CLS_PANEL = 2, CLS_GEMM = 3, CLS_TRSM = 4, CLS_SWAP = 5, CLS_SOLVE = 6 };
 
+
+// Trailing-update GEMM: 3M products for complex FP32; four real products for
+// FP64, whose three 3M accumulator sets would not fit two waves per SIMD.
+template <typename R, int ASSIGN>
+static void launch_gemm(hipStream_t s, dim3 grid, const C2<R>* A, int64_t lda, const C2<R>* B, int64_t ldb, C2<R>* C,
+                        int64_t ldc, int M, int N, int K) {
+    if constexpr (sizeof(R) == sizeof(float))
+        hipLaunchKernelGGL((k_gemm3m<R, ASSIGN>), grid, dim3(512), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
+    else
+        hipLaunchKernelGGL((k_gemm_sub<R, ASSIGN>), grid, dim3(512), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
+}
 
 template <typename R>
 static void launch_laswp(hipStream_t s, C2<R>* A, int64_t lda, int k0, int w, const int* ipiv, int cb0, int ce0,
@@ -1016,8 +1152,8 @@ static void launch_update(hipStream_t s, C2<R>* A, int64_t lda, int r0, int c0, 
     if (M <= 0 || N <= 0 || K <= 0) return;
     prof_begin(prof, s, CLS_GEMM);
     dim3 grid((N + kGN - 1) / kGN, (M + kGM - 1) / kGM);
-    hipLaunchKernelGGL((k_gemm_sub<R, 0>), grid, dim3(512), 0, s, A + (int64_t)r0 * lda + ka, lda,
-                       A + (int64_t)ka * lda + c0, lda, A + (int64_t)r0 * lda + c0, lda, M, N, K);
+    launch_gemm<R, 0>(s, grid, A + (int64_t)r0 * lda + ka, lda, A + (int64_t)ka * lda + c0, lda,
+                      A + (int64_t)r0 * lda + c0, lda, M, N, K);
     prof_end(prof, s, CLS_GEMM, 8.0 * (double)M * N * K, (2.0 * (double)M * N + (double)(M + N) * K) * sizeof(C2<R>));
 }
 
@@ -1025,43 +1161,76 @@ static void launch_update(hipStream_t s, C2<R>* A, int64_t lda, int r0, int c0, 
 // the trailing update of each outer block is one K = 256 MFMA GEMM.
 constexpr int kOuter = 256;
 
+// The panels of outer block [K0, Kend): tournament panel, pivot-block
+// factors, L21, and the updates confined to the block's own columns.
 template <typename R>
-int lu_factor(hipStream_t s, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv, PanelWS& ws, void* prof) {
+static int panel_phase(hipStream_t s, C2<R>* A, int64_t lda, int n, int K0, int Kend, int* ipiv, C2<R>* inv,
+                       PanelWS& ws, void* prof) {
+    for (int k0 = K0; k0 < Kend; k0 += kNB) {
+        const int w = (Kend - k0) < kNB ? (Kend - k0) : kNB;
+        const int m = n - k0;
+        const int G = (m + kTsRows - 1) / kTsRows;
+        if (G > ws.max_blocks) return 2;
+        prof_begin(prof, s, CLS_PANEL);
+        hipLaunchKernelGGL((k_tslu<R>), dim3(G), dim3(kTsThreads), 0, s, A, lda, n, k0, w, ws);
+        hipLaunchKernelGGL((k_tslu_apply<R>), dim3(1), dim3(256), 0, s, A, lda, n, k0, w, ws, ipiv, inv);
+        if (m > w) {
+            // L21 = A21 inv(U11), in place
+            const C2<R>* Ui = inv + (size_t)(2 * (k0 / kNB) + 1) * kNB * kNB;
+            dim3 grid(1, (m - w + kGM - 1) / kGM);
+            launch_gemm<R, 1>(s, grid, A + (int64_t)(k0 + w) * lda + k0, lda, Ui, (int64_t)kNB,
+                              A + (int64_t)(k0 + w) * lda + k0, lda, m - w, w, w);
+        }
+        prof_end(prof, s, CLS_PANEL, 8.0 * (double)m * w * w / 2.0, 2.0 * (double)m * w * sizeof(C2<R>));
+        // this panel's interchanges inside the outer block
+        launch_laswp<R>(s, A, lda, k0, w, ipiv, K0, k0, k0 + w, Kend, prof);
+        if (k0 + w < Kend) {
+            launch_trsm<R>(s, A, lda, k0, w, k0 + w, Kend, inv, prof);
+            launch_update<R>(s, A, lda, k0 + w, k0 + w, k0, n - k0 - w, Kend - k0 - w, w, prof);
+        }
+    }
+    return 0;
+}
+
+// Two-level right-looking LU: 64-wide panels inside 256-wide outer blocks, so
+// the trailing update of each outer block is one K = 256 MFMA GEMM.  With a
+// side stream (LuStreams::side) the panels of outer block j + 1 run there,
+// overlapped with block j's trailing update: the update of block j + 1's
+// columns goes first (event `ready`), block j + 1's panel phase follows on the
+// side stream (event `panel`), and the main stream updates the remaining
+// columns meanwhile, then waits for `panel` before applying block j + 1's
+// interchanges elsewhere.  Columns touched concurrently are disjoint.
+template <typename R>
+int lu_factor(const LuStreams& st, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv, PanelWS& ws, void* prof) {
+    hipStream_t s = st.main;
+    const bool ahead = st.side != nullptr;
+    int rc = panel_phase<R>(s, A, lda, n, 0, std::min(n, kOuter), ipiv, inv, ws, prof);
+    if (rc) return rc;
     for (int K0 = 0; K0 < n; K0 += kOuter) {
         const int KW = (n - K0) < kOuter ? (n - K0) : kOuter;
         const int Kend = K0 + KW;
-        for (int k0 = K0; k0 < Kend; k0 += kNB) {
-            const int w = (Kend - k0) < kNB ? (Kend - k0) : kNB;
-            const int m = n - k0;
-            const int G = (m + kTsRows - 1) / kTsRows;
-            if (G > ws.max_blocks) return 2;
-            prof_begin(prof, s, CLS_PANEL);
-            hipLaunchKernelGGL((k_tslu<R>), dim3(G), dim3(kTsThreads), 0, s, A, lda, n, k0, w, ws);
-            hipLaunchKernelGGL((k_tslu_apply<R>), dim3(1), dim3(256), 0, s, A, lda, n, k0, w, ws, ipiv, inv);
-            if (m > w) {
-                // L21 = A21 inv(U11), in place
-                const C2<R>* Ui = inv + (size_t)(2 * (k0 / kNB) + 1) * kNB * kNB;
-                dim3 grid(1, (m - w + kGM - 1) / kGM);
-                hipLaunchKernelGGL((k_gemm_sub<R, 1>), grid, dim3(512), 0, s, A + (int64_t)(k0 + w) * lda + k0, lda,
-                                   Ui, (int64_t)kNB, A + (int64_t)(k0 + w) * lda + k0, lda, m - w, w, w);
-            }
-            prof_end(prof, s, CLS_PANEL, 8.0 * (double)m * w * w / 2.0, 2.0 * (double)m * w * sizeof(C2<R>));
-            // this panel's interchanges inside the outer block
-            launch_laswp<R>(s, A, lda, k0, w, ipiv, K0, k0, k0 + w, Kend, prof);
-            if (k0 + w < Kend) {
-                launch_trsm<R>(s, A, lda, k0, w, k0 + w, Kend, inv, prof);
-                launch_update<R>(s, A, lda, k0 + w, k0 + w, k0, n - k0 - w, Kend - k0 - w, w, prof);
-            }
+        if (K0 > 0) {
+            if (ahead) HIPCHECK_LU(hipStreamWaitEvent(s, st.panel, 0));
+            else if ((rc = panel_phase<R>(s, A, lda, n, K0, Kend, ipiv, inv, ws, prof))) return rc;
         }
         // the outer block's interchanges on every other column
         launch_laswp<R>(s, A, lda, K0, KW, ipiv, 0, K0, Kend, n, prof);
-        if (Kend < n) {
-            // U12 = inv(L11) A12 by 64-row blocks, then the K = KW trailing update
-            for (int i0 = K0; i0 < Kend; i0 += kNB) {
-                const int wi = (Kend - i0) < kNB ? (Kend - i0) : kNB;
-                launch_trsm<R>(s, A, lda, i0, wi, Kend, n, inv, prof);
-                launch_update<R>(s, A, lda, i0 + wi, Kend, i0, Kend - i0 - wi, n - Kend, wi, prof);
-            }
+        if (Kend >= n) break;
+        // U12 = inv(L11) A12 by 64-row blocks, then the K = KW trailing update
+        for (int i0 = K0; i0 < Kend; i0 += kNB) {
+            const int wi = (Kend - i0) < kNB ? (Kend - i0) : kNB;
+            launch_trsm<R>(s, A, lda, i0, wi, Kend, n, inv, prof);
+            launch_update<R>(s, A, lda, i0 + wi, Kend, i0, Kend - i0 - wi, n - Kend, wi, prof);
+        }
+        const int Kend2 = std::min(n, Kend + kOuter);
+        if (ahead) {
+            launch_update<R>(s, A, lda, Kend, Kend, K0, n - Kend, Kend2 - Kend, KW, prof);
+            HIPCHECK_LU(hipEventRecord(st.ready, s));
+            HIPCHECK_LU(hipStreamWaitEvent(st.side, st.ready, 0));
+            if ((rc = panel_phase<R>(st.side, A, lda, n, Kend, Kend2, ipiv, inv, ws, prof))) return rc;
+            HIPCHECK_LU(hipEventRecord(st.panel, st.side));
+            launch_update<R>(s, A, lda, Kend, Kend2, K0, n - Kend, n - Kend2, KW, prof);
+        } else {
             launch_update<R>(s, A, lda, Kend, Kend, K0, n - Kend, n - Kend, KW, prof);
         }
     }
@@ -1244,8 +1413,8 @@ template void launch_prep_rhs<double>(hipStream_t, int, const z64*, const double
 template void launch_prep_rhs<float>(hipStream_t, int, const z64*, const double*, double, z32*);
 template void launch_finish<double>(hipStream_t, int, const z64*, double, const double*, z64*, int, int*);
 template void launch_finish<float>(hipStream_t, int, const z32*, double, const double*, z64*, int, int*);
-template int lu_factor<double>(hipStream_t, z64*, int64_t, int, int*, z64*, PanelWS&, void*);
-template int lu_factor<float>(hipStream_t, z32*, int64_t, int, int*, z32*, PanelWS&, void*);
+template int lu_factor<double>(const LuStreams&, z64*, int64_t, int, int*, z64*, PanelWS&, void*);
+template int lu_factor<float>(const LuStreams&, z32*, int64_t, int, int*, z32*, PanelWS&, void*);
 template void lu_solve<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, const z64*, z64*, z64*, void*);
 template void lu_solve<float>(hipStream_t, const z32*, int64_t, int, const int*, const z32*, const z32*, z32*, z32*, void*);
 template void lu_solve_h<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, const z64*, z64*, z64*, void*);

@@ -10,6 +10,12 @@ FP64 far field.  |dTS| is checked against the same sweep solved all-FP64.
 
 Prints one JSON line (value = solves/s of the timed sweep).
 
+`--workload hmatrix` measures the hierarchical-matrix path: a rigid bent
+capped cylinder (radius 2 mm, centerline 60 mm, bend radius 120 mm,
+hemispherical caps) at 200k triangles, k = 1000 rad/m, cluster leaf 64,
+eta = 2, ACA 1e-6 in complex FP32, GMRES(50) to an FP64 relative residual of
+1e-8 for 64 incidences uniformly spaced in [0, pi], solved as one batch.
+
 `--workload serve` measures the served form instead: seeded Poisson streams of
 random bodies of revolution (rigid / pressure-release / fluid, N in [2000,
 40000]) at 50, 200 and 800 requests/s through the server, reporting solves/s
@@ -183,12 +189,70 @@ def serve_main(args):
     ctx.close()
 
 
+def hmatrix_main(args):
+    import bemhip as bh
+
+    ctx = bh.Context(0)
+    body = bh.Cylinder(0.002, 0.06, 0.12, 0.002)
+    nodes, tri = bh.cylinder_arrays(body, args.hm_nphi)
+    mesh = bh.Mesh(nodes, tri, ctx=ctx)
+    k = 1000.0
+    log(f"bent cylinder: {mesh.n} triangles, k = {k}")
+    warm = bh.mesh(bh.Sphere(0.01), level=3, ctx=ctx)  # loads every kernel once
+    bh.HMatrix(warm, bh.Rigid(), 100.0).solve([(0.0, 0.0)])
+    angles = [(math.pi * c / (args.hm_rhs - 1), 0.0) for c in range(args.hm_rhs)]
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    H = bh.HMatrix(mesh, bh.Rigid(), k, leaf_size=64, eta=2.0, aca_tolerance=1e-6, gmres_restart=50,
+                   gmres_tolerance=1e-8)
+    t1 = time.perf_counter()
+    sol = H.solve(angles)
+    t2 = time.perf_counter()
+    ctx.set_profiling(False)
+    info = H.info()
+    diags = [sol.diagnostics(r) for r in range(len(angles))]
+    ts = [bh.target_strength(sol.far_field(r)[0]) for r in range(len(angles))]
+    stats = {name: ctx.kernel_stats(name) for name in ("h_apply", "h_aca", "h_ortho")}
+    ap = stats["h_apply"]
+    gbs = ap["bytes"] / (ap["ms"] * 1e-3) / 1e9 if ap["ms"] > 0 else 0.0
+    tfs = ap["flops"] / (ap["ms"] * 1e-3) / 1e12 if ap["ms"] > 0 else 0.0
+    log(f"build {t1 - t0:.2f}s solve {t2 - t1:.2f}s, {diags[0]['iterations']} iterations, "
+        f"apply {ap['ms'] / max(1, ap['launches']):.1f} ms ({gbs:.0f} GB/s, {tfs:.1f} TF/s)")
+    out = {
+        "metric": "H-matrix batched solve: incidences solved per second (build + GMRES)",
+        "value": len(angles) / (t2 - t0), "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": (t2 - t0) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c32 blocks / f64 GMRES", "data": "synthetic",
+        "config": {"workload": f"rigid bent capped cylinder a=2 mm, L=60 mm, R=120 mm, {mesh.n} triangles, "
+                               f"k=1000, leaf 64, eta 2, ACA 1e-6 (c32), GMRES(50) to 1e-8, "
+                               f"{len(angles)} incidences in [0, pi]"},
+        "build_s": t1 - t0, "solve_s": t2 - t1, "gmres_iterations": diags[0]["iterations"],
+        "max_relative_residual": max(d["relative_residual"] for d in diags),
+        "converged": sum(d["converged"] for d in diags),
+        "hmatrix": {key: info[key] for key in ("leaves", "dense_blocks", "lowrank_blocks", "dense_bytes",
+                                                 "lowrank_bytes", "max_rank", "mean_rank", "unconverged_blocks",
+                                                 "plan_ms", "assembly_ms", "aca_ms")},
+        "roofline": {"kernel": "compressed apply (64 right-hand sides)", "bound": "hbm", "achieved": gbs,
+                     "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0, "traffic": None,
+                     "fp64_tflops": tfs, "launches": ap["launches"],
+                     "avg_launch_ms": ap["ms"] / max(1, ap["launches"])},
+        "kernels": {kname: {"launches": v["launches"], "ms": round(v["ms"], 3)} for kname, v in stats.items()},
+        "ts_db_min_max": [min(ts), max(ts)],
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["sweep", "serve"], default="sweep")
+    ap.add_argument("--workload", choices=["sweep", "serve", "hmatrix"], default="sweep")
+    ap.add_argument("--hm-nphi", type=int, default=138)
+    ap.add_argument("--hm-rhs", type=int, default=64)
     ap.add_argument("--rates", type=float, nargs="+", default=[50.0, 200.0, 800.0])
     ap.add_argument("--requests", type=int, default=200)
-    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--lanes", type=int, default=8, help="concurrent solves (sweep) or server lanes (serve)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
@@ -204,16 +268,25 @@ def main():
     if args.workload == "serve":
         serve_main(args)
         return
+    if args.workload == "hmatrix":
+        hmatrix_main(args)
+        return
 
     import bemhip as bh
 
     ctx = bh.Context(0)
+    ctx.set_lanes(args.lanes)
     plan, meshes = sweep_plan(bh, ctx, args.points, args.max_level)
     sizes = {}
     for (_, _, L) in plan:
         sizes[meshes[L].n] = sizes.get(meshes[L].n, 0) + 1
     log(f"plan: {len(plan)} solves, panels per class {sizes}")
 
+    # load every kernel once from one thread before the concurrent lanes start
+    fl = bh.FluidFilled(G_RHO, H_C)
+    small = meshes[min(meshes)]
+    for prec in ("double", "mixed"):
+        bh.batched_sweep([small], [bh.make_problem(fl, plan[0][1], precision=prec)], [(0.0, 0.0)], ctx=ctx)
     log("FP64 reference sweep ...")
     a64, lat64, _, t64 = run_sweep(bh, ctx, plan, meshes, "double")
     log(f"FP64 sweep {t64:.2f}s ({len(plan) / t64:.2f} solves/s)")
@@ -260,6 +333,7 @@ def main():
                          f"clamped to N<={20 * 4 ** args.max_level}, Mueller 2N unknowns, mixed precision"
                          + ("" if full else " [REDUCED]")),
             "panels_per_class": {str(k): v for k, v in sorted(sizes.items())},
+            "lanes": args.lanes,
             "global_batch": len(plan),
         },
         "max_abs_dTS_dB": dts,

@@ -119,10 +119,14 @@ int bem_ctx_memory(bem_ctx ctx, uint64_t *used, uint64_t *budget);
 int bem_ctx_set_profiling(bem_ctx ctx, int32_t enable);
 /* Accumulated event time and algorithmic work of one kernel class since the
  * last reset.  Classes: "assemble", "near", "lu_panel", "lu_gemm", "lu_trsm",
- * "lu_swap", "solve", "residual", "far_field". */
+ * "lu_swap", "solve", "residual", "far_field", "h_apply" (compressed operator
+ * apply; bytes = blocks + vectors), "h_aca", "h_ortho" (GMRES Gram-Schmidt). */
 int bem_ctx_kernel_stats(bem_ctx ctx, const char *name, int64_t *launches, double *total_ms,
                          double *flops, double *bytes);
 int bem_ctx_reset_stats(bem_ctx ctx);
+/* Largest number of solves bem_sweep runs concurrently (1..8, default 8; the
+ * HBM budget may allow fewer). */
+int bem_ctx_set_lanes(bem_ctx ctx, int32_t max_lanes);
 
 /* ---- mesh generators (host only, no device needed) --------------------- */
 /* Sizes of the level-`level` icosphere: 10*4^L+2 nodes, 20*4^L triangles. */

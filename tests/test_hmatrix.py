@@ -116,3 +116,15 @@ def test_hmatrix_argument_errors(ctx, sphere4):
         bh.HMatrix(gm, bh.Rigid(), K1, max_rank=1000)
     with pytest.raises(ValueError):
         bh.HMatrix(gm, bh.Rigid(), -1.0)
+
+
+def test_solve_full_bem_compression_keyword(ctx, sphere4):
+    """compression=(method=:hmatrix,) routes solve_full_bem to the compressed path."""
+    gm, _ = sphere4
+    p, q, _, d = bh.solve_full_bem(bh.Rigid(), K1, gm, incidence_angle=0.3, compression={"method": "hmatrix"},
+                                   gmres_kwargs={"reltol": 1e-10}, return_diagnostics=True)
+    p0, q0, _ = bh.solve_full_bem(bh.Rigid(), K1, gm, incidence_angle=0.3)
+    assert d["method"] == "gmres" and d["converged"] and d["relative_residual"] <= 1e-10
+    assert np.abs(p - p0).max() / np.abs(p0).max() < 2e-5
+    with pytest.raises(ValueError):
+        bh.solve_full_bem(bh.Rigid(), K1, gm, compression={"method": "fmm"})
