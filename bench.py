@@ -343,7 +343,7 @@ def main():
         "p99_latency_ms": float(np.percentile(lats, 99)),
         "fp64_solves_per_s": len(plan) / t64,
         "speedup_vs_fp64": value / (len(plan) / t64),
-        "roofline": {"kernel": "k_gemm_sub<float> (LU trailing update)", "bound": "mfma",
+        "roofline": {"kernel": "k_gemm3m<float,0> (LU trailing update, 3M complex)", "bound": "mfma",
                      "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / MFMA_FP32_PEAK, "traffic": None,
                      "launches": g["launches"], "avg_launch_ms": g["ms"] / max(1, g["launches"])},
