@@ -671,9 +671,11 @@ def frequency_sweep(solve, frequencies, sound_speed):
     return {"frequencies": freqs, "k": ks, "ts": 20 * np.log10(np.abs(amps)), "amplitudes": amps}
 
 
-def debug_lu(A, single=False, b=None, ctx=None):
-    """Device LU of a dense complex matrix (test hook): returns (LU, 1-based ipiv[, x])."""
+def debug_lu(A, single=False, b=None, ctx=None, adjoint=False):
+    """Device LU of a dense complex matrix (test hook): returns (LU, 1-based
+    ipiv[, x]) with x solving A x = b, or A^H x = b when adjoint."""
     ctx = ctx or default_context()
+    flags = (1 if single else 0) | (2 if adjoint else 0)
     A = np.asfortranarray(np.asarray(A, dtype=np.complex128))
     n = A.shape[0]
     LU = np.empty((n, n), dtype=np.complex128, order="F")
@@ -681,10 +683,10 @@ def debug_lu(A, single=False, b=None, ctx=None):
     if b is not None:
         b = np.ascontiguousarray(b, dtype=np.complex128)
         x = np.empty(n, dtype=np.complex128)
-        ctx.check(ctx.lib.bem_debug_lu(ctx._h, n, A.ctypes.data_as(_dp), 1 if single else 0, LU.ctypes.data_as(_dp),
+        ctx.check(ctx.lib.bem_debug_lu(ctx._h, n, A.ctypes.data_as(_dp), flags, LU.ctypes.data_as(_dp),
                                        _i(piv), b.ctypes.data_as(_dp), x.ctypes.data_as(_dp)))
         return LU, piv, x
-    ctx.check(ctx.lib.bem_debug_lu(ctx._h, n, A.ctypes.data_as(_dp), 1 if single else 0, LU.ctypes.data_as(_dp),
+    ctx.check(ctx.lib.bem_debug_lu(ctx._h, n, A.ctypes.data_as(_dp), flags, LU.ctypes.data_as(_dp),
                                    _i(piv), None, None))
     return LU, piv
 

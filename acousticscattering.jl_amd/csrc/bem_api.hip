@@ -284,7 +284,7 @@ Lane* make_lane(bem_ctx c) {
     const int nodes = 2 * G + 16;
     L->ws.max_blocks = G;
     L->ws.max_nodes = nodes;
-    const size_t bytes = 64 + sizeof(int) * nodes * (2 + 64) + sizeof(int) * 80 + 8 * 256;
+    const size_t bytes = 64 + sizeof(int) * nodes * (2 + 64) + sizeof(int) * 80 + 10 * 256;
     char* m = (char*)c->alloc(bytes);
     L->ws_mem = m;
     HIPCHECK(hipMemset(m, 0, bytes));
@@ -295,7 +295,13 @@ Lane* make_lane(bem_ctx c) {
     L->ws.node_cnt = (int*)take(sizeof(int) * nodes);
     L->ws.node_rows = (int*)take(sizeof(int) * nodes * 64);
     L->ws.root = (int*)take(sizeof(int) * 80);
+    L->ws.tri_ctr = (unsigned*)take(2 * sizeof(unsigned));
+    L->ws.tri_prog = (unsigned long long*)take(2 * sizeof(unsigned long long));
     if (off > bytes) throw BemError(BEM_ERR_DEVICE, "workspace layout");
+    // a device-side wait of the triangular solves gives up after 10 s
+    int khz = 0;
+    HIPCHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+    L->ws.spin_ticks = (long long)(khz > 0 ? khz : 100000) * 10000;
     HIPCHECK(hipDeviceSynchronize());
     return L;
 }
@@ -311,6 +317,16 @@ void free_lane(bem_ctx c, Lane* L) {
     if (L->ev_ready) hipEventDestroy(L->ev_ready);
     if (L->ev_panel) hipEventDestroy(L->ev_panel);
     delete L;
+}
+
+// After a triangular solve whose device-side wait expired (ws.status[2]):
+// clear the flag and the ticket counters, then fail the request.
+void tri_check(Lane* L, int expired) {
+    if (!expired) return;
+    launch_fill_int(L->s, L->ws.status + 2, 1, 0);
+    launch_fill_int(L->s, (int*)L->ws.tri_ctr, 2, 0);
+    (void)hipStreamSynchronize(L->s);
+    throw BemError(BEM_ERR_DEVICE, "triangular solve: a device-side wait expired");
 }
 
 template <typename F>
@@ -436,12 +452,15 @@ double estimate_rcond(bem_operator op, double anorm) {
         HIPCHECK(hipMemcpyAsync(op->wv, buf.data(), sizeof(z32) * n, hipMemcpyHostToDevice, s));
         if (adjoint_of_B)
             lu_solve<float>(s, (const z32*)op->LU, n, n, op->perm, (const z32*)op->inv, (const z32*)op->wv,
-                            (z32*)op->wv2, (z32*)op->wv3, &op->lane->pl);
+                            (z32*)op->wv2, (z32*)op->wv3, op->lane->ws, &op->lane->pl);
         else
             lu_solve_h<float>(s, (const z32*)op->LU, n, n, op->perm, (const z32*)op->inv, (const z32*)op->wv,
-                              (z32*)op->wv2, (z32*)op->wv3, &op->lane->pl);
+                              (z32*)op->wv2, (z32*)op->wv3, op->lane->ws, &op->lane->pl);
+        int expired = 0;
         HIPCHECK(hipMemcpyAsync(buf.data(), op->wv3, sizeof(z32) * n, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(&expired, op->lane->ws.status + 2, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
+        tri_check(op->lane, expired);
         for (int i = 0; i < n; ++i) v[i] = cd(buf[i].x, buf[i].y);
     };
     auto norm1 = [&]() {
@@ -560,14 +579,15 @@ bool correction(bem_operator op, const z64* rhs, z64* out, int accumulate, int* 
     if (op->precision == 1) {
         launch_prep_rhs<float>(s, n, rhs, op->rows, 1.0 / mag, (z32*)op->wv);
         lu_solve<float>(s, (const z32*)op->LU, n, n, op->perm, (const z32*)op->inv, (const z32*)op->wv,
-                        (z32*)op->wv2, (z32*)op->wv3, &op->lane->pl);
+                        (z32*)op->wv2, (z32*)op->wv3, op->lane->ws, &op->lane->pl);
         launch_finish<float>(s, n, (const z32*)op->wv3, mag, op->cols, out, accumulate, changed);
     } else {
         const z64* LU = (const z64*)(op->LU64 ? op->LU64 : op->LU);
         const int* perm = op->LU64 ? op->perm64 : op->perm;
         const z64* inv = (const z64*)(op->LU64 ? op->inv64 : op->inv);
         launch_prep_rhs<double>(s, n, rhs, op->rows, 1.0 / mag, (z64*)op->wv);
-        lu_solve<double>(s, LU, n, n, perm, inv, (const z64*)op->wv, (z64*)op->wv2, (z64*)op->wv3, &op->lane->pl);
+        lu_solve<double>(s, LU, n, n, perm, inv, (const z64*)op->wv, (z64*)op->wv2, (z64*)op->wv3, op->lane->ws,
+                         &op->lane->pl);
         launch_finish<double>(s, n, (const z64*)op->wv3, mag, op->cols, out, accumulate, changed);
     }
     return true;
@@ -850,8 +870,11 @@ void solve_rhs(bem_operator op, const double* ddir, z64* P, z64* Q, bem_diagnost
     launch_norm_stats(s, n, op->vr, op->vb, op->rows, op->red);
     {
         std::vector<double> h(4 * kRedBlocks);
+        int expired = 0;
         HIPCHECK(hipMemcpyAsync(h.data(), op->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(&expired, op->lane->ws.status + 2, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
+        tri_check(op->lane, expired);
         double ss[4] = {0, 0, 0, 0};
         for (int b = 0; b < kRedBlocks; ++b)
             for (int q = 0; q < 4; ++q) ss[q] += h[4 * b + q];
@@ -2644,12 +2667,14 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t nrhs, const double* angles, bem_sol
     });
 }
 
-int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t single, double* LUout, int64_t* ipiv_out,
+int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t flags, double* LUout, int64_t* ipiv_out,
                  const double* b, double* x) {
-    if (!ctx || !A || n64 <= 0 || n64 > 65536) return BEM_ERR_ARGUMENT;
+    if (!ctx || !A || n64 <= 0 || n64 > 65536 || (flags & ~3) != 0) return BEM_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     return guarded(ctx, [&] {
         const int n = (int)n64;
+        const int single = flags & 1;
+        const bool adjoint = (flags & 2) != 0;
         hipStream_t s = ctx->lane0()->s;
         const size_t esz = single ? sizeof(z32) : sizeof(z64);
         std::vector<char> host((size_t)n * n * esz);
@@ -2698,9 +2723,22 @@ int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t single, doub
                 op.red = (double*)al(sizeof(double) * 5 * kRedBlocks);
                 op.precision = single ? 1 : 0;
                 HIPCHECK(hipMemcpyAsync(op.vb, b, sizeof(z64) * n, hipMemcpyHostToDevice, s));
-                if (!correction(&op, op.vb, op.vx, 0, nullptr)) throw BemError(BEM_ERR_ARGUMENT, "nonfinite b");
+                if (!adjoint) {
+                    if (!correction(&op, op.vb, op.vx, 0, nullptr)) throw BemError(BEM_ERR_ARGUMENT, "nonfinite b");
+                } else if (single) {
+                    launch_prep_rhs<float>(s, n, op.vb, nullptr, 1.0, (z32*)op.wv);
+                    lu_solve_h<float>(s, (const z32*)op.LU, n, n, op.perm, (const z32*)op.inv, (const z32*)op.wv,
+                                      (z32*)op.wv2, (z32*)op.wv3, op.lane->ws, &op.lane->pl);
+                    launch_finish<float>(s, n, (const z32*)op.wv3, 1.0, nullptr, op.vx, 0, nullptr);
+                } else {
+                    lu_solve_h<double>(s, (const z64*)op.LU, n, n, op.perm, (const z64*)op.inv, op.vb,
+                                       (z64*)op.wv2, op.vx, op.lane->ws, &op.lane->pl);
+                }
+                int expired = 0;
                 HIPCHECK(hipMemcpyAsync(x, op.vx, sizeof(z64) * n, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipMemcpyAsync(&expired, op.lane->ws.status + 2, sizeof(int), hipMemcpyDeviceToHost, s));
                 HIPCHECK(hipStreamSynchronize(s));
+                tri_check(op.lane, expired);
             }
         } catch (...) {
             hipStreamSynchronize(s);

@@ -127,13 +127,18 @@ void launch_matvec(hipStream_t s, const C2<R>* A, int64_t lda, int rows, int col
 
 // Workspace of the tournament-pivoting panel (one per stream).
 struct PanelWS {
-    int* status;       // [0] first zero pivot (INT_MAX if none), [1] reserved
+    int* status;       // [0] first zero pivot (INT_MAX if none), [1] inconsistent pivot selection,
+                       // [2] a triangular-solve wait expired
     int* tickets;      // [max_nodes] merge tickets, zero between launches
     int* node_cnt;     // [max_nodes] rows nominated by each tree node
     int* node_rows;    // [max_nodes][64] their global row indices
     int* root;         // [0] count, [1..65) the panel's pivot rows in order
     int max_blocks;    // largest leaf count (128 rows per leaf)
     int max_nodes;
+    unsigned* tri_ctr = nullptr;             // [2] triangular-solve tickets, zero between launches
+    unsigned long long* tri_prog = nullptr;  // [2] forward / backward progress words
+    unsigned tri_epoch = 0;                  // host: triangular solves launched on this workspace
+    long long spin_ticks = 0;                // wall-clock ticks one device-side wait may last
 };
 // Streams of one LU: main, and an optional side stream with two events for
 // the panel look-ahead (side = null: everything on main).
@@ -145,14 +150,16 @@ struct LuStreams {
 template <typename R>
 int lu_factor(const LuStreams& st, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv, PanelWS& ws,
               void* prof);
+// x = inv(A) b (one launch; ws.status[2] is raised if a device-side wait
+// expires, and x then holds NaN).  b, work and x must not overlap.
 template <typename R>
 void lu_solve(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm,
-              const C2<R>* inv, const C2<R>* b, C2<R>* work, C2<R>* x, void* prof);
+              const C2<R>* inv, const C2<R>* b, C2<R>* work, C2<R>* x, PanelWS& ws, void* prof);
 
 // x = inv(A)^H b with the same factors (conjugate-transpose solves).
 template <typename R>
 void lu_solve_h(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm, const C2<R>* inv,
-                const C2<R>* b, C2<R>* work, C2<R>* x, void* prof);
+                const C2<R>* b, C2<R>* work, C2<R>* x, PanelWS& ws, void* prof);
 
 // Refinement helpers (FP64 vectors).
 template <typename R>

@@ -16,18 +16,39 @@
 namespace bem {
 
 // ---------------------------------------------------------------------------
-// (1) T_b[l][c] = sum_j V_b[l][j] X[(col_lo + j) nr + c], 16 ranks per tile.
+// Both phases run on the FP64 matrix cores (v_mfma_f64_16x16x4f64: A[i][k]
+// from lane i + 16k, B[k][j] from lane j + 16k, result slot r of lane l at row
+// l/16 + 4r, column l%16), complex as four real products.  FP32 block values
+// are widened to FP64 as they are read: FP64 arithmetic throughout.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_hlr_t(const HBlk* __restrict__ blk, const z32* __restrict__ uv,
-                                               const HTile* __restrict__ tiles, const int64_t* __restrict__ toff,
-                                               const z64* __restrict__ X, int nr, z64* __restrict__ T) {
+typedef double dv4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dv4 mfma64(double a, double b, dv4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+// acc += a * b for complex a (A operand) and b (B operand), four real products
+__device__ __forceinline__ void cmfma64(double ar, double ai, double br, double bi, dv4& cr, dv4& ci) {
+    cr = mfma64(ar, br, cr);
+    cr = mfma64(-ai, bi, cr);
+    ci = mfma64(ar, bi, ci);
+    ci = mfma64(ai, br, ci);
+}
+
+// (1) T_b[l][c] = sum_j V_b[l][j] X[(col_lo + j) nr + c]: 16 ranks x 64
+// right-hand sides per workgroup; wave w owns columns 16w..16w+15.
+__global__ __launch_bounds__(256) void k_hlr_t_mfma(const HBlk* __restrict__ blk, const z32* __restrict__ uv,
+                                                    const HTile* __restrict__ tiles,
+                                                    const int64_t* __restrict__ toff, const z64* __restrict__ X,
+                                                    int nr, z64* __restrict__ T) {
     const HTile t = tiles[blockIdx.x];
     const HBlk b = blk[t.blk];
-    const z32* V = uv + b.off + (int64_t)b.rank * b.m;  // compact: U is m x rank
-    const int tid = threadIdx.x, c = tid & 63, lg = tid >> 6;
+    const z32* V = uv + b.off + (int64_t)b.rank * b.m;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
     const int lcnt = min(16, b.rank - t.l0);
-    __shared__ z32 sV[16][64];
-    z64 acc[4] = {};
+    const int c = 16 * w + li;  // B-operand column of this lane
+    __shared__ z32 sV[16][68];
+    dv4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
     for (int j0 = 0; j0 < b.n; j0 += 64) {
         __syncthreads();
         for (int e = tid; e < 16 * 64; e += 256) {
@@ -35,93 +56,100 @@ __global__ __launch_bounds__(256) void k_hlr_t(const HBlk* __restrict__ blk, con
             sV[l][jj] = (l < lcnt && j0 + jj < b.n) ? V[(int64_t)(t.l0 + l) * b.n + j0 + jj] : z32{0.f, 0.f};
         }
         __syncthreads();
-        if (c < nr) {
-            const int jn = min(64, b.n - j0);
-            for (int jj = 0; jj < jn; ++jj) {
-                const z64 x = X[(int64_t)(b.col_lo + j0 + jj) * nr + c];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const z32 v = sV[lg * 4 + q][jj];
-                    acc[q].x += (double)v.x * x.x - (double)v.y * x.y;
-                    acc[q].y += (double)v.x * x.y + (double)v.y * x.x;
-                }
-            }
+        const int jn = min(64, b.n - j0);
+        for (int k0 = 0; k0 < jn; k0 += 4) {
+            const int k = k0 + lk;
+            const z32 a = sV[li][k];  // A[i = rank][k = column]; zero beyond jn (staged zeros)
+            z64 x{0, 0};
+            if (k < jn && c < nr) x = X[(int64_t)(b.col_lo + j0 + k) * nr + c];
+            cmfma64(a.x, a.y, x.x, x.y, cr, ci);
         }
     }
-    if (c < nr)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int l = lg * 4 + q;
-            if (l < lcnt) T[toff[t.blk] + (int64_t)(t.l0 + l) * kHRhs + c] = acc[q];
-        }
+    for (int r = 0; r < 4; ++r) {
+        const int i = lk + 4 * r;
+        if (i < lcnt && c < nr) T[toff[t.blk] + (int64_t)(t.l0 + i) * kHRhs + c] = z64{cr[r], ci[r]};
+    }
 }
 
-// ---------------------------------------------------------------------------
-// (2) one workgroup per leaf row cluster: thread (c, row group of 16).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_hrows(const HLeafs L, const HBlk* __restrict__ blk,
-                                               const z32* __restrict__ dval, const z32* __restrict__ uv,
-                                               const int64_t* __restrict__ toff, const z64* __restrict__ T,
-                                               const z64* __restrict__ X, int nr, z64* __restrict__ Y) {
+// (2) one workgroup per leaf row cluster (dense blocks times X, then U_b T_b
+// for the low-rank blocks covering it); wave w owns rows 16w..16w+15, all 64
+// columns.
+__global__ __launch_bounds__(256) void k_hrows_mfma(const HLeafs L, const HBlk* __restrict__ blk,
+                                                    const z32* __restrict__ dval, const z32* __restrict__ uv,
+                                                    const int64_t* __restrict__ toff, const z64* __restrict__ T,
+                                                    const z64* __restrict__ X, int nr, z64* __restrict__ Y) {
     const int leaf = blockIdx.x;
     const int lo = L.lo[leaf], m = L.m[leaf];
-    const int tid = threadIdx.x, c = tid & 63, rg = tid >> 6;
-    __shared__ z32 sD[64][64];
-    z64 acc[16] = {};
-    // dense blocks of this row
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int row = 16 * w + li;  // A-operand row of this lane
+    __shared__ z32 sD[64][68];
+    dv4 cr[4], ci[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cr[q] = ci[q] = dv4{0, 0, 0, 0};
     for (int e = L.dptr[leaf]; e < L.dptr[leaf + 1]; ++e) {
         const HBlk b = blk[L.dlist[e]];
         __syncthreads();
-        for (int q = tid; q < b.m * b.n; q += 256) sD[q / b.n][q % b.n] = dval[b.off + q];
+        for (int q = tid; q < 64 * 64; q += 256) {
+            const int i = q >> 6, j = q & 63;
+            sD[i][j] = (i < b.m && j < b.n) ? dval[b.off + i * b.n + j] : z32{0.f, 0.f};
+        }
         __syncthreads();
-        if (c < nr)
-            for (int j = 0; j < b.n; ++j) {
-                const z64 x = X[(int64_t)(b.col_lo + j) * nr + c];
+        for (int k0 = 0; k0 < b.n; k0 += 4) {
+            const int k = k0 + lk;
+            const z32 a = sD[row][k];
+            const z64* xr = X + (int64_t)(b.col_lo + k) * nr;
 #pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    const z32 d = sD[rg * 16 + q][j];  // rows beyond m read stale values, never stored
-                    acc[q].x += (double)d.x * x.x - (double)d.y * x.y;
-                    acc[q].y += (double)d.x * x.y + (double)d.y * x.x;
-                }
+            for (int q = 0; q < 4; ++q) {
+                const int cc = 16 * q + li;
+                z64 x{0, 0};
+                if (k < b.n && cc < nr) x = xr[cc];
+                cmfma64(a.x, a.y, x.x, x.y, cr[q], ci[q]);
             }
+        }
     }
-    // low-rank blocks covering this row: U_b[o + i][l] T_b[l][c]
     for (int e = L.lptr[leaf]; e < L.lptr[leaf + 1]; ++e) {
         const HRef ref = L.llist[e];
         const HBlk b = blk[ref.blk];
         const z32* U = uv + b.off;
+        const z64* Tb = T + toff[ref.blk];
         for (int l0 = 0; l0 < b.rank; l0 += 64) {
             const int lc = min(64, b.rank - l0);
             __syncthreads();
             for (int q = tid; q < 64 * 64; q += 256) {
-                const int l = q >> 6, i = q & 63;
-                sD[l][i] = (l < lc && i < m) ? U[(int64_t)(l0 + l) * b.m + ref.o + i] : z32{0.f, 0.f};
+                const int l = q >> 6, i = q & 63;  // stored transposed: sD[i][l] = U[o + i][l0 + l]
+                sD[i][l] = (l < lc && i < m) ? U[(int64_t)(l0 + l) * b.m + ref.o + i] : z32{0.f, 0.f};
             }
             __syncthreads();
-            if (c < nr)
-                for (int l = 0; l < lc; ++l) {
-                    const z64 t = T[toff[ref.blk] + (int64_t)(l0 + l) * kHRhs + c];
+            for (int k0 = 0; k0 < lc; k0 += 4) {
+                const int k = k0 + lk;
+                const z32 a = sD[row][k];
+                const z64* tr = Tb + (int64_t)(l0 + k) * kHRhs;
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) {
-                        const z32 u = sD[l][rg * 16 + q];
-                        acc[q].x += (double)u.x * t.x - (double)u.y * t.y;
-                        acc[q].y += (double)u.x * t.y + (double)u.y * t.x;
-                    }
+                for (int q = 0; q < 4; ++q) {
+                    const int cc = 16 * q + li;
+                    z64 x{0, 0};
+                    if (k < lc && cc < nr) x = tr[cc];
+                    cmfma64(a.x, a.y, x.x, x.y, cr[q], ci[q]);
                 }
+            }
         }
     }
-    if (c < nr)
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int i = rg * 16 + q;
-            if (i < m) Y[(int64_t)(lo + i) * nr + c] = acc[q];
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * w + lk + 4 * r, cc = 16 * q + li;
+            if (i < m && cc < nr) Y[(int64_t)(lo + i) * nr + cc] = z64{cr[q][r], ci[q][r]};
         }
 }
 
 void launch_happly(hipStream_t s, const HLeafs& L, int ntiles, const HTile* tiles, const HBlk* blk, const z32* dval,
                    const z32* uv, const int64_t* toff, z64* T, const z64* X, int nr, z64* Y) {
-    if (ntiles > 0) hipLaunchKernelGGL(k_hlr_t, dim3(ntiles), dim3(256), 0, s, blk, uv, tiles, toff, X, nr, T);
-    hipLaunchKernelGGL(k_hrows, dim3(L.count), dim3(256), 0, s, L, blk, dval, uv, toff, T, X, nr, Y);
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_hlr_t_mfma, dim3(ntiles), dim3(256), 0, s, blk, uv, tiles, toff, X, nr, T);
+    hipLaunchKernelGGL(k_hrows_mfma, dim3(L.count), dim3(256), 0, s, L, blk, dval, uv, toff, T, X, nr, Y);
 }
 
 // ---------------------------------------------------------------------------
@@ -139,7 +167,6 @@ __global__ __launch_bounds__(256) void k_hjacobi_inv(const HLeafs L, const HBlk*
     __shared__ z64 A[64][65];
     __shared__ int piv[64];
     __shared__ double sv[4];
-    __shared__ int si[4];
     __shared__ int s_bad;
     for (int q = tid; q < 64 * 64; q += 256) {
         const int i = q >> 6, j = q & 63;

@@ -1238,163 +1238,309 @@ int lu_factor(const LuStreams& st, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R
 }
 
 // ---------------------------------------------------------------------------
-// triangular solves with the stored block inverses
+// Triangular solves with the stored diagonal-block inverses: one launch per
+// solve, forward and backward phases together.
+//
+// Workgroup t owns one 64-row block: t < nblk is forward block t, t >= nblk
+// is backward block 2 nblk - 1 - t.  The index comes from a ticket counter
+// taken on entry, so a workgroup only ever waits on workgroups that started
+// before it, and the lowest unfinished one always has its inputs: progress
+// does not depend on the dispatch order.  A block depends on every earlier
+// block of its phase, so the blocks of a phase finish in order and one word
+// per phase, (epoch << 32) | blocks finished, tells a waiter which of its
+// inputs are ready.  The epoch counts the solves run on the workspace, so
+// words left by earlier solves read as no progress and nothing is reset
+// between solves.  A wait lasts at most `limit` wall-clock ticks; the first
+// to expire raises the abort word, which releases every other waiter, so
+// every workgroup exits and the host reports the failure.  The last
+// workgroup to finish zeroes the ticket counters for the next launch.
+//
+// Solution blocks and progress words move through relaxed agent-scope
+// atomic loads and stores (coherent across XCDs); a producer's solution
+// stores have completed (s_waitcnt) before its wave stores the progress word.
 // ---------------------------------------------------------------------------
-template <typename R>
-__global__ void k_gather(int n, const int* perm, const C2<R>* b, C2<R>* y) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = b[perm[i]];
-}
+struct TriSync {
+    unsigned* ctr;             // [0] ticket, [1] finished workgroups
+    unsigned long long* prog;  // [0] forward, [1] backward progress
+    int* abort;                // nonzero once a wait has expired
+    unsigned epoch;
+    long long limit;           // wall-clock ticks one wait may last
+};
 
-// Block kb of a forward (lower = 1) or backward (lower = 0) substitution:
-//   z = inv(D_kb) y[block]; out[block] = z;
-//   y[r] -= sum_c T[r][r0 + c] z[c] for rows r after (forward) / before (backward).
-template <typename R, int LOWER>
-__global__ __launch_bounds__(256) void k_trsv_block(const C2<R>* __restrict__ T, int64_t lda, int n, int kb,
-                                                    const C2<R>* __restrict__ inv, C2<R>* __restrict__ y,
-                                                    C2<R>* __restrict__ out) {
-    __shared__ C2<R> z[kNB];
-    const int r0 = kb * kNB;
-    const int bs = (n - r0) < kNB ? (n - r0) : kNB;
-    const int tid = threadIdx.x;
-    const C2<R>* Di = inv + (size_t)(2 * kb + (LOWER ? 0 : 1)) * kNB * kNB;
-    if (tid < kNB) {
-        C2<R> v{0, 0};
-        if (tid < bs)
-            for (int c = 0; c < bs; ++c) v = cadd(v, cmul(Di[tid * kNB + c], y[r0 + c]));
-        z[tid] = v;
-        if (blockIdx.x == 0 && tid < bs) out[r0 + tid] = v;
+// Tile geometry: a 64 x 64 tile of complex R read with 16-byte loads.
+template <typename R>
+struct TriGeom {
+    static constexpr int V = 16 / (int)sizeof(C2<R>);  // complex values per load
+    static constexpr int TPR = kNB / V;                // threads per tile row
+    static constexpr int RPP = 256 / TPR;              // tile rows per pass
+    static constexpr int P = kNB / RPP;                // passes per tile
+};
+template <typename R>
+struct alignas(16) Vec16 {
+    C2<R> e[TriGeom<R>::V];
+};
+
+// A 64 x 64 tile held in registers: thread tid holds the memory elements
+// (mr + RPP p, V qg + v), mr = tid / TPR, qg = tid % TPR.  TR = 0 multiplies
+// by the tile (partial sums per tile row), TR = 1 by its conjugate transpose
+// (partial sums per tile column).  Products accumulate in FP64.
+template <typename R, int TR>
+struct TriTile {
+    using G = TriGeom<R>;
+    static constexpr int NX = TR ? G::P : G::V;  // vector entries a thread multiplies
+    static constexpr int NA = TR ? G::V : G::P;  // partial sums a thread holds
+    C2<R> t[G::P][G::V];
+
+    __device__ static int xidx(int tid, int k) {
+        return TR ? tid / G::TPR + G::RPP * k : (tid % G::TPR) * G::V + k;
     }
-    __syncthreads();
-    const int lane = tid & 63, wv = tid >> 6;
-    const int sub = lane >> 4, l16 = lane & 15;
-    const int rbeg = LOWER ? r0 + bs : 0;
-    const int rend = LOWER ? n : r0;
-    for (int base = rbeg + (blockIdx.x * 4 + wv) * 4; base < rend; base += gridDim.x * 16) {
-        const int r = base + sub;
-        double sx = 0, sy = 0;
-        if (r < rend) {
+    // rows x cols: valid extent of the tile in memory (zeros outside).
+    __device__ void load(const C2<R>* base, int64_t ld, int rows, int cols, bool vec, int tid) {
+        const int mr = tid / G::TPR, q0 = (tid % G::TPR) * G::V;
+        if (vec && rows == kNB && cols == kNB) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int c = l16 * 4 + q;
-                if (c < bs) {
-                    const C2<R> t = T[(int64_t)r * lda + r0 + c];
-                    const C2<R> zc = z[c];
-                    sx += (double)t.x * zc.x - (double)t.y * zc.y;
-                    sy += (double)t.x * zc.y + (double)t.y * zc.x;
+            for (int p = 0; p < G::P; ++p) {
+                const Vec16<R> w = *(const Vec16<R>*)(base + (int64_t)(mr + G::RPP * p) * ld + q0);
+#pragma unroll
+                for (int v = 0; v < G::V; ++v) t[p][v] = w.e[v];
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < G::P; ++p)
+#pragma unroll
+                for (int v = 0; v < G::V; ++v) {
+                    const int m = mr + G::RPP * p, q = q0 + v;
+                    t[p][v] = (m < rows && q < cols) ? base[(int64_t)m * ld + q] : C2<R>{0, 0};
+                }
+        }
+    }
+    __device__ void fma(const z64 (&xv)[NX], z64 (&acc)[NA]) const {
+#pragma unroll
+        for (int p = 0; p < G::P; ++p)
+#pragma unroll
+            for (int v = 0; v < G::V; ++v) {
+                const double tr = t[p][v].x, ti = t[p][v].y;
+                if (TR == 0) {
+                    acc[p].x += tr * xv[v].x - ti * xv[v].y;
+                    acc[p].y += tr * xv[v].y + ti * xv[v].x;
+                } else {
+                    acc[v].x += tr * xv[p].x + ti * xv[p].y;
+                    acc[v].y += tr * xv[p].y - ti * xv[p].x;
                 }
             }
-        }
+    }
+};
+
+// Sums the partial sums of all threads into out[0..64); ends with a barrier.
+template <typename R, int TR, int NA>
+__device__ __forceinline__ void tri_reduce(z64 (&acc)[NA], z64* out, z64* part, int tid) {
+    using G = TriGeom<R>;
+    if (TR == 0) {
 #pragma unroll
-        for (int off = 8; off >= 1; off >>= 1) {
-            sx += __shfl_xor(sx, off, 64);
-            sy += __shfl_xor(sy, off, 64);
+        for (int p = 0; p < G::P; ++p)
+#pragma unroll
+            for (int off = G::TPR / 2; off >= 1; off >>= 1) {
+                acc[p].x += __shfl_xor(acc[p].x, off, 64);
+                acc[p].y += __shfl_xor(acc[p].y, off, 64);
+            }
+        if (tid % G::TPR == 0)
+#pragma unroll
+            for (int p = 0; p < G::P; ++p) out[tid / G::TPR + G::RPP * p] = acc[p];
+        __syncthreads();
+    } else {
+        const int mr = tid / G::TPR, q0 = (tid % G::TPR) * G::V;
+#pragma unroll
+        for (int v = 0; v < G::V; ++v) part[mr * kNB + q0 + v] = acc[v];
+        __syncthreads();
+        if (tid < kNB) {
+            z64 a{0, 0};
+#pragma unroll
+            for (int r = 0; r < G::RPP; ++r) {
+                a.x += part[r * kNB + tid].x;
+                a.y += part[r * kNB + tid].y;
+            }
+            out[tid] = a;
         }
-        if (r < rend && l16 == 0) {
-            C2<R> v = y[r];
-            v.x -= (R)sx;
-            v.y -= (R)sy;
-            y[r] = v;
-        }
+        __syncthreads();
     }
 }
 
-// Block kb of a solve with the conjugate transpose of a triangular factor.
-// UPPER = 1: forward substitution with U^H (lower), z = inv(U_kb)^H y[block],
-//            then y[r] -= sum_s conj(U[r0+s][r]) z[s] for rows after the block.
-// UPPER = 0: backward substitution with L^H (unit upper), z = inv(L_kb)^H y[block],
-//            then y[r] -= sum_s conj(L[r0+s][r]) z[s] for rows before the block.
-// One thread per row r: the reads U[r0+s][r] are coalesced across threads.
-template <typename R, int UPPER>
-__global__ __launch_bounds__(256) void k_trsvH_block(const C2<R>* __restrict__ T, int64_t lda, int n, int kb,
-                                                     const C2<R>* __restrict__ inv, C2<R>* __restrict__ y,
-                                                     C2<R>* __restrict__ out) {
-    __shared__ C2<R> z[kNB];
-    const int r0 = kb * kNB;
-    const int bs = (n - r0) < kNB ? (n - r0) : kNB;
+// Wait until the phase word shows at least `need` finished blocks of this
+// epoch; returns the count seen, or -1 on abort or expiry (wave-uniform).
+__device__ __forceinline__ int tri_wait(const unsigned long long* w, unsigned epoch, int need, int* abortw,
+                                     long long limit) {
+    long long t0 = 0;
+    for (int it = 0;; ++it) {
+        const unsigned long long v = ld_ag(w);
+        const int cnt = (unsigned)(v >> 32) == epoch ? (int)(unsigned)(v & 0xffffffffu) : 0;
+        if (cnt >= need) return cnt;
+        if (ld_ag(abortw) != 0) return -1;
+        const long long now = wall_clock64();
+        if (it == 0) {
+            t0 = now;
+        } else if (now - t0 > limit) {
+            st_ag(abortw, 1);
+            return -1;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// ADJ = 0: x = inv(A) b for P A = L U (y = inv(L) P b, x = inv(U) y).
+// ADJ = 1: x = inv(A)^H b = P^T inv(L^H) inv(U^H) b.
+// Forward solutions go to `work`; backward ones to x (ADJ = 0) or to `work`
+// in place and x[perm[r]] (ADJ = 1).  b, work and x must not overlap.
+template <typename R, int ADJ>
+__global__ __launch_bounds__(256) void k_trsolve(const C2<R>* __restrict__ LU, int64_t lda, int n, int nblk,
+                                                 const int* __restrict__ perm, const C2<R>* __restrict__ inv,
+                                                 const C2<R>* __restrict__ b, C2<R>* work, C2<R>* x, TriSync sy,
+                                                 int vec) {
+    using G = TriGeom<R>;
+    using Tile = TriTile<R, ADJ>;
+    __shared__ z64 s_acc[kNB];
+    __shared__ z64 s_y[kNB];
+    __shared__ z64 s_part[ADJ ? G::RPP * kNB : 1];
+    __shared__ unsigned s_t;
     const int tid = threadIdx.x;
-    const C2<R>* Di = inv + (size_t)(2 * kb + (UPPER ? 1 : 0)) * kNB * kNB;
-    if (tid < kNB) {
-        C2<R> v{0, 0};
-        if (tid < bs)
-            for (int c = 0; c < bs; ++c) {
-                const C2<R> d = Di[c * kNB + tid];  // (inv^H)[tid][c] = conj(inv[c][tid])
-                const C2<R> yc = y[r0 + c];
-                v.x += d.x * yc.x + d.y * yc.y;
-                v.y += d.x * yc.y - d.y * yc.x;
+    if (tid == 0) s_t = __hip_atomic_fetch_add(&sy.ctr[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const unsigned t = s_t;
+    if (t >= gridDim.x) {
+        if (tid == 0) st_ag(sy.abort, 2);
+    } else {
+        const bool fwd = (int)t < nblk;
+        const int blk = fwd ? (int)t : 2 * nblk - 1 - (int)t;
+        const int r0 = blk * kNB;
+        const int bs = (n - r0) < kNB ? (n - r0) : kNB;
+        const int cnt = fwd ? blk : nblk - 1 - blk;
+        const C2<R>* xs = (ADJ || fwd) ? work : x;
+        const unsigned long long* pw = &sy.prog[fwd ? 0 : 1];
+        Tile D;
+        D.load(inv + (size_t)(2 * blk + (fwd != (ADJ != 0) ? 0 : 1)) * kNB * kNB, kNB, bs, bs, true, tid);
+        z64 acc[Tile::NA];
+#pragma unroll
+        for (int k = 0; k < Tile::NA; ++k) acc[k] = z64{0, 0};
+        int ready = 0;
+        bool ab = false;
+        for (int jj = 0; jj < cnt; ++jj) {
+            const int j = fwd ? jj : nblk - 1 - jj;
+            const int c0 = j * kNB;
+            const int cs = (n - c0) < kNB ? (n - c0) : kNB;
+            Tile T;
+            if (ADJ)
+                T.load(LU + (int64_t)c0 * lda + r0, lda, cs, bs, vec != 0, tid);
+            else
+                T.load(LU + (int64_t)r0 * lda + c0, lda, bs, cs, vec != 0, tid);
+            if (jj >= ready) {
+                ready = tri_wait(pw, sy.epoch, jj + 1, sy.abort, sy.limit);
+                if (ready < 0) {
+                    ab = true;
+                    break;
+                }
             }
-        z[tid] = v;
-        if (blockIdx.x == 0 && tid < bs) out[r0 + tid] = v;
+            z64 xv[Tile::NX];
+#pragma unroll
+            for (int k = 0; k < Tile::NX; ++k) {
+                const int c = Tile::xidx(tid, k);
+                xv[k] = z64{0, 0};
+                if (c < cs) {
+                    const C2<R> v = ld_ag(xs + c0 + c);
+                    xv[k] = z64{(double)v.x, (double)v.y};
+                }
+            }
+            T.fma(xv, acc);
+        }
+        // the backward phase starts from this block's forward solution
+        if (!ab && !fwd && tri_wait(&sy.prog[0], sy.epoch, blk + 1, sy.abort, sy.limit) < 0) ab = true;
+        if (__syncthreads_or(ab)) {
+            if (tid < bs) {
+                const C2<R> nan{(R)NAN, (R)NAN};
+                if (fwd)
+                    st_ag(work + r0 + tid, nan);
+                else if (ADJ)
+                    x[perm[r0 + tid]] = nan;
+                else
+                    st_ag(x + r0 + tid, nan);
+            }
+        } else {
+            tri_reduce<R, ADJ>(acc, s_acc, s_part, tid);
+            if (tid < kNB) {
+                z64 y{0, 0};
+                if (tid < bs) {
+                    C2<R> rv;
+                    if (fwd)
+                        rv = ADJ ? b[r0 + tid] : b[perm[r0 + tid]];
+                    else
+                        rv = ld_ag(work + r0 + tid);
+                    y = z64{(double)rv.x - s_acc[tid].x, (double)rv.y - s_acc[tid].y};
+                }
+                s_y[tid] = y;
+            }
+            __syncthreads();
+            z64 yv[Tile::NX];
+#pragma unroll
+            for (int k = 0; k < Tile::NX; ++k) yv[k] = s_y[Tile::xidx(tid, k)];
+#pragma unroll
+            for (int k = 0; k < Tile::NA; ++k) acc[k] = z64{0, 0};
+            D.fma(yv, acc);
+            tri_reduce<R, ADJ>(acc, s_acc, s_part, tid);
+            if (tid < kNB) {
+                if (tid < bs) {
+                    const C2<R> z{(R)s_acc[tid].x, (R)s_acc[tid].y};
+                    if (ADJ) {
+                        st_ag(work + r0 + tid, z);
+                        if (!fwd) x[perm[r0 + tid]] = z;
+                    } else {
+                        st_ag((fwd ? work : x) + r0 + tid, z);
+                    }
+                }
+                wait_mem();
+                if (tid == 0)
+                    st_ag(&sy.prog[fwd ? 0 : 1],
+                          ((unsigned long long)sy.epoch << 32) | (unsigned)(fwd ? blk + 1 : nblk - blk));
+            }
+        }
     }
     __syncthreads();
-    const int rbeg = UPPER ? r0 + bs : 0;
-    const int rend = UPPER ? n : r0;
-    for (int r = rbeg + blockIdx.x * 256 + tid; r < rend; r += gridDim.x * 256) {
-        double sx = 0, sy = 0;
-        for (int s2 = 0; s2 < bs; ++s2) {
-            const C2<R> t = T[(int64_t)(r0 + s2) * lda + r];
-            const C2<R> zc = z[s2];
-            sx += (double)t.x * zc.x + (double)t.y * zc.y;
-            sy += (double)t.x * zc.y - (double)t.y * zc.x;
+    if (tid == 0) {
+        const unsigned d = __hip_atomic_fetch_add(&sy.ctr[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (d == gridDim.x - 1) {
+            st_ag(&sy.ctr[0], 0u);
+            st_ag(&sy.ctr[1], 0u);
         }
-        C2<R> v = y[r];
-        v.x -= (R)sx;
-        v.y -= (R)sy;
-        y[r] = v;
     }
 }
 
 template <typename R>
-__global__ void k_scatter(int n, const int* perm, const C2<R>* w, C2<R>* y) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[perm[i]] = w[i];
+static void tri_launch(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm, const C2<R>* inv,
+                       const C2<R>* b, C2<R>* work, C2<R>* x, PanelWS& ws, int adj) {
+    if (n <= 0) return;
+    const int nblk = (n + kNB - 1) / kNB;
+    if (++ws.tri_epoch == 0) ws.tri_epoch = 1;
+    const TriSync sy{ws.tri_ctr, ws.tri_prog, ws.status + 2, ws.tri_epoch, ws.spin_ticks};
+    const int vec = (sizeof(C2<R>) == 16 || (lda % 2 == 0 && ((uintptr_t)LU & 15) == 0)) ? 1 : 0;
+    if (adj)
+        hipLaunchKernelGGL((k_trsolve<R, 1>), dim3(2 * nblk), dim3(256), 0, s, LU, lda, n, nblk, perm, inv, b, work,
+                           x, sy, vec);
+    else
+        hipLaunchKernelGGL((k_trsolve<R, 0>), dim3(2 * nblk), dim3(256), 0, s, LU, lda, n, nblk, perm, inv, b, work,
+                           x, sy, vec);
 }
 
-// x = inv(A)^H b for P A = L U:  A^H = U^H L^H P, so x = P^T inv(L^H) inv(U^H) b.
+// x = inv(A)^H b for P A = L U.
 template <typename R>
 void lu_solve_h(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm, const C2<R>* inv,
-                const C2<R>* b, C2<R>* work, C2<R>* x, void* prof) {
-    const int nblk = (n + kNB - 1) / kNB;
+                const C2<R>* b, C2<R>* work, C2<R>* x, PanelWS& ws, void* prof) {
     prof_begin(prof, s, CLS_SOLVE);
-    (void)hipMemcpyAsync(work, b, sizeof(C2<R>) * n, hipMemcpyDeviceToDevice, s);
-    for (int kb = 0; kb < nblk; ++kb) {
-        const int rows = n - (kb + 1) * kNB;
-        int grid = rows > 0 ? (rows + 255) / 256 : 1;
-        if (grid > 1024) grid = 1024;
-        hipLaunchKernelGGL((k_trsvH_block<R, 1>), dim3(grid), dim3(256), 0, s, LU, lda, n, kb, inv, work, x);
-    }
-    (void)hipMemcpyAsync(work, x, sizeof(C2<R>) * n, hipMemcpyDeviceToDevice, s);
-    for (int kb = nblk - 1; kb >= 0; --kb) {
-        const int rows = kb * kNB;
-        int grid = rows > 0 ? (rows + 255) / 256 : 1;
-        if (grid > 1024) grid = 1024;
-        hipLaunchKernelGGL((k_trsvH_block<R, 0>), dim3(grid), dim3(256), 0, s, LU, lda, n, kb, inv, work, x);
-    }
-    hipLaunchKernelGGL((k_scatter<R>), dim3((n + 255) / 256), dim3(256), 0, s, n, perm, x, work);
-    (void)hipMemcpyAsync(x, work, sizeof(C2<R>) * n, hipMemcpyDeviceToDevice, s);
+    tri_launch<R>(s, LU, lda, n, perm, inv, b, work, x, ws, 1);
     prof_end(prof, s, CLS_SOLVE, 8.0 * (double)n * n, (double)n * n * sizeof(C2<R>));
 }
 
 template <typename R>
 void lu_solve(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm, const C2<R>* inv,
-              const C2<R>* b, C2<R>* work, C2<R>* x, void* prof) {
-    const int nblk = (n + kNB - 1) / kNB;
+              const C2<R>* b, C2<R>* work, C2<R>* x, PanelWS& ws, void* prof) {
     prof_begin(prof, s, CLS_SOLVE);
-    hipLaunchKernelGGL((k_gather<R>), dim3((n + 255) / 256), dim3(256), 0, s, n, perm, b, work);
-    for (int kb = 0; kb < nblk; ++kb) {
-        const int rows = n - (kb + 1) * kNB;
-        int grid = rows > 0 ? (rows + 15) / 16 : 1;
-        if (grid > 1024) grid = 1024;
-        hipLaunchKernelGGL((k_trsv_block<R, 1>), dim3(grid), dim3(256), 0, s, LU, lda, n, kb, inv, work, x);
-    }
-    // backward: x holds L^{-1} P b; reuse `work` as the running right-hand side
-    (void)hipMemcpyAsync(work, x, sizeof(C2<R>) * n, hipMemcpyDeviceToDevice, s);
-    for (int kb = nblk - 1; kb >= 0; --kb) {
-        const int rows = kb * kNB;
-        int grid = rows > 0 ? (rows + 15) / 16 : 1;
-        if (grid > 1024) grid = 1024;
-        hipLaunchKernelGGL((k_trsv_block<R, 0>), dim3(grid), dim3(256), 0, s, LU, lda, n, kb, inv, work, x);
-    }
+    tri_launch<R>(s, LU, lda, n, perm, inv, b, work, x, ws, 0);
     prof_end(prof, s, CLS_SOLVE, 8.0 * (double)n * n, (double)n * n * sizeof(C2<R>));
 }
 
@@ -1415,9 +1561,9 @@ template void launch_finish<double>(hipStream_t, int, const z64*, double, const 
 template void launch_finish<float>(hipStream_t, int, const z32*, double, const double*, z64*, int, int*);
 template int lu_factor<double>(const LuStreams&, z64*, int64_t, int, int*, z64*, PanelWS&, void*);
 template int lu_factor<float>(const LuStreams&, z32*, int64_t, int, int*, z32*, PanelWS&, void*);
-template void lu_solve<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, const z64*, z64*, z64*, void*);
-template void lu_solve<float>(hipStream_t, const z32*, int64_t, int, const int*, const z32*, const z32*, z32*, z32*, void*);
-template void lu_solve_h<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, const z64*, z64*, z64*, void*);
-template void lu_solve_h<float>(hipStream_t, const z32*, int64_t, int, const int*, const z32*, const z32*, z32*, z32*, void*);
+template void lu_solve<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, const z64*, z64*, z64*, PanelWS&, void*);
+template void lu_solve<float>(hipStream_t, const z32*, int64_t, int, const int*, const z32*, const z32*, z32*, z32*, PanelWS&, void*);
+template void lu_solve_h<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, const z64*, z64*, z64*, PanelWS&, void*);
+template void lu_solve_h<float>(hipStream_t, const z32*, int64_t, int, const int*, const z32*, const z32*, z32*, z32*, PanelWS&, void*);
 
 }  // namespace bem

@@ -312,13 +312,14 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t n_rhs, const double *angles, bem_so
 
 /* ---- test hook: dense LU of a caller matrix ------------------------------
  * Factors the column-major n x n complex matrix A on the device with the
- * library's LU (no equilibration; FP32 when single != 0) and returns the
- * factors column-major in LU (unit-lower L below the diagonal, U on and above)
- * and the 1-based pivot rows in ipiv, the LAPACK zgetrf/cgetrf convention the
- * reference relies on (src/solvers/bem/full.jl:404).  When b and x are
- * non-null it also solves A x = b with those factors.  No counterpart in the
- * reference: it exists so tests can check the factorization by itself. */
-int bem_debug_lu(bem_ctx ctx, int64_t n, const double *A, int32_t single, double *LU, int64_t *ipiv,
+ * library's LU (no equilibration; FP32 when bit 0 of flags is set) and returns
+ * the factors column-major in LU (unit-lower L below the diagonal, U on and
+ * above) and the 1-based pivot rows in ipiv, the LAPACK zgetrf/cgetrf
+ * convention the reference relies on (src/solvers/bem/full.jl:404).  When b
+ * and x are non-null it also solves A x = b with those factors, or
+ * A^H x = b when bit 1 of flags is set.  No counterpart in the reference: it
+ * exists so tests can check the factorization and the solves by themselves. */
+int bem_debug_lu(bem_ctx ctx, int64_t n, const double *A, int32_t flags, double *LU, int64_t *ipiv,
                  const double *b, double *x);
 
 #ifdef __cplusplus

@@ -78,9 +78,17 @@ def test_device_lu(ctx, n):
         assert np.array_equal(piv, piv0 + 1)
         assert np.abs(LU - lu0).max() < 1e-10 * np.abs(lu0).max()
     LU32, piv32, x32 = bh.debug_lu(A, single=True, b=b, ctx=ctx)
-    x_lapack32 = sla.lu_solve(sla.lu_factor(A.astype(np.complex64)), b.astype(np.complex64))
+    lu32 = sla.lu_factor(A.astype(np.complex64))
+    x_lapack32 = sla.lu_solve(lu32, b.astype(np.complex64))
     ref32 = np.abs(x_lapack32 - xs).max()
     assert np.abs(x32 - xs).max() < 20 * ref32 + 1e-6 * np.abs(xs).max()
+    # conjugate-transpose solves with the same factors (the rcond estimator's B^H products)
+    _, _, xh = bh.debug_lu(A, b=b, ctx=ctx, adjoint=True)
+    xhs = np.linalg.solve(A.conj().T, b)
+    assert np.abs(xh - xhs).max() < 1e-9 * np.abs(xhs).max()
+    _, _, xh32 = bh.debug_lu(A, single=True, b=b, ctx=ctx, adjoint=True)
+    refh32 = np.abs(sla.lu_solve(lu32, b.astype(np.complex64), trans=2) - xhs).max()
+    assert np.abs(xh32 - xhs).max() < 20 * refh32 + 1e-6 * np.abs(xhs).max()
 
 
 @pytest.mark.parametrize("n", [300, 1000, 2600])
