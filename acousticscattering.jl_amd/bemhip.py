@@ -70,12 +70,16 @@ class Diagnostics(ctypes.Structure):
                 ("componentwise_backward_error", ctypes.c_double),
                 ("single_reciprocal_condition", ctypes.c_double),
                 ("assembly_ms", ctypes.c_double), ("factor_ms", ctypes.c_double),
-                ("solve_ms", ctypes.c_double), ("iterations", ctypes.c_int32), ("method", ctypes.c_int32)]
+                ("solve_ms", ctypes.c_double), ("iterations", ctypes.c_int32), ("method", ctypes.c_int32),
+                ("derivative_evaluation", ctypes.c_int32)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["factor_precision"] = "double" if self.factor_precision == 0 else "single"
         d["method"] = ["direct", "mixed_refinement", "gmres"][self.method]
+        # full.jl:336-339: how K' and H of each medium were evaluated (Mueller only)
+        ev = {1: "direct", 2: "calderon"}.get(self.derivative_evaluation)
+        d["derivative_evaluation"] = None if ev is None else {"exterior": ev, "interior": ev}
         d["refinement_reason"] = REASONS[self.refinement_reason]
         d["converged"] = bool(self.converged)
         return d

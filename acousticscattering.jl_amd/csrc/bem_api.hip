@@ -242,6 +242,7 @@ struct bem_operator_s {
     int* perm = nullptr;
     void* inv = nullptr;
     int* ipiv64 = nullptr;
+    bool calderon = false;        // low-frequency Mueller: K', H from the Calderon identities
     int* perm64 = nullptr;
     void* inv64 = nullptr;
     double* rows = nullptr;       // equilibration (null when off)
@@ -433,6 +434,86 @@ void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) 
     }
     HIPCHECK(hipMemcpyAsync(perm, pm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
+}
+
+// Low-frequency Mueller systems take their derivative blocks from the
+// Calderon identities S K' = D S and S H = D D - I/4 (full.jl:187-207,
+// 319-339): when max(k, k_int) * radius <= pi/2, with radius the largest
+// distance of a collocation point from their mean (quadrature/fluid.jl:1-7;
+// the collocation points here are the panel centroids).
+double collocation_radius(const HostMesh& h) {
+    double c[3] = {0, 0, 0};
+    for (int64_t j = 0; j < h.nt; ++j)
+        for (int d = 0; d < 3; ++d) c[d] += h.cen[3 * j + d];
+    for (int d = 0; d < 3; ++d) c[d] /= (double)h.nt;
+    double r = 0;
+    for (int64_t j = 0; j < h.nt; ++j) {
+        const double dx = h.cen[3 * j] - c[0], dy = h.cen[3 * j + 1] - c[1], dz = h.cen[3 * j + 2] - c[2];
+        r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    return r;
+}
+bool calderon_applies(const HostMesh& h, const bem_problem& p) {
+    if (p.boundary != BEM_FLUID || p.formulation != BEM_MULLER) return false;
+    const double kmax = std::max(p.k, p.k / p.soundspeed_contrast);
+    return kmax * collocation_radius(h) <= M_PI / 2;
+}
+// Device bytes of the Calderon workspace: S, D and [K' | H] of both media in
+// FP64, the pivots and the block inverses of S.
+size_t calderon_bytes(int64_t nt) {
+    return sizeof(z64) * 6 * (size_t)nt * nt + sizeof(int) * nt + sizeof(z64) * 2 * 4096 * (size_t)((nt + 63) / 64) +
+           8 * 256;
+}
+
+// Overwrites the bottom block row of op->A (assembled with direct K' and H)
+// with [-He + Hi / g, I + K'e - K'i] from the Calderon blocks of each medium:
+// M = [D S | D D - I/4], then M = inv(S) M with the device LU of S.
+template <typename R>
+void calderon_blocks(bem_operator op) {
+    Lane* L = op->lane;
+    hipStream_t s = L->s;
+    const int nt = op->nt;
+    const bem_problem& pr = op->prob;
+    const size_t sq = (size_t)nt * nt;
+    z64* S = (z64*)op_alloc(op, sizeof(z64) * sq);
+    z64* D = (z64*)op_alloc(op, sizeof(z64) * sq);
+    z64* M[2] = {(z64*)op_alloc(op, sizeof(z64) * 2 * sq), (z64*)op_alloc(op, sizeof(z64) * 2 * sq)};
+    int* ipiv = (int*)op_alloc(op, sizeof(int) * nt);
+    z64* inv = (z64*)op_alloc(op, sizeof(z64) * 2 * 4096 * (size_t)((nt + 63) / 64));
+    const int64_t ld2 = 2 * (int64_t)nt;
+    for (int med = 0; med < 2; ++med) {
+        AsmParams ap;
+        ap.kind = ASM_SD;
+        ap.k = med == 0 ? pr.k : pr.k / pr.soundspeed_contrast;
+        ap.kint = ap.k;
+        ap.g = 1.0;
+        ap.ld = nt;
+        launch_assemble<double>(s, op->mesh->d, ap, S, D);
+        launch_near_self<double>(s, op->mesh->d, ap, S, D);
+        gemm_assign<double>(s, D, nt, S, nt, M[med], ld2, nt, nt, nt);
+        gemm_assign<double>(s, D, nt, D, nt, M[med] + nt, ld2, nt, nt, nt);
+        launch_add_diag(s, nt, M[med] + nt, ld2, -0.25);
+        launch_fill_int(s, L->ws.status, 1, INT_MAX);
+        launch_fill_int(s, L->ws.status + 1, 1, 0);
+        LuStreams st;
+        st.main = s;
+        st.side = L->s2;
+        st.ready = L->ev_ready;
+        st.panel = L->ev_panel;
+        const int rc = lu_factor<double>(st, S, nt, nt, ipiv, inv, L->ws, &L->pl);
+        if (rc == 2) throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (262144 rows)");
+        if (rc != 0) throw BemError(BEM_ERR_DEVICE, "LU panel launch failed");
+        int status[2];
+        HIPCHECK(hipMemcpyAsync(status, L->ws.status, sizeof status, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        if (status[1] != 0) throw BemError(BEM_ERR_DEVICE, "LU panel pivot selection was inconsistent");
+        if (status[0] != INT_MAX)
+            throw BemError(BEM_ERR_NUMERIC, "singular single-layer operator in the Calderon reconstruction:" +
+                                                std::to_string(status[0]));
+        lu_solve_many<double>(s, S, nt, nt, ipiv, inv, M[med], ld2, 2 * nt, &L->pl);
+    }
+    launch_calderon_bottom<R>(s, nt, M[0], M[1], pr.density_contrast, (C2<R>*)op->A);
+    HIPCHECK(hipGetLastError());
 }
 
 // Reciprocal condition number in the infinity norm of the equilibrated
@@ -740,6 +821,11 @@ void operator_create(bem_ctx ctx, Lane* lane, bem_mesh mesh, const bem_problem* 
         if (op->single) launch_near_self<float>(s, mesh->d, ap, (z32*)o0, (z32*)o1);
         else launch_near_self<double>(s, mesh->d, ap, (z64*)o0, (z64*)o1);
         prof_end(&op->lane->pl, s, C_NEAR, (double)mesh->d.nnear * kNearPts * (fluid ? 2.0 : 1.0), 0.0);
+        op->calderon = calderon_applies(mesh->h, *prob);
+        if (op->calderon) {
+            if (op->single) calderon_blocks<float>(op);
+            else calderon_blocks<double>(op);
+        }
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipStreamSynchronize(s));
         const double t1 = now_ms();
@@ -863,6 +949,7 @@ void solve_rhs(bem_operator op, const double* ddir, z64* P, z64* Q, bem_diagnost
     }
     dg.factor_precision = op->precision;
     dg.method = pr.precision == BEM_MIXED ? 1 : 0;
+    dg.derivative_evaluation = (fluid && pr.formulation == BEM_MULLER) ? (op->calderon ? 2 : 1) : 0;
     dg.refinement_reason = op->reason;
     dg.single_reciprocal_condition = op->rcond;
     // residual report on the original equations: r = b - A x
@@ -957,6 +1044,7 @@ size_t sweep_need(const bem_mesh m, const bem_problem& p) {
     b += esz * 2 * 4096 * ((n + 63) / 64);                           // block inverses
     b += sizeof(z64) * n * 8 + sizeof(double) * n * 3 + sizeof(int) * n * 2;
     b += sizeof(z64) * (2 * nt + kFarBlocks + 4) + 64 * 1024;
+    if (calderon_applies(m->h, p)) b += calderon_bytes(m->d.nt);
     return b;
 }
 

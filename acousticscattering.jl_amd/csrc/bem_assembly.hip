@@ -98,7 +98,7 @@ __device__ __forceinline__ void kpoint(R k, R rx, R ry, R rz, R nx0, R nx1, R nx
 
 template <int KIND>
 struct Need {
-    static constexpr int v = (KIND == ASM_BM || KIND == ASM_MULLER) ? 15 : 3;
+    static constexpr int v = (KIND == ASM_BM || KIND == ASM_MULLER) ? 15 : 3;  // operator bits S, D, K', H
     static constexpr bool two_media = (KIND == ASM_MULLER || KIND == ASM_FLUID_CBIE);
 };
 
@@ -118,6 +118,9 @@ __device__ __forceinline__ void store_entries(C2<R>* __restrict__ out0, C2<R>* _
         double h = diag ? 0.5 : 0.0;
         out0[(int64_t)i * ld + j] = C2<R>{R(h - e.D.x), R(-e.D.y)};
         out1[(int64_t)i * ld + j] = C2<R>{R(e.S.x), R(e.S.y)};
+    } else if (KIND == ASM_SD) {
+        out0[(int64_t)i * ld + j] = C2<R>{R(e.S.x), R(e.S.y)};
+        out1[(int64_t)i * ld + j] = C2<R>{R(e.D.x), R(e.D.y)};
     } else if (KIND == ASM_MULLER) {
         // [d - De + Di, Se - g Si; -He + Hi/g, d + K'e - K'i]
         double d1 = diag ? 1.0 : 0.0, ig = 1.0 / g;
@@ -312,6 +315,7 @@ void launch_assemble(hipStream_t s, const DevMesh& m, const AsmParams& p, C2<R>*
     case ASM_BM: launch_assemble_k<R, ASM_BM>(s, m, p, o0, o1); break;
     case ASM_CBIE_RS: launch_assemble_k<R, ASM_CBIE_RS>(s, m, p, o0, o1); break;
     case ASM_MULLER: launch_assemble_k<R, ASM_MULLER>(s, m, p, o0, o1); break;
+    case ASM_SD: launch_assemble_k<R, ASM_SD>(s, m, p, o0, o1); break;
     default: launch_assemble_k<R, ASM_FLUID_CBIE>(s, m, p, o0, o1); break;
     }
 }
@@ -331,6 +335,7 @@ void launch_near_self(hipStream_t s, const DevMesh& m, const AsmParams& p, C2<R>
     case ASM_BM: launch_near_self_k<R, ASM_BM>(s, m, p, o0, o1); break;
     case ASM_CBIE_RS: launch_near_self_k<R, ASM_CBIE_RS>(s, m, p, o0, o1); break;
     case ASM_MULLER: launch_near_self_k<R, ASM_MULLER>(s, m, p, o0, o1); break;
+    case ASM_SD: launch_near_self_k<R, ASM_SD>(s, m, p, o0, o1); break;
     default: launch_near_self_k<R, ASM_FLUID_CBIE>(s, m, p, o0, o1); break;
     }
 }

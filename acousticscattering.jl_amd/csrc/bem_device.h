@@ -58,6 +58,7 @@ enum AsmKind : int {
     ASM_CBIE_RS = 1,     // out0 = 1/2 I - D       ; out1 = S
     ASM_MULLER = 2,      // out0 = 2N x 2N Mueller system
     ASM_FLUID_CBIE = 3,  // out0 = 2N x 2N CBIE fluid system
+    ASM_SD = 4,          // out0 = S, out1 = D (one medium, wavenumber k)
 };
 
 struct AsmParams {
@@ -155,6 +156,25 @@ int lu_factor(const LuStreams& st, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R
 template <typename R>
 void lu_solve(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* perm,
               const C2<R>* inv, const C2<R>* b, C2<R>* work, C2<R>* x, PanelWS& ws, void* prof);
+
+// C = A B (complex, row-major).  C may alias B when M <= 128: a workgroup
+// reads all of its B columns before it writes its C tile, and a single
+// tile row means no other workgroup reads those columns.
+template <typename R>
+void gemm_assign(hipStream_t s, const C2<R>* A, int64_t lda, const C2<R>* B, int64_t ldb, C2<R>* C, int64_t ldc,
+                 int M, int N, int K);
+// X = inv(A) X for P A = L U and ncols right-hand sides (row-major n x ncols,
+// leading dimension ldx), in place.
+template <typename R>
+void lu_solve_many(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* ipiv, const C2<R>* inv, C2<R>* X,
+                   int64_t ldx, int ncols, void* prof);
+// X[i][i] += v for i < n.
+void launch_add_diag(hipStream_t s, int n, z64* X, int64_t ldx, double v);
+// Bottom block row of the low-frequency Mueller system from the Calderon
+// blocks of both media: Me = [K'e | He], Mi = [K'i | Hi] (nt x 2nt each):
+// A[nt + i][j] = -He + Hi / g, A[nt + i][nt + j] = d_ij + K'e - K'i.
+template <typename R>
+void launch_calderon_bottom(hipStream_t s, int nt, const z64* Me, const z64* Mi, double g, C2<R>* A);
 
 // x = inv(A)^H b with the same factors (conjugate-transpose solves).
 template <typename R>

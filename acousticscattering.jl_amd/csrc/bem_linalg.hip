@@ -17,6 +17,7 @@
 //   gemm    A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32),
 //           complex as four real products, 128x128 tiles staged in LDS
 // Solves use the stored block inverses: one launch per 64-row block.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <vector>
@@ -981,99 +982,130 @@ __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A
             }
 }
 
-// C -= A B (or C = A B), complex, row-major; 3M products on MFMA (Gauss:
+// C -= A B (or C = A B), complex FP32, row-major; 3M products on MFMA (Gauss:
 // T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi); Re = T1 - T2,
 // Im = T3 - T1 - T2), 25 % fewer matrix-core products than four real GEMMs.
 // 512 threads = 8 waves as 2 (rows) x 4 (columns), each wave 64 x 32 (4 x 2
-// MFMA tiles); K staged in chunks of KC through double-buffered LDS planes
-// (re, im, re + im; rows padded to 144 so an operand read hits distinct banks),
-// one barrier per chunk, the next chunk's global loads overlapping the MFMAs.
-template <typename R>
-struct G3;
-template <>
-struct G3<float> {
-    static constexpr int KC = 16;
-};
-template <>
-struct G3<double> {
-    static constexpr int KC = 8;
-};
-constexpr int kG3Pad = 144;
+// MFMA tiles).  K runs in chunks of 32 through double-buffered LDS holding the
+// re and im planes of both operands k-contiguous (A as [m][k], B as [n][k],
+// rows padded to 36 floats so a wave's 16-byte reads hit distinct banks).  A
+// lane reads four consecutive k of its row with one 16-byte LDS read; the
+// four MFMA steps of a 16-wide k group take one of the four each (MFMA step t
+// covers k = 4q + t, q = 0..3, for A and B alike), and Ar + Ai, Br + Bi are
+// formed in registers.  One barrier per 32-wide chunk; the next chunk's
+// global loads overlap the chunk's 192 MFMAs per wave.  Workgroups are mapped to tiles so that each XCD (workgroups are
+// dealt to the 8 XCDs round-robin) owns a contiguous range of tiles, walked in
+// groups of 8 tile rows, so the A and B panels its CUs share stay in its L2.
+constexpr int kG3K = 32;       // K chunk
+constexpr int kG3P = 36;       // LDS row pitch (floats)
+constexpr int kG3Group = 8;    // tile rows per group
 
-template <typename R, int ASSIGN>
-__global__ __launch_bounds__(512, 1) void k_gemm3m(const C2<R>* __restrict__ A, int64_t lda,
-                                                   const C2<R>* __restrict__ B, int64_t ldb, C2<R>* __restrict__ C,
-                                                   int64_t ldc, int M, int N, int K) {
-    typedef typename Mfma<R>::v4 v4;
-    constexpr int KC = G3<R>::KC;
-    constexpr int PER = KC * 128 / 512;  // elements of each operand a thread stages per chunk
-    __shared__ R sA[2][3][KC][kG3Pad];
-    __shared__ R sB[2][3][KC][kG3Pad];
+__device__ __forceinline__ void g3_tile(int nx, int ny, int& tm, int& tn) {
+    const int L = blockIdx.y * nx + blockIdx.x;
+    const int T = nx * ny;
+    const int q = T >> 3, r = T & 7, xcd = L & 7, idx = L >> 3;
+    const int lin = xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+    const int per = kG3Group * nx;
+    const int g = lin / per, first = g * kG3Group;
+    const int gs = (ny - first) < kG3Group ? (ny - first) : kG3Group;
+    const int w = lin - g * per;
+    tm = first + w % gs;
+    tn = w / gs;
+}
+
+template <int ASSIGN>
+__global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, int64_t lda, const z32* __restrict__ B,
+                                                   int64_t ldb, z32* __restrict__ C, int64_t ldc, int M, int N,
+                                                   int K) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    constexpr int KV = kG3P / 4;  // v4 slots per LDS row
+    __shared__ v4 sA[2][2][kGM][KV];
+    __shared__ v4 sB[2][2][kGN][KV];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = (wv >> 2) * 64, wn = (wv & 3) * 32;
-    const int m0 = blockIdx.y * kGM, n0 = blockIdx.x * kGN;
+    int tm, tn;
+    g3_tile(gridDim.x, gridDim.y, tm, tn);
+    const int m0 = tm * kGM, n0 = tn * kGN;
+    const bool full = m0 + kGM <= M && n0 + kGN <= N;
     v4 t1[4][2], t2[4][2], t3[4][2];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) t1[x][y] = t2[x][y] = t3[x][y] = v4{0, 0, 0, 0};
-    const int lm = tid & 127, lk = (tid >> 7) * PER;
-    C2<R> ra[PER], rb[PER];
+    // staging: A row m0 + lm and B column n0 + lm, k = kc + lk .. kc + lk + 7
+    constexpr int PT = kG3K / 4;  // k values per thread and chunk
+    const int lm = tid & 127, lk = (tid >> 7) * PT;
+    const z32* pa = A + (int64_t)(m0 + lm) * lda + lk;
+    const z32* pb = B + (int64_t)lk * ldb + n0 + lm;
+    z32 ra[PT], rb[PT];
     auto gload = [&](int kc) {
-        const int gm = m0 + lm, gn = n0 + lm;
+        if (full && kc + kG3K <= K) {
 #pragma unroll
-        for (int t = 0; t < PER; ++t) {
-            const int gk = kc + lk + t;
-            ra[t] = (gm < M && gk < K) ? A[(int64_t)gm * lda + gk] : C2<R>{0, 0};
-            rb[t] = (gn < N && gk < K) ? B[(int64_t)gk * ldb + gn] : C2<R>{0, 0};
+            for (int t = 0; t < PT; ++t) {
+                ra[t] = pa[kc + t];
+                rb[t] = pb[(int64_t)(kc + t) * ldb];
+            }
+        } else {
+            const bool okm = m0 + lm < M, okn = n0 + lm < N;
+#pragma unroll
+            for (int t = 0; t < PT; ++t) {
+                const int gk = kc + lk + t;
+                ra[t] = (okm && gk < K) ? pa[kc + t] : z32{0.f, 0.f};
+                rb[t] = (okn && gk < K) ? pb[(int64_t)(kc + t) * ldb] : z32{0.f, 0.f};
+            }
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int t = 0; t < PER; ++t) {
-            sA[buf][0][lk + t][lm] = ra[t].x;
-            sA[buf][1][lk + t][lm] = ra[t].y;
-            sA[buf][2][lk + t][lm] = ra[t].x + ra[t].y;
-            sB[buf][0][lk + t][lm] = rb[t].x;
-            sB[buf][1][lk + t][lm] = rb[t].y;
-            sB[buf][2][lk + t][lm] = rb[t].x + rb[t].y;
+        for (int h = 0; h < PT / 4; ++h) {
+            const int q = (lk >> 2) + h;
+            sA[buf][0][lm][q] = v4{ra[4 * h].x, ra[4 * h + 1].x, ra[4 * h + 2].x, ra[4 * h + 3].x};
+            sA[buf][1][lm][q] = v4{ra[4 * h].y, ra[4 * h + 1].y, ra[4 * h + 2].y, ra[4 * h + 3].y};
+            sB[buf][0][lm][q] = v4{rb[4 * h].x, rb[4 * h + 1].x, rb[4 * h + 2].x, rb[4 * h + 3].x};
+            sB[buf][1][lm][q] = v4{rb[4 * h].y, rb[4 * h + 1].y, rb[4 * h + 2].y, rb[4 * h + 3].y};
         }
     };
     gload(0);
     lstore(0);
     __syncthreads();
+    const int li = lane & 15, lq = lane >> 4;
     int buf = 0;
-    for (int kc = 0; kc < K; kc += KC) {
-        const bool more = kc + KC < K;
-        if (more) gload(kc + KC);
+    for (int kc = 0; kc < K; kc += kG3K) {
+        const bool more = kc + kG3K < K;
+        if (more) gload(kc + kG3K);
 #pragma unroll
-        for (int kk = 0; kk < KC; kk += 4) {
-            const int kr = kk + (lane >> 4), cl = lane & 15;
-            R a0[4], a1[4], a2[4], b0[2], b1[2], b2[2];
+        for (int h = 0; h < kG3K / 16; ++h) {
+            v4 ar[4], ai[4], br[2], bi[2];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                a0[q] = sA[buf][0][kr][wm + q * 16 + cl];
-                a1[q] = sA[buf][1][kr][wm + q * 16 + cl];
-                a2[q] = sA[buf][2][kr][wm + q * 16 + cl];
+            for (int x = 0; x < 4; ++x) {
+                ar[x] = sA[buf][0][wm + 16 * x + li][4 * h + lq];
+                ai[x] = sA[buf][1][wm + 16 * x + li][4 * h + lq];
             }
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                b0[q] = sB[buf][0][kr][wn + q * 16 + cl];
-                b1[q] = sB[buf][1][kr][wn + q * 16 + cl];
-                b2[q] = sB[buf][2][kr][wn + q * 16 + cl];
+            for (int y = 0; y < 2; ++y) {
+                br[y] = sB[buf][0][wn + 16 * y + li][4 * h + lq];
+                bi[y] = sB[buf][1][wn + 16 * y + li][4 * h + lq];
             }
 #pragma unroll
-            for (int x = 0; x < 4; ++x)
+            for (int t = 0; t < 4; ++t) {
 #pragma unroll
-                for (int y = 0; y < 2; ++y) t1[x][y] = Mfma<R>::op(a0[x], b0[y], t1[x][y]);
+                for (int x = 0; x < 4; ++x)
 #pragma unroll
-            for (int x = 0; x < 4; ++x)
+                    for (int y = 0; y < 2; ++y) t1[x][y] = Mfma<float>::op(ar[x][t], br[y][t], t1[x][y]);
 #pragma unroll
-                for (int y = 0; y < 2; ++y) t2[x][y] = Mfma<R>::op(a1[x], b1[y], t2[x][y]);
+                for (int x = 0; x < 4; ++x)
 #pragma unroll
-            for (int x = 0; x < 4; ++x)
+                    for (int y = 0; y < 2; ++y) t2[x][y] = Mfma<float>::op(ai[x][t], bi[y][t], t2[x][y]);
+                float as[4], bs[2];
 #pragma unroll
-                for (int y = 0; y < 2; ++y) t3[x][y] = Mfma<R>::op(a2[x], b2[y], t3[x][y]);
+                for (int x = 0; x < 4; ++x) as[x] = ar[x][t] + ai[x][t];
+#pragma unroll
+                for (int y = 0; y < 2; ++y) bs[y] = br[y][t] + bi[y][t];
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) t3[x][y] = Mfma<float>::op(as[x], bs[y], t3[x][y]);
+            }
         }
         if (more) lstore(buf ^ 1);
         __syncthreads();
@@ -1085,15 +1117,15 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const C2<R>* __restrict__ A, 
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm + x * 16 + Mfma<R>::row(lane, r);
-                const int gn = n0 + wn + y * 16 + (lane & 15);
+                const int gm = m0 + wm + x * 16 + Mfma<float>::row(lane, r);
+                const int gn = n0 + wn + y * 16 + li;
                 if (gm < M && gn < N) {
-                    const R re = t1[x][y][r] - t2[x][y][r];
-                    const R im = t3[x][y][r] - t1[x][y][r] - t2[x][y][r];
+                    const float re = t1[x][y][r] - t2[x][y][r];
+                    const float im = t3[x][y][r] - t1[x][y][r] - t2[x][y][r];
                     if (ASSIGN) {
-                        C[(int64_t)gm * ldc + gn] = C2<R>{re, im};
+                        C[(int64_t)gm * ldc + gn] = z32{re, im};
                     } else {
-                        C2<R> c = C[(int64_t)gm * ldc + gn];
+                        z32 c = C[(int64_t)gm * ldc + gn];
                         c.x -= re;
                         c.y -= im;
                         C[(int64_t)gm * ldc + gn] = c;
@@ -1121,7 +1153,8 @@ template <typename R, int ASSIGN>
 static void launch_gemm(hipStream_t s, dim3 grid, const C2<R>* A, int64_t lda, const C2<R>* B, int64_t ldb, C2<R>* C,
                         int64_t ldc, int M, int N, int K) {
     if constexpr (sizeof(R) == sizeof(float))
-        hipLaunchKernelGGL((k_gemm3m<R, ASSIGN>), grid, dim3(512), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
+        hipLaunchKernelGGL((k_gemm3m<ASSIGN>), grid, dim3(512), 0, s, (const z32*)A, lda, (const z32*)B, ldb,
+                           (z32*)C, ldc, M, N, K);
     else
         hipLaunchKernelGGL((k_gemm_sub<R, ASSIGN>), grid, dim3(512), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
 }
@@ -1544,6 +1577,79 @@ void lu_solve(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* per
     prof_end(prof, s, CLS_SOLVE, 8.0 * (double)n * n, (double)n * n * sizeof(C2<R>));
 }
 
+// ---------------------------------------------------------------------------
+// Dense products and many-right-hand-side solves (low-frequency Calderon
+// blocks of the Mueller system).
+// ---------------------------------------------------------------------------
+template <typename R>
+void gemm_assign(hipStream_t s, const C2<R>* A, int64_t lda, const C2<R>* B, int64_t ldb, C2<R>* C, int64_t ldc,
+                 int M, int N, int K) {
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    dim3 grid((N + kGN - 1) / kGN, (M + kGM - 1) / kGM);
+    launch_gemm<R, 1>(s, grid, A, lda, B, ldb, C, ldc, M, N, K);
+}
+
+// Row interchanges of ipiv, then block forward substitution with L and
+// backward substitution with U: per 64-row block, one in-place product with
+// the stored inverse of the diagonal block (M <= 64, so C may alias B) and
+// one trailing product for the rows still to be solved.
+template <typename R>
+void lu_solve_many(hipStream_t s, const C2<R>* LU, int64_t lda, int n, const int* ipiv, const C2<R>* inv, C2<R>* X,
+                   int64_t ldx, int ncols, void* prof) {
+    if (n <= 0 || ncols <= 0) return;
+    prof_begin(prof, s, CLS_SOLVE);
+    for (int k0 = 0; k0 < n; k0 += kNB) launch_laswp<R>(s, X, ldx, k0, std::min(kNB, n - k0), ipiv, 0, ncols, 0, 0, nullptr);
+    const int nblk = (n + kNB - 1) / kNB;
+    dim3 g1((ncols + kGN - 1) / kGN, 1);
+    for (int kb = 0; kb < nblk; ++kb) {
+        const int r0 = kb * kNB, bs = std::min(kNB, n - r0);
+        launch_gemm<R, 1>(s, g1, inv + (size_t)(2 * kb) * kNB * kNB, (int64_t)kNB, X + (int64_t)r0 * ldx, ldx,
+                          X + (int64_t)r0 * ldx, ldx, bs, ncols, bs);
+        const int rest = n - r0 - bs;
+        if (rest > 0) {
+            dim3 g((ncols + kGN - 1) / kGN, (rest + kGM - 1) / kGM);
+            launch_gemm<R, 0>(s, g, LU + (int64_t)(r0 + bs) * lda + r0, lda, X + (int64_t)r0 * ldx, ldx,
+                              X + (int64_t)(r0 + bs) * ldx, ldx, rest, ncols, bs);
+        }
+    }
+    for (int kb = nblk - 1; kb >= 0; --kb) {
+        const int r0 = kb * kNB, bs = std::min(kNB, n - r0);
+        launch_gemm<R, 1>(s, g1, inv + (size_t)(2 * kb + 1) * kNB * kNB, (int64_t)kNB, X + (int64_t)r0 * ldx, ldx,
+                          X + (int64_t)r0 * ldx, ldx, bs, ncols, bs);
+        if (r0 > 0) {
+            dim3 g((ncols + kGN - 1) / kGN, (r0 + kGM - 1) / kGM);
+            launch_gemm<R, 0>(s, g, LU + r0, lda, X + (int64_t)r0 * ldx, ldx, X, ldx, r0, ncols, bs);
+        }
+    }
+    prof_end(prof, s, CLS_SOLVE, 8.0 * (double)n * n * ncols, (double)n * n * sizeof(C2<R>));
+}
+
+__global__ void k_add_diag(int n, z64* X, int64_t ldx, double v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) X[(int64_t)i * ldx + i].x += v;
+}
+void launch_add_diag(hipStream_t s, int n, z64* X, int64_t ldx, double v) {
+    hipLaunchKernelGGL(k_add_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, X, ldx, v);
+}
+
+template <typename R>
+__global__ void k_calderon_bottom(int nt, const z64* __restrict__ Me, const z64* __restrict__ Mi, double ig,
+                                  C2<R>* __restrict__ A) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nt * nt) return;
+    const int i = (int)(e / nt), j = (int)(e % nt);
+    const int64_t ld = 2 * (int64_t)nt, src = (int64_t)i * ld;
+    const z64 ke = Me[src + j], he = Me[src + nt + j], ki = Mi[src + j], hi = Mi[src + nt + j];
+    A[(int64_t)(nt + i) * ld + j] = C2<R>{R(-he.x + hi.x * ig), R(-he.y + hi.y * ig)};
+    A[(int64_t)(nt + i) * ld + nt + j] = C2<R>{R((i == j ? 1.0 : 0.0) + ke.x - ki.x), R(ke.y - ki.y)};
+}
+template <typename R>
+void launch_calderon_bottom(hipStream_t s, int nt, const z64* Me, const z64* Mi, double g, C2<R>* A) {
+    const int64_t tot = (int64_t)nt * nt;
+    hipLaunchKernelGGL((k_calderon_bottom<R>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nt, Me, Mi,
+                       1.0 / g, A);
+}
+
 // explicit instantiations
 template void launch_row_stats<double>(hipStream_t, const z64*, int64_t, int, double*, double*);
 template void launch_row_stats<float>(hipStream_t, const z32*, int64_t, int, double*, double*);
@@ -1559,6 +1665,10 @@ template void launch_prep_rhs<double>(hipStream_t, int, const z64*, const double
 template void launch_prep_rhs<float>(hipStream_t, int, const z64*, const double*, double, z32*);
 template void launch_finish<double>(hipStream_t, int, const z64*, double, const double*, z64*, int, int*);
 template void launch_finish<float>(hipStream_t, int, const z32*, double, const double*, z64*, int, int*);
+template void gemm_assign<double>(hipStream_t, const z64*, int64_t, const z64*, int64_t, z64*, int64_t, int, int, int);
+template void lu_solve_many<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, z64*, int64_t, int, void*);
+template void launch_calderon_bottom<double>(hipStream_t, int, const z64*, const z64*, double, z64*);
+template void launch_calderon_bottom<float>(hipStream_t, int, const z64*, const z64*, double, z32*);
 template int lu_factor<double>(const LuStreams&, z64*, int64_t, int, int*, z64*, PanelWS&, void*);
 template int lu_factor<float>(const LuStreams&, z32*, int64_t, int, int*, z32*, PanelWS&, void*);
 template void lu_solve<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, const z64*, z64*, z64*, PanelWS&, void*);

@@ -97,6 +97,9 @@ typedef struct {
     double solve_ms;
     int32_t iterations;          /* GMRES iterations (hierarchical path), else 0      */
     int32_t method;              /* 0 direct LU, 1 mixed refinement, 2 GMRES           */
+    int32_t derivative_evaluation; /* Mueller K' and H of both media: 1 direct quadrature,
+                                      2 Calderon reconstruction (full.jl:201-207,336-339);
+                                      0 for other formulations                           */
 } bem_diagnostics;
 
 /* Library version string. */

@@ -11,6 +11,7 @@ src/solvers/bem/fluid_system.jl:217).  The solve drivers restate:
   rigid/soft Burton-Miller RHS and outputs     src/solvers/bem/full.jl:130-184
   fluid equilibration, LU, unscaling           src/solvers/bem/full.jl:364-440
   fluid forcing and scattered traces           src/solvers/bem/full.jl:442-495
+  low-frequency Calderon derivative blocks     src/solvers/bem/full.jl:187-207,319-339
   MixedRefinement (cFP32 LU + FP64 refinement) src/solvers/bem/fluid_system.jl:172-309
   residual report                              src/numerics/diagnostics.jl:9-18
 """
@@ -188,6 +189,41 @@ def equilibrate(A):
     return rows, cols
 
 
+def collocation_radius(mesh):
+    """Largest distance of a collocation point from their mean
+    (src/solvers/bem/quadrature/fluid.jl:1-7, with the panel centroids as the
+    collocation points)."""
+    c = mesh.centroids.mean(axis=0)
+    return float(np.max(np.linalg.norm(mesh.centroids - c, axis=1)))
+
+
+def calderon_applies(mesh, k, h, formulation):
+    """Mueller at max(k, k_int) * radius <= pi/2 (src/solvers/bem/full.jl:319-320)."""
+    return formulation == "muller" and max(k, k / h) * collocation_radius(mesh) <= math.pi / 2
+
+
+def calderon_blocks(mesh, k):
+    """K' = S^-1 (D S) and H = S^-1 (D D - I/4) at wavenumber k
+    (src/solvers/bem/full.jl:201-207)."""
+    S = mesh.assemble(OP_S, k)
+    D = mesh.assemble(OP_D, k)
+    lu = sla.lu_factor(S)
+    return sla.lu_solve(lu, D @ S), sla.lu_solve(lu, D @ D - 0.25 * np.eye(mesh.n))
+
+
+def assemble_fluid(mesh, g, h, k, formulation="muller"):
+    """The 2N x 2N fluid system (src/solvers/bem/full.jl:289-347): Mueller
+    takes its derivative blocks from the Calderon identities at low frequency."""
+    A = mesh.assemble(FLUID_MULLER if formulation == "muller" else FLUID_CBIE, k, g, h)
+    if calderon_applies(mesh, k, h, formulation):
+        n = mesh.n
+        ke, he = calderon_blocks(mesh, k)
+        ki, hi = calderon_blocks(mesh, k / h)
+        A[n:, :n] = -he + hi / g
+        A[n:, n:] = np.eye(n) + ke - ki
+    return A
+
+
 def fluid_forcing(A, n, p_inc, q_inc, formulation):
     if formulation == "muller":
         return np.concatenate([p_inc, q_inc])
@@ -205,7 +241,7 @@ def solve_fluid(mesh, g, h, k, beta=0.0, alpha=0.0, formulation="muller",
         raise ValueError("fluid density and soundspeed contrasts must be finite and positive")
     n = mesh.n
     if A is None:
-        A = mesh.assemble(FLUID_MULLER if formulation == "muller" else FLUID_CBIE, k, g, h)
+        A = assemble_fluid(mesh, g, h, k, formulation)
     p_inc, q_inc = mesh.incident(k, direction(beta, alpha))
     b = fluid_forcing(A, n, p_inc, q_inc, formulation)
     diag = {"method": "direct", "formulation": formulation, "unknown_count": 2 * n}

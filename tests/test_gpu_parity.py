@@ -273,6 +273,32 @@ def test_spheroid_L4_fluid_matches_oracle(ctx):
     assert abs(bh.target_strength(bh.scattering_amplitude(sm)) - bh.target_strength(f)) <= 0.01
 
 
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_low_frequency_mueller_calderon_blocks(ctx, precision):
+    """At max(k, k_int) * radius <= pi/2 the Mueller derivative blocks come
+    from the Calderon identities (full.jl:201-207,319-339): the device's
+    system equals the oracle's, and so do the traces."""
+    say(f"[gpu] low-frequency Mueller (Calderon blocks), spheroid L2, {precision}")
+    nodes, tri = meshes.spheroid(0.02, 0.01, 2)
+    gm = bh.Mesh(nodes, tri + 1, ctx=ctx)
+    om = O.Mesh(nodes, tri)
+    g, h, k = 1.2, 1.1, 50.0
+    assert O.calderon_applies(om, k, h, "muller")
+    A = bh.Operator(gm, bh.FluidFilled(g, h), k, precision=precision).matrix()
+    R = O.assemble_fluid(om, g, h, k)
+    err = np.abs(A - R).max() / np.abs(R).max()
+    assert err < (1e-10 if precision == "double" else 2e-6), err
+    p, q, _, d = bh.solve_full_bem(bh.FluidFilled(g, h), k, gm, incidence_angle=0.4, precision=precision,
+                                   return_diagnostics=True)
+    # the mixed path refines against its FP32-stored system: solve that one
+    p0, q0, _ = O.solve_fluid(om, g, h, k, beta=0.4, A=None if precision == "double" else A)
+    assert np.abs(p - p0).max() / np.abs(p0).max() < 1e-9
+    assert np.abs(q - q0).max() / np.abs(q0).max() < 1e-9
+    assert d["derivative_evaluation"] == {"exterior": "calderon", "interior": "calderon"}
+    _, _, _, d2 = bh.solve_full_bem(bh.FluidFilled(g, h), 2000.0, gm, return_diagnostics=True)
+    assert d2["derivative_evaluation"] == {"exterior": "direct", "interior": "direct"}
+
+
 def test_incidence_sweep_matches_single_solves(ctx):
     say("[gpu] incidence-angle sweep vs single solves")
     gm, _ = _sphere(ctx, 2)

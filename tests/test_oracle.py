@@ -212,6 +212,31 @@ def test_matched_medium_invisible():
     assert abs(f) < 1e-12
 
 
+def test_calderon_blocks_approach_the_direct_operators():
+    """The Calderon blocks (full.jl:201-207) equal the directly assembled K' and
+    H up to the discretisation error, measured on smooth traces of a
+    non-spherical body (S and D commute on a sphere).  A transposed product
+    S D gives twice the K' error here, and a sign error in D D - I/4 an H
+    error above 1."""
+    m = O.Mesh(*meshes.spheroid(1.5, 1.0, 3))
+    k = 1.0
+    kp, hh = O.calderon_blocks(m, k)
+    kd, hd = m.assemble(O.OP_KP, k), m.assemble(O.OP_H, k)
+    p, q = m.incident(k, O.direction(0.7, 0.3))
+    assert np.linalg.norm((kp - kd) @ q) / np.linalg.norm(kd @ q) < 0.07
+    assert np.linalg.norm((hh - hd) @ p) / np.linalg.norm(hd @ p) < 0.1
+
+
+def test_calderon_rule_follows_the_collocation_radius():
+    """Mueller only, max(k, k_int) * radius <= pi/2 (full.jl:319-320)."""
+    m = O.Mesh(*meshes.sphere(1.0, 2))
+    r = O.collocation_radius(m)
+    assert 0.97 < r < 1.0
+    assert O.calderon_applies(m, 1.5, 1.0, "muller") == (1.5 * r <= math.pi / 2)
+    assert O.calderon_applies(m, 1.0, 0.5, "muller") is False  # k_int = 2 decides
+    assert O.calderon_applies(m, 0.1, 1.0, "cbie") is False
+
+
 SPHERE_FLUID_GOLDENS = [
     # test/extended/bem/sphere/full3d.jl:209-233: (g, h, k) and amplitudes at -x, +x, +y
     (0.0012, 0.23, 1.0, (0.08627220103266989 + 0.5718263756162451j, -1.169649453390533 + 0.8439868202224056j,
@@ -232,9 +257,10 @@ def test_reference_sphere_fluid_goldens(g, h, k, refs):
         assert abs(fi - r) / abs(r) < 0.01
 
 
-@pytest.mark.xfail(reason="resonant gas bubble at ka=0.0138 (|f|~72): the flat-panel Mueller "
-                          "discretisation misplaces the Minnaert resonance; the reference's "
-                          "low-frequency Calderon reconstruction is not built", strict=True)
+@pytest.mark.xfail(reason="resonant gas bubble at ka=0.0138 (|f|~72): with the Calderon blocks the "
+                          "flat-panel Mueller solution converges toward the golden (34 % at L3, 9.4 % "
+                          "and -0.035 dB at L4) but needs finer meshes than the oracle runs for 1 %",
+                   strict=True)
 def test_reference_gas_bubble_resonance_golden():
     nodes, tri = meshes.sphere(1.0, 3)
     m = O.Mesh(nodes, tri)
