@@ -552,9 +552,10 @@ def solve_full_bem(boundary, k, surface, *, incidence_angle=0.0, incidence_azimu
     Returns (p_scat, dpdn_scat, quad[, diagnostics]) at the panel collocation
     points.  Without compression the dense direct solve replaces the
     reference's GMRES for rigid/soft (its converged limit).  With
-    compression={"method": "hmatrix", ...} a rigid surface is solved by the
-    compressed operator and GMRES (HMatrix; the remaining compression keys
-    are HMatrix options, gmres_kwargs may set reltol / restart / maxiter).
+    compression={"method": "hmatrix", ...} a rigid surface or a fluid body
+    (Mueller) is solved by the compressed operator and GMRES (HMatrix; the
+    remaining compression keys are HMatrix options, gmres_kwargs may set
+    reltol / restart / maxiter).
     `correction` is accepted for signature compatibility only.
     """
     if incident is not None:
@@ -580,7 +581,8 @@ def solve_full_bem(boundary, k, surface, *, incidence_angle=0.0, incidence_azimu
         p, q = sol.traces(0)
         if return_diagnostics:
             d = sol.diagnostics(0)
-            d["formulation"] = {BEM_BURTON_MILLER: "burton_miller", BEM_CBIE: "cbie"}[H.problem.formulation]
+            d["formulation"] = {BEM_BURTON_MILLER: "burton_miller", BEM_CBIE: "cbie",
+                                BEM_MULLER: "muller"}[H.problem.formulation]
             d["compression"] = dict(compression)
             return p, q, surface, d
         return p, q, surface
@@ -822,15 +824,17 @@ def hmatrix_plan(points, leaf_size=64, eta=2.0, near_radius=0.0):
 
 
 class HMatrix:
-    """Compressed rigid-surface system (the compression=(method=:hmatrix,)
-    path of solve_full_bem, src/solvers/bem/full.jl:68-184 with
-    src/solvers/bem/compression/): cluster tree, dense leaves, ACA blocks in
+    """Compressed system (the compression=(method=:hmatrix,) path of
+    solve_full_bem): rigid surfaces (src/solvers/bem/full.jl:68-184 with
+    src/solvers/bem/compression/) and fluid bodies in the Mueller formulation
+    (compression/fluid.jl:20-106, one H-matrix over the 2N pressure and
+    normal-derivative unknowns).  Cluster tree, dense leaves, ACA blocks in
     complex FP32; solve() runs batched FP64 GMRES for many incidences."""
 
     def __init__(self, surface, boundary, k, *, formulation=None, leaf_size=64, eta=2.0, aca_tolerance=1e-6,
                  max_rank=256, gmres_restart=50, gmres_maxiter=1000, gmres_tolerance=1e-8, preconditioner=True):
-        if not isinstance(boundary, Rigid):
-            raise ValueError("the hierarchical path solves rigid surfaces")
+        if not isinstance(boundary, (Rigid, FluidFilled)):
+            raise ValueError("the hierarchical path solves rigid surfaces and fluid bodies")
         if not (math.isfinite(k) and k > 0):
             raise ValueError("k must be finite and positive")
         self.mesh = surface
@@ -843,7 +847,9 @@ class HMatrix:
         self.ctx.check(self.ctx.lib.bem_hmatrix_create(self.ctx._h, surface._h, ctypes.byref(self.problem),
                                                        ctypes.byref(o), ctypes.byref(h)))
         self._h = h
-        self.n = surface.n
+        # unknowns: one per panel (rigid), pressure and normal derivative per
+        # panel (fluid Mueller: [p; dp/dn], 2N)
+        self.n = int(self.info()["n"])
 
     def __del__(self):
         if getattr(self, "_h", None) and self.ctx._h:
@@ -856,9 +862,12 @@ class HMatrix:
         return {name: getattr(st, name) for name, _ in st._fields_}
 
     def matvec(self, x):
-        """A x for x of shape (n,) or (n, nrhs), panel order."""
+        """A x for x of shape (n,) or (n, nrhs) in unknown order (panels; for
+        fluid bodies the N pressures, then the N normal derivatives)."""
         x = np.asarray(x, dtype=np.complex128)
         one = x.ndim == 1
+        if x.shape[0] != self.n or x.ndim > 2:
+            raise ValueError(f"x must have {self.n} rows")
         X = np.asfortranarray(x.reshape(self.n, -1))
         Y = np.empty_like(X, order="F")
         self.ctx.check(self.ctx.lib.bem_hmatrix_matvec(self._h, X.shape[1], X.ctypes.data_as(_dp),

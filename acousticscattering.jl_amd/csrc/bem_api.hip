@@ -1373,17 +1373,20 @@ struct bem_hmatrix_s {
     bem_problem prob{};
     bem_hmatrix_options opt{};
     Lane* lane = nullptr;
-    int64_t n = 0;
+    int64_t n = 0;                // unknowns: panels (rigid) or 2 x panels (fluid Mueller)
+    int64_t nt = 0;               // panels
+    bool fluid = false;
+    HPar par{};
     int nd = 0, nl = 0, nleaf = 0;
     std::vector<void*> bufs;
     int* perm = nullptr;
     std::vector<int> hperm, leaf_lo, leaf_m, leaf_of_pos;
     std::vector<HBlk> plan_blk;   // first-pass layout (rank capacity 64)
     int64_t dense_count = 0, first_cap = 0;
-    // near-pair and self slots of the dense leaves
-    int *npi = nullptr, *npj = nullptr;
+    // near-pair and same-panel slots of the dense leaves (unknown indices)
+    int *npi = nullptr, *npj = nullptr, *nsi = nullptr, *nsj = nullptr;
     int64_t *ndst = nullptr, *sdst = nullptr;
-    int64_t nnear = 0;
+    int64_t nnear = 0, nself = 0;
     HOp A, B;        // system and right-hand-side operators
     z64* minv = nullptr;
     bem_hmatrix_info info{};
@@ -1416,15 +1419,22 @@ void hm_free(bem_hmatrix h) {
 // Plan: cluster tree, block tree, dense-leaf layout and near/self slots.
 void hm_plan(bem_hmatrix h) {
     const HostMesh& hm = h->mesh->h;
-    const int64_t n = hm.nt;
+    const int64_t nt = hm.nt;
+    // fluid Mueller: unknown i is the pressure and nt + i the normal
+    // derivative on panel i, both clustered at the panel centroid
+    const int nq = h->fluid ? 2 : 1;
+    const int64_t n = nq * nt;
     h->n = n;
+    h->nt = nt;
     const auto& o = h->opt;
     const double t0 = now_ms();
+    std::vector<double> pts(hm.cen.begin(), hm.cen.end());
+    if (h->fluid) pts.insert(pts.end(), hm.cen.begin(), hm.cen.end());
     // near pairs never fall in low-rank blocks (box distance > 2 hmax)
     double hmax = 0;
     for (double v : hm.hmax) hmax = std::max(hmax, v);
     HPlan P;
-    hmatrix_plan(hm.cen.data(), n, o.leaf_size, o.eta, kNearEta * hmax * (1.0 + 1e-9), P);
+    hmatrix_plan(pts.data(), n, o.leaf_size, o.eta, kNearEta * hmax * (1.0 + 1e-9), P);
     std::vector<int> leaf_of_node(P.nodes.size(), -1);
     std::vector<std::pair<int64_t, int32_t>> lv;
     for (size_t id = 0; id < P.nodes.size(); ++id)
@@ -1451,32 +1461,46 @@ void hm_plan(bem_hmatrix h) {
         dmap[(int64_t)leaf_of_node[P.dense[b].row] * h->nleaf + leaf_of_node[P.dense[b].col]] = b;
     }
     h->dense_count = doff;
-    std::vector<int> npi, npj;
-    std::vector<int64_t> ndst, sdst(n);
-    for (int64_t i = 0; i < n; ++i) {
-        const int64_t pi = P.iperm[i];
-        const int li = h->leaf_of_pos[pi];
-        for (int32_t e = hm.near_ptr[i]; e < hm.near_ptr[i + 1]; ++e) {
-            const int64_t j = hm.near_col[e];
-            const int64_t pj = P.iperm[j];
-            auto it = dmap.find((int64_t)li * h->nleaf + h->leaf_of_pos[pj]);
-            if (it == dmap.end()) throw BemError(BEM_ERR_DEVICE, "near pair outside the dense leaves");
-            const HBlk& b = h->plan_blk[it->second];
-            npi.push_back((int)i);
-            npj.push_back((int)j);
-            ndst.push_back(b.off + (pi - b.row_lo) * b.n + (pj - b.col_lo));
-        }
-        auto it = dmap.find((int64_t)li * h->nleaf + li);
-        if (it == dmap.end()) throw BemError(BEM_ERR_DEVICE, "diagonal leaf block missing");
+    std::vector<int> npi, npj, nsi, nsj;
+    std::vector<int64_t> ndst, sdst;
+    auto slot = [&](int64_t I, int64_t J) {
+        const int64_t pI = P.iperm[I], pJ = P.iperm[J];
+        auto it = dmap.find((int64_t)h->leaf_of_pos[pI] * h->nleaf + h->leaf_of_pos[pJ]);
+        if (it == dmap.end()) throw BemError(BEM_ERR_DEVICE, "near pair outside the dense leaves");
         const HBlk& b = h->plan_blk[it->second];
-        sdst[i] = b.off + (pi - b.row_lo) * b.n + (pi - b.col_lo);
+        return b.off + (pI - b.row_lo) * b.n + (pJ - b.col_lo);
+    };
+    for (int64_t I = 0; I < n; ++I) {
+        const int64_t i = I % nt;
+        for (int32_t e = hm.near_ptr[i]; e < hm.near_ptr[i + 1]; ++e)
+            for (int q = 0; q < nq; ++q) {
+                const int64_t J = hm.near_col[e] + q * nt;
+                npi.push_back((int)I);
+                npj.push_back((int)J);
+                ndst.push_back(slot(I, J));
+            }
+        for (int q = 0; q < nq; ++q) {  // unknowns on the same panel
+            const int64_t J = i + q * nt;
+            nsi.push_back((int)I);
+            nsj.push_back((int)J);
+            sdst.push_back(slot(I, J));
+        }
     }
     const int cap1 = std::min(64, o.max_rank);
+    const double kmax = std::max(h->par.k, h->par.ki);
     int64_t uoff = 0;
     for (int b = 0; b < h->nl; ++b) {
         const HNode &r = P.nodes[P.lowrank[b].row], &c = P.nodes[P.lowrank[b].col];
         HBlk x{(int)r.lo, (int)(r.hi - r.lo), (int)c.lo, (int)(c.hi - c.lo), 0, 0, uoff};
         x.rmax = std::min({x.m, x.n, cap1});
+        if (h->fluid) {
+            double d2 = 0;
+            for (int q = 0; q < 3; ++q) {
+                const double gap = std::max({0.0, c.bmin[q] - r.bmax[q], r.bmin[q] - c.bmax[q]});
+                d2 += gap * gap;
+            }
+            x.sig = (float)(kmax + 1.0 / std::sqrt(d2));  // admissible blocks have dist > 0
+        }
         uoff += (int64_t)x.rmax * (x.m + x.n);
         h->plan_blk.push_back(x);
     }
@@ -1486,8 +1510,11 @@ void hm_plan(bem_hmatrix h) {
     h->npi = hm_upload(h, npi);
     h->npj = hm_upload(h, npj);
     h->ndst = hm_upload(h, ndst);
+    h->nsi = hm_upload(h, nsi);
+    h->nsj = hm_upload(h, nsj);
     h->sdst = hm_upload(h, sdst);
     h->nnear = (int64_t)npi.size();
+    h->nself = (int64_t)nsi.size();
     h->info.n = n;
     h->info.nodes = (int64_t)P.nodes.size();
     h->info.leaves = h->nleaf;
@@ -1511,8 +1538,9 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
     const double t1 = now_ms();
     op.dblk = hm_upload(h, op.blk);
     op.dval = (z32*)hm_alloc(h, sizeof(z32) * std::max<int64_t>(1, h->dense_count));
-    launch_hdense(s, dm, kind, h->prob.k, h->perm, nd, op.dblk, op.dval);
-    launch_hnear_self(s, dm, kind, h->prob.k, h->nnear, h->npi, h->npj, h->ndst, h->sdst, op.dval);
+    launch_hdense(s, dm, kind, h->par, h->perm, nd, op.dblk, op.dval);
+    launch_hnear_self(s, dm, kind, h->par, h->nnear, h->npi, h->npj, h->ndst, h->nself, h->nsi, h->nsj, h->sdst,
+                      op.dval);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s));
     const double t2 = now_ms();
@@ -1529,7 +1557,7 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
     std::vector<int> hconv(nd + nl);
     try {
         prof_begin(&h->lane->pl, s, C_HACA);
-        launch_aca(s, dm, kind, h->prob.k, h->perm, nl, dlist, op.dblk, uv1, conv, o.aca_tolerance);
+        launch_aca(s, dm, kind, h->par, h->perm, nl, dlist, op.dblk, uv1, conv, o.aca_tolerance);
         prof_end(&h->lane->pl, s, C_HACA, 0.0, 0.0);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(op.blk.data(), op.dblk, sizeof(HBlk) * op.blk.size(), hipMemcpyDeviceToHost, s));
@@ -1553,7 +1581,7 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
             uv2 = (z32*)ctx->alloc(sizeof(z32) * uoff2);
             HIPCHECK(hipMemcpyAsync(op.dblk, op.blk.data(), sizeof(HBlk) * op.blk.size(), hipMemcpyHostToDevice, s));
             int* dredo = hm_upload(h, redo);
-            launch_aca(s, dm, kind, h->prob.k, h->perm, (int)redo.size(), dredo, op.dblk, uv2, conv,
+            launch_aca(s, dm, kind, h->par, h->perm, (int)redo.size(), dredo, op.dblk, uv2, conv,
                        o.aca_tolerance);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipMemcpyAsync(op.blk.data(), op.dblk, sizeof(HBlk) * op.blk.size(), hipMemcpyDeviceToHost, s));
@@ -1592,6 +1620,7 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
         uv1 = uv2 = nullptr;
         op.blk = compact;
         HIPCHECK(hipMemcpyAsync(op.dblk, op.blk.data(), sizeof(HBlk) * op.blk.size(), hipMemcpyHostToDevice, s));
+        if (kind == HK_MULLER) launch_hunscale(s, nl, op.dblk, dlist, h->perm, (int)h->nt, op.uv);
         op.max_rank = rmx;
         op.unconverged = unconv;
         op.mean_rank = nl ? rsum / nl : 0.0;
@@ -1653,8 +1682,13 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
 void hm_build(bem_hmatrix h) {
     const bool bm = h->prob.formulation == BEM_BURTON_MILLER;
     hm_plan(h);
-    hm_operator(h, h->A, bm ? HK_BM : HK_CBIE);
-    hm_operator(h, h->B, bm ? HK_BM_RHS : HK_CBIE_RHS);
+    if (h->fluid) {
+        // Mueller: the right-hand side is the incident traces themselves
+        hm_operator(h, h->A, HK_MULLER);
+    } else {
+        hm_operator(h, h->A, bm ? HK_BM : HK_CBIE);
+        hm_operator(h, h->B, bm ? HK_BM_RHS : HK_CBIE_RHS);
+    }
     if (h->opt.preconditioner) {
         h->minv = (z64*)hm_alloc(h, sizeof(z64) * 4096 * (size_t)h->nleaf);
         int* st = (int*)hm_alloc(h, sizeof(int));
@@ -1669,7 +1703,7 @@ void hm_build(bem_hmatrix h) {
     h->info.unconverged_blocks = h->A.unconverged + h->B.unconverged;
     h->info.mean_rank = h->A.mean_rank;
     h->info.lowrank_bytes = h->A.lowrank_bytes + h->B.lowrank_bytes;
-    h->info.dense_bytes *= 2;
+    if (!h->fluid) h->info.dense_bytes *= 2;
 }
 
 // Y = op X on the device (tree-position order, nr <= kHRhs)
@@ -2596,10 +2630,21 @@ int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, c
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     return guarded(ctx, [&] {
         validate_problem(*problem);
-        if (problem->boundary != BEM_RIGID)
-            throw BemError(BEM_ERR_ARGUMENT, "the hierarchical path solves rigid surfaces");
-        if (problem->formulation != BEM_BURTON_MILLER && problem->formulation != BEM_CBIE)
+        const bool fluid = problem->boundary == BEM_FLUID;
+        if (problem->boundary != BEM_RIGID && !fluid)
+            throw BemError(BEM_ERR_ARGUMENT, "the hierarchical path solves rigid surfaces and fluid bodies");
+        if (fluid) {
+            // compression/fluid.jl:20-24
+            if (problem->formulation != BEM_MULLER)
+                throw BemError(BEM_ERR_ARGUMENT, "compressed fluid BEM requires formulation=:muller");
+            if (calderon_applies(mesh->h, *problem))
+                throw BemError(BEM_ERR_ARGUMENT,
+                               "compressed fluid BEM at max(k, k_int) * radius <= pi/2 takes K' and H from the "
+                               "Calderon identities through a hierarchical LU of S, which is not built here: use "
+                               "the dense path");
+        } else if (problem->formulation != BEM_BURTON_MILLER && problem->formulation != BEM_CBIE) {
             throw BemError(BEM_ERR_ARGUMENT, "formulation must be :burton_miller or :cbie");
+        }
         bem_hmatrix_options o;
         bem_hmatrix_defaults(&o);
         if (options) o = *options;
@@ -2615,6 +2660,9 @@ int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, c
         mesh->refs++;
         h->prob = *problem;
         h->opt = o;
+        h->fluid = fluid;
+        h->par = HPar{problem->k, fluid ? problem->k / problem->soundspeed_contrast : problem->k,
+                      fluid ? problem->density_contrast : 1.0};
         try {
             h->lane = make_lane(ctx);
             hm_build(h);
@@ -2691,7 +2739,7 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t nrhs, const double* angles, bem_sol
         for (int r = 0; r < 2 * nrhs; ++r)
             if (!std::isfinite(angles[r])) throw BemError(BEM_ERR_ARGUMENT, "incidence angles must be finite");
         const int64_t n = h->n;
-        const int nt = (int)n;
+        const int nt = (int)h->nt;
         hipStream_t s = h->lane->s;
         auto* sol = new bem_solution_s();
         sol->ctx = ctx;
@@ -2719,16 +2767,28 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t nrhs, const double* angles, bem_sol
             std::vector<bem_diagnostics> dg;
             for (int c0 = 0; c0 < nrhs; c0 += kHRhs) {
                 const int nr = std::min(kHRhs, nrhs - c0);
-                // b = B q_inc (the right-hand side of the scattered-field equation)
-                launch_hqinc(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, X);
-                hm_apply(h, h->B, X, nr, B);
+                if (h->fluid) {
+                    // Mueller: b = [p_inc; q_inc] (full.jl:442-447)
+                    launch_hfluid_rhs(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, B);
+                } else {
+                    // b = B q_inc (the right-hand side of the scattered-field equation)
+                    launch_hqinc(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, X);
+                    hm_apply(h, h->B, X, nr, B);
+                }
                 dg.assign(nr, bem_diagnostics{});
                 hm_gmres(h, nr, B, X, dg);
                 // traces of this chunk: columns c0 .. c0 + nr of P, Q
-                launch_htraces(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, X, sol->p + (size_t)c0 * nt,
-                               sol->q + (size_t)c0 * nt);
+                if (h->fluid)
+                    launch_hfluid_traces(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, X,
+                                         sol->p + (size_t)c0 * nt, sol->q + (size_t)c0 * nt);
+                else
+                    launch_htraces(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, X, sol->p + (size_t)c0 * nt,
+                                   sol->q + (size_t)c0 * nt);
                 HIPCHECK(hipGetLastError());
-                for (int c = 0; c < nr; ++c) sol->diag[c0 + c] = dg[c];
+                for (int c = 0; c < nr; ++c) {
+                    sol->diag[c0 + c] = dg[c];
+                    sol->diag[c0 + c].derivative_evaluation = h->fluid ? 1 : 0;
+                }
             }
             sol->hp.resize((size_t)nt * nrhs);
             sol->hq.resize((size_t)nt * nrhs);

@@ -435,15 +435,24 @@ void launch_far_field(hipStream_t s, const DevMesh& m, double k, int nrhs, int n
 }
 
 // ---------------------------------------------------------------------------
-// Hierarchical path: entries of single-medium rigid systems by tree position.
-// Regular pairs in FP32 (7-point rule); near pairs and self panels in FP64
-// into the dense-block slots the host resolved.
+// Hierarchical path: entries by unknown index (the host maps tree positions
+// to unknowns).  Rigid kinds have one unknown per panel; the Mueller kind has
+// two (pressure I = i, normal derivative I = N + i).  Regular pairs in FP32
+// (7-point rule); near pairs and same-panel pairs in FP64 into the
+// dense-block slots the host resolved.
 // ---------------------------------------------------------------------------
 template <int KIND>
 struct HNeed {
-    static constexpr int v = KIND == HK_BM ? 10 : KIND == HK_CBIE ? 2 : KIND == HK_BM_RHS ? 5 : 1;
+    static constexpr int v = KIND == HK_BM       ? 10
+                             : KIND == HK_CBIE   ? 2
+                             : KIND == HK_BM_RHS ? 5
+                             : KIND == HK_MULLER ? 15
+                                                 : 1;
 };
-// off-diagonal entry from the operator values (diag adds the 1/2 I terms)
+struct HParF {
+    float k, ki, g;
+};
+// single-medium entry from the operator values (diag adds the 1/2 I terms)
 template <int KIND, typename R>
 __device__ __forceinline__ z32 hcombine(const Acc4<R>& e, R ik, bool diag) {
     const R h = diag ? R(0.5) : R(0);
@@ -452,9 +461,20 @@ __device__ __forceinline__ z32 hcombine(const Acc4<R>& e, R ik, bool diag) {
     if (KIND == HK_BM_RHS) return z32{float(e.S.x - e.K.y * ik), float(e.S.y + (e.K.x + h) * ik)};
     return z32{float(e.S.x), float(e.S.y)};
 }
+// Mueller entry of quadrant (a, b) from both media's operator values
+// (full.jl:341); diag adds the identity of the diagonal quadrants.
+template <typename R>
+__device__ __forceinline__ z32 mcombine(const Acc4<R>& e, const Acc4<R>& f, int a, int b, R g, bool diag) {
+    const R d1 = diag ? R(1) : R(0);
+    if (a == 0 && b == 0) return z32{float(d1 - e.D.x + f.D.x), float(-e.D.y + f.D.y)};
+    if (a == 0) return z32{float(e.S.x - g * f.S.x), float(e.S.y - g * f.S.y)};
+    if (b == 0) return z32{float(-e.H.x + f.H.x / g), float(-e.H.y + f.H.y / g)};
+    return z32{float(d1 + e.K.x - f.K.x), float(e.K.y - f.K.y)};
+}
 
-template <int KIND>
-__device__ __forceinline__ z32 hentry(const MeshView<float>& v, int nt, int i, int j, float k, float ik) {
+// 7-point regular sums of the V operators for target panel i, source panel j
+template <int V>
+__device__ __forceinline__ Acc4<float> hsum(const MeshView<float>& v, int nt, int i, int j, float k) {
     const float x0 = v.cen[i], x1 = v.cen[nt + i], x2 = v.cen[2 * nt + i];
     const float nx0 = v.nrm[i], nx1 = v.nrm[nt + i], nx2 = v.nrm[2 * nt + i];
     const float ny0 = v.nrm[j], ny1 = v.nrm[nt + j], ny2 = v.nrm[2 * nt + j];
@@ -466,40 +486,74 @@ __device__ __forceinline__ z32 hentry(const MeshView<float>& v, int nt, int i, i
         const float rx = x0 - v.gp[(size_t)(3 * q) * nt + j];
         const float ry = x1 - v.gp[(size_t)(3 * q + 1) * nt + j];
         const float rz = x2 - v.gp[(size_t)(3 * q + 2) * nt + j];
-        kpoint<float, HNeed<KIND>::v>(k, rx, ry, rz, nx0, nx1, nx2, ny0, ny1, ny2, c, float(c_q7[q][3]) * aj, e);
+        kpoint<float, V>(k, rx, ry, rz, nx0, nx1, nx2, ny0, ny1, ny2, c, float(c_q7[q][3]) * aj, e);
     }
-    return hcombine<KIND>(e, ik, false);
+    return e;
 }
 
 template <int KIND>
-__global__ __launch_bounds__(256) void k_hdense(DevMesh mesh, float k, const int* __restrict__ perm,
+__device__ __forceinline__ int hpanel(int I, int nt) {
+    return (KIND == HK_MULLER && I >= nt) ? I - nt : I;
+}
+
+// regular entry (I, J) of unknowns on different panels
+template <int KIND>
+__device__ __forceinline__ z32 hentry(const MeshView<float>& v, int nt, int I, int J, const HParF& p) {
+    if constexpr (KIND == HK_MULLER) {
+        const int a = I >= nt, b = J >= nt;
+        const int i = I - a * nt, j = J - b * nt;
+        const Acc4<float> e = hsum<15>(v, nt, i, j, p.k);
+        const Acc4<float> f = hsum<15>(v, nt, i, j, p.ki);
+        return mcombine(e, f, a, b, p.g, false);
+    } else {
+        return hcombine<KIND>(hsum<HNeed<KIND>::v>(v, nt, I, J, p.k), 1.f / p.k, false);
+    }
+}
+
+// regular entry of the block scaled for ACA (Mueller: rows N + i by 1/sig,
+// columns N + j by sig)
+template <int KIND>
+__device__ __forceinline__ z32 hentry_aca(const MeshView<float>& v, int nt, int I, int J, const HParF& p, float sig) {
+    z32 a = hentry<KIND>(v, nt, I, J, p);
+    if (KIND == HK_MULLER) {
+        const float f = (I >= nt ? 1.f / sig : 1.f) * (J >= nt ? sig : 1.f);
+        a.x *= f;
+        a.y *= f;
+    }
+    return a;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_hdense(DevMesh mesh, HParF p, const int* __restrict__ perm,
                                                 const HBlk* __restrict__ blk, z32* __restrict__ val) {
     const HBlk b = blk[blockIdx.x];
     const MeshView<float> v = view_of(mesh, 0.f);
-    const float ik = 1.f / k;
+    const int nt = mesh.nt;
     for (int e = threadIdx.x; e < b.m * b.n; e += 256) {
         const int r = e / b.n, c = e - r * b.n;
-        const int i = perm[b.row_lo + r], j = perm[b.col_lo + c];
-        val[b.off + e] = i == j ? z32{0.f, 0.f} : hentry<KIND>(v, mesh.nt, i, j, k, ik);
+        const int I = perm[b.row_lo + r], J = perm[b.col_lo + c];
+        // same-panel pairs are written by k_hself
+        val[b.off + e] = hpanel<KIND>(I, nt) == hpanel<KIND>(J, nt) ? z32{0.f, 0.f} : hentry<KIND>(v, nt, I, J, p);
     }
 }
 
 template <int KIND>
-__global__ __launch_bounds__(256) void k_hnear(DevMesh m, double k, int64_t npairs, const int* __restrict__ pi,
+__global__ __launch_bounds__(256) void k_hnear(DevMesh m, HPar p, int64_t npairs, const int* __restrict__ pi,
                                                const int* __restrict__ pj, const int64_t* __restrict__ dst,
                                                z32* __restrict__ val) {
     const int lane = threadIdx.x & 63;
     const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (e >= npairs) return;
     const int nt = m.nt;
-    const int i = pi[e], j = pj[e];
+    const int I = pi[e], J = pj[e];
+    const int i = hpanel<KIND>(I, nt), j = hpanel<KIND>(J, nt);
     const double x0 = m.cen[i], x1 = m.cen[nt + i], x2 = m.cen[2 * nt + i];
     const double nx0 = m.nrm[i], nx1 = m.nrm[nt + i], nx2 = m.nrm[2 * nt + i];
     const double ny0 = m.nrm[j], ny1 = m.nrm[nt + j], ny2 = m.nrm[2 * nt + j];
     const double c = nx0 * ny0 + nx1 * ny1 + nx2 * ny2;
     const double* V = m.vert + 9 * (size_t)j;
     const double aj = m.area[j];
-    Acc4<double> a{};
+    Acc4<double> a{}, b{};
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
         const int pt = lane * 7 + q;
@@ -508,62 +562,77 @@ __global__ __launch_bounds__(256) void k_hnear(DevMesh m, double k, int64_t npai
         const double y0 = l0 * V[0] + l1 * V[3] + l2 * V[6];
         const double y1 = l0 * V[1] + l1 * V[4] + l2 * V[7];
         const double y2 = l0 * V[2] + l1 * V[5] + l2 * V[8];
-        kpoint<double, HNeed<KIND>::v>(k, x0 - y0, x1 - y1, x2 - y2, nx0, nx1, nx2, ny0, ny1, ny2, c, w, a);
+        kpoint<double, HNeed<KIND>::v>(p.k, x0 - y0, x1 - y1, x2 - y2, nx0, nx1, nx2, ny0, ny1, ny2, c, w, a);
+        if (KIND == HK_MULLER)
+            kpoint<double, 15>(p.ki, x0 - y0, x1 - y1, x2 - y2, nx0, nx1, nx2, ny0, ny1, ny2, c, w, b);
     }
     wave_sum_acc(a);
-    if (lane == 0) val[dst[e]] = hcombine<KIND>(a, 1.0 / k, false);
+    if (KIND == HK_MULLER) wave_sum_acc(b);
+    if (lane == 0)
+        val[dst[e]] = KIND == HK_MULLER ? mcombine(a, b, I >= nt, J >= nt, p.g, false) : hcombine<KIND>(a, 1.0 / p.k, false);
 }
 
+// Pairs of unknowns on one flat panel: D_ii = K'_ii = 0; S_ii and the finite
+// part H_ii in closed form.
 template <int KIND>
-__global__ __launch_bounds__(128) void k_hself(DevMesh m, double k, const int64_t* __restrict__ dst,
+__global__ __launch_bounds__(128) void k_hself(DevMesh m, HPar p, int64_t nself, const int* __restrict__ si,
+                                               const int* __restrict__ sj, const int64_t* __restrict__ dst,
                                                z32* __restrict__ val) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int nt = m.nt;
-    if (i >= nt) return;
-    // flat panel: D_ii = K'_ii = 0; S_ii and the finite part H_ii in closed form
-    Acc4<double> a{};
+    if (e >= nself) return;
+    const int I = si[e], J = sj[e];
+    const int i = hpanel<KIND>(I, nt);
+    Acc4<double> a{}, b{};
     if (KIND != HK_CBIE) {
         double V[9], x[3];
         for (int c = 0; c < 9; ++c) V[c] = m.vert[9 * (size_t)i + c];
         x[0] = m.cen[i];
         x[1] = m.cen[nt + i];
         x[2] = m.cen[2 * nt + i];
-        self_terms(V, x, k, a.S, a.H);
+        self_terms(V, x, p.k, a.S, a.H);
+        if (KIND == HK_MULLER) self_terms(V, x, p.ki, b.S, b.H);
     }
-    val[dst[i]] = hcombine<KIND>(a, 1.0 / k, true);
+    val[dst[e]] = KIND == HK_MULLER ? mcombine(a, b, I >= nt, J >= nt, p.g, I == J)
+                                    : hcombine<KIND>(a, 1.0 / p.k, true);
 }
 
-template <int KIND>
-static void launch_hdense_k(hipStream_t s, const DevMesh& m, double k, const int* perm, int nblk, const HBlk* blk,
-                            z32* val) {
-    hipLaunchKernelGGL((k_hdense<KIND>), dim3(nblk), dim3(256), 0, s, m, (float)k, perm, blk, val);
-}
-void launch_hdense(hipStream_t s, const DevMesh& m, int kind, double k, const int* perm, int nblk, const HBlk* blk,
-                   z32* val) {
+static HParF hpar_f(const HPar& p) { return HParF{(float)p.k, (float)p.ki, (float)p.g}; }
+
+void launch_hdense(hipStream_t s, const DevMesh& m, int kind, const HPar& p, const int* perm, int nblk,
+                   const HBlk* blk, z32* val) {
     if (nblk <= 0) return;
+#define BEM_HD(K) hipLaunchKernelGGL((k_hdense<K>), dim3(nblk), dim3(256), 0, s, m, hpar_f(p), perm, blk, val)
     switch (kind) {
-    case HK_BM: launch_hdense_k<HK_BM>(s, m, k, perm, nblk, blk, val); break;
-    case HK_CBIE: launch_hdense_k<HK_CBIE>(s, m, k, perm, nblk, blk, val); break;
-    case HK_BM_RHS: launch_hdense_k<HK_BM_RHS>(s, m, k, perm, nblk, blk, val); break;
-    default: launch_hdense_k<HK_CBIE_RHS>(s, m, k, perm, nblk, blk, val); break;
+    case HK_BM: BEM_HD(HK_BM); break;
+    case HK_CBIE: BEM_HD(HK_CBIE); break;
+    case HK_BM_RHS: BEM_HD(HK_BM_RHS); break;
+    case HK_MULLER: BEM_HD(HK_MULLER); break;
+    default: BEM_HD(HK_CBIE_RHS); break;
     }
+#undef BEM_HD
 }
 
 template <int KIND>
-static void launch_hnear_self_k(hipStream_t s, const DevMesh& m, double k, int64_t npairs, const int* pi,
-                                const int* pj, const int64_t* dst, const int64_t* self_dst, z32* val) {
+static void launch_hnear_self_k(hipStream_t s, const DevMesh& m, const HPar& p, int64_t npairs, const int* pi,
+                                const int* pj, const int64_t* dst, int64_t nself, const int* si, const int* sj,
+                                const int64_t* self_dst, z32* val) {
     if (npairs > 0)
-        hipLaunchKernelGGL((k_hnear<KIND>), dim3((unsigned)((npairs + 3) / 4)), dim3(256), 0, s, m, k, npairs, pi, pj,
+        hipLaunchKernelGGL((k_hnear<KIND>), dim3((unsigned)((npairs + 3) / 4)), dim3(256), 0, s, m, p, npairs, pi, pj,
                            dst, val);
-    hipLaunchKernelGGL((k_hself<KIND>), dim3((m.nt + 127) / 128), dim3(128), 0, s, m, k, self_dst, val);
+    if (nself > 0)
+        hipLaunchKernelGGL((k_hself<KIND>), dim3((unsigned)((nself + 127) / 128)), dim3(128), 0, s, m, p, nself, si,
+                           sj, self_dst, val);
 }
-void launch_hnear_self(hipStream_t s, const DevMesh& m, int kind, double k, int64_t npairs, const int* pi,
-                       const int* pj, const int64_t* dst, const int64_t* self_dst, z32* val) {
+void launch_hnear_self(hipStream_t s, const DevMesh& m, int kind, const HPar& p, int64_t npairs, const int* pi,
+                       const int* pj, const int64_t* dst, int64_t nself, const int* si, const int* sj,
+                       const int64_t* self_dst, z32* val) {
     switch (kind) {
-    case HK_BM: launch_hnear_self_k<HK_BM>(s, m, k, npairs, pi, pj, dst, self_dst, val); break;
-    case HK_CBIE: launch_hnear_self_k<HK_CBIE>(s, m, k, npairs, pi, pj, dst, self_dst, val); break;
-    case HK_BM_RHS: launch_hnear_self_k<HK_BM_RHS>(s, m, k, npairs, pi, pj, dst, self_dst, val); break;
-    default: launch_hnear_self_k<HK_CBIE_RHS>(s, m, k, npairs, pi, pj, dst, self_dst, val); break;
+    case HK_BM: launch_hnear_self_k<HK_BM>(s, m, p, npairs, pi, pj, dst, nself, si, sj, self_dst, val); break;
+    case HK_CBIE: launch_hnear_self_k<HK_CBIE>(s, m, p, npairs, pi, pj, dst, nself, si, sj, self_dst, val); break;
+    case HK_BM_RHS: launch_hnear_self_k<HK_BM_RHS>(s, m, p, npairs, pi, pj, dst, nself, si, sj, self_dst, val); break;
+    case HK_MULLER: launch_hnear_self_k<HK_MULLER>(s, m, p, npairs, pi, pj, dst, nself, si, sj, self_dst, val); break;
+    default: launch_hnear_self_k<HK_CBIE_RHS>(s, m, p, npairs, pi, pj, dst, nself, si, sj, self_dst, val); break;
     }
 }
 
@@ -618,14 +687,13 @@ __device__ inline void aca_sums(double (&x)[NV], double* sd) {
 // approximation updated exactly (cross terms included), or at the block's
 // rank capacity (conv = 0).
 template <int KIND>
-__global__ __launch_bounds__(256) void k_aca(DevMesh mesh, float k, const int* __restrict__ perm, const int* list,
+__global__ __launch_bounds__(256) void k_aca(DevMesh mesh, HParF p, const int* __restrict__ perm, const int* list,
                                              HBlk* __restrict__ blks, z32* __restrict__ uv, int* __restrict__ conv,
                                              float tol) {
     const int bid = list ? list[blockIdx.x] : blockIdx.x;
     const HBlk b = blks[bid];
     const MeshView<float> v = view_of(mesh, 0.f);
     const int nt = mesh.nt, tid = threadIdx.x;
-    const float ik = 1.f / k;
     z32* U = uv + b.off;
     z32* V = uv + b.off + (int64_t)b.rmax * b.m;
     __shared__ z32 coef[kAcaCap];
@@ -642,7 +710,7 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, float k, const int* _
         float bv = -1.f;
         int bj = 0;
         for (int j = tid; j < b.n; j += 256) {
-            z32 a = hentry<KIND>(v, nt, gi, perm[b.col_lo + j], k, ik);
+            z32 a = hentry_aca<KIND>(v, nt, gi, perm[b.col_lo + j], p, b.sig);
             for (int l = 0; l < r; ++l) a = csub(a, cmul(coef[l], V[(int64_t)l * b.n + j]));
             V[(int64_t)r * b.n + j] = a;
             const float m2 = a.x * a.x + a.y * a.y;
@@ -671,7 +739,7 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, float k, const int* _
         int bi = 0;
         double acc[2] = {0.0, 0.0};  // |u|^2, |v|^2
         for (int i = tid; i < b.m; i += 256) {
-            z32 a = hentry<KIND>(v, nt, perm[b.row_lo + i], gj, k, ik);
+            z32 a = hentry_aca<KIND>(v, nt, perm[b.row_lo + i], gj, p, b.sig);
             for (int l = 0; l < r; ++l) a = csub(a, cmul(U[(int64_t)l * b.m + i], coef[l]));
             U[(int64_t)r * b.m + i] = a;
             const float m2 = a.x * a.x + a.y * a.y;
@@ -737,14 +805,15 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, float k, const int* _
     }
 }
 
-void launch_aca(hipStream_t s, const DevMesh& m, int kind, double k, const int* perm, int nlist, const int* list,
+void launch_aca(hipStream_t s, const DevMesh& m, int kind, const HPar& p, const int* perm, int nlist, const int* list,
                 HBlk* blk, z32* uv, int* conv, double tol) {
     if (nlist <= 0) return;
-#define BEM_ACA(K) hipLaunchKernelGGL((k_aca<K>), dim3(nlist), dim3(256), 0, s, m, (float)k, perm, list, blk, uv, conv, (float)tol)
+#define BEM_ACA(K) hipLaunchKernelGGL((k_aca<K>), dim3(nlist), dim3(256), 0, s, m, hpar_f(p), perm, list, blk, uv, conv, (float)tol)
     switch (kind) {
     case HK_BM: BEM_ACA(HK_BM); break;
     case HK_CBIE: BEM_ACA(HK_CBIE); break;
     case HK_BM_RHS: BEM_ACA(HK_BM_RHS); break;
+    case HK_MULLER: BEM_ACA(HK_MULLER); break;
     default: BEM_ACA(HK_CBIE_RHS); break;
     }
 #undef BEM_ACA

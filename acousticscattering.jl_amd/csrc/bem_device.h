@@ -92,24 +92,39 @@ struct HBlk {
     int row_lo, m, col_lo, n;
     int rmax, rank;
     int64_t off;
+    // Mueller low-rank blocks: ACA runs on T A S, T and S scaling the
+    // normal-derivative rows by 1/sig and columns by sig (sig = k + 1/dist),
+    // which equalises the quadrants' magnitudes (S ~ 1, D and K' ~ sig,
+    // H ~ sig^2), so the relative tolerance holds for every quadrant; the
+    // stored factors are unscaled afterwards.
+    float sig = 1.f;
 };
-// Operators of the hierarchical path (single medium, rigid surface): the
-// system matrix and the right-hand-side operator of the scattered-field
-// equations (reference src/solvers/bem/full.jl:103-111,144-148).
+// Operators of the hierarchical path: for rigid surfaces the system matrix
+// and the right-hand-side operator of the scattered-field equations
+// (reference src/solvers/bem/full.jl:103-111,144-148); for fluid bodies the
+// 2N x 2N Mueller system (full.jl:341), whose unknown I < N is the pressure
+// and N + i the normal derivative on panel i.
 enum HKind : int {
     HK_BM = 0,        // 1/2 I - D - (i/k) H
     HK_CBIE = 1,      // 1/2 I - D
     HK_BM_RHS = 2,    // S + (i/k)(1/2 I + K')
     HK_CBIE_RHS = 3,  // S
+    HK_MULLER = 4,    // [I - De + Di, Se - g Si; -He + Hi / g, I + K'e - K'i]
+};
+// Wavenumbers and density contrast of an operator (ki and g: Mueller only).
+struct HPar {
+    double k, ki, g;
 };
 constexpr int kAcaCap = 256;  // largest rank of one ACA block
-void launch_hdense(hipStream_t s, const DevMesh& m, int kind, double k, const int* perm, int nblk, const HBlk* blk,
-                   z32* val);
-// near pairs (original panel indices pi, pj) and self panels into dense-block slots
-void launch_hnear_self(hipStream_t s, const DevMesh& m, int kind, double k, int64_t npairs, const int* pi,
-                       const int* pj, const int64_t* dst, const int64_t* self_dst, z32* val);
+void launch_hdense(hipStream_t s, const DevMesh& m, int kind, const HPar& p, const int* perm, int nblk,
+                   const HBlk* blk, z32* val);
+// near pairs (unknown indices pi, pj) and pairs of unknowns on one panel (si,
+// sj) into dense-block slots
+void launch_hnear_self(hipStream_t s, const DevMesh& m, int kind, const HPar& p, int64_t npairs, const int* pi,
+                       const int* pj, const int64_t* dst, int64_t nself, const int* si, const int* sj,
+                       const int64_t* self_dst, z32* val);
 // partial-pivot ACA of the listed low-rank blocks (all when list is null)
-void launch_aca(hipStream_t s, const DevMesh& m, int kind, double k, const int* perm, int nlist, const int* list,
+void launch_aca(hipStream_t s, const DevMesh& m, int kind, const HPar& p, const int* perm, int nlist, const int* list,
                 HBlk* blk, z32* uv, int* conv, double tol);
 
 // ---- bem_linalg.hip ----

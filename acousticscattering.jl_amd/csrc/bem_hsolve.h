@@ -47,6 +47,14 @@ void launch_gm_sub(hipStream_t s, int64_t tot, const z64* a, const z64* b, z64* 
 void launch_hqinc(hipStream_t s, const DevMesh& m, double k, int nr, const double* dirs, const int* perm, z64* Q);
 void launch_htraces(hipStream_t s, const DevMesh& m, double k, int nr, const double* dirs, const int* perm,
                     const z64* X, z64* P, z64* Q);
+// Mueller: [p_inc; q_inc] in tree order over 2 nt unknowns, and the
+// scattered traces P, Q (nt x nr) from the total-field solution X
+void launch_hfluid_rhs(hipStream_t s, const DevMesh& m, double k, int nr, const double* dirs, const int* perm,
+                       z64* B);
+void launch_hfluid_traces(hipStream_t s, const DevMesh& m, double k, int nr, const double* dirs, const int* perm,
+                          const z64* X, z64* P, z64* Q);
+// undo the Mueller ACA scaling (HBlk::sig) of the listed compacted blocks
+void launch_hunscale(hipStream_t s, int nblk, const HBlk* blk, const int* list, const int* perm, int nt, z32* uv);
 void launch_hcompact(hipStream_t s, int nblk, const HBlk* oldb, const HBlk* newb, const z32* src, z32* dst);
 
 }  // namespace bem

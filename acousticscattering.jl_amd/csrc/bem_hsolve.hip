@@ -432,6 +432,47 @@ __global__ void k_htraces(DevMesh m, double k, int nr, const double* __restrict_
     Q[(int64_t)c * m.nt + i] = z64{-q.x, -q.y};
 }
 
+// Mueller right-hand side [p_inc; q_inc] in tree order (unknown I < nt: the
+// pressure of panel I; I >= nt: the normal derivative of panel I - nt).
+__global__ void k_hfluid_rhs(DevMesh m, double k, int nr, const double* __restrict__ dirs, const int* __restrict__ perm,
+                             z64* __restrict__ B) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= 2 * (int64_t)m.nt * nr) return;
+    const int pos = (int)(e / nr), c = (int)(e % nr);
+    const int I = perm[pos];
+    z64 p, q;
+    inc_at(m, k, dirs + 3 * c, I < m.nt ? I : I - m.nt, p, q);
+    B[e] = I < m.nt ? p : q;
+}
+// scattered traces from the Mueller solution (total fields): x - incident
+__global__ void k_hfluid_traces(DevMesh m, double k, int nr, const double* __restrict__ dirs,
+                                const int* __restrict__ perm, const z64* __restrict__ X, z64* __restrict__ P,
+                                z64* __restrict__ Q) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= 2 * (int64_t)m.nt * nr) return;
+    const int pos = (int)(e / nr), c = (int)(e % nr);
+    const int I = perm[pos];
+    const int i = I < m.nt ? I : I - m.nt;
+    z64 p, q;
+    inc_at(m, k, dirs + 3 * c, i, p, q);
+    const z64 x = X[e];
+    if (I < m.nt)
+        P[(int64_t)c * m.nt + i] = z64{x.x - p.x, x.y - p.y};
+    else
+        Q[(int64_t)c * m.nt + i] = z64{x.x - q.x, x.y - q.y};
+}
+void launch_hfluid_rhs(hipStream_t s, const DevMesh& m, double k, int nr, const double* dirs, const int* perm,
+                       z64* B) {
+    const int64_t tot = 2 * (int64_t)m.nt * nr;
+    hipLaunchKernelGGL(k_hfluid_rhs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, k, nr, dirs, perm, B);
+}
+void launch_hfluid_traces(hipStream_t s, const DevMesh& m, double k, int nr, const double* dirs, const int* perm,
+                          const z64* X, z64* P, z64* Q) {
+    const int64_t tot = 2 * (int64_t)m.nt * nr;
+    hipLaunchKernelGGL(k_hfluid_traces, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, k, nr, dirs, perm,
+                       X, P, Q);
+}
+
 void launch_hqinc(hipStream_t s, const DevMesh& m, double k, int nr, const double* dirs, const int* perm, z64* Q) {
     const int64_t tot = (int64_t)m.nt * nr;
     hipLaunchKernelGGL(k_hqinc, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, k, nr, dirs, perm, Q);
@@ -441,6 +482,30 @@ void launch_htraces(hipStream_t s, const DevMesh& m, double k, int nr, const dou
     const int64_t tot = (int64_t)m.nt * nr;
     hipLaunchKernelGGL(k_htraces, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, k, nr, dirs, perm, X, P,
                        Q);
+}
+
+// Undo the ACA scaling of Mueller blocks in the compacted factors: rows of
+// normal-derivative unknowns of U times sig, of V divided by sig.
+__global__ void k_hunscale(int nblk, const HBlk* __restrict__ blk, const int* __restrict__ list,
+                           const int* __restrict__ perm, int nt, z32* __restrict__ uv) {
+    const HBlk b = blk[list[blockIdx.x]];
+    if (b.rank == 0 || b.sig == 1.f) return;
+    z32* U = uv + b.off;
+    z32* V = uv + b.off + (int64_t)b.rank * b.m;
+    const float is = 1.f / b.sig;
+    for (int64_t e = threadIdx.x; e < (int64_t)b.rank * b.m; e += 256)
+        if (perm[b.row_lo + (int)(e % b.m)] >= nt) {
+            U[e].x *= b.sig;
+            U[e].y *= b.sig;
+        }
+    for (int64_t e = threadIdx.x; e < (int64_t)b.rank * b.n; e += 256)
+        if (perm[b.col_lo + (int)(e % b.n)] >= nt) {
+            V[e].x *= is;
+            V[e].y *= is;
+        }
+}
+void launch_hunscale(hipStream_t s, int nblk, const HBlk* blk, const int* list, const int* perm, int nt, z32* uv) {
+    if (nblk > 0) hipLaunchKernelGGL(k_hunscale, dim3(nblk), dim3(256), 0, s, nblk, blk, list, perm, nt, uv);
 }
 
 // compact copy of low-rank factors: dst[i] for i < len from src

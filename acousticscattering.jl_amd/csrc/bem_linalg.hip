@@ -367,6 +367,7 @@ template <typename R>
 struct TsCand {
     R v;
     int pos;
+    C2<R> ip;  // inverse of the row's pivot value (zero for a zero pivot)
 };
 
 template <typename R>
@@ -467,11 +468,16 @@ __device__ __forceinline__ C2<R> cfms(C2<R> a, C2<R> l, C2<R> p) {
 // Elimination steps [j0, j1) with a live window of W columns per lane (32
 // while the upper half-row still holds live columns, 16 once fewer than 16
 // remain).  The winner of step j writes its global row index to sh.sel[j].
+// Each wave publishes its best live row, its key and the inverse of its
+// pivot value before the barrier; after it, every lane loads the winning
+// half-row in one batch of LDS reads (8-value groups for FP64, to bound the
+// registers) and eliminates.
 template <typename R, int W>
 __device__ __forceinline__ void ts_steps(C2<R> (&a)[32], int j0, int j1, bool& done, int& mypos, int myrow,
                                          TsLds<R>& sh) {
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     const bool upper = lane >= 32;
+    constexpr int G = sizeof(R) == sizeof(float) ? W : 8;  // pivot values per LDS batch
 #pragma unroll 1
     for (int j = j0; j < j1; ++j) {
         const int b = j & 1;
@@ -479,11 +485,16 @@ __device__ __forceinline__ void ts_steps(C2<R> (&a)[32], int j0, int j1, bool& d
         const R v = upper ? R(-1) : ts_key(a[0], !done);
         const R wv = wave_max_u(v);
         const int wp = wave_min_u((!upper && v == wv) ? mypos : INT_MAX);
-        if (lane == 0) {
-            sh.cand[b][g].v = wv;
-            sh.cand[b][g].pos = wp;
+        const bool best = !done && mypos == wp;  // both lanes of that row
+        const unsigned long long bl = __ballot(best && !upper);
+        C2<R> ipw{0, 0};
+        if (bl != 0) {
+            const int src = __ffsll(bl) - 1;
+            const C2<R> pv{rdl(a[0].x, src), rdl(a[0].y, src)};
+            if (pv.x != 0 || pv.y != 0) ipw = ts_inv(pv);
         }
-        if (!done && mypos == wp) {  // both lanes of the wave's best row
+        if (lane == 0) sh.cand[b][g] = TsCand<R>{wv, wp, ipw};
+        if (best) {
 #pragma unroll
             for (int c = 0; c < W; ++c) sh.prow[b][g][(upper ? kTsPS : 0) + c] = a[c];
         }
@@ -494,17 +505,17 @@ __device__ __forceinline__ void ts_steps(C2<R> (&a)[32], int j0, int j1, bool& d
         for (int q = 0; q < 4; ++q) cq[q] = sh.cand[b][q];
         R gv = cq[0].v;
         int gp = cq[0].pos, gw = 0;
+        C2<R> ip = cq[0].ip;
 #pragma unroll
         for (int q = 1; q < 4; ++q) {
-            const bool t = cq[q].v > gv || (cq[q].v == gv && cq[q].pos < gp);
+            const bool t = (cq[q].v > gv) | ((cq[q].v == gv) & (cq[q].pos < gp));
             gv = t ? cq[q].v : gv;
             gp = t ? cq[q].pos : gp;
             gw = t ? q : gw;
+            ip.x = t ? cq[q].ip.x : ip.x;
+            ip.y = t ? cq[q].ip.y : ip.y;
         }
-        const C2<R> pv = sh.prow[b][gw][0];
         const C2<R>* pr = sh.prow[b][gw] + (upper ? kTsPS : 0);
-        const bool nz = pv.x != 0 || pv.y != 0;
-        const C2<R> ip = nz ? ts_inv(pv) : C2<R>{0, 0};
         const C2<R> l0 = cmul(a[0], ip);
         if (g == gw && !done && mypos == gp) {
             done = true;
@@ -514,20 +525,26 @@ __device__ __forceinline__ void ts_steps(C2<R> (&a)[32], int j0, int j1, bool& d
         }
         // eliminate column j and shift the window left by one column; retired
         // rows compute values nobody reads
+        C2<R> l = l0, o{0, 0};
         if (W == 32) {
             // one half-wave swap: the lower lane sends the multiplier, the
             // upper lane its window column 32 (which moves to the lower lane)
             const C2<R> s = upper ? a[0] : l0;
-            const C2<R> o{xhalf(s.x, upper), xhalf(s.y, upper)};
-            const C2<R> l = upper ? o : l0;
-            const C2<R> c32 = sh.prow[b][gw][kTsPS];
-#pragma unroll
-            for (int c = 0; c < W - 1; ++c) a[c] = cfms(a[c + 1], l, pr[c + 1]);
-            a[W - 1] = upper ? C2<R>{0, 0} : cfms(o, l, c32);
-        } else {
-#pragma unroll
-            for (int c = 0; c < W - 1; ++c) a[c] = cfms(a[c + 1], l0, pr[c + 1]);
+            o = C2<R>{xhalf(s.x, upper), xhalf(s.y, upper)};
+            l = upper ? o : l0;
         }
+        const C2<R> c32 = W == 32 ? sh.prow[b][gw][kTsPS] : C2<R>{0, 0};
+#pragma unroll
+        for (int c0 = 0; c0 < W - 1; c0 += G) {
+            C2<R> p[G];
+#pragma unroll
+            for (int t = 0; t < G; ++t)
+                if (c0 + t + 1 < W) p[t] = pr[c0 + t + 1];
+#pragma unroll
+            for (int t = 0; t < G; ++t)
+                if (c0 + t < W - 1) a[c0 + t] = cfms(a[c0 + t + 1], l, p[t]);
+        }
+        if (W == 32) a[W - 1] = upper ? C2<R>{0, 0} : cfms(o, l, c32);
     }
 }
 

@@ -268,10 +268,18 @@ int bem_server_stats_get(bem_server srv, bem_server_stats *out);
  * path's scattered-field equation (full.jl:103-111,144-148): the system
  * operator 1/2 I - D [- (i/k) H] and the right-hand-side operator
  * S [+ (i/k)(1/2 I + K')] are both compressed over one cluster tree; the
- * scattered traces are x and -dp_inc/dn.  Blocks are complex FP32 (dense
- * leaves and ACA factors U V^T); GMRES runs in FP64 with FP64 residuals of
- * that operator, right-preconditioned by block Jacobi on the diagonal leaf
- * blocks. */
+ * scattered traces are x and -dp_inc/dn.  Fluid bodies in the Mueller
+ * formulation (compression/fluid.jl:20-106): the 2N x 2N system
+ * [I - De + Di, Se - g Si; -He + Hi/g, I + K'e - K'i] is compressed as one
+ * operator over a cluster tree of its 2N unknowns (unknown i < N the
+ * pressure, N + i the normal derivative on panel i, both at the centroid),
+ * the right-hand side is [p_inc; dp_inc/dn] and the scattered traces are x
+ * minus the incident traces; bodies in the low-frequency regime
+ * (max(k, k_int) * radius <= pi/2, where the reference takes K' and H from
+ * the Calderon identities through a hierarchical LU of S) are refused with
+ * BEM_ERR_ARGUMENT.  Blocks are complex FP32 (dense leaves and ACA factors
+ * U V^T); GMRES runs in FP64 with FP64 residuals of that operator,
+ * right-preconditioned by block Jacobi on the diagonal leaf blocks. */
 typedef struct bem_hmatrix_s *bem_hmatrix;
 
 typedef struct {
@@ -306,7 +314,8 @@ int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem *problem,
                        const bem_hmatrix_options *options, bem_hmatrix *out);
 int bem_hmatrix_destroy(bem_hmatrix h);
 int bem_hmatrix_get_info(bem_hmatrix h, bem_hmatrix_info *out);
-/* y = A x for n_rhs columns (panel order, n x n_rhs column-major complex). */
+/* y = A x for n_rhs columns (unknown order, n x n_rhs column-major complex;
+ * n = bem_hmatrix_info.n: N panels, or 2N unknowns for a fluid body). */
 int bem_hmatrix_matvec(bem_hmatrix h, int32_t n_rhs, const double *x, double *y);
 /* GMRES for n_rhs plane-wave incidences (angles as in bem_operator_solve);
  * the solution carries traces, far field and per-incidence diagnostics

@@ -1,7 +1,8 @@
 """Hierarchical-matrix path on the GPU: compressed apply against the dense
-system matrix, GMRES against the dense LU solve of the same equation,
-and the physics goldens (sphere modal series; the reference's bent-cylinder
-amplitudes, test/extended/bem/cylinder/full3d.jl:44-47).
+system matrix, GMRES against the dense LU solve of the same equation (rigid
+surfaces and fluid bodies in the Mueller formulation), and the physics
+goldens (sphere modal series; the reference's bent-cylinder amplitudes,
+test/extended/bem/cylinder/full3d.jl:44-47).
 
 Tolerances: the compressed operator holds FP32 blocks with ACA tolerance
 1e-6, so apply and solution agree with the FP64 dense forms to 2e-5
@@ -128,3 +129,71 @@ def test_solve_full_bem_compression_keyword(ctx, sphere4):
     assert np.abs(p - p0).max() / np.abs(p0).max() < 2e-5
     with pytest.raises(ValueError):
         bh.solve_full_bem(bh.Rigid(), K1, gm, compression={"method": "fmm"})
+
+
+# fluid Mueller (compression/fluid.jl:20-106): a spheroid at a wavenumber
+# above the Calderon regime (k * radius = 10 > pi/2)
+FLUID = (1.2, 1.1)
+K_F = 500.0
+
+
+@pytest.fixture(scope="module")
+def spheroid3(ctx):
+    nodes, tri = meshes.spheroid(0.02, 0.01, 3)
+    return bh.Mesh(nodes, tri + 1, ctx=ctx), O.Mesh(nodes, tri)
+
+
+def test_fluid_apply_matches_dense_mueller(ctx, spheroid3):
+    say("[gpu] hmatrix fluid Mueller apply vs dense, spheroid L3")
+    gm, om = spheroid3
+    assert not O.calderon_applies(om, K_F, FLUID[1], "muller")
+    H = bh.HMatrix(gm, bh.FluidFilled(*FLUID), K_F)
+    info = H.info()
+    assert info["n"] == 2 * gm.n and info["lowrank_blocks"] > 0 and info["unconverged_blocks"] == 0
+    A = bh.Operator(gm, bh.FluidFilled(*FLUID), K_F).matrix()
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((2 * gm.n, 3)) + 1j * rng.standard_normal((2 * gm.n, 3))
+    Y = H.matvec(X)
+    ref = A @ X
+    assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) < 2e-5
+
+
+def test_fluid_gmres_matches_dense_solve(ctx, spheroid3):
+    """Dense and compressed Mueller agree to the compression accuracy (the
+    reference's own bar is 2e-5, test/basic/bem/formulations.jl:344-345)."""
+    say("[gpu] hmatrix fluid Mueller GMRES vs dense LU, spheroid L3")
+    gm, om = spheroid3
+    angles = [(0.0, 0.0), (1.1, 0.4)]
+    sol = bh.HMatrix(gm, bh.FluidFilled(*FLUID), K_F).solve(angles)
+    dense = bh.Operator(gm, bh.FluidFilled(*FLUID), K_F).solve(angles)
+    for r in range(len(angles)):
+        d = sol.diagnostics(r)
+        assert d["method"] == "gmres" and d["converged"] and d["relative_residual"] <= 1e-8
+        assert d["derivative_evaluation"] == {"exterior": "direct", "interior": "direct"}
+        p, q = sol.traces(r)
+        p0, q0 = dense.traces(r)
+        assert np.abs(p - p0).max() / np.abs(p0).max() < 2e-5
+        assert np.abs(q - q0).max() / np.abs(q0).max() < 2e-5
+        f, f0 = sol.far_field(r)[0], dense.far_field(r)[0]
+        assert abs(f - f0) / abs(f0) < 2e-5
+
+
+def test_fluid_compression_argument_errors(ctx, spheroid3):
+    gm, _ = spheroid3
+    with pytest.raises(ValueError, match="muller"):
+        bh.HMatrix(gm, bh.FluidFilled(*FLUID), K_F, formulation="cbie")
+    with pytest.raises(ValueError, match="Calderon"):
+        bh.HMatrix(gm, bh.FluidFilled(*FLUID), 10.0)
+    with pytest.raises(ValueError):
+        bh.HMatrix(gm, bh.PressureRelease(), K_F)
+
+
+def test_solve_full_bem_fluid_compression_keyword(ctx, spheroid3):
+    gm, _ = spheroid3
+    bc = bh.FluidFilled(*FLUID)
+    p, q, _, d = bh.solve_full_bem(bc, K_F, gm, incidence_angle=0.3, compression={"method": "hmatrix"},
+                                   return_diagnostics=True)
+    p0, q0, _ = bh.solve_full_bem(bc, K_F, gm, incidence_angle=0.3)
+    assert d["method"] == "gmres" and d["formulation"] == "muller" and d["converged"]
+    assert np.abs(p - p0).max() / np.abs(p0).max() < 2e-5
+    assert np.abs(q - q0).max() / np.abs(q0).max() < 2e-5
