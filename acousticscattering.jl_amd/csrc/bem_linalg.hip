@@ -4,7 +4,7 @@
 // float (mixed path).  Row-major keeps the partial-pivoting row interchanges
 // and the panel's per-row reads contiguous.
 //
-// LU: right-looking, 64-wide panels inside 256-wide outer blocks.  The
+// LU: right-looking, 64-wide panels inside 512-wide outer blocks.  The
 // reference factors with LAPACK zgetrf/cgetrf (partial pivoting,
 // src/solvers/bem/full.jl:404, src/solvers/bem/fluid_system.jl:217); the
 // panels here pivot by tournament (communication-avoiding LU), which has the
@@ -14,9 +14,11 @@
 //           L21 = A21 inv(U11) on MFMA
 //   swap    row interchanges outside the panel (coalesced row segments)
 //   trsm    U12 = inv(L11) A12
-//   gemm    A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32),
-//           complex as four real products, 128x128 tiles staged in LDS
-// Solves use the stored block inverses: one launch per 64-row block.
+//   gemm    A22 -= L21 U12 on MFMA: v_mfma_f32_16x16x4f32 with three real
+//           products per complex product (FP32), v_mfma_f64_16x16x4f64 with
+//           four (FP64); 128x128 tiles staged in LDS
+// Solves use the stored block inverses: one launch per solve, forward and
+// backward phases together.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -1038,17 +1040,37 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
     constexpr int KV = kG3P / 4;  // v4 slots per LDS row
     __shared__ v4 sA[2][2][kGM][KV];
     __shared__ v4 sB[2][2][kGN][KV];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: tile offsets stay scalar
     const int wm = (wv >> 2) * 64, wn = (wv & 3) * 32;
     int tm, tn;
     g3_tile(gridDim.x, gridDim.y, tm, tn);
     const int m0 = tm * kGM, n0 = tn * kGN;
     const bool full = m0 + kGM <= M && n0 + kGN <= N;
+    const int li = lane & 15, lq = lane >> 4;
     v4 t1[4][2], t2[4][2], t3[4][2];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) t1[x][y] = t2[x][y] = t3[x][y] = v4{0, 0, 0, 0};
+    // lane (li, lq) holds rows 4 lq + r and column li of each 16 x 16 block
+    const z32* cl = C + (int64_t)(4 * lq) * ldc + li;
+    if (!ASSIGN) {
+        // C - A B: the accumulators start from C (t1 = -Re C, t3 = -Re C - Im C),
+        // so C is read once, up front, and the epilogue only stores
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int rb = m0 + wm + x * 16 + r, cb = n0 + wn + y * 16;
+                    z32 c{0.f, 0.f};
+                    if (full || (rb + 4 * lq < M && cb + li < N)) c = cl[(int64_t)rb * ldc + cb];
+                    t1[x][y][r] = -c.x;
+                    t3[x][y][r] = -c.x - c.y;
+                }
+    }
     // staging: A row m0 + lm and B column n0 + lm, k = kc + lk .. kc + lk + 7
     constexpr int PT = kG3K / 4;  // k values per thread and chunk
     const int lm = tid & 127, lk = (tid >> 7) * PT;
@@ -1085,7 +1107,6 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
     gload(0);
     lstore(0);
     __syncthreads();
-    const int li = lane & 15, lq = lane >> 4;
     int buf = 0;
     for (int kc = 0; kc < K; kc += kG3K) {
         const bool more = kc + kG3K < K;
@@ -1128,25 +1149,18 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
         __syncthreads();
         buf ^= 1;
     }
+    z32* cs = C + (int64_t)(4 * lq) * ldc + li;
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm + x * 16 + Mfma<float>::row(lane, r);
-                const int gn = n0 + wn + y * 16 + li;
-                if (gm < M && gn < N) {
+                const int rb = m0 + wm + x * 16 + r, cb = n0 + wn + y * 16;
+                if (full || (rb + 4 * lq < M && cb + li < N)) {
                     const float re = t1[x][y][r] - t2[x][y][r];
                     const float im = t3[x][y][r] - t1[x][y][r] - t2[x][y][r];
-                    if (ASSIGN) {
-                        C[(int64_t)gm * ldc + gn] = z32{re, im};
-                    } else {
-                        z32 c = C[(int64_t)gm * ldc + gn];
-                        c.x -= re;
-                        c.y -= im;
-                        C[(int64_t)gm * ldc + gn] = c;
-                    }
+                    cs[(int64_t)rb * ldc + cb] = ASSIGN ? z32{re, im} : z32{-re, -im};
                 }
             }
 }
@@ -1207,9 +1221,11 @@ static void launch_update(hipStream_t s, C2<R>* A, int64_t lda, int r0, int c0, 
     prof_end(prof, s, CLS_GEMM, 8.0 * (double)M * N * K, (2.0 * (double)M * N + (double)(M + N) * K) * sizeof(C2<R>));
 }
 
-// Two-level right-looking LU: 64-wide panels inside 256-wide outer blocks, so
-// the trailing update of each outer block is one K = 256 MFMA GEMM.
-constexpr int kOuter = 256;
+// Two-level right-looking LU: 64-wide panels inside 512-wide outer blocks, so
+// the trailing update of each outer block is one K = 512 MFMA GEMM (deep
+// enough that reading and writing C once per update costs little against the
+// products).
+constexpr int kOuter = 512;
 
 // The panels of outer block [K0, Kend): tournament panel, pivot-block
 // factors, L21, and the updates confined to the block's own columns.
@@ -1242,8 +1258,8 @@ static int panel_phase(hipStream_t s, C2<R>* A, int64_t lda, int n, int K0, int 
     return 0;
 }
 
-// Two-level right-looking LU: 64-wide panels inside 256-wide outer blocks, so
-// the trailing update of each outer block is one K = 256 MFMA GEMM.  With a
+// Two-level right-looking LU: 64-wide panels inside 512-wide outer blocks, so
+// the trailing update of each outer block is one K = 512 MFMA GEMM.  With a
 // side stream (LuStreams::side) the panels of outer block j + 1 run there,
 // overlapped with block j's trailing update: the update of block j + 1's
 // columns goes first (event `ready`), block j + 1's panel phase follows on the
