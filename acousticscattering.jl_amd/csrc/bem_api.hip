@@ -47,9 +47,9 @@ struct BemError : std::runtime_error {
             throw BemError(BEM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
-const char* kClassNames[] = {"assemble", "near", "lu_panel", "lu_gemm", "lu_trsm", "lu_swap",
-                             "solve",    "residual", "far_field", "h_apply", "h_aca", "h_ortho"};
-constexpr int kNumClasses = 12;
+const char* kClassNames[] = {"assemble", "near",      "lu_panel", "lu_gemm", "lu_trsm", "lu_swap",     "solve",
+                             "residual", "far_field", "h_apply",  "h_aca",   "h_ortho", "lu_gemm_fp64"};
+constexpr int kNumClasses = 13;
 enum { C_ASSEMBLE = 0, C_NEAR = 1, C_RESIDUAL = 7, C_FAR = 8, C_HAPPLY = 9, C_HACA = 10, C_HORTHO = 11 };
 
 struct ProfRec {

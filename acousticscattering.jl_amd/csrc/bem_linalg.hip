@@ -1175,7 +1175,7 @@ struct ProfHook;  // defined in the API unit; null when profiling is off
     } while (0)
 void prof_begin(void* prof, hipStream_t s, int cls);
 void prof_end(void* prof, hipStream_t s, int cls, double flops, double bytes);
-enum { CLS_PANEL = 2, CLS_GEMM = 3, CLS_TRSM = 4, CLS_SWAP = 5, CLS_SOLVE = 6 };
+enum { CLS_PANEL = 2, CLS_GEMM = 3, CLS_TRSM = 4, CLS_SWAP = 5, CLS_SOLVE = 6, CLS_GEMM64 = 12 };
 
 
 // Trailing-update GEMM: 3M products for complex FP32; four real products for
@@ -1210,15 +1210,17 @@ static void launch_trsm(hipStream_t s, C2<R>* A, int64_t lda, int k0, int bs, in
 }
 
 // C[M x N] -= A[M x K] B[K x N] with C at (r0, c0), A at (r0, ka), B at (ka, c0).
+// Timed as lu_gemm (FP32, the 3M kernel) or lu_gemm_fp64.
 template <typename R>
 static void launch_update(hipStream_t s, C2<R>* A, int64_t lda, int r0, int c0, int ka, int M, int N, int K,
                           void* prof) {
     if (M <= 0 || N <= 0 || K <= 0) return;
-    prof_begin(prof, s, CLS_GEMM);
+    const int cls = sizeof(R) == sizeof(float) ? CLS_GEMM : CLS_GEMM64;
+    prof_begin(prof, s, cls);
     dim3 grid((N + kGN - 1) / kGN, (M + kGM - 1) / kGM);
     launch_gemm<R, 0>(s, grid, A + (int64_t)r0 * lda + ka, lda, A + (int64_t)ka * lda + c0, lda,
                       A + (int64_t)r0 * lda + c0, lda, M, N, K);
-    prof_end(prof, s, CLS_GEMM, 8.0 * (double)M * N * K, (2.0 * (double)M * N + (double)(M + N) * K) * sizeof(C2<R>));
+    prof_end(prof, s, cls, 8.0 * (double)M * N * K, (2.0 * (double)M * N + (double)(M + N) * K) * sizeof(C2<R>));
 }
 
 // Two-level right-looking LU: 64-wide panels inside 512-wide outer blocks, so

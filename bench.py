@@ -64,16 +64,48 @@ def sweep_plan(bh, ctx, n_points, max_level):
     return plan, meshes
 
 
-def run_sweep(bh, ctx, plan, meshes, precision, chunk=64):
+TRAFFIC_PROFILE = os.path.join("profiles", "lu_gemm3m_traffic_fluid_spheroid_n20480.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FP32 trailing update, from the committed
+    rocprofv3 --pmc summary of `bench.py --workload solve` (one N = 20480
+    Mueller solve; the solves of that size carry most of the sweep's GEMM
+    time): FETCH_SIZE and WRITE_SIZE passes, each scaled by the counter's
+    reading for a plain copy of known size.  None when the summary is
+    absent."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def run_sweep(bh, ctx, plan, meshes, precision, chunk=64, serial=False):
     """One pass over the sweep, largest meshes first, in synchronous chunks of
     `chunk` solves (so progress can be reported); returns per-solve results in
-    plan order and the wall time of the pass."""
+    plan order and the wall time of the pass.  serial: one solve at a time on
+    the calling thread (bem), for profilers that cannot follow the sweep's
+    worker threads; the same solves and kernels."""
     fl = bh.FluidFilled(G_RHO, H_C)
     order = sorted(range(len(plan)), key=lambda i: -plan[i][2])
     amps = np.empty(len(plan), dtype=np.complex128)
     lat = np.empty(len(plan))
     diags = [None] * len(plan)
     t0 = time.perf_counter()
+    if serial:
+        for n, i in enumerate(order):
+            t1 = time.perf_counter()
+            sol = bh.bem(meshes[plan[i][2]], fl, plan[i][1], incidence_angle=0.0, precision=precision)
+            amps[i] = bh.scattering_amplitude(sol)
+            lat[i] = (time.perf_counter() - t1) * 1e3
+            diags[i] = sol.diagnostics
+            if (n + 1) % chunk == 0 or n + 1 == len(order):
+                log(f"  {precision}: {n + 1}/{len(plan)} solves, {time.perf_counter() - t0:.1f}s")
+        return amps, lat, diags, time.perf_counter() - t0
     for c0 in range(0, len(order), chunk):
         idx = order[c0:c0 + chunk]
         ms = [meshes[plan[i][2]] for i in idx]
@@ -189,6 +221,53 @@ def serve_main(args):
     ctx.close()
 
 
+def solve_main(args):
+    """Config 2: one fluid Mueller solve of the 0.02 x 0.01 m spheroid at
+    k = 500 rad/m, N = 20480 panels (2N = 40960 unknowns), mixed precision,
+    on the calling thread (the workload the traffic counters are read on)."""
+    import bemhip as bh
+
+    ctx = bh.Context(0)
+    fl = bh.FluidFilled(G_RHO, H_C)
+    warm = bh.mesh(bh.Spheroid(A_MAJOR, B_MINOR), level=3, ctx=ctx)  # loads every kernel once
+    bh.bem(warm, fl, 500.0, incidence_angle=0.0, precision="mixed")
+    mesh = bh.mesh(bh.Spheroid(A_MAJOR, B_MINOR), level=5, ctx=ctx)
+    times = []
+    for s in range(args.warmup + args.steps):
+        if s == args.warmup:
+            ctx.reset_stats()
+            ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        sol = bh.bem(mesh, fl, 500.0, incidence_angle=0.0, precision="mixed")
+        f = bh.scattering_amplitude(sol)
+        if s >= args.warmup:
+            times.append(time.perf_counter() - t0)
+        log(f"solve {s + 1}: {time.perf_counter() - t0:.3f}s, TS {bh.target_strength(f):.4f} dB")
+    ctx.set_profiling(False)
+    g = ctx.kernel_stats("lu_gemm")
+    achieved = g["flops"] / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
+    t = float(np.mean(times))
+    out = {
+        "metric": "single fluid Mueller solve time (N=20480, 2N=40960 unknowns, mixed precision)",
+        "value": t * 1e3, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c32 LU / f64 refinement", "data": "synthetic",
+        "config": {"workload": "fluid spheroid 0.02x0.01 m (g=1.0357, h=1.0279), k=500 rad/m, "
+                               "icosphere L5 (N=20480), Mueller, mixed precision, one incidence"},
+        "target_strength_db": bh.target_strength(f),
+        "diagnostics": {k: v for k, v in sol.diagnostics.items() if isinstance(v, (int, float, str, bool))},
+        "roofline": {"kernel": "k_gemm3m<float,0> (LU trailing update, 3M complex)", "bound": "mfma",
+                     "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / MFMA_FP32_PEAK, "traffic": measured_traffic(),
+                     "traffic_source": TRAFFIC_PROFILE,
+                     "algorithmic_bytes_per_launch": g["bytes"] / max(1, g["launches"]),
+                     "launches": g["launches"], "avg_launch_ms": g["ms"] / max(1, g["launches"])},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def hmatrix_main(args):
     import bemhip as bh
 
@@ -247,7 +326,7 @@ def hmatrix_main(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["sweep", "serve", "hmatrix"], default="sweep")
+    ap.add_argument("--workload", choices=["sweep", "serve", "hmatrix", "solve"], default="sweep")
     ap.add_argument("--hm-nphi", type=int, default=138)
     ap.add_argument("--hm-rhs", type=int, default=64)
     ap.add_argument("--rates", type=float, nargs="+", default=[50.0, 200.0, 800.0])
@@ -259,6 +338,8 @@ def main():
     ap.add_argument("--points", type=int, default=256)
     ap.add_argument("--max-level", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serial", action="store_true",
+                    help="sweep one solve at a time on the main thread (for rocprofv3 --pmc)")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -270,6 +351,9 @@ def main():
         return
     if args.workload == "hmatrix":
         hmatrix_main(args)
+        return
+    if args.workload == "solve":
+        solve_main(args)
         return
 
     import bemhip as bh
@@ -286,24 +370,28 @@ def main():
     fl = bh.FluidFilled(G_RHO, H_C)
     small = meshes[min(meshes)]
     for prec in ("double", "mixed"):
-        bh.batched_sweep([small], [bh.make_problem(fl, plan[0][1], precision=prec)], [(0.0, 0.0)], ctx=ctx)
+        if args.serial:
+            bh.bem(small, fl, plan[0][1], precision=prec)
+        else:
+            bh.batched_sweep([small], [bh.make_problem(fl, plan[0][1], precision=prec)], [(0.0, 0.0)], ctx=ctx)
     log("FP64 reference sweep ...")
-    a64, lat64, _, t64 = run_sweep(bh, ctx, plan, meshes, "double")
+    a64, lat64, _, t64 = run_sweep(bh, ctx, plan, meshes, "double", serial=args.serial)
     log(f"FP64 sweep {t64:.2f}s ({len(plan) / t64:.2f} solves/s)")
     for w in range(args.warmup):
         log(f"warmup {w + 1}/{args.warmup} ...")
-        run_sweep(bh, ctx, plan, meshes, "mixed")
+        run_sweep(bh, ctx, plan, meshes, "mixed", serial=args.serial)
     ctx.reset_stats()
     ctx.set_profiling(True)
     times, lats, amps, diags = [], [], None, None
     for s in range(args.steps):
-        amps, lat, diags, t = run_sweep(bh, ctx, plan, meshes, "mixed")
+        amps, lat, diags, t = run_sweep(bh, ctx, plan, meshes, "mixed", serial=args.serial)
         times.append(t)
         lats.extend(lat.tolist())
         log(f"step {s + 1}/{args.steps}: {t:.2f}s ({len(plan) / t:.2f} solves/s)")
     ctx.set_profiling(False)
     stats = {name: ctx.kernel_stats(name) for name in
-             ("assemble", "near", "lu_panel", "lu_gemm", "lu_trsm", "lu_swap", "solve", "residual", "far_field")}
+             ("assemble", "near", "lu_panel", "lu_gemm", "lu_gemm_fp64", "lu_trsm", "lu_swap", "solve", "residual",
+              "far_field")}
     ts64 = 20 * np.log10(np.abs(a64))
     tsm = 20 * np.log10(np.abs(amps))
     dts = float(np.max(np.abs(tsm - ts64)))
@@ -333,7 +421,7 @@ def main():
                          f"clamped to N<={20 * 4 ** args.max_level}, Mueller 2N unknowns, mixed precision"
                          + ("" if full else " [REDUCED]")),
             "panels_per_class": {str(k): v for k, v in sorted(sizes.items())},
-            "lanes": args.lanes,
+            "lanes": "serial" if args.serial else args.lanes,
             "global_batch": len(plan),
         },
         "max_abs_dTS_dB": dts,
@@ -345,7 +433,9 @@ def main():
         "speedup_vs_fp64": value / (len(plan) / t64),
         "roofline": {"kernel": "k_gemm3m<float,0> (LU trailing update, 3M complex)", "bound": "mfma",
                      "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / MFMA_FP32_PEAK, "traffic": None,
+                     "frac": achieved / MFMA_FP32_PEAK, "traffic": measured_traffic(),
+                     "traffic_source": TRAFFIC_PROFILE,
+                     "algorithmic_bytes_per_launch": g["bytes"] / max(1, g["launches"]),
                      "launches": g["launches"], "avg_launch_ms": g["ms"] / max(1, g["launches"])},
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in stats.items()},
         "cpu_baseline": cpu,

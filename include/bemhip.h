@@ -121,7 +121,8 @@ int bem_ctx_memory(bem_ctx ctx, uint64_t *used, uint64_t *budget);
 /* Kernel timing with HIP events on the library's stream: 1 enables. */
 int bem_ctx_set_profiling(bem_ctx ctx, int32_t enable);
 /* Accumulated event time and algorithmic work of one kernel class since the
- * last reset.  Classes: "assemble", "near", "lu_panel", "lu_gemm", "lu_trsm",
+ * last reset.  Classes: "assemble", "near", "lu_panel", "lu_gemm" (FP32
+ * trailing updates), "lu_gemm_fp64" (FP64 trailing updates), "lu_trsm",
  * "lu_swap", "solve", "residual", "far_field", "h_apply" (compressed operator
  * apply; bytes = blocks + vectors), "h_aca", "h_ortho" (GMRES Gram-Schmidt). */
 int bem_ctx_kernel_stats(bem_ctx ctx, const char *name, int64_t *launches, double *total_ms,

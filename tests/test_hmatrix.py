@@ -90,20 +90,29 @@ def test_sphere_against_modal_series(ctx, sphere4):
     assert abs(f - fm) / abs(fm) < 0.01
 
 
-def test_bent_cylinder_reference_amplitudes(ctx):
+# test/extended/bem/cylinder/full3d.jl:38-43: reference30 (beta = pi/6) and
+# reference60 (beta = pi/3), azimuth 0.4, bend radius 2
+BENT_GOLDENS = [
+    (math.pi / 6, [-0.115044319980511 + 0.02223869525583432j, 0.05724218769794465 + 0.01330966815636354j,
+                   -0.07441192295622054 + 0.01052766721323333j]),
+    (math.pi / 3, [-0.2798784736915524 - 0.03732557787576306j, 0.1148429765445959 + 0.02836089828639042j,
+                   -0.04491043881999867 + 0.01607119990249349j]),
+]
+
+
+@pytest.mark.parametrize("beta,reference", BENT_GOLDENS, ids=["reference30", "reference60"])
+def test_bent_cylinder_reference_amplitudes(ctx, beta, reference):
     """Rigid bent capped cylinder (radius 0.5, length 2, bend radius 2, end
-    caps 0.5), k = 1, incidence (pi/6, 0.4): amplitudes toward -d, d and +z
+    caps 0.5), k = 1, incidence (beta, 0.4): amplitudes toward -d, d and +z
     against the reference's independent golden values."""
-    say("[gpu] hmatrix bent cylinder vs reference30")
-    reference30 = [-0.115044319980511 + 0.02223869525583432j, 0.05724218769794465 + 0.01330966815636354j,
-                   -0.07441192295622054 + 0.01052766721323333j]
+    say(f"[gpu] hmatrix bent cylinder, beta = {beta:.4f}")
     body = bh.Cylinder(0.5, 2.0, 2.0, 0.5)
     gm = bh.mesh(body, n_phi=96, ctx=ctx)
-    beta, alpha = math.pi / 6, 0.4
+    alpha = 0.4
     sol = bh.HMatrix(gm, bh.Rigid(), 1.0).solve([(beta, alpha)])
     d = O.direction(beta, alpha)
     f = sol.far_field(0, np.array([-d, d, [0.0, 0.0, 1.0]]))
-    for a, b in zip(f, reference30):
+    for a, b in zip(f, reference):
         assert abs(bh.target_strength(a) - bh.target_strength(b)) < 0.1
         assert abs(a - b) / abs(b) < 0.01
     assert sol.diagnostics(0)["converged"]
