@@ -38,6 +38,7 @@ REASONS = ["none", "conversion", "single_singular", "condition", "nonfinite",
 EXPORTED = [
     "bem_version", "bem_problem_defaults", "bem_ctx_create", "bem_ctx_destroy", "bem_last_error",
     "bem_ctx_memory", "bem_ctx_set_profiling", "bem_ctx_kernel_stats", "bem_ctx_reset_stats", "bem_ctx_set_lanes",
+    "bem_ctx_set_heavy_lanes",
     "bem_icosphere_size", "bem_icosphere", "bem_revolution_size", "bem_revolution",
     "bem_mesh_create", "bem_mesh_validate", "bem_mesh_destroy", "bem_mesh_info", "bem_mesh_panels",
     "bem_mesh_quadrature", "bem_mesh_near_count", "bem_operator_create", "bem_operator_destroy",
@@ -135,6 +136,7 @@ def load_library(path=LIB_PATH):
         "bem_ctx_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_char_p, _ip, _dp, _dp, _dp]),
         "bem_ctx_reset_stats": (ctypes.c_int, [_vp]),
         "bem_ctx_set_lanes": (ctypes.c_int, [_vp, ctypes.c_int32]),
+        "bem_ctx_set_heavy_lanes": (ctypes.c_int, [_vp, ctypes.c_int32]),
         "bem_icosphere_size": (ctypes.c_int, [ctypes.c_int32, _ip, _ip]),
         "bem_icosphere": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, _dp, _ip]),
@@ -287,6 +289,11 @@ class Context:
     def set_lanes(self, max_lanes):
         """Largest number of concurrent solves of batched sweeps (1..8)."""
         self.check(self.lib.bem_ctx_set_lanes(self._h, int(max_lanes)))
+
+    def set_heavy_lanes(self, max_heavy):
+        """Largest number of concurrent sweep solves with 16384 or more
+        unknowns (1..8); the other lanes take smaller solves meanwhile."""
+        self.check(self.lib.bem_ctx_set_heavy_lanes(self._h, int(max_heavy)))
 
     def kernel_stats(self, name):
         n = ctypes.c_int64()

@@ -148,6 +148,7 @@ struct bem_ctx_s {
     Prof prof;
     std::vector<Lane*> lanes;  // lanes[0] serves the single-solve entry points
     int max_lanes = 8;         // concurrent solves of bem_sweep
+    int max_heavy = 2;         // of which heavy ones (kHeavyUnknowns)
 
     Lane* lane0() { return lanes[0]; }
     // Device memory: a caching allocator.  Released blocks go to a free pool
@@ -1035,10 +1036,23 @@ void operator_solve(bem_operator op, int nrhs, const double* angles, bem_solutio
     *out = sol;
 }
 
+// Unknowns of one sweep item's dense system (Mueller fluid systems carry
+// pressure and normal derivative per panel).
+size_t sweep_unknowns(const bem_mesh m, const bem_problem& p) {
+    const size_t nt = (size_t)m->d.nt;
+    return p.boundary == BEM_FLUID ? 2 * nt : nt;
+}
+
+// From this many unknowns on, the first trailing updates of an LU are GEMMs of
+// over 15000 128 x 128 tiles (about 60 per CU), and the factorization is bound
+// by the matrix cores rather than by its panel chain (at 10240 unknowns the
+// panel chain takes about three times the GEMM time).
+constexpr size_t kHeavyUnknowns = 16384;
+
 // Device bytes one sweep item holds in a lane arena.
 size_t sweep_need(const bem_mesh m, const bem_problem& p) {
     const size_t nt = (size_t)m->d.nt;
-    const size_t n = p.boundary == BEM_FLUID ? 2 * nt : nt;
+    const size_t n = sweep_unknowns(m, p);
     const size_t esz = p.precision == BEM_MIXED ? sizeof(z32) : sizeof(z64);
     size_t b = esz * n * n * (p.boundary == BEM_FLUID ? 2 : 3);    // A, LU (+ B)
     b += esz * 2 * 4096 * ((n + 63) / 64);                           // block inverses
@@ -1995,6 +2009,13 @@ int bem_ctx_set_lanes(bem_ctx ctx, int32_t max_lanes) {
     return BEM_OK;
 }
 
+int bem_ctx_set_heavy_lanes(bem_ctx ctx, int32_t max_heavy) {
+    if (!ctx || max_heavy < 1 || max_heavy > 8) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    ctx->max_heavy = max_heavy;
+    return BEM_OK;
+}
+
 int bem_ctx_reset_stats(bem_ctx ctx) {
     if (!ctx) return BEM_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
@@ -2380,32 +2401,66 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh* meshes, const bem_problem*
             }
             L->off = 0;
         }
-        std::atomic<int> next{0};
+        // At most ctx->max_heavy heavy items (kHeavyUnknowns) run at once;
+        // lanes that may not start one take light items (latency-bound panel
+        // chains) meanwhile.  Both lists run largest first.
+        std::vector<int> heavy, light;
+        for (int i : order) (sweep_unknowns(meshes[i], problems[i]) >= kHeavyUnknowns ? heavy : light).push_back(i);
+        size_t hnext = 0, lnext = 0;
+        int heavy_running = 0;
+        const int max_heavy = ctx->max_heavy;
+        std::mutex qmu;
+        std::condition_variable qcv;
         std::atomic<bool> failed{false};
-        std::mutex emu;
         int err_code = BEM_OK;
         std::string err_msg;
+        // next item for a lane, or -1 when none is left for it (or after a failure)
+        auto take = [&](bool& is_heavy) -> int {
+            std::unique_lock<std::mutex> lk(qmu);
+            for (;;) {
+                if (failed.load()) return -1;
+                if (heavy_running < max_heavy && hnext < heavy.size()) {
+                    ++heavy_running;
+                    is_heavy = true;
+                    return heavy[hnext++];
+                }
+                if (lnext < light.size()) {
+                    is_heavy = false;
+                    return light[lnext++];
+                }
+                if (hnext >= heavy.size()) return -1;
+                qcv.wait(lk);  // only heavy items remain and max_heavy are running
+            }
+        };
+        auto done = [&](bool is_heavy, int code, const char* msg) {
+            std::lock_guard<std::mutex> lk(qmu);
+            if (is_heavy) --heavy_running;
+            if (msg && !failed.exchange(true)) {
+                err_code = code;
+                err_msg = msg;
+            }
+            qcv.notify_all();
+        };
         auto worker = [&](int li) {
             (void)hipSetDevice(ctx->device);
             Lane* L = ctx->lanes[li];
             for (;;) {
-                const int k = next.fetch_add(1);
-                if (k >= n || failed.load()) break;
-                const int i = order[k];
+                bool is_heavy = false;
+                const int i = take(is_heavy);
+                if (i < 0) break;
                 const double t0 = now_ms();
                 try {
                     sweep_item(ctx, L, meshes[i], problems[i], angles + 2 * i, amplitudes + 2 * i,
                                diagnostics ? diagnostics + i : nullptr);
                 } catch (const BemError& e) {
-                    std::lock_guard<std::mutex> lk(emu);
-                    if (!failed.exchange(true)) { err_code = e.code; err_msg = e.what(); }
+                    done(is_heavy, e.code, e.what());
                     break;
                 } catch (const std::exception& e) {
-                    std::lock_guard<std::mutex> lk(emu);
-                    if (!failed.exchange(true)) { err_code = BEM_ERR_DEVICE; err_msg = e.what(); }
+                    done(is_heavy, BEM_ERR_DEVICE, e.what());
                     break;
                 }
                 if (latency_ms) latency_ms[i] = now_ms() - t0;
+                done(is_heavy, BEM_OK, nullptr);
             }
         };
         std::vector<std::thread> th;

@@ -1071,30 +1071,44 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
                     t3[x][y][r] = -c.x - c.y;
                 }
     }
-    // staging: A row m0 + lm and B column n0 + lm, k = kc + lk .. kc + lk + 7
+    // staging: A row m0 + lm and B column n0 + lm, k = kc + lk .. kc + lk + 7.
+    // Rows, columns and k beyond the matrix are clamped to its last one, so
+    // every load is issued unconditionally into the same registers, and the
+    // values that lie outside are zeroed when they are stored to LDS, after
+    // the chunk's products: no branch between the loads and their first use,
+    // whose merge would force a wait for the prefetch before the products.
     constexpr int PT = kG3K / 4;  // k values per thread and chunk
     const int lm = tid & 127, lk = (tid >> 7) * PT;
-    const z32* pa = A + (int64_t)(m0 + lm) * lda + lk;
-    const z32* pb = B + (int64_t)lk * ldb + n0 + lm;
+    const bool okm = m0 + lm < M, okn = n0 + lm < N;
+    const z32* pa = A + (int64_t)(okm ? m0 + lm : M - 1) * lda + lk;
+    const z32* pb = B + (int64_t)lk * ldb + (okn ? n0 + lm : N - 1);
     z32 ra[PT], rb[PT];
     auto gload = [&](int kc) {
-        if (full && kc + kG3K <= K) {
+        if (kc + kG3K <= K) {
 #pragma unroll
             for (int t = 0; t < PT; ++t) {
                 ra[t] = pa[kc + t];
                 rb[t] = pb[(int64_t)(kc + t) * ldb];
             }
         } else {
-            const bool okm = m0 + lm < M, okn = n0 + lm < N;
+            // k clamped to K - 1 (an offset relative to lk, so possibly negative)
 #pragma unroll
             for (int t = 0; t < PT; ++t) {
-                const int gk = kc + lk + t;
-                ra[t] = (okm && gk < K) ? pa[kc + t] : z32{0.f, 0.f};
-                rb[t] = (okn && gk < K) ? pb[(int64_t)(kc + t) * ldb] : z32{0.f, 0.f};
+                const int dk = min(kc + t, K - 1 - lk);
+                ra[t] = pa[dk];
+                rb[t] = pb[(int64_t)dk * ldb];
             }
         }
     };
-    auto lstore = [&](int buf) {
+    auto lstore = [&](int buf, int kc) {
+        if (!full || kc + kG3K > K) {
+#pragma unroll
+            for (int t = 0; t < PT; ++t) {
+                const bool okk = kc + lk + t < K;
+                if (!(okm && okk)) ra[t] = z32{0.f, 0.f};
+                if (!(okn && okk)) rb[t] = z32{0.f, 0.f};
+            }
+        }
 #pragma unroll
         for (int h = 0; h < PT / 4; ++h) {
             const int q = (lk >> 2) + h;
@@ -1105,7 +1119,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
         }
     };
     gload(0);
-    lstore(0);
+    lstore(0, 0);
     __syncthreads();
     int buf = 0;
     for (int kc = 0; kc < K; kc += kG3K) {
@@ -1145,7 +1159,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
                     for (int y = 0; y < 2; ++y) t3[x][y] = Mfma<float>::op(as[x], bs[y], t3[x][y]);
             }
         }
-        if (more) lstore(buf ^ 1);
+        if (more) lstore(buf ^ 1, kc + kG3K);
         __syncthreads();
         buf ^= 1;
     }

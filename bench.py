@@ -10,6 +10,13 @@ FP64 far field.  |dTS| is checked against the same sweep solved all-FP64.
 
 Prints one JSON line (value = solves/s of the timed sweep).
 
+The sweep runs eight lanes (two HIP streams each) and sets the HIP runtime's
+hardware-queue count (GPU_MAX_HW_QUEUES, read once when the runtime starts)
+to 16 unless the environment already names one: with the runtime's default
+of 4, the lanes' streams share queues and a lane's kernels wait in order
+behind another lane's.  At most two solves with 16384 or more unknowns run at
+once (the library's default), which keeps their latency near a solo solve's.
+
 `--workload hmatrix` measures the hierarchical-matrix path: a rigid bent
 capped cylinder (radius 2 mm, centerline 60 mm, bend radius 120 mm,
 hemispherical caps) at 200k triangles, k = 1000 rad/m, cluster leaf 64,
@@ -29,6 +36,9 @@ import sys
 import time
 
 import numpy as np
+
+# before anything starts the HIP runtime (see the module docstring)
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "acousticscattering.jl_amd")]
@@ -85,11 +95,12 @@ def measured_traffic():
 
 
 def run_sweep(bh, ctx, plan, meshes, precision, chunk=64, serial=False):
-    """One pass over the sweep, largest meshes first, in synchronous chunks of
-    `chunk` solves (so progress can be reported); returns per-solve results in
+    """One pass over the sweep as one bem_sweep call (the library orders the
+    solves, largest first, and runs the matrix-core-bound ones one at a time
+    while its other lanes take the small ones); returns per-solve results in
     plan order and the wall time of the pass.  serial: one solve at a time on
     the calling thread (bem), for profilers that cannot follow the sweep's
-    worker threads; the same solves and kernels."""
+    worker threads; the same solves and kernels, progress every `chunk`."""
     fl = bh.FluidFilled(G_RHO, H_C)
     order = sorted(range(len(plan)), key=lambda i: -plan[i][2])
     amps = np.empty(len(plan), dtype=np.complex128)
@@ -106,14 +117,12 @@ def run_sweep(bh, ctx, plan, meshes, precision, chunk=64, serial=False):
             if (n + 1) % chunk == 0 or n + 1 == len(order):
                 log(f"  {precision}: {n + 1}/{len(plan)} solves, {time.perf_counter() - t0:.1f}s")
         return amps, lat, diags, time.perf_counter() - t0
-    for c0 in range(0, len(order), chunk):
-        idx = order[c0:c0 + chunk]
-        ms = [meshes[plan[i][2]] for i in idx]
-        probs = [bh.make_problem(fl, plan[i][1], precision=precision) for i in idx]
-        a, l, d = bh.batched_sweep(ms, probs, [(0.0, 0.0)] * len(idx), ctx=ctx)
-        for j, i in enumerate(idx):
-            amps[i], lat[i], diags[i] = a[j], l[j], d[j]
-        log(f"  {precision}: {c0 + len(idx)}/{len(plan)} solves, {time.perf_counter() - t0:.1f}s")
+    ms = [meshes[plan[i][2]] for i in order]
+    probs = [bh.make_problem(fl, plan[i][1], precision=precision) for i in order]
+    a, l, d = bh.batched_sweep(ms, probs, [(0.0, 0.0)] * len(order), ctx=ctx)
+    for j, i in enumerate(order):
+        amps[i], lat[i], diags[i] = a[j], l[j], d[j]
+    log(f"  {precision}: {len(plan)}/{len(plan)} solves, {time.perf_counter() - t0:.1f}s")
     return amps, lat, diags, time.perf_counter() - t0
 
 
@@ -332,6 +341,8 @@ def main():
     ap.add_argument("--rates", type=float, nargs="+", default=[50.0, 200.0, 800.0])
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--lanes", type=int, default=8, help="concurrent solves (sweep) or server lanes (serve)")
+    ap.add_argument("--heavy-lanes", type=int, default=2,
+                    help="sweep solves with >= 16384 unknowns allowed to run at once")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
@@ -360,6 +371,7 @@ def main():
 
     ctx = bh.Context(0)
     ctx.set_lanes(args.lanes)
+    ctx.set_heavy_lanes(args.heavy_lanes)
     plan, meshes = sweep_plan(bh, ctx, args.points, args.max_level)
     sizes = {}
     for (_, _, L) in plan:
@@ -422,6 +434,8 @@ def main():
                          + ("" if full else " [REDUCED]")),
             "panels_per_class": {str(k): v for k, v in sorted(sizes.items())},
             "lanes": "serial" if args.serial else args.lanes,
+            "heavy_lanes": args.heavy_lanes,
+            "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
             "global_batch": len(plan),
         },
         "max_abs_dTS_dB": dts,

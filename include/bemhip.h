@@ -131,6 +131,10 @@ int bem_ctx_reset_stats(bem_ctx ctx);
 /* Largest number of solves bem_sweep runs concurrently (1..8, default 8; the
  * HBM budget may allow fewer). */
 int bem_ctx_set_lanes(bem_ctx ctx, int32_t max_lanes);
+/* Largest number of heavy solves (16384 or more unknowns, whose trailing
+ * updates alone fill the GPU's matrix cores) that bem_sweep runs at once
+ * (1..8, default 2); the other lanes take the lighter solves meanwhile. */
+int bem_ctx_set_heavy_lanes(bem_ctx ctx, int32_t max_heavy);
 
 /* ---- mesh generators (host only, no device needed) --------------------- */
 /* Sizes of the level-`level` icosphere: 10*4^L+2 nodes, 20*4^L triangles. */

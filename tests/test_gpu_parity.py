@@ -324,6 +324,30 @@ def test_batched_sweep_matches_single_solves(ctx):
     assert np.all(lat > 0)
 
 
+def test_sweep_heavy_lane_limit_keeps_results(ctx):
+    """Solves with 16384 or more unknowns are admitted at most `heavy lanes`
+    at a time while lighter ones fill the other lanes; the schedule must not
+    change any amplitude."""
+    say("[gpu] bem_sweep heavy-lane limit 1 vs 8")
+    heavy = bh.mesh(bh.Sphere(0.01), level=5, ctx=ctx)  # 20480 unknowns
+    light = bh.mesh(bh.Sphere(0.01), level=3, ctx=ctx)
+    ms = [heavy, light, heavy, light, light]
+    probs = [bh.make_problem(bh.Rigid(), k, precision="mixed") for k in (200.0, 300.0, 100.0, 150.0, 250.0)]
+    out = {}
+    try:
+        for lim in (1, 8):
+            ctx.set_heavy_lanes(lim)
+            out[lim], _, diags = bh.batched_sweep(ms, probs, [(0.0, 0.0)] * len(ms), ctx=ctx)
+            assert all(d["converged"] for d in diags)
+        with pytest.raises(ValueError):
+            ctx.set_heavy_lanes(0)
+        with pytest.raises(ValueError):
+            ctx.set_heavy_lanes(9)
+    finally:
+        ctx.set_heavy_lanes(2)
+    assert np.all(np.abs(out[1] - out[8]) <= 1e-12 * np.abs(out[8]))
+
+
 def test_argument_errors(ctx):
     say("[gpu] argument errors")
     gm, _ = _sphere(ctx, 1, 1.0)
