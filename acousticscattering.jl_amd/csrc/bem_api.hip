@@ -58,14 +58,39 @@ struct ProfRec {
     double flops, bytes;
 };
 
+}  // namespace
+
+namespace bem {
+struct ProfLane;
+}
+
+namespace {
+
+// Per-class totals of the context's kernel timing.  Each lane records into
+// its own ProfLane (records and event pool under the lane's own lock), so
+// lanes launching concurrently never wait on each other here; records are
+// folded into these totals once their events have completed.
 struct Prof {
-    std::mutex mu;
-    bool on = false;
-    std::vector<ProfRec> recs;
-    std::vector<hipEvent_t> pool;
+    std::mutex mu;  // totals and the lane list
+    std::atomic<bool> on{false};
+    std::vector<bem::ProfLane*> lanes;
     int64_t launches[kNumClasses] = {};
     double ms[kNumClasses] = {}, flops[kNumClasses] = {}, bytes[kNumClasses] = {};
-    hipEvent_t cur = nullptr;
+    void drain();                   // every lane's records, waiting for their events
+    void attach(bem::ProfLane* pl);
+    void detach(bem::ProfLane* pl);  // drains pl first
+};
+
+}  // namespace
+
+namespace bem {
+// Event timing of one launch (or launch group) on a lane's streams.
+struct ProfLane {
+    Prof* prof = nullptr;
+    hipEvent_t cur = nullptr;  // begin event of the open record
+    std::mutex mu;             // the lane's thread and statistics readers
+    std::deque<ProfRec> recs;  // in issue order
+    std::vector<hipEvent_t> pool;
     hipEvent_t get() {
         if (!pool.empty()) {
             hipEvent_t e = pool.back();
@@ -73,56 +98,103 @@ struct Prof {
             return e;
         }
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-        return e;
+        return hipEventCreate(&e) == hipSuccess ? e : nullptr;
     }
-    void drain() {
-        for (auto& r : recs) {
-            hipEventSynchronize(r.b);
-            float t = 0;
-            hipEventElapsedTime(&t, r.a, r.b);
-            launches[r.cls] += 1;
-            ms[r.cls] += t;
-            flops[r.cls] += r.flops;
-            bytes[r.cls] += r.bytes;
+    // Fold records into the totals, oldest first: all of them (wait), or
+    // those whose end event has completed.  Without waiting, nothing is
+    // queried while an error is pending on this thread, and the not-ready
+    // status a query leaves is cleared, so the caller's own launch check
+    // still sees only its launches' errors.
+    void harvest(bool wait) {
+        int64_t n[kNumClasses] = {};
+        double t[kNumClasses] = {}, f[kNumClasses] = {}, b[kNumClasses] = {};
+        bool any = false;
+        if (!wait && hipPeekAtLastError() != hipSuccess) return;
+        while (!recs.empty()) {
+            const ProfRec r = recs.front();
+            if (wait) {
+                (void)hipEventSynchronize(r.b);
+            } else if (hipEventQuery(r.b) != hipSuccess) {
+                (void)hipGetLastError();
+                break;
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, r.a, r.b);
+            n[r.cls] += 1;
+            t[r.cls] += ms;
+            f[r.cls] += r.flops;
+            b[r.cls] += r.bytes;
             pool.push_back(r.a);
             pool.push_back(r.b);
+            recs.pop_front();
+            any = true;
         }
-        recs.clear();
+        if (!any) return;
+        std::lock_guard<std::mutex> g(prof->mu);
+        for (int c = 0; c < kNumClasses; ++c) {
+            prof->launches[c] += n[c];
+            prof->ms[c] += t[c];
+            prof->flops[c] += f[c];
+            prof->bytes[c] += b[c];
+        }
     }
-};
-
-}  // namespace
-
-namespace bem {
-// Event timing of one launch (or launch group) on stream s.  The begin event
-// is kept per stream so several lanes can time concurrently.
-struct ProfLane {
-    Prof* prof;
-    hipEvent_t cur = nullptr;
+    ~ProfLane() {
+        for (auto& r : recs) {
+            hipEventDestroy(r.a);
+            hipEventDestroy(r.b);
+        }
+        for (auto e : pool) hipEventDestroy(e);
+        if (cur) hipEventDestroy(cur);
+    }
 };
 void prof_begin(void* prof, hipStream_t s, int cls) {
     ProfLane* pl = static_cast<ProfLane*>(prof);
-    if (!pl || !pl->prof->on) return;
+    if (!pl || !pl->prof || !pl->prof->on.load()) return;
     (void)cls;
-    Prof* p = pl->prof;
-    std::lock_guard<std::mutex> g(p->mu);
-    pl->cur = p->get();
+    std::lock_guard<std::mutex> g(pl->mu);
+    if (!pl->cur) pl->cur = pl->get();
     if (pl->cur) hipEventRecord(pl->cur, s);
 }
 void prof_end(void* prof, hipStream_t s, int cls, double flops, double bytes) {
     ProfLane* pl = static_cast<ProfLane*>(prof);
-    if (!pl || !pl->prof->on || !pl->cur) return;
-    Prof* p = pl->prof;
-    std::lock_guard<std::mutex> g(p->mu);
-    hipEvent_t e = p->get();
+    if (!pl || !pl->prof || !pl->prof->on.load()) return;
+    std::lock_guard<std::mutex> g(pl->mu);
+    if (!pl->cur) return;
+    hipEvent_t e = pl->get();
     if (!e) return;
     hipEventRecord(e, s);
-    p->recs.push_back({cls, pl->cur, e, flops, bytes});
+    pl->recs.push_back({cls, pl->cur, e, flops, bytes});
     pl->cur = nullptr;
-    if (p->recs.size() > 4096) p->drain();
+    if (pl->recs.size() >= 64) pl->harvest(pl->recs.size() > 65536);
 }
 }  // namespace bem
+
+namespace {
+void Prof::drain() {
+    std::vector<bem::ProfLane*> ls;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        ls = lanes;
+    }
+    for (auto* pl : ls) {
+        std::lock_guard<std::mutex> g(pl->mu);
+        pl->harvest(true);
+    }
+}
+void Prof::attach(bem::ProfLane* pl) {
+    pl->prof = this;
+    std::lock_guard<std::mutex> g(mu);
+    lanes.push_back(pl);
+}
+void Prof::detach(bem::ProfLane* pl) {
+    {
+        std::lock_guard<std::mutex> g(pl->mu);
+        pl->harvest(true);
+    }
+    std::lock_guard<std::mutex> g(mu);
+    lanes.erase(std::remove(lanes.begin(), lanes.end(), pl), lanes.end());
+}
+}  // namespace
 
 // One execution lane: a HIP stream, the panel workspace of the LU on that
 // stream, and (for batched sweeps) a bump-allocated device arena that holds a
@@ -135,7 +207,7 @@ struct Lane {
     void* ws_mem = nullptr;
     char* arena = nullptr;
     size_t cap = 0, off = 0;
-    ProfLane pl{nullptr};
+    ProfLane pl;
 };
 
 struct bem_ctx_s {
@@ -277,7 +349,7 @@ namespace {
 // A lane: its own stream and tournament-pivoting workspace (and no arena).
 Lane* make_lane(bem_ctx c) {
     auto* L = new Lane();
-    L->pl.prof = &c->prof;
+    c->prof.attach(&L->pl);
     HIPCHECK(hipStreamCreateWithFlags(&L->s, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&L->s2, hipStreamNonBlocking));
     HIPCHECK(hipEventCreateWithFlags(&L->ev_ready, hipEventDisableTiming));
@@ -312,6 +384,7 @@ void free_lane(bem_ctx c, Lane* L) {
     if (!L) return;
     if (L->s) hipStreamSynchronize(L->s);
     if (L->s2) hipStreamSynchronize(L->s2);
+    c->prof.detach(&L->pl);
     c->release(L->ws_mem);
     c->release(L->arena);
     if (L->s) hipStreamDestroy(L->s);
@@ -1961,8 +2034,6 @@ int bem_ctx_create(int32_t device, uint64_t budget, bem_ctx* out) {
 int bem_ctx_destroy(bem_ctx ctx) {
     if (!ctx) return BEM_ERR_ARGUMENT;
     for (Lane* L : ctx->lanes) hipStreamSynchronize(L->s);
-    ctx->prof.drain();
-    for (auto e : ctx->prof.pool) hipEventDestroy(e);
     for (Lane* L : ctx->lanes) free_lane(ctx, L);
     ctx->trim_all();
     delete ctx;
@@ -2385,6 +2456,7 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh* meshes, const bem_problem*
         S = std::max(1, std::min(S, n));
         while ((int)ctx->lanes.size() < S) ctx->lanes.push_back(make_lane(ctx));
         // release arenas that are too small or unused, then (re)allocate
+        bool realloc = false;
         for (int l = 0; l < (int)ctx->lanes.size(); ++l) {
             Lane* L = ctx->lanes[l];
             if (L->arena && (l >= S || L->cap < max_need || L->cap > max_need + max_need / 4)) {
@@ -2392,7 +2464,14 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh* meshes, const bem_problem*
                 L->arena = nullptr;
                 L->cap = 0;
             }
+            realloc |= l < S && !L->arena;
         }
+        // New arenas start from an empty pool: blocks pooled by an earlier
+        // sweep of another size (an FP64 sweep's arenas are twice a mixed
+        // one's) would otherwise stay mapped beside the new ones and fill HBM
+        // (a mixed sweep after an FP64 one ran 16 % slower so).  The lanes are
+        // idle here, so hipFree's device-wide wait costs nothing.
+        if (realloc) ctx->trim_all();
         for (int l = 0; l < S; ++l) {
             Lane* L = ctx->lanes[l];
             if (!L->arena) {

@@ -344,15 +344,18 @@ __device__ __forceinline__ void wait_mem() {
 // ---------------------------------------------------------------------------
 // Panel factorization by tournament pivoting (CALU / TSLU: Grigori, Demmel and
 // Xiang, SIAM J. Matrix Anal. Appl. 32 (2011)).  The m x w panel is split into
-// 128-row leaves.  Each leaf runs partial pivoting on its rows (max |re|+|im|,
+// 256-row leaves.  Each leaf runs partial pivoting on its rows (max |re|+|im|,
 // ties to the lowest current position, as LAPACK's i?amax) and nominates w
-// rows; nominations are merged pairwise up a binary tree, each merge again
-// running partial pivoting on the 2w candidate rows, re-read from A by row
-// index (A is not written while the tree runs).  The root's selection order
-// is the panel's pivot order.  No grid-wide barrier: a merge is done by the
-// second workgroup to arrive at a tree node (atomic ticket), the first exits.
+// rows; nominations are merged four at a time up the tree, each merge again
+// running partial pivoting on the (up to 4w) candidate rows, re-read from A by
+// row index (A is not written while the tree runs).  The root's selection
+// order is the panel's pivot order.  Every level costs w sequential
+// elimination steps whatever its width, so the four-way tree (half the levels
+// of a binary one) halves the panel's latency.  No grid-wide barrier: a merge
+// is done by the last workgroup to arrive at a tree node (atomic ticket), the
+// others exit.
 //
-// One 128-row elimination (256 threads, 4 waves of 32 rows): lanes r and
+// One 256-row elimination (512 threads, 8 waves of 32 rows): lanes r and
 // r + 32 of a wave hold columns 0-31 and 32-63 of row r's active window.
 // Rows never move: a selected row retires and its LAPACK position is swapped
 // in bookkeeping only.  After each step the window shifts left by one column,
@@ -361,8 +364,10 @@ __device__ __forceinline__ void wait_mem() {
 // (double-buffered by step parity) before it, every wave picks the global
 // winner after it.
 // ---------------------------------------------------------------------------
-constexpr int kTsRows = 128;     // rows per leaf / merge
-constexpr int kTsThreads = 256;  // 4 waves x 32 rows, two lanes per row
+constexpr int kTsRows = 256;     // rows per leaf / merge
+constexpr int kTsWaves = 8;      // 32 rows per wave, two lanes per row
+constexpr int kTsThreads = 64 * kTsWaves;
+constexpr int kTsArity = 4;      // children per merge
 constexpr int kTsPS = 34;        // LDS stride of a half-row: 16-byte aligned, off the other half's banks
 
 template <typename R>
@@ -374,8 +379,8 @@ struct TsCand {
 
 template <typename R>
 struct TsLds {
-    TsCand<R> cand[2][4];
-    __attribute__((aligned(16))) C2<R> prow[2][4][2 * kTsPS];
+    TsCand<R> cand[2][kTsWaves];
+    __attribute__((aligned(16))) C2<R> prow[2][kTsWaves][2 * kTsPS];
     int sel[kNB];
     int ticket;
 };
@@ -502,14 +507,14 @@ __device__ __forceinline__ void ts_steps(C2<R> (&a)[32], int j0, int j1, bool& d
         }
         __syncthreads();
         // the block's winner
-        TsCand<R> cq[4];
+        TsCand<R> cq[kTsWaves];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) cq[q] = sh.cand[b][q];
+        for (int q = 0; q < kTsWaves; ++q) cq[q] = sh.cand[b][q];
         R gv = cq[0].v;
         int gp = cq[0].pos, gw = 0;
         C2<R> ip = cq[0].ip;
 #pragma unroll
-        for (int q = 1; q < 4; ++q) {
+        for (int q = 1; q < kTsWaves; ++q) {
             const bool t = (cq[q].v > gv) | ((cq[q].v == gv) & (cq[q].pos < gp));
             gv = t ? cq[q].v : gv;
             gp = t ? cq[q].pos : gp;
@@ -575,16 +580,17 @@ __device__ __forceinline__ void ts_load_row(C2<R> (&a)[32], const C2<R>* __restr
     }
 }
 
+// Nodes of tree level L (leaves are level 0) and the index of its first node.
 __device__ inline int ts_level_size(int G, int L) {
     int sz = G;
-    for (int i = 0; i < L; ++i) sz = (sz + 1) >> 1;
+    for (int i = 0; i < L; ++i) sz = (sz + kTsArity - 1) / kTsArity;
     return sz;
 }
 __device__ inline int ts_level_off(int G, int L) {
     int off = 0, sz = G;
     for (int i = 0; i < L; ++i) {
         off += sz;
-        sz = (sz + 1) >> 1;
+        sz = (sz + kTsArity - 1) / kTsArity;
     }
     return off;
 }
@@ -617,24 +623,29 @@ __global__ __launch_bounds__(kTsThreads) void k_tslu(const C2<R>* __restrict__ A
         if (tid == 0) st_ag(&ws.node_cnt[nd], cnt);
         wait_mem();
         __syncthreads();
-        const int parent = ts_level_off(G, level + 1) + (idx >> 1);
-        const int sib = idx ^ 1;
-        if (sib < size) {
+        const int parent = ts_level_off(G, level + 1) + idx / kTsArity;
+        const int first = idx - idx % kTsArity;  // the parent's first child
+        const int nch = min(kTsArity, size - first);
+        if (nch > 1) {
             if (tid == 0) {
                 const int t = __hip_atomic_fetch_add(&ws.tickets[parent], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (t != 0) st_ag(&ws.tickets[parent], 0);
+                if (t == nch - 1) st_ag(&ws.tickets[parent], 0);
                 sh.ticket = t;
             }
             __syncthreads();
-            if (sh.ticket == 0) return;  // the sibling will merge
+            if (sh.ticket != nch - 1) return;  // a later sibling will merge
         }
-        // merge the children of `parent` (one or two nodes)
-        const int left = ts_level_off(G, level) + (idx & ~1);
-        const int nl = ld_ag(&ws.node_cnt[left]);
-        const int nr = (sib < size) ? ld_ag(&ws.node_cnt[left + 1]) : 0;
-        nv = nl + nr;
-        myrow = slot < nl ? ld_ag(&ws.node_rows[left * kNB + slot])
-                          : (slot < nv ? ld_ag(&ws.node_rows[(left + 1) * kNB + slot - nl]) : -1);
+        // merge the parent's children, their nominations concatenated in order
+        const int c0 = ts_level_off(G, level) + first;
+        int cum[kTsArity + 1];
+        cum[0] = 0;
+#pragma unroll
+        for (int c = 0; c < kTsArity; ++c) cum[c + 1] = cum[c] + (c < nch ? ld_ag(&ws.node_cnt[c0 + c]) : 0);
+        nv = cum[kTsArity];
+        myrow = -1;
+#pragma unroll
+        for (int c = 0; c < kTsArity; ++c)
+            if (slot >= cum[c] && slot < cum[c + 1]) myrow = ld_ag(&ws.node_rows[(c0 + c) * kNB + slot - cum[c]]);
         if (slot < nv && (myrow < k0 || myrow >= n)) {  // never expected: flag it, keep every access in bounds
             st_ag(&ws.status[1], 1);
             myrow = -1;
@@ -642,7 +653,7 @@ __global__ __launch_bounds__(kTsThreads) void k_tslu(const C2<R>* __restrict__ A
         ts_load_row(a, A, lda, myrow, k0, w, part);
         cnt = ts_eliminate(a, nv, w, myrow, sh);
         level += 1;
-        idx >>= 1;
+        idx /= kTsArity;
     }
 }
 

@@ -59,10 +59,10 @@ def _permute(A, piv):
     return PA
 
 
-@pytest.mark.parametrize("n", [8, 64, 65, 128, 130, 300, 1000, 2600])
+@pytest.mark.parametrize("n", [8, 64, 65, 128, 130, 256, 300, 1000, 2600])
 def test_device_lu(ctx, n):
     """Device LU against LAPACK zgetrf: P A = L U to FP64 rounding, the solve
-    against numpy, and (n <= 128: one 128-row tournament leaf, which is plain
+    against numpy, and (n <= 256: one 256-row tournament leaf, which is plain
     partial pivoting) the identical pivot rows and factors."""
     say(f"[gpu] device LU n={n}")
     rng = np.random.default_rng(n)
@@ -73,7 +73,7 @@ def test_device_lu(ctx, n):
     assert np.abs(_permute(A, piv) - LUp).max() < 1e-12 * np.abs(A).max() * max(1.0, n / 100)
     xs = np.linalg.solve(A, b)
     assert np.abs(x - xs).max() < 1e-9 * np.abs(xs).max()
-    if n <= 128:
+    if n <= 256:
         lu0, piv0 = sla.lu_factor(A)
         assert np.array_equal(piv, piv0 + 1)
         assert np.abs(LU - lu0).max() < 1e-10 * np.abs(lu0).max()
@@ -91,7 +91,7 @@ def test_device_lu(ctx, n):
     assert np.abs(xh32 - xhs).max() < 20 * refh32 + 1e-6 * np.abs(xhs).max()
 
 
-@pytest.mark.parametrize("n", [300, 1000, 2600])
+@pytest.mark.parametrize("n", [300, 1000, 2600, 4100])
 def test_device_lu_tournament_pivots(ctx, n):
     """Multi-leaf panels: the device's pivot sequence is the tournament's
     (tests/tslu_model.py), index for index, in FP64."""

@@ -17,7 +17,7 @@ def _permute(A, ipiv):
 
 def test_single_leaf_is_lapack_partial_pivoting():
     rng = np.random.default_rng(3)
-    for n in (7, 64, 100, 128):
+    for n in (7, 64, 100, 128, 200, 256):
         A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
         assert np.array_equal(tm.blocked_lu_pivots(A), sla.lu_factor(A)[1])
 
@@ -30,11 +30,23 @@ def test_ties_go_to_the_lowest_position():
 
 
 def test_tournament_finds_forced_winners_across_leaves():
-    # one dominant entry per column, each in a different 128-row leaf
+    # one dominant entry per column, spread over the four 256-row leaves
     rng = np.random.default_rng(5)
     m, w = 1000, 6
     P = 1e-3 * (rng.standard_normal((m, w)) + 1j * rng.standard_normal((m, w)))
     rows = [905, 17, 611, 250, 999, 388]
+    for j, r in enumerate(rows):
+        P[r, j] = 100.0 * (j + 1)
+    assert tm.tournament(P) == rows
+
+
+def test_tournament_forced_winners_through_short_groups():
+    # 17 leaves: the last leaf is alone in its group of four at level 0 and
+    # its node again at level 1 (17 -> 5 -> 2 -> 1 nodes)
+    rng = np.random.default_rng(6)
+    m, w = 4100, 5
+    P = 1e-3 * (rng.standard_normal((m, w)) + 1j * rng.standard_normal((m, w)))
+    rows = [4099, 3, 2048, 4096, 1023]
     for j, r in enumerate(rows):
         P[r, j] = 100.0 * (j + 1)
     assert tm.tournament(P) == rows

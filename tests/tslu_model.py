@@ -1,17 +1,20 @@
 """CPU model of the device LU's pivot choice: right-looking blocked LU with
 64-column panels whose pivot rows come from tournament pivoting (CALU/TSLU,
-Grigori, Demmel and Xiang 2011) over 128-row leaves and a binary merge tree.
+Grigori, Demmel and Xiang 2011) over 256-row leaves and a merge tree that
+joins four nodes' nominations at a time (fewer when a level's last group is
+short; a lone node is re-run by itself, which keeps its order).
 
 Test infrastructure only: it fixes the pivot sequence the device must produce,
 so the GPU test can compare LAPACK-style ipiv arrays index for index.  The
 reference solves its dense systems with LAPACK's partially pivoted LU
-(Julia's `\\` on a dense complex matrix); for one leaf (n <= 128) the
+(Julia's `\\` on a dense complex matrix); for one leaf (n <= 256) the
 tournament is exactly that, and the GPU test checks it against scipy.
 """
 import numpy as np
 
 NB = 64
-LEAF = 128
+LEAF = 256
+ARITY = 4
 
 
 def _cabs1(z):
@@ -46,8 +49,8 @@ def tournament(P):
     nodes = [[c[s] for s in gepp_select(P[c])] for c in nodes]
     while len(nodes) > 1:
         nxt = []
-        for i in range(0, len(nodes), 2):
-            cand = nodes[i] + (nodes[i + 1] if i + 1 < len(nodes) else [])
+        for i in range(0, len(nodes), ARITY):
+            cand = [r for nd in nodes[i:i + ARITY] for r in nd]
             nxt.append([cand[s] for s in gepp_select(P[cand])])
         nodes = nxt
     return nodes[0]
