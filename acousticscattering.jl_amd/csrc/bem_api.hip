@@ -220,7 +220,7 @@ struct bem_ctx_s {
     Prof prof;
     std::vector<Lane*> lanes;  // lanes[0] serves the single-solve entry points
     int max_lanes = 8;         // concurrent solves of bem_sweep
-    int max_heavy = 2;         // of which heavy ones (kHeavyUnknowns)
+    int max_heavy = 1;         // of which heavy ones (kHeavyUnknowns)
 
     Lane* lane0() { return lanes[0]; }
     // Device memory: a caching allocator.  Released blocks go to a free pool

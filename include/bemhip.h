@@ -133,7 +133,7 @@ int bem_ctx_reset_stats(bem_ctx ctx);
 int bem_ctx_set_lanes(bem_ctx ctx, int32_t max_lanes);
 /* Largest number of heavy solves (16384 or more unknowns, whose trailing
  * updates alone fill the GPU's matrix cores) that bem_sweep runs at once
- * (1..8, default 2); the other lanes take the lighter solves meanwhile. */
+ * (1..8, default 1); the other lanes take the lighter solves meanwhile. */
 int bem_ctx_set_heavy_lanes(bem_ctx ctx, int32_t max_heavy);
 
 /* ---- mesh generators (host only, no device needed) --------------------- */

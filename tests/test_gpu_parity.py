@@ -344,7 +344,7 @@ def test_sweep_heavy_lane_limit_keeps_results(ctx):
         with pytest.raises(ValueError):
             ctx.set_heavy_lanes(9)
     finally:
-        ctx.set_heavy_lanes(2)
+        ctx.set_heavy_lanes(1)
     assert np.all(np.abs(out[1] - out[8]) <= 1e-12 * np.abs(out[8]))
 
 
