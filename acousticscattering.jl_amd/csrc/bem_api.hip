@@ -205,6 +205,7 @@ struct Lane {
     hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
     PanelWS ws{};
     void* ws_mem = nullptr;
+    void* rec_mem = nullptr;           // ws.rec
     char* arena = nullptr;
     size_t cap = 0, off = 0;
     ProfLane pl;
@@ -354,7 +355,7 @@ Lane* make_lane(bem_ctx c) {
     HIPCHECK(hipStreamCreateWithFlags(&L->s2, hipStreamNonBlocking));
     HIPCHECK(hipEventCreateWithFlags(&L->ev_ready, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&L->ev_panel, hipEventDisableTiming));
-    const int G = 2048;  // 262144 rows (128 per leaf)
+    const int G = 2048;  // 524288 rows (256 per leaf)
     const int nodes = 2 * G + 16;
     L->ws.max_blocks = G;
     L->ws.max_nodes = nodes;
@@ -372,6 +373,8 @@ Lane* make_lane(bem_ctx c) {
     L->ws.tri_ctr = (unsigned*)take(2 * sizeof(unsigned));
     L->ws.tri_prog = (unsigned long long*)take(2 * sizeof(unsigned long long));
     if (off > bytes) throw BemError(BEM_ERR_DEVICE, "workspace layout");
+    L->rec_mem = c->alloc(kPanelRecBytes);
+    L->ws.rec = L->rec_mem;
     // a device-side wait of the triangular solves gives up after 10 s
     int khz = 0;
     HIPCHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
@@ -386,6 +389,7 @@ void free_lane(bem_ctx c, Lane* L) {
     if (L->s2) hipStreamSynchronize(L->s2);
     c->prof.detach(&L->pl);
     c->release(L->ws_mem);
+    c->release(L->rec_mem);
     c->release(L->arena);
     if (L->s) hipStreamDestroy(L->s);
     if (L->s2) hipStreamDestroy(L->s2);

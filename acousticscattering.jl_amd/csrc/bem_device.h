@@ -142,6 +142,11 @@ void launch_matvec(hipStream_t s, const C2<R>* A, int64_t lda, int rows, int col
                    const z64* b, double* den);
 
 // Workspace of the tournament-pivoting panel (one per stream).
+// Device bytes of PanelWS::rec: the tournament root's multipliers for 64 steps
+// x 256 rows, its 64 pivot-row windows in rows of 128 (complex FP64, the
+// larger case) and 64 slots.
+constexpr size_t kPanelRecBytes = 16 * (64 * 256 + 64 * 128) + 4 * 64;
+
 struct PanelWS {
     int* status;       // [0] first zero pivot (INT_MAX if none), [1] inconsistent pivot selection,
                        // [2] a triangular-solve wait expired
@@ -149,8 +154,9 @@ struct PanelWS {
     int* node_cnt;     // [max_nodes] rows nominated by each tree node
     int* node_rows;    // [max_nodes][64] their global row indices
     int* root;         // [0] count, [1..65) the panel's pivot rows in order
-    int max_blocks;    // largest leaf count (128 rows per leaf)
+    int max_blocks;    // largest leaf count (256 rows per leaf)
     int max_nodes;
+    void* rec = nullptr;  // the tournament root's record of the diagonal block's factors (kPanelRecBytes)
     unsigned* tri_ctr = nullptr;             // [2] triangular-solve tickets, zero between launches
     unsigned long long* tri_prog = nullptr;  // [2] forward / backward progress words
     unsigned tri_epoch = 0;                  // host: triangular solves launched on this workspace

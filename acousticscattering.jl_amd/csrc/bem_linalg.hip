@@ -472,6 +472,33 @@ __device__ __forceinline__ C2<R> cfms(C2<R> a, C2<R> l, C2<R> p) {
     return C2<R>{fma(-l.x, p.x, fma(l.y, p.y, a.x)), fma(-l.x, p.y, fma(-l.y, p.x, a.y))};
 }
 
+// Where the tournament's root records its elimination, which is the LU
+// without pivoting of the selected rows in pivot order: the multiplier of
+// every live row at every step, the pivot row's active window (row j of U),
+// and the slot of step j's pivot.  k_tslu_apply assembles the panel's
+// diagonal-block factors from it instead of factoring those rows again.
+constexpr int kTsRecU = 2 * kNB;  // row pitch of the recorded U windows
+template <typename R>
+struct TsRec {
+    C2<R>* l;  // [kNB][kTsRows]: step j, slot s
+    C2<R>* u;  // [kNB][kTsRecU]: row j, column c >= j; columns >= kNB are padding
+    int* slot; // [kNB]
+};
+template <typename R>
+__host__ __device__ inline TsRec<R> ts_rec(void* base) {
+    C2<R>* l = static_cast<C2<R>*>(base);
+    C2<R>* u = l + kNB * kTsRows;
+    return TsRec<R>{l, u, reinterpret_cast<int*>(u + kNB * kTsRecU)};
+}
+static_assert(sizeof(C2<double>) * (kNB * kTsRows + kNB * kTsRecU) + sizeof(int) * kNB == kPanelRecBytes,
+              "PanelWS::rec holds the FP64 record");
+
+// Block barrier for LDS traffic: waits for this wave's LDS operations only.
+// __syncthreads() also releases global memory, which would make every
+// elimination step wait for the root's record stores (nothing in the block
+// reads them back; the kernel's end publishes them).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // Elimination steps [j0, j1) with a live window of W columns per lane (32
 // while the upper half-row still holds live columns, 16 once fewer than 16
 // remain).  The winner of step j writes its global row index to sh.sel[j].
@@ -481,7 +508,7 @@ __device__ __forceinline__ C2<R> cfms(C2<R> a, C2<R> l, C2<R> p) {
 // registers) and eliminates.
 template <typename R, int W>
 __device__ __forceinline__ void ts_steps(C2<R> (&a)[32], int j0, int j1, bool& done, int& mypos, int myrow,
-                                         TsLds<R>& sh) {
+                                         int slot, TsLds<R>& sh, const TsRec<R>& rec, bool record) {
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     const bool upper = lane >= 32;
     constexpr int G = sizeof(R) == sizeof(float) ? W : 8;  // pivot values per LDS batch
@@ -505,7 +532,7 @@ __device__ __forceinline__ void ts_steps(C2<R> (&a)[32], int j0, int j1, bool& d
 #pragma unroll
             for (int c = 0; c < W; ++c) sh.prow[b][g][(upper ? kTsPS : 0) + c] = a[c];
         }
-        __syncthreads();
+        lds_barrier();
         // the block's winner
         TsCand<R> cq[kTsWaves];
 #pragma unroll
@@ -524,6 +551,15 @@ __device__ __forceinline__ void ts_steps(C2<R> (&a)[32], int j0, int j1, bool& d
         }
         const C2<R>* pr = sh.prow[b][gw] + (upper ? kTsPS : 0);
         const C2<R> l0 = cmul(a[0], ip);
+        if (record) {
+            // lower lane: this row's multiplier.  Wave 0: the winner's window,
+            // staged in LDS for the elimination, one value per lane (lower
+            // half: columns j .. j + 31, upper: j + 32 .. j + 63; beyond 63 the
+            // stores land in the row's padding, j + 63 < kTsRecU).
+            if (!upper && !done) rec.l[j * kTsRows + slot] = l0;
+            if (g == 0) rec.u[j * kTsRecU + j + lane] = sh.prow[b][gw][(upper ? kTsPS - 32 : 0) + lane];
+            if (g == gw && !done && mypos == gp && !upper) rec.slot[j] = slot;
+        }
         if (g == gw && !done && mypos == gp) {
             done = true;
             if (!upper) sh.sel[j] = myrow;
@@ -556,15 +592,20 @@ __device__ __forceinline__ void ts_steps(C2<R> (&a)[32], int j0, int j1, bool& d
 }
 
 // Partial pivoting over the block's nv rows (w columns); on return
-// sh.sel[0 .. cnt) holds the selected global rows in pivot order.
+// sh.sel[0 .. cnt) holds the selected global rows in pivot order.  The root
+// records its factors in `rec` (record = true).
 template <typename R>
-__device__ int ts_eliminate(C2<R> (&a)[32], int nv, int w, int myrow, TsLds<R>& sh) {
+__device__ int ts_eliminate(C2<R> (&a)[32], int nv, int w, int myrow, TsLds<R>& sh, TsRec<R> rec, bool record) {
     const int slot = (threadIdx.x >> 6) * 32 + (threadIdx.x & 31);
     const int cnt = nv < w ? nv : w;
     bool done = myrow < 0;
     int mypos = done ? 1024 + slot : slot;
-    ts_steps<R, 32>(a, 0, cnt < 48 ? cnt : 48, done, mypos, myrow, sh);
-    ts_steps<R, 16>(a, 48, cnt, done, mypos, myrow, sh);
+    // the row loads have landed: otherwise the step loop would wait on its
+    // first use of them in every iteration, and on gfx950 that count also
+    // covers the root's record stores
+    __builtin_amdgcn_s_waitcnt(0);
+    ts_steps<R, 32>(a, 0, cnt < 48 ? cnt : 48, done, mypos, myrow, slot, sh, rec, record);
+    ts_steps<R, 16>(a, 48, cnt, done, mypos, myrow, slot, sh, rec, record);
     __syncthreads();
     return cnt;
 }
@@ -608,8 +649,9 @@ __global__ __launch_bounds__(kTsThreads) void k_tslu(const C2<R>* __restrict__ A
     int myrow = slot < nv ? k0 + idx * kTsRows + slot : -1;
     if (tid < kNB) sh.sel[tid] = -1;
     C2<R> a[32];
+    const TsRec<R> rec = ts_rec<R>(ws.rec);
     ts_load_row(a, A, lda, myrow, k0, w, part);
-    int cnt = ts_eliminate(a, nv, w, myrow, sh);
+    int cnt = ts_eliminate(a, nv, w, myrow, sh, rec, G == 1);
     for (;;) {
         const int size = ts_level_size(G, level);
         if (size == 1) {
@@ -651,7 +693,7 @@ __global__ __launch_bounds__(kTsThreads) void k_tslu(const C2<R>* __restrict__ A
             myrow = -1;
         }
         ts_load_row(a, A, lda, myrow, k0, w, part);
-        cnt = ts_eliminate(a, nv, w, myrow, sh);
+        cnt = ts_eliminate(a, nv, w, myrow, sh, rec, ts_level_size(G, level + 1) == 1);
         level += 1;
         idx /= kTsArity;
     }
@@ -741,7 +783,7 @@ __global__ __launch_bounds__(256) void k_tslu_apply(C2<R>* __restrict__ A, int64
     __shared__ C2<R> D[kNB][kNB + 1];
     __shared__ C2<R> B[kNB][kNB + 1];
     __shared__ int s_pos[kNB], s_row[kNB], s_cnt, s_root[kNB];
-    __shared__ int s_dst[kNB], s_src[kNB], s_nd;
+    __shared__ int s_dst[kNB], s_src[kNB], s_nd, s_slot[kNB];
     const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
     if (tid <= kNB) {
         int v = ld_ag(&ws.root[tid]);
@@ -795,30 +837,28 @@ __global__ __launch_bounds__(256) void k_tslu_apply(C2<R>* __restrict__ A, int64
     }
     __syncthreads();
     const int nd = s_nd;
-    // read every source row (original values) before anything is written:
-    // the selected rows into D, the rows leaving the top block into B
-    for (int r = g; r < kNB; r += 4)
-        D[r][c] = (r < cnt && c < w) ? A[(int64_t)s_root[r] * lda + k0 + c] : C2<R>{0, 0};
+    // The diagonal block's LU without pivoting, as the tournament's root
+    // recorded it: the root's elimination ran on exactly these rows, in this
+    // order, with these pivots.  Recorded slots are range-checked like the
+    // row indices above.
+    const TsRec<R> rec = ts_rec<R>(ws.rec);
+    if (tid < kNB) {
+        int sl = tid < cnt ? rec.slot[tid] : 0;
+        if (sl < 0 || sl >= kTsRows) {
+            st_ag(&ws.status[1], 1);
+            sl = 0;
+        }
+        s_slot[tid] = sl;
+    }
+    // the rows leaving the top block, read before any is written
     for (int t = g; t < nd; t += 4) B[t][c] = (c < w) ? A[(int64_t)s_src[t] * lda + k0 + c] : C2<R>{0, 0};
     __syncthreads();
     for (int t = g; t < nd; t += 4)
         if (c < w) A[(int64_t)s_dst[t] * lda + k0 + c] = B[t][c];
+    for (int r = g; r < kNB; r += 4)
+        D[r][c] = (r < cnt && c < w) ? (r > c ? rec.l[c * kTsRows + s_slot[r]] : rec.u[r * kTsRecU + c]) : C2<R>{0, 0};
     __syncthreads();
-    // LU without pivoting of the diagonal block
-    for (int j = 0; j < w; ++j) {
-        const C2<R> pv = D[j][j];
-        const bool nz = pv.x != 0 || pv.y != 0;
-        if (tid == 0 && !nz) atomicMin(&ws.status[0], k0 + j);
-        const C2<R> ip = nz ? cinv(pv) : C2<R>{0, 0};
-        __syncthreads();
-        if (tid > j && tid < w) D[tid][j] = cmul(D[tid][j], ip);
-        __syncthreads();
-        for (int e = tid; e < kNB * kNB; e += 256) {
-            const int r = e / kNB, cc = e % kNB;
-            if (r > j && r < w && cc > j && cc < w) D[r][cc] = csub(D[r][cc], cmul(D[r][j], D[j][cc]));
-        }
-        __syncthreads();
-    }
+    if (tid < w && D[tid][tid].x == 0 && D[tid][tid].y == 0) atomicMin(&ws.status[0], k0 + tid);
     for (int r = g; r < w; r += 4)
         if (c < w) A[(int64_t)(k0 + r) * lda + k0 + c] = D[r][c];
     __syncthreads();
