@@ -324,9 +324,9 @@ def hmatrix_main(args):
         "hmatrix": {key: info[key] for key in ("leaves", "dense_blocks", "lowrank_blocks", "dense_bytes",
                                                  "lowrank_bytes", "max_rank", "mean_rank", "unconverged_blocks",
                                                  "plan_ms", "assembly_ms", "aca_ms")},
-        # 64 right-hand sides give the apply ~100 flops per block byte, ten
-        # times FP64's machine balance (78.6 TFLOP/s / 8 TB/s): the products,
-        # FP64 on the matrix cores, bound it, not the block stream
+        # 64 right-hand sides give the apply ~60 flops per byte moved (blocks
+        # and vectors), six times FP64's machine balance (78.6 TFLOP/s over
+        # 8 TB/s): the products, FP64 on the matrix cores, bound it
         "roofline": {"kernel": "compressed apply (64 right-hand sides)", "bound": "mfma", "achieved": tfs,
                      "peak": MFMA_FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": tfs * 1e12 / MFMA_FP64_PEAK,
                      "traffic": None, "hbm_gbs": gbs, "hbm_frac": gbs / 8000.0,
