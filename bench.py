@@ -224,7 +224,8 @@ def serve_main(args):
         "dtype": "c32 LU / f64 refinement", "data": "synthetic",
         "config": {"workload": "served stream: bodies of revolution r=a sin(pi s)(1+eps cos 3 pi s), "
                                "a~U[5,50] mm, aspect~U[1,8], eps~U[0,0.2], rigid/soft/fluid, ka~U[0.5,15], "
-                               f"N in [2000,40000], mixed precision, {args.lanes} lanes, factors resident",
+                               f"N in [2000,40000], mixed precision, {args.lanes} lanes, factors resident, "
+                               f"{os.environ['GPU_MAX_HW_QUEUES']} hardware queues",
                    "rates": args.rates, "requests_per_rate": args.requests},
         "p50_latency_ms": top["p50_latency_ms"], "p99_latency_ms": top["p99_latency_ms"],
         "per_rate": rates,
@@ -350,8 +351,9 @@ def main():
     ap.add_argument("--lanes", type=int, default=8, help="concurrent solves (sweep) or server lanes (serve)")
     ap.add_argument("--heavy-lanes", type=int, default=1,
                     help="sweep solves with >= 16384 unknowns allowed to run at once")
-    ap.add_argument("--hw-queues", type=int, default=16, choices=range(1, 33), metavar="1..32",
-                    help="HIP hardware queues (GPU_MAX_HW_QUEUES) the benchmark runs under")
+    ap.add_argument("--hw-queues", type=int, default=None, choices=range(1, 33), metavar="1..32",
+                    help="HIP hardware queues (GPU_MAX_HW_QUEUES) the benchmark runs under "
+                         "(default two per lane, at most 32)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
@@ -362,6 +364,8 @@ def main():
                     help="sweep one solve at a time on the main thread (for rocprofv3 --pmc)")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     args = ap.parse_args()
+    if args.hw_queues is None:
+        args.hw_queues = min(32, 2 * args.lanes)
     if os.environ.get("GPU_MAX_HW_QUEUES") != str(args.hw_queues):
         env = dict(os.environ, GPU_MAX_HW_QUEUES=str(args.hw_queues))
         sys.exit(subprocess.run([sys.executable] + sys.argv, env=env).returncode)
