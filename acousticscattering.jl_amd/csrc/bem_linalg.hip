@@ -962,8 +962,9 @@ __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A
                                                      const C2<R>* __restrict__ B, int64_t ldb,
                                                      C2<R>* __restrict__ C, int64_t ldc, int M, int N, int K) {
     typedef typename Mfma<R>::v4 v4;
-    __shared__ R sA[2][kGK][kGM];
-    __shared__ R sB[2][kGK][kGN];
+    // double-buffered LDS (128 KB in FP64): one barrier per K chunk
+    __shared__ R sA[2][2][kGK][kGM];
+    __shared__ R sB[2][2][kGK][kGN];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = (wv >> 2) * 64, wn = (wv & 3) * 32;
     const int m0 = blockIdx.y * kGM, n0 = blockIdx.x * kGN;
@@ -986,32 +987,36 @@ __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A
             rb[t] = (gn < N && gk < K) ? B[(int64_t)gk * ldb + gn] : C2<R>{0, 0};
         }
     };
-    gload(0);
-    for (int kc = 0; kc < K; kc += kGK) {
-        __syncthreads();
+    auto lstore = [&](int buf) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            sA[0][lk + t][lm] = ra[t].x;
-            sA[1][lk + t][lm] = ra[t].y;
-            sB[0][lk + t][lm] = rb[t].x;
-            sB[1][lk + t][lm] = rb[t].y;
+            sA[buf][0][lk + t][lm] = ra[t].x;
+            sA[buf][1][lk + t][lm] = ra[t].y;
+            sB[buf][0][lk + t][lm] = rb[t].x;
+            sB[buf][1][lk + t][lm] = rb[t].y;
         }
-        __syncthreads();
-        if (kc + kGK < K) gload(kc + kGK);
+    };
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int kc = 0; kc < K; kc += kGK) {
+        const bool more = kc + kGK < K;
+        if (more) gload(kc + kGK);
 #pragma unroll
         for (int kk = 0; kk < kGK; kk += 4) {
             const int kr = kk + (lane >> 4), cl = lane & 15;
             R ar[4], ai[4], na[4], br[2], bi[2];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                ar[q] = sA[0][kr][wm + q * 16 + cl];
-                ai[q] = sA[1][kr][wm + q * 16 + cl];
+                ar[q] = sA[buf][0][kr][wm + q * 16 + cl];
+                ai[q] = sA[buf][1][kr][wm + q * 16 + cl];
                 na[q] = -ai[q];
             }
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                br[q] = sB[0][kr][wn + q * 16 + cl];
-                bi[q] = sB[1][kr][wn + q * 16 + cl];
+                br[q] = sB[buf][0][kr][wn + q * 16 + cl];
+                bi[q] = sB[buf][1][kr][wn + q * 16 + cl];
             }
 #pragma unroll
             for (int x = 0; x < 4; ++x)
@@ -1030,6 +1035,9 @@ __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A
 #pragma unroll
                 for (int y = 0; y < 2; ++y) accI[x][y] = Mfma<R>::op(ai[x], br[y], accI[x][y]);
         }
+        if (more) lstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
     }
 #pragma unroll
     for (int x = 0; x < 4; ++x)
