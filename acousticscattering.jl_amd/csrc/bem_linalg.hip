@@ -1091,7 +1091,10 @@ __device__ __forceinline__ void g3_tile(int nx, int ny, int& tm, int& tn) {
     tn = w / gs;
 }
 
-template <int ASSIGN>
+// FULL: M and N multiples of the tile and K of the chunk (every trailing
+// update of a 128-aligned system), so no load, store or product needs an edge
+// test; the edge handling cost 7 % when it was compiled into every launch.
+template <int ASSIGN, bool FULL>
 __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, int64_t lda, const z32* __restrict__ B,
                                                    int64_t ldb, z32* __restrict__ C, int64_t ldc, int M, int N,
                                                    int K) {
@@ -1105,7 +1108,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
     int tm, tn;
     g3_tile(gridDim.x, gridDim.y, tm, tn);
     const int m0 = tm * kGM, n0 = tn * kGN;
-    const bool full = m0 + kGM <= M && n0 + kGN <= N;
+    const bool full = FULL || (m0 + kGM <= M && n0 + kGN <= N);
     const int li = lane & 15, lq = lane >> 4;
     v4 t1[4][2], t2[4][2], t3[4][2];
 #pragma unroll
@@ -1143,7 +1146,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
     const z32* pb = B + (int64_t)lk * ldb + (okn ? n0 + lm : N - 1);
     z32 ra[PT], rb[PT];
     auto gload = [&](int kc) {
-        if (kc + kG3K <= K) {
+        if (FULL || kc + kG3K <= K) {
 #pragma unroll
             for (int t = 0; t < PT; ++t) {
                 ra[t] = pa[kc + t];
@@ -1160,7 +1163,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
         }
     };
     auto lstore = [&](int buf, int kc) {
-        if (!full || kc + kG3K > K) {
+        if (!FULL && (!full || kc + kG3K > K)) {
 #pragma unroll
             for (int t = 0; t < PT; ++t) {
                 const bool okk = kc + lk + t < K;
@@ -1186,6 +1189,10 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
         if (more) gload(kc + kG3K);
 #pragma unroll
         for (int h = 0; h < kG3K / 16; ++h) {
+            // each 16-wide k group reads its operands once the previous
+            // group's products have issued (hoisting all reads of the chunk
+            // holds 48 more registers and spills the full-tile form)
+            __builtin_amdgcn_sched_barrier(0);
             v4 ar[4], ai[4], br[2], bi[2];
 #pragma unroll
             for (int x = 0; x < 4; ++x) {
@@ -1256,9 +1263,14 @@ enum { CLS_PANEL = 2, CLS_GEMM = 3, CLS_TRSM = 4, CLS_SWAP = 5, CLS_SOLVE = 6, C
 template <typename R, int ASSIGN>
 static void launch_gemm(hipStream_t s, dim3 grid, const C2<R>* A, int64_t lda, const C2<R>* B, int64_t ldb, C2<R>* C,
                         int64_t ldc, int M, int N, int K) {
-    if constexpr (sizeof(R) == sizeof(float))
-        hipLaunchKernelGGL((k_gemm3m<ASSIGN>), grid, dim3(512), 0, s, (const z32*)A, lda, (const z32*)B, ldb,
-                           (z32*)C, ldc, M, N, K);
+    if constexpr (sizeof(R) == sizeof(float)) {
+        if (M % kGM == 0 && N % kGN == 0 && K % kG3K == 0)
+            hipLaunchKernelGGL((k_gemm3m<ASSIGN, true>), grid, dim3(512), 0, s, (const z32*)A, lda, (const z32*)B,
+                               ldb, (z32*)C, ldc, M, N, K);
+        else
+            hipLaunchKernelGGL((k_gemm3m<ASSIGN, false>), grid, dim3(512), 0, s, (const z32*)A, lda, (const z32*)B,
+                               ldb, (z32*)C, ldc, M, N, K);
+    }
     else
         hipLaunchKernelGGL((k_gemm_sub<R, ASSIGN>), grid, dim3(512), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
 }
