@@ -957,7 +957,8 @@ constexpr int kGM = 128, kGN = 128, kGK = 16;
 
 // 512 threads: 8 waves as 2 (rows) x 4 (columns), each wave a 64 x 32 tile of
 // 4 x 2 MFMA blocks (lane layouts in Mfma<R>).
-template <typename R, int ASSIGN>
+// FULL: M and N multiples of the tile and K of the chunk; no edge tests.
+template <typename R, int ASSIGN, bool FULL>
 __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A, int64_t lda,
                                                      const C2<R>* __restrict__ B, int64_t ldb,
                                                      C2<R>* __restrict__ C, int64_t ldc, int M, int N, int K) {
@@ -983,8 +984,8 @@ __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int gk = kc + lk + t;
-            ra[t] = (gm < M && gk < K) ? A[(int64_t)gm * lda + gk] : C2<R>{0, 0};
-            rb[t] = (gn < N && gk < K) ? B[(int64_t)gk * ldb + gn] : C2<R>{0, 0};
+            ra[t] = (FULL || (gm < M && gk < K)) ? A[(int64_t)gm * lda + gk] : C2<R>{0, 0};
+            rb[t] = (FULL || (gn < N && gk < K)) ? B[(int64_t)gk * ldb + gn] : C2<R>{0, 0};
         }
     };
     auto lstore = [&](int buf) {
@@ -1047,7 +1048,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A
             for (int r = 0; r < 4; ++r) {
                 const int gm = m0 + wm + x * 16 + Mfma<R>::row(lane, r);
                 const int gn = n0 + wn + y * 16 + (lane & 15);
-                if (gm < M && gn < N) {
+                if (FULL || (gm < M && gn < N)) {
                     if (ASSIGN) {
                         C[(int64_t)gm * ldc + gn] = C2<R>{accR[x][y][r], accI[x][y][r]};
                     } else {
@@ -1260,19 +1261,52 @@ enum { CLS_PANEL = 2, CLS_GEMM = 3, CLS_TRSM = 4, CLS_SWAP = 5, CLS_SOLVE = 6, C
 
 // Trailing-update GEMM: 3M products for complex FP32; four real products for
 // FP64, whose three 3M accumulator sets would not fit two waves per SIMD.
+// The 128-aligned block of C in the full-tile form and the right and bottom
+// edge strips (fewer than 128 columns / rows) in the general form, each
+// element of C written by exactly one launch: taken when C is aligned (one
+// launch) or when the aligned block spans enough waves of the chip that the
+// full form's saving outweighs the strips' two extra serial launches.
+constexpr int64_t kSplitTiles = 2048;
+static inline bool split_pays(int M, int N, int Ma, int Na) {
+    if (Ma == 0 || Na == 0) return false;
+    if (Ma == M && Na == N) return true;
+    return (int64_t)(Ma / kGM) * (Na / kGN) >= kSplitTiles;
+}
+
 template <typename R, int ASSIGN>
 static void launch_gemm(hipStream_t s, dim3 grid, const C2<R>* A, int64_t lda, const C2<R>* B, int64_t ldb, C2<R>* C,
                         int64_t ldc, int M, int N, int K) {
     if constexpr (sizeof(R) == sizeof(float)) {
-        if (M % kGM == 0 && N % kGN == 0 && K % kG3K == 0)
-            hipLaunchKernelGGL((k_gemm3m<ASSIGN, true>), grid, dim3(512), 0, s, (const z32*)A, lda, (const z32*)B,
-                               ldb, (z32*)C, ldc, M, N, K);
-        else
+        const int Ma = M - M % kGM, Na = N - N % kGN;
+        if (K % kG3K == 0 && split_pays(M, N, Ma, Na)) {
+            hipLaunchKernelGGL((k_gemm3m<ASSIGN, true>), dim3(Na / kGN, Ma / kGM), dim3(512), 0, s, (const z32*)A,
+                               lda, (const z32*)B, ldb, (z32*)C, ldc, Ma, Na, K);
+            if (Na < N)
+                hipLaunchKernelGGL((k_gemm3m<ASSIGN, false>), dim3(1, (M + kGM - 1) / kGM), dim3(512), 0, s,
+                                   (const z32*)A, lda, (const z32*)(B + Na), ldb, (z32*)(C + Na), ldc, M, N - Na, K);
+            if (Ma < M)
+                hipLaunchKernelGGL((k_gemm3m<ASSIGN, false>), dim3(Na / kGN, 1), dim3(512), 0, s,
+                                   (const z32*)(A + (int64_t)Ma * lda), lda, (const z32*)B, ldb,
+                                   (z32*)(C + (int64_t)Ma * ldc), ldc, M - Ma, Na, K);
+        } else {
             hipLaunchKernelGGL((k_gemm3m<ASSIGN, false>), grid, dim3(512), 0, s, (const z32*)A, lda, (const z32*)B,
                                ldb, (z32*)C, ldc, M, N, K);
+        }
+    } else {
+        const int Ma = M - M % kGM, Na = N - N % kGN;
+        if (K % kGK == 0 && split_pays(M, N, Ma, Na)) {
+            hipLaunchKernelGGL((k_gemm_sub<R, ASSIGN, true>), dim3(Na / kGN, Ma / kGM), dim3(512), 0, s, A, lda, B,
+                               ldb, C, ldc, Ma, Na, K);
+            if (Na < N)
+                hipLaunchKernelGGL((k_gemm_sub<R, ASSIGN, false>), dim3(1, (M + kGM - 1) / kGM), dim3(512), 0, s, A,
+                                   lda, B + Na, ldb, C + Na, ldc, M, N - Na, K);
+            if (Ma < M)
+                hipLaunchKernelGGL((k_gemm_sub<R, ASSIGN, false>), dim3(Na / kGN, 1), dim3(512), 0, s,
+                                   A + (int64_t)Ma * lda, lda, B, ldb, C + (int64_t)Ma * ldc, ldc, M - Ma, Na, K);
+        } else {
+            hipLaunchKernelGGL((k_gemm_sub<R, ASSIGN, false>), grid, dim3(512), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
+        }
     }
-    else
-        hipLaunchKernelGGL((k_gemm_sub<R, ASSIGN>), grid, dim3(512), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
 }
 
 template <typename R>
