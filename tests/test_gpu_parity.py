@@ -91,6 +91,25 @@ def test_device_lu(ctx, n):
     assert np.abs(xh32 - xhs).max() < 20 * refh32 + 1e-6 * np.abs(xhs).max()
 
 
+def test_device_lu_aligned_block_and_edge_strips(ctx):
+    """n = 6500: the first outer trailing update is 5988 x 5988 with K = 512,
+    whose 128-aligned block (46 x 46 = 2116 tiles) runs in the full-tile
+    form and its 100-wide right and bottom strips in the general form (FP32
+    3M and FP64 products).  A strip written twice, missed or offset would
+    wreck the factors, so the solves are checked against numpy / LAPACK."""
+    n = 6500
+    say(f"[gpu] device LU n={n} (aligned block + edge strips)")
+    rng = np.random.default_rng(6500)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    xs = np.linalg.solve(A, b)
+    _, _, x = bh.debug_lu(A, b=b, ctx=ctx)
+    assert np.abs(x - xs).max() < 1e-9 * np.abs(xs).max()
+    _, _, x32 = bh.debug_lu(A, single=True, b=b, ctx=ctx)
+    ref32 = np.abs(sla.lu_solve(sla.lu_factor(A.astype(np.complex64)), b.astype(np.complex64)) - xs).max()
+    assert np.abs(x32 - xs).max() < 20 * ref32 + 1e-6 * np.abs(xs).max()
+
+
 @pytest.mark.parametrize("n", [300, 1000, 2600, 4100])
 def test_device_lu_tournament_pivots(ctx, n):
     """Multi-leaf panels: the device's pivot sequence is the tournament's
