@@ -235,7 +235,7 @@ def serve_main(args):
 
 
 def solve_main(args):
-    """Config 2: one fluid Mueller solve of the 0.02 x 0.01 m spheroid at
+    """One fluid Mueller solve of the 0.02 x 0.01 m spheroid at
     k = 500 rad/m, N = 20480 panels (2N = 40960 unknowns), mixed precision,
     on the calling thread (the workload the traffic counters are read on)."""
     import bemhip as bh

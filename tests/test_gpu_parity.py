@@ -21,9 +21,9 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-K_CONFIG1 = 2 * math.pi * 38000.0 / 1477.4
-A_CONFIG1 = 0.01
-FLUID_CONFIG1 = (1028.9 / 1026.8, 1480.3 / 1477.4)
+K_SPHERE38 = 2 * math.pi * 38000.0 / 1477.4
+A_SPHERE38 = 0.01
+FLUID_SPHERE38 = (1028.9 / 1026.8, 1480.3 / 1477.4)
 
 
 def say(msg):
@@ -39,7 +39,7 @@ def ctx():
     c.close()
 
 
-def _sphere(ctx, level, radius=A_CONFIG1):
+def _sphere(ctx, level, radius=A_SPHERE38):
     nodes, tri = meshes.sphere(radius, level)
     return bh.Mesh(nodes, tri + 1, ctx=ctx), O.Mesh(nodes, tri)
 
@@ -135,7 +135,7 @@ CASES = [
     ("soft_bm", bh.PressureRelease(), O.SOFT_BM, None),
     ("rigid_cbie", bh.Rigid(), O.RIGID_CBIE, "cbie"),
     ("soft_cbie", bh.PressureRelease(), O.SOFT_CBIE, "cbie"),
-    ("fluid_muller", bh.FluidFilled(*FLUID_CONFIG1), O.FLUID_MULLER, None),
+    ("fluid_muller", bh.FluidFilled(*FLUID_SPHERE38), O.FLUID_MULLER, None),
     ("fluid_cbie", bh.FluidFilled(1.2, 1.1), O.FLUID_CBIE, "cbie"),
 ]
 
@@ -145,30 +145,30 @@ CASES = [
 def test_assembled_matrix_matches_oracle(ctx, name, bc, kind, form, precision):
     say(f"[gpu] assembly {name} {precision} L2 sphere")
     gm, om = _sphere(ctx, 2)
-    op = bh.Operator(gm, bc, K_CONFIG1, formulation=form, precision=precision)
+    op = bh.Operator(gm, bc, K_SPHERE38, formulation=form, precision=precision)
     A = op.matrix()
     g, h = (bc.density_contrast, bc.soundspeed_contrast) if isinstance(bc, bh.FluidFilled) else (1.0, 1.0)
-    R = om.assemble(kind, K_CONFIG1, g, h)
+    R = om.assemble(kind, K_SPHERE38, g, h)
     err = np.abs(A - R).max() / np.abs(R).max()
     tol = 1e-12 if precision == "double" else 2e-6
     assert err < tol, f"{name} {precision}: rel err {err:.3e}"
 
 
 @pytest.mark.parametrize("bcname", ["rigid", "soft", "fluid"])
-def test_config1_solution_matches_oracle_fp64(ctx, bcname):
-    say(f"[gpu] config-1 {bcname} FP64 vs oracle")
+def test_sphere38_solution_matches_oracle_fp64(ctx, bcname):
+    say(f"[gpu] 38 kHz sphere {bcname} FP64 vs oracle")
     gm, om = _sphere(ctx, 3)
     if bcname == "fluid":
-        bc = bh.FluidFilled(*FLUID_CONFIG1)
-        p0, q0, _ = O.solve_fluid(om, *FLUID_CONFIG1, K_CONFIG1, beta=0.0)
+        bc = bh.FluidFilled(*FLUID_SPHERE38)
+        p0, q0, _ = O.solve_fluid(om, *FLUID_SPHERE38, K_SPHERE38, beta=0.0)
     else:
         bc = bh.Rigid() if bcname == "rigid" else bh.PressureRelease()
-        p0, q0, _ = O.solve_boundary(om, bcname, K_CONFIG1, beta=0.0)
-    p, q, _, diag = bh.solve_full_bem(bc, K_CONFIG1, gm, incidence_angle=0.0, return_diagnostics=True)
+        p0, q0, _ = O.solve_boundary(om, bcname, K_SPHERE38, beta=0.0)
+    p, q, _, diag = bh.solve_full_bem(bc, K_SPHERE38, gm, incidence_angle=0.0, return_diagnostics=True)
     assert np.abs(p - p0).max() / np.abs(p0).max() < 1e-9
     assert np.abs(q - q0).max() / np.abs(q0).max() < 1e-9
-    f0 = om.far_field(K_CONFIG1, -O.direction(0.0), p0, q0)[0]
-    sol = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0)
+    f0 = om.far_field(K_SPHERE38, -O.direction(0.0), p0, q0)[0]
+    sol = bh.bem(gm, bc, K_SPHERE38, incidence_angle=0.0)
     f = bh.scattering_amplitude(sol)
     assert abs(f - f0) / abs(f0) < 1e-9
     assert diag["relative_residual"] < 1e-12
@@ -177,22 +177,22 @@ def test_config1_solution_matches_oracle_fp64(ctx, bcname):
 @pytest.mark.parametrize("bcname,level", [("rigid", 3), ("fluid", 3), ("rigid", 4), ("soft", 4), ("fluid", 4)])
 def test_sphere_against_modal_series(ctx, bcname, level):
     say(f"[gpu] modal {bcname} L{level}")
-    gm = bh.mesh(bh.Sphere(A_CONFIG1), level=level, ctx=ctx)
-    bc = {"rigid": bh.Rigid(), "soft": bh.PressureRelease(), "fluid": bh.FluidFilled(*FLUID_CONFIG1)}[bcname]
-    mbc = {"rigid": ("rigid",), "soft": ("soft",), "fluid": ("fluid",) + FLUID_CONFIG1}[bcname]
-    sol = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0)
+    gm = bh.mesh(bh.Sphere(A_SPHERE38), level=level, ctx=ctx)
+    bc = {"rigid": bh.Rigid(), "soft": bh.PressureRelease(), "fluid": bh.FluidFilled(*FLUID_SPHERE38)}[bcname]
+    mbc = {"rigid": ("rigid",), "soft": ("soft",), "fluid": ("fluid",) + FLUID_SPHERE38}[bcname]
+    sol = bh.bem(gm, bc, K_SPHERE38, incidence_angle=0.0)
     f = bh.scattering_amplitude(sol)
-    fm = modal.form_function(mbc, K_CONFIG1, A_CONFIG1)
+    fm = modal.form_function(mbc, K_SPHERE38, A_SPHERE38)
     assert abs(bh.target_strength(f) - modal.target_strength(fm)) < 0.1
     assert abs(f - fm) / abs(fm) < 0.01
 
 
-def test_mixed_matches_double_config1_fluid(ctx):
-    say("[gpu] mixed vs double, config-1 fluid L3")
+def test_mixed_matches_double_sphere38_fluid(ctx):
+    say("[gpu] mixed vs double, 38 kHz fluid sphere L3")
     gm, _ = _sphere(ctx, 3)
-    bc = bh.FluidFilled(*FLUID_CONFIG1)
-    sd = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0)
-    sm = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0, precision="mixed")
+    bc = bh.FluidFilled(*FLUID_SPHERE38)
+    sd = bh.bem(gm, bc, K_SPHERE38, incidence_angle=0.0)
+    sm = bh.bem(gm, bc, K_SPHERE38, incidence_angle=0.0, precision="mixed")
     fd, fm = bh.scattering_amplitude(sd), bh.scattering_amplitude(sm)
     d = sm.diagnostics
     assert d["factor_precision"] == "single"
@@ -207,10 +207,10 @@ def test_mixed_matches_double_config1_fluid(ctx):
 def test_mixed_refinement_reaches_fp32_system_solution(ctx):
     say("[gpu] mixed refinement vs oracle solve of the same FP32-assembled matrix")
     gm, om = _sphere(ctx, 2)
-    bc = bh.FluidFilled(*FLUID_CONFIG1)
-    op = bh.Operator(gm, bc, K_CONFIG1, precision="mixed")
+    bc = bh.FluidFilled(*FLUID_SPHERE38)
+    op = bh.Operator(gm, bc, K_SPHERE38, precision="mixed")
     A = op.matrix()  # FP32-assembled system, widened to FP64
-    p0, q0, dd = O.solve_fluid(om, *FLUID_CONFIG1, K_CONFIG1, A=A)
+    p0, q0, dd = O.solve_fluid(om, *FLUID_SPHERE38, K_SPHERE38, A=A)
     sol = op.solve([(0.0, 0.0)])
     p, q = sol.traces(0)
     assert np.abs(p - p0).max() / np.abs(p0).max() < 1e-9
@@ -220,11 +220,11 @@ def test_mixed_refinement_reaches_fp32_system_solution(ctx):
 def test_forced_promotion(ctx):
     say("[gpu] forced promotion to FP64")
     gm, _ = _sphere(ctx, 2)
-    bc = bh.FluidFilled(*FLUID_CONFIG1)
-    s = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0, precision="mixed", _force_promotion=True)
+    bc = bh.FluidFilled(*FLUID_SPHERE38)
+    s = bh.bem(gm, bc, K_SPHERE38, incidence_angle=0.0, precision="mixed", _force_promotion=True)
     d = s.diagnostics
     assert d["factor_precision"] == "double" and d["refinement_reason"] == "condition"
-    sd = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0)
+    sd = bh.bem(gm, bc, K_SPHERE38, incidence_angle=0.0)
     assert abs(bh.scattering_amplitude(s) - bh.scattering_amplitude(sd)) / abs(bh.scattering_amplitude(sd)) < 1e-5
 
 
@@ -233,7 +233,7 @@ def test_condition_guard_forces_promotion(ctx):
     (test/basic/bem/formulations.jl:424-425)."""
     say("[gpu] condition guard 1.0")
     gm, _ = _sphere(ctx, 2)
-    s = bh.bem(gm, bh.FluidFilled(*FLUID_CONFIG1), K_CONFIG1, incidence_angle=0.0, precision="mixed",
+    s = bh.bem(gm, bh.FluidFilled(*FLUID_SPHERE38), K_SPHERE38, incidence_angle=0.0, precision="mixed",
                refinement={"condition_guard": 1.0})
     d = s.diagnostics
     assert d["factor_precision"] == "double" and d["refinement_reason"] == "condition"
@@ -245,15 +245,15 @@ def test_reciprocal_condition_estimate_matches_lapack(ctx):
     say("[gpu] rcond estimate vs cgecon")
     from scipy.linalg import lapack
     gm, _ = _sphere(ctx, 2)
-    bc = bh.FluidFilled(*FLUID_CONFIG1)
-    op = bh.Operator(gm, bc, K_CONFIG1, precision="mixed")
+    bc = bh.FluidFilled(*FLUID_SPHERE38)
+    op = bh.Operator(gm, bc, K_SPHERE38, precision="mixed")
     A = op.matrix()
     rows, cols = O.equilibrate(A)
     low = (A / rows[:, None] / cols[None, :]).astype(np.complex64)
     anorm = float(np.abs(low).sum(axis=1).max())
     lu, piv, info = lapack.cgetrf(low)
     rc_ref, _ = lapack.cgecon(lu, anorm, norm="I")
-    s = bh.bem(gm, bc, K_CONFIG1, incidence_angle=0.0, precision="mixed")
+    s = bh.bem(gm, bc, K_SPHERE38, incidence_angle=0.0, precision="mixed")
     rc = s.diagnostics["single_reciprocal_condition"]
     assert np.isfinite(rc) and 0 < rc <= 1
     assert 0.1 < rc / rc_ref < 10
@@ -262,7 +262,7 @@ def test_reciprocal_condition_estimate_matches_lapack(ctx):
 def test_iteration_limit_promotes(ctx):
     say("[gpu] maxiter=0 forces iteration_limit promotion")
     gm, _ = _sphere(ctx, 2)
-    s = bh.bem(gm, bh.FluidFilled(*FLUID_CONFIG1), K_CONFIG1, incidence_angle=0.0, precision="mixed",
+    s = bh.bem(gm, bh.FluidFilled(*FLUID_SPHERE38), K_SPHERE38, incidence_angle=0.0, precision="mixed",
                refinement={"maxiter": 0})
     d = s.diagnostics
     assert d["factor_precision"] == "double"
@@ -322,9 +322,9 @@ def test_incidence_sweep_matches_single_solves(ctx):
     say("[gpu] incidence-angle sweep vs single solves")
     gm, _ = _sphere(ctx, 2)
     angles = [0.0, 0.7, 1.9, math.pi]
-    amps, ts = bh.incidence_angle_sweep(gm, bh.Rigid(), K_CONFIG1, angles)
+    amps, ts = bh.incidence_angle_sweep(gm, bh.Rigid(), K_SPHERE38, angles)
     for a, f in zip(angles, amps):
-        s = bh.bem(gm, bh.Rigid(), K_CONFIG1, incidence_angle=a)
+        s = bh.bem(gm, bh.Rigid(), K_SPHERE38, incidence_angle=a)
         assert abs(bh.scattering_amplitude(s) - f) <= 1e-12 * abs(f)
 
 

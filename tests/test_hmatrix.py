@@ -19,7 +19,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-K1 = 2 * math.pi * 38000.0 / 1477.4  # config-1 wavenumber, sphere a = 0.01
+K1 = 2 * math.pi * 38000.0 / 1477.4  # 38 kHz in c = 1477.4 m/s, sphere a = 0.01 m
 
 
 def say(msg):

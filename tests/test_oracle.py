@@ -302,7 +302,7 @@ def test_tetrahedron_fixture_solves():
 
 @pytest.mark.parametrize("g,h", [(1.5, 1.3), (0.5, 0.8)])
 def test_fluid_cbie_and_muller_against_modal(g, h):
-    """Both fluid formulations at strong contrast.  (At the weak config-1
+    """Both fluid formulations at strong contrast.  (At the weak 38 kHz sphere
     contrast CBIE's right-hand side is a difference of nearly cancelling
     interior operators and carries ~0.6 dB of O(h) error; Mueller is used
     there, as the reference's default.)"""
