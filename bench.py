@@ -97,6 +97,21 @@ def measured_traffic():
     return None
 
 
+def size_lanes(bh, ctx, plan, meshes, lanes, precision):
+    """Allocate every lane's device arena at the sweep's largest footprint,
+    untimed: one solve of the largest mesh and small ones on the other lanes.
+    A context allocates its lane arenas once and reuses them for later sweeps
+    of that footprint; allocating ~8 x 14 GB inside the timed sweep added
+    0-3.4 s to it, depending on what the device was still clearing from the
+    previous process."""
+    fl = bh.FluidFilled(G_RHO, H_C)
+    big, small = max(meshes), min(meshes)
+    k_big = max(k for (_, k, L) in plan if L == big)
+    ms = [meshes[big]] + [meshes[small]] * (lanes - 1)
+    probs = [bh.make_problem(fl, k_big if i == 0 else plan[0][1], precision=precision) for i in range(lanes)]
+    bh.batched_sweep(ms, probs, [(0.0, 0.0)] * lanes, ctx=ctx)
+
+
 def run_sweep(bh, ctx, plan, meshes, precision, chunk=64, serial=False):
     """One pass over the sweep as one bem_sweep call (the library orders the
     solves, largest first, and runs the matrix-core-bound ones one at a time
@@ -406,6 +421,8 @@ def main():
     # sweep that follows an FP64 one in the same process (whose lane arenas
     # are twice as large and are freed for the mixed ones) ran its trailing
     # updates 18 % slower, while the FP64 sweep is not slowed by running second.
+    if not args.serial:
+        size_lanes(bh, ctx, plan, meshes, args.lanes, "mixed")
     for w in range(args.warmup):
         log(f"warmup {w + 1}/{args.warmup} ...")
         run_sweep(bh, ctx, plan, meshes, "mixed", serial=args.serial)
@@ -437,6 +454,8 @@ def main():
             "avg_launch_ms": gs["ms"] / max(1, gs["launches"]),
             "algorithmic_bytes_per_launch": gs["bytes"] / max(1, gs["launches"])}
     log("FP64 reference sweep ...")
+    if not args.serial:
+        size_lanes(bh, ctx, plan, meshes, args.lanes, "double")
     a64, lat64, _, t64 = run_sweep(bh, ctx, plan, meshes, "double", serial=args.serial)
     log(f"FP64 sweep {t64:.2f}s ({len(plan) / t64:.2f} solves/s)")
     ts64 = 20 * np.log10(np.abs(a64))
