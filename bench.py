@@ -10,10 +10,12 @@ FP64 far field.  |dTS| is checked against the same sweep solved all-FP64.
 
 Prints one JSON line (value = solves/s of the timed sweep).
 
-The sweep runs eight lanes (two HIP streams each) under 16 hardware queues
-(GPU_MAX_HW_QUEUES, `--hw-queues`): with the runtime's default of 4, the
-lanes' streams share queues and a lane's kernels wait in order behind
-another lane's.  The runtime reads the count once, when it starts, so when
+The sweep runs four lanes (two HIP streams each) under 8 hardware queues
+(GPU_MAX_HW_QUEUES, `--hw-queues`, two per lane): with the runtime's default
+of 4, the lanes' streams share queues and a lane's kernels wait in order
+behind another lane's.  The sweep is bound by its chain of heavy solves, so
+4, 6 and 8 lanes give the same throughput; four give the lowest median
+latency.  The runtime reads the count once, when it starts, so when
 the environment holds another value the benchmark runs as a child process
 started with this one, and exits with the child's status.  One solve with
 16384 or more unknowns runs at a time (`--heavy-lanes`, the library's
@@ -363,7 +365,8 @@ def main():
     ap.add_argument("--hm-rhs", type=int, default=64)
     ap.add_argument("--rates", type=float, nargs="+", default=[50.0, 200.0, 800.0])
     ap.add_argument("--requests", type=int, default=200)
-    ap.add_argument("--lanes", type=int, default=8, help="concurrent solves (sweep) or server lanes (serve)")
+    ap.add_argument("--lanes", type=int, default=None,
+                    help="concurrent solves (sweep, default 4) or server lanes (serve, default 8)")
     ap.add_argument("--heavy-lanes", type=int, default=1,
                     help="sweep solves with >= 16384 unknowns allowed to run at once")
     ap.add_argument("--hw-queues", type=int, default=None, choices=range(1, 33), metavar="1..32",
@@ -379,6 +382,8 @@ def main():
                     help="sweep one solve at a time on the main thread (for rocprofv3 --pmc)")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     args = ap.parse_args()
+    if args.lanes is None:
+        args.lanes = 8 if args.workload == "serve" else 4
     if args.hw_queues is None:
         args.hw_queues = min(32, 2 * args.lanes)
     if os.environ.get("GPU_MAX_HW_QUEUES") != str(args.hw_queues):
@@ -439,7 +444,7 @@ def main():
              ("assemble", "near", "lu_panel", "lu_gemm", "lu_gemm_fp64", "lu_trsm", "lu_swap", "solve", "residual",
               "far_field")}
     # The same trailing-update kernel on one heavy solve with nothing else on
-    # the GPU: under eight lanes a launch's event time includes sharing the
+    # the GPU: under several lanes a launch's event time includes sharing the
     # CUs with the other lanes' kernels, which the sweep's figure cannot undo.
     heavy_mesh = meshes[max(meshes)]
     k_heavy = max(k for (_, k, L) in plan if meshes[L] is heavy_mesh)
