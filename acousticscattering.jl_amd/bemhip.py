@@ -47,7 +47,8 @@ EXPORTED = [
     "bem_solution_diagnostics", "bem_sweep", "bem_server_create", "bem_server_destroy",
     "bem_server_submit", "bem_server_poll", "bem_server_wait", "bem_server_error",
     "bem_server_stats_get", "bem_hmatrix_defaults", "bem_hmatrix_plan", "bem_hmatrix_create",
-    "bem_hmatrix_destroy", "bem_hmatrix_get_info", "bem_hmatrix_matvec", "bem_hmatrix_solve", "bem_debug_lu",
+    "bem_hmatrix_destroy", "bem_hmatrix_get_info", "bem_hmatrix_matvec", "bem_hmatrix_apply", "bem_hmatrix_solve",
+    "bem_debug_lu",
 ]
 
 
@@ -182,6 +183,7 @@ def load_library(path=LIB_PATH):
         "bem_hmatrix_destroy": (ctypes.c_int, [_vp]),
         "bem_hmatrix_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(HMatrixInfo)]),
         "bem_hmatrix_matvec": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
+        "bem_hmatrix_apply": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _dp, _dp]),
         "bem_hmatrix_solve": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.POINTER(_vp)]),
     }
     for name, (res, args) in sig.items():
@@ -868,17 +870,20 @@ class HMatrix:
         self.ctx.check(self.ctx.lib.bem_hmatrix_get_info(self._h, ctypes.byref(st)))
         return {name: getattr(st, name) for name, _ in st._fields_}
 
-    def matvec(self, x):
+    def matvec(self, x, operator="system"):
         """A x for x of shape (n,) or (n, nrhs) in unknown order (panels; for
-        fluid bodies the N pressures, then the N normal derivatives)."""
+        fluid bodies the N pressures, then the N normal derivatives).
+        operator="rhs" applies the right-hand-side operator of a rigid
+        surface instead (b = B q_inc)."""
+        which = {"system": 0, "rhs": 1}[operator]
         x = np.asarray(x, dtype=np.complex128)
         one = x.ndim == 1
         if x.shape[0] != self.n or x.ndim > 2:
             raise ValueError(f"x must have {self.n} rows")
         X = np.asfortranarray(x.reshape(self.n, -1))
         Y = np.empty_like(X, order="F")
-        self.ctx.check(self.ctx.lib.bem_hmatrix_matvec(self._h, X.shape[1], X.ctypes.data_as(_dp),
-                                                       Y.ctypes.data_as(_dp)))
+        self.ctx.check(self.ctx.lib.bem_hmatrix_apply(self._h, which, X.shape[1], X.ctypes.data_as(_dp),
+                                                      Y.ctypes.data_as(_dp)))
         return Y[:, 0] if one else Y
 
     def solve(self, angles):

@@ -2828,10 +2828,16 @@ int bem_hmatrix_get_info(bem_hmatrix h, bem_hmatrix_info* out) {
 }
 
 int bem_hmatrix_matvec(bem_hmatrix h, int32_t nrhs, const double* x, double* y) {
-    if (!h || nrhs <= 0 || !x || !y) return BEM_ERR_ARGUMENT;
+    return bem_hmatrix_apply(h, 0, nrhs, x, y);
+}
+
+int bem_hmatrix_apply(bem_hmatrix h, int32_t which, int32_t nrhs, const double* x, double* y) {
+    if (!h || nrhs <= 0 || !x || !y || (which != 0 && which != 1)) return BEM_ERR_ARGUMENT;
     bem_ctx ctx = h->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     return guarded(ctx, [&] {
+        if (which == 1 && h->fluid)
+            throw BemError(BEM_ERR_ARGUMENT, "a fluid body's right-hand side is not an operator");
         const int64_t n = h->n;
         hipStream_t s = h->lane->s;
         z64* X = (z64*)ctx->alloc(sizeof(z64) * n * kHRhs);
@@ -2847,7 +2853,7 @@ int bem_hmatrix_matvec(bem_hmatrix h, int32_t nrhs, const double* x, double* y) 
                         hx[pos * nr + c] = z64{src[0], src[1]};
                     }
                 HIPCHECK(hipMemcpyAsync(X, hx.data(), sizeof(z64) * n * nr, hipMemcpyHostToDevice, s));
-                hm_apply(h, h->A, X, nr, Y);
+                hm_apply(h, which == 0 ? h->A : h->B, X, nr, Y);
                 HIPCHECK(hipGetLastError());
                 HIPCHECK(hipMemcpyAsync(hy.data(), Y, sizeof(z64) * n * nr, hipMemcpyDeviceToHost, s));
                 HIPCHECK(hipStreamSynchronize(s));

@@ -9,6 +9,11 @@ the characteristic radius max_s hypot(x, r) (the reference's
 `_characteristic_radius`, src/geometry/mesh.jl:6).  Each body is meshed by
 rings of revolution at edge <= lambda / 6, clamped to N in [2000, 40000]
 triangles.
+
+Frequency sweep: 256 frequencies log-spaced over 1-200 kHz at c = 1500 m/s on
+a fluid-filled prolate spheroid (semi-axes 0.02 x 0.01 m along x, g = 1.0357,
+h = 1.0279); each frequency takes the smallest mapped icosphere (levels 3-5,
+N = 1280, 5120, 20480 triangles) whose longest edge is <= lambda / 6.
 """
 import math
 
@@ -84,3 +89,31 @@ def serving_stream(rate, count, seed):
         out.append({"t": float(t[i]), "a": a, "aspect": aspect, "eps": eps, "boundary": bc, "k": float(k),
                      "ka": float(ka), "nodes": nodes, "triangles": tri})
     return out
+
+
+SWEEP_C = 1500.0
+SWEEP_AXES = (0.02, 0.01)
+SWEEP_FLUID = (1.0357, 1.0279)
+
+
+def sweep_plan(ctx, n_points=256, max_level=5, min_level=3):
+    """The frequency sweep: returns plan [(f, k, level)] and the device meshes
+    {level: Mesh}.  k = 2 pi f / c (src/postprocessing/sweeps.jl:107)."""
+    a, b = SWEEP_AXES
+    freqs = np.logspace(math.log10(1e3), math.log10(200e3), n_points)
+    ks = 2 * math.pi * freqs / SWEEP_C
+    levels = list(range(min_level, max_level + 1))
+    meshes = {}
+    for L in levels:
+        nodes, tri = bh.icosphere_arrays(L, a, b, b)
+        meshes[L] = bh.Mesh(nodes, tri, ctx=ctx)
+    plan = []
+    for f, k in zip(freqs, ks):
+        lam = SWEEP_C / f
+        lev = levels[-1]
+        for L in levels:
+            if meshes[L].resolution <= lam / 6.0:
+                lev = L
+                break
+        plan.append((float(f), float(k), lev))
+    return plan, meshes

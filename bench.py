@@ -15,13 +15,17 @@ The sweep runs four lanes (two HIP streams each) under 8 hardware queues
 of 4, the lanes' streams share queues and a lane's kernels wait in order
 behind another lane's.  The sweep is bound by its chain of heavy solves, so
 4, 6 and 8 lanes give the same throughput; four give the lowest median
-latency.  The runtime reads the count once, when it starts, so when
-the environment holds another value the benchmark runs as a child process
-started with this one, and exits with the child's status.  One solve with
-16384 or more unknowns runs at a time (`--heavy-lanes`, the library's
-default) while the other lanes take the smaller ones: the sweep's throughput
-is within 2 % of letting them run together, at a third of the median and
-half of the tail latency.
+latency.  The variable is set in this process before the library (and so
+the HIP runtime) is loaded.  One solve with 16384 or more unknowns runs at a
+time (`--heavy-lanes`, the library's default) while the other lanes take the
+smaller ones: the sweep's throughput is within 2 % of letting them run
+together, at a third of the median and half of the tail latency.
+
+Time budget (`--budget-s`, default 400 s for the whole process): after
+`--min-steps` timed sweeps (3), further warm-up and timed sweeps run only
+while the rest of the run (the solo heavy solve, the FP64 reference sweep at
+about 2.1x a mixed sweep, and the CPU leg) still fits.  `steps` and `warmup`
+in the JSON are the counts actually run, beside the requested ones.
 
 `--workload hmatrix` measures the hierarchical-matrix path: a rigid bent
 capped cylinder (radius 2 mm, centerline 60 mm, bend radius 120 mm,
@@ -38,7 +42,6 @@ import argparse
 import json
 import math
 import os
-import subprocess
 import sys
 import time
 
@@ -60,23 +63,9 @@ def log(msg):
 
 
 def sweep_plan(bh, ctx, n_points, max_level):
-    freqs = np.logspace(math.log10(1e3), math.log10(200e3), n_points)
-    ks = 2 * math.pi * freqs / C_SOUND
-    levels = list(range(3, max_level + 1))
-    meshes = {}
-    for L in levels:
-        nodes, tri = bh.icosphere_arrays(L, A_MAJOR, B_MINOR, B_MINOR)
-        meshes[L] = bh.Mesh(nodes, tri, ctx=ctx)
-    plan = []
-    for f, k in zip(freqs, ks):
-        lam = C_SOUND / f
-        lev = levels[-1]
-        for L in levels:
-            if meshes[L].resolution <= lam / 6.0:
-                lev = L
-                break
-        plan.append((f, k, lev))
-    return plan, meshes
+    import workloads as W
+
+    return W.sweep_plan(ctx, n_points, max_level)
 
 
 TRAFFIC_PROFILE = os.path.join("profiles", "lu_gemm3m_traffic_fluid_spheroid_n20480.md")
@@ -97,6 +86,38 @@ def measured_traffic():
     except (OSError, ValueError):
         pass
     return None
+
+
+# Work of one fused kernel evaluation in the regular-pair assembly
+# (k_assemble, all four operators S, D, K', H at one target, one source
+# Gauss point and one medium), counting an FMA as 2 flops and the
+# transcendentals apart: |r|^2 5, R and kR 2, the 1/(4 pi R) weight 2,
+# e^{ikR} scaling 2, S 2, D 16, K' 16, H 18 = 63 flops, plus r = x - y
+# (3 flops) shared by the two media of a Mueller pair (1.5 per medium),
+# and 1 rsqrt + 1 sincos.
+F_K_MULLER = 64.5
+FP32_VECTOR_PEAK = 157.3e12  # published; packed FP32 (v_pk_*) is disabled in this build, which halves it
+
+
+def assembly_roofline(sweep, solo):
+    """Achieved FLOP rate of the regular-pair assembly (Mueller, FP32 in the
+    mixed form) from the library's event timers: evaluations (N^2 x 7 points
+    x 2 media per launch) x F_K_MULLER / event time."""
+    def rate(st):
+        return st["flops"] * F_K_MULLER / (st["ms"] * 1e-3) if st["ms"] > 0 else 0.0
+
+    r_solo, r_sweep = rate(solo), rate(sweep)
+    t_solo = solo["ms"] * 1e-3 / max(1, solo["launches"])
+    return {"kernel": "k_assemble<float, MULLER> (regular panel pairs, both media)", "bound": "valu",
+            "unit": "TFLOP/s", "achieved": r_solo / 1e12, "peak": FP32_VECTOR_PEAK / 1e12,
+            "frac": r_solo / FP32_VECTOR_PEAK, "frac_of_nonpacked_peak": 2 * r_solo / FP32_VECTOR_PEAK,
+            "basis": "one N=20480 launch alone (solo solve)",
+            "sweep_achieved": r_sweep / 1e12, "sweep_launches": sweep["launches"],
+            "flops_per_evaluation": F_K_MULLER, "transcendentals_per_evaluation": {"sincos": 1, "rsqrt": 1},
+            "evaluations_per_launch": solo["flops"] / max(1, solo["launches"]),
+            "bytes_written_per_launch": solo["bytes"] / max(1, solo["launches"]),
+            "store_hbm_frac": (solo["bytes"] / max(1, solo["launches"]) / t_solo / 8e12) if t_solo > 0 else None,
+            "avg_launch_ms": t_solo * 1e3}
 
 
 def size_lanes(bh, ctx, plan, meshes, lanes, precision):
@@ -146,11 +167,18 @@ def run_sweep(bh, ctx, plan, meshes, precision, chunk=64, serial=False):
     return amps, lat, diags, time.perf_counter() - t0
 
 
-def cpu_baseline(plan, budget_s=30.0):
-    """Oracle (C assembly with OpenMP + LAPACK LU) on a bounded sample; the
-    sweep rate is extrapolated per mesh class (assembly ~ N^2, LU ~ N^3)."""
+def cpu_baseline(plan, budget_s=60.0):
+    """Oracle (C assembly with OpenMP + LAPACK LU through scipy) on a bounded
+    sample of the sweep: one full solve per mesh class up to N = 5120
+    (2N = 10240 unknowns), then one LU at 2N = 20480 of the N = 5120 system
+    tiled as [[A, A/2], [A/2, A]] (nonsingular: its blocks diagonalise to
+    (1 +- 1/2) A), when the 10240 LU's time x 8 fits the remaining budget.
+    The sweep rate is extrapolated per class: assembly ~ N^2 from N = 5120,
+    LU ~ n^3 from the largest LU timed (8x from 2N = 20480 to the heavy
+    class's 2N = 40960)."""
     from oracle import meshes as om
     from oracle import oracle as O
+    import scipy.linalg as sla
 
     threads = O.lib().orc_num_threads()
     classes = sorted(set(L for (_, _, L) in plan))
@@ -158,6 +186,7 @@ def cpu_baseline(plan, budget_s=30.0):
     per = {}
     sample = []
     t_start = time.perf_counter()
+    A_keep = None
     for L in classes:
         if L > 4 or time.perf_counter() - t_start > budget_s:
             continue
@@ -171,23 +200,43 @@ def cpu_baseline(plan, budget_s=30.0):
         t2 = time.perf_counter()
         log(f"  cpu baseline N={m.n}: assembly {t1 - t0:.2f}s, LU+solve {t2 - t1:.2f}s")
         per[L] = (t1 - t0, t2 - t1, m.n)
-        sample.append(f"N={m.n}: assembly {t1 - t0:.2f}s, LU+solve {t2 - t1:.2f}s")
+        sample.append(f"N={m.n} (2N={2 * m.n}): assembly {t1 - t0:.2f}s, LU+solve {t2 - t1:.2f}s")
+        A_keep = A
     if not per:
         return None
     lmax = max(per)
     ta, tl, nref = per[lmax]
+    lu_n, lu_t = 2 * nref, tl
+    left = budget_s - (time.perf_counter() - t_start)
+    if A_keep is not None and 9.0 * tl < left:
+        n2 = 2 * A_keep.shape[0]
+        big = np.empty((n2, n2), dtype=np.complex128, order="F")
+        h = A_keep.shape[0]
+        big[:h, :h] = A_keep
+        big[h:, h:] = A_keep
+        big[:h, h:] = 0.5 * A_keep
+        big[h:, :h] = 0.5 * A_keep
+        del A_keep
+        t0 = time.perf_counter()
+        sla.lu_factor(big, overwrite_a=True, check_finite=False)
+        lu_t = time.perf_counter() - t0
+        lu_n = n2
+        del big
+        log(f"  cpu baseline LU n={n2}: {lu_t:.2f}s")
+        sample.append(f"complex LU n={n2} (tiled N={nref} system): {lu_t:.2f}s")
     total = 0.0
     for L in classes:
         n = 20 * 4 ** L
         if L in per:
             t = per[L][0] + per[L][1]
         else:
-            r = n / nref
-            t = ta * r ** 2 + tl * r ** 3
+            t = ta * (n / nref) ** 2 + lu_t * (2 * n / lu_n) ** 3
         total += counts[L] * t
     return {"value": len(plan) / total, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": "FP64 oracle timed on " + "; ".join(sample) +
-                      f"; sweep time extrapolated per class from N={nref} (assembly ~N^2, LU ~N^3)"}
+            "sample": "FP64 oracle (OpenMP assembly, LAPACK LU) timed on " + "; ".join(sample) +
+                      f"; sweep time extrapolated per class: assembly ~N^2 from N={nref}, "
+                      f"LU ~n^3 from n={lu_n}",
+            "seconds": time.perf_counter() - t_start}
 
 
 def serve_rate(bh, ctx, rate, count, seed, lanes):
@@ -358,6 +407,13 @@ def hmatrix_main(args):
     ctx.close()
 
 
+T_START = time.perf_counter()
+
+
+def elapsed():
+    return time.perf_counter() - T_START
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=["sweep", "serve", "hmatrix", "solve"], default="sweep")
@@ -373,26 +429,33 @@ def main():
                     help="HIP hardware queues (GPU_MAX_HW_QUEUES) the benchmark runs under "
                          "(default two per lane, at most 32)")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--min-steps", type=int, default=3,
+                    help="timed sweeps run even when the budget estimate says they do not fit")
+    ap.add_argument("--budget-s", type=float, default=400.0,
+                    help="wall-clock budget of the whole sweep run (setup, warm-up, timed sweeps, FP64 "
+                         "reference sweep, CPU leg): warm-up and timed sweeps beyond --min-steps stop "
+                         "when the rest would not fit")
     ap.add_argument("--points", type=int, default=256)
     ap.add_argument("--max-level", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serial", action="store_true",
                     help="sweep one solve at a time on the main thread (for rocprofv3 --pmc)")
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--cpu-budget", type=float, default=60.0)
     args = ap.parse_args()
-    if args.lanes is None:
-        args.lanes = 8 if args.workload == "serve" else 4
-    if args.hw_queues is None:
-        args.hw_queues = min(32, 2 * args.lanes)
-    if os.environ.get("GPU_MAX_HW_QUEUES") != str(args.hw_queues):
-        env = dict(os.environ, GPU_MAX_HW_QUEUES=str(args.hw_queues))
-        sys.exit(subprocess.run([sys.executable] + sys.argv, env=env).returncode)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only (--gpus 1); across GPUs the frequency sweep would "
               "shard its frequencies, with no exchange but the amplitudes.", file=sys.stderr)
         sys.exit(2)
+    if args.lanes is None:
+        args.lanes = 8 if args.workload == "serve" else 4
+    if args.hw_queues is None:
+        args.hw_queues = min(32, 2 * args.lanes)
+    # The HIP runtime reads GPU_MAX_HW_QUEUES when it initialises, which
+    # happens when libbemhip is loaded below (nothing before it touches the
+    # GPU), so setting it here takes effect in this process.
+    os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     if args.workload == "serve":
         serve_main(args)
         return
@@ -402,7 +465,10 @@ def main():
     if args.workload == "solve":
         solve_main(args)
         return
+    sweep_main(args)
 
+
+def sweep_main(args):
     import bemhip as bh
 
     ctx = bh.Context(0)
@@ -428,17 +494,42 @@ def main():
     # updates 18 % slower, while the FP64 sweep is not slowed by running second.
     if not args.serial:
         size_lanes(bh, ctx, plan, meshes, args.lanes, "mixed")
+    log(f"setup done at {elapsed():.1f}s")
+
+    # Budget: what must still run after the timed sweeps is the solo heavy
+    # solve, the FP64 sweep (measured at about 2.1x the mixed one, with its
+    # lane arenas) and the CPU leg.
+    est = [30.0]  # one mixed sweep [s], until one is measured
+
+    def reserve():
+        cpu = 0.0 if args.no_cpu else args.cpu_budget + 15.0
+        return 2.4 * est[0] + 20.0 + cpu
+
+    def fits(n_sweeps):
+        return elapsed() + n_sweeps * est[0] + reserve() <= args.budget_s
+
+    warm_done = 0
     for w in range(args.warmup):
+        if not fits(1 + args.min_steps):
+            break
         log(f"warmup {w + 1}/{args.warmup} ...")
-        run_sweep(bh, ctx, plan, meshes, "mixed", serial=args.serial)
+        _, _, _, t = run_sweep(bh, ctx, plan, meshes, "mixed", serial=args.serial)
+        est[0] = t
+        warm_done += 1
     ctx.reset_stats()
     ctx.set_profiling(True)
     times, lats, amps, diags = [], [], None, None
+    all_diags = []
     for s in range(args.steps):
+        if s >= args.min_steps and not fits(1):
+            log(f"budget: stopping after {s} timed sweeps ({elapsed():.0f}s of {args.budget_s:.0f}s used)")
+            break
         amps, lat, diags, t = run_sweep(bh, ctx, plan, meshes, "mixed", serial=args.serial)
         times.append(t)
+        est[0] = max(times)
         lats.extend(lat.tolist())
-        log(f"step {s + 1}/{args.steps}: {t:.2f}s ({len(plan) / t:.2f} solves/s)")
+        all_diags.extend(diags)
+        log(f"step {s + 1}/{args.steps}: {t:.2f}s ({len(plan) / t:.2f} solves/s), at {elapsed():.0f}s")
     ctx.set_profiling(False)
     stats = {name: ctx.kernel_stats(name) for name in
              ("assemble", "near", "lu_panel", "lu_gemm", "lu_gemm_fp64", "lu_trsm", "lu_swap", "solve", "residual",
@@ -453,34 +544,43 @@ def main():
     bh.bem(heavy_mesh, fl, k_heavy, incidence_angle=0.0, precision="mixed")
     ctx.set_profiling(False)
     gs = ctx.kernel_stats("lu_gemm")
+    ga = ctx.kernel_stats("assemble")
     solo_rate = gs["flops"] / (gs["ms"] * 1e-3) if gs["ms"] > 0 else 0.0
     solo = {"workload": f"one mixed solve alone after the sweep: N={heavy_mesh.n} (2N unknowns), k={k_heavy:.1f}",
             "achieved": solo_rate / 1e12, "frac": solo_rate / MFMA_FP32_PEAK, "launches": gs["launches"],
             "avg_launch_ms": gs["ms"] / max(1, gs["launches"]),
             "algorithmic_bytes_per_launch": gs["bytes"] / max(1, gs["launches"])}
-    log("FP64 reference sweep ...")
+    log(f"FP64 reference sweep at {elapsed():.0f}s ...")
     if not args.serial:
         size_lanes(bh, ctx, plan, meshes, args.lanes, "double")
     a64, lat64, _, t64 = run_sweep(bh, ctx, plan, meshes, "double", serial=args.serial)
-    log(f"FP64 sweep {t64:.2f}s ({len(plan) / t64:.2f} solves/s)")
+    log(f"FP64 sweep {t64:.2f}s ({len(plan) / t64:.2f} solves/s), at {elapsed():.0f}s")
     ts64 = 20 * np.log10(np.abs(a64))
     tsm = 20 * np.log10(np.abs(amps))
     dts = float(np.max(np.abs(tsm - ts64)))
-    promoted = sum(1 for d in diags if d["factor_precision"] == "double")
+    rel = float(np.max(np.abs(amps - a64) / np.abs(a64)))
+    promoted = sum(1 for d in all_diags if d["factor_precision"] == "double")
+    unconverged = sum(1 for d in all_diags if not d["converged"])
+    worst_be = max(max(d["normwise_backward_error"], d["componentwise_backward_error"]) for d in all_diags)
     t_step = float(np.mean(times))
     value = len(plan) / t_step
     g = stats["lu_gemm"]
     achieved = g["flops"] / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
-    cpu = None if args.no_cpu else cpu_baseline(plan, args.cpu_budget)
+    cpu_left = args.budget_s - elapsed() - 5.0
+    cpu = None if args.no_cpu else cpu_baseline(plan, max(10.0, min(args.cpu_budget, cpu_left)))
     full = args.points == 256 and args.max_level == 5
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": len(times),
+        "warmup": warm_done,
+        "steps_requested": args.steps,
+        "warmup_requested": args.warmup,
+        "budget_s": args.budget_s,
         "ms_per_step": t_step * 1e3,
+        "step_s": [round(t, 3) for t in times],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -498,8 +598,11 @@ def main():
             "global_batch": len(plan),
         },
         "max_abs_dTS_dB": dts,
+        "max_rel_amplitude_vs_fp64": rel,
         "dTS_ok": dts <= 0.01,
         "promoted_to_fp64": promoted,
+        "unconverged": unconverged,
+        "max_backward_error": worst_be,
         "p50_latency_ms": float(np.percentile(lats, 50)),
         "p99_latency_ms": float(np.percentile(lats, 99)),
         "fp64_solves_per_s": len(plan) / t64,
@@ -514,8 +617,10 @@ def main():
                      "algorithmic_bytes_per_launch": g["bytes"] / max(1, g["launches"]),
                      "launches": g["launches"], "avg_launch_ms": g["ms"] / max(1, g["launches"]),
                      "solo": solo},
+        "assembly_roofline": assembly_roofline(stats["assemble"], ga),
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in stats.items()},
         "cpu_baseline": cpu,
+        "wall_s": elapsed(),
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -322,6 +322,12 @@ int bem_hmatrix_get_info(bem_hmatrix h, bem_hmatrix_info *out);
 /* y = A x for n_rhs columns (unknown order, n x n_rhs column-major complex;
  * n = bem_hmatrix_info.n: N panels, or 2N unknowns for a fluid body). */
 int bem_hmatrix_matvec(bem_hmatrix h, int32_t n_rhs, const double *x, double *y);
+/* Either compressed operator: which = 0 the system (as bem_hmatrix_matvec),
+ * 1 the right-hand-side operator S [+ (i/k)(1/2 I + K')] of a rigid surface
+ * (full.jl:144-148), so a caller can recompute b = B q_inc and the residual
+ * b - A x outside GMRES.  Fluid bodies have no right-hand-side operator
+ * (BEM_ERR_ARGUMENT for which = 1). */
+int bem_hmatrix_apply(bem_hmatrix h, int32_t which, int32_t n_rhs, const double *x, double *y);
 /* GMRES for n_rhs plane-wave incidences (angles as in bem_operator_solve);
  * the solution carries traces, far field and per-incidence diagnostics
  * (method 2, iterations, relative_residual = ||b - A x|| / ||b||). */

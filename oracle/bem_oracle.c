@@ -425,6 +425,59 @@ int orc_assemble(const orc_mesh *m, int kind, double k, double g, double h, doub
     return 0;
 }
 
+/* Selected rows of the same operators, for checks at sizes whose full matrix
+ * is out of reach (a 200k-panel surface): rows[r] indexes the n x n (or
+ * 2N x 2N fluid) matrix of orc_assemble; out is row-major, nr x ncols
+ * complex.                                                                   */
+int orc_assemble_rows(const orc_mesh *m, int kind, double k, double g, double h, int64_t nr,
+                      const int64_t *rows, double *outd) {
+    init_rules();
+    if (!(k > 0) || !isfinite(k)) return 1;
+    int64_t n = m->nt;
+    int fluid = (kind == ORC_FLUID_MULLER || kind == ORC_FLUID_CBIE);
+    int64_t ncols = fluid ? 2 * n : n;
+    for (int64_t r = 0; r < nr; ++r)
+        if (rows[r] < 0 || rows[r] >= ncols) return 2;
+    zc *out = (zc *)outd;
+    zc alpha = I / k;
+    double kint = (h > 0) ? k / h : 0.0;
+#pragma omp parallel for schedule(dynamic, 64) collapse(2)
+    for (int64_t r = 0; r < nr; ++r) {
+        for (int64_t j = 0; j < n; ++j) {
+            int64_t row = rows[r];
+            int64_t i = fluid ? row % n : row;
+            zc e[4];
+            pair_ops(m, i, j, k, e);
+            double dij = (i == j) ? 1.0 : 0.0;
+            zc *o = out + r * ncols;
+            switch (kind) {
+            case ORC_RIGID_BM: o[j] = 0.5 * dij - e[1] - alpha * e[3]; break;
+            case ORC_SOFT_BM: o[j] = e[0] + alpha * (0.5 * dij + e[2]); break;
+            case ORC_RIGID_CBIE: o[j] = 0.5 * dij - e[1]; break;
+            case ORC_SOFT_CBIE: case ORC_OPERATOR_S: o[j] = e[0]; break;
+            case ORC_OPERATOR_D: o[j] = e[1]; break;
+            case ORC_OPERATOR_KP: o[j] = e[2]; break;
+            case ORC_OPERATOR_H: o[j] = e[3]; break;
+            case ORC_FLUID_MULLER:
+            case ORC_FLUID_CBIE: {
+                zc f[4];
+                pair_ops(m, i, j, kint, f);
+                int top = row < n;
+                if (kind == ORC_FLUID_MULLER) {
+                    o[j] = top ? dij - e[1] + f[1] : -e[3] + f[3] / g;
+                    o[j + n] = top ? e[0] - g * f[0] : dij + e[2] - f[2];
+                } else {
+                    o[j] = top ? 0.5 * dij - e[1] : 0.5 * dij + f[1];
+                    o[j + n] = top ? e[0] : -g * f[0];
+                }
+            } break;
+            default: break;
+            }
+        }
+    }
+    return 0;
+}
+
 /* Polar angle from +x, azimuth from +y toward +z (full.jl:15-18). */
 void orc_direction(double beta, double alpha, double *d) {
     d[0] = cos(beta);

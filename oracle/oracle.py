@@ -52,6 +52,8 @@ def lib():
                                          ctypes.c_double, ctypes.c_int, _dp]
         L.orc_assemble.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int64]
+        L.orc_assemble_rows.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_int64, _ip, _dp]
         L.orc_incident.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _dp, _dp]
         L.orc_far_field.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, _dp,
                                     _dp, _dp, _dp]
@@ -124,6 +126,17 @@ class Mesh:
         if rc != 0:
             raise ValueError("k must be finite and positive")
         return A
+
+    def assemble_rows(self, kind, k, rows, g=1.0, h=1.0):
+        """Rows of the matrix assemble() would return (row-major, complex)."""
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        n = self.n * (2 if kind in (FLUID_MULLER, FLUID_CBIE) else 1)
+        out = np.empty((rows.size, n), dtype=np.complex128)
+        rc = lib().orc_assemble_rows(self._h, kind, k, g, h, rows.size, rows.ctypes.data_as(_ip),
+                                     out.ctypes.data_as(_dp))
+        if rc != 0:
+            raise ValueError("k must be finite and positive and rows inside the matrix")
+        return out
 
     def incident(self, k, d):
         d = np.ascontiguousarray(d, dtype=np.float64)

@@ -380,3 +380,19 @@ def test_argument_errors():
     for g, h in ((0.0, 1.0), (1.0, -1.0), (math.inf, 1.0), (1.0, math.nan)):
         with pytest.raises(ValueError):
             O.solve_fluid(m, g, h, 1.0)
+
+
+@pytest.mark.parametrize("kind", [O.RIGID_BM, O.SOFT_BM, O.RIGID_CBIE, O.SOFT_CBIE, O.FLUID_MULLER,
+                                  O.FLUID_CBIE, O.OP_S, O.OP_D, O.OP_KP, O.OP_H])
+def test_assemble_rows_equal_full_assembly(kind):
+    """The row sampler used by the large-surface checks returns exactly the
+    rows of the full assembly (self, near and regular pairs, both fluid
+    block rows): a swapped block or an offset row index fails."""
+    nodes, tri = meshes.spheroid(0.02, 0.01, 2)
+    m = O.Mesh(nodes, tri)
+    A = m.assemble(kind, 300.0, 1.2, 1.1)
+    rows = np.array([0, 7, m.n - 1, A.shape[0] - 1, A.shape[0] // 2 + 3])
+    R = m.assemble_rows(kind, 300.0, rows, 1.2, 1.1)
+    assert np.array_equal(R, A[rows, :])
+    with pytest.raises(ValueError):
+        m.assemble_rows(kind, 300.0, np.array([A.shape[0]]))
