@@ -47,10 +47,14 @@ struct BemError : std::runtime_error {
             throw BemError(BEM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
-const char* kClassNames[] = {"assemble", "near",      "lu_panel", "lu_gemm", "lu_trsm", "lu_swap",     "solve",
-                             "residual", "far_field", "h_apply",  "h_aca",   "h_ortho", "lu_gemm_fp64"};
-constexpr int kNumClasses = 13;
-enum { C_ASSEMBLE = 0, C_NEAR = 1, C_RESIDUAL = 7, C_FAR = 8, C_HAPPLY = 9, C_HACA = 10, C_HORTHO = 11 };
+// "h_apply" is reported as the sum of its two kernels' classes, "h_proj" and
+// "h_rows" (the event timers do not nest).
+const char* kClassNames[] = {"assemble", "near",      "lu_panel", "lu_gemm", "lu_trsm", "lu_swap",
+                             "solve",    "residual",  "far_field", "h_apply", "h_aca",  "h_ortho",
+                             "lu_gemm_fp64", "h_proj", "h_rows"};
+constexpr int kNumClasses = 15;
+enum { C_ASSEMBLE = 0, C_NEAR = 1, C_RESIDUAL = 7, C_FAR = 8, C_HAPPLY = 9, C_HACA = 10, C_HORTHO = 11,
+       C_HPROJ = 13, C_HROWS = 14 };
 
 struct ProfRec {
     int cls;
@@ -1451,6 +1455,8 @@ struct HOp {
     z32* uv = nullptr;
     int64_t* toff = nullptr;
     HTile* tiles = nullptr;
+    int64_t* vrow = nullptr;  // V row offsets of the stacked projection rows
+    int64_t v_entries = 0, lowrank_entries = 0;  // sum rank * n and sum rank * (m + n)
     z64* T = nullptr;
     HLeafs L;
     int max_rank = 0, unconverged = 0;
@@ -1716,18 +1722,39 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
         op.unconverged = unconv;
         op.mean_rank = nl ? rsum / nl : 0.0;
         op.lowrank_bytes = sizeof(z32) * (uint64_t)coff;
+        // Projection tiles: the low-rank blocks are grouped by column cluster
+        // (same col_lo and n), their V rows stacked in block order and their
+        // products T laid out in that order, so one tile of up to kHProjRows
+        // stacked rows reads its X columns once for several blocks.
+        std::map<std::pair<int, int>, std::vector<int>> groups;
+        for (int b = nd; b < nd + nl; ++b)
+            if (op.blk[b].rank > 0) groups[{op.blk[b].col_lo, op.blk[b].n}].push_back(b);
         std::vector<HTile> tiles;
-        std::vector<int64_t> toff(nd + nl, 0);
+        std::vector<int64_t> vrow, toff(nd + nl, 0);
         int64_t tsum = 0;
-        for (int b = nd; b < nd + nl; ++b) {
-            toff[b] = tsum;
-            tsum += (int64_t)op.blk[b].rank * kHRhs;
-            for (int l0 = 0; l0 < op.blk[b].rank; l0 += 16) tiles.push_back({b, l0});
+        for (const auto& g : groups) {
+            const int64_t g0 = tsum;
+            for (int b : g.second) {
+                const HBlk& x = op.blk[b];
+                toff[b] = tsum * kHRhs;
+                for (int l = 0; l < x.rank; ++l) vrow.push_back(x.off + (int64_t)x.rank * x.m + (int64_t)l * x.n);
+                tsum += x.rank;
+                op.v_entries += (int64_t)x.rank * x.n;
+                op.lowrank_entries += (int64_t)x.rank * (x.m + x.n);
+            }
+            for (int64_t r0 = g0; r0 < tsum; r0 += kHProjRows)
+                tiles.push_back({r0, (int)std::min<int64_t>(kHProjRows, tsum - r0), g.first.first, g.first.second});
         }
+        // most work first (columns x rows of the tile), so the long tiles do
+        // not trail the launch
+        std::stable_sort(tiles.begin(), tiles.end(), [](const HTile& x, const HTile& y) {
+            return (int64_t)x.n * x.rows > (int64_t)y.n * y.rows;
+        });
         op.ntiles = (int)tiles.size();
         op.tiles = hm_upload(h, tiles);
+        op.vrow = hm_upload(h, vrow);
         op.toff = hm_upload(h, toff);
-        op.T = (z64*)hm_alloc(h, sizeof(z64) * std::max<int64_t>(1, tsum));
+        op.T = (z64*)hm_alloc(h, sizeof(z64) * std::max<int64_t>(1, tsum * kHRhs));
     } catch (...) {
         hipStreamSynchronize(s);
         ctx->release(uv1);
@@ -1767,6 +1794,16 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
     op.L.dlist = hm_upload(h, dlst);
     op.L.lptr = hm_upload(h, lptr);
     op.L.llist = hm_upload(h, llst);
+    // launch order of the row kernel: leaves by decreasing K chunks
+    std::vector<int64_t> work(nleaf, 0);
+    for (int l = 0; l < nleaf; ++l) {
+        for (int b : dl[l]) work[l] += (op.blk[b].n + 31) / 32;
+        for (const HRef& r : ll[l]) work[l] += (op.blk[r.blk].rank + 31) / 32;
+    }
+    std::vector<int> order(nleaf);
+    for (int l = 0; l < nleaf; ++l) order[l] = l;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    op.L.order = hm_upload(h, order);
     HIPCHECK(hipStreamSynchronize(s));
 }
 
@@ -1803,9 +1840,16 @@ void hm_apply(bem_hmatrix h, HOp& op, const z64* X, int nr, z64* Y) {
     const double bytes = (double)h->dense_count * sizeof(z32) + (double)op.lowrank_bytes +
                          2.0 * (double)h->n * nr * sizeof(z64);
     const double flops = 8.0 * nr * ((double)h->dense_count + (double)op.lowrank_bytes / sizeof(z32));
-    prof_begin(&h->lane->pl, h->lane->s, C_HAPPLY);
-    launch_happly(h->lane->s, op.L, op.ntiles, op.tiles, op.dblk, op.dval, op.uv, op.toff, op.T, X, nr, Y);
-    prof_end(&h->lane->pl, h->lane->s, C_HAPPLY, flops, bytes);
+    // split of the work: projections read the V factors and X, the rows
+    // kernel the dense leaves, the U factors and T
+    const double pf = 8.0 * nr * (double)op.v_entries;
+    const double pb = sizeof(z32) * (double)op.v_entries + (double)h->n * nr * sizeof(z64);
+    prof_begin(&h->lane->pl, h->lane->s, C_HPROJ);
+    launch_hproj(h->lane->s, op.ntiles, op.tiles, op.vrow, op.uv, X, nr, op.T);
+    prof_end(&h->lane->pl, h->lane->s, C_HPROJ, pf, pb);
+    prof_begin(&h->lane->pl, h->lane->s, C_HROWS);
+    launch_hrows(h->lane->s, op.L, op.dblk, op.dval, op.uv, op.toff, op.T, X, nr, Y);
+    prof_end(&h->lane->pl, h->lane->s, C_HROWS, flops - pf, bytes - pb);
 }
 
 // A M^{-1} X (M^{-1} = block Jacobi or identity); Z is scratch
@@ -2065,6 +2109,14 @@ int bem_ctx_kernel_stats(bem_ctx ctx, const char* name, int64_t* launches, doubl
     if (!ctx || !name) return BEM_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     ctx->prof.drain();
+    if (std::strcmp(name, "h_apply") == 0) {
+        const Prof& P = ctx->prof;
+        if (launches) *launches = P.launches[C_HROWS];
+        if (total_ms) *total_ms = P.ms[C_HPROJ] + P.ms[C_HROWS];
+        if (flops) *flops = P.flops[C_HPROJ] + P.flops[C_HROWS];
+        if (bytes) *bytes = P.bytes[C_HPROJ] + P.bytes[C_HROWS];
+        return BEM_OK;
+    }
     for (int c = 0; c < kNumClasses; ++c)
         if (std::strcmp(name, kClassNames[c]) == 0) {
             if (launches) *launches = ctx->prof.launches[c];

@@ -6,9 +6,14 @@ namespace bem {
 
 constexpr int kHRhs = 64;  // right-hand sides per batched apply (stride of T)
 
-// Tile of the low-rank projection: ranks [l0, l0 + 16) of block `blk`.
+// Tile of the low-rank projection: up to kHProjRows consecutive rows of the
+// stacked V factors of low-rank blocks that share one column cluster
+// [col_lo, col_lo + n).  Row s of the tile is V row vrow[row0 + s] (offset into
+// the factor store) and its product lands in T row row0 + s (stride kHRhs).
+constexpr int kHProjRows = 64;
 struct HTile {
-    int blk, l0;
+    int64_t row0;
+    int rows, col_lo, n;
 };
 // A low-rank block covering a leaf row cluster, at row offset o in the block.
 struct HRef {
@@ -25,12 +30,17 @@ struct HLeafs {
     const int* dlist = nullptr;
     const int* lptr = nullptr;
     const HRef* llist = nullptr;
+    const int* order = nullptr;  // leaves by decreasing apply work (launch order)
 };
 
-// Y = A X for nr <= kHRhs right-hand sides (tree-position order); T is scratch
-// of kHRhs * (sum of ranks) complex doubles at offsets toff.
-void launch_happly(hipStream_t s, const HLeafs& L, int ntiles, const HTile* tiles, const HBlk* blk, const z32* dval,
-                   const z32* uv, const int64_t* toff, z64* T, const z64* X, int nr, z64* Y);
+// Y = A X for nr <= kHRhs right-hand sides (tree-position order) in two
+// launches: the projections T = V X of the low-rank blocks (T: kHRhs complex
+// doubles per stacked rank row), then per leaf row the dense blocks times X
+// plus U T of the low-rank blocks covering it.
+void launch_hproj(hipStream_t s, int ntiles, const HTile* tiles, const int64_t* vrow, const z32* uv, const z64* X,
+                  int nr, z64* T);
+void launch_hrows(hipStream_t s, const HLeafs& L, const HBlk* blk, const z32* dval, const z32* uv,
+                  const int64_t* toff, const z64* T, const z64* X, int nr, z64* Y);
 void launch_hjacobi_inv(hipStream_t s, const HLeafs& L, const HBlk* blk, const z32* dval, z64* minv, int* status);
 void launch_hjacobi_apply(hipStream_t s, const HLeafs& L, const z64* minv, const z64* X, int nr, z64* Y);
 

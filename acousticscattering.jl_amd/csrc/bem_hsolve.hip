@@ -10,6 +10,8 @@
 // block, then (2) for every leaf row cluster, the sum of its dense blocks
 // times X and of U_b (its rows) times T_b for the low-rank blocks covering it,
 // in a fixed order (no atomics: the result is deterministic).
+#include <algorithm>
+
 #include "bem_device.h"
 #include "bem_hsolve.h"
 
@@ -26,130 +28,260 @@ typedef double dv4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ dv4 mfma64(double a, double b, dv4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
-// acc += a * b for complex a (A operand) and b (B operand), four real products
-__device__ __forceinline__ void cmfma64(double ar, double ai, double br, double bi, dv4& cr, dv4& ci) {
-    cr = mfma64(ar, br, cr);
-    cr = mfma64(-ai, bi, cr);
-    ci = mfma64(ar, bi, ci);
-    ci = mfma64(ai, br, ci);
+// Both kernels stage K chunks of KC columns of their two operands in LDS
+// (block values as stored, complex FP32; vectors complex FP64) and keep the
+// next chunk's loads in flight in registers while the matrix cores work on
+// the current one.  A workgroup is 4 waves; wave w owns right-hand-side
+// columns 16w..16w+15 and up to four 16-row tiles of the output, so each wave
+// holds up to 8 independent accumulators and issues the real and imaginary
+// products of consecutive MFMAs into different ones.
+//
+// LDS strides (64 banks of 4 bytes): a block-operand row is 34 z32 = 68 words,
+// so the 16 lanes of one 128-bit store pass (4 rows x 4 quarter rows) and the
+// 32 lanes of one 64-bit read pass (16 rows x 2 k) touch distinct banks; a
+// vector row is 72 z64 = 288 words (= 32 mod 64), so the 16 lanes of a
+// 128-bit pass (8 columns x 2 rows, or 16 columns of one row) do too.
+template <int KC>
+struct HStage {
+    static constexpr int kAPad = KC + 2;  // block-operand row stride (z32)
+    static constexpr int kBPad = 72;      // vector row stride (z64)
+    z32 sA[64][kAPad];  // sA[row][k]
+    z64 sB[KC][kBPad];  // sB[k][rhs]
+};
+
+// Registers of one thread's share of a chunk of KC columns (64 x KC block
+// values and KC x 64 vector values over 256 threads).
+template <int KC>
+struct HChunkRegs {
+    z32 a[KC / 4];
+    z64 b[KC / 4];
+};
+
+// Vector operand rows r0 + kb0 + t / (256 / KC), columns t % (256 / KC) +
+// (256 / KC) u; rows at or beyond kmax and columns at or beyond nr are zero.
+template <int KC>
+__device__ __forceinline__ void load_vec(const z64* __restrict__ V, int64_t r0, int stride, int kb0, int kmax,
+                                         int nr, int t, z64 (&b)[KC / 4]) {
+    constexpr int TPR = 256 / KC;
+    const int kb = t / TPR, cb = t % TPR;
+    const bool ok = kb0 + kb < kmax;
+    const z64* src = V + (r0 + kb0 + kb) * (int64_t)stride + cb;
+#pragma unroll
+    for (int u = 0; u < KC / 4; ++u) b[u] = (ok && cb + TPR * u < nr) ? src[TPR * u] : z64{0.0, 0.0};
+}
+template <int KC>
+__device__ __forceinline__ void store_vec(HStage<KC>& S, int t, const z64 (&b)[KC / 4]) {
+    constexpr int TPR = 256 / KC;
+    const int kb = t / TPR, cb = t % TPR;
+#pragma unroll
+    for (int u = 0; u < KC / 4; ++u) S.sB[kb][cb + TPR * u] = b[u];
+}
+// Row-major block operand row (a dense leaf row, or one stacked V row) at
+// src: columns k0 + (t % 4) * KC / 4 .. + KC / 4 (< kmax) of row t / 4.
+template <int KC>
+__device__ __forceinline__ void load_rowmajor(const z32* __restrict__ src, bool row_ok, int k0, int kmax, int t,
+                                              z32 (&a)[KC / 4]) {
+    const int j = (t & 3) * (KC / 4);
+#pragma unroll
+    for (int u = 0; u < KC / 4; ++u) a[u] = (row_ok && k0 + j + u < kmax) ? src[k0 + j + u] : z32{0.f, 0.f};
+}
+template <int KC>
+__device__ __forceinline__ void store_rowmajor(HStage<KC>& S, int t, const z32 (&a)[KC / 4]) {
+    const int i = t >> 2, j = (t & 3) * (KC / 4);
+#pragma unroll
+    for (int u = 0; u < KC / 4; ++u) S.sA[i][j + u] = a[u];
+}
+// U factor rows of a leaf (U[l][o + i] = base[l * ld + o + i]): row t % 64
+// (< rows), ranks k0 + (t / 64) * KC / 4 .. + KC / 4 (< kmax); stored
+// transposed.
+template <int KC>
+__device__ __forceinline__ void load_colmajor(const z32* __restrict__ base, int ld, int rows, int k0, int kmax,
+                                              int t, z32 (&a)[KC / 4]) {
+    const int i = t & 63, l = (t >> 6) * (KC / 4);
+    const z32* src = base + (int64_t)(k0 + l) * ld + i;
+#pragma unroll
+    for (int u = 0; u < KC / 4; ++u)
+        a[u] = (i < rows && k0 + l + u < kmax) ? src[(int64_t)u * ld] : z32{0.f, 0.f};
+}
+template <int KC>
+__device__ __forceinline__ void store_colmajor(HStage<KC>& S, int t, const z32 (&a)[KC / 4]) {
+    const int i = t & 63, l = (t >> 6) * (KC / 4);
+#pragma unroll
+    for (int u = 0; u < KC / 4; ++u) S.sA[i][l + u] = a[u];
 }
 
-// (1) T_b[l][c] = sum_j V_b[l][j] X[(col_lo + j) nr + c]: 16 ranks x 64
-// right-hand sides per workgroup; wave w owns columns 16w..16w+15.
-__global__ __launch_bounds__(256) void k_hlr_t_mfma(const HBlk* __restrict__ blk, const z32* __restrict__ uv,
-                                                    const HTile* __restrict__ tiles,
-                                                    const int64_t* __restrict__ toff, const z64* __restrict__ X,
-                                                    int nr, z64* __restrict__ T) {
-    const HTile t = tiles[blockIdx.x];
-    const HBlk b = blk[t.blk];
-    const z32* V = uv + b.off + (int64_t)b.rank * b.m;
+// acc[rt] += sA[16 rt + i][k] * sB[k][16 w + j] over one staged chunk, for
+// the row tiles rt < NT.
+template <int NT, int KC>
+__device__ __forceinline__ void chunk_products(const HStage<KC>& S, int w, int li, int lk, dv4 (&cr)[NT],
+                                               dv4 (&ci)[NT]) {
+#pragma unroll 2
+    for (int kk = 0; kk < KC; kk += 4) {
+        const int k = kk + lk;
+        const z64 b = S.sB[k][16 * w + li];
+        double ar[NT], ai[NT];
+#pragma unroll
+        for (int rt = 0; rt < NT; ++rt) {
+            const z32 a = S.sA[16 * rt + li][k];
+            ar[rt] = a.x;
+            ai[rt] = a.y;
+        }
+#pragma unroll
+        for (int rt = 0; rt < NT; ++rt) {
+            cr[rt] = mfma64(ar[rt], b.x, cr[rt]);
+            ci[rt] = mfma64(ar[rt], b.y, ci[rt]);
+        }
+#pragma unroll
+        for (int rt = 0; rt < NT; ++rt) {
+            cr[rt] = mfma64(-ai[rt], b.y, cr[rt]);
+            ci[rt] = mfma64(ai[rt], b.x, ci[rt]);
+        }
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void store_tiles(const dv4 (&cr)[NT], const dv4 (&ci)[NT], z64* __restrict__ out,
+                                            int64_t row_stride, int rows, int nr, int w, int li, int lk) {
+    const int c = 16 * w + li;
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * rt + lk + 4 * r;
+            if (i < rows && c < nr) out[(int64_t)i * row_stride + c] = z64{cr[rt][r], ci[rt][r]};
+        }
+}
+
+// (1) Projection T = V X_cols for the stacked V rows of one tile (up to 64
+// rows of low-rank blocks that share a column cluster): one X chunk in LDS
+// serves every stacked row, so X is read once per tile rather than once per
+// block.  (A variant in which each workgroup streams several tiles, with the
+// next tile's first chunk in flight across the boundary, needed 184 VGPRs,
+// two workgroups per CU instead of three, and ran 8% slower.)
+constexpr int kProjKC = 32;  // K chunk of the projection
+
+template <int NT>
+__device__ void hproj_tile(const HTile& tl, const int64_t* __restrict__ vrow, const z32* __restrict__ uv,
+                           const z64* __restrict__ X, int nr, z64* __restrict__ T, HStage<kProjKC>& S) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int lcnt = min(16, b.rank - t.l0);
-    const int c = 16 * w + li;  // B-operand column of this lane
-    __shared__ z32 sV[16][68];
-    dv4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-    for (int j0 = 0; j0 < b.n; j0 += 64) {
-        __syncthreads();
-        for (int e = tid; e < 16 * 64; e += 256) {
-            const int l = e >> 6, jj = e & 63;
-            sV[l][jj] = (l < lcnt && j0 + jj < b.n) ? V[(int64_t)(t.l0 + l) * b.n + j0 + jj] : z32{0.f, 0.f};
-        }
-        __syncthreads();
-        const int jn = min(64, b.n - j0);
-        for (int k0 = 0; k0 < jn; k0 += 4) {
-            const int k = k0 + lk;
-            const z32 a = sV[li][k];  // A[i = rank][k = column]; zero beyond jn (staged zeros)
-            z64 x{0, 0};
-            if (k < jn && c < nr) x = X[(int64_t)(b.col_lo + j0 + k) * nr + c];
-            cmfma64(a.x, a.y, x.x, x.y, cr, ci);
-        }
-    }
+    const int ia = tid >> 2;
+    const bool row_ok = ia < tl.rows;
+    const z32* vsrc = row_ok ? uv + vrow[tl.row0 + ia] : uv;
+    dv4 cr[NT], ci[NT];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int i = lk + 4 * r;
-        if (i < lcnt && c < nr) T[toff[t.blk] + (int64_t)(t.l0 + i) * kHRhs + c] = z64{cr[r], ci[r]};
+    for (int q = 0; q < NT; ++q) cr[q] = ci[q] = dv4{0, 0, 0, 0};
+    HChunkRegs<kProjKC> R;
+    load_rowmajor<kProjKC>(vsrc, row_ok, 0, tl.n, tid, R.a);
+    load_vec<kProjKC>(X, tl.col_lo, nr, 0, tl.n, nr, tid, R.b);
+    for (int k0 = 0; k0 < tl.n; k0 += kProjKC) {
+        __syncthreads();
+        store_rowmajor(S, tid, R.a);
+        store_vec(S, tid, R.b);
+        __syncthreads();
+        if (k0 + kProjKC < tl.n) {
+            load_rowmajor<kProjKC>(vsrc, row_ok, k0 + kProjKC, tl.n, tid, R.a);
+            load_vec<kProjKC>(X, tl.col_lo, nr, k0 + kProjKC, tl.n, nr, tid, R.b);
+        }
+        chunk_products<NT, kProjKC>(S, w, li, lk, cr, ci);
+    }
+    store_tiles<NT>(cr, ci, T + tl.row0 * kHRhs, kHRhs, tl.rows, nr, w, li, lk);
+}
+
+__global__ __launch_bounds__(256) void k_hproj(const HTile* __restrict__ tiles, const int64_t* __restrict__ vrow,
+                                               const z32* __restrict__ uv, const z64* __restrict__ X, int nr,
+                                               z64* __restrict__ T) {
+    const HTile tl = tiles[blockIdx.x];
+    __shared__ HStage<kProjKC> S;
+    switch ((tl.rows + 15) >> 4) {
+    case 1: hproj_tile<1>(tl, vrow, uv, X, nr, T, S); break;
+    case 2: hproj_tile<2>(tl, vrow, uv, X, nr, T, S); break;
+    case 3: hproj_tile<3>(tl, vrow, uv, X, nr, T, S); break;
+    default: hproj_tile<4>(tl, vrow, uv, X, nr, T, S); break;
     }
 }
 
-// (2) one workgroup per leaf row cluster (dense blocks times X, then U_b T_b
-// for the low-rank blocks covering it); wave w owns rows 16w..16w+15, all 64
-// columns.
-__global__ __launch_bounds__(256) void k_hrows_mfma(const HLeafs L, const HBlk* __restrict__ blk,
-                                                    const z32* __restrict__ dval, const z32* __restrict__ uv,
-                                                    const int64_t* __restrict__ toff, const z64* __restrict__ T,
-                                                    const z64* __restrict__ X, int nr, z64* __restrict__ Y) {
-    const int leaf = blockIdx.x;
+// (2) one workgroup per leaf row cluster (leaves in `order`, most work
+// first): its dense blocks times X, then U_b T_b for the low-rank blocks
+// covering it, chunk after chunk in list order (fixed summation order).
+constexpr int kRowsKC = 16;  // K chunk of the row kernel
+
+template <int NT>
+__device__ void hrows_leaf(const HLeafs& L, int leaf, const HBlk* __restrict__ blk, const z32* __restrict__ dval,
+                           const z32* __restrict__ uv, const int64_t* __restrict__ toff, const z64* __restrict__ T,
+                           const z64* __restrict__ X, int nr, z64* __restrict__ Y, HStage<kRowsKC>& S) {
     const int lo = L.lo[leaf], m = L.m[leaf];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int row = 16 * w + li;  // A-operand row of this lane
-    __shared__ z32 sD[64][68];
-    dv4 cr[4], ci[4];
+    const int d0 = L.dptr[leaf], nd = L.dptr[leaf + 1] - d0, l0 = L.lptr[leaf];
+    const int nitems = nd + (L.lptr[leaf + 1] - l0);
+    dv4 cr[NT], ci[NT];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) cr[q] = ci[q] = dv4{0, 0, 0, 0};
-    for (int e = L.dptr[leaf]; e < L.dptr[leaf + 1]; ++e) {
-        const HBlk b = blk[L.dlist[e]];
+    for (int q = 0; q < NT; ++q) cr[q] = ci[q] = dv4{0, 0, 0, 0};
+    // item e < nd: dense block dlist[d0 + e]; else low-rank ref llist[l0 + e - nd]
+    auto kdim = [&](int e) { return e < nd ? blk[L.dlist[d0 + e]].n : blk[L.llist[l0 + e - nd].blk].rank; };
+    auto load = [&](int e, int k0, HChunkRegs<kRowsKC>& R) {
+        if (e < nd) {
+            const HBlk b = blk[L.dlist[d0 + e]];
+            const int ia = tid >> 2;
+            load_rowmajor<kRowsKC>(dval + b.off + (int64_t)ia * b.n, ia < m, k0, b.n, tid, R.a);
+            load_vec<kRowsKC>(X, b.col_lo, nr, k0, b.n, nr, tid, R.b);
+        } else {
+            const HRef ref = L.llist[l0 + e - nd];
+            const HBlk b = blk[ref.blk];
+            load_colmajor<kRowsKC>(uv + b.off + ref.o, b.m, m, k0, b.rank, tid, R.a);
+            load_vec<kRowsKC>(T + toff[ref.blk], 0, kHRhs, k0, b.rank, nr, tid, R.b);
+        }
+    };
+    int e = 0, k0 = 0, kmax = 0;
+    while (e < nitems && (kmax = kdim(e)) <= 0) ++e;
+    HChunkRegs<kRowsKC> R;
+    if (e < nitems) load(e, 0, R);
+    while (e < nitems) {
+        const bool dense = e < nd;
         __syncthreads();
-        for (int q = tid; q < 64 * 64; q += 256) {
-            const int i = q >> 6, j = q & 63;
-            sD[i][j] = (i < b.m && j < b.n) ? dval[b.off + i * b.n + j] : z32{0.f, 0.f};
-        }
+        if (dense) store_rowmajor(S, tid, R.a);
+        else store_colmajor(S, tid, R.a);
+        store_vec(S, tid, R.b);
         __syncthreads();
-        for (int k0 = 0; k0 < b.n; k0 += 4) {
-            const int k = k0 + lk;
-            const z32 a = sD[row][k];
-            const z64* xr = X + (int64_t)(b.col_lo + k) * nr;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int cc = 16 * q + li;
-                z64 x{0, 0};
-                if (k < b.n && cc < nr) x = xr[cc];
-                cmfma64(a.x, a.y, x.x, x.y, cr[q], ci[q]);
-            }
+        // next chunk: the rest of this item, else the next item with work
+        k0 += kRowsKC;
+        if (k0 >= kmax) {
+            k0 = 0;
+            ++e;
+            while (e < nitems && (kmax = kdim(e)) <= 0) ++e;
         }
+        if (e < nitems) load(e, k0, R);
+        chunk_products<NT, kRowsKC>(S, w, li, lk, cr, ci);
     }
-    for (int e = L.lptr[leaf]; e < L.lptr[leaf + 1]; ++e) {
-        const HRef ref = L.llist[e];
-        const HBlk b = blk[ref.blk];
-        const z32* U = uv + b.off;
-        const z64* Tb = T + toff[ref.blk];
-        for (int l0 = 0; l0 < b.rank; l0 += 64) {
-            const int lc = min(64, b.rank - l0);
-            __syncthreads();
-            for (int q = tid; q < 64 * 64; q += 256) {
-                const int l = q >> 6, i = q & 63;  // stored transposed: sD[i][l] = U[o + i][l0 + l]
-                sD[i][l] = (l < lc && i < m) ? U[(int64_t)(l0 + l) * b.m + ref.o + i] : z32{0.f, 0.f};
-            }
-            __syncthreads();
-            for (int k0 = 0; k0 < lc; k0 += 4) {
-                const int k = k0 + lk;
-                const z32 a = sD[row][k];
-                const z64* tr = Tb + (int64_t)(l0 + k) * kHRhs;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int cc = 16 * q + li;
-                    z64 x{0, 0};
-                    if (k < lc && cc < nr) x = tr[cc];
-                    cmfma64(a.x, a.y, x.x, x.y, cr[q], ci[q]);
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = 16 * w + lk + 4 * r, cc = 16 * q + li;
-            if (i < m && cc < nr) Y[(int64_t)(lo + i) * nr + cc] = z64{cr[q][r], ci[q][r]};
-        }
+    store_tiles<NT>(cr, ci, Y + (int64_t)lo * nr, nr, m, nr, w, li, lk);
 }
 
-void launch_happly(hipStream_t s, const HLeafs& L, int ntiles, const HTile* tiles, const HBlk* blk, const z32* dval,
-                   const z32* uv, const int64_t* toff, z64* T, const z64* X, int nr, z64* Y) {
-    if (ntiles > 0)
-        hipLaunchKernelGGL(k_hlr_t_mfma, dim3(ntiles), dim3(256), 0, s, blk, uv, tiles, toff, X, nr, T);
-    hipLaunchKernelGGL(k_hrows_mfma, dim3(L.count), dim3(256), 0, s, L, blk, dval, uv, toff, T, X, nr, Y);
+// Four waves per SIMD (four workgroups per CU): at a 16-wide K chunk the
+// staging registers fit 128 VGPRs, and the extra resident workgroups hide
+// the chunk loads better than a 32-wide chunk at three per CU (7.3 against
+// 8.3 ms per 64-RHS apply on the 200k-panel bent cylinder).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_hrows(const HLeafs L, const HBlk* __restrict__ blk,
+                                               const z32* __restrict__ dval, const z32* __restrict__ uv,
+                                               const int64_t* __restrict__ toff, const z64* __restrict__ T,
+                                               const z64* __restrict__ X, int nr, z64* __restrict__ Y) {
+    const int leaf = L.order[blockIdx.x];
+    __shared__ HStage<kRowsKC> S;
+    switch ((L.m[leaf] + 15) >> 4) {
+    case 1: hrows_leaf<1>(L, leaf, blk, dval, uv, toff, T, X, nr, Y, S); break;
+    case 2: hrows_leaf<2>(L, leaf, blk, dval, uv, toff, T, X, nr, Y, S); break;
+    case 3: hrows_leaf<3>(L, leaf, blk, dval, uv, toff, T, X, nr, Y, S); break;
+    default: hrows_leaf<4>(L, leaf, blk, dval, uv, toff, T, X, nr, Y, S); break;
+    }
+}
+
+void launch_hproj(hipStream_t s, int ntiles, const HTile* tiles, const int64_t* vrow, const z32* uv, const z64* X,
+                  int nr, z64* T) {
+    if (ntiles > 0) hipLaunchKernelGGL(k_hproj, dim3(ntiles), dim3(256), 0, s, tiles, vrow, uv, X, nr, T);
+}
+void launch_hrows(hipStream_t s, const HLeafs& L, const HBlk* blk, const z32* dval, const z32* uv,
+                  const int64_t* toff, const z64* T, const z64* X, int nr, z64* Y) {
+    hipLaunchKernelGGL(k_hrows, dim3(L.count), dim3(256), 0, s, L, blk, dval, uv, toff, T, X, nr, Y);
 }
 
 // ---------------------------------------------------------------------------

@@ -21,9 +21,9 @@ time (`--heavy-lanes`, the library's default) while the other lanes take the
 smaller ones: the sweep's throughput is within 2 % of letting them run
 together, at a third of the median and half of the tail latency.
 
-Time budget (`--budget-s`, default 400 s for the whole process): after
-`--min-steps` timed sweeps (3), further warm-up and timed sweeps run only
-while the rest of the run (the solo heavy solve, the FP64 reference sweep at
+Time budget (`--budget-s`, default 400 s for the whole process): at most one
+warm-up sweep runs; after `--min-steps` timed sweeps (3), further timed
+sweeps run only while the rest of the run (the solo heavy solve, the FP64 reference sweep at
 about 2.1x a mixed sweep, and the CPU leg) still fits.  `steps` and `warmup`
 in the JSON are the counts actually run, beside the requested ones.
 
@@ -371,7 +371,7 @@ def hmatrix_main(args):
     info = H.info()
     diags = [sol.diagnostics(r) for r in range(len(angles))]
     ts = [bh.target_strength(sol.far_field(r)[0]) for r in range(len(angles))]
-    stats = {name: ctx.kernel_stats(name) for name in ("h_apply", "h_aca", "h_ortho")}
+    stats = {name: ctx.kernel_stats(name) for name in ("h_apply", "h_proj", "h_rows", "h_aca", "h_ortho")}
     ap = stats["h_apply"]
     gbs = ap["bytes"] / (ap["ms"] * 1e-3) / 1e9 if ap["ms"] > 0 else 0.0
     tfs = ap["flops"] / (ap["ms"] * 1e-3) / 1e12 if ap["ms"] > 0 else 0.0
@@ -398,7 +398,13 @@ def hmatrix_main(args):
                      "peak": MFMA_FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": tfs * 1e12 / MFMA_FP64_PEAK,
                      "traffic": None, "hbm_gbs": gbs, "hbm_frac": gbs / 8000.0,
                      "flops_per_byte": ap["flops"] / ap["bytes"] if ap["bytes"] > 0 else None,
-                     "launches": ap["launches"], "avg_launch_ms": ap["ms"] / max(1, ap["launches"])},
+                     "launches": ap["launches"], "avg_launch_ms": ap["ms"] / max(1, ap["launches"]),
+                     "kernels": {kn: {"avg_launch_ms": stats[kn]["ms"] / max(1, stats[kn]["launches"]),
+                                      "tflops": (stats[kn]["flops"] / (stats[kn]["ms"] * 1e-3) / 1e12
+                                                 if stats[kn]["ms"] > 0 else 0.0),
+                                      "frac": (stats[kn]["flops"] / (stats[kn]["ms"] * 1e-3) / MFMA_FP64_PEAK
+                                               if stats[kn]["ms"] > 0 else 0.0)}
+                                 for kn in ("h_proj", "h_rows")}},
         "kernels": {kname: {"launches": v["launches"], "ms": round(v["ms"], 3)} for kname, v in stats.items()},
         "ts_db_min_max": [min(ts), max(ts)],
         "cpu_baseline": None,
@@ -508,8 +514,11 @@ def sweep_main(args):
     def fits(n_sweeps):
         return elapsed() + n_sweeps * est[0] + reserve() <= args.budget_s
 
+    # One warm-up sweep at most: more measured no difference (the arenas are
+    # sized and every kernel loaded before it), and the budget goes to timed
+    # sweeps instead.
     warm_done = 0
-    for w in range(args.warmup):
+    for w in range(min(args.warmup, 1)):
         if not fits(1 + args.min_steps):
             break
         log(f"warmup {w + 1}/{args.warmup} ...")
