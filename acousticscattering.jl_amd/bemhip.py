@@ -48,7 +48,7 @@ EXPORTED = [
     "bem_server_submit", "bem_server_poll", "bem_server_wait", "bem_server_error",
     "bem_server_stats_get", "bem_hmatrix_defaults", "bem_hmatrix_plan", "bem_hmatrix_create",
     "bem_hmatrix_destroy", "bem_hmatrix_get_info", "bem_hmatrix_matvec", "bem_hmatrix_apply", "bem_hmatrix_solve",
-    "bem_debug_lu",
+    "bem_debug_lu", "bem_far_field_noalloc",
 ]
 
 
@@ -162,6 +162,7 @@ def load_library(path=LIB_PATH):
         "bem_solution_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), _ip]),
         "bem_solution_traces": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
         "bem_far_field": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _dp, _dp]),
+        "bem_far_field_noalloc": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _dp, _dp]),
         "bem_solution_diagnostics": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(Diagnostics)]),
         "bem_sweep": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(_vp), ctypes.POINTER(Problem),
                                      _dp, _dp, _dp, ctypes.POINTER(Diagnostics)]),

@@ -226,6 +226,7 @@ struct bem_ctx_s {
     std::vector<Lane*> lanes;  // lanes[0] serves the single-solve entry points
     int max_lanes = 8;         // concurrent solves of bem_sweep
     int max_heavy = 1;         // of which heavy ones (kHeavyUnknowns)
+    void* scratch = nullptr;   // this device's address of the fixed scratch array (bem_far_field_noalloc)
 
     Lane* lane0() { return lanes[0]; }
     // Device memory: a caching allocator.  Released blocks go to a free pool
@@ -2024,6 +2025,40 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
     for (void* p : tmp) h->ctx->release(p);
 }
 
+// ---------------------------------------------------------------------------
+// Fixed device scratch of bem_far_field_noalloc: the staged directions, the
+// per-block partial sums and the amplitudes of at most kNoAllocMaxDirs
+// directions, laid out at fixed offsets.  One copy per device lives for the
+// life of the process.  In host code the array's name is only a stand-in:
+// each context resolves its device's address once, at creation, with
+// hipGetSymbolAddress and passes that pointer to the kernels.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocMaxDirs = 1024;
+constexpr size_t kScratchDirsOff = 0;                                              // 3 doubles per direction
+constexpr size_t kScratchPartOff = 32 * 1024;                                      // kFarBlocks z64 per direction
+constexpr size_t kScratchResOff = kScratchPartOff + sizeof(z64) * kFarBlocks * kNoAllocMaxDirs;
+constexpr size_t kScratchBytes = kScratchResOff + sizeof(z64) * kNoAllocMaxDirs;  // 4.05 MiB
+static_assert(sizeof(double) * 3 * kNoAllocMaxDirs <= kScratchPartOff, "direction staging overlaps partials");
+static_assert(kScratchResOff % 256 == 0, "amplitudes must stay aligned");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// This device's address of `scratch` (the calling thread has the device
+// current); resolved once per device and kept.
+void* device_scratch(int device) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw BemError(BEM_ERR_DEVICE, std::string("scratch symbol address: ") + hipGetErrorString(e));
+    addr[device] = ptr;
+    return ptr;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2066,6 +2101,7 @@ int bem_ctx_create(int32_t device, uint64_t budget, bem_ctx* out) {
         HIPCHECK(hipMemGetInfo(&fr, &tot));
         c->budget = budget ? budget : (uint64_t)(0.9 * (double)fr);
         if (upload_rules() != 0) throw BemError(BEM_ERR_DEVICE, "constant table upload failed");
+        c->scratch = device_scratch(device);
         c->lanes.push_back(make_lane(c));
         HIPCHECK(hipDeviceSynchronize());
     });
@@ -2481,6 +2517,51 @@ int bem_far_field(bem_solution sol, int32_t r, int64_t ndir, const double* dirs,
             throw;
         }
         ctx->release(dd); ctx->release(part); ctx->release(res);
+    });
+}
+
+int bem_far_field_noalloc(bem_solution sol, int32_t r, int64_t ndir, const double* dirs, double* out) {
+    if (!sol || !out || r < 0 || r >= sol->nrhs || ndir <= 0) return BEM_ERR_ARGUMENT;
+    bem_ctx ctx = sol->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        if (ndir > kNoAllocMaxDirs)
+            throw BemError(BEM_ERR_ARGUMENT, "bem_far_field_noalloc: at most 1024 directions per call");
+        if (!ctx->scratch) throw BemError(BEM_ERR_STATE, "context has no scratch address");
+        hipStream_t s = ctx->lane0()->s;
+        unsigned char* base = (unsigned char*)ctx->scratch;
+        double* dd = (double*)(base + kScratchDirsOff);
+        z64* part = (z64*)(base + kScratchPartOff);
+        z64* res = (z64*)(base + kScratchResOff);
+        // Every element the kernels read is written earlier in this call
+        // (directions by the copy, partials by k_far_partial, amplitudes by
+        // k_far_final), so nothing left by a previous call is read.
+        int64_t nd = ndir;
+        if (dirs) {
+            for (int64_t d = 0; d < ndir; ++d) {
+                const double* x = dirs + 3 * d;
+                if (!(std::isfinite(x[0]) && std::isfinite(x[1]) && std::isfinite(x[2])))
+                    throw BemError(BEM_ERR_ARGUMENT, "direction must be finite");
+            }
+            HIPCHECK(hipMemcpyAsync(dd, dirs, sizeof(double) * 3 * ndir, hipMemcpyHostToDevice, s));
+        } else {
+            // backscatter: one direction, its amplitude repeated below
+            double d[3];
+            direction(sol->angles[2 * r], sol->angles[2 * r + 1], d);
+            for (double& c : d) c = -c;
+            nd = 1;
+            HIPCHECK(hipMemcpyAsync(dd, d, sizeof d, hipMemcpyHostToDevice, s));
+            HIPCHECK(hipStreamSynchronize(s));
+        }
+        launch_far_field(s, sol->mesh->d, sol->k, 1, (int)nd, dd, sol->p + (size_t)r * sol->nt,
+                         sol->q + (size_t)r * sol->nt, part, res);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(out, res, sizeof(z64) * nd, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        for (int64_t d = nd; d < ndir; ++d) {
+            out[2 * d] = out[0];
+            out[2 * d + 1] = out[1];
+        }
     });
 }
 

@@ -207,6 +207,15 @@ int bem_solution_traces(bem_solution sol, int32_t r, double *p_scat, double *dpd
  * means backscatter xhat = -d (src/solvers/bem/postprocessing.jl:4-6). */
 int bem_far_field(bem_solution sol, int32_t r, int64_t n_dir, const double *dirs, double *out);
 int bem_solution_diagnostics(bem_solution sol, int32_t r, bem_diagnostics *out);
+/* bem_far_field without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, hosts that forbid hipMalloc).
+ * Same arguments, results and kernels as bem_far_field, for at most 1024
+ * directions per call (more: BEM_ERR_ARGUMENT).  Its staging and partial sums
+ * live in a fixed 4.05 MiB device array per GPU, held until the process
+ * ends; calls that overlap on one GPU (from two contexts on the same device)
+ * race on that array, calls through one context are serialised. */
+int bem_far_field_noalloc(bem_solution sol, int32_t r, int64_t n_dir, const double *dirs, double *out);
 
 /* ---- batched frequency sweep (src/postprocessing/sweeps.jl:103-110) ----
  * Solves `n` independent problems (mesh_i, problem_i) at incidence
