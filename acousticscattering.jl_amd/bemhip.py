@@ -48,7 +48,7 @@ EXPORTED = [
     "bem_server_submit", "bem_server_poll", "bem_server_wait", "bem_server_error",
     "bem_server_stats_get", "bem_hmatrix_defaults", "bem_hmatrix_plan", "bem_hmatrix_create",
     "bem_hmatrix_destroy", "bem_hmatrix_get_info", "bem_hmatrix_matvec", "bem_hmatrix_apply", "bem_hmatrix_solve",
-    "bem_debug_lu", "bem_far_field_noalloc",
+    "bem_debug_lu", "bem_far_field_noalloc", "bem_pressure",
 ]
 
 
@@ -163,6 +163,7 @@ def load_library(path=LIB_PATH):
         "bem_solution_traces": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
         "bem_far_field": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _dp, _dp]),
         "bem_far_field_noalloc": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _dp, _dp]),
+        "bem_pressure": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _dp, ctypes.c_int32, _dp]),
         "bem_solution_diagnostics": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(Diagnostics)]),
         "bem_sweep": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(_vp), ctypes.POINTER(Problem),
                                      _dp, _dp, _dp, ctypes.POINTER(Diagnostics)]),
@@ -547,6 +548,24 @@ class _Solution:
         self.ctx.check(self.ctx.lib.bem_far_field(self._h, r, dirs.shape[0], _d(dirs), out.ctypes.data_as(_dp)))
         return out
 
+    def pressure(self, points, field="total", r=0):
+        """Complex pressure at points (3 x n, or a sequence of 3-vectors)."""
+        codes = {"total": 0, "scattered": 1, "incident": 2, "interior": 3}
+        if field not in codes:
+            raise ValueError("field must be :total, :scattered, :incident or :interior")
+        pts = np.asarray(points, dtype=np.float64)
+        if pts.ndim == 1:
+            pts = pts.reshape(3, 1)
+        elif pts.shape[0] != 3 and pts.shape[-1] == 3:
+            pts = pts.T
+        if pts.shape[0] != 3:
+            raise ValueError("points must have three coordinates")
+        P = np.ascontiguousarray(pts.T)
+        out = np.empty(P.shape[0], dtype=np.complex128)
+        self.ctx.check(self.ctx.lib.bem_pressure(self._h, r, P.shape[0], _d(P), codes[field],
+                                                 out.ctypes.data_as(_dp)))
+        return out
+
     def diagnostics(self, r=0):
         d = Diagnostics()
         self.ctx.check(self.ctx.lib.bem_solution_diagnostics(self._h, r, ctypes.byref(d)))
@@ -660,6 +679,23 @@ def target_strength(x, **kw):
     if isinstance(x, BEMSolution):
         x = scattering_amplitude(x, **kw)
     return 20.0 * math.log10(abs(x))
+
+
+def pressure(solution, points, *, field="total", region=None):
+    """pressure(solution, points; field) of a full-3D BEM solution
+    (src/fields/pressure.jl:1-20,404-472): one point (3-vector) returns a
+    complex number, a 3 x N array (points in columns) or a sequence of
+    3-vectors returns an array."""
+    if region is not None:
+        raise ValueError("region selection requires a coupled fluid BEM solution")
+    pts = np.asarray(points, dtype=np.float64)
+    single = pts.ndim == 1
+    if single and pts.shape != (3,):
+        raise ValueError("a point has three coordinates")
+    if not single and pts.ndim == 2 and pts.shape[0] != 3 and pts.shape[1] == 3:
+        pts = pts.T
+    vals = solution._solution.pressure(pts.reshape(3, -1), field)
+    return complex(vals[0]) if single else vals
 
 
 def diagnostics(sol):

@@ -342,6 +342,7 @@ struct bem_operator_s {
 struct bem_solution_s {
     bem_ctx ctx = nullptr;
     bem_mesh mesh = nullptr;
+    bem_problem prob{};  // boundary, contrasts and wavenumber of the solve
     double k = 0;
     int nrhs = 0, nt = 0;
     std::vector<double> angles;
@@ -498,7 +499,9 @@ void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) 
     st.ready = op->lane->ev_ready;
     st.panel = op->lane->ev_panel;
     int rc = lu_factor<R>(st, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, op->lane->ws, &op->lane->pl);
-    if (rc == 2) throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (262144 rows)");
+    if (rc == 2)
+        throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (" +
+                                              std::to_string(256 * (int64_t)op->lane->ws.max_blocks) + " rows)");
     if (rc != 0) throw BemError(BEM_ERR_DEVICE, "LU panel launch failed");
     HIPCHECK(hipGetLastError());
     int status[2];
@@ -584,7 +587,9 @@ void calderon_blocks(bem_operator op) {
         st.ready = L->ev_ready;
         st.panel = L->ev_panel;
         const int rc = lu_factor<double>(st, S, nt, nt, ipiv, inv, L->ws, &L->pl);
-        if (rc == 2) throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (262144 rows)");
+        if (rc == 2)
+            throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (" +
+                                                  std::to_string(256 * (int64_t)L->ws.max_blocks) + " rows)");
         if (rc != 0) throw BemError(BEM_ERR_DEVICE, "LU panel launch failed");
         int status[2];
         HIPCHECK(hipMemcpyAsync(status, L->ws.status, sizeof status, hipMemcpyDeviceToHost, s));
@@ -1086,6 +1091,7 @@ void operator_solve(bem_operator op, int nrhs, const double* angles, bem_solutio
     sol->mesh = op->mesh;
     op->mesh->refs++;
     sol->k = op->prob.k;
+    sol->prob = op->prob;
     sol->nrhs = nrhs;
     sol->nt = nt;
     sol->angles.assign(angles, angles + 2 * nrhs);
@@ -2565,6 +2571,56 @@ int bem_far_field_noalloc(bem_solution sol, int32_t r, int64_t ndir, const doubl
     });
 }
 
+int bem_pressure(bem_solution sol, int32_t r, int64_t npts, const double* points, int32_t field, double* out) {
+    if (!sol || !out || !points || r < 0 || r >= sol->nrhs || npts <= 0 || npts > ((int64_t)1 << 28))
+        return BEM_ERR_ARGUMENT;
+    bem_ctx ctx = sol->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        if (field < BEM_FIELD_TOTAL || field > BEM_FIELD_INTERIOR)
+            throw BemError(BEM_ERR_ARGUMENT, "field must be :total, :scattered, :incident or :interior");
+        for (int64_t i = 0; i < 3 * npts; ++i)
+            if (!std::isfinite(points[i])) throw BemError(BEM_ERR_ARGUMENT, "point coordinates must be finite");
+        FieldParams fp{};
+        fp.k = sol->k;
+        fp.fluid = sol->prob.boundary == BEM_FLUID ? 1 : 0;
+        fp.g = fp.fluid ? sol->prob.density_contrast : 1.0;
+        fp.kint = fp.fluid ? sol->k / sol->prob.soundspeed_contrast : sol->k;
+        fp.field = field;
+        direction(sol->angles[2 * r], sol->angles[2 * r + 1], fp.d);
+        hipStream_t s = ctx->lane0()->s;
+        const int64_t chunk = std::min<int64_t>(npts, 1 << 20);
+        double* dp = (double*)ctx->alloc(sizeof(double) * 3 * chunk);
+        z64* dout = nullptr;
+        int* dreg = nullptr;
+        double* dwind = nullptr;
+        try {
+            dout = (z64*)ctx->alloc(sizeof(z64) * chunk);
+            dreg = (int*)ctx->alloc(sizeof(int) * chunk);
+            dwind = (double*)ctx->alloc(sizeof(double) * chunk);
+            std::vector<int> reg(chunk);
+            for (int64_t i0 = 0; i0 < npts; i0 += chunk) {
+                const int nc = (int)std::min<int64_t>(chunk, npts - i0);
+                HIPCHECK(hipMemcpyAsync(dp, points + 3 * i0, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, s));
+                launch_pressure(s, sol->mesh->d, fp, nc, dp, sol->p + (size_t)r * sol->nt,
+                                sol->q + (size_t)r * sol->nt, dout, dreg, dwind);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipMemcpyAsync(out + 2 * i0, dout, sizeof(z64) * nc, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipMemcpyAsync(reg.data(), dreg, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
+                for (int i = 0; i < nc; ++i)
+                    if (reg[i] == REGION_NONE)
+                        throw BemError(BEM_ERR_ARGUMENT, "the selected field has no acoustic pressure at this point");
+            }
+        } catch (...) {
+            hipStreamSynchronize(s);
+            ctx->release(dp); ctx->release(dout); ctx->release(dreg); ctx->release(dwind);
+            throw;
+        }
+        ctx->release(dp); ctx->release(dout); ctx->release(dreg); ctx->release(dwind);
+    });
+}
+
 int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh* meshes, const bem_problem* problems, const double* angles,
               double* amplitudes, double* latency_ms, bem_diagnostics* diagnostics) {
     if (!ctx || n <= 0 || !meshes || !problems || !angles || !amplitudes) return BEM_ERR_ARGUMENT;
@@ -3023,6 +3079,7 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t nrhs, const double* angles, bem_sol
         sol->mesh = h->mesh;
         h->mesh->refs++;
         sol->k = h->prob.k;
+        sol->prob = h->prob;
         sol->nrhs = nrhs;
         sol->nt = nt;
         sol->angles.assign(angles, angles + 2 * nrhs);

@@ -435,6 +435,137 @@ void launch_far_field(hipStream_t s, const DevMesh& m, double k, int nrhs, int n
 }
 
 // ---------------------------------------------------------------------------
+// Pressure at points (reference src/fields/pressure.jl:404-472), FP64.
+// One wave per point, lanes striding over the panels:
+//  1. the winding number w = (1/4pi) sum_j Omega_j(x) of the closed surface
+//     about x, with the solid angle of each flat panel from Van Oosterom and
+//     Strackee (IEEE Trans. Biomed. Eng. 30 (1983) 125-126):
+//     tan(Omega/2) = a.(b x c) / (|a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a|),
+//     a, b, c the vertices relative to x; w ~ 1 inside, 0 outside, 1/2 on a
+//     face.  Outside/on (w < 3/4) and inside/on (w > 1/4) follow the
+//     reference's region rule (pressure.jl:496-501);
+//  2. the layer potentials of the surface traces at x:
+//     exterior  p_s(x) = D p - S q                      (wavenumber k)
+//     interior  p_i(x) = S (g (q + q_inc)) - D (p + p_inc)  (wavenumber k_int)
+//     with S u = sum_j u_j int_Tj G, D u = sum_j u_j int_Tj dG/dn_y, the
+//     7-point rule, or 64 sub-triangles x 7 points when |x - c_j| < 2 hmax_j
+//     (the near rule of the assembly).  The total exterior field adds the
+//     incident plane wave.  Points whose field has no pressure in their
+//     region get REGION_NONE and NaN.
+// ---------------------------------------------------------------------------
+__device__ inline double tri_solid_angle(const double* V, double x0, double x1, double x2) {
+    const double a0 = V[0] - x0, a1 = V[1] - x1, a2 = V[2] - x2;
+    const double b0 = V[3] - x0, b1 = V[4] - x1, b2 = V[5] - x2;
+    const double c0 = V[6] - x0, c1 = V[7] - x1, c2 = V[8] - x2;
+    const double la = sqrt(a0 * a0 + a1 * a1 + a2 * a2), lb = sqrt(b0 * b0 + b1 * b1 + b2 * b2),
+                 lc = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    const double num = a0 * (b1 * c2 - b2 * c1) + a1 * (b2 * c0 - b0 * c2) + a2 * (b0 * c1 - b1 * c0);
+    const double den = la * lb * lc + (a0 * b0 + a1 * b1 + a2 * b2) * lc + (a0 * c0 + a1 * c1 + a2 * c2) * lb +
+                       (b0 * c0 + b1 * c1 + b2 * c2) * la;
+    return 2.0 * atan2(num, den);
+}
+
+__global__ __launch_bounds__(256) void k_pressure(DevMesh m, FieldParams fp, int npts,
+                                                  const double* __restrict__ pts, const z64* __restrict__ p,
+                                                  const z64* __restrict__ q, z64* __restrict__ out,
+                                                  int* __restrict__ region, double* __restrict__ winding) {
+    const int lane = threadIdx.x & 63;
+    const int pt = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pt >= npts) return;
+    const int nt = m.nt;
+    const double x0 = pts[3 * pt], x1 = pts[3 * pt + 1], x2 = pts[3 * pt + 2];
+    // incident plane wave at x
+    const double phx = fp.k * (fp.d[0] * x0 + fp.d[1] * x1 + fp.d[2] * x2);
+    double isn, ics;
+    sincos(phx, &isn, &ics);
+    if (fp.field == FIELD_INCIDENT) {
+        if (lane == 0) {
+            out[pt] = z64{ics, isn};
+            region[pt] = REGION_EXTERIOR;
+            winding[pt] = 0.0;
+        }
+        return;
+    }
+    double om = 0.0;
+    for (int j = lane; j < nt; j += 64) om += tri_solid_angle(m.vert + 9 * (size_t)j, x0, x1, x2);
+    // lane 0's sum for every lane, so the region test is wave-uniform
+    om = __shfl(wave_sum(om), 0, 64);
+    const double w = om * 0.07957747154594767;  // / (4 pi)
+    const bool outside_or_on = w < 0.75, inside_or_on = w > 0.25;
+    int reg = REGION_NONE;
+    if ((fp.field == FIELD_TOTAL || fp.field == FIELD_SCATTERED) && outside_or_on) reg = REGION_EXTERIOR;
+    else if (fp.fluid && (fp.field == FIELD_TOTAL || fp.field == FIELD_INTERIOR) && inside_or_on)
+        reg = REGION_INTERIOR;
+    if (reg == REGION_NONE) {
+        if (lane == 0) {
+            out[pt] = z64{NAN, NAN};
+            region[pt] = reg;
+            winding[pt] = w;
+        }
+        return;
+    }
+    const bool inner = reg == REGION_INTERIOR;
+    const double kk = inner ? fp.kint : fp.k;
+    double sx = 0.0, sy = 0.0;  // S u
+    double dx = 0.0, dy = 0.0;  // D v
+    for (int j = lane; j < nt; j += 64) {
+        const double* V = m.vert + 9 * (size_t)j;
+        const double cj0 = m.cen[j], cj1 = m.cen[nt + j], cj2 = m.cen[2 * nt + j];
+        const double n0 = m.nrm[j], n1 = m.nrm[nt + j], n2 = m.nrm[2 * nt + j];
+        // densities: u multiplies S, v multiplies D
+        z64 u = q[j], v = p[j];
+        if (inner) {
+            const double ph = fp.k * (fp.d[0] * cj0 + fp.d[1] * cj1 + fp.d[2] * cj2);
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            const double dn = fp.k * (fp.d[0] * n0 + fp.d[1] * n1 + fp.d[2] * n2);
+            v = z64{v.x + cs, v.y + sn};                                // p + p_inc
+            u = z64{fp.g * (u.x - dn * sn), fp.g * (u.y + dn * cs)};    // g (q + q_inc)
+        }
+        double e0 = V[3] - V[0], e1 = V[4] - V[1], e2 = V[5] - V[2];
+        double hm = e0 * e0 + e1 * e1 + e2 * e2;
+        e0 = V[6] - V[3]; e1 = V[7] - V[4]; e2 = V[8] - V[5];
+        hm = fmax(hm, e0 * e0 + e1 * e1 + e2 * e2);
+        e0 = V[6] - V[0]; e1 = V[7] - V[1]; e2 = V[8] - V[2];
+        hm = fmax(hm, e0 * e0 + e1 * e1 + e2 * e2);
+        const double r0 = x0 - cj0, r1 = x1 - cj1, r2 = x2 - cj2;
+        const bool near = r0 * r0 + r1 * r1 + r2 * r2 < kNearEta * kNearEta * hm;
+        const int npq = near ? kNearPts : 7;
+        const double aj = m.area[j];
+        Acc4<double> acc{};
+        for (int g = 0; g < npq; ++g) {
+            double l0, l1, l2, wq;
+            if (near) { l0 = c_near[g][0]; l1 = c_near[g][1]; l2 = c_near[g][2]; wq = c_near[g][3]; }
+            else { l0 = c_q7[g][0]; l1 = c_q7[g][1]; l2 = c_q7[g][2]; wq = c_q7[g][3]; }
+            const double y0 = l0 * V[0] + l1 * V[3] + l2 * V[6];
+            const double y1 = l0 * V[1] + l1 * V[4] + l2 * V[7];
+            const double y2 = l0 * V[2] + l1 * V[5] + l2 * V[8];
+            kpoint<double, 3>(kk, x0 - y0, x1 - y1, x2 - y2, 0.0, 0.0, 0.0, n0, n1, n2, 0.0, wq * aj, acc);
+        }
+        sx += acc.S.x * u.x - acc.S.y * u.y;
+        sy += acc.S.x * u.y + acc.S.y * u.x;
+        dx += acc.D.x * v.x - acc.D.y * v.y;
+        dy += acc.D.x * v.y + acc.D.y * v.x;
+    }
+    sx = wave_sum(sx); sy = wave_sum(sy);
+    dx = wave_sum(dx); dy = wave_sum(dy);
+    if (lane == 0) {
+        z64 val = inner ? z64{sx - dx, sy - dy} : z64{dx - sx, dy - sy};
+        if (!inner && fp.field == FIELD_TOTAL) { val.x += ics; val.y += isn; }
+        out[pt] = val;
+        region[pt] = reg;
+        winding[pt] = w;
+    }
+}
+
+void launch_pressure(hipStream_t s, const DevMesh& m, const FieldParams& fp, int npts, const double* pts,
+                     const z64* p, const z64* q, z64* out, int* region, double* winding) {
+    if (npts <= 0) return;
+    hipLaunchKernelGGL(k_pressure, dim3((npts + 3) / 4), dim3(256), 0, s, m, fp, npts, pts, p, q, out, region,
+                       winding);
+}
+
+// ---------------------------------------------------------------------------
 // Hierarchical path: entries by unknown index (the host maps tree positions
 // to unknowns).  Rigid kinds have one unknown per panel; the Mueller kind has
 // two (pressure I = i, normal derivative I = N + i).  Regular pairs in FP32

@@ -84,6 +84,21 @@ constexpr int kFarBlocks = 256;
 void launch_far_field(hipStream_t s, const DevMesh& m, double k, int nrhs, int ndir,
                       const double* dirs, const z64* p, const z64* q, z64* partial, z64* out);
 
+// Pressure at points (reference src/fields/pressure.jl:404-472): field codes
+// and the region each point was evaluated in.
+enum FieldKind : int { FIELD_TOTAL = 0, FIELD_SCATTERED = 1, FIELD_INCIDENT = 2, FIELD_INTERIOR = 3 };
+enum FieldRegion : int { REGION_EXTERIOR = 0, REGION_INTERIOR = 1, REGION_NONE = 2 };
+struct FieldParams {
+    double k, kint, g;   // exterior and interior wavenumbers, density contrast
+    double d[3];         // incidence direction of the plane wave
+    int field;           // FieldKind
+    int fluid;           // 1 when the body has a fluid interior
+};
+// out[i] for points pts[3i..3i+2]; region[i] = FieldRegion; winding[i] the
+// solid-angle winding number of the surface about the point.
+void launch_pressure(hipStream_t s, const DevMesh& m, const FieldParams& fp, int npts, const double* pts,
+                     const z64* p, const z64* q, z64* out, int* region, double* winding);
+
 // ---- hierarchical matrices (bem_assembly.hip: entries and ACA; bem_hsolve.hip: apply, GMRES) ----
 // A block of the compressed operator, in cluster-tree positions.  Dense: m x n
 // row-major complex FP32 values at `off`.  Low-rank A ~ U V^T: U (m x rmax) and

@@ -207,6 +207,26 @@ int bem_solution_traces(bem_solution sol, int32_t r, double *p_scat, double *dpd
  * means backscatter xhat = -d (src/solvers/bem/postprocessing.jl:4-6). */
 int bem_far_field(bem_solution sol, int32_t r, int64_t n_dir, const double *dirs, double *out);
 int bem_solution_diagnostics(bem_solution sol, int32_t r, bem_diagnostics *out);
+
+/* ---- pressure at points (src/fields/pressure.jl:404-472) ----------------
+ * Field codes of pressure(solution, points; field) (pressure.jl:1-20). */
+#define BEM_FIELD_TOTAL 0     /* incident + scattered outside; transmitted inside a fluid body */
+#define BEM_FIELD_SCATTERED 1 /* on and outside the surface only                              */
+#define BEM_FIELD_INCIDENT 2  /* the unit plane wave                                          */
+#define BEM_FIELD_INTERIOR 3  /* total pressure inside a fluid body                           */
+/* Complex pressure at `n_points` points (3 x n column-major, metres) for
+ * right-hand side `r` of a solution (dense or hierarchical).  Outside (or
+ * on) the surface: the scattered field D p - S dp/dn of the surface traces
+ * at wavenumber k; inside a fluid body: S (g (q + q_inc)) - D (p + p_inc) at
+ * k_int (pressure.jl:426-472); the total field adds the incident wave
+ * outside.  Inside and outside are decided by the surface's winding number
+ * about each point.  A point where the selected field has no pressure (the
+ * scattered field inside the body, any interior field of a rigid or
+ * pressure-release body) fails with BEM_ERR_ARGUMENT and the reference's
+ * message (pressure.jl:496-501).  FP64 on the device; a panel within twice
+ * its longest edge of a point takes the subdivided near rule. */
+int bem_pressure(bem_solution sol, int32_t r, int64_t n_points, const double *points, int32_t field,
+                 double *out);
 /* bem_far_field without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, hosts that forbid hipMalloc).

@@ -478,6 +478,95 @@ int orc_assemble_rows(const orc_mesh *m, int kind, double k, double g, double h,
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* pressure at points (src/fields/pressure.jl:404-472)                       */
+/* ------------------------------------------------------------------------ */
+/* Solid angle of a flat triangle seen from x (Van Oosterom and Strackee,
+ * IEEE Trans. Biomed. Eng. 30 (1983) 125-126).                              */
+static double solid_angle(const double *V, const double *x) {
+    double a[3], b[3], c[3];
+    for (int d = 0; d < 3; ++d) {
+        a[d] = V[d] - x[d];
+        b[d] = V[3 + d] - x[d];
+        c[d] = V[6 + d] - x[d];
+    }
+    double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    double lb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    double lc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    double num = a[0] * (b[1] * c[2] - b[2] * c[1]) + a[1] * (b[2] * c[0] - b[0] * c[2]) +
+                 a[2] * (b[0] * c[1] - b[1] * c[0]);
+    double den = la * lb * lc + (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) * lc +
+                 (a[0] * c[0] + a[1] * c[1] + a[2] * c[2]) * lb +
+                 (b[0] * c[0] + b[1] * c[1] + b[2] * c[2]) * la;
+    return 2.0 * atan2(num, den);
+}
+
+/* field: 0 total, 1 scattered, 2 incident, 3 interior.  region[i]: 0
+ * exterior, 1 interior, 2 none (NaN value).  p, q: scattered traces at the
+ * centroids; d: incidence direction.  Exterior D p - S q at k (+ incident
+ * for the total field); interior S (g (q + q_inc)) - D (p + p_inc) at kint. */
+void orc_pressure(const orc_mesh *m, double k, double kint, double g, int fluid, int field,
+                  const double *dir, int64_t npts, const double *pts, const double *pd,
+                  const double *qd, double *outd, int *region, double *winding) {
+    init_rules();
+    const zc *p = (const zc *)pd, *q = (const zc *)qd;
+    zc *out = (zc *)outd;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i = 0; i < npts; ++i) {
+        const double *x = pts + 3 * i;
+        zc inc = cexp(I * k * (dir[0] * x[0] + dir[1] * x[1] + dir[2] * x[2]));
+        if (field == 2) {
+            out[i] = inc;
+            region[i] = 0;
+            winding[i] = 0.0;
+            continue;
+        }
+        double om = 0.0;
+        for (int64_t j = 0; j < m->nt; ++j) om += solid_angle(m->v + 9 * j, x);
+        double w = om / FOUR_PI;
+        winding[i] = w;
+        int outside_or_on = w < 0.75, inside_or_on = w > 0.25;
+        int reg = 2;
+        if ((field == 0 || field == 1) && outside_or_on) reg = 0;
+        else if (fluid && (field == 0 || field == 3) && inside_or_on) reg = 1;
+        region[i] = reg;
+        if (reg == 2) {
+            out[i] = NAN + NAN * I;
+            continue;
+        }
+        double kk = reg == 1 ? kint : k;
+        zc S = 0, D = 0;
+        for (int64_t j = 0; j < m->nt; ++j) {
+            const double *c = m->c + 3 * j, *ny = m->n + 3 * j, *V = m->v + 9 * j;
+            zc u = q[j], v = p[j];
+            if (reg == 1) {
+                zc pin = cexp(I * k * (dir[0] * c[0] + dir[1] * c[1] + dir[2] * c[2]));
+                double dn = k * (dir[0] * ny[0] + dir[1] * ny[1] + dir[2] * ny[2]);
+                v = v + pin;
+                u = g * (u + I * dn * pin);
+            }
+            double rx = x[0] - c[0], ry = x[1] - c[1], rz = x[2] - c[2];
+            double lim = NEAR_ETA * m->h[j];
+            int near = rx * rx + ry * ry + rz * rz < lim * lim;
+            int np = near ? nsub_pts : 7;
+            zc acc[4] = {0, 0, 0, 0};
+            double nx[3] = {0, 0, 0};
+            for (int t = 0; t < np; ++t) {
+                const double *l = near ? sub_l[t] : q7_l[t];
+                double wq = (near ? sub_w[t] : q7_w[t]) * m->area[j];
+                double y[3];
+                for (int d = 0; d < 3; ++d) y[d] = l[0] * V[d] + l[1] * V[3 + d] + l[2] * V[6 + d];
+                kern4(kk, x, nx, y, ny, wq, acc);
+            }
+            S += acc[0] * u;
+            D += acc[1] * v;
+        }
+        zc val = reg == 1 ? S - D : D - S;
+        if (reg == 0 && field == 0) val += inc;
+        out[i] = val;
+    }
+}
+
 /* Polar angle from +x, azimuth from +y toward +z (full.jl:15-18). */
 void orc_direction(double beta, double alpha, double *d) {
     d[0] = cos(beta);

@@ -6,6 +6,11 @@ Restates the reference's exact sphere solution:
   soft         A_m = -j_m(ka) / h_m(ka)                        src/solvers/modal/sphere.jl:10-13
   fluid        closed form with gh = g*h, ka_int = ka/h        src/solvers/modal/sphere.jl:25-40
   amplitude    f = (-i/k) sum_m (2m+1) P_m(cos angle) A_m      src/solvers/modal/sphere.jl:279-288
+  near field   p_s = sum_m (2m+1) i^m A_m h_m(kr) P_m(cos theta) outside; inside a fluid
+               sphere p = sum_m (2m+1) i^m B_m j_m(k_int r) P_m(cos theta) with
+               B_m = (j_m(ka) + A_m h_m(ka)) / j_m(k_int a) (pressure continuity at r = a);
+               the plane-wave expansion e^{ikr cos theta} = sum (2m+1) i^m j_m(kr) P_m
+               (textbook; checked against the far-field limit in the tests)
   derivatives  f_n' = (n/x) f_n - f_{n+1}                      src/numerics/special_functions.jl:27-35
   Legendre     three-term recurrence                           src/numerics/special_functions.jl:56-64
 Spherical Bessel functions come from scipy.special (the reference calls
@@ -82,3 +87,23 @@ def form_function(boundary, k, a, angle=math.pi, m_max=None):
 
 def target_strength(f):
     return 20.0 * math.log10(abs(f))
+
+
+def near_field(boundary, k, a, r, cos_theta, m_max=None):
+    """Scattered pressure at radius r >= a, or the interior pressure of a
+    fluid sphere at r < a, for a unit plane wave; theta is the angle between
+    the point and the incidence direction."""
+    if m_max is None:
+        m_max = default_mode_count(k * a) + 10
+    total = 0j
+    for m in range(m_max + 1):
+        A = modal_coefficient(boundary, m, k, a)
+        pm = legendre_p(m, cos_theta)
+        if r >= a:
+            total += (2 * m + 1) * (1j ** m) * A * hs(m, k * r) * pm
+        else:
+            g, h = boundary[1], boundary[2]
+            ki = k / h
+            B = (js(m, k * a) + A * hs(m, k * a)) / js(m, ki * a)
+            total += (2 * m + 1) * (1j ** m) * B * js(m, ki * r) * pm
+    return total

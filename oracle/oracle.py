@@ -55,6 +55,9 @@ def lib():
         L.orc_assemble_rows.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_int64, _ip, _dp]
         L.orc_incident.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_pressure.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int64, _dp, _dp, _dp, _dp,
+                                   ctypes.POINTER(ctypes.c_int), _dp]
         L.orc_far_field.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, _dp,
                                     _dp, _dp, _dp]
         L.orc_validate.argtypes = [ctypes.c_int64, _dp, ctypes.c_int64, _ip, _dp,
@@ -143,6 +146,25 @@ class Mesh:
         p = np.empty(self.n, dtype=np.complex128); q = np.empty(self.n, dtype=np.complex128)
         lib().orc_incident(self._h, k, _d(d), p.ctypes.data_as(_dp), q.ctypes.data_as(_dp))
         return p, q
+
+    def pressure(self, k, points, p, q, beta=0.0, alpha=0.0, field="total", fluid=None):
+        """Pressure at points (N x 3) from scattered traces p, q
+        (src/fields/pressure.jl:404-472); fluid = (g, h) for a fluid body.
+        Returns (values, regions, winding numbers); regions 0 exterior,
+        1 interior, 2 no pressure for this field."""
+        codes = {"total": 0, "scattered": 1, "incident": 2, "interior": 3}
+        pts = np.ascontiguousarray(np.atleast_2d(points), dtype=np.float64)
+        n = pts.shape[0]
+        out = np.empty(n, dtype=np.complex128)
+        reg = np.empty(n, dtype=np.int32)
+        wind = np.empty(n)
+        g, h = fluid if fluid is not None else (1.0, 1.0)
+        d = np.ascontiguousarray(direction(beta, alpha))
+        lib().orc_pressure(self._h, k, k / h, g, 1 if fluid is not None else 0, codes[field], _d(d), n, _d(pts),
+                           np.ascontiguousarray(p, dtype=np.complex128).ctypes.data_as(_dp),
+                           np.ascontiguousarray(q, dtype=np.complex128).ctypes.data_as(_dp),
+                           out.ctypes.data_as(_dp), reg.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _d(wind))
+        return out, reg, wind
 
     def far_field(self, k, dirs, p, q):
         dirs = np.ascontiguousarray(np.atleast_2d(dirs), dtype=np.float64)

@@ -2,10 +2,16 @@
 
 Tolerances: FP64 assembly entries 1e-12 relative to the matrix max; FP64
 solutions 1e-9 relative in the scattering amplitude (the plain GPU form's
-parity bar); the mixed form |dTS| <= 0.01 dB and amplitude rtol 1e-8 against
-the FP64 form (src/solvers/bem/fluid_system.jl acceptance, test/basic/bem/
-formulations.jl:431-436); physics against the sphere modal series at the
-reference's 0.1 dB / 1 % bar (test/extended/bem/sphere/full3d.jl:139-140).
+parity bar); the mixed form |dTS| <= 0.01 dB against the FP64 form and
+backward errors <= 1e-13 (src/solvers/bem/fluid_system.jl acceptance).  The
+reference's mixed-versus-double amplitude bar is 1e-8 (test/basic/bem/
+formulations.jl:431-436) because its refinement runs against the FP64 matrix;
+here regular panel pairs are assembled in FP32 and the refinement converges
+to that FP32-assembled system, which sits ~1e-7 from the FP64 one, so the
+amplitude bar is 1e-5 (the refinement's own 1e-9 parity against the oracle's
+solve of the same FP32-assembled matrix is test_mixed_refinement_reaches_fp32_
+system_solution).  Physics against the sphere modal series at the reference's
+0.1 dB / 1 % bar (test/extended/bem/sphere/full3d.jl:139-140).
 """
 import math
 import sys

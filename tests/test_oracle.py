@@ -273,8 +273,10 @@ def test_reference_gas_bubble_resonance_golden():
 @pytest.mark.slow
 def test_reference_spheroid_rigid_golden_L4():
     """test/extended/bem/spheroid/full3d.jl:10-24 (k=2, beta=pi/3).  At 5120
-    flat panels the amplitude error is ~1.07 %; TS is inside 0.1 dB.  The GPU
-    suite checks the full 1 % bar at 20480 panels."""
+    flat panels the amplitude error is ~1.07 %, so this CPU check relaxes the
+    reference's 1 % bar to 1.2 % (TS stays inside 0.1 dB); the GPU test
+    test_reference_rigid_spheroid_golden_n20480 (tests/test_gpu_full_size.py)
+    holds the full 1 % bar at 20480 panels (0.11-0.57 % measured)."""
     nodes, tri = meshes.spheroid(1.5, 1.0, 4)
     m = O.Mesh(nodes, tri)
     p, q, _ = O.solve_boundary(m, "rigid", 2.0, beta=math.pi / 3)
@@ -396,3 +398,95 @@ def test_assemble_rows_equal_full_assembly(kind):
     assert np.array_equal(R, A[rows, :])
     with pytest.raises(ValueError):
         m.assemble_rows(kind, 300.0, np.array([A.shape[0]]))
+
+
+# ---------------------------------------------------------------------------
+# pressure at points (src/fields/pressure.jl:404-472)
+# ---------------------------------------------------------------------------
+def _dirs(n, seed):
+    v = np.random.default_rng(seed).standard_normal((n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def test_modal_near_field_far_limit_and_plane_wave_expansion():
+    """The modal near field tends to f(theta) e^{ikr}/r (the reference's form
+    function) and the series behind the interior coefficients reproduces the
+    plane wave: two closed forms the near-field restatement must meet."""
+    k, a = K1, A1
+    for bc in (("rigid",), ("soft",), ("fluid",) + FLUID1):
+        for th in (0.3, 1.7, math.pi):
+            r = 2.0e6  # the next term of h_m's expansion is m(m+1)/(2kr) ~ 1e-8 here
+            ps = modal.near_field(bc, k, a, r, math.cos(th))
+            f = modal.form_function(bc, k, a, angle=th)
+            assert abs(ps * r * np.exp(-1j * k * r) - f) < 1e-6 * abs(f)
+    for kr, c in ((0.7, 0.2), (3.1, -0.6), (9.0, 0.95)):
+        s = sum((2 * m + 1) * (1j ** m) * modal.js(m, kr) * modal.legendre_p(m, c) for m in range(60))
+        assert abs(s - np.exp(1j * kr * c)) < 1e-12
+
+
+@pytest.mark.parametrize("bc", ["rigid", "soft", "fluid"])
+def test_oracle_pressure_matches_modal_near_field(bc):
+    """Sphere a = 1 cm at 38 kHz (ka = 1.62), 5120 panels: the oracle's
+    field from its own BEM traces against the exact series outside (r =
+    1.5a .. 3a) and, for the fluid body, inside (r = 0.3a .. 0.6a), at the
+    discretisation's accuracy (the amplitude errs ~1 % here)."""
+    nodes, tri = meshes.sphere(A1, 4)
+    m = O.Mesh(nodes, tri)
+    if bc == "fluid":
+        p, q, _ = O.solve_fluid(m, *FLUID1, K1, beta=0.0)
+        mbc, fluid = ("fluid",) + FLUID1, FLUID1
+    else:
+        p, q, _ = O.solve_boundary(m, bc, K1, beta=0.0)
+        mbc, fluid = (bc,), None
+    dirs = _dirs(12, 3)
+    radii = np.array([1.5, 2.0, 3.0, 1.5, 2.0, 3.0, 1.5, 2.0, 3.0, 2.5, 1.8, 2.2]) * A1
+    pts = dirs * radii[:, None]
+    vals, reg, wind = m.pressure(K1, pts, p, q, field="scattered", fluid=fluid)
+    assert np.all(reg == 0) and np.all(np.abs(wind) < 1e-9)
+    ref = np.array([modal.near_field(mbc, K1, A1, r, d[0]) for r, d in zip(radii, dirs)])
+    assert np.max(np.abs(vals - ref)) < 0.02 * np.max(np.abs(ref))
+    tot, _, _ = m.pressure(K1, pts, p, q, field="total", fluid=fluid)
+    assert np.allclose(tot - vals, np.exp(1j * K1 * pts[:, 0]), atol=1e-12)
+    if bc == "fluid":
+        ri = np.array([0.3, 0.45, 0.6, 0.5]) * A1
+        pin = _dirs(4, 5) * ri[:, None]
+        vi, regi, wi = m.pressure(K1, pin, p, q, field="interior", fluid=fluid)
+        assert np.all(regi == 1) and np.all(np.abs(wi - 1) < 1e-9)
+        refi = np.array([modal.near_field(mbc, K1, A1, r, d[0]) for r, d in zip(ri, _dirs(4, 5))])
+        assert np.max(np.abs(vi - refi)) < 0.02 * np.max(np.abs(refi))
+    else:
+        _, regi, _ = m.pressure(K1, np.array([[0.0, 0.0, 0.0]]), p, q, field="total")
+        assert regi[0] == 2
+
+
+def test_oracle_pressure_far_limit_matches_far_field():
+    """At 1000 m the field of the surface traces equals the oracle's far-field
+    amplitude times e^{ikR}/R (the representation formula and far_field are
+    one integral apart): a sign or factor error in either fails."""
+    nodes, tri = meshes.spheroid(0.02, 0.01, 2)
+    m = O.Mesh(nodes, tri)
+    k = 300.0
+    p, q, _ = O.solve_fluid(m, 1.2, 1.1, k, beta=0.7)
+    dirs = _dirs(5, 9)
+    R = 1000.0
+    vals, reg, _ = m.pressure(k, dirs * R, p, q, beta=0.7, field="scattered", fluid=(1.2, 1.1))
+    f = m.far_field(k, dirs, p, q)
+    assert np.all(reg == 0)
+    assert np.max(np.abs(vals * R * np.exp(-1j * k * R) - f)) < 1e-4 * np.max(np.abs(f))
+
+
+def test_oracle_pressure_matched_medium_and_winding():
+    """g = h = 1: no scattered field outside, and the interior field is the
+    incident wave; the winding number is 1 inside and 0 outside."""
+    nodes, tri = meshes.sphere(1.0, 3)
+    m = O.Mesh(nodes, tri)
+    k = 2.0
+    p, q, _ = O.solve_fluid(m, 1.0, 1.0, k, beta=0.4)
+    out_pts = _dirs(6, 1) * 1.7
+    in_pts = _dirs(6, 2) * 0.5
+    vs, rs, ws = m.pressure(k, out_pts, p, q, beta=0.4, field="scattered", fluid=(1.0, 1.0))
+    assert np.all(rs == 0) and np.max(np.abs(vs)) < 1e-10
+    vi, ri, wi = m.pressure(k, in_pts, p, q, beta=0.4, field="total", fluid=(1.0, 1.0))
+    d = O.direction(0.4)
+    assert np.all(ri == 1) and np.all(np.abs(wi - 1.0) < 1e-9)
+    assert np.max(np.abs(vi - np.exp(1j * k * in_pts @ d))) < 5e-3
