@@ -48,7 +48,7 @@ EXPORTED = [
     "bem_server_submit", "bem_server_poll", "bem_server_wait", "bem_server_error",
     "bem_server_stats_get", "bem_hmatrix_defaults", "bem_hmatrix_plan", "bem_hmatrix_create",
     "bem_hmatrix_destroy", "bem_hmatrix_get_info", "bem_hmatrix_matvec", "bem_hmatrix_apply", "bem_hmatrix_solve",
-    "bem_debug_lu", "bem_far_field_noalloc", "bem_pressure",
+    "bem_debug_lu", "bem_far_field_noalloc", "bem_pressure", "bem_reduced_defaults", "bem_reduced_sweep",
 ]
 
 
@@ -118,6 +118,30 @@ class HMatrixInfo(ctypes.Structure):
                 ("assembly_ms", ctypes.c_double), ("aca_ms", ctypes.c_double)]
 
 
+class ReducedOptions(ctypes.Structure):
+    _fields_ = [("tolerance", ctypes.c_double), ("rank_tolerance", ctypes.c_double), ("capacity", ctypes.c_int32),
+                ("maxbytes", ctypes.c_int64)]
+
+
+class ReducedReport(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_double), ("reduced", ctypes.c_int32), ("reason", ctypes.c_int32),
+                ("rank", ctypes.c_int32), ("kept", ctypes.c_int32), ("projected_original", ctypes.c_double),
+                ("projected_scaled", ctypes.c_double), ("residual_original", ctypes.c_double),
+                ("residual_scaled", ctypes.c_double)]
+
+    REASONS = {0: "accepted", 1: "empty", 2: "residual", 3: "rank", 4: "nonfinite"}
+
+    def as_dict(self, training=False):
+        d = {"k": self.k, "rank": self.rank, "residual": {"original": self.residual_original,
+                                                          "scaled": self.residual_scaled}}
+        if training:
+            d["kept"] = bool(self.kept)
+        else:
+            d.update(reduced=bool(self.reduced), reason=self.REASONS[self.reason],
+                     projected={"original": self.projected_original, "scaled": self.projected_scaled})
+        return d
+
+
 def load_library(path=LIB_PATH):
     """Load libbemhip.so (raises when it has not been built)."""
     global _LIB
@@ -164,6 +188,11 @@ def load_library(path=LIB_PATH):
         "bem_far_field": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _dp, _dp]),
         "bem_far_field_noalloc": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _dp, _dp]),
         "bem_pressure": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _dp, ctypes.c_int32, _dp]),
+        "bem_reduced_defaults": (None, [ctypes.POINTER(ReducedOptions)]),
+        "bem_reduced_sweep": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_int32, _dp,
+                                             ctypes.c_int32, _dp, ctypes.c_double, ctypes.c_double,
+                                             ctypes.POINTER(ReducedOptions), _dp, ctypes.POINTER(ReducedReport),
+                                             ctypes.POINTER(ReducedReport), ctypes.POINTER(ctypes.c_int32)]),
         "bem_solution_diagnostics": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(Diagnostics)]),
         "bem_sweep": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(_vp), ctypes.POINTER(Problem),
                                      _dp, _dp, _dp, ctypes.POINTER(Diagnostics)]),
@@ -711,8 +740,16 @@ def incidence_angle_sweep(surface, boundary, k, angles, *, incidence_azimuth=0.0
     return amps, 20.0 * np.log10(np.abs(amps))
 
 
-def frequency_sweep(solve, frequencies, sound_speed):
-    """Sequential map over k = 2 pi f / c (src/postprocessing/sweeps.jl:103-110)."""
+def frequency_sweep(first, *args, **kw):
+    """frequency_sweep(solve, frequencies, c): sequential map over
+    k = 2 pi f / c (src/postprocessing/sweeps.jl:103-110).
+    frequency_sweep(surface::Mesh, material::FluidFilled, frequencies, c;
+    training_frequencies, ...): the reduced-basis sweep (sweeps.jl:208-306),
+    reduced_frequency_sweep."""
+    if isinstance(first, Mesh):
+        return reduced_frequency_sweep(first, *args, **kw)
+    solve = first
+    frequencies, sound_speed = args
     if not (math.isfinite(sound_speed) and sound_speed > 0):
         raise ValueError("sound_speed must be finite and positive")
     freqs = np.asarray(frequencies, dtype=np.float64)
@@ -721,6 +758,56 @@ def frequency_sweep(solve, frequencies, sound_speed):
     ks = 2 * math.pi * freqs / sound_speed
     amps = np.array([scattering_amplitude(solve(k)) for k in ks])
     return {"frequencies": freqs, "k": ks, "ts": 20 * np.log10(np.abs(amps)), "amplitudes": amps}
+
+
+def reduced_frequency_sweep(surface, material, frequencies, sound_speed, *, training_frequencies,
+                            incidence_angle=math.pi / 2, incidence_azimuth=0.0, tolerance=1e-10,
+                            rank_tolerance=1e-12, capacity=16, maxbytes=64 * 1024 ** 2, return_diagnostics=False):
+    """Reduced-order frequency sweep for dense single-interface fluid CBIE
+    (src/postprocessing/sweeps.jl:208-306) on the device: frozen snapshot
+    basis from full FP64 training solves; each query accepts the basis
+    least-squares solution when its original and row-scaled residuals meet
+    `tolerance`, else takes the full solve.  Returns {frequencies, k, ts,
+    amplitudes} (backscatter) and, with return_diagnostics, the report."""
+    if not isinstance(material, FluidFilled):
+        raise ValueError("the reduced sweep solves fluid bodies")
+    if not (math.isfinite(sound_speed) and sound_speed > 0):
+        raise ValueError("sound_speed must be finite and positive")
+    f = np.asarray(frequencies, dtype=np.float64).ravel()
+    ft = np.asarray(training_frequencies if training_frequencies is not None else [], dtype=np.float64).ravel()
+    for v in (f, ft):
+        if v.size == 0 or not np.all(np.isfinite(v)) or np.any(v <= 0):
+            raise ValueError("query and training frequencies must be nonempty, finite and positive")
+    if not (math.isfinite(incidence_angle) and math.isfinite(incidence_azimuth)):
+        raise ValueError("incidence angles must be finite")
+    for t in (tolerance, rank_tolerance):
+        if not (math.isfinite(t) and 0 < t < 1):
+            raise ValueError("residual and rank tolerances must lie in (0, 1)")
+    if capacity < 0 or maxbytes < 0:
+        raise ValueError("negative storage bound")
+    k = 2 * math.pi * f / sound_speed
+    kt = 2 * math.pi * ft / sound_speed
+    ctx = surface.ctx
+    opts = ReducedOptions(float(tolerance), float(rank_tolerance), int(capacity), int(maxbytes))
+    amps = np.empty(k.size, dtype=np.complex128)
+    tr = (ReducedReport * kt.size)()
+    qr = (ReducedReport * k.size)()
+    rank = ctypes.c_int32()
+    ctx.check(ctx.lib.bem_reduced_sweep(ctx._h, surface._h, material.density_contrast,
+                                        material.soundspeed_contrast, kt.size, _d(np.ascontiguousarray(kt)),
+                                        k.size, _d(np.ascontiguousarray(k)), float(incidence_angle),
+                                        float(incidence_azimuth), ctypes.byref(opts), amps.ctypes.data_as(_dp),
+                                        tr, qr, ctypes.byref(rank)))
+    sweep = {"frequencies": f, "k": k, "ts": 20 * np.log10(np.abs(amps)), "amplitudes": amps}
+    if not return_diagnostics:
+        return sweep
+    report = {"method": "reduced_basis", "formulation": "cbie", "compression": {"method": "none"},
+              "training": [dict(r.as_dict(True), frequency=float(fq)) for r, fq in zip(tr, ft)],
+              "queries": [dict(r.as_dict(), frequency=float(fq)) for r, fq in zip(qr, f)],
+              "basis_rank": rank.value, "basis_bytes": 16 * 2 * surface.n * rank.value,
+              "tolerance": tolerance, "rank_tolerance": rank_tolerance, "capacity": capacity, "maxbytes": maxbytes,
+              "incidence_angle": incidence_angle, "incidence_azimuth": incidence_azimuth}
+    return {"sweep": sweep, "diagnostics": report}
 
 
 def debug_lu(A, single=False, b=None, ctx=None, adjoint=False):

@@ -863,8 +863,12 @@ void op_free(bem_operator op) {
     delete op;
 }
 
+void operator_factor(bem_operator op);
+
+// Assembles the system of `prob` on `mesh`; unless `defer_factor`, also
+// equilibrates and factors it (operator_factor).
 void operator_create(bem_ctx ctx, Lane* lane, bem_mesh mesh, const bem_problem* prob, bem_operator* out,
-                     bool use_arena = false) {
+                     bool use_arena = false, bool defer_factor = false) {
     if (!mesh || !prob || !out) throw BemError(BEM_ERR_ARGUMENT, "null argument");
     validate_problem(*prob);
     auto* op = new bem_operator_s();
@@ -930,6 +934,24 @@ void operator_create(bem_ctx ctx, Lane* lane, bem_mesh mesh, const bem_problem* 
         op->wv3 = op_alloc(op, sizeof(z64) * n);
         op->red = (double*)op_alloc(op, sizeof(double) * 5 * kRedBlocks);
         op->flag = (int*)op_alloc(op, sizeof(int) * 4);
+        if (!defer_factor) operator_factor(op);
+    } catch (...) {
+        op_free(op);
+        throw;
+    }
+    *out = op;
+}
+
+// Equilibration, factorization and the single-precision guards of an
+// assembled operator (full.jl:364-409; fluid_system.jl:196-227).  On failure
+// the caller frees the operator.
+void operator_factor(bem_operator op) {
+    hipStream_t s = op->lane->s;
+    const bem_problem* prob = &op->prob;
+    const int n = op->n;
+    const size_t esz = op->single ? sizeof(z32) : sizeof(z64);
+    const double t1 = now_ms();
+    {
         op->ipiv = (int*)op_alloc(op, sizeof(int) * n);
         op->perm = (int*)op_alloc(op, sizeof(int) * n);
         op->inv = op_alloc(op, esz * 2 * 4096 * (size_t)((n + 63) / 64));
@@ -990,24 +1012,19 @@ void operator_create(bem_ctx ctx, Lane* lane, bem_mesh mesh, const bem_problem* 
         if (op->single && op->precision == 1 && prob->force_promotion) promote(op, BEM_REASON_CONDITION);
         HIPCHECK(hipStreamSynchronize(s));
         op->factor_ms = now_ms() - t1;
-    } catch (...) {
-        op_free(op);
-        throw;
     }
-    *out = op;
 }
 
 // One right-hand side: plane wave with device direction ddir (3 doubles);
 // scattered traces into P, Q (nt each) and the report into dg.
-void solve_rhs(bem_operator op, const double* ddir, z64* P, z64* Q, bem_diagnostics& dg) {
+// Plane-wave traces (op->vt: p_inc then q_inc) and the right-hand side b
+// (op->vb) of the operator's equations for device direction ddir
+// (full.jl:144-153,442-454).
+void build_rhs(bem_operator op, const double* ddir) {
     hipStream_t s = op->lane->s;
     const int nt = op->nt, n = op->n;
     const bem_problem& pr = op->prob;
     const bool fluid = pr.boundary == BEM_FLUID;
-    const double t0 = now_ms();
-    std::memset(&dg, 0, sizeof dg);
-    dg.unknown_count = n;
-    dg.quadrature_nodes = nt;
     z64* pinc = op->vt;        // nt
     z64* qinc = op->vt + nt;   // nt
     launch_incident(s, op->mesh->d, pr.k, 1, ddir, pinc, qinc);
@@ -1028,6 +1045,20 @@ void solve_rhs(bem_operator op, const double* ddir, z64* P, z64* Q, bem_diagnost
         if (op->single) launch_matvec<float>(s, (const z32*)op->B, n, n, n, src, op->vb, nullptr, nullptr);
         else launch_matvec<double>(s, (const z64*)op->B, n, n, n, src, op->vb, nullptr, nullptr);
     }
+}
+
+void solve_rhs(bem_operator op, const double* ddir, z64* P, z64* Q, bem_diagnostics& dg) {
+    hipStream_t s = op->lane->s;
+    const int nt = op->nt, n = op->n;
+    const bem_problem& pr = op->prob;
+    const bool fluid = pr.boundary == BEM_FLUID;
+    const double t0 = now_ms();
+    std::memset(&dg, 0, sizeof dg);
+    dg.unknown_count = n;
+    dg.quadrature_nodes = nt;
+    z64* pinc = op->vt;        // nt
+    z64* qinc = op->vt + nt;   // nt
+    build_rhs(op, ddir);
     if (pr.precision == BEM_MIXED) {
         solve_refined(op, dg);
     } else {
@@ -2032,6 +2063,231 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
 }
 
 // ---------------------------------------------------------------------------
+// Reduced-basis frequency sweep (reference src/postprocessing/sweeps.jl:
+// 113-306, FrequencyReduction): dense FP64 CBIE fluid equations per
+// frequency; training solves fill a bounded orthonormal snapshot space, which
+// is frozen; each query assembles its own matrix, projects onto the basis by
+// least squares on the row-scaled images A V / r (pivoted QR), accepts when
+// both the original and the row-scaled relative residuals meet the
+// tolerance, and otherwise falls back to the full equilibrated LU solve, which
+// must meet the same residual.  Assembly, images, residuals, LU and far field
+// run on the device; the m <= capacity column QR and the snapshot
+// orthogonalisation run on the host.
+// ---------------------------------------------------------------------------
+using zc = std::complex<double>;
+
+// Householder QR with column pivoting (Businger and Golub 1965; largest
+// remaining column norm first) of W (n x m, column-major), then the
+// least-squares solution of W y = c when the numerical rank is m.  Returns the
+// rank: count of |R_jj| > rank_tol * max |R_jj|.
+int pivoted_qr_solve(std::vector<zc>& W, int n, int m, std::vector<zc> c, double rank_tol, std::vector<zc>& y) {
+    std::vector<int> piv(m);
+    for (int j = 0; j < m; ++j) piv[j] = j;
+    std::vector<double> rdiag(m, 0.0);
+    for (int j = 0; j < m; ++j) {
+        // pivot: the remaining column with the largest norm below row j
+        int best = j;
+        double bn = -1.0;
+        for (int q = j; q < m; ++q) {
+            double v = 0.0;
+            for (int i = j; i < n; ++i) v += std::norm(W[(size_t)q * n + i]);
+            if (v > bn) { bn = v; best = q; }
+        }
+        if (best != j) {
+            for (int i = 0; i < n; ++i) std::swap(W[(size_t)j * n + i], W[(size_t)best * n + i]);
+            std::swap(piv[j], piv[best]);
+        }
+        zc* a = &W[(size_t)j * n];
+        double xn = 0.0;
+        for (int i = j; i < n; ++i) xn += std::norm(a[i]);
+        xn = std::sqrt(xn);
+        if (xn == 0.0) { rdiag[j] = 0.0; continue; }
+        const zc alpha = a[j] == zc(0) ? zc(-xn) : -std::exp(zc(0, std::arg(a[j]))) * xn;  // R_jj
+        // v = a - alpha e_j, H = I - 2 v v^H / (v^H v)
+        a[j] -= alpha;
+        double vv = 0.0;
+        for (int i = j; i < n; ++i) vv += std::norm(a[i]);
+        for (int q = j + 1; q < m; ++q) {
+            zc* b = &W[(size_t)q * n];
+            zc d = 0.0;
+            for (int i = j; i < n; ++i) d += std::conj(a[i]) * b[i];
+            const zc f = 2.0 * d / vv;
+            for (int i = j; i < n; ++i) b[i] -= f * a[i];
+        }
+        zc d = 0.0;
+        for (int i = j; i < n; ++i) d += std::conj(a[i]) * c[i];
+        const zc f = 2.0 * d / vv;
+        for (int i = j; i < n; ++i) c[i] -= f * a[i];
+        rdiag[j] = std::abs(alpha);
+        a[j] = alpha;  // R_jj; rows above j of later columns hold R
+    }
+    double mx = 0.0;
+    for (double v : rdiag) mx = std::max(mx, v);
+    int rank = 0;
+    for (double v : rdiag)
+        if (v > rank_tol * mx) ++rank;
+    if (rank < m) return rank;
+    // back substitution R z = (Q^H c)[0:m], then y[piv[j]] = z[j]
+    std::vector<zc> z(m);
+    for (int j = m - 1; j >= 0; --j) {
+        zc acc = c[j];
+        for (int q = j + 1; q < m; ++q) acc -= W[(size_t)q * n + j] * z[q];
+        z[j] = acc / W[(size_t)j * n + j];
+    }
+    y.assign(m, 0.0);
+    for (int j = 0; j < m; ++j) y[piv[j]] = z[j];
+    return rank;
+}
+
+// Snapshot space of the training solves (sweeps.jl:115-147): two passes of
+// Gram-Schmidt against the kept columns; kept when the remainder of the
+// normalised snapshot exceeds rank_tol.
+bool add_snapshot(std::vector<std::vector<zc>>& V, int capacity, const std::vector<zc>& x, double rank_tol) {
+    if ((int)V.size() >= capacity) return false;
+    double mag = 0.0;
+    for (const zc& v : x) mag += std::norm(v);
+    mag = std::sqrt(mag);
+    if (!std::isfinite(mag)) throw BemError(BEM_ERR_ARGUMENT, "snapshot norm overflow");
+    if (mag == 0.0) return false;
+    std::vector<zc> v(x.size());
+    for (size_t i = 0; i < x.size(); ++i) v[i] = x[i] / mag;
+    for (int pass = 0; pass < 2; ++pass)
+        for (const auto& q : V) {
+            zc d = 0.0;
+            for (size_t i = 0; i < v.size(); ++i) d += std::conj(q[i]) * v[i];
+            for (size_t i = 0; i < v.size(); ++i) v[i] -= d * q[i];
+        }
+    double rem = 0.0;
+    for (const zc& e : v) rem += std::norm(e);
+    rem = std::sqrt(rem);
+    if (!(rem > rank_tol)) return false;
+    for (zc& e : v) e /= rem;
+    V.push_back(std::move(v));
+    return true;
+}
+
+double rel_or_inf(double r, double b) { return b == 0.0 ? (r == 0.0 ? 0.0 : INFINITY) : r / b; }
+
+// One equation of the reduced sweep at wavenumber prob.k: returns the solution
+// x (host) and fills the report and the backscatter amplitude.
+std::vector<zc> reduced_equation(bem_ctx ctx, bem_mesh mesh, const bem_problem& prob, const double* ddir,
+                                 const double* dback, const std::vector<std::vector<zc>>& V, double tol,
+                                 double rank_tol, bem_reduced_report& rep, double* amp) {
+    Lane* L = ctx->lane0();
+    hipStream_t s = L->s;
+    bem_operator op = nullptr;
+    operator_create(ctx, L, mesh, &prob, &op, false, true);
+    std::vector<void*> tmp;
+    try {
+        const int n = op->n, nt = op->nt;
+        const int m = (int)V.size();
+        build_rhs(op, ddir);
+        double* rw = (double*)op_alloc(op, sizeof(double) * n);
+        launch_row_stats<double>(s, (const z64*)op->A, n, n, rw, op->vden);
+        launch_fix_scale(s, n, rw);
+        std::vector<double> hr(n);
+        std::vector<zc> hb(n), x(n, 0.0);
+        HIPCHECK(hipMemcpyAsync(hr.data(), rw, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(hb.data(), op->vb, sizeof(z64) * n, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        for (double v : hr)
+            if (!std::isfinite(v)) throw BemError(BEM_ERR_ARGUMENT, "row norm overflow");
+        double bnorm = 0.0;
+        for (const zc& v : hb) bnorm += std::norm(v);
+        rep.reason = m == 0 ? BEM_REDUCED_EMPTY : BEM_REDUCED_RESIDUAL;
+        rep.rank = 0;
+        if (bnorm != 0.0 && m > 0) {
+            // images A V / rows, recomputed at this frequency
+            z64* dV = (z64*)ctx->alloc(sizeof(z64) * (size_t)n * m);
+            tmp.push_back(dV);
+            z64* dW = (z64*)ctx->alloc(sizeof(z64) * (size_t)n * m);
+            tmp.push_back(dW);
+            for (int c = 0; c < m; ++c)
+                HIPCHECK(hipMemcpyAsync(dV + (size_t)c * n, V[c].data(), sizeof(z64) * n, hipMemcpyHostToDevice, s));
+            for (int c = 0; c < m; ++c)
+                launch_matvec<double>(s, (const z64*)op->A, n, n, n, dV + (size_t)c * n, dW + (size_t)c * n, nullptr,
+                                      nullptr);
+            HIPCHECK(hipGetLastError());
+            std::vector<zc> W((size_t)n * m);
+            HIPCHECK(hipMemcpyAsync(W.data(), dW, sizeof(z64) * n * m, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            bool finite = true;
+            for (int c = 0; c < m; ++c)
+                for (int i = 0; i < n; ++i) {
+                    zc& w = W[(size_t)c * n + i];
+                    w /= hr[i];
+                    finite = finite && std::isfinite(w.real()) && std::isfinite(w.imag());
+                }
+            if (finite) {
+                std::vector<zc> cvec(n), y;
+                for (int i = 0; i < n; ++i) cvec[i] = hb[i] / hr[i];
+                rep.rank = pivoted_qr_solve(W, n, m, cvec, rank_tol, y);
+                if (rep.rank == m) {
+                    for (int c = 0; c < m; ++c)
+                        for (int i = 0; i < n; ++i) x[i] += V[c][i] * y[c];
+                } else {
+                    rep.reason = BEM_REDUCED_RANK;
+                }
+            } else {
+                rep.reason = BEM_REDUCED_NONFINITE;
+            }
+        }
+        auto residuals = [&](double& orig, double& scaled) {
+            launch_matvec<double>(s, (const z64*)op->A, n, n, n, op->vx, op->vr, op->vb, nullptr);
+            launch_norm_stats(s, n, op->vr, op->vb, rw, op->red);
+            std::vector<double> h(4 * kRedBlocks);
+            HIPCHECK(hipMemcpyAsync(h.data(), op->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            double ss[4] = {0, 0, 0, 0};
+            for (int b = 0; b < kRedBlocks; ++b)
+                for (int q = 0; q < 4; ++q) ss[q] += h[4 * b + q];
+            orig = rel_or_inf(std::sqrt(ss[0]), std::sqrt(ss[1]));
+            scaled = rel_or_inf(std::sqrt(ss[2]), std::sqrt(ss[3]));
+        };
+        HIPCHECK(hipMemcpyAsync(op->vx, x.data(), sizeof(z64) * n, hipMemcpyHostToDevice, s));
+        residuals(rep.projected_original, rep.projected_scaled);
+        bool xfinite = true;
+        for (const zc& v : x) xfinite = xfinite && std::isfinite(v.real()) && std::isfinite(v.imag());
+        const bool accepted =
+            xfinite && std::max(rep.projected_original, rep.projected_scaled) <= tol;
+        if (accepted) {
+            rep.reduced = 1;
+            rep.reason = BEM_REDUCED_ACCEPTED;
+            rep.residual_original = rep.projected_original;
+            rep.residual_scaled = rep.projected_scaled;
+        } else {
+            if (!xfinite) rep.reason = BEM_REDUCED_NONFINITE;
+            rep.reduced = 0;
+            operator_factor(op);
+            if (!correction(op, op->vb, op->vx, 0, nullptr))
+                throw BemError(BEM_ERR_ARGUMENT, "right-hand side must be finite");
+            residuals(rep.residual_original, rep.residual_scaled);
+            if (!(std::max(rep.residual_original, rep.residual_scaled) <= tol))
+                throw BemError(BEM_ERR_NUMERIC, "full solve failed current-equation residual acceptance");
+            HIPCHECK(hipMemcpyAsync(x.data(), op->vx, sizeof(z64) * n, hipMemcpyDeviceToHost, s));
+        }
+        // backscatter amplitude of the CBIE traces x = [p_scat; dp_scat/dn] (far_field, full.jl:509-519)
+        z64* part = (z64*)ctx->alloc(sizeof(z64) * kFarBlocks);
+        tmp.push_back(part);
+        z64* fo = (z64*)ctx->alloc(sizeof(z64));
+        tmp.push_back(fo);
+        launch_far_field(s, mesh->d, prob.k, 1, 1, dback, op->vx, op->vx + nt, part, fo);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(amp, fo, sizeof(z64), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        for (void* p : tmp) ctx->release(p);
+        tmp.clear();
+        op_free(op);
+        return x;
+    } catch (...) {
+        hipStreamSynchronize(s);
+        for (void* p : tmp) ctx->release(p);
+        op_free(op);
+        throw;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Fixed device scratch of bem_far_field_noalloc: the staged directions, the
 // per-block partial sums and the amplitudes of at most kNoAllocMaxDirs
 // directions, laid out at fixed offsets.  One copy per device lives for the
@@ -2618,6 +2874,82 @@ int bem_pressure(bem_solution sol, int32_t r, int64_t npts, const double* points
             throw;
         }
         ctx->release(dp); ctx->release(dout); ctx->release(dreg); ctx->release(dwind);
+    });
+}
+
+void bem_reduced_defaults(bem_reduced_options* o) {
+    if (!o) return;
+    o->tolerance = 1e-10;
+    o->rank_tolerance = 1e-12;
+    o->capacity = 16;
+    o->maxbytes = 64ll * 1024 * 1024;
+}
+
+int bem_reduced_sweep(bem_ctx ctx, bem_mesh mesh, double g, double h, int32_t n_train, const double* train_k,
+                      int32_t n_query, const double* query_k, double beta, double alpha,
+                      const bem_reduced_options* opts, double* amplitudes, bem_reduced_report* train,
+                      bem_reduced_report* query, int32_t* basis_rank) {
+    if (!ctx || !mesh || !train_k || !query_k || !opts || !amplitudes) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> gd(ctx->mu);
+    return guarded(ctx, [&] {
+        if (n_train <= 0 || n_query <= 0)
+            throw BemError(BEM_ERR_ARGUMENT, "query and training frequencies must be nonempty, finite and positive");
+        for (int i = 0; i < n_train; ++i)
+            if (!(std::isfinite(train_k[i]) && train_k[i] > 0))
+                throw BemError(BEM_ERR_ARGUMENT, "converted wavenumbers must be finite and positive");
+        for (int i = 0; i < n_query; ++i)
+            if (!(std::isfinite(query_k[i]) && query_k[i] > 0))
+                throw BemError(BEM_ERR_ARGUMENT, "converted wavenumbers must be finite and positive");
+        if (!(std::isfinite(beta) && std::isfinite(alpha)))
+            throw BemError(BEM_ERR_ARGUMENT, "incidence angles must be finite");
+        const double tol = opts->tolerance, rtol = opts->rank_tolerance;
+        if (!(std::isfinite(tol) && tol > 0 && tol < 1 && std::isfinite(rtol) && rtol > 0 && rtol < 1))
+            throw BemError(BEM_ERR_ARGUMENT, "residual and rank tolerances must lie in (0, 1)");
+        if (opts->capacity < 0 || opts->maxbytes < 0) throw BemError(BEM_ERR_ARGUMENT, "negative storage bound");
+        bem_problem prob;
+        bem_problem_defaults(&prob, BEM_FLUID);
+        prob.formulation = BEM_CBIE;
+        prob.precision = BEM_DOUBLE;
+        prob.density_contrast = g;
+        prob.soundspeed_contrast = h;
+        const int64_t n = 2 * mesh->d.nt;
+        const int capacity = (int)std::min<int64_t>(
+            std::min<int64_t>(opts->capacity, n), opts->maxbytes / (int64_t)sizeof(z64) / n);
+        hipStream_t s = ctx->lane0()->s;
+        double d[6];
+        direction(beta, alpha, d);
+        for (int c = 0; c < 3; ++c) d[3 + c] = -d[c];
+        double* dd = (double*)ctx->alloc(sizeof(double) * 6);
+        try {
+            HIPCHECK(hipMemcpyAsync(dd, d, sizeof d, hipMemcpyHostToDevice, s));
+            std::vector<std::vector<zc>> V;
+            const std::vector<std::vector<zc>> empty;
+            for (int i = 0; i < n_train; ++i) {
+                bem_reduced_report rep{};
+                rep.k = train_k[i];
+                prob.k = train_k[i];
+                validate_problem(prob);
+                double amp[2];
+                std::vector<zc> x = reduced_equation(ctx, mesh, prob, dd, dd + 3, empty, tol, rtol, rep, amp);
+                rep.kept = add_snapshot(V, capacity, x, rtol) ? 1 : 0;
+                rep.rank = (int)V.size();
+                if (train) train[i] = rep;
+            }
+            for (int i = 0; i < n_query; ++i) {
+                bem_reduced_report rep{};
+                rep.k = query_k[i];
+                prob.k = query_k[i];
+                validate_problem(prob);
+                reduced_equation(ctx, mesh, prob, dd, dd + 3, V, tol, rtol, rep, amplitudes + 2 * i);
+                if (query) query[i] = rep;
+            }
+            if (basis_rank) *basis_rank = (int32_t)V.size();
+        } catch (...) {
+            hipStreamSynchronize(s);
+            ctx->release(dd);
+            throw;
+        }
+        ctx->release(dd);
     });
 }
 

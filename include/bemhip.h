@@ -246,6 +246,48 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh *meshes, const bem_problem 
               const double *angles, double *amplitudes, double *latency_ms,
               bem_diagnostics *diagnostics);
 
+/* ---- reduced-basis frequency sweep (src/postprocessing/sweeps.jl:113-306) --
+ * frequency_sweep(surface, material::FluidFilled, frequencies, c;
+ * training_frequencies, ...): dense FP64 CBIE fluid equations at each
+ * wavenumber.  Training solves (full FP64 solves) fill an orthonormal snapshot
+ * space of at most min(capacity, 2N, maxbytes / (16 * 2N)) columns, which is
+ * then frozen; every query assembles its own equations, solves the
+ * least-squares problem on the row-scaled images A V / r by pivoted QR, and
+ * accepts it when both the original and the row-scaled relative residuals are
+ * <= tolerance; otherwise (empty basis, rank loss, non-finite values or a
+ * larger residual) it takes the full equilibrated LU solve, which must meet
+ * the same residual (else BEM_ERR_NUMERIC).  Queries never enrich the basis.
+ * amplitudes: backscatter f(-d) per query (2 doubles each). */
+#define BEM_REDUCED_ACCEPTED 0
+#define BEM_REDUCED_EMPTY 1
+#define BEM_REDUCED_RESIDUAL 2
+#define BEM_REDUCED_RANK 3
+#define BEM_REDUCED_NONFINITE 4
+
+typedef struct {
+    double tolerance;      /* relative residual acceptance (1e-10)               */
+    double rank_tolerance; /* QR rank cutoff and snapshot remainder (1e-12)      */
+    int32_t capacity;      /* basis columns (16)                                 */
+    int64_t maxbytes;      /* basis storage bound in bytes (64 MiB)              */
+} bem_reduced_options;
+
+typedef struct {
+    double k;
+    int32_t reduced;  /* 1: the basis solution was accepted                      */
+    int32_t reason;   /* BEM_REDUCED_*                                           */
+    int32_t rank;     /* query: rank of the images; training: basis size after   */
+    int32_t kept;     /* training: the snapshot enlarged the basis               */
+    double projected_original, projected_scaled; /* residuals of the basis solution */
+    double residual_original, residual_scaled;   /* residuals of the returned one   */
+} bem_reduced_report;
+
+void bem_reduced_defaults(bem_reduced_options *options);
+int bem_reduced_sweep(bem_ctx ctx, bem_mesh mesh, double density_contrast, double soundspeed_contrast,
+                      int32_t n_train, const double *train_k, int32_t n_query, const double *query_k,
+                      double incidence_angle, double incidence_azimuth, const bem_reduced_options *options,
+                      double *amplitudes, bem_reduced_report *train, bem_reduced_report *query,
+                      int32_t *basis_rank);
+
 /* ---- serving: continuous request loop -------------------------------------
  * The reference has no server; its unit of work is the blocking solve that
  * `bem(surface, bc, k; incidence_angle, incidence_azimuth)` performs

@@ -490,3 +490,60 @@ def test_oracle_pressure_matched_medium_and_winding():
     d = O.direction(0.4)
     assert np.all(ri == 1) and np.all(np.abs(wi - 1.0) < 1e-9)
     assert np.max(np.abs(vi - np.exp(1j * k * in_pts @ d))) < 5e-3
+
+
+# ---------------------------------------------------------------------------
+# reduced-basis frequency sweep (src/postprocessing/sweeps.jl:113-306)
+# ---------------------------------------------------------------------------
+def _direct_cbie(m, g, h, k, beta, alpha):
+    from oracle import reduced as RB
+    A, b = RB.equations(m, g, h, k, beta, alpha)
+    x = RB.full_solve(A, b)
+    return m.far_field(k, -O.direction(beta, alpha), x[:m.n], x[m.n:])[0]
+
+
+def test_reduced_sweep_reference_scenario():
+    """test/basic/sampling.jl:117-139 on the flat-panel unit sphere: k = 0.6,
+    0.7, 1.2, 0.7 with training 0.6, 0.8, g = 1.05, h = 1.02, incidence
+    (0.7, 0.3): basis rank 2, the training frequency reduces, at least one
+    query falls back, every returned residual <= 1e-10 and the amplitudes
+    equal direct CBIE solves to 1e-8."""
+    from oracle import reduced as RB
+    nodes, tri = meshes.sphere(1.0, 2)
+    m = O.Mesh(nodes, tri)
+    ks, kt = [0.6, 0.7, 1.2, 0.7], [0.6, 0.8]
+    amps, rep = RB.reduced_sweep(m, 1.05, 1.02, ks, kt, 0.7, 0.3)
+    direct = np.array([_direct_cbie(m, 1.05, 1.02, k, 0.7, 0.3) for k in ks])
+    assert rep["basis_rank"] == 2
+    assert rep["queries"][0]["reduced"]
+    assert any(not q["reduced"] for q in rep["queries"])
+    assert all(max(q["residual"].values()) <= 1e-10 for q in rep["queries"])
+    assert np.max(np.abs(amps - direct) / np.abs(direct)) < 1e-8
+    # empty storage: every query is an independently validated full solve
+    amps0, rep0 = RB.reduced_sweep(m, 1.05, 1.02, ks, kt, 0.7, 0.3, maxbytes=0)
+    assert rep0["basis_rank"] == 0
+    assert all(not q["reduced"] and q["reason"] == "empty" for q in rep0["queries"])
+    assert np.max(np.abs(amps0 - direct) / np.abs(direct)) < 1e-12
+
+
+def test_reduced_snapshot_space_and_least_squares():
+    """The snapshot space keeps orthonormal columns and rejects a dependent
+    snapshot; a right-hand side whose solution lies in the basis is solved
+    by the reduced path exactly (rank = basis size, residual ~ eps)."""
+    from oracle import reduced as RB
+    rng = np.random.default_rng(4)
+    n = 40
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 8 * np.eye(n)
+    V = []
+    cap = RB.capacity(n, 16, 64 * 1024 ** 2)
+    xs = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(3)]
+    for x in xs:
+        assert RB.add_snapshot(V, cap, x)
+    assert not RB.add_snapshot(V, cap, 2 * xs[0] - 1j * xs[2])
+    B = np.stack(V, axis=1)
+    assert np.allclose(B.conj().T @ B, np.eye(3), atol=1e-13)
+    xt = B @ np.array([1.0, -2j, 0.5])
+    res = RB.solve_reduced(A, A @ xt, B)
+    assert res["reduced"] and res["rank"] == 3
+    assert np.linalg.norm(res["x"] - xt) < 1e-12 * np.linalg.norm(xt)
+    assert RB.capacity(n, 16, 0) == 0 and RB.capacity(n, 2, 10 ** 9) == 2
