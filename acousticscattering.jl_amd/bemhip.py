@@ -107,7 +107,8 @@ class ServerStats(ctypes.Structure):
 class HMatrixOptions(ctypes.Structure):
     _fields_ = [("leaf_size", ctypes.c_int32), ("eta", ctypes.c_double), ("aca_tolerance", ctypes.c_double),
                 ("max_rank", ctypes.c_int32), ("gmres_restart", ctypes.c_int32), ("gmres_maxiter", ctypes.c_int32),
-                ("gmres_tolerance", ctypes.c_double), ("preconditioner", ctypes.c_int32)]
+                ("gmres_tolerance", ctypes.c_double), ("preconditioner", ctypes.c_int32),
+                ("recycle_dimension", ctypes.c_int32)]
 
 
 class HMatrixInfo(ctypes.Structure):
@@ -965,7 +966,8 @@ class HMatrix:
     complex FP32; solve() runs batched FP64 GMRES for many incidences."""
 
     def __init__(self, surface, boundary, k, *, formulation=None, leaf_size=64, eta=2.0, aca_tolerance=1e-6,
-                 max_rank=256, gmres_restart=50, gmres_maxiter=1000, gmres_tolerance=1e-8, preconditioner=True):
+                 max_rank=256, gmres_restart=50, gmres_maxiter=1000, gmres_tolerance=1e-8, preconditioner="twolevel",
+                 recycle_dimension=8):
         if not isinstance(boundary, (Rigid, FluidFilled)):
             raise ValueError("the hierarchical path solves rigid surfaces and fluid bodies")
         if not (math.isfinite(k) and k > 0):
@@ -974,8 +976,13 @@ class HMatrix:
         self.ctx = surface.ctx
         self.k = float(k)
         self.problem = make_problem(boundary, k, formulation=formulation)
+        prec = {False: 0, None: 0, "none": 0, True: 1, "block_jacobi": 1, "twolevel": 2}
+        if preconditioner not in prec:
+            raise ValueError("preconditioner must be :none, :block_jacobi or :twolevel")
+        if recycle_dimension < 0:
+            raise ValueError("recycle_dimension must be nonnegative")
         o = HMatrixOptions(int(leaf_size), float(eta), float(aca_tolerance), int(max_rank), int(gmres_restart),
-                           int(gmres_maxiter), float(gmres_tolerance), 1 if preconditioner else 0)
+                           int(gmres_maxiter), float(gmres_tolerance), prec[preconditioner], int(recycle_dimension))
         h = _vp()
         self.ctx.check(self.ctx.lib.bem_hmatrix_create(self.ctx._h, surface._h, ctypes.byref(self.problem),
                                                        ctypes.byref(o), ctypes.byref(h)))

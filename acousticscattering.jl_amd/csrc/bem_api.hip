@@ -1524,6 +1524,15 @@ struct bem_hmatrix_s {
     int64_t nnear = 0, nself = 0;
     HOp A, B;        // system and right-hand-side operators
     z64* minv = nullptr;
+    // two-level preconditioner: orthonormal coarse basis Q (n x nq, tree
+    // order, column-major), Cc = Q - M^{-1} A Q and E^{-1} = (Q^H A Q)^{-1}
+    int nq = 0;
+    z64 *cq = nullptr, *cc = nullptr, *einv = nullptr;
+    // recycled solutions (GMRES warm starts across solves): U and images C = A U
+    // with C^H C = I, device (n x rcap) and host copies
+    int rcap = 0, rcount = 0;
+    z64 *ru = nullptr, *rc = nullptr;
+    std::vector<std::complex<double>> hru, hrc;
     bem_hmatrix_info info{};
 };
 
@@ -1845,6 +1854,8 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
     HIPCHECK(hipStreamSynchronize(s));
 }
 
+void hm_coarse(bem_hmatrix h);
+
 void hm_build(bem_hmatrix h) {
     const bool bm = h->prob.formulation == BEM_BURTON_MILLER;
     hm_plan(h);
@@ -1855,13 +1866,20 @@ void hm_build(bem_hmatrix h) {
         hm_operator(h, h->A, bm ? HK_BM : HK_CBIE);
         hm_operator(h, h->B, bm ? HK_BM_RHS : HK_CBIE_RHS);
     }
-    if (h->opt.preconditioner) {
+    if (h->opt.preconditioner >= 1) {
         h->minv = (z64*)hm_alloc(h, sizeof(z64) * 4096 * (size_t)h->nleaf);
         int* st = (int*)hm_alloc(h, sizeof(int));
         launch_fill_int(h->lane->s, st, 1, 0);
         launch_hjacobi_inv(h->lane->s, h->A.L, h->A.dblk, h->A.dval, h->minv, st);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipStreamSynchronize(h->lane->s));
+    }
+    if (h->opt.preconditioner == 2) hm_coarse(h);
+    const int64_t rbytes = 64ll * 1024 * 1024 / (32 * h->n);
+    h->rcap = (int)std::min<int64_t>(std::min<int64_t>(h->opt.recycle_dimension, h->n), rbytes);
+    if (h->rcap > 0) {
+        h->ru = (z64*)hm_alloc(h, sizeof(z64) * h->n * h->rcap);
+        h->rc = (z64*)hm_alloc(h, sizeof(z64) * h->n * h->rcap);
     }
     h->info.assembly_ms = h->A.assembly_ms + h->B.assembly_ms;
     h->info.aca_ms = h->A.aca_ms + h->B.aca_ms;
@@ -1890,14 +1908,157 @@ void hm_apply(bem_hmatrix h, HOp& op, const z64* X, int nr, z64* Y) {
     prof_end(&h->lane->pl, h->lane->s, C_HROWS, flops - pf, bytes - pb);
 }
 
-// A M^{-1} X (M^{-1} = block Jacobi or identity); Z is scratch
-void hm_apply_prec(bem_hmatrix h, const z64* X, int nr, z64* Z, z64* Y) {
-    if (h->minv) {
-        launch_hjacobi_apply(h->lane->s, h->A.L, h->minv, X, nr, Z);
-        hm_apply(h, h->A, Z, nr, Y);
-    } else {
-        hm_apply(h, h->A, X, nr, Y);
+// Y = P^{-1} X for the right preconditioner: identity, block Jacobi M^{-1},
+// or the two-level form of the reference's coarse preconditioner
+// (compression/fluid.jl:166-181,274-289):
+//   P^{-1} x = M^{-1} x + (Q - M^{-1} A Q) E^{-1} Q^H x,  E = Q^H A Q.
+// `work` holds 2 * nq * nr + kGmPartial * nq * nr complex doubles.
+constexpr int kGmPartial = 512;
+void hm_prec(bem_hmatrix h, const z64* X, int nr, z64* Y, z64* work) {
+    hipStream_t s = h->lane->s;
+    if (!h->minv) {
+        HIPCHECK(hipMemcpyAsync(Y, X, sizeof(z64) * h->n * nr, hipMemcpyDeviceToDevice, s));
+        return;
     }
+    launch_hjacobi_apply(s, h->A.L, h->minv, X, nr, Y);
+    if (h->nq > 0) {
+        z64* w = work;
+        z64* w2 = work + (size_t)h->nq * nr;
+        z64* part = work + 2 * (size_t)h->nq * nr;
+        launch_basis_dots(s, h->n, nr, h->nq, h->cq, X, part, w);
+        launch_small_apply(s, h->nq, nr, h->einv, w, w2);
+        launch_basis_combine(s, h->n, nr, h->nq, h->cc, w2, Y);
+    }
+}
+size_t hm_prec_work(bem_hmatrix h, int nr) { return (size_t)(2 + kGmPartial) * std::max(1, h->nq) * nr; }
+
+// A P^{-1} X; Z receives P^{-1} X
+void hm_apply_prec(bem_hmatrix h, const z64* X, int nr, z64* Z, z64* Y, z64* work) {
+    hm_prec(h, X, nr, Z, work);
+    hm_apply(h, h->A, Z, nr, Y);
+}
+
+// Coarse space of the two-level preconditioner: the regular harmonic
+// polynomials of degree <= 2 (the span of the reference's nine regular
+// Helmholtz traces at k = 0, quadrature/fluid.jl:19-61, sampled at the panel
+// centroids relative to their centre and radius) on each trace's unknowns,
+// 9 columns for a rigid surface and 18 for a fluid body
+// (compression/fluid.jl:193-205), orthonormalised.
+void hm_coarse(bem_hmatrix h) {
+    hipStream_t s = h->lane->s;
+    const int64_t n = h->n;
+    const int nt = (int)h->nt;
+    const int nq = h->fluid ? 18 : 9;
+    std::vector<double> c(3 * (size_t)nt);
+    HIPCHECK(hipMemcpyAsync(c.data(), h->mesh->d.cen, sizeof(double) * 3 * nt, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    double ctr[3] = {0, 0, 0};
+    for (int i = 0; i < nt; ++i)
+        for (int d = 0; d < 3; ++d) ctr[d] += c[(size_t)d * nt + i] / nt;
+    double rad = 0;
+    for (int i = 0; i < nt; ++i) {
+        double r2 = 0;
+        for (int d = 0; d < 3; ++d) r2 += (c[(size_t)d * nt + i] - ctr[d]) * (c[(size_t)d * nt + i] - ctr[d]);
+        rad = std::max(rad, std::sqrt(r2));
+    }
+    if (!(rad > 0)) rad = 1.0;
+    std::vector<std::complex<double>> Q((size_t)n * nq, 0.0);
+    for (int64_t pos = 0; pos < n; ++pos) {
+        const int idx = h->hperm[pos];
+        const int panel = idx % nt;
+        const int block = idx / nt;  // 0 pressure, 1 normal derivative (fluid)
+        const double x = (c[panel] - ctr[0]) / rad, y = (c[(size_t)nt + panel] - ctr[1]) / rad,
+                     z = (c[2 * (size_t)nt + panel] - ctr[2]) / rad;
+        const double hv[9] = {1.0, x, y, z, x * y, y * z, x * z, x * x - y * y, 2 * z * z - x * x - y * y};
+        for (int j = 0; j < 9; ++j) Q[(size_t)(9 * block + j) * n + pos] = hv[j];
+    }
+    // orthonormal columns: modified Gram-Schmidt, twice
+    for (int j = 0; j < nq; ++j) {
+        std::complex<double>* qj = &Q[(size_t)j * n];
+        for (int pass = 0; pass < 2; ++pass)
+            for (int i = 0; i < j; ++i) {
+                const std::complex<double>* qi = &Q[(size_t)i * n];
+                std::complex<double> d = 0;
+                for (int64_t p = 0; p < n; ++p) d += std::conj(qi[p]) * qj[p];
+                for (int64_t p = 0; p < n; ++p) qj[p] -= d * qi[p];
+            }
+        double nn = 0;
+        for (int64_t p = 0; p < n; ++p) nn += std::norm(qj[p]);
+        nn = std::sqrt(nn);
+        if (!(nn > 0)) throw BemError(BEM_ERR_NUMERIC, "degenerate coarse space");
+        for (int64_t p = 0; p < n; ++p) qj[p] /= nn;
+    }
+    // A Q and M^{-1} A Q in the batched vector layout [pos * nq + j]
+    std::vector<z64> X((size_t)n * nq);
+    for (int64_t p = 0; p < n; ++p)
+        for (int j = 0; j < nq; ++j) X[p * nq + j] = z64{Q[(size_t)j * n + p].real(), Q[(size_t)j * n + p].imag()};
+    z64* dX = (z64*)h->ctx->alloc(sizeof(z64) * n * nq);
+    z64* dY = nullptr;
+    z64* dZ = nullptr;
+    try {
+        dY = (z64*)h->ctx->alloc(sizeof(z64) * n * nq);
+        dZ = (z64*)h->ctx->alloc(sizeof(z64) * n * nq);
+        HIPCHECK(hipMemcpyAsync(dX, X.data(), sizeof(z64) * n * nq, hipMemcpyHostToDevice, s));
+        hm_apply(h, h->A, dX, nq, dY);
+        launch_hjacobi_apply(s, h->A.L, h->minv, dY, nq, dZ);
+        std::vector<z64> AQ((size_t)n * nq), MAQ((size_t)n * nq);
+        HIPCHECK(hipMemcpyAsync(AQ.data(), dY, sizeof(z64) * n * nq, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(MAQ.data(), dZ, sizeof(z64) * n * nq, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        // E = Q^H A Q and its inverse (Gauss-Jordan with partial pivoting)
+        std::vector<std::complex<double>> E((size_t)nq * nq, 0.0), Ei((size_t)nq * nq, 0.0);
+        for (int i = 0; i < nq; ++i)
+            for (int j = 0; j < nq; ++j) {
+                std::complex<double> d = 0;
+                for (int64_t p = 0; p < n; ++p)
+                    d += std::conj(Q[(size_t)i * n + p]) * std::complex<double>(AQ[p * nq + j].x, AQ[p * nq + j].y);
+                E[(size_t)i * nq + j] = d;
+            }
+        for (int i = 0; i < nq; ++i) Ei[(size_t)i * nq + i] = 1.0;
+        for (int k = 0; k < nq; ++k) {
+            int pr = k;
+            for (int i = k + 1; i < nq; ++i)
+                if (std::abs(E[(size_t)i * nq + k]) > std::abs(E[(size_t)pr * nq + k])) pr = i;
+            if (!(std::abs(E[(size_t)pr * nq + k]) > 0)) throw BemError(BEM_ERR_NUMERIC, "singular coarse operator");
+            if (pr != k)
+                for (int j = 0; j < nq; ++j) {
+                    std::swap(E[(size_t)pr * nq + j], E[(size_t)k * nq + j]);
+                    std::swap(Ei[(size_t)pr * nq + j], Ei[(size_t)k * nq + j]);
+                }
+            const std::complex<double> ip = 1.0 / E[(size_t)k * nq + k];
+            for (int j = 0; j < nq; ++j) {
+                E[(size_t)k * nq + j] *= ip;
+                Ei[(size_t)k * nq + j] *= ip;
+            }
+            for (int i = 0; i < nq; ++i)
+                if (i != k) {
+                    const std::complex<double> f = E[(size_t)i * nq + k];
+                    if (f == 0.0) continue;
+                    for (int j = 0; j < nq; ++j) {
+                        E[(size_t)i * nq + j] -= f * E[(size_t)k * nq + j];
+                        Ei[(size_t)i * nq + j] -= f * Ei[(size_t)k * nq + j];
+                    }
+                }
+        }
+        std::vector<z64> hq((size_t)n * nq), hc((size_t)n * nq), he((size_t)nq * nq);
+        for (int j = 0; j < nq; ++j)
+            for (int64_t p = 0; p < n; ++p) {
+                const std::complex<double> q = Q[(size_t)j * n + p];
+                hq[(size_t)j * n + p] = z64{q.real(), q.imag()};
+                hc[(size_t)j * n + p] = z64{q.real() - MAQ[p * nq + j].x, q.imag() - MAQ[p * nq + j].y};
+            }
+        for (size_t e = 0; e < he.size(); ++e) he[e] = z64{Ei[e].real(), Ei[e].imag()};
+        h->cq = hm_upload(h, hq);
+        h->cc = hm_upload(h, hc);
+        h->einv = hm_upload(h, he);
+        h->nq = nq;
+        HIPCHECK(hipStreamSynchronize(s));
+    } catch (...) {
+        hipStreamSynchronize(s);
+        h->ctx->release(dX); h->ctx->release(dY); h->ctx->release(dZ);
+        throw;
+    }
+    h->ctx->release(dX); h->ctx->release(dY); h->ctx->release(dZ);
 }
 
 __global__ void k_inv_norm(int nr, const z64* sq, double* scale, double* norm) {
@@ -1910,7 +2071,9 @@ __global__ void k_inv_norm(int nr, const z64* sq, double* scale, double* norm) {
 
 // Restarted GMRES (right-preconditioned, CGS2 Arnoldi, complex Givens) for nr
 // right-hand sides B at once, all in FP64.  X receives the solution.
-void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagnostics>& dg) {
+// target: absolute residual target per column (null: tolerance * ||B||).
+void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagnostics>& dg,
+              const double* target = nullptr) {
     hipStream_t s = h->lane->s;
     const int64_t n = h->n;
     const int m = std::max(1, h->opt.gmres_restart);
@@ -1931,7 +2094,9 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
         double* scale = (double*)al(sizeof(double) * nr);
         double* norm = (double*)al(sizeof(double) * nr);
         z64* ycoef = (z64*)al(sizeof(z64) * m * nr);
+        z64* pw = (z64*)al(sizeof(z64) * hm_prec_work(h, nr));
         launch_fill_double(s, (double*)U, 2 * vs, 0.0);
+        launch_fill_double(s, (double*)X, 2 * vs, 0.0);
         // ||b||
         std::vector<double> bn(nr), rn(nr);
         launch_gm_dots(s, n, nr, 1, B, vs, B, part, sq);
@@ -1939,6 +2104,8 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
         HIPCHECK(hipMemcpyAsync(hs.data(), sq, sizeof(z64) * nr, hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
         for (int c = 0; c < nr; ++c) bn[c] = std::sqrt(std::max(hs[c].x, 0.0));
+        std::vector<double> thr(nr);
+        for (int c = 0; c < nr; ++c) thr[c] = target ? target[c] : tol * bn[c];
         HIPCHECK(hipMemcpyAsync(R, B, sizeof(z64) * vs, hipMemcpyDeviceToDevice, s));
         int iters = 0;
         bool done = false;
@@ -1952,7 +2119,7 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
             HIPCHECK(hipMemcpyAsync(rn.data(), norm, sizeof(double) * nr, hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));
             bool all = true;
-            for (int c = 0; c < nr; ++c) all &= !(rn[c] > tol * bn[c]);
+            for (int c = 0; c < nr; ++c) all &= !(rn[c] > thr[c]);
             if (all || iters >= h->opt.gmres_maxiter) {
                 done = all;
                 break;
@@ -1963,7 +2130,7 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
             int jend = 0;
             for (int j = 0; j < m; ++j) {
                 z64* Vj = Vb + (int64_t)j * vs;
-                hm_apply_prec(h, Vj, nr, Z, W);
+                hm_apply_prec(h, Vj, nr, Z, W, pw);
                 // classical Gram-Schmidt, twice
                 prof_begin(&h->lane->pl, s, C_HORTHO);
                 launch_gm_dots(s, n, nr, j + 1, Vb, vs, W, part, h1);
@@ -2016,7 +2183,7 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
                     std::complex<double>& gj = g[(size_t)c * (m + 1) + j];
                     g[(size_t)c * (m + 1) + j + 1] = -std::conj(ss) * gj;
                     gj = cc * gj;
-                    conv &= !(std::abs(g[(size_t)c * (m + 1) + j + 1]) > tol * bn[c]);
+                    conv &= !(std::abs(g[(size_t)c * (m + 1) + j + 1]) > thr[c]);
                 }
                 if (conv || iters >= h->opt.gmres_maxiter) break;
             }
@@ -2034,8 +2201,7 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
             }
             HIPCHECK(hipMemcpyAsync(ycoef, y.data(), sizeof(z64) * y.size(), hipMemcpyHostToDevice, s));
             launch_gm_combine(s, n, nr, jend, Vb, vs, ycoef, 1.0, U);
-            if (h->minv) launch_hjacobi_apply(s, h->A.L, h->minv, U, nr, X);
-            else HIPCHECK(hipMemcpyAsync(X, U, sizeof(z64) * vs, hipMemcpyDeviceToDevice, s));
+            hm_prec(h, U, nr, X, pw);
             hm_apply(h, h->A, X, nr, W);
             launch_gm_sub(s, vs, B, W, R);
             HIPCHECK(hipGetLastError());
@@ -2050,7 +2216,7 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
             d.iterations = iters;
             d.method = 2;
             d.factor_precision = 1;
-            d.converged = !(rn[c] > tol * bn[c]) ? 1 : 0;
+            d.converged = !(rn[c] > thr[c]) ? 1 : 0;
         }
         (void)done;
     } catch (...) {
@@ -2060,6 +2226,181 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
     }
     hipStreamSynchronize(s);
     for (void* p : tmp) h->ctx->release(p);
+}
+
+// ---------------------------------------------------------------------------
+// GMRES warm-started from recycled solutions (reference compression/fluid.jl:
+// 303-419).  The hierarchical matrix keeps up to rcap accepted solutions U
+// with their images C = A U, orthonormalised in image space (C^H C = I, the
+// same transformations applied to U), so x0 = U C^H b minimises the residual
+// over the kept space.  A batch of right-hand sides starts from x0 (or from
+// zero when the space is empty or does not reduce every residual), GMRES
+// solves A d = b - A x0 to the original targets tolerance * ||b||, the true
+// residual of x0 + d is checked, and a recycled attempt that misses it is
+// repeated from zero with the space cleared.  Accepted solutions enter the
+// space while it has room.
+// ---------------------------------------------------------------------------
+void hm_norms(bem_hmatrix h, int nr, const z64* V, z64* part, z64* sq, std::vector<double>& out) {
+    hipStream_t s = h->lane->s;
+    const int64_t vs = h->n * nr;
+    launch_gm_dots(s, h->n, nr, 1, V, vs, V, part, sq);
+    std::vector<z64> hs(nr);
+    HIPCHECK(hipMemcpyAsync(hs.data(), sq, sizeof(z64) * nr, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    out.resize(nr);
+    for (int c = 0; c < nr; ++c) out[c] = std::sqrt(std::max(hs[c].x, 0.0));
+}
+
+void hm_solve_recycled(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagnostics>& dg) {
+    if (h->rcap == 0) {
+        hm_gmres(h, nr, B, X, dg);
+        return;
+    }
+    hipStream_t s = h->lane->s;
+    const int64_t n = h->n, vs = n * nr;
+    const double tol = h->opt.gmres_tolerance;
+    std::vector<void*> tmp;
+    auto al = [&](size_t b) { void* p = h->ctx->alloc(b); tmp.push_back(p); return p; };
+    try {
+        z64* R0 = (z64*)al(sizeof(z64) * vs);
+        z64* W = (z64*)al(sizeof(z64) * vs);
+        z64* D = (z64*)al(sizeof(z64) * vs);
+        z64* part = (z64*)al(sizeof(z64) * 512 * (size_t)std::max(1, h->rcap) * nr);
+        z64* coef = (z64*)al(sizeof(z64) * (size_t)std::max(1, h->rcap) * nr);
+        z64* ones = (z64*)al(sizeof(z64) * nr);
+        z64* sq = (z64*)al(sizeof(z64) * nr);
+        std::vector<z64> hones(nr, z64{1.0, 0.0});
+        HIPCHECK(hipMemcpyAsync(ones, hones.data(), sizeof(z64) * nr, hipMemcpyHostToDevice, s));
+        std::vector<double> bn, rn, tgt(nr);
+        hm_norms(h, nr, B, part, sq, bn);
+        for (int c = 0; c < nr; ++c) tgt[c] = tol * bn[c];
+        launch_fill_double(s, (double*)X, 2 * vs, 0.0);
+        auto report = [&](const std::vector<double>& r, int extra_iters) {
+            for (int c = 0; c < nr; ++c) {
+                bem_diagnostics& d = dg[c];
+                const int it = d.iterations + extra_iters;
+                std::memset(&d, 0, sizeof d);
+                d.unknown_count = n;
+                d.quadrature_nodes = n;
+                d.absolute_residual = r[c];
+                d.relative_residual = bn[c] > 0 ? r[c] / bn[c] : r[c];
+                d.iterations = it;
+                d.method = 2;
+                d.factor_precision = 1;
+                d.converged = !(r[c] > tgt[c]) ? 1 : 0;
+            }
+        };
+        bool reused = false;
+        if (h->rcount > 0) {
+            // x0 = U C^H b and its residual
+            launch_basis_dots(s, n, nr, h->rcount, h->rc, B, part, coef);
+            launch_basis_combine(s, n, nr, h->rcount, h->ru, coef, X);
+            hm_apply(h, h->A, X, nr, W);
+            launch_gm_sub(s, vs, B, W, R0);
+            hm_norms(h, nr, R0, part, sq, rn);
+            bool all = true, shrinks = true;
+            for (int c = 0; c < nr; ++c) {
+                all &= !(rn[c] > tgt[c]);
+                shrinks &= rn[c] < bn[c] || bn[c] == 0.0;
+            }
+            if (all) {
+                for (auto& d : dg) std::memset(&d, 0, sizeof d);
+                report(rn, 0);
+                for (void* p : tmp) h->ctx->release(p);
+                return;
+            }
+            if (shrinks) {
+                reused = true;
+            } else {
+                launch_fill_double(s, (double*)X, 2 * vs, 0.0);  // the space does not fit: start over
+                h->rcount = 0;
+            }
+        }
+        if (reused) {
+            hm_gmres(h, nr, R0, D, dg, tgt.data());
+            launch_gm_combine(s, n, nr, 1, D, vs, ones, 1.0, X);  // x = x0 + d
+        } else {
+            hm_gmres(h, nr, B, X, dg, tgt.data());
+        }
+        // true residual of the returned solution
+        hm_apply(h, h->A, X, nr, W);
+        launch_gm_sub(s, vs, B, W, R0);
+        hm_norms(h, nr, R0, part, sq, rn);
+        bool accepted = true;
+        for (int c = 0; c < nr; ++c) accepted &= !(rn[c] > tgt[c]);
+        if (reused && !accepted) {
+            // a recycled attempt that misses the target is repeated from zero;
+            // its iterations count, and it must not seed later solves
+            std::vector<int> first(nr);
+            for (int c = 0; c < nr; ++c) first[c] = dg[c].iterations;
+            h->rcount = 0;
+            hm_gmres(h, nr, B, X, dg, tgt.data());
+            for (int c = 0; c < nr; ++c) dg[c].iterations += first[c];
+            hm_apply(h, h->A, X, nr, W);
+            launch_gm_sub(s, vs, B, W, R0);
+            hm_norms(h, nr, R0, part, sq, rn);
+            accepted = true;
+            for (int c = 0; c < nr; ++c) accepted &= !(rn[c] > tgt[c]);
+        }
+        report(rn, 0);
+        // accepted solutions enter the space while it has room: images
+        // orthonormalised, the same combination applied to the solutions
+        for (int c = 0; c < nr && h->rcount < h->rcap; ++c) {
+            if (rn[c] > tgt[c] || !(bn[c] > 0)) continue;
+            std::vector<z64> xu(n), xc(n);
+            // D and R0 hold n * nr >= n entries each; neither is needed any more
+            launch_gather_col(s, n, nr, c, X, D);
+            HIPCHECK(hipMemcpyAsync(xu.data(), D, sizeof(z64) * n, hipMemcpyDeviceToHost, s));
+            launch_gather_col(s, n, nr, c, W, R0);
+            HIPCHECK(hipMemcpyAsync(xc.data(), R0, sizeof(z64) * n, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            std::vector<std::complex<double>> u(n), cimg(n);
+            for (int64_t p = 0; p < n; ++p) {
+                u[p] = {xu[p].x, xu[p].y};
+                cimg[p] = {xc[p].x, xc[p].y};
+            }
+            double orig = 0;
+            for (const auto& v : cimg) orig += std::norm(v);
+            orig = std::sqrt(orig);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int j = 0; j < h->rcount; ++j) {
+                    const std::complex<double>* cj = &h->hrc[(size_t)j * n];
+                    const std::complex<double>* uj = &h->hru[(size_t)j * n];
+                    std::complex<double> d = 0;
+                    for (int64_t p = 0; p < n; ++p) d += std::conj(cj[p]) * cimg[p];
+                    for (int64_t p = 0; p < n; ++p) {
+                        cimg[p] -= d * cj[p];
+                        u[p] -= d * uj[p];
+                    }
+                }
+            double nc = 0;
+            for (const auto& v : cimg) nc += std::norm(v);
+            nc = std::sqrt(nc);
+            if (!(std::isfinite(nc) && nc > std::sqrt(2.220446049250313e-16) * orig)) continue;
+            const int j = h->rcount;
+            h->hru.resize((size_t)(j + 1) * n);
+            h->hrc.resize((size_t)(j + 1) * n);
+            for (int64_t p = 0; p < n; ++p) {
+                h->hru[(size_t)j * n + p] = u[p] / nc;
+                h->hrc[(size_t)j * n + p] = cimg[p] / nc;
+                xu[p] = z64{(u[p] / nc).real(), (u[p] / nc).imag()};
+                xc[p] = z64{(cimg[p] / nc).real(), (cimg[p] / nc).imag()};
+            }
+            HIPCHECK(hipMemcpyAsync(h->ru + (size_t)j * n, xu.data(), sizeof(z64) * n, hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(h->rc + (size_t)j * n, xc.data(), sizeof(z64) * n, hipMemcpyHostToDevice, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            h->rcount = j + 1;
+        }
+        if (h->rcount == 0) {
+            h->hru.clear();
+            h->hrc.clear();
+        }
+        for (void* p : tmp) h->ctx->release(p);
+    } catch (...) {
+        hipStreamSynchronize(s);
+        for (void* p : tmp) h->ctx->release(p);
+        throw;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3247,7 +3588,8 @@ void bem_hmatrix_defaults(bem_hmatrix_options* o) {
     o->gmres_restart = 50;
     o->gmres_maxiter = 1000;
     o->gmres_tolerance = 1e-8;
-    o->preconditioner = 1;
+    o->preconditioner = 2;
+    o->recycle_dimension = 8;
 }
 
 int bem_hmatrix_plan(int64_t n, const double* points, int32_t leaf, double eta, double near_radius, int64_t* perm,
@@ -3313,6 +3655,9 @@ int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, c
         if (o.max_rank < 1 || o.max_rank > kAcaCap) throw BemError(BEM_ERR_ARGUMENT, "max_rank must be in 1..256");
         if (o.gmres_restart < 1 || o.gmres_maxiter < 1 || !(o.gmres_tolerance > 0))
             throw BemError(BEM_ERR_ARGUMENT, "invalid GMRES options");
+        if (o.preconditioner < 0 || o.preconditioner > 2)
+            throw BemError(BEM_ERR_ARGUMENT, "preconditioner must be 0 (none), 1 (block Jacobi) or 2 (two-level)");
+        if (o.recycle_dimension < 0) throw BemError(BEM_ERR_ARGUMENT, "recycle_dimension must be nonnegative");
         auto* h = new bem_hmatrix_s();
         h->ctx = ctx;
         h->mesh = mesh;
@@ -3442,7 +3787,7 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t nrhs, const double* angles, bem_sol
                     hm_apply(h, h->B, X, nr, B);
                 }
                 dg.assign(nr, bem_diagnostics{});
-                hm_gmres(h, nr, B, X, dg);
+                hm_solve_recycled(h, nr, B, X, dg);
                 // traces of this chunk: columns c0 .. c0 + nr of P, Q
                 if (h->fluid)
                     launch_hfluid_traces(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, X,

@@ -47,6 +47,14 @@ void launch_hjacobi_apply(hipStream_t s, const HLeafs& L, const z64* minv, const
 // out[i][c] = sum_p conj(V_i[p][c]) W[p][c] for i < nb (partial: 512 * nb * nr)
 void launch_gm_dots(hipStream_t s, int64_t n, int nr, int nb, const z64* V, int64_t vstride, const z64* W,
                     z64* partial, z64* out);
+// Shared basis Q (nb columns of n): out[i][c] = Q_i^H X_c (partial: 512 * nb * nr),
+// Y_c += Q coef_c, and out = M w for a small nb x nb row-major M.
+void launch_basis_dots(hipStream_t s, int64_t n, int nr, int nb, const z64* Q, const z64* X, z64* partial,
+                       z64* out);
+void launch_basis_combine(hipStream_t s, int64_t n, int nr, int nb, const z64* Q, const z64* coef, z64* Y);
+void launch_small_apply(hipStream_t s, int nb, int nr, const z64* M, const z64* w, z64* out);
+// out = column c of X (n rows, nr columns interleaved)
+void launch_gather_col(hipStream_t s, int64_t n, int nr, int c, const z64* X, z64* out);
 // W += sign * sum_i V_i coef[i]
 void launch_gm_combine(hipStream_t s, int64_t n, int nr, int nb, const z64* V, int64_t vstride, const z64* coef,
                        double sign, z64* W);

@@ -478,6 +478,98 @@ void launch_gm_dots(hipStream_t s, int64_t n, int nr, int nb, const z64* V, int6
     hipLaunchKernelGGL(k_gm_reduce, dim3((nb * nr + 255) / 256), dim3(256), 0, s, kGmBlocks, nb, nr, partial, out);
 }
 
+// Shared-basis products for the coarse space and the recycled solutions: Q
+// holds nb columns of n entries (column-major), one set for every
+// right-hand side.  out[i][c] = sum_p conj(Q[i][p]) X[p][c] (fixed-order
+// two-pass reduction), and Y[p][c] += sum_i Q[i][p] coef[i][c].
+__global__ __launch_bounds__(256) void k_basis_dots(int64_t n, int nr, int nb, const z64* __restrict__ Q,
+                                                    const z64* __restrict__ X, z64* __restrict__ partial) {
+    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+    __shared__ z64 red[4][8][64];
+    for (int i0 = 0; i0 < nb; i0 += 8) {
+        z64 acc[8] = {};
+        if (c < nr)
+            for (int64_t p = (int64_t)blockIdx.x * 4 + g; p < n; p += (int64_t)gridDim.x * 4) {
+                const z64 x = X[p * nr + c];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (i0 + q < nb) {
+                        const z64 v = Q[(int64_t)(i0 + q) * n + p];
+                        acc[q].x += v.x * x.x + v.y * x.y;
+                        acc[q].y += v.x * x.y - v.y * x.x;
+                    }
+            }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) red[g][q][c] = acc[q];
+        __syncthreads();
+        for (int e = tid; e < 8 * 64; e += 256) {
+            const int q = e >> 6, cc = e & 63;
+            if (i0 + q < nb && cc < nr) {
+                z64 sum = red[0][q][cc];
+                for (int gg = 1; gg < 4; ++gg) {
+                    sum.x += red[gg][q][cc].x;
+                    sum.y += red[gg][q][cc].y;
+                }
+                partial[((int64_t)blockIdx.x * nb + i0 + q) * nr + cc] = sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_basis_combine(int64_t n, int nr, int nb, const z64* __restrict__ Q,
+                                                       const z64* __restrict__ coef, z64* __restrict__ Y) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * nr) return;
+    const int64_t p = e / nr;
+    const int c = (int)(e % nr);
+    double sx = 0, sy = 0;
+    for (int i = 0; i < nb; ++i) {
+        const z64 v = Q[(int64_t)i * n + p], h = coef[i * nr + c];
+        sx += v.x * h.x - v.y * h.y;
+        sy += v.x * h.y + v.y * h.x;
+    }
+    Y[e] = z64{Y[e].x + sx, Y[e].y + sy};
+}
+
+// out[i][c] = sum_j M[i][j] w[j][c] for a small nb x nb matrix M (row-major)
+__global__ void k_small_apply(int nb, int nr, const z64* __restrict__ M, const z64* __restrict__ w,
+                              z64* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * nr) return;
+    const int i = e / nr, c = e % nr;
+    double sx = 0, sy = 0;
+    for (int j = 0; j < nb; ++j) {
+        const z64 a = M[i * nb + j], b = w[j * nr + c];
+        sx += a.x * b.x - a.y * b.y;
+        sy += a.x * b.y + a.y * b.x;
+    }
+    out[e] = z64{sx, sy};
+}
+
+// out[p] = X[p * nr + c]: one right-hand side's column, contiguous
+__global__ void k_gather_col(int64_t n, int nr, int c, const z64* __restrict__ X, z64* __restrict__ out) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p < n) out[p] = X[p * nr + c];
+}
+void launch_gather_col(hipStream_t s, int64_t n, int nr, int c, const z64* X, z64* out) {
+    hipLaunchKernelGGL(k_gather_col, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, nr, c, X, out);
+}
+
+void launch_basis_dots(hipStream_t s, int64_t n, int nr, int nb, const z64* Q, const z64* X, z64* partial,
+                       z64* out) {
+    hipLaunchKernelGGL(k_basis_dots, dim3(kGmBlocks), dim3(256), 0, s, n, nr, nb, Q, X, partial);
+    hipLaunchKernelGGL(k_gm_reduce, dim3((nb * nr + 255) / 256), dim3(256), 0, s, kGmBlocks, nb, nr, partial, out);
+}
+void launch_basis_combine(hipStream_t s, int64_t n, int nr, int nb, const z64* Q, const z64* coef, z64* Y) {
+    const int64_t tot = n * nr;
+    hipLaunchKernelGGL(k_basis_combine, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nr, nb, Q, coef,
+                       Y);
+}
+void launch_small_apply(hipStream_t s, int nb, int nr, const z64* M, const z64* w, z64* out) {
+    hipLaunchKernelGGL(k_small_apply, dim3((nb * nr + 63) / 64), dim3(64), 0, s, nb, nr, M, w, out);
+}
+
 // W[p][c] += sign * sum_i V_i[p][c] coef[i][c]
 __global__ __launch_bounds__(256) void k_gm_combine(int64_t n, int nr, int nb, const z64* __restrict__ V,
                                                     int64_t vstride, const z64* __restrict__ coef, double sign,

@@ -366,7 +366,14 @@ typedef struct {
     int32_t gmres_restart;   /* Krylov dimension per cycle (50)                       */
     int32_t gmres_maxiter;   /* total iterations (1000)                               */
     double gmres_tolerance;  /* relative FP64 residual ||b - A x|| / ||b|| (1e-8)      */
-    int32_t preconditioner;  /* 1 block Jacobi on the diagonal leaf blocks, 0 none (1) */
+    int32_t preconditioner;  /* right preconditioner: 0 none, 1 block Jacobi on the
+                                diagonal leaf blocks, 2 two-level: block Jacobi plus an
+                                orthonormal coarse space of the degree <= 2 harmonic
+                                polynomials on each trace, P^-1 = M^-1 + (I - M^-1 A) Q
+                                (Q^H A Q)^-1 Q^H (compression/fluid.jl:166-301) (2) */
+    int32_t recycle_dimension; /* solutions kept to warm-start later solves of the same
+                                matrix, bounded by 64 MiB of solution/image pairs
+                                (compression/fluid.jl:303-419); 0 disables (8) */
 } bem_hmatrix_options;
 
 typedef struct {
