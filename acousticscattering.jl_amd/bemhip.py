@@ -963,7 +963,10 @@ class HMatrix:
     src/solvers/bem/compression/) and fluid bodies in the Mueller formulation
     (compression/fluid.jl:20-106, one H-matrix over the 2N pressure and
     normal-derivative unknowns).  Cluster tree, dense leaves, ACA blocks in
-    complex FP32; solve() runs batched FP64 GMRES for many incidences."""
+    complex FP32; solve() runs batched FP64 GMRES for many incidences.  At
+    max(k, k_int) * radius <= pi/2 a fluid body's K' and H come from the
+    Calderon identities (full.jl:201-207): compressed S and D of each medium
+    over one panel tree, S^{-1} by inner GMRES on the compressed S."""
 
     def __init__(self, surface, boundary, k, *, formulation=None, leaf_size=64, eta=2.0, aca_tolerance=1e-6,
                  max_rank=256, gmres_restart=50, gmres_maxiter=1000, gmres_tolerance=1e-8, preconditioner="twolevel",

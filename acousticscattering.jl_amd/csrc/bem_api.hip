@@ -17,6 +17,7 @@
 #include <atomic>
 #include <condition_variable>
 #include <deque>
+#include <functional>
 #include <list>
 #include <map>
 #include <unordered_map>
@@ -1533,10 +1534,41 @@ struct bem_hmatrix_s {
     int rcap = 0, rcount = 0;
     z64 *ru = nullptr, *rc = nullptr;
     std::vector<std::complex<double>> hru, hrc;
+    // Low-frequency Mueller (max(k, k_int) * radius <= pi/2): the system acts
+    // on [p; q] (rows 0..N-1 and N..2N-1, each in tree order over a panel
+    // tree) through compressed S and (1/2 I - D) of both media, with K' and H
+    // from the Calderon identities applied by inner solves with S.
+    bool calderon = false;
+    int64_t inner_iters = 0;          // inner S solves' GMRES iterations (diagnostics)
+    HOp cal[4];                       // S_e, (1/2 I - D_e), S_i, (1/2 I - D_i)
+    z64 *minv_e = nullptr, *minv_i = nullptr;  // block Jacobi of S_e, S_i
+    z64* lblk = nullptr;              // inverses of the local 2m x 2m Mueller blocks (nleaf x 128 x 128)
+    int64_t nsys = 0;                 // unknowns of the system (n, or 2N on the panel tree)
+    std::vector<int> sysperm;         // unknown of each system row (nsys)
+    int* dsysperm = nullptr;
     bem_hmatrix_info info{};
 };
 
 namespace {
+
+// A linear system for the batched GMRES: y = A x and the right preconditioner
+// y = P^{-1} x on n rows x nr right-hand sides (layout [row * nr + c]), with
+// `work` scratch of work_size(nr) complex doubles for either.
+struct HSys {
+    int64_t n = 0;
+    double tol = 1e-8;
+    int restart = 50, maxiter = 1000;
+    std::function<void(const z64*, int, z64*, z64*)> apply, prec;
+    std::function<size_t(int)> work_size;
+};
+
+void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagnostics>& dg,
+              const double* target = nullptr, const HSys* sys = nullptr);
+HSys hm_system(bem_hmatrix h);
+void hm_apply(bem_hmatrix h, HOp& op, const z64* X, int nr, z64* Y);
+void hm_coarse_with(bem_hmatrix h, const std::function<void(const z64*, int, z64*)>& apply,
+                    const std::function<void(const z64*, int, z64*)>& local);
+void hm_coarse_correct(bem_hmatrix h, const z64* X, int nr, z64* Y, z64* work);
 
 void* hm_alloc(bem_hmatrix h, size_t bytes) {
     void* p = h->ctx->alloc(bytes);
@@ -1566,14 +1598,14 @@ void hm_plan(bem_hmatrix h) {
     const int64_t nt = hm.nt;
     // fluid Mueller: unknown i is the pressure and nt + i the normal
     // derivative on panel i, both clustered at the panel centroid
-    const int nq = h->fluid ? 2 : 1;
+    const int nq = (h->fluid && !h->calderon) ? 2 : 1;
     const int64_t n = nq * nt;
     h->n = n;
     h->nt = nt;
     const auto& o = h->opt;
     const double t0 = now_ms();
     std::vector<double> pts(hm.cen.begin(), hm.cen.end());
-    if (h->fluid) pts.insert(pts.end(), hm.cen.begin(), hm.cen.end());
+    if (nq == 2) pts.insert(pts.end(), hm.cen.begin(), hm.cen.end());
     // near pairs never fall in low-rank blocks (box distance > 2 hmax)
     double hmax = 0;
     for (double v : hm.hmax) hmax = std::max(hmax, v);
@@ -1637,7 +1669,7 @@ void hm_plan(bem_hmatrix h) {
         const HNode &r = P.nodes[P.lowrank[b].row], &c = P.nodes[P.lowrank[b].col];
         HBlk x{(int)r.lo, (int)(r.hi - r.lo), (int)c.lo, (int)(c.hi - c.lo), 0, 0, uoff};
         x.rmax = std::min({x.m, x.n, cap1});
-        if (h->fluid) {
+        if (h->fluid && !h->calderon) {
             double d2 = 0;
             for (int q = 0; q < 3; ++q) {
                 const double gap = std::max({0.0, c.bmin[q] - r.bmax[q], r.bmin[q] - c.bmax[q]});
@@ -1856,9 +1888,18 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
 
 void hm_coarse(bem_hmatrix h);
 
+void hm_build_calderon(bem_hmatrix h);
+
 void hm_build(bem_hmatrix h) {
     const bool bm = h->prob.formulation == BEM_BURTON_MILLER;
+    if (h->calderon) {
+        hm_build_calderon(h);
+        return;
+    }
     hm_plan(h);
+    h->nsys = h->n;
+    h->sysperm = h->hperm;
+    h->dsysperm = h->perm;
     if (h->fluid) {
         // Mueller: the right-hand side is the incident traces themselves
         hm_operator(h, h->A, HK_MULLER);
@@ -1888,6 +1929,244 @@ void hm_build(bem_hmatrix h) {
     h->info.mean_rank = h->A.mean_rank;
     h->info.lowrank_bytes = h->A.lowrank_bytes + h->B.lowrank_bytes;
     if (!h->fluid) h->info.dense_bytes *= 2;
+}
+
+// ---------------------------------------------------------------------------
+// Low-frequency compressed Mueller (reference compression/fluid.jl:20-106 in
+// the Calderon regime, max(k, k_int) * radius <= pi/2).  Compressed S and
+// C = 1/2 I - D of each medium over one panel tree; with t_e = S_e q - D_e p
+// and t_i = g S_i q - D_i p the system is
+//   y_p = p + t_e - t_i
+//   y_q = q + S_e^{-1} (D_e t_e + p/4) - S_i^{-1} (D_i t_i + p/4) / g,
+// which is [I - D_e + D_i, S_e - g S_i; -H_e + H_i/g, I + K'_e - K'_i] with
+// K' = S^{-1} D S and H = S^{-1} (D^2 - I/4) (full.jl:201-207), never formed.
+// The reference applies S^{-1} through a hierarchical LU at a hundredth of
+// the compression tolerance; here each S^{-1} is an inner GMRES on the
+// compressed S (block Jacobi on its diagonal leaves) to a hundredth of the
+// outer tolerance.  The outer preconditioner inverts the local 2m x 2m
+// Mueller blocks of the diagonal leaves, built from the leaves' S and D
+// blocks by the same identities (fluid.jl:57-72), plus the coarse space.
+// ---------------------------------------------------------------------------
+void hm_build_calderon(bem_hmatrix h) {
+    hipStream_t s = h->lane->s;
+    hm_plan(h);  // panel tree (h->calderon set): n = N
+    const int64_t N = h->n;
+    h->nsys = 2 * N;
+    h->sysperm.resize(2 * N);
+    for (int64_t pos = 0; pos < N; ++pos) {
+        h->sysperm[pos] = h->hperm[pos];
+        h->sysperm[N + pos] = (int)(N + h->hperm[pos]);
+    }
+    h->dsysperm = hm_upload(h, h->sysperm);
+    const HPar keep = h->par;
+    const double km[2] = {keep.k, keep.ki};
+    for (int med = 0; med < 2; ++med) {
+        h->par = HPar{km[med], km[med], 1.0};
+        hm_operator(h, h->cal[2 * med], HK_CBIE_RHS);  // S
+        hm_operator(h, h->cal[2 * med + 1], HK_CBIE);  // 1/2 I - D
+    }
+    h->par = keep;
+    // block Jacobi of S_e and S_i for the inner solves
+    int* st = (int*)hm_alloc(h, sizeof(int));
+    launch_fill_int(s, st, 1, 0);
+    h->minv_e = (z64*)hm_alloc(h, sizeof(z64) * 4096 * (size_t)h->nleaf);
+    h->minv_i = (z64*)hm_alloc(h, sizeof(z64) * 4096 * (size_t)h->nleaf);
+    launch_hjacobi_inv(s, h->cal[0].L, h->cal[0].dblk, h->cal[0].dval, h->minv_e, st);
+    launch_hjacobi_inv(s, h->cal[2].L, h->cal[2].dblk, h->cal[2].dval, h->minv_i, st);
+    HIPCHECK(hipGetLastError());
+    if (h->opt.preconditioner >= 1) {
+        // local Mueller blocks of the diagonal leaves, inverted on the host
+        using zc = std::complex<double>;
+        const double g = keep.g;
+        std::vector<int> diag(h->nleaf, -1);
+        for (int b = 0; b < h->nd; ++b) {
+            const HBlk& x = h->plan_blk[b];
+            if (x.row_lo == x.col_lo && x.m == x.n) diag[h->leaf_of_pos[x.row_lo]] = b;
+        }
+        std::vector<std::vector<z32>> dv(4, std::vector<z32>(std::max<int64_t>(1, h->dense_count)));
+        for (int q = 0; q < 4; ++q)
+            HIPCHECK(hipMemcpyAsync(dv[q].data(), h->cal[q].dval, sizeof(z32) * h->dense_count,
+                                    hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        std::vector<z64> lb((size_t)h->nleaf * 128 * 128, z64{0, 0});
+        auto inv = [](std::vector<zc>& A, int n) {  // in-place Gauss-Jordan with partial pivoting
+            std::vector<int> piv(n);
+            for (int k = 0; k < n; ++k) {
+                int p = k;
+                for (int i = k + 1; i < n; ++i)
+                    if (std::abs(A[(size_t)i * n + k]) > std::abs(A[(size_t)p * n + k])) p = i;
+                if (!(std::abs(A[(size_t)p * n + k]) > 0)) return false;
+                piv[k] = p;
+                if (p != k)
+                    for (int j = 0; j < n; ++j) std::swap(A[(size_t)p * n + j], A[(size_t)k * n + j]);
+                const zc ip = 1.0 / A[(size_t)k * n + k];
+                A[(size_t)k * n + k] = 1.0;
+                for (int j = 0; j < n; ++j) A[(size_t)k * n + j] *= ip;
+                for (int i = 0; i < n; ++i)
+                    if (i != k) {
+                        const zc f = A[(size_t)i * n + k];
+                        if (f == 0.0) continue;
+                        A[(size_t)i * n + k] = 0.0;
+                        for (int j = 0; j < n; ++j) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+                    }
+            }
+            for (int k = n - 1; k >= 0; --k)
+                if (piv[k] != k)
+                    for (int i = 0; i < n; ++i) std::swap(A[(size_t)i * n + k], A[(size_t)i * n + piv[k]]);
+            return true;
+        };
+        auto mul = [](const std::vector<zc>& A, const std::vector<zc>& B, int n) {
+            std::vector<zc> C((size_t)n * n, 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < n; ++k) {
+                    const zc a = A[(size_t)i * n + k];
+                    for (int j = 0; j < n; ++j) C[(size_t)i * n + j] += a * B[(size_t)k * n + j];
+                }
+            return C;
+        };
+        for (int l = 0; l < h->nleaf; ++l) {
+            const int b = diag[l];
+            if (b < 0) throw BemError(BEM_ERR_DEVICE, "leaf without a diagonal block");
+            const int m = h->plan_blk[b].m;
+            const int64_t off = h->plan_blk[b].off;
+            std::vector<zc> se(m * m), de(m * m), si(m * m), di(m * m);
+            for (int e = 0; e < m * m; ++e) {
+                const int i = e / m, j = e % m;
+                const double half = i == j ? 0.5 : 0.0;
+                se[e] = zc(dv[0][off + e].x, dv[0][off + e].y);
+                de[e] = zc(half - dv[1][off + e].x, -dv[1][off + e].y);
+                si[e] = zc(dv[2][off + e].x, dv[2][off + e].y);
+                di[e] = zc(half - dv[3][off + e].x, -dv[3][off + e].y);
+            }
+            auto sinv_e = se, sinv_i = si;
+            if (!inv(sinv_e, m) || !inv(sinv_i, m)) throw BemError(BEM_ERR_NUMERIC, "singular local single layer");
+            auto dde = mul(de, de, m), ddi = mul(di, di, m);
+            for (int i = 0; i < m; ++i) {
+                dde[(size_t)i * m + i] -= 0.25;
+                ddi[(size_t)i * m + i] -= 0.25;
+            }
+            const auto ke = mul(sinv_e, mul(de, se, m), m), he = mul(sinv_e, dde, m);
+            const auto ki = mul(sinv_i, mul(di, si, m), m), hi = mul(sinv_i, ddi, m);
+            std::vector<zc> B((size_t)4 * m * m);
+            const int n2 = 2 * m;
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j) {
+                    const size_t e = (size_t)i * m + j;
+                    const double d1 = i == j ? 1.0 : 0.0;
+                    B[(size_t)i * n2 + j] = d1 - de[e] + di[e];
+                    B[(size_t)i * n2 + m + j] = se[e] - g * si[e];
+                    B[(size_t)(m + i) * n2 + j] = -he[e] + hi[e] / g;
+                    B[(size_t)(m + i) * n2 + m + j] = d1 + ke[e] - ki[e];
+                }
+            if (!inv(B, n2)) throw BemError(BEM_ERR_NUMERIC, "singular local Mueller block");
+            for (int i = 0; i < n2; ++i)
+                for (int j = 0; j < n2; ++j) {
+                    const zc v = B[(size_t)i * n2 + j];
+                    lb[(size_t)l * 128 * 128 + (size_t)i * 128 + j] = z64{v.real(), v.imag()};
+                }
+        }
+        h->lblk = hm_upload(h, lb);
+    }
+    if (h->opt.preconditioner == 2) {
+        const HSys S = hm_system(h);
+        z64* work = (z64*)hm_alloc(h, sizeof(z64) * std::max<size_t>(1, S.work_size(18)));
+        hm_coarse_with(h, [&](const z64* X, int nr, z64* Y) { S.apply(X, nr, Y, work); },
+                       [&](const z64* X, int nr, z64* Y) {
+                           launch_hblock2_apply(s, h->cal[0].L, N, h->lblk, X, nr, Y);
+                       });
+    }
+    const int64_t rbytes = 64ll * 1024 * 1024 / (32 * h->nsys);
+    h->rcap = (int)std::min<int64_t>(std::min<int64_t>(h->opt.recycle_dimension, h->nsys), rbytes);
+    if (h->rcap > 0) {
+        h->ru = (z64*)hm_alloc(h, sizeof(z64) * h->nsys * h->rcap);
+        h->rc = (z64*)hm_alloc(h, sizeof(z64) * h->nsys * h->rcap);
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    h->info.assembly_ms = h->info.aca_ms = 0;
+    h->info.max_rank = 0;
+    h->info.unconverged_blocks = 0;
+    h->info.lowrank_bytes = 0;
+    for (const HOp& op : h->cal) {
+        h->info.assembly_ms += op.assembly_ms;
+        h->info.aca_ms += op.aca_ms;
+        h->info.max_rank = std::max(h->info.max_rank, op.max_rank);
+        h->info.unconverged_blocks += op.unconverged;
+        h->info.lowrank_bytes += op.lowrank_bytes;
+    }
+    h->info.mean_rank = h->cal[0].mean_rank;
+    h->info.dense_bytes *= 4;
+    h->info.n = h->nsys;
+}
+
+// Inner solve S x = r of one medium (block Jacobi on S's diagonal leaves).
+HSys hm_single_layer_system(bem_hmatrix h, int med) {
+    HSys S;
+    S.n = h->n;
+    S.tol = std::max(1e-14, 0.01 * h->opt.gmres_tolerance);
+    S.restart = 100;
+    S.maxiter = 5000;
+    HOp* op = &h->cal[2 * med];
+    z64* minv = med == 0 ? h->minv_e : h->minv_i;
+    S.apply = [h, op](const z64* X, int nr, z64* Y, z64*) { hm_apply(h, *op, X, nr, Y); };
+    S.prec = [h, op, minv](const z64* X, int nr, z64* Y, z64*) {
+        launch_hjacobi_apply(h->lane->s, op->L, minv, X, nr, Y);
+    };
+    S.work_size = [](int) { return (size_t)0; };
+    return S;
+}
+
+HSys hm_calderon_system(bem_hmatrix h) {
+    HSys S;
+    S.n = h->nsys;
+    S.tol = h->opt.gmres_tolerance;
+    S.restart = std::max(1, h->opt.gmres_restart);
+    S.maxiter = h->opt.gmres_maxiter;
+    S.apply = [h](const z64* X, int nr, z64* Y, z64*) {
+        hipStream_t s = h->lane->s;
+        const int64_t N = h->n, vs = N * nr;
+        const double g = h->par.g;
+        std::vector<void*> tmp;
+        auto al = [&]() { void* p = h->ctx->alloc(sizeof(z64) * vs); tmp.push_back(p); return (z64*)p; };
+        try {
+            z64 *A1 = al(), *A2 = al(), *te = al(), *ti = al(), *re = al(), *ri = al(), *de = al(), *di = al();
+            const z64 *Xp = X, *Xq = X + vs;
+            z64 *Yp = Y, *Yq = Y + vs;
+            hm_apply(h, h->cal[0], Xq, nr, A1);
+            hm_apply(h, h->cal[1], Xp, nr, A2);
+            launch_lincomb3(s, vs, 1.0, A1, 1.0, A2, -0.5, Xp, te);  // S_e q - D_e p
+            hm_apply(h, h->cal[2], Xq, nr, A1);
+            hm_apply(h, h->cal[3], Xp, nr, A2);
+            launch_lincomb3(s, vs, g, A1, 1.0, A2, -0.5, Xp, ti);    // g S_i q - D_i p
+            launch_lincomb3(s, vs, 1.0, Xp, 1.0, te, -1.0, ti, Yp);
+            hm_apply(h, h->cal[1], te, nr, A1);
+            launch_lincomb3(s, vs, 0.5, te, -1.0, A1, 0.25, Xp, re);  // D_e t_e + p/4
+            hm_apply(h, h->cal[3], ti, nr, A2);
+            launch_lincomb3(s, vs, 0.5, ti, -1.0, A2, 0.25, Xp, ri);  // D_i t_i + p/4
+            HIPCHECK(hipGetLastError());
+            std::vector<bem_diagnostics> dgi(nr);
+            const HSys Se = hm_single_layer_system(h, 0), Si = hm_single_layer_system(h, 1);
+            hm_gmres(h, nr, re, de, dgi, nullptr, &Se);
+            for (const auto& d : dgi) h->inner_iters += d.iterations;
+            hm_gmres(h, nr, ri, di, dgi, nullptr, &Si);
+            for (const auto& d : dgi) h->inner_iters += d.iterations;
+            launch_lincomb3(s, vs, 1.0, Xq, 1.0, de, -1.0 / g, di, Yq);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            hipStreamSynchronize(s);
+            for (void* p : tmp) h->ctx->release(p);
+            throw;
+        }
+        for (void* p : tmp) h->ctx->release(p);
+    };
+    S.prec = [h](const z64* X, int nr, z64* Y, z64* work) {
+        hipStream_t s = h->lane->s;
+        if (h->lblk) launch_hblock2_apply(s, h->cal[0].L, h->n, h->lblk, X, nr, Y);
+        else HIPCHECK(hipMemcpyAsync(Y, X, sizeof(z64) * h->nsys * nr, hipMemcpyDeviceToDevice, s));
+        hm_coarse_correct(h, X, nr, Y, work);
+    };
+    S.work_size = [h](int nr) { return (size_t)(2 + 512) * std::max(1, h->nq) * nr; };
+    return S;
 }
 
 // Y = op X on the device (tree-position order, nr <= kHRhs)
@@ -1921,22 +2200,22 @@ void hm_prec(bem_hmatrix h, const z64* X, int nr, z64* Y, z64* work) {
         return;
     }
     launch_hjacobi_apply(s, h->A.L, h->minv, X, nr, Y);
-    if (h->nq > 0) {
-        z64* w = work;
-        z64* w2 = work + (size_t)h->nq * nr;
-        z64* part = work + 2 * (size_t)h->nq * nr;
-        launch_basis_dots(s, h->n, nr, h->nq, h->cq, X, part, w);
-        launch_small_apply(s, h->nq, nr, h->einv, w, w2);
-        launch_basis_combine(s, h->n, nr, h->nq, h->cc, w2, Y);
-    }
+    hm_coarse_correct(h, X, nr, Y, work);
+}
+
+// Y += (Q - M^{-1} A Q) E^{-1} Q^H X when the coarse space is built
+void hm_coarse_correct(bem_hmatrix h, const z64* X, int nr, z64* Y, z64* work) {
+    if (h->nq <= 0) return;
+    hipStream_t s = h->lane->s;
+    z64* w = work;
+    z64* w2 = work + (size_t)h->nq * nr;
+    z64* part = work + 2 * (size_t)h->nq * nr;
+    launch_basis_dots(s, h->nsys, nr, h->nq, h->cq, X, part, w);
+    launch_small_apply(s, h->nq, nr, h->einv, w, w2);
+    launch_basis_combine(s, h->nsys, nr, h->nq, h->cc, w2, Y);
 }
 size_t hm_prec_work(bem_hmatrix h, int nr) { return (size_t)(2 + kGmPartial) * std::max(1, h->nq) * nr; }
 
-// A P^{-1} X; Z receives P^{-1} X
-void hm_apply_prec(bem_hmatrix h, const z64* X, int nr, z64* Z, z64* Y, z64* work) {
-    hm_prec(h, X, nr, Z, work);
-    hm_apply(h, h->A, Z, nr, Y);
-}
 
 // Coarse space of the two-level preconditioner: the regular harmonic
 // polynomials of degree <= 2 (the span of the reference's nine regular
@@ -1945,8 +2224,17 @@ void hm_apply_prec(bem_hmatrix h, const z64* X, int nr, z64* Z, z64* Y, z64* wor
 // 9 columns for a rigid surface and 18 for a fluid body
 // (compression/fluid.jl:193-205), orthonormalised.
 void hm_coarse(bem_hmatrix h) {
+    hm_coarse_with(h, [h](const z64* X, int nr, z64* Y) { hm_apply(h, h->A, X, nr, Y); },
+                   [h](const z64* X, int nr, z64* Y) { launch_hjacobi_apply(h->lane->s, h->A.L, h->minv, X, nr, Y); });
+}
+
+// The coarse space over the system's unknowns (h->nsys rows; unknown u of
+// row `pos` is h->sysperm[pos]: panel u % N, trace u / N), with the
+// system's apply and local preconditioner.
+void hm_coarse_with(bem_hmatrix h, const std::function<void(const z64*, int, z64*)>& apply,
+                    const std::function<void(const z64*, int, z64*)>& local) {
     hipStream_t s = h->lane->s;
-    const int64_t n = h->n;
+    const int64_t n = h->nsys;
     const int nt = (int)h->nt;
     const int nq = h->fluid ? 18 : 9;
     std::vector<double> c(3 * (size_t)nt);
@@ -1964,7 +2252,7 @@ void hm_coarse(bem_hmatrix h) {
     if (!(rad > 0)) rad = 1.0;
     std::vector<std::complex<double>> Q((size_t)n * nq, 0.0);
     for (int64_t pos = 0; pos < n; ++pos) {
-        const int idx = h->hperm[pos];
+        const int idx = h->sysperm[pos];
         const int panel = idx % nt;
         const int block = idx / nt;  // 0 pressure, 1 normal derivative (fluid)
         const double x = (c[panel] - ctr[0]) / rad, y = (c[(size_t)nt + panel] - ctr[1]) / rad,
@@ -1999,8 +2287,8 @@ void hm_coarse(bem_hmatrix h) {
         dY = (z64*)h->ctx->alloc(sizeof(z64) * n * nq);
         dZ = (z64*)h->ctx->alloc(sizeof(z64) * n * nq);
         HIPCHECK(hipMemcpyAsync(dX, X.data(), sizeof(z64) * n * nq, hipMemcpyHostToDevice, s));
-        hm_apply(h, h->A, dX, nq, dY);
-        launch_hjacobi_apply(s, h->A.L, h->minv, dY, nq, dZ);
+        apply(dX, nq, dY);
+        local(dY, nq, dZ);
         std::vector<z64> AQ((size_t)n * nq), MAQ((size_t)n * nq);
         HIPCHECK(hipMemcpyAsync(AQ.data(), dY, sizeof(z64) * n * nq, hipMemcpyDeviceToHost, s));
         HIPCHECK(hipMemcpyAsync(MAQ.data(), dZ, sizeof(z64) * n * nq, hipMemcpyDeviceToHost, s));
@@ -2071,13 +2359,32 @@ __global__ void k_inv_norm(int nr, const z64* sq, double* scale, double* norm) {
 
 // Restarted GMRES (right-preconditioned, CGS2 Arnoldi, complex Givens) for nr
 // right-hand sides B at once, all in FP64.  X receives the solution.
-// target: absolute residual target per column (null: tolerance * ||B||).
-void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagnostics>& dg,
-              const double* target = nullptr) {
+HSys hm_calderon_system(bem_hmatrix h);
+
+// The hierarchical matrix's own system (A with its preconditioner).
+HSys hm_system(bem_hmatrix h) {
+    if (h->calderon) return hm_calderon_system(h);
+    HSys S;
+    S.n = h->n;
+    S.tol = h->opt.gmres_tolerance;
+    S.restart = std::max(1, h->opt.gmres_restart);
+    S.maxiter = h->opt.gmres_maxiter;
+    S.apply = [h](const z64* X, int nr, z64* Y, z64*) { hm_apply(h, h->A, X, nr, Y); };
+    S.prec = [h](const z64* X, int nr, z64* Y, z64* w) { hm_prec(h, X, nr, Y, w); };
+    S.work_size = [h](int nr) { return hm_prec_work(h, nr); };
+    return S;
+}
+
+// target: absolute residual target per column (null: tolerance * ||B||);
+// sys: the system (null: the hierarchical matrix's own).
+void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagnostics>& dg, const double* target,
+              const HSys* sys) {
+    const HSys own = sys ? HSys{} : hm_system(h);
+    const HSys& S = sys ? *sys : own;
     hipStream_t s = h->lane->s;
-    const int64_t n = h->n;
-    const int m = std::max(1, h->opt.gmres_restart);
-    const double tol = h->opt.gmres_tolerance;
+    const int64_t n = S.n;
+    const int m = std::max(1, S.restart);
+    const double tol = S.tol;
     const int64_t vs = n * nr;
     z64* Vb = (z64*)h->ctx->alloc(sizeof(z64) * vs * (m + 1));
     std::vector<void*> tmp{Vb};
@@ -2094,7 +2401,7 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
         double* scale = (double*)al(sizeof(double) * nr);
         double* norm = (double*)al(sizeof(double) * nr);
         z64* ycoef = (z64*)al(sizeof(z64) * m * nr);
-        z64* pw = (z64*)al(sizeof(z64) * hm_prec_work(h, nr));
+        z64* pw = (z64*)al(sizeof(z64) * std::max<size_t>(1, S.work_size(nr)));
         launch_fill_double(s, (double*)U, 2 * vs, 0.0);
         launch_fill_double(s, (double*)X, 2 * vs, 0.0);
         // ||b||
@@ -2120,7 +2427,7 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
             HIPCHECK(hipStreamSynchronize(s));
             bool all = true;
             for (int c = 0; c < nr; ++c) all &= !(rn[c] > thr[c]);
-            if (all || iters >= h->opt.gmres_maxiter) {
+            if (all || iters >= S.maxiter) {
                 done = all;
                 break;
             }
@@ -2130,7 +2437,8 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
             int jend = 0;
             for (int j = 0; j < m; ++j) {
                 z64* Vj = Vb + (int64_t)j * vs;
-                hm_apply_prec(h, Vj, nr, Z, W, pw);
+                S.prec(Vj, nr, Z, pw);
+                S.apply(Z, nr, W, pw);
                 // classical Gram-Schmidt, twice
                 prof_begin(&h->lane->pl, s, C_HORTHO);
                 launch_gm_dots(s, n, nr, j + 1, Vb, vs, W, part, h1);
@@ -2185,7 +2493,7 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
                     gj = cc * gj;
                     conv &= !(std::abs(g[(size_t)c * (m + 1) + j + 1]) > thr[c]);
                 }
-                if (conv || iters >= h->opt.gmres_maxiter) break;
+                if (conv || iters >= S.maxiter) break;
             }
             // y = H^{-1} g per column; U += V y; X = M^{-1} U; R = B - A X
             std::vector<z64> y((size_t)jend * nr);
@@ -2201,8 +2509,8 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
             }
             HIPCHECK(hipMemcpyAsync(ycoef, y.data(), sizeof(z64) * y.size(), hipMemcpyHostToDevice, s));
             launch_gm_combine(s, n, nr, jend, Vb, vs, ycoef, 1.0, U);
-            hm_prec(h, U, nr, X, pw);
-            hm_apply(h, h->A, X, nr, W);
+            S.prec(U, nr, X, pw);
+            S.apply(X, nr, W, pw);
             launch_gm_sub(s, vs, B, W, R);
             HIPCHECK(hipGetLastError());
         }
@@ -2242,8 +2550,8 @@ void hm_gmres(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<bem_diagn
 // ---------------------------------------------------------------------------
 void hm_norms(bem_hmatrix h, int nr, const z64* V, z64* part, z64* sq, std::vector<double>& out) {
     hipStream_t s = h->lane->s;
-    const int64_t vs = h->n * nr;
-    launch_gm_dots(s, h->n, nr, 1, V, vs, V, part, sq);
+    const int64_t vs = h->nsys * nr;
+    launch_gm_dots(s, h->nsys, nr, 1, V, vs, V, part, sq);
     std::vector<z64> hs(nr);
     HIPCHECK(hipMemcpyAsync(hs.data(), sq, sizeof(z64) * nr, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
@@ -2257,13 +2565,15 @@ void hm_solve_recycled(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<
         return;
     }
     hipStream_t s = h->lane->s;
-    const int64_t n = h->n, vs = n * nr;
+    const HSys S = hm_system(h);
+    const int64_t n = S.n, vs = n * nr;
     const double tol = h->opt.gmres_tolerance;
     std::vector<void*> tmp;
     auto al = [&](size_t b) { void* p = h->ctx->alloc(b); tmp.push_back(p); return p; };
     try {
         z64* R0 = (z64*)al(sizeof(z64) * vs);
         z64* W = (z64*)al(sizeof(z64) * vs);
+        z64* pw = (z64*)al(sizeof(z64) * std::max<size_t>(1, S.work_size(nr)));
         z64* D = (z64*)al(sizeof(z64) * vs);
         z64* part = (z64*)al(sizeof(z64) * 512 * (size_t)std::max(1, h->rcap) * nr);
         z64* coef = (z64*)al(sizeof(z64) * (size_t)std::max(1, h->rcap) * nr);
@@ -2295,7 +2605,7 @@ void hm_solve_recycled(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<
             // x0 = U C^H b and its residual
             launch_basis_dots(s, n, nr, h->rcount, h->rc, B, part, coef);
             launch_basis_combine(s, n, nr, h->rcount, h->ru, coef, X);
-            hm_apply(h, h->A, X, nr, W);
+            S.apply(X, nr, W, pw);
             launch_gm_sub(s, vs, B, W, R0);
             hm_norms(h, nr, R0, part, sq, rn);
             bool all = true, shrinks = true;
@@ -2317,13 +2627,13 @@ void hm_solve_recycled(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<
             }
         }
         if (reused) {
-            hm_gmres(h, nr, R0, D, dg, tgt.data());
+            hm_gmres(h, nr, R0, D, dg, tgt.data(), &S);
             launch_gm_combine(s, n, nr, 1, D, vs, ones, 1.0, X);  // x = x0 + d
         } else {
-            hm_gmres(h, nr, B, X, dg, tgt.data());
+            hm_gmres(h, nr, B, X, dg, tgt.data(), &S);
         }
         // true residual of the returned solution
-        hm_apply(h, h->A, X, nr, W);
+        S.apply(X, nr, W, pw);
         launch_gm_sub(s, vs, B, W, R0);
         hm_norms(h, nr, R0, part, sq, rn);
         bool accepted = true;
@@ -2334,9 +2644,9 @@ void hm_solve_recycled(bem_hmatrix h, int nr, const z64* B, z64* X, std::vector<
             std::vector<int> first(nr);
             for (int c = 0; c < nr; ++c) first[c] = dg[c].iterations;
             h->rcount = 0;
-            hm_gmres(h, nr, B, X, dg, tgt.data());
+            hm_gmres(h, nr, B, X, dg, tgt.data(), &S);
             for (int c = 0; c < nr; ++c) dg[c].iterations += first[c];
-            hm_apply(h, h->A, X, nr, W);
+            S.apply(X, nr, W, pw);
             launch_gm_sub(s, vs, B, W, R0);
             hm_norms(h, nr, R0, part, sq, rn);
             accepted = true;
@@ -3638,11 +3948,6 @@ int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, c
             // compression/fluid.jl:20-24
             if (problem->formulation != BEM_MULLER)
                 throw BemError(BEM_ERR_ARGUMENT, "compressed fluid BEM requires formulation=:muller");
-            if (calderon_applies(mesh->h, *problem))
-                throw BemError(BEM_ERR_ARGUMENT,
-                               "compressed fluid BEM at max(k, k_int) * radius <= pi/2 takes K' and H from the "
-                               "Calderon identities through a hierarchical LU of S, which is not built here: use "
-                               "the dense path");
         } else if (problem->formulation != BEM_BURTON_MILLER && problem->formulation != BEM_CBIE) {
             throw BemError(BEM_ERR_ARGUMENT, "formulation must be :burton_miller or :cbie");
         }
@@ -3665,6 +3970,9 @@ int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, c
         h->prob = *problem;
         h->opt = o;
         h->fluid = fluid;
+        // compression/fluid.jl: at max(k, k_int) * radius <= pi/2 the exterior
+        // and interior K' and H come from the Calderon identities
+        h->calderon = fluid && calderon_applies(mesh->h, *problem);
         h->par = HPar{problem->k, fluid ? problem->k / problem->soundspeed_contrast : problem->k,
                       fluid ? problem->density_contrast : 1.0};
         try {
@@ -3704,28 +4012,32 @@ int bem_hmatrix_apply(bem_hmatrix h, int32_t which, int32_t nrhs, const double* 
     return guarded(ctx, [&] {
         if (which == 1 && h->fluid)
             throw BemError(BEM_ERR_ARGUMENT, "a fluid body's right-hand side is not an operator");
-        const int64_t n = h->n;
+        const HSys S = hm_system(h);
+        const int64_t n = S.n;
         hipStream_t s = h->lane->s;
         z64* X = (z64*)ctx->alloc(sizeof(z64) * n * kHRhs);
         z64* Y = nullptr;
+        z64* pw = nullptr;
         try {
             Y = (z64*)ctx->alloc(sizeof(z64) * n * kHRhs);
+            pw = (z64*)ctx->alloc(sizeof(z64) * std::max<size_t>(1, S.work_size(kHRhs)));
             std::vector<z64> hx((size_t)n * kHRhs), hy((size_t)n * kHRhs);
             for (int c0 = 0; c0 < nrhs; c0 += kHRhs) {
                 const int nr = std::min(kHRhs, nrhs - c0);
                 for (int64_t pos = 0; pos < n; ++pos)
                     for (int c = 0; c < nr; ++c) {
-                        const double* src = x + 2 * ((size_t)(c0 + c) * n + h->hperm[pos]);
+                        const double* src = x + 2 * ((size_t)(c0 + c) * n + h->sysperm[pos]);
                         hx[pos * nr + c] = z64{src[0], src[1]};
                     }
                 HIPCHECK(hipMemcpyAsync(X, hx.data(), sizeof(z64) * n * nr, hipMemcpyHostToDevice, s));
-                hm_apply(h, which == 0 ? h->A : h->B, X, nr, Y);
+                if (which == 0) S.apply(X, nr, Y, pw);
+                else hm_apply(h, h->B, X, nr, Y);
                 HIPCHECK(hipGetLastError());
                 HIPCHECK(hipMemcpyAsync(hy.data(), Y, sizeof(z64) * n * nr, hipMemcpyDeviceToHost, s));
                 HIPCHECK(hipStreamSynchronize(s));
                 for (int64_t pos = 0; pos < n; ++pos)
                     for (int c = 0; c < nr; ++c) {
-                        double* dst = y + 2 * ((size_t)(c0 + c) * n + h->hperm[pos]);
+                        double* dst = y + 2 * ((size_t)(c0 + c) * n + h->sysperm[pos]);
                         dst[0] = hy[pos * nr + c].x;
                         dst[1] = hy[pos * nr + c].y;
                     }
@@ -3734,10 +4046,12 @@ int bem_hmatrix_apply(bem_hmatrix h, int32_t which, int32_t nrhs, const double* 
             hipStreamSynchronize(s);
             ctx->release(X);
             ctx->release(Y);
+            ctx->release(pw);
             throw;
         }
         ctx->release(X);
         ctx->release(Y);
+        ctx->release(pw);
     });
 }
 
@@ -3748,7 +4062,7 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t nrhs, const double* angles, bem_sol
     return guarded(ctx, [&] {
         for (int r = 0; r < 2 * nrhs; ++r)
             if (!std::isfinite(angles[r])) throw BemError(BEM_ERR_ARGUMENT, "incidence angles must be finite");
-        const int64_t n = h->n;
+        const int64_t n = h->nsys;
         const int nt = (int)h->nt;
         hipStream_t s = h->lane->s;
         auto* sol = new bem_solution_s();
@@ -3780,7 +4094,7 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t nrhs, const double* angles, bem_sol
                 const int nr = std::min(kHRhs, nrhs - c0);
                 if (h->fluid) {
                     // Mueller: b = [p_inc; q_inc] (full.jl:442-447)
-                    launch_hfluid_rhs(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, B);
+                    launch_hfluid_rhs(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->dsysperm, B);
                 } else {
                     // b = B q_inc (the right-hand side of the scattered-field equation)
                     launch_hqinc(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, X);
@@ -3790,7 +4104,7 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t nrhs, const double* angles, bem_sol
                 hm_solve_recycled(h, nr, B, X, dg);
                 // traces of this chunk: columns c0 .. c0 + nr of P, Q
                 if (h->fluid)
-                    launch_hfluid_traces(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, X,
+                    launch_hfluid_traces(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->dsysperm, X,
                                          sol->p + (size_t)c0 * nt, sol->q + (size_t)c0 * nt);
                 else
                     launch_htraces(s, h->mesh->d, h->prob.k, nr, dd + 3 * c0, h->perm, X, sol->p + (size_t)c0 * nt,
@@ -3798,7 +4112,7 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t nrhs, const double* angles, bem_sol
                 HIPCHECK(hipGetLastError());
                 for (int c = 0; c < nr; ++c) {
                     sol->diag[c0 + c] = dg[c];
-                    sol->diag[c0 + c].derivative_evaluation = h->fluid ? 1 : 0;
+                    sol->diag[c0 + c].derivative_evaluation = h->fluid ? (h->calderon ? 2 : 1) : 0;
                 }
             }
             sol->hp.resize((size_t)nt * nrhs);

@@ -53,6 +53,12 @@ void launch_basis_dots(hipStream_t s, int64_t n, int nr, int nb, const z64* Q, c
                        z64* out);
 void launch_basis_combine(hipStream_t s, int64_t n, int nr, int nb, const z64* Q, const z64* coef, z64* Y);
 void launch_small_apply(hipStream_t s, int nb, int nr, const z64* M, const z64* w, z64* out);
+// out = a X + b Y + c Z over tot complex entries (Z may be null)
+void launch_lincomb3(hipStream_t s, int64_t tot, double a, const z64* X, double b, const z64* Y, double c,
+                     const z64* Z, z64* out);
+// local 2m x 2m block inverses of the low-frequency Mueller system on [p; q]
+void launch_hblock2_apply(hipStream_t s, const HLeafs& L, int64_t nt, const z64* lblk, const z64* X, int nr,
+                          z64* Y);
 // out = column c of X (n rows, nr columns interleaved)
 void launch_gather_col(hipStream_t s, int64_t n, int nr, int c, const z64* X, z64* out);
 // W += sign * sum_i V_i coef[i]

@@ -556,6 +556,50 @@ void launch_gather_col(hipStream_t s, int64_t n, int nr, int c, const z64* X, z6
     hipLaunchKernelGGL(k_gather_col, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, nr, c, X, out);
 }
 
+// out = a X + b Y + c Z (complex vectors, real scalars; Z may be null)
+__global__ void k_lincomb3(int64_t tot, double a, const z64* __restrict__ X, double b, const z64* __restrict__ Y,
+                           double c, const z64* __restrict__ Z, z64* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= tot) return;
+    z64 v{a * X[e].x + b * Y[e].x, a * X[e].y + b * Y[e].y};
+    if (Z) {
+        v.x += c * Z[e].x;
+        v.y += c * Z[e].y;
+    }
+    out[e] = v;
+}
+void launch_lincomb3(hipStream_t s, int64_t tot, double a, const z64* X, double b, const z64* Y, double c,
+                     const z64* Z, z64* out) {
+    hipLaunchKernelGGL(k_lincomb3, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, tot, a, X, b, Y, c, Z, out);
+}
+
+// Local 2m x 2m block inverses on [p; q] (rows 0..N-1 and N..2N-1 of X):
+// [y_p; y_q] of leaf l = Binv_l [x_p; x_q] with Binv_l at lblk + l * 128 * 128
+// (row-major, leading dimension 128).  One workgroup per leaf, a thread per
+// (output row, right-hand side).
+__global__ __launch_bounds__(256) void k_hblock2_apply(const HLeafs L, int64_t nt, const z64* __restrict__ lblk,
+                                                       const z64* __restrict__ X, int nr, z64* __restrict__ Y) {
+    const int leaf = blockIdx.x;
+    const int lo = L.lo[leaf], m = L.m[leaf];
+    const z64* Bl = lblk + (int64_t)leaf * 128 * 128;
+    for (int e = threadIdx.x; e < 2 * m * nr; e += 256) {
+        const int i = e / nr, c = e % nr;
+        double sx = 0, sy = 0;
+        for (int j = 0; j < 2 * m; ++j) {
+            const int64_t row = j < m ? lo + j : nt + lo + (j - m);
+            const z64 a = Bl[i * 128 + j], x = X[row * nr + c];
+            sx += a.x * x.x - a.y * x.y;
+            sy += a.x * x.y + a.y * x.x;
+        }
+        const int64_t orow = i < m ? lo + i : nt + lo + (i - m);
+        Y[orow * nr + c] = z64{sx, sy};
+    }
+}
+void launch_hblock2_apply(hipStream_t s, const HLeafs& L, int64_t nt, const z64* lblk, const z64* X, int nr,
+                          z64* Y) {
+    hipLaunchKernelGGL(k_hblock2_apply, dim3(L.count), dim3(256), 0, s, L, nt, lblk, X, nr, Y);
+}
+
 void launch_basis_dots(hipStream_t s, int64_t n, int nr, int nb, const z64* Q, const z64* X, z64* partial,
                        z64* out) {
     hipLaunchKernelGGL(k_basis_dots, dim3(kGmBlocks), dim3(256), 0, s, n, nr, nb, Q, X, partial);
