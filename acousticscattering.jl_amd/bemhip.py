@@ -31,6 +31,7 @@ BEM_ERR_ADMISSION, BEM_ERR_NUMERIC, BEM_ERR_STATE = 4, 5, 6
 BEM_RIGID, BEM_PRESSURE_RELEASE, BEM_FLUID = 0, 1, 2
 BEM_BURTON_MILLER, BEM_CBIE, BEM_MULLER = 0, 1, 2
 BEM_DOUBLE, BEM_MIXED = 0, 1
+_PRECISION = {"double": BEM_DOUBLE, "mixed": BEM_MIXED}
 REASONS = ["none", "conversion", "single_singular", "condition", "nonfinite",
            "iteration_limit", "stagnation"]
 
@@ -38,7 +39,7 @@ REASONS = ["none", "conversion", "single_singular", "condition", "nonfinite",
 EXPORTED = [
     "bem_version", "bem_problem_defaults", "bem_ctx_create", "bem_ctx_destroy", "bem_last_error",
     "bem_ctx_memory", "bem_ctx_set_profiling", "bem_ctx_kernel_stats", "bem_ctx_reset_stats", "bem_ctx_set_lanes",
-    "bem_ctx_set_heavy_lanes",
+    "bem_ctx_set_heavy_lanes", "bem_ctx_set_lu_tuning", "bem_ctx_autotune", "bem_ctx_lu_tune",
     "bem_icosphere_size", "bem_icosphere", "bem_revolution_size", "bem_revolution",
     "bem_mesh_create", "bem_mesh_validate", "bem_mesh_destroy", "bem_mesh_info", "bem_mesh_panels",
     "bem_mesh_quadrature", "bem_mesh_near_count", "bem_operator_create", "bem_operator_destroy",
@@ -50,6 +51,21 @@ EXPORTED = [
     "bem_hmatrix_destroy", "bem_hmatrix_get_info", "bem_hmatrix_matvec", "bem_hmatrix_apply", "bem_hmatrix_solve",
     "bem_debug_lu", "bem_far_field_noalloc", "bem_pressure", "bem_reduced_defaults", "bem_reduced_sweep",
 ]
+
+
+class LuTune(ctypes.Structure):
+    """bem_lu_tune: the shape of one factorization and the tuner's timings."""
+    _fields_ = [("unknowns", ctypes.c_int64), ("precision", ctypes.c_int32), ("outer", ctypes.c_int32),
+                ("group", ctypes.c_int32), ("split", ctypes.c_int32), ("ahead", ctypes.c_int32),
+                ("graph", ctypes.c_int32), ("tuned", ctypes.c_int32), ("candidates", ctypes.c_int32),
+                ("best_ms", ctypes.c_double), ("default_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["precision"] = "mixed" if self.precision == BEM_MIXED else "double"
+        for k in ("split", "ahead", "graph", "tuned"):
+            d[k] = bool(d[k])
+        return d
 
 
 class Problem(ctypes.Structure):
@@ -163,6 +179,9 @@ def load_library(path=LIB_PATH):
         "bem_ctx_reset_stats": (ctypes.c_int, [_vp]),
         "bem_ctx_set_lanes": (ctypes.c_int, [_vp, ctypes.c_int32]),
         "bem_ctx_set_heavy_lanes": (ctypes.c_int, [_vp, ctypes.c_int32]),
+        "bem_ctx_set_lu_tuning": (ctypes.c_int, [_vp, ctypes.c_int32]),
+        "bem_ctx_autotune": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp]),
+        "bem_ctx_lu_tune": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp]),
         "bem_icosphere_size": (ctypes.c_int, [ctypes.c_int32, _ip, _ip]),
         "bem_icosphere": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, _dp, _ip]),
@@ -328,6 +347,28 @@ class Context:
         """Largest number of concurrent sweep solves with 16384 or more
         unknowns (1..8); the other lanes take smaller solves meanwhile."""
         self.check(self.lib.bem_ctx_set_heavy_lanes(self._h, int(max_heavy)))
+
+    def set_lu_tuning(self, mode):
+        """"default" (outer 512, FP32 3M updates, look-ahead), "split" (the
+        default with bf16-split outer updates for complex FP32) or "tuned"
+        (per size class the tuner's choice; the default)."""
+        m = {"default": 0, "split": 1, "tuned": 2}
+        if mode not in m:
+            raise ValueError("mode must be 'default', 'split' or 'tuned'")
+        self.check(self.lib.bem_ctx_set_lu_tuning(self._h, m[mode]))
+
+    def autotune(self, n, precision="mixed"):
+        """Time candidate factorization shapes for n unknowns and keep the
+        fastest for that size class; returns the tuner's report."""
+        r = LuTune()
+        self.check(self.lib.bem_ctx_autotune(self._h, int(n), _PRECISION[precision], ctypes.byref(r)))
+        return r.as_dict()
+
+    def lu_tune(self, n, precision="mixed"):
+        """The factorization shape n unknowns would use now."""
+        r = LuTune()
+        self.check(self.lib.bem_ctx_lu_tune(self._h, int(n), _PRECISION[precision], ctypes.byref(r)))
+        return r.as_dict()
 
     def kernel_stats(self, name):
         n = ctypes.c_int64()
@@ -811,11 +852,17 @@ def reduced_frequency_sweep(surface, material, frequencies, sound_speed, *, trai
     return {"sweep": sweep, "diagnostics": report}
 
 
-def debug_lu(A, single=False, b=None, ctx=None, adjoint=False):
+def debug_lu(A, single=False, b=None, ctx=None, adjoint=False, updates=None, graph=False):
     """Device LU of a dense complex matrix (test hook): returns (LU, 1-based
-    ipiv[, x]) with x solving A x = b, or A^H x = b when adjoint."""
+    ipiv[, x]) with x solving A x = b, or A^H x = b when adjoint.  updates:
+    None (the context's shape for n), "split" (bf16-split outer updates,
+    single only) or "fp32"; graph: replay the factorization as a captured
+    graph."""
     ctx = ctx or default_context()
-    flags = (1 if single else 0) | (2 if adjoint else 0)
+    if updates not in (None, "split", "fp32") or (updates == "split" and not single):
+        raise ValueError("updates must be None, 'fp32' or (single precision) 'split'")
+    flags = (1 if single else 0) | (2 if adjoint else 0) | (4 if updates == "split" else 0) | \
+        (8 if updates == "fp32" else 0) | (16 if graph else 0)
     A = np.asfortranarray(np.asarray(A, dtype=np.complex128))
     n = A.shape[0]
     LU = np.empty((n, n), dtype=np.complex128, order="F")

@@ -20,6 +20,7 @@
 #include <functional>
 #include <list>
 #include <map>
+#include <set>
 #include <unordered_map>
 #include <thread>
 #include <mutex>
@@ -52,10 +53,10 @@ struct BemError : std::runtime_error {
 // "h_rows" (the event timers do not nest).
 const char* kClassNames[] = {"assemble", "near",      "lu_panel", "lu_gemm", "lu_trsm", "lu_swap",
                              "solve",    "residual",  "far_field", "h_apply", "h_aca",  "h_ortho",
-                             "lu_gemm_fp64", "h_proj", "h_rows"};
-constexpr int kNumClasses = 15;
+                             "lu_gemm_fp64", "h_proj", "h_rows", "lu_split", "lu_gemm_bf16", "lu_graph"};
+constexpr int kNumClasses = 18;
 enum { C_ASSEMBLE = 0, C_NEAR = 1, C_RESIDUAL = 7, C_FAR = 8, C_HAPPLY = 9, C_HACA = 10, C_HORTHO = 11,
-       C_HPROJ = 13, C_HROWS = 14 };
+       C_HPROJ = 13, C_HROWS = 14, C_LUGRAPH = 17 };
 
 struct ProfRec {
     int cls;
@@ -204,10 +205,22 @@ void Prof::detach(bem::ProfLane* pl) {
 // One execution lane: a HIP stream, the panel workspace of the LU on that
 // stream, and (for batched sweeps) a bump-allocated device arena that holds a
 // whole solve, so concurrent lanes never call hipMalloc/hipFree mid-sweep.
+// A factorization captured as a HIP graph: replayed when a lane factors the
+// same buffers with the same shape again (a sweep's arena hands each item of
+// a size class the same addresses).
+struct LuGraph {
+    const void *lu, *ipiv, *inv, *split;
+    int n, single;
+    LuTune t;
+    hipGraphExec_t exec;
+};
+constexpr size_t kLaneGraphs = 6;
+
 struct Lane {
     hipStream_t s = nullptr;
     hipStream_t s2 = nullptr;          // LU panel look-ahead
     hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
+    std::vector<LuGraph> graphs;       // oldest first, at most kLaneGraphs
     PanelWS ws{};
     void* ws_mem = nullptr;
     void* rec_mem = nullptr;           // ws.rec
@@ -228,6 +241,12 @@ struct bem_ctx_s {
     int max_lanes = 8;         // concurrent solves of bem_sweep
     int max_heavy = 1;         // of which heavy ones (kHeavyUnknowns)
     void* scratch = nullptr;   // this device's address of the fixed scratch array (bem_far_field_noalloc)
+    // LU shapes: 0 fixed (FP32 3M updates), 1 fixed with bf16-split outer
+    // updates, 2 tuned per size class (bem_ctx_autotune, or by bem_sweep
+    // for the classes it meets)
+    int lu_mode = 2;
+    std::mutex tmu;
+    std::map<std::pair<int, int>, bem_lu_tune> tunes;  // (unknowns, single) -> tuner's report
 
     Lane* lane0() { return lanes[0]; }
     // Device memory: a caching allocator.  Released blocks go to a free pool
@@ -321,6 +340,7 @@ struct bem_operator_s {
     int* ipiv = nullptr;
     int* perm = nullptr;
     void* inv = nullptr;
+    void* split = nullptr;        // bf16 planes of the factorization's outer updates
     int* ipiv64 = nullptr;
     bool calderon = false;        // low-frequency Mueller: K', H from the Calderon identities
     int* perm64 = nullptr;
@@ -394,6 +414,8 @@ void free_lane(bem_ctx c, Lane* L) {
     if (!L) return;
     if (L->s) hipStreamSynchronize(L->s);
     if (L->s2) hipStreamSynchronize(L->s2);
+    for (LuGraph& g : L->graphs) hipGraphExecDestroy(g.exec);
+    L->graphs.clear();
     c->prof.detach(&L->pl);
     c->release(L->ws_mem);
     c->release(L->rec_mem);
@@ -487,23 +509,115 @@ void* op_alloc(bem_operator op, size_t bytes) {
     return p;
 }
 
+// The shape of an n-unknown factorization (bem_ctx_set_lu_tuning): the
+// tuner's choice for a tuned size class; otherwise the default shape, with
+// bf16-split outer updates for complex FP32 from kSplitMin unknowns.
+constexpr int kSplitMin = 4096;
+LuTune tune_of(const bem_lu_tune& r) {
+    LuTune t;
+    t.outer = r.outer;
+    t.group = r.group;
+    t.split = r.split;
+    t.ahead = r.ahead;
+    t.graph = r.graph;
+    return t;
+}
+LuTune lu_tune_for(bem_ctx c, int n, bool single) {
+    std::lock_guard<std::mutex> g(c->tmu);
+    if (c->lu_mode == 2) {
+        auto it = c->tunes.find({n, single ? 1 : 0});
+        if (it != c->tunes.end()) return tune_of(it->second);
+    }
+    LuTune t;
+    t.split = single && (c->lu_mode == 1 || (c->lu_mode == 2 && n >= kSplitMin));
+    return t;
+}
+
+// Launches the factorization of LUbuf on the operator's lane with shape t:
+// directly, or (t.graph and `graphs`) by replaying the lane's captured graph
+// for these buffers and shape, capturing it first if the lane has none.
 template <typename R>
-void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) {
-    bem_ctx ctx = op->ctx;
-    hipStream_t s = op->lane->s;
+void lu_launch(bem_operator op, void* LUbuf, int* ipiv, void* inv, const LuTune& t, bool graphs) {
+    Lane* L = op->lane;
+    hipStream_t s = L->s;
     const int n = op->n;
-    launch_fill_int(s, op->lane->ws.status, 1, INT_MAX);
-    launch_fill_int(s, op->lane->ws.status + 1, 1, 0);
+    void* split = nullptr;
+    if (sizeof(R) == sizeof(float) && t.split) {
+        if (!op->split) op->split = op_alloc(op, lu_split_bytes(n));
+        split = op->split;
+    }
     LuStreams st;
     st.main = s;
-    st.side = op->lane->s2;
-    st.ready = op->lane->ev_ready;
-    st.panel = op->lane->ev_panel;
-    int rc = lu_factor<R>(st, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, op->lane->ws, &op->lane->pl);
-    if (rc == 2)
-        throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (" +
-                                              std::to_string(256 * (int64_t)op->lane->ws.max_blocks) + " rows)");
-    if (rc != 0) throw BemError(BEM_ERR_DEVICE, "LU panel launch failed");
+    st.side = L->s2;
+    st.ready = L->ev_ready;
+    st.panel = L->ev_panel;
+    auto fail = [&](int rc) {
+        if (rc == 2)
+            throw BemError(BEM_ERR_ADMISSION, "system larger than the panel workspace (" +
+                                                  std::to_string(256 * (int64_t)L->ws.max_blocks) + " rows)");
+        throw BemError(BEM_ERR_DEVICE, "LU panel launch failed");
+    };
+    if (!(t.graph && graphs)) {
+        const int rc = lu_factor<R>(st, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, L->ws, &L->pl, t, split);
+        if (rc) fail(rc);
+        return;
+    }
+    const int single = sizeof(R) == sizeof(float);
+    auto same = [&](const LuGraph& g) {
+        return g.lu == LUbuf && g.ipiv == ipiv && g.inv == inv && g.split == split && g.n == n &&
+               g.single == single && g.t.outer == t.outer && g.t.group == t.group && g.t.split == t.split &&
+               g.t.ahead == t.ahead;
+    };
+    auto it = std::find_if(L->graphs.begin(), L->graphs.end(), same);
+    if (it == L->graphs.end()) {
+        HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        const int rc = lu_factor<R>(st, (C2<R>*)LUbuf, n, n, ipiv, (C2<R>*)inv, L->ws, nullptr, t, split);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (rc || e != hipSuccess || !graph) {
+            if (graph) hipGraphDestroy(graph);
+            if (rc) fail(rc);
+            throw BemError(BEM_ERR_DEVICE, std::string("LU graph capture failed: ") + hipGetErrorString(e));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e2 != hipSuccess)
+            throw BemError(BEM_ERR_DEVICE, std::string("LU graph instantiation failed: ") + hipGetErrorString(e2));
+        if (L->graphs.size() >= kLaneGraphs) {
+            hipGraphExecDestroy(L->graphs.front().exec);
+            L->graphs.erase(L->graphs.begin());
+        }
+        L->graphs.push_back(LuGraph{LUbuf, ipiv, inv, split, n, single, t, exec});
+        it = L->graphs.end() - 1;
+    }
+    const double n3 = (double)n * n * n;
+    prof_begin(&L->pl, s, C_LUGRAPH);
+    HIPCHECK(hipGraphLaunch(it->exec, s));
+    prof_end(&L->pl, s, C_LUGRAPH, 8.0 / 3.0 * n3, 2.0 * (double)n * n * sizeof(C2<R>));
+}
+
+// Drops the lane's captured factorizations of buffers about to be released.
+void drop_graphs(Lane* L, const void* lu) {
+    for (auto it = L->graphs.begin(); it != L->graphs.end();) {
+        if (it->lu == lu) {
+            hipGraphExecDestroy(it->exec);
+            it = L->graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+template <typename R>
+void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv, const LuTune* force = nullptr) {
+    hipStream_t s = op->lane->s;
+    const int n = op->n;
+    const LuTune t = force ? *force : lu_tune_for(op->ctx, n, sizeof(R) == sizeof(float));
+    launch_fill_int(s, op->lane->ws.status, 1, INT_MAX);
+    launch_fill_int(s, op->lane->ws.status + 1, 1, 0);
+    // graphs only where the buffers repeat: a sweep's arena, or when forced
+    lu_launch<R>(op, LUbuf, ipiv, inv, t, op->use_arena || force != nullptr);
     HIPCHECK(hipGetLastError());
     int status[2];
     std::vector<int> hp(n);
@@ -521,6 +635,127 @@ void factor_into(bem_operator op, void* LUbuf, int* ipiv, int* perm, void* inv) 
     }
     HIPCHECK(hipMemcpyAsync(perm, pm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
+}
+
+// The tuner: factorizations of a synthetic n x n matrix on lane 0, timed
+// with HIP events, by coordinate descent from the default shape: bf16-split
+// updates (complex FP32), outer width 256, L2 group 4 and 16 (complex FP32),
+// no look-ahead, graph replay (up to 16384 unknowns).  A knob moves when the
+// time drops by more than 1.5 %.  Each shape is timed 3 times (once from
+// 16384 unknowns, where a factorization takes long enough to be steady)
+// after one untimed run.
+template <typename R>
+bem_lu_tune tune_lu(bem_ctx c, int n) {
+    constexpr bool single = sizeof(R) == sizeof(float);
+    Lane* L = c->lane0();
+    hipStream_t s = L->s;
+    const size_t mat = sizeof(C2<R>) * (size_t)n * n;
+    bem_operator_s op;
+    op.ctx = c;
+    op.lane = L;
+    op.n = n;
+    op.single = single;
+    std::vector<void*> bufs;
+    auto al = [&](size_t b) { void* p = c->alloc(b); bufs.push_back(p); return p; };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bem_lu_tune rep{};
+    try {
+        void* A0 = al(mat);
+        op.LU = al(mat);
+        op.ipiv = (int*)al(sizeof(int) * n);
+        op.inv = al(sizeof(C2<R>) * 2 * 4096 * (size_t)((n + 63) / 64));
+        launch_fill_test<R>(s, (C2<R>*)A0, n);
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        auto once = [&](const LuTune& t) {
+            HIPCHECK(hipMemcpyAsync(op.LU, A0, mat, hipMemcpyDeviceToDevice, s));
+            launch_fill_int(s, L->ws.status, 1, INT_MAX);
+            launch_fill_int(s, L->ws.status + 1, 1, 0);
+            HIPCHECK(hipEventRecord(e0, s));
+            lu_launch<R>(&op, op.LU, op.ipiv, op.inv, t, true);
+            HIPCHECK(hipEventRecord(e1, s));
+            int status[2];
+            HIPCHECK(hipMemcpyAsync(status, L->ws.status, sizeof status, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            if (status[0] != INT_MAX || status[1] != 0) throw BemError(BEM_ERR_DEVICE, "tuning factorization failed");
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms;
+        };
+        const int reps = n >= 16384 ? 1 : 3;
+        auto measure = [&](const LuTune& t) {
+            once(t);
+            double best = 1e300;
+            for (int r = 0; r < reps; ++r) best = std::min(best, once(t));
+            return best;
+        };
+        LuTune best;
+        double best_ms = measure(best);
+        rep.default_ms = best_ms;
+        int cand = 1;
+        auto consider = [&](LuTune t) {
+            const double ms = measure(t);
+            ++cand;
+            if (ms < 0.985 * best_ms) {
+                best = t;
+                best_ms = ms;
+            }
+        };
+        if (single) {
+            LuTune t = best;
+            t.split = 1;
+            consider(t);
+        }
+        {
+            LuTune t = best;
+            t.outer = 256;
+            consider(t);
+        }
+        if (single) {
+            const LuTune base = best;
+            for (int gsz : {4, 16}) {
+                LuTune t = base;
+                t.group = gsz;
+                consider(t);
+            }
+        }
+        {
+            LuTune t = best;
+            t.ahead = 0;
+            consider(t);
+        }
+        if (n <= 16384) {
+            LuTune t = best;
+            t.graph = 1;
+            consider(t);
+        }
+        rep.unknowns = n;
+        rep.precision = single ? BEM_MIXED : BEM_DOUBLE;
+        rep.outer = best.outer;
+        rep.group = best.group;
+        rep.split = best.split;
+        rep.ahead = best.ahead;
+        rep.graph = best.graph;
+        rep.tuned = 1;
+        rep.candidates = cand;
+        rep.best_ms = best_ms;
+    } catch (...) {
+        hipStreamSynchronize(s);
+        drop_graphs(L, op.LU);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        for (void* p : bufs) c->release(p);
+        for (void* p : op.heap) c->release(p);
+        throw;
+    }
+    drop_graphs(L, op.LU);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    for (void* p : bufs) c->release(p);
+    for (void* p : op.heap) c->release(p);
+    std::lock_guard<std::mutex> g(c->tmu);
+    c->tunes[{n, single ? 1 : 0}] = rep;
+    return rep;
 }
 
 // Low-frequency Mueller systems take their derivative blocks from the
@@ -1176,6 +1411,7 @@ size_t sweep_need(const bem_mesh m, const bem_problem& p) {
     const size_t esz = p.precision == BEM_MIXED ? sizeof(z32) : sizeof(z64);
     size_t b = esz * n * n * (p.boundary == BEM_FLUID ? 2 : 3);    // A, LU (+ B)
     b += esz * 2 * 4096 * ((n + 63) / 64);                           // block inverses
+    if (p.precision == BEM_MIXED) b += lu_split_bytes((int64_t)n);  // bf16 planes of the outer updates
     b += sizeof(z64) * n * 8 + sizeof(double) * n * 3 + sizeof(int) * n * 2;
     b += sizeof(z64) * (2 * nt + kFarBlocks + 4) + 64 * 1024;
     if (calderon_applies(m->h, p)) b += calderon_bytes(m->d.nt);
@@ -3092,6 +3328,47 @@ int bem_ctx_set_heavy_lanes(bem_ctx ctx, int32_t max_heavy) {
     return BEM_OK;
 }
 
+int bem_ctx_set_lu_tuning(bem_ctx ctx, int32_t mode) {
+    if (!ctx || mode < 0 || mode > 2) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    std::lock_guard<std::mutex> t(ctx->tmu);
+    ctx->lu_mode = mode;
+    return BEM_OK;
+}
+
+int bem_ctx_autotune(bem_ctx ctx, int64_t n, int32_t precision, bem_lu_tune* out) {
+    if (!ctx || n < 64 || n > 524288 || (precision != BEM_DOUBLE && precision != BEM_MIXED)) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        const bem_lu_tune r = precision == BEM_MIXED ? tune_lu<float>(ctx, (int)n) : tune_lu<double>(ctx, (int)n);
+        if (out) *out = r;
+    });
+}
+
+int bem_ctx_lu_tune(bem_ctx ctx, int64_t n, int32_t precision, bem_lu_tune* out) {
+    if (!ctx || !out || n < 1 || n > 524288 || (precision != BEM_DOUBLE && precision != BEM_MIXED))
+        return BEM_ERR_ARGUMENT;
+    const bool single = precision == BEM_MIXED;
+    {
+        std::lock_guard<std::mutex> g(ctx->tmu);
+        auto it = ctx->tunes.find({(int)n, single ? 1 : 0});
+        if (ctx->lu_mode == 2 && it != ctx->tunes.end()) {
+            *out = it->second;
+            return BEM_OK;
+        }
+    }
+    const LuTune t = lu_tune_for(ctx, (int)n, single);
+    *out = bem_lu_tune{};
+    out->unknowns = n;
+    out->precision = precision;
+    out->outer = t.outer;
+    out->group = t.group;
+    out->split = t.split;
+    out->ahead = t.ahead;
+    out->graph = t.graph;
+    return BEM_OK;
+}
+
 int bem_ctx_reset_stats(bem_ctx ctx) {
     if (!ctx) return BEM_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
@@ -3623,6 +3900,23 @@ int bem_sweep(bem_ctx ctx, int32_t n, const bem_mesh* meshes, const bem_problem*
             need[i] = sweep_need(meshes[i], problems[i]);
         }
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return need[a] > need[b]; });
+        // the tuner's first pass over each size class (lane 0, before the lanes start)
+        if (ctx->lu_mode == 2) {
+            std::set<std::pair<int, int>> classes;
+            for (int i = 0; i < n; ++i)
+                classes.insert({(int)sweep_unknowns(meshes[i], problems[i]), problems[i].precision == BEM_MIXED});
+            for (const auto& cl : classes) {
+                bool known;
+                {
+                    std::lock_guard<std::mutex> t(ctx->tmu);
+                    known = ctx->tunes.count(cl) != 0;
+                }
+                if (!known) {
+                    if (cl.second) tune_lu<float>(ctx, cl.first);
+                    else tune_lu<double>(ctx, cl.first);
+                }
+            }
+        }
         const size_t max_need = need[order[0]];
         // lanes: as many as the HBM budget holds arenas for (at most 8)
         uint64_t arena_now = 0;
@@ -4142,7 +4436,8 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t nrhs, const double* angles, bem_sol
 
 int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t flags, double* LUout, int64_t* ipiv_out,
                  const double* b, double* x) {
-    if (!ctx || !A || n64 <= 0 || n64 > 65536 || (flags & ~3) != 0) return BEM_ERR_ARGUMENT;
+    if (!ctx || !A || n64 <= 0 || n64 > 65536 || (flags & ~31) != 0) return BEM_ERR_ARGUMENT;
+    if ((flags & 4) && ((flags & 8) || !(flags & 1))) return BEM_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     return guarded(ctx, [&] {
         const int n = (int)n64;
@@ -4170,8 +4465,13 @@ int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t flags, doubl
             op.perm = (int*)al(sizeof(int) * n);
             op.inv = al(esz * 2 * 4096 * (size_t)((n + 63) / 64));
             HIPCHECK(hipMemcpyAsync(op.LU, host.data(), host.size(), hipMemcpyHostToDevice, s));
-            if (single) factor_into<float>(&op, op.LU, op.ipiv, op.perm, op.inv);
-            else factor_into<double>(&op, op.LU, op.ipiv, op.perm, op.inv);
+            LuTune t = lu_tune_for(ctx, n, single != 0);
+            if (flags & 4) t.split = 1;
+            if (flags & 8) t.split = 0;
+            if (flags & 16) t.graph = 1;
+            const LuTune* force = (flags & 28) ? &t : nullptr;
+            if (single) factor_into<float>(&op, op.LU, op.ipiv, op.perm, op.inv, force);
+            else factor_into<double>(&op, op.LU, op.ipiv, op.perm, op.inv, force);
             HIPCHECK(hipMemcpyAsync(host.data(), op.LU, host.size(), hipMemcpyDeviceToHost, s));
             std::vector<int> hp(n);
             HIPCHECK(hipMemcpyAsync(hp.data(), op.ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
@@ -4215,14 +4515,20 @@ int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t flags, doubl
             }
         } catch (...) {
             hipStreamSynchronize(s);
+            drop_graphs(op.lane, op.LU);
             for (void* p : bufs) ctx->release(p);
-            op.LU = op.inv = op.wv = op.wv2 = op.wv3 = nullptr;
+            for (void* p : op.heap) ctx->release(p);
+            op.heap.clear();
+            op.LU = op.inv = op.wv = op.wv2 = op.wv3 = op.split = nullptr;
             op.ipiv = op.perm = nullptr;
             op.vb = op.vx = nullptr;
             op.red = nullptr;
             throw;
         }
+        drop_graphs(op.lane, op.LU);
         for (void* p : bufs) ctx->release(p);
+        for (void* p : op.heap) ctx->release(p);
+        op.heap.clear();
     });
 }
 

@@ -183,10 +183,26 @@ struct LuStreams {
     hipStream_t main = nullptr, side = nullptr;
     hipEvent_t ready = nullptr, panel = nullptr;
 };
+// A synthetic n x n matrix for timing factorizations (diagonally weighted).
+template <typename R>
+void launch_fill_test(hipStream_t s, C2<R>* A, int n);
+// Shape of one factorization, chosen per size class by the context's tuner.
+struct LuTune {
+    int outer = 512;  // outer block width = K of the outer trailing updates (256 or 512)
+    int group = 8;    // tile rows per L2 group in the trailing updates' tile order (4, 8 or 16)
+    int split = 0;    // complex FP32 only: outer trailing updates as bf16-split 3M products
+    int ahead = 1;    // panels of block j + 1 on the side stream during block j's update
+    int graph = 0;    // host side: replay the launches as a captured HIP graph (lu_factor ignores it)
+};
+// Device bytes of the bf16 split planes of an n-unknown factorization
+// (L21 and U12 of one outer block: 9 planes each, rows padded to 128).
+inline size_t lu_split_bytes(int64_t n) { return (size_t)2 * 9 * 512 * 2 * (size_t)((n + 127) / 128 * 128); }
 // Returns 0, 2 (panel workspace too small) or 3 (HIP stream/event failure).
+// `split` (lu_split_bytes(n), complex FP32 with tune.split) holds the bf16
+// planes of the outer trailing updates.
 template <typename R>
 int lu_factor(const LuStreams& st, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv, PanelWS& ws,
-              void* prof);
+              void* prof, const LuTune& tune = LuTune{}, void* split = nullptr);
 // x = inv(A) b (one launch; ws.status[2] is raised if a device-side wait
 // expires, and x then holds NaN).  b, work and x must not overlap.
 template <typename R>

@@ -1077,16 +1077,16 @@ __global__ __launch_bounds__(512, 1) void k_gemm_sub(const C2<R>* __restrict__ A
 // groups of 8 tile rows, so the A and B panels its CUs share stay in its L2.
 constexpr int kG3K = 32;       // K chunk
 constexpr int kG3P = 36;       // LDS row pitch (floats)
-constexpr int kG3Group = 8;    // tile rows per group
+constexpr int kG3Group = 8;    // tile rows per group (default; the tuner may pick 4 or 16)
 
-__device__ __forceinline__ void g3_tile(int nx, int ny, int& tm, int& tn) {
+__device__ __forceinline__ void g3_tile(int nx, int ny, int group, int& tm, int& tn) {
     const int L = blockIdx.y * nx + blockIdx.x;
     const int T = nx * ny;
     const int q = T >> 3, r = T & 7, xcd = L & 7, idx = L >> 3;
     const int lin = xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
-    const int per = kG3Group * nx;
-    const int g = lin / per, first = g * kG3Group;
-    const int gs = (ny - first) < kG3Group ? (ny - first) : kG3Group;
+    const int per = group * nx;
+    const int g = lin / per, first = g * group;
+    const int gs = (ny - first) < group ? (ny - first) : group;
     const int w = lin - g * per;
     tm = first + w % gs;
     tn = w / gs;
@@ -1098,7 +1098,7 @@ __device__ __forceinline__ void g3_tile(int nx, int ny, int& tm, int& tn) {
 template <int ASSIGN, bool FULL>
 __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, int64_t lda, const z32* __restrict__ B,
                                                    int64_t ldb, z32* __restrict__ C, int64_t ldc, int M, int N,
-                                                   int K) {
+                                                   int K, int group) {
     typedef float v4 __attribute__((ext_vector_type(4)));
     constexpr int KV = kG3P / 4;  // v4 slots per LDS row
     __shared__ v4 sA[2][2][kGM][KV];
@@ -1107,7 +1107,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: tile offsets stay scalar
     const int wm = (wv >> 2) * 64, wn = (wv & 3) * 32;
     int tm, tn;
-    g3_tile(gridDim.x, gridDim.y, tm, tn);
+    g3_tile(gridDim.x, gridDim.y, group, tm, tn);
     const int m0 = tm * kGM, n0 = tn * kGN;
     const bool full = FULL || (m0 + kGM <= M && n0 + kGN <= N);
     const int li = lane & 15, lq = lane >> 4;
@@ -1247,6 +1247,213 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m(const z32* __restrict__ A, in
 }
 
 // ---------------------------------------------------------------------------
+// Outer trailing updates as bf16-split products.  Each FP32 operand value is
+// split three ways, x = hi + mid + lo with every piece a bf16 (round to
+// nearest even; the residuals x - hi and x - hi - mid are exact in FP32, so
+// the three pieces carry all 24 significand bits), and a real product is the
+// sum of the six piece products whose size is at least 2^-16 of the leading
+// one (hi hi, hi mid, mid hi, mid mid, hi lo, lo hi; the three dropped ones
+// are below 2^-24 relative, FP32's own rounding unit), each on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation (Henry, Tang and Heinecke,
+// "Leveraging the bfloat16 artificial intelligence datatype for
+// higher-precision computations", ARITH 2019).  With the 3M form's three real
+// products the update runs 18 bf16 products per complex product: 36 MNK
+// matrix-core flops where the FP32 3M kernel runs 6 MNK at one sixteenth the
+// rate, 2.67 times the FP32 kernel's ceiling.
+//
+// The operands are split once per outer block into planes
+// [stream * 3 + piece][k / 8][row][8] (streams re, im, re + im of the 3M form;
+// rows of L21, columns of U12; rows padded by zeros to a multiple of 128), so
+// the GEMM stages 16-byte runs of 8 k, the MFMA operand of one lane, with
+// coalesced loads and no conversion work.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
+    h = (__bf16)x;
+    const float r1 = x - (float)h;
+    m = (__bf16)r1;
+    l = (__bf16)(r1 - (float)m);
+}
+
+// the nine planes of eight consecutive k of one row or column
+__device__ __forceinline__ void split_store(const z32 (&v)[8], bf8v* __restrict__ out, int64_t plane, int64_t at) {
+    bf8v o[9];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        __bf16 h, m, l;
+        split3(v[j].x, h, m, l);
+        o[0][j] = h;
+        o[1][j] = m;
+        o[2][j] = l;
+        split3(v[j].y, h, m, l);
+        o[3][j] = h;
+        o[4][j] = m;
+        o[5][j] = l;
+        split3(v[j].x + v[j].y, h, m, l);
+        o[6][j] = h;
+        o[7][j] = m;
+        o[8][j] = l;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) out[q * plane + at] = o[q];
+}
+
+// L21 (M rows x K, row-major at A): thread (row m, k group kg), rows fastest.
+__global__ __launch_bounds__(256) void k_split_rows(const z32* __restrict__ A, int64_t lda, int M, int Mpad, int K,
+                                                    bf8v* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int KG = K / 8;
+    if (e >= (int64_t)Mpad * KG) return;
+    const int m = (int)(e % Mpad), kg = (int)(e / Mpad);
+    z32 v[8];
+    if (m < M) {
+        const z32* p = A + (int64_t)m * lda + 8 * kg;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = p[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = z32{0.f, 0.f};
+    }
+    split_store(v, out, (int64_t)KG * Mpad, (int64_t)kg * Mpad + m);
+}
+
+// U12 (K x N, row-major at B): thread (column c, k group kg), columns fastest.
+__global__ __launch_bounds__(256) void k_split_cols(const z32* __restrict__ B, int64_t ldb, int N, int Npad, int K,
+                                                    bf8v* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int KG = K / 8;
+    if (e >= (int64_t)Npad * KG) return;
+    const int c = (int)(e % Npad), kg = (int)(e / Npad);
+    z32 v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = c < N ? B[(int64_t)(8 * kg + j) * ldb + c] : z32{0.f, 0.f};
+    split_store(v, out, (int64_t)KG * Npad, (int64_t)kg * Npad + c);
+}
+
+// C[M x N] -= A B from split planes SA (rows, Mpad) and SB (columns, Npad,
+// starting at column boff, a multiple of 128); K a multiple of 32.  128 x 128
+// tiles, 8 waves as 2 x 4 of 64 x 32 (4 x 2 MFMA blocks), the tile order of
+// k_gemm3m.  Each 32-wide k chunk runs as three phases, one per 3M stream:
+// a phase stages the stream's three pieces of both operands (48 KB) in one
+// half of a double-buffered LDS (rows padded to 80 bytes, so the 16 rows a
+// 16-lane group reads fall on distinct banks) while the previous phase's 48
+// MFMAs per wave run; one barrier per phase.  The padded rows and columns
+// are zero, so only the stores of C test the edges.
+constexpr int kBfPitch = 5;  // LDS row pitch in 16-byte units
+__global__ __launch_bounds__(512, 1) void k_gemm3m_bf(const bf8v* __restrict__ SA, int Mpad,
+                                                      const bf8v* __restrict__ SB, int Npad, int boff,
+                                                      z32* __restrict__ C, int64_t ldc, int M, int N, int K,
+                                                      int group) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    __shared__ bf8v sA[2][3][kGM][kBfPitch];
+    __shared__ bf8v sB[2][3][kGN][kBfPitch];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = (wv >> 2) * 64, wn = (wv & 3) * 32;
+    int tm, tn;
+    g3_tile(gridDim.x, gridDim.y, group, tm, tn);
+    const int m0 = tm * kGM, n0 = tn * kGN;
+    const bool full = m0 + kGM <= M && n0 + kGN <= N;
+    const int li = lane & 15, lq = lane >> 4;
+    v4 t1[4][2], t2[4][2], t3[4][2];
+    // C - A B: t1 = -Re C + Ar Br, t2 = Ai Bi, t3 = -Re C - Im C + (Ar + Ai)(Br + Bi)
+    const z32* cl = C + (int64_t)(4 * lq) * ldc + li;
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            t2[x][y] = v4{0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rb = m0 + wm + x * 16 + r, cb = n0 + wn + y * 16;
+                z32 c{0.f, 0.f};
+                if (full || (rb + 4 * lq < M && cb + li < N)) c = cl[(int64_t)rb * ldc + cb];
+                t1[x][y][r] = -c.x;
+                t3[x][y][r] = -c.x - c.y;
+            }
+        }
+    const int lm = tid & 127, lk = tid >> 7;
+    const int KG = K / 8;
+    const int64_t psA = (int64_t)KG * Mpad, psB = (int64_t)KG * Npad;
+    const bf8v* pa = SA + (int64_t)lk * Mpad + m0 + lm;
+    const bf8v* pb = SB + (int64_t)lk * Npad + boff + n0 + lm;
+    bf8v ra[3], rb[3];
+    auto gload = [&](int c, int st) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            ra[p] = pa[(st * 3 + p) * psA + (int64_t)(4 * c) * Mpad];
+            rb[p] = pb[(st * 3 + p) * psB + (int64_t)(4 * c) * Npad];
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            sA[buf][p][lm][lk] = ra[p];
+            sB[buf][p][lm][lk] = rb[p];
+        }
+    };
+    auto products = [&](int buf, v4 (&acc)[4][2]) {
+        bf8v a[3][4], b[3][2];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+#pragma unroll
+            for (int x = 0; x < 4; ++x) a[p][x] = sA[buf][p][wm + 16 * x + li][lq];
+#pragma unroll
+            for (int y = 0; y < 2; ++y) b[p][y] = sB[buf][p][wn + 16 * y + li][lq];
+        }
+        // small terms first: mid mid, hi lo, lo hi, hi mid, mid hi, hi hi
+#define BF_TERM(P, Q)                                                                                     \
+    _Pragma("unroll") for (int x = 0; x < 4; ++x) _Pragma("unroll") for (int y = 0; y < 2; ++y) acc[x][y] = \
+        __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[P][x], b[Q][y], acc[x][y], 0, 0, 0);
+        BF_TERM(1, 1)
+        BF_TERM(0, 2)
+        BF_TERM(2, 0)
+        BF_TERM(0, 1)
+        BF_TERM(1, 0)
+        BF_TERM(0, 0)
+#undef BF_TERM
+    };
+    const int nc = K / 32;
+    gload(0, 0);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int c = 0; c < nc; ++c) {
+        gload(c, 1);
+        products(buf, t1);
+        lstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+        gload(c, 2);
+        products(buf, t2);
+        lstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+        const bool more = c + 1 < nc;
+        if (more) gload(c + 1, 0);
+        products(buf, t3);
+        if (more) lstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    z32* cs = C + (int64_t)(4 * lq) * ldc + li;
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rb = m0 + wm + x * 16 + r, cb = n0 + wn + y * 16;
+                if (full || (rb + 4 * lq < M && cb + li < N)) {
+                    const float re = t1[x][y][r] - t2[x][y][r];
+                    const float im = t3[x][y][r] - t1[x][y][r] - t2[x][y][r];
+                    cs[(int64_t)rb * ldc + cb] = z32{-re, -im};
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
 // drivers
 // ---------------------------------------------------------------------------
 struct ProfHook;  // defined in the API unit; null when profiling is off
@@ -1256,7 +1463,8 @@ struct ProfHook;  // defined in the API unit; null when profiling is off
     } while (0)
 void prof_begin(void* prof, hipStream_t s, int cls);
 void prof_end(void* prof, hipStream_t s, int cls, double flops, double bytes);
-enum { CLS_PANEL = 2, CLS_GEMM = 3, CLS_TRSM = 4, CLS_SWAP = 5, CLS_SOLVE = 6, CLS_GEMM64 = 12 };
+enum { CLS_PANEL = 2, CLS_GEMM = 3, CLS_TRSM = 4, CLS_SWAP = 5, CLS_SOLVE = 6, CLS_GEMM64 = 12, CLS_SPLIT = 15,
+       CLS_GEMMBF = 16 };
 
 
 // Trailing-update GEMM: 3M products for complex FP32; four real products for
@@ -1275,22 +1483,23 @@ static inline bool split_pays(int M, int N, int Ma, int Na) {
 
 template <typename R, int ASSIGN>
 static void launch_gemm(hipStream_t s, dim3 grid, const C2<R>* A, int64_t lda, const C2<R>* B, int64_t ldb, C2<R>* C,
-                        int64_t ldc, int M, int N, int K) {
+                        int64_t ldc, int M, int N, int K, int group = kG3Group) {
     if constexpr (sizeof(R) == sizeof(float)) {
         const int Ma = M - M % kGM, Na = N - N % kGN;
         if (K % kG3K == 0 && split_pays(M, N, Ma, Na)) {
             hipLaunchKernelGGL((k_gemm3m<ASSIGN, true>), dim3(Na / kGN, Ma / kGM), dim3(512), 0, s, (const z32*)A,
-                               lda, (const z32*)B, ldb, (z32*)C, ldc, Ma, Na, K);
+                               lda, (const z32*)B, ldb, (z32*)C, ldc, Ma, Na, K, group);
             if (Na < N)
                 hipLaunchKernelGGL((k_gemm3m<ASSIGN, false>), dim3(1, (M + kGM - 1) / kGM), dim3(512), 0, s,
-                                   (const z32*)A, lda, (const z32*)(B + Na), ldb, (z32*)(C + Na), ldc, M, N - Na, K);
+                                   (const z32*)A, lda, (const z32*)(B + Na), ldb, (z32*)(C + Na), ldc, M, N - Na, K,
+                                   group);
             if (Ma < M)
                 hipLaunchKernelGGL((k_gemm3m<ASSIGN, false>), dim3(Na / kGN, 1), dim3(512), 0, s,
                                    (const z32*)(A + (int64_t)Ma * lda), lda, (const z32*)B, ldb,
-                                   (z32*)(C + (int64_t)Ma * ldc), ldc, M - Ma, Na, K);
+                                   (z32*)(C + (int64_t)Ma * ldc), ldc, M - Ma, Na, K, group);
         } else {
             hipLaunchKernelGGL((k_gemm3m<ASSIGN, false>), grid, dim3(512), 0, s, (const z32*)A, lda, (const z32*)B,
-                               ldb, (z32*)C, ldc, M, N, K);
+                               ldb, (z32*)C, ldc, M, N, K, group);
         }
     } else {
         const int Ma = M - M % kGM, Na = N - N % kGN;
@@ -1332,14 +1541,51 @@ static void launch_trsm(hipStream_t s, C2<R>* A, int64_t lda, int k0, int bs, in
 // Timed as lu_gemm (FP32, the 3M kernel) or lu_gemm_fp64.
 template <typename R>
 static void launch_update(hipStream_t s, C2<R>* A, int64_t lda, int r0, int c0, int ka, int M, int N, int K,
-                          void* prof) {
+                          void* prof, int group = kG3Group) {
     if (M <= 0 || N <= 0 || K <= 0) return;
     const int cls = sizeof(R) == sizeof(float) ? CLS_GEMM : CLS_GEMM64;
     prof_begin(prof, s, cls);
     dim3 grid((N + kGN - 1) / kGN, (M + kGM - 1) / kGM);
     launch_gemm<R, 0>(s, grid, A + (int64_t)r0 * lda + ka, lda, A + (int64_t)ka * lda + c0, lda,
-                      A + (int64_t)r0 * lda + c0, lda, M, N, K);
+                      A + (int64_t)r0 * lda + c0, lda, M, N, K, group);
     prof_end(prof, s, cls, 8.0 * (double)M * N * K, (2.0 * (double)M * N + (double)(M + N) * K) * sizeof(C2<R>));
+}
+
+// The split planes of outer block [K0, K0 + K): L21 (rows r0 .. n) and U12
+// (columns r0 .. n) of the square trailing matrix at r0 = K0 + K.
+struct SplitPlanes {
+    bf8v *rows = nullptr, *cols = nullptr;
+    int M = 0, Mpad = 0, K = 0;
+};
+static SplitPlanes launch_split(hipStream_t s, z32* A, int64_t lda, int n, int K0, int K, void* buf, void* prof) {
+    SplitPlanes P;
+    P.M = n - K0 - K;
+    P.Mpad = (P.M + 127) / 128 * 128;
+    P.K = K;
+    P.rows = (bf8v*)buf;
+    P.cols = P.rows + (size_t)9 * (K / 8) * P.Mpad;
+    const int r0 = K0 + K;
+    const int64_t tot = (int64_t)P.Mpad * (K / 8);
+    prof_begin(prof, s, CLS_SPLIT);
+    hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       (const z32*)(A + (int64_t)r0 * lda + K0), lda, P.M, P.Mpad, K, P.rows);
+    hipLaunchKernelGGL(k_split_cols, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       (const z32*)(A + (int64_t)K0 * lda + r0), lda, P.M, P.Mpad, K, P.cols);
+    prof_end(prof, s, CLS_SPLIT, 0.0, 2.0 * (double)P.M * K * (sizeof(z32) + 9 * sizeof(__bf16)));
+    return P;
+}
+// Columns [c0, c0 + N) of the trailing matrix (c0 relative to its first
+// column, a multiple of 128) from the split planes: the bf16 GEMM, timed as
+// lu_gemm_bf16 with the algorithmic 8 MNK flops of the complex update.
+static void launch_update_split(hipStream_t s, z32* A, int64_t lda, int r0, const SplitPlanes& P, int c0, int N,
+                                int group, void* prof) {
+    if (P.M <= 0 || N <= 0) return;
+    prof_begin(prof, s, CLS_GEMMBF);
+    dim3 grid((N + kGN - 1) / kGN, P.Mpad / kGM);
+    hipLaunchKernelGGL(k_gemm3m_bf, grid, dim3(512), 0, s, P.rows, P.Mpad, P.cols, P.Mpad, c0,
+                       A + (int64_t)r0 * lda + r0 + c0, lda, P.M, N, P.K, group);
+    prof_end(prof, s, CLS_GEMMBF, 8.0 * (double)P.M * N * P.K,
+             (2.0 * (double)P.M * N * sizeof(z32) + 9.0 * (double)(P.M + N) * P.K * sizeof(__bf16)));
 }
 
 // Two-level right-looking LU: 64-wide panels inside 512-wide outer blocks, so
@@ -1352,7 +1598,7 @@ constexpr int kOuter = 512;
 // factors, L21, and the updates confined to the block's own columns.
 template <typename R>
 static int panel_phase(hipStream_t s, C2<R>* A, int64_t lda, int n, int K0, int Kend, int* ipiv, C2<R>* inv,
-                       PanelWS& ws, void* prof) {
+                       PanelWS& ws, void* prof, int group) {
     for (int k0 = K0; k0 < Kend; k0 += kNB) {
         const int w = (Kend - k0) < kNB ? (Kend - k0) : kNB;
         const int m = n - k0;
@@ -1373,32 +1619,41 @@ static int panel_phase(hipStream_t s, C2<R>* A, int64_t lda, int n, int K0, int 
         launch_laswp<R>(s, A, lda, k0, w, ipiv, K0, k0, k0 + w, Kend, prof);
         if (k0 + w < Kend) {
             launch_trsm<R>(s, A, lda, k0, w, k0 + w, Kend, inv, prof);
-            launch_update<R>(s, A, lda, k0 + w, k0 + w, k0, n - k0 - w, Kend - k0 - w, w, prof);
+            launch_update<R>(s, A, lda, k0 + w, k0 + w, k0, n - k0 - w, Kend - k0 - w, w, prof, group);
         }
     }
     return 0;
 }
 
-// Two-level right-looking LU: 64-wide panels inside 512-wide outer blocks, so
-// the trailing update of each outer block is one K = 512 MFMA GEMM.  With a
-// side stream (LuStreams::side) the panels of outer block j + 1 run there,
+// Two-level right-looking LU: 64-wide panels inside outer blocks of
+// tune.outer columns (512 by default), so the trailing update of each outer
+// block is one K = tune.outer MFMA GEMM.  With a side stream
+// (LuStreams::side, tune.ahead) the panels of outer block j + 1 run there,
 // overlapped with block j's trailing update: the update of block j + 1's
 // columns goes first (event `ready`), block j + 1's panel phase follows on the
 // side stream (event `panel`), and the main stream updates the remaining
 // columns meanwhile, then waits for `panel` before applying block j + 1's
-// interchanges elsewhere.  Columns touched concurrently are disjoint.
+// interchanges elsewhere.  Columns touched concurrently are disjoint.  With
+// tune.split (complex FP32) the outer trailing updates run from bf16 split
+// planes of L21 and U12 (k_gemm3m_bf), written once per outer block into
+// `split`.  The launch sequence depends only on n and the tune, never on the
+// data, so it can be captured once and replayed.
 template <typename R>
-int lu_factor(const LuStreams& st, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv, PanelWS& ws, void* prof) {
+int lu_factor(const LuStreams& st, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R>* inv, PanelWS& ws, void* prof,
+              const LuTune& tune, void* split) {
     hipStream_t s = st.main;
-    const bool ahead = st.side != nullptr;
-    int rc = panel_phase<R>(s, A, lda, n, 0, std::min(n, kOuter), ipiv, inv, ws, prof);
+    const bool ahead = st.side != nullptr && tune.ahead;
+    const int outer = tune.outer == 256 ? 256 : kOuter;
+    const int group = tune.group;
+    const bool use_split = sizeof(R) == sizeof(float) && tune.split && split != nullptr;
+    int rc = panel_phase<R>(s, A, lda, n, 0, std::min(n, outer), ipiv, inv, ws, prof, group);
     if (rc) return rc;
-    for (int K0 = 0; K0 < n; K0 += kOuter) {
-        const int KW = (n - K0) < kOuter ? (n - K0) : kOuter;
+    for (int K0 = 0; K0 < n; K0 += outer) {
+        const int KW = (n - K0) < outer ? (n - K0) : outer;
         const int Kend = K0 + KW;
         if (K0 > 0) {
             if (ahead) HIPCHECK_LU(hipStreamWaitEvent(s, st.panel, 0));
-            else if ((rc = panel_phase<R>(s, A, lda, n, K0, Kend, ipiv, inv, ws, prof))) return rc;
+            else if ((rc = panel_phase<R>(s, A, lda, n, K0, Kend, ipiv, inv, ws, prof, group))) return rc;
         }
         // the outer block's interchanges on every other column
         launch_laswp<R>(s, A, lda, K0, KW, ipiv, 0, K0, Kend, n, prof);
@@ -1407,18 +1662,32 @@ int lu_factor(const LuStreams& st, C2<R>* A, int64_t lda, int n, int* ipiv, C2<R
         for (int i0 = K0; i0 < Kend; i0 += kNB) {
             const int wi = (Kend - i0) < kNB ? (Kend - i0) : kNB;
             launch_trsm<R>(s, A, lda, i0, wi, Kend, n, inv, prof);
-            launch_update<R>(s, A, lda, i0 + wi, Kend, i0, Kend - i0 - wi, n - Kend, wi, prof);
+            launch_update<R>(s, A, lda, i0 + wi, Kend, i0, Kend - i0 - wi, n - Kend, wi, prof, group);
         }
-        const int Kend2 = std::min(n, Kend + kOuter);
+        const int Kend2 = std::min(n, Kend + outer);
+        // update of columns [Kend + c0, Kend + c0 + N)
+        SplitPlanes P;
+        if constexpr (sizeof(R) == sizeof(float)) {
+            if (use_split) P = launch_split(s, (z32*)A, lda, n, K0, KW, split, prof);
+        }
+        auto update = [&](int c0, int N) {
+            if constexpr (sizeof(R) == sizeof(float)) {
+                if (use_split) {
+                    launch_update_split(s, (z32*)A, lda, Kend, P, c0, N, group, prof);
+                    return;
+                }
+            }
+            launch_update<R>(s, A, lda, Kend, Kend + c0, K0, n - Kend, N, KW, prof, group);
+        };
         if (ahead) {
-            launch_update<R>(s, A, lda, Kend, Kend, K0, n - Kend, Kend2 - Kend, KW, prof);
+            update(0, Kend2 - Kend);
             HIPCHECK_LU(hipEventRecord(st.ready, s));
             HIPCHECK_LU(hipStreamWaitEvent(st.side, st.ready, 0));
-            if ((rc = panel_phase<R>(st.side, A, lda, n, Kend, Kend2, ipiv, inv, ws, prof))) return rc;
+            if ((rc = panel_phase<R>(st.side, A, lda, n, Kend, Kend2, ipiv, inv, ws, prof, group))) return rc;
             HIPCHECK_LU(hipEventRecord(st.panel, st.side));
-            launch_update<R>(s, A, lda, Kend, Kend2, K0, n - Kend, n - Kend2, KW, prof);
+            update(Kend2 - Kend, n - Kend2);
         } else {
-            launch_update<R>(s, A, lda, Kend, Kend, K0, n - Kend, n - Kend, KW, prof);
+            update(0, n - Kend);
         }
     }
     return 0;
@@ -1804,7 +2073,29 @@ void launch_calderon_bottom(hipStream_t s, int nt, const z64* Me, const z64* Mi,
                        1.0 / g, A);
 }
 
+// Synthetic matrix for timing factorizations: entries in [-1/2, 1/2) from a
+// hash of their index, the diagonal raised by 2 so no pivot vanishes.
+template <typename R>
+__global__ void k_fill_test(C2<R>* A, int n) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    uint64_t h = (uint64_t)e * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    const double re = (double)(h & 0xFFFFFF) / 16777216.0 - 0.5, im = (double)((h >> 24) & 0xFFFFFF) / 16777216.0 - 0.5;
+    const bool diag = e / n == e % n;
+    A[e] = C2<R>{(R)(re + (diag ? 2.0 : 0.0)), (R)im};
+}
+template <typename R>
+void launch_fill_test(hipStream_t s, C2<R>* A, int n) {
+    const int64_t tot = (int64_t)n * n;
+    hipLaunchKernelGGL((k_fill_test<R>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, A, n);
+}
+
 // explicit instantiations
+template void launch_fill_test<double>(hipStream_t, z64*, int);
+template void launch_fill_test<float>(hipStream_t, z32*, int);
 template void launch_row_stats<double>(hipStream_t, const z64*, int64_t, int, double*, double*);
 template void launch_row_stats<float>(hipStream_t, const z32*, int64_t, int, double*, double*);
 template void launch_col_max<double>(hipStream_t, const z64*, int64_t, int, const double*, double*);
@@ -1823,8 +2114,10 @@ template void gemm_assign<double>(hipStream_t, const z64*, int64_t, const z64*, 
 template void lu_solve_many<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, z64*, int64_t, int, void*);
 template void launch_calderon_bottom<double>(hipStream_t, int, const z64*, const z64*, double, z64*);
 template void launch_calderon_bottom<float>(hipStream_t, int, const z64*, const z64*, double, z32*);
-template int lu_factor<double>(const LuStreams&, z64*, int64_t, int, int*, z64*, PanelWS&, void*);
-template int lu_factor<float>(const LuStreams&, z32*, int64_t, int, int*, z32*, PanelWS&, void*);
+template int lu_factor<double>(const LuStreams&, z64*, int64_t, int, int*, z64*, PanelWS&, void*, const LuTune&,
+                               void*);
+template int lu_factor<float>(const LuStreams&, z32*, int64_t, int, int*, z32*, PanelWS&, void*, const LuTune&,
+                              void*);
 template void lu_solve<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, const z64*, z64*, z64*, PanelWS&, void*);
 template void lu_solve<float>(hipStream_t, const z32*, int64_t, int, const int*, const z32*, const z32*, z32*, z32*, PanelWS&, void*);
 template void lu_solve_h<double>(hipStream_t, const z64*, int64_t, int, const int*, const z64*, const z64*, z64*, z64*, PanelWS&, void*);

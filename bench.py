@@ -55,6 +55,10 @@ A_MAJOR, B_MINOR = 0.02, 0.01
 G_RHO, H_C = 1.0357, 1.0279
 MFMA_FP32_PEAK = 157.3e12
 MFMA_FP64_PEAK = 78.6e12
+MFMA_BF16_PEAK = 2.5e15  # dense
+# matrix-core flops the bf16-split trailing update executes per algorithmic
+# complex flop: 3 real products (3M) x 6 piece products x 2 M N K, over 8 M N K
+BF16_SPLIT_FLOPS = 36.0 / 8.0
 METRIC = "BEM sweep solves/s at |ΔTS|≤0.01 dB vs FP64 (N=20480); p50/p99 latency"
 
 
@@ -69,9 +73,40 @@ def sweep_plan(bh, ctx, n_points, max_level):
 
 
 TRAFFIC_PROFILE = os.path.join("profiles", "lu_gemm3m_traffic_fluid_spheroid_n20480.md")
+TRAFFIC_PROFILE_BF16 = os.path.join("profiles", "lu_gemm_bf16_traffic_fluid_spheroid_n20480.md")
 
 
-def measured_traffic():
+def gemm_roofline(fp32, bf16, traffic_note):
+    """Roofline of the trailing update that carried most of the time: the
+    FP32 3M kernel (k_gemm3m, vs the FP32 MFMA peak) or the bf16-split one
+    (k_gemm3m_bf, vs the dense bf16 MFMA peak in the matrix-core flops it
+    executes, 4.5 per algorithmic flop; `algorithmic` = 8 M N K / time, which
+    may exceed the FP32 peak)."""
+    def rate(st):
+        return st["flops"] / (st["ms"] * 1e-3) if st["ms"] > 0 else 0.0
+
+    if bf16["ms"] > fp32["ms"]:
+        g, r = bf16, rate(bf16)
+        out = {"kernel": "k_gemm3m_bf (LU outer trailing update, 3M complex, bf16-split FP32 products)",
+               "bound": "mfma", "achieved": r * BF16_SPLIT_FLOPS / 1e12, "peak": MFMA_BF16_PEAK / 1e12,
+               "unit": "TFLOP/s", "frac": r * BF16_SPLIT_FLOPS / MFMA_BF16_PEAK,
+               "algorithmic_tflops": r / 1e12, "algorithmic_vs_fp32_peak": r / MFMA_FP32_PEAK,
+               "traffic": measured_traffic(TRAFFIC_PROFILE_BF16), "traffic_source": TRAFFIC_PROFILE_BF16}
+    else:
+        g, r = fp32, rate(fp32)
+        out = {"kernel": "k_gemm3m<0, FULL> (LU trailing update, 3M complex)", "bound": "mfma",
+               "achieved": r / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
+               "frac": r / MFMA_FP32_PEAK, "traffic": measured_traffic(TRAFFIC_PROFILE),
+               "traffic_source": TRAFFIC_PROFILE}
+    out.update({"traffic_basis": traffic_note,
+                "algorithmic_bytes_per_launch": g["bytes"] / max(1, g["launches"]),
+                "launches": g["launches"], "avg_launch_ms": g["ms"] / max(1, g["launches"]),
+                "other_update_kernel": {"launches": (fp32 if g is bf16 else bf16)["launches"],
+                                        "ms": (fp32 if g is bf16 else bf16)["ms"]}})
+    return out
+
+
+def measured_traffic(path=TRAFFIC_PROFILE):
     """HBM bytes per launch of the FP32 trailing update, from the committed
     rocprofv3 --pmc summary of `bench.py --workload solve` (one N = 20480
     Mueller solve; the solves of that size carry most of the sweep's GEMM
@@ -79,7 +114,7 @@ def measured_traffic():
     reading for a plain copy of known size.  None when the summary is
     absent."""
     try:
-        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+        with open(os.path.join(ROOT, path)) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -311,6 +346,7 @@ def solve_main(args):
     warm = bh.mesh(bh.Spheroid(A_MAJOR, B_MINOR), level=3, ctx=ctx)  # loads every kernel once
     bh.bem(warm, fl, 500.0, incidence_angle=0.0, precision="mixed")
     mesh = bh.mesh(bh.Spheroid(A_MAJOR, B_MINOR), level=5, ctx=ctx)
+    log(f"tuning the factorization shape for {2 * mesh.n} unknowns: {ctx.autotune(2 * mesh.n, 'mixed')}")
     times = []
     for s in range(args.warmup + args.steps):
         if s == args.warmup:
@@ -323,8 +359,7 @@ def solve_main(args):
             times.append(time.perf_counter() - t0)
         log(f"solve {s + 1}: {time.perf_counter() - t0:.3f}s, TS {bh.target_strength(f):.4f} dB")
     ctx.set_profiling(False)
-    g = ctx.kernel_stats("lu_gemm")
-    achieved = g["flops"] / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
+    g, gb = ctx.kernel_stats("lu_gemm"), ctx.kernel_stats("lu_gemm_bf16")
     t = float(np.mean(times))
     out = {
         "metric": "single fluid Mueller solve time (N=20480, 2N=40960 unknowns, mixed precision)",
@@ -335,12 +370,10 @@ def solve_main(args):
                                "icosphere L5 (N=20480), Mueller, mixed precision, one incidence"},
         "target_strength_db": bh.target_strength(f),
         "diagnostics": {k: v for k, v in sol.diagnostics.items() if isinstance(v, (int, float, str, bool))},
-        "roofline": {"kernel": "k_gemm3m<0, FULL> (LU trailing update, 3M complex)", "bound": "mfma",
-                     "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / MFMA_FP32_PEAK, "traffic": measured_traffic(),
-                     "traffic_source": TRAFFIC_PROFILE,
-                     "algorithmic_bytes_per_launch": g["bytes"] / max(1, g["launches"]),
-                     "launches": g["launches"], "avg_launch_ms": g["ms"] / max(1, g["launches"])},
+        "lu_tune": ctx.lu_tune(2 * mesh.n, "mixed"),
+        "roofline": gemm_roofline(g, gb, "per launch of this solve's trailing updates"),
+        "kernels": {k: ctx.kernel_stats(k) for k in ("lu_panel", "lu_gemm", "lu_gemm_bf16", "lu_split", "lu_trsm",
+                                                        "lu_swap", "solve", "residual")},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -486,6 +519,17 @@ def sweep_main(args):
         sizes[meshes[L].n] = sizes.get(meshes[L].n, 0) + 1
     log(f"plan: {len(plan)} solves, panels per class {sizes}")
 
+    # the tuner's pass over each size class and precision (untimed; a sweep
+    # would otherwise run it on its first call)
+    t_tune = time.perf_counter()
+    tunes = {}
+    for n_panels in sorted(sizes):
+        for prec in ("mixed", "double"):
+            r = ctx.autotune(2 * n_panels, prec)
+            tunes[f"{2 * n_panels}/{prec}"] = r
+            log(f"tuned {2 * n_panels} unknowns, {prec}: outer {r['outer']} group {r['group']} split {r['split']} "
+                f"ahead {r['ahead']} graph {r['graph']}: {r['best_ms']:.1f} ms (default {r['default_ms']:.1f} ms)")
+    tune_s = time.perf_counter() - t_tune
     # load every kernel once from one thread before the concurrent lanes start
     fl = bh.FluidFilled(G_RHO, H_C)
     small = meshes[min(meshes)]
@@ -541,8 +585,8 @@ def sweep_main(args):
         log(f"step {s + 1}/{args.steps}: {t:.2f}s ({len(plan) / t:.2f} solves/s), at {elapsed():.0f}s")
     ctx.set_profiling(False)
     stats = {name: ctx.kernel_stats(name) for name in
-             ("assemble", "near", "lu_panel", "lu_gemm", "lu_gemm_fp64", "lu_trsm", "lu_swap", "solve", "residual",
-              "far_field")}
+             ("assemble", "near", "lu_panel", "lu_gemm", "lu_gemm_bf16", "lu_split", "lu_graph", "lu_gemm_fp64",
+              "lu_trsm", "lu_swap", "solve", "residual", "far_field")}
     # The same trailing-update kernel on one heavy solve with nothing else on
     # the GPU: under several lanes a launch's event time includes sharing the
     # CUs with the other lanes' kernels, which the sweep's figure cannot undo.
@@ -552,13 +596,10 @@ def sweep_main(args):
     ctx.set_profiling(True)
     bh.bem(heavy_mesh, fl, k_heavy, incidence_angle=0.0, precision="mixed")
     ctx.set_profiling(False)
-    gs = ctx.kernel_stats("lu_gemm")
     ga = ctx.kernel_stats("assemble")
-    solo_rate = gs["flops"] / (gs["ms"] * 1e-3) if gs["ms"] > 0 else 0.0
-    solo = {"workload": f"one mixed solve alone after the sweep: N={heavy_mesh.n} (2N unknowns), k={k_heavy:.1f}",
-            "achieved": solo_rate / 1e12, "frac": solo_rate / MFMA_FP32_PEAK, "launches": gs["launches"],
-            "avg_launch_ms": gs["ms"] / max(1, gs["launches"]),
-            "algorithmic_bytes_per_launch": gs["bytes"] / max(1, gs["launches"])}
+    solo = gemm_roofline(ctx.kernel_stats("lu_gemm"), ctx.kernel_stats("lu_gemm_bf16"),
+                         "per launch of this solve's trailing updates")
+    solo["workload"] = f"one mixed solve alone after the sweep: N={heavy_mesh.n} (2N unknowns), k={k_heavy:.1f}"
     log(f"FP64 reference sweep at {elapsed():.0f}s ...")
     if not args.serial:
         size_lanes(bh, ctx, plan, meshes, args.lanes, "double")
@@ -573,8 +614,6 @@ def sweep_main(args):
     worst_be = max(max(d["normwise_backward_error"], d["componentwise_backward_error"]) for d in all_diags)
     t_step = float(np.mean(times))
     value = len(plan) / t_step
-    g = stats["lu_gemm"]
-    achieved = g["flops"] / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
     cpu_left = args.budget_s - elapsed() - 5.0
     cpu = None if args.no_cpu else cpu_baseline(plan, max(10.0, min(args.cpu_budget, cpu_left)))
     full = args.points == 256 and args.max_level == 5
@@ -616,16 +655,12 @@ def sweep_main(args):
         "p99_latency_ms": float(np.percentile(lats, 99)),
         "fp64_solves_per_s": len(plan) / t64,
         "speedup_vs_fp64": value / (len(plan) / t64),
-        "roofline": {"kernel": "k_gemm3m<0, FULL> (LU trailing update, 3M complex)", "bound": "mfma",
-                     "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / MFMA_FP32_PEAK, "traffic": measured_traffic(),
-                     "traffic_source": TRAFFIC_PROFILE,
-                     "traffic_basis": ("per launch of an N=20480 solve's trailing updates (the launches of "
+        "roofline": dict(gemm_roofline(stats["lu_gemm"], stats["lu_gemm_bf16"],
+                                       "per launch of an N=20480 solve's trailing updates (the launches of "
                                        "'solo'; compare with solo.algorithmic_bytes_per_launch): the counters "
-                                       "are read one solve at a time"),
-                     "algorithmic_bytes_per_launch": g["bytes"] / max(1, g["launches"]),
-                     "launches": g["launches"], "avg_launch_ms": g["ms"] / max(1, g["launches"]),
-                     "solo": solo},
+                                       "are read one solve at a time"), solo=solo),
+        "lu_tunes": tunes,
+        "tune_s": tune_s,
         "assembly_roofline": assembly_roofline(stats["assemble"], ga),
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in stats.items()},
         "cpu_baseline": cpu,

@@ -124,7 +124,11 @@ int bem_ctx_set_profiling(bem_ctx ctx, int32_t enable);
  * last reset.  Classes: "assemble", "near", "lu_panel", "lu_gemm" (FP32
  * trailing updates), "lu_gemm_fp64" (FP64 trailing updates), "lu_trsm",
  * "lu_swap", "solve", "residual", "far_field", "h_apply" (compressed operator
- * apply; bytes = blocks + vectors), "h_aca", "h_ortho" (GMRES Gram-Schmidt). */
+ * apply; bytes = blocks + vectors; its two kernels "h_proj" and "h_rows"),
+ * "h_aca", "h_ortho" (GMRES Gram-Schmidt), "lu_split" (bf16 split planes of
+ * the outer trailing updates), "lu_gemm_bf16" (those updates from the split
+ * planes; flops = the complex update's 8 M N K), "lu_graph" (factorizations
+ * replayed as captured graphs; flops = 8/3 n^3). */
 int bem_ctx_kernel_stats(bem_ctx ctx, const char *name, int64_t *launches, double *total_ms,
                          double *flops, double *bytes);
 int bem_ctx_reset_stats(bem_ctx ctx);
@@ -135,6 +139,35 @@ int bem_ctx_set_lanes(bem_ctx ctx, int32_t max_lanes);
  * updates alone fill the GPU's matrix cores) that bem_sweep runs at once
  * (1..8, default 1); the other lanes take the lighter solves meanwhile. */
 int bem_ctx_set_heavy_lanes(bem_ctx ctx, int32_t max_heavy);
+
+/* ---- LU tuning ----------------------------------------------------------
+ * No counterpart in the reference (it calls LAPACK, src/solvers/bem/full.jl:404).
+ * The shape of a factorization: outer block width (the K of the trailing
+ * updates: 256 or 512), tile rows per L2 group of the update's tile order
+ * (4, 8, 16), whether the complex-FP32 outer updates run as bf16-split 3M
+ * products (FP32-accurate piece products on the bf16 matrix cores), whether
+ * the next block's panels run ahead on a side stream, and whether a sweep
+ * replays the factorization as a captured HIP graph. */
+typedef struct {
+    int64_t unknowns;
+    int32_t precision;   /* BEM_DOUBLE or BEM_MIXED */
+    int32_t outer, group, split, ahead, graph;
+    int32_t tuned;       /* 1: chosen by the tuner; 0: the default shape */
+    int32_t candidates;  /* shapes timed */
+    double best_ms;      /* factorization time of the chosen shape */
+    double default_ms;   /* of the default shape (outer 512, group 8, FP32 updates, look-ahead) */
+} bem_lu_tune;
+/* 0: the default shape everywhere; 1: the default with bf16-split updates
+ * for complex FP32; 2 (default): per size class the tuner's choice where the
+ * class was tuned, else the default with bf16-split updates from 4096
+ * unknowns. */
+int bem_ctx_set_lu_tuning(bem_ctx ctx, int32_t mode);
+/* Times candidate shapes for an n-unknown factorization of the given
+ * precision on a synthetic matrix (coordinate descent over the five knobs)
+ * and keeps the fastest for that size class. */
+int bem_ctx_autotune(bem_ctx ctx, int64_t n, int32_t precision, bem_lu_tune *out);
+/* The shape a factorization of n unknowns would use now. */
+int bem_ctx_lu_tune(bem_ctx ctx, int64_t n, int32_t precision, bem_lu_tune *out);
 
 /* ---- mesh generators (host only, no device needed) --------------------- */
 /* Sizes of the level-`level` icosphere: 10*4^L+2 nodes, 20*4^L triangles. */
@@ -418,7 +451,9 @@ int bem_hmatrix_solve(bem_hmatrix h, int32_t n_rhs, const double *angles, bem_so
  * above) and the 1-based pivot rows in ipiv, the LAPACK zgetrf/cgetrf
  * convention the reference relies on (src/solvers/bem/full.jl:404).  When b
  * and x are non-null it also solves A x = b with those factors, or
- * A^H x = b when bit 1 of flags is set.  No counterpart in the reference: it
+ * A^H x = b when bit 1 of flags is set.  Bits 2, 3 and 4 force the
+ * factorization's shape (FP32 only for bit 2): bf16-split outer updates, FP32
+ * updates, replay through a captured graph.  No counterpart in the reference: it
  * exists so tests can check the factorization and the solves by themselves. */
 int bem_debug_lu(bem_ctx ctx, int64_t n, const double *A, int32_t flags, double *LU, int64_t *ipiv,
                  const double *b, double *x);
