@@ -50,6 +50,7 @@ EXPORTED = [
     "bem_server_stats_get", "bem_hmatrix_defaults", "bem_hmatrix_plan", "bem_hmatrix_create",
     "bem_hmatrix_destroy", "bem_hmatrix_get_info", "bem_hmatrix_matvec", "bem_hmatrix_apply", "bem_hmatrix_solve",
     "bem_debug_lu", "bem_far_field_noalloc", "bem_pressure", "bem_reduced_defaults", "bem_reduced_sweep",
+    "bem_axi_operators", "bem_axi_solve", "bem_axi_info", "bem_axi_traces", "bem_axi_far_field", "bem_axi_destroy",
 ]
 
 
@@ -180,6 +181,15 @@ def load_library(path=LIB_PATH):
         "bem_ctx_set_lanes": (ctypes.c_int, [_vp, ctypes.c_int32]),
         "bem_ctx_set_heavy_lanes": (ctypes.c_int, [_vp, ctypes.c_int32]),
         "bem_ctx_set_lu_tuning": (ctypes.c_int, [_vp, ctypes.c_int32]),
+        "bem_axi_operators": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp, ctypes.c_double, ctypes.c_int32,
+                                             ctypes.c_int32, _dp, _dp]),
+        "bem_axi_solve": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp, ctypes.c_void_p, ctypes.c_double,
+                                         ctypes.c_int32, ctypes.POINTER(_vp)]),
+        "bem_axi_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                        _dp, _dp, _dp]),
+        "bem_axi_traces": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
+        "bem_axi_far_field": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp]),
+        "bem_axi_destroy": (ctypes.c_int, [_vp]),
         "bem_ctx_autotune": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp]),
         "bem_ctx_lu_tune": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp]),
         "bem_icosphere_size": (ctypes.c_int, [ctypes.c_int32, _ip, _ip]),
@@ -715,10 +725,179 @@ class BEMSolution:
     _solution: object = None
 
 
+# ---------------------------------------------------------------------------
+# axisymmetric path (src/solvers/bem/axisymmetric.jl)
+# ---------------------------------------------------------------------------
+class MeridianMesh:
+    """MeridianMesh(rho, z) (src/geometry/meridian.jl:16-19): the generating
+    curve of a body of revolution, north pole to south pole, z along the axis
+    of symmetry (Cartesian x of the 3D path)."""
+
+    def __init__(self, rho, z):
+        self.rho = np.ascontiguousarray(np.asarray(rho, dtype=np.float64))
+        self.z = np.ascontiguousarray(np.asarray(z, dtype=np.float64))
+        if self.rho.shape != self.z.shape or self.rho.ndim != 1:
+            raise ValueError("a meridian needs matching rho/z vectors")
+
+    @property
+    def n(self):
+        return len(self.rho) - 1
+
+
+def sphere_mesh(a, n):
+    """src/geometry/meridian.jl:201-205: n panels at uniform polar angle."""
+    if n < 3:
+        raise ValueError("n must be at least 3 (one panel per segment)")
+    psi = np.linspace(0.0, math.pi, n + 1)
+    return MeridianMesh(a * np.sin(psi), a * np.cos(psi))
+
+
+def spheroid_mesh(a, b, n):
+    """src/geometry/meridian.jl:238-259: semi-axis a along the axis, b
+    equatorial, nodes at equal arc length along the meridian ellipse."""
+    if n < 3:
+        raise ValueError("n must be at least 3 (one panel per segment)")
+    psi = np.linspace(0.0, math.pi, max(2000, 40 * n))
+    speed = np.hypot(b * np.cos(psi), a * np.sin(psi))
+    arc = np.concatenate([[0.0], np.cumsum(0.5 * (speed[1:] + speed[:-1]) * np.diff(psi))])
+    nodes = np.interp(np.linspace(0.0, arc[-1], n + 1), arc, psi)
+    nodes[-1] = math.pi
+    return MeridianMesh(b * np.sin(nodes), a * np.cos(nodes))
+
+
+def _characteristic_radius(body):
+    if isinstance(body, Sphere):
+        return body.radius
+    if isinstance(body, Spheroid):
+        return max(body.a, body.b)
+    raise ValueError("the axisymmetric path takes a Sphere or a Spheroid")
+
+
+def bem_panel_count(k, characteristic_radius, elements_per_wavelength=30):
+    """src/geometry/meridian.jl:190-192."""
+    return max(20, math.ceil(elements_per_wavelength * k * characteristic_radius / 2))
+
+
+def default_mode_count(ka):
+    """src/solvers/modal/sphere.jl:1-3."""
+    return max(20, math.ceil(2.5 * ka + 4.0 * np.cbrt(ka + 1.0) + 10.0))
+
+
+def axisymmetric_mesh(body, n):
+    """_axisymmetric_mesh (src/geometry/mesh.jl:24-26)."""
+    if isinstance(body, Sphere):
+        return sphere_mesh(body.radius, n)
+    if isinstance(body, Spheroid):
+        return spheroid_mesh(body.a, body.b, n)
+    raise ValueError("the axisymmetric path takes a Sphere or a Spheroid")
+
+
+class AxisymmetricSolution:
+    """BEMSolution{_AxisymmetricSurfaceData}: per-mode scattered traces on a
+    meridian, far field by the mode sum (src/postprocessing/farfield.jl)."""
+
+    def __init__(self, ctx, h, body, boundary, k, mesh, incidence_angle, m_max):
+        self.ctx, self._h = ctx, h
+        self.body, self.boundary, self.k = body, boundary, float(k)
+        self.method = "axisymmetric"
+        self.mesh = mesh
+        self.incidence_angle = float(incidence_angle)
+        npan, nm = ctypes.c_int32(), ctypes.c_int32()
+        res, ta, ts = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        ctx.check(ctx.lib.bem_axi_info(h, ctypes.byref(npan), ctypes.byref(nm), ctypes.byref(res), ctypes.byref(ta),
+                                       ctypes.byref(ts)))
+        self.n, self.n_modes = npan.value, nm.value
+        self.diagnostics = {"method": "axisymmetric", "relative_residual": res.value, "converged": None,
+                            "iterations": None, "modes": list(range(self.n_modes)),
+                            "solver_options": {"n": self.n, "incidence_angle": self.incidence_angle,
+                                               "m_max": m_max},
+                            "assembly_ms": ta.value, "solve_ms": ts.value}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self.ctx._h:
+            self.ctx.lib.bem_axi_destroy(self._h)
+            self._h = None
+
+    def traces(self, mode=0):
+        """(p_scat, dpdn_scat) of Fourier mode `mode` on the panels."""
+        p = np.empty(self.n, dtype=np.complex128)
+        d = np.empty(self.n, dtype=np.complex128)
+        self.ctx.check(self.ctx.lib.bem_axi_traces(self._h, int(mode), p.ctypes.data_as(_dp), d.ctypes.data_as(_dp)))
+        return p, d
+
+    def far_field(self, angles):
+        a = np.ascontiguousarray(np.asarray(angles, dtype=np.float64).reshape(-1, 2))
+        out = np.empty(a.shape[0], dtype=np.complex128)
+        self.ctx.check(self.ctx.lib.bem_axi_far_field(self._h, a.shape[0], a.ctypes.data_as(_dp),
+                                                      out.ctypes.data_as(_dp)))
+        return out
+
+
+def _bem_axisymmetric(body_or_mesh, boundary, k, incidence_angle, n=None, m_max=None, ctx=None):
+    if not isinstance(boundary, (Rigid, PressureRelease, FluidFilled)):
+        raise ValueError("the axisymmetric path solves rigid, pressure-release and fluid-filled bodies")
+    if not math.isfinite(incidence_angle):
+        raise ValueError("k must be positive and finite and incidence angles finite")
+    if isinstance(body_or_mesh, MeridianMesh):
+        mesh, body = body_or_mesh, None
+        radius = float(np.max(np.hypot(mesh.rho, mesh.z - 0.5 * (mesh.z.max() + mesh.z.min()))))
+    else:
+        body = body_or_mesh
+        radius = _characteristic_radius(body)
+        mesh = axisymmetric_mesh(body, int(n) if n is not None else bem_panel_count(k, radius))
+    if m_max is None:
+        m_max = default_mode_count(k * radius)
+    if m_max < 0:
+        raise ValueError("m_max must be nonnegative")
+    ctx = ctx or default_context()
+    prob = make_problem(boundary, k)
+    h = _vp()
+    ctx.check(ctx.lib.bem_axi_solve(ctx._h, len(mesh.rho), mesh.rho.ctypes.data_as(_dp), mesh.z.ctypes.data_as(_dp),
+                                    ctypes.byref(prob), float(incidence_angle), int(m_max), ctypes.byref(h)))
+    return AxisymmetricSolution(ctx, h, body, boundary, k, mesh, incidence_angle, int(m_max))
+
+
+def axisymmetric_operators(mesh, k, modes, ctx=None):
+    """assemble_cbie_operators(mesh, k; m) for m in `modes` (a contiguous
+    range): returns (K, V), arrays of shape (len(modes), n, n)."""
+    modes = list(modes)
+    if not modes or modes != list(range(modes[0], modes[0] + len(modes))):
+        raise ValueError("modes must be a nonempty contiguous range")
+    ctx = ctx or default_context()
+    n = mesh.n
+    K = np.empty((len(modes), n, n), dtype=np.complex128, order="C")
+    V = np.empty((len(modes), n, n), dtype=np.complex128, order="C")
+    Kf = np.empty(len(modes) * n * n, dtype=np.complex128)
+    Vf = np.empty(len(modes) * n * n, dtype=np.complex128)
+    ctx.check(ctx.lib.bem_axi_operators(ctx._h, len(mesh.rho), mesh.rho.ctypes.data_as(_dp),
+                                        mesh.z.ctypes.data_as(_dp), float(k), modes[0], len(modes),
+                                        Kf.ctypes.data_as(_dp), Vf.ctypes.data_as(_dp)))
+    for c in range(len(modes)):
+        K[c] = Kf[c * n * n:(c + 1) * n * n].reshape(n, n, order="F")
+        V[c] = Vf[c * n * n:(c + 1) * n * n].reshape(n, n, order="F")
+    return K, V
+
+
 def bem(surface, boundary, k, *, incidence_angle=math.pi / 2, incidence_azimuth=0.0, **kw):
-    """Drop-in for bem(surface::Mesh, boundary, k; ...) (src/solvers/bem/surface.jl:10-22)."""
+    """Drop-in for bem(surface::Mesh, boundary, k; ...) (src/solvers/bem/surface.jl:10-22)
+    and bem(body, boundary, k; method=:axisymmetric, n, m_max) for a Sphere or
+    Spheroid (src/solvers/bem/dispatch.jl:79-104; axisymmetric by default, as
+    there) or a MeridianMesh."""
     if not (math.isfinite(k) and k > 0):
         raise ValueError("k must be finite and positive")
+    if isinstance(surface, (Sphere, Spheroid, MeridianMesh)):
+        method = kw.pop("method", "axisymmetric")
+        if method == "axisymmetric":
+            n, m_max, ctx = kw.pop("n", None), kw.pop("m_max", None), kw.pop("ctx", None)
+            kw.pop("rtol", None)
+            if kw:
+                raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
+            return _bem_axisymmetric(surface, boundary, k, incidence_angle, n=n, m_max=m_max, ctx=ctx)
+        if method != "full" or isinstance(surface, MeridianMesh):
+            raise ValueError(f"bem(...) supports method=:axisymmetric or :full, got {method}")
+        surface = mesh(surface, level=kw.pop("level", 3), ctx=kw.pop("ctx", None))
+    else:
+        kw.pop("method", None)
     op = Operator(surface, boundary, k, formulation=kw.pop("formulation", None),
                   precision=kw.pop("precision", "double"), refinement=kw.pop("refinement", None),
                   equilibrate=kw.pop("equilibrate", True),
@@ -735,8 +914,16 @@ def bem(surface, boundary, k, *, incidence_angle=math.pi / 2, incidence_azimuth=
                        incidence_azimuth, d, sol)
 
 
-def scattering_amplitude(sol, direction=None):
-    """Far-field amplitude [m] (src/solvers/bem/postprocessing.jl:1-13); default backscatter."""
+def scattering_amplitude(sol, direction=None, *, angle=None, azimuth=None):
+    """Far-field amplitude [m] (src/solvers/bem/postprocessing.jl:1-13); default backscatter.
+    Axisymmetric solutions take (angle, azimuth) from the axis and the plane
+    of incidence (src/solvers/shared/postprocessing.jl:10-14)."""
+    if isinstance(sol, AxisymmetricSolution):
+        if direction is not None:
+            raise ValueError("axisymmetric solutions take angle and azimuth")
+        th = math.pi - sol.incidence_angle if angle is None else float(angle)
+        ph = math.pi if azimuth is None else float(azimuth)
+        return complex(sol.far_field([(th, ph)])[0])
     if direction is None:
         return complex(sol._solution.far_field(0)[0])
     d = np.asarray(direction, dtype=np.float64)
@@ -747,7 +934,7 @@ def scattering_amplitude(sol, direction=None):
 
 def target_strength(x, **kw):
     """TS = 20 log10 |f| in dB re 1 m^2 (src/postprocessing/target_strength.jl:8,26-28)."""
-    if isinstance(x, BEMSolution):
+    if isinstance(x, (BEMSolution, AxisymmetricSolution)):
         x = scattering_amplitude(x, **kw)
     return 20.0 * math.log10(abs(x))
 

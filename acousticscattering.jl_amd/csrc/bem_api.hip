@@ -3213,6 +3213,129 @@ void* device_scratch(int device) {
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// Axisymmetric BEM (reference src/solvers/bem/axisymmetric.jl): meridian
+// panels, Fourier modes, one dense FP64 system per mode.
+// ---------------------------------------------------------------------------
+struct bem_axi_s {
+    bem_ctx ctx = nullptr;
+    int n = 0;                       // panels
+    bem_problem prob{};
+    double beta = 0;
+    std::vector<int> modes;          // 0 (axial) or 0 .. m_max
+    std::vector<double> geo;         // host SoA panels: r1 z1 r2 z2 rm zm L nr nz (9 n)
+    double* dgeo = nullptr;          // device copy
+    z64 *dp = nullptr, *dd = nullptr;  // device traces [mode][panel]
+    std::vector<z64> hp, hd;          // host traces
+    double max_residual = 0;
+    double assembly_ms = 0, solve_ms = 0;
+};
+
+namespace {
+
+AxiRules axi_rules() {
+    AxiRules R;
+    double x[16], w[16];
+    gauss_legendre(8, x, w);
+    for (int i = 0; i < 8; ++i) {
+        R.x8[i] = 0.5 * (x[i] + 1.0);
+        R.w8[i] = 0.5 * w[i];
+    }
+    gauss_legendre(16, x, w);
+    for (int i = 0; i < 16; ++i) {
+        R.x16[i] = 0.5 * (x[i] + 1.0);
+        R.w16[i] = 0.5 * w[i];
+    }
+    return R;
+}
+
+// Meridian checks (reference src/geometry/meridian.jl:16-80 for an open
+// meridian): finite coordinates, at least three panels, both ends on the
+// axis and every other node strictly off it, no zero-length panel, and the
+// north-to-south traversal that makes (-dz, drho) / L the outward normal
+// (the profile closed along the axis runs clockwise in the (rho, z) plane).
+void axi_validate(int64_t nn, const double* rho, const double* z) {
+    if (nn < 4) throw BemError(BEM_ERR_ARGUMENT, "a meridian needs at least three panels");
+    double scale = 0, area2 = 0;
+    for (int64_t i = 0; i < nn; ++i) {
+        if (!std::isfinite(rho[i]) || !std::isfinite(z[i]))
+            throw BemError(BEM_ERR_ARGUMENT, "meridian coordinates must be finite");
+        scale = std::max(scale, std::max(std::fabs(rho[i]), std::fabs(z[i])));
+    }
+    if (!(scale > 0)) throw BemError(BEM_ERR_ARGUMENT, "meridian has zero extent");
+    const double tol = 64 * 2.220446049250313e-16 * scale;
+    if (std::fabs(rho[0]) > tol || std::fabs(rho[nn - 1]) > tol)
+        throw BemError(BEM_ERR_ARGUMENT, "an open meridian must meet the axis only at its two endpoints");
+    for (int64_t i = 1; i + 1 < nn; ++i)
+        if (!(rho[i] > tol))
+            throw BemError(BEM_ERR_ARGUMENT, "an open meridian must meet the axis only at its two endpoints");
+    for (int64_t i = 0; i + 1 < nn; ++i) {
+        if (!(std::hypot(rho[i + 1] - rho[i], z[i + 1] - z[i]) > tol))
+            throw BemError(BEM_ERR_ARGUMENT, "meridian has a repeated or unresolved vertex");
+        area2 += rho[i] * z[i + 1] - rho[i + 1] * z[i];
+    }
+    area2 += rho[nn - 1] * z[0] - rho[0] * z[nn - 1];
+    if (!(area2 < 0))
+        throw BemError(BEM_ERR_ARGUMENT,
+                       "the meridian must run from the north pole to the south pole (outward normals)");
+}
+
+void axi_upload(bem_axi h, int64_t nn, const double* rho, const double* z) {
+    const int n = (int)(nn - 1);
+    h->n = n;
+    h->geo.assign(9 * (size_t)n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        const double r1 = rho[j], z1 = z[j], r2 = rho[j + 1], z2 = z[j + 1];
+        const double dr = r2 - r1, dz = z2 - z1, L = std::hypot(dr, dz);
+        const double v[9] = {r1, z1, r2, z2, 0.5 * (r1 + r2), 0.5 * (z1 + z2), L, -dz / L, dr / L};
+        for (int c = 0; c < 9; ++c) h->geo[(size_t)c * n + j] = v[c];
+    }
+    h->dgeo = (double*)h->ctx->alloc(sizeof(double) * h->geo.size());
+    HIPCHECK(hipMemcpy(h->dgeo, h->geo.data(), sizeof(double) * h->geo.size(), hipMemcpyHostToDevice));
+}
+
+AxiPanels axi_panels(bem_axi h) {
+    const int n = h->n;
+    const double* g = h->dgeo;
+    return AxiPanels{g, g + n, g + 2 * n, g + 3 * n, g + 4 * n, g + 5 * n, g + 6 * n, g + 7 * n, g + 8 * n, n};
+}
+
+void axi_free(bem_axi h) {
+    if (!h) return;
+    bem_ctx c = h->ctx;
+    hipStreamSynchronize(c->lane0()->s);
+    c->release(h->dgeo);
+    c->release(h->dp);
+    c->release(h->dd);
+    delete h;
+}
+
+// Incident plane wave e^{ik(z cos b + rho sin b cos phi)}, mode m, at the
+// panel midpoints (axisymmetric.jl:967-979,1021-1025).
+void axi_incident(const bem_axi h, double k, int m, std::vector<std::complex<double>>& p,
+                  std::vector<std::complex<double>>& dp) {
+    const int n = h->n;
+    const double* g = h->geo.data();
+    const double sb = std::sin(h->beta), cb = std::cos(h->beta);
+    const double eps = m == 0 ? 1.0 : 2.0;
+    std::complex<double> im_m(1.0, 0.0);
+    for (int q = 0; q < (m & 3); ++q) im_m *= std::complex<double>(0.0, 1.0);
+    p.resize(n);
+    dp.resize(n);
+    for (int j = 0; j < n; ++j) {
+        const double rm = g[4 * n + j], zm = g[5 * n + j], nr = g[7 * n + j], nz = g[8 * n + j];
+        const double u = k * rm * sb;
+        const double jm = std::cyl_bessel_j((double)m, u);
+        const double jd = m == 0 ? -std::cyl_bessel_j(1.0, u)
+                                 : 0.5 * (std::cyl_bessel_j((double)(m - 1), u) - std::cyl_bessel_j((double)(m + 1), u));
+        const std::complex<double> ph = eps * im_m * std::exp(std::complex<double>(0.0, k * zm * cb));
+        p[j] = ph * jm;
+        dp[j] = ph * (std::complex<double>(0.0, k * cb * nz) * jm + k * sb * nr * jd);
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* bem_version(void) { return "bemhip 0.1.0 (gfx950)"; }
@@ -4530,6 +4653,309 @@ int bem_debug_lu(bem_ctx ctx, int64_t n64, const double* A, int32_t flags, doubl
         for (void* p : op.heap) ctx->release(p);
         op.heap.clear();
     });
+}
+
+
+// ---------------------------------------------------------------------------
+// axisymmetric BEM
+// ---------------------------------------------------------------------------
+int bem_axi_operators(bem_ctx ctx, int64_t n_nodes, const double* rho, const double* z, double k, int32_t m0,
+                      int32_t n_modes, double* K, double* V) {
+    if (!ctx || !rho || !z || !K || !V || m0 < 0 || n_modes < 1 || n_modes > 256 || !(std::isfinite(k) && k > 0))
+        return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        axi_validate(n_nodes, rho, z);
+        auto* h = new bem_axi_s();
+        h->ctx = ctx;
+        std::vector<void*> tmp;
+        try {
+            axi_upload(h, n_nodes, rho, z);
+            const int n = h->n;
+            hipStream_t s = ctx->lane0()->s;
+            const size_t nn = (size_t)n * n;
+            z64* dK = (z64*)ctx->alloc(sizeof(z64) * nn * kAxiModes);
+            tmp.push_back(dK);
+            z64* dV = (z64*)ctx->alloc(sizeof(z64) * nn * kAxiModes);
+            tmp.push_back(dV);
+            const AxiRules R = axi_rules();
+            std::vector<z64> hk(nn * kAxiModes), hv(nn * kAxiModes);
+            for (int c0 = 0; c0 < n_modes; c0 += kAxiModes) {
+                const int nm = std::min(kAxiModes, n_modes - c0);
+                launch_axi_ops(s, axi_panels(h), R, k, m0 + c0, nm, 0, dK, dV, nullptr, nullptr);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipMemcpyAsync(hk.data(), dK, sizeof(z64) * nn * nm, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipMemcpyAsync(hv.data(), dV, sizeof(z64) * nn * nm, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
+                // mode c0 + c, column-major n x n, interleaved complex
+                for (int c = 0; c < nm; ++c)
+                    for (int i = 0; i < n; ++i)
+                        for (int j = 0; j < n; ++j) {
+                            const size_t src = ((size_t)c * n + i) * n + j;
+                            const size_t dst = 2 * (((size_t)(c0 + c) * n + j) * n + i);
+                            K[dst] = hk[src].x;
+                            K[dst + 1] = hk[src].y;
+                            V[dst] = hv[src].x;
+                            V[dst + 1] = hv[src].y;
+                        }
+            }
+        } catch (...) {
+            for (void* q : tmp) ctx->release(q);
+            axi_free(h);
+            throw;
+        }
+        for (void* q : tmp) ctx->release(q);
+        axi_free(h);
+    });
+}
+
+int bem_axi_solve(bem_ctx ctx, int64_t n_nodes, const double* rho, const double* z, const bem_problem* problem,
+                  double incidence_angle, int32_t m_max, bem_axi* out) {
+    if (!ctx || !rho || !z || !problem || !out) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        const bem_problem& pr = *problem;
+        if (!(std::isfinite(pr.k) && pr.k > 0)) throw BemError(BEM_ERR_ARGUMENT, "k must be finite and positive");
+        if (!std::isfinite(incidence_angle)) throw BemError(BEM_ERR_ARGUMENT, "incidence angles must be finite");
+        if (m_max < 0 || m_max > 255) throw BemError(BEM_ERR_ARGUMENT, "m_max must be in 0..255");
+        const bool fluid = pr.boundary == BEM_FLUID;
+        if (pr.boundary != BEM_RIGID && pr.boundary != BEM_PRESSURE_RELEASE && !fluid)
+            throw BemError(BEM_ERR_ARGUMENT, "unknown boundary condition");
+        if (fluid && !(std::isfinite(pr.density_contrast) && pr.density_contrast > 0 &&
+                       std::isfinite(pr.soundspeed_contrast) && pr.soundspeed_contrast > 0))
+            throw BemError(BEM_ERR_ARGUMENT, "fluid density and soundspeed contrasts must be finite and positive");
+        axi_validate(n_nodes, rho, z);
+        const double t0 = now_ms();
+        auto* h = new bem_axi_s();
+        h->ctx = ctx;
+        h->prob = pr;
+        h->beta = incidence_angle;
+        const bool axial = incidence_angle == 0.0;
+        if (axial) h->modes = {0};
+        else
+            for (int m = 0; m <= m_max; ++m) h->modes.push_back(m);
+        std::vector<void*> tmp;
+        auto al = [&](size_t b) { void* q = ctx->alloc(b); tmp.push_back(q); return q; };
+        Lane* L = ctx->lane0();
+        hipStream_t s = L->s;
+        try {
+            axi_upload(h, n_nodes, rho, z);
+            const int n = h->n, nm_all = (int)h->modes.size();
+            const int ns = fluid ? 2 * n : n;
+            const size_t nn = (size_t)n * n;
+            const double k = pr.k, g = fluid ? pr.density_contrast : 1.0;
+            const double ki = fluid ? k / pr.soundspeed_contrast : k;
+            const AxiPanels P = axi_panels(h);
+            const AxiRules R = axi_rules();
+            const int chunk = std::min(kAxiModes, nm_all);
+            z64* Ke = (z64*)al(sizeof(z64) * nn * chunk);
+            z64* Ve = (z64*)al(sizeof(z64) * nn * chunk);
+            z64 *Ki = nullptr, *Vi = nullptr, *RK = nullptr, *RV = nullptr;
+            if (fluid) {
+                Ki = (z64*)al(sizeof(z64) * nn * chunk);
+                Vi = (z64*)al(sizeof(z64) * nn * chunk);
+            }
+            if (axial && !fluid) {
+                RK = (z64*)al(sizeof(z64) * nn);
+                RV = (z64*)al(sizeof(z64) * nn);
+            }
+            z64* A = (z64*)al(sizeof(z64) * (size_t)ns * ns);
+            z64* b = (z64*)al(sizeof(z64) * ns);
+            z64* x = (z64*)al(sizeof(z64) * ns);
+            z64* r = (z64*)al(sizeof(z64) * ns);
+            z64* w1 = (z64*)al(sizeof(z64) * ns);
+            z64* w2 = (z64*)al(sizeof(z64) * n);
+            z64* vin = (z64*)al(sizeof(z64) * 2 * n);
+            bem_operator_s op;
+            op.ctx = ctx;
+            op.lane = L;
+            op.n = ns;
+            op.single = false;
+            op.LU = al(sizeof(z64) * (size_t)ns * ns);
+            op.ipiv = (int*)al(sizeof(int) * ns);
+            op.perm = (int*)al(sizeof(int) * ns);
+            op.inv = al(sizeof(z64) * 2 * 4096 * (size_t)((ns + 63) / 64));
+            h->dp = (z64*)ctx->alloc(sizeof(z64) * (size_t)n * nm_all);
+            h->dd = (z64*)ctx->alloc(sizeof(z64) * (size_t)n * nm_all);
+            h->hp.resize((size_t)n * nm_all);
+            h->hd.resize((size_t)n * nm_all);
+            const z64 one{1.0, 0.0}, zero{0.0, 0.0}, half{0.5, 0.0}, mone{-1.0, 0.0};
+            std::vector<std::complex<double>> pinc, dinc;
+            std::vector<z64> hb(ns), hv2(2 * n), hx(ns), hr(ns), hw(n);
+            double t_asm = 0;
+            for (int c0 = 0; c0 < nm_all; c0 += chunk) {
+                const int nm = std::min(chunk, nm_all - c0);
+                const double ta = now_ms();
+                launch_axi_ops(s, P, R, k, h->modes[c0], nm, axial && !fluid, Ke, Ve, RK, RV);
+                if (fluid) launch_axi_ops(s, P, R, ki, h->modes[c0], nm, 0, Ki, Vi, nullptr, nullptr);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipStreamSynchronize(s));
+                t_asm += now_ms() - ta;
+                for (int c = 0; c < nm; ++c) {
+                    const int mi = c0 + c, m = h->modes[mi];
+                    const z64 *Kc = Ke + nn * c, *Vc = Ve + nn * c;
+                    axi_incident(h, k, m, pinc, dinc);
+                    for (int j = 0; j < n; ++j) {
+                        hv2[j] = z64{pinc[j].real(), pinc[j].imag()};
+                        hv2[n + j] = z64{dinc[j].real(), dinc[j].imag()};
+                    }
+                    HIPCHECK(hipMemcpyAsync(vin, hv2.data(), sizeof(z64) * 2 * n, hipMemcpyHostToDevice, s));
+                    // the system and its right-hand side (axisymmetric.jl:683-729,995-1057,1185-1300)
+                    if (pr.boundary == BEM_RIGID) {
+                        launch_axi_block(s, n, half, mone, Kc, zero, nullptr, A, n);
+                        if (axial) launch_axi_rowsum(s, n, RV, b);
+                        else launch_matvec<double>(s, Vc, n, n, n, vin + n, b, nullptr, nullptr);
+                    } else if (pr.boundary == BEM_PRESSURE_RELEASE) {
+                        launch_axi_block(s, n, zero, one, Vc, zero, nullptr, A, n);
+                        // K p - p / 2 with p = -p_inc; axial: -rowsum(RK)
+                        if (axial) launch_axi_rowsum(s, n, RK, w2);
+                        else launch_matvec<double>(s, Kc, n, n, n, vin, w2, nullptr, nullptr);
+                        HIPCHECK(hipMemcpyAsync(hw.data(), w2, sizeof(z64) * n, hipMemcpyDeviceToHost, s));
+                        HIPCHECK(hipStreamSynchronize(s));
+                        for (int j = 0; j < n; ++j) {
+                            const std::complex<double> v = -std::complex<double>(hw[j].x, hw[j].y) + 0.5 * pinc[j];
+                            hb[j] = z64{v.real(), v.imag()};
+                        }
+                        HIPCHECK(hipMemcpyAsync(b, hb.data(), sizeof(z64) * n, hipMemcpyHostToDevice, s));
+                    } else {
+                        const z64 *Kic = Ki + nn * c, *Vic = Vi + nn * c;
+                        launch_axi_block(s, n, half, mone, Kc, zero, nullptr, A, ns);
+                        launch_axi_block(s, n, zero, one, Vc, zero, nullptr, A + n, ns);
+                        launch_axi_block(s, n, half, one, Kic, zero, nullptr, A + (size_t)n * ns, ns);
+                        launch_axi_block(s, n, zero, z64{-g, 0.0}, Vic, zero, nullptr, A + (size_t)n * ns + n, ns);
+                        // -[(1/2 I + Ki) p_inc - g Vi dpdn_inc]
+                        launch_matvec<double>(s, Kic, n, n, n, vin, w2, nullptr, nullptr);
+                        launch_matvec<double>(s, Vic, n, n, n, vin + n, w1, nullptr, nullptr);
+                        HIPCHECK(hipMemcpyAsync(hw.data(), w2, sizeof(z64) * n, hipMemcpyDeviceToHost, s));
+                        HIPCHECK(hipMemcpyAsync(hx.data(), w1, sizeof(z64) * n, hipMemcpyDeviceToHost, s));
+                        HIPCHECK(hipStreamSynchronize(s));
+                        for (int j = 0; j < n; ++j) {
+                            hb[j] = zero;
+                            const std::complex<double> v = -(0.5 * pinc[j] + std::complex<double>(hw[j].x, hw[j].y) -
+                                                              g * std::complex<double>(hx[j].x, hx[j].y));
+                            hb[n + j] = z64{v.real(), v.imag()};
+                        }
+                        HIPCHECK(hipMemcpyAsync(b, hb.data(), sizeof(z64) * ns, hipMemcpyHostToDevice, s));
+                    }
+                    HIPCHECK(hipGetLastError());
+                    // FP64 LU of a copy (A stays for the residual), solve, residual
+                    HIPCHECK(hipMemcpyAsync(op.LU, A, sizeof(z64) * (size_t)ns * ns, hipMemcpyDeviceToDevice, s));
+                    factor_into<double>(&op, op.LU, op.ipiv, op.perm, op.inv);
+                    lu_solve<double>(s, (const z64*)op.LU, ns, ns, op.perm, (const z64*)op.inv, b, w1, x, L->ws,
+                                     &L->pl);
+                    launch_matvec<double>(s, A, ns, ns, ns, x, r, b, nullptr);
+                    int expired = 0;
+                    HIPCHECK(hipMemcpyAsync(hx.data(), x, sizeof(z64) * ns, hipMemcpyDeviceToHost, s));
+                    HIPCHECK(hipMemcpyAsync(hr.data(), r, sizeof(z64) * ns, hipMemcpyDeviceToHost, s));
+                    HIPCHECK(hipMemcpyAsync(hb.data(), b, sizeof(z64) * ns, hipMemcpyDeviceToHost, s));
+                    HIPCHECK(hipMemcpyAsync(&expired, L->ws.status + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+                    HIPCHECK(hipStreamSynchronize(s));
+                    tri_check(L, expired);
+                    double rn = 0, bn = 0;
+                    for (int q = 0; q < ns; ++q) {
+                        rn += hr[q].x * hr[q].x + hr[q].y * hr[q].y;
+                        bn += hb[q].x * hb[q].x + hb[q].y * hb[q].y;
+                    }
+                    const double rel = bn > 0 ? std::sqrt(rn / bn) : std::sqrt(rn);
+                    if (!std::isfinite(rel)) throw BemError(BEM_ERR_NUMERIC, "non-finite solution");
+                    h->max_residual = std::max(h->max_residual, rel);
+                    // scattered traces of this mode
+                    for (int j = 0; j < n; ++j) {
+                        z64 pj, dj;
+                        if (pr.boundary == BEM_RIGID) {
+                            pj = hx[j];
+                            dj = z64{-dinc[j].real(), -dinc[j].imag()};
+                        } else if (pr.boundary == BEM_PRESSURE_RELEASE) {
+                            pj = z64{-pinc[j].real(), -pinc[j].imag()};
+                            dj = hx[j];
+                        } else {
+                            pj = hx[j];
+                            dj = hx[n + j];
+                        }
+                        h->hp[(size_t)mi * n + j] = pj;
+                        h->hd[(size_t)mi * n + j] = dj;
+                    }
+                }
+            }
+            HIPCHECK(hipMemcpyAsync(h->dp, h->hp.data(), sizeof(z64) * h->hp.size(), hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(h->dd, h->hd.data(), sizeof(z64) * h->hd.size(), hipMemcpyHostToDevice, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            h->assembly_ms = t_asm;
+            h->solve_ms = now_ms() - t0 - t_asm;
+        } catch (...) {
+            hipStreamSynchronize(s);
+            for (void* q : tmp) ctx->release(q);
+            axi_free(h);
+            throw;
+        }
+        for (void* q : tmp) ctx->release(q);
+        *out = h;
+    });
+}
+
+int bem_axi_info(bem_axi sol, int32_t* n_panels, int32_t* n_modes, double* max_relative_residual,
+                 double* assembly_ms, double* solve_ms) {
+    if (!sol) return BEM_ERR_ARGUMENT;
+    if (n_panels) *n_panels = sol->n;
+    if (n_modes) *n_modes = (int32_t)sol->modes.size();
+    if (max_relative_residual) *max_relative_residual = sol->max_residual;
+    if (assembly_ms) *assembly_ms = sol->assembly_ms;
+    if (solve_ms) *solve_ms = sol->solve_ms;
+    return BEM_OK;
+}
+
+int bem_axi_traces(bem_axi sol, int32_t mode, double* p_scat, double* dpdn_scat) {
+    if (!sol || mode < 0 || mode >= (int32_t)sol->modes.size()) return BEM_ERR_ARGUMENT;
+    const int n = sol->n;
+    for (int j = 0; j < n; ++j) {
+        const z64 p = sol->hp[(size_t)mode * n + j], d = sol->hd[(size_t)mode * n + j];
+        if (p_scat) {
+            p_scat[2 * j] = p.x;
+            p_scat[2 * j + 1] = p.y;
+        }
+        if (dpdn_scat) {
+            dpdn_scat[2 * j] = d.x;
+            dpdn_scat[2 * j + 1] = d.y;
+        }
+    }
+    return BEM_OK;
+}
+
+int bem_axi_far_field(bem_axi sol, int64_t n_dir, const double* angles, double* out) {
+    if (!sol || n_dir <= 0 || n_dir > 65535 || !angles || !out) return BEM_ERR_ARGUMENT;
+    bem_ctx ctx = sol->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        for (int64_t q = 0; q < 2 * n_dir; ++q)
+            if (!std::isfinite(angles[q])) throw BemError(BEM_ERR_ARGUMENT, "directions must be finite");
+        hipStream_t s = ctx->lane0()->s;
+        double* da = (double*)ctx->alloc(sizeof(double) * 2 * n_dir);
+        z64* dout = nullptr;
+        try {
+            dout = (z64*)ctx->alloc(sizeof(z64) * n_dir);
+            HIPCHECK(hipMemcpyAsync(da, angles, sizeof(double) * 2 * n_dir, hipMemcpyHostToDevice, s));
+            launch_axi_far(s, axi_panels(sol), axi_rules(), sol->prob.k, (int)sol->modes.size(), sol->dp, sol->dd,
+                           (int)n_dir, da, dout);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(out, dout, sizeof(z64) * n_dir, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            hipStreamSynchronize(s);
+            ctx->release(da);
+            ctx->release(dout);
+            throw;
+        }
+        ctx->release(da);
+        ctx->release(dout);
+    });
+}
+
+int bem_axi_destroy(bem_axi sol) {
+    if (!sol) return BEM_ERR_ARGUMENT;
+    bem_ctx ctx = sol->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    axi_free(sol);
+    return BEM_OK;
 }
 
 }  // extern "C"

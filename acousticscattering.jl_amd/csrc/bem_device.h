@@ -183,6 +183,31 @@ struct LuStreams {
     hipStream_t main = nullptr, side = nullptr;
     hipEvent_t ready = nullptr, panel = nullptr;
 };
+// ---- axisymmetric path ----------------------------------------------------
+// Meridian panels on the device (SoA, n panels): endpoints, midpoints,
+// lengths and outward normals (-dz, drho) / L.
+struct AxiPanels {
+    const double *r1, *z1, *r2, *z2, *rm, *zm, *L, *nr, *nz;
+    int n;
+};
+// Gauss-Legendre rules on [0, 1].
+struct AxiRules {
+    double x8[8], w8[8], x16[16], w16[16];
+};
+constexpr int kAxiModes = 8;  // Fourier modes per operator launch
+// K[c], V[c] (row-major n x n, c = 0 .. nm - 1) of modes m0 + c; with axial,
+// also RK[i][j], RV[i][j]: panel j's mode-0 kernels against e^{ikz} and
+// ik n_z e^{ikz} along the panel.
+void launch_axi_ops(hipStream_t s, const AxiPanels& P, const AxiRules& R, double k, int m0, int nm, int axial, z64* K,
+                    z64* V, z64* RK, z64* RV);
+// out (leading dimension ldo) = a I + b X + c Y over an n x n block.
+void launch_axi_block(hipStream_t s, int n, z64 a, z64 b, const z64* X, z64 c, const z64* Y, z64* out, int64_t ldo);
+void launch_axi_rowsum(hipStream_t s, int n, const z64* M, z64* y);
+// Far field of the mode traces (modes 0 .. nm - 1, nm <= 256) at ndir
+// directions (theta, phi pairs).
+void launch_axi_far(hipStream_t s, const AxiPanels& P, const AxiRules& R, double k, int nm, const z64* Pm,
+                    const z64* Dm, int ndir, const double* dirs, z64* out);
+
 // A synthetic n x n matrix for timing factorizations (diagonally weighted).
 template <typename R>
 void launch_fill_test(hipStream_t s, C2<R>* A, int n);

@@ -444,6 +444,44 @@ int bem_hmatrix_apply(bem_hmatrix h, int32_t which, int32_t n_rhs, const double 
  * (method 2, iterations, relative_residual = ||b - A x|| / ||b||). */
 int bem_hmatrix_solve(bem_hmatrix h, int32_t n_rhs, const double *angles, bem_solution *out);
 
+/* ---- axisymmetric BEM (src/solvers/bem/axisymmetric.jl) -------------------
+ * A body of revolution given by its meridian: n_nodes points (rho_i, z_i),
+ * z along the axis of symmetry (Cartesian x of the 3D path), from the north
+ * pole (rho = 0, z max) to the south pole (rho = 0), so that (-dz, drho)/L is
+ * the outward normal of each straight panel (MeridianMesh,
+ * src/geometry/meridian.jl:16-162).  Piecewise-constant traces collocated at
+ * the panel midpoints, azimuthal Fourier modes m = 0 .. m_max (only m = 0
+ * for axial incidence), FP64 throughout. */
+typedef struct bem_axi_s *bem_axi;
+/* Mode operators K_m, V_m of assemble_cbie_operators_modes
+ * (axisymmetric.jl:389-465,574-607) for m = m0 .. m0 + n_modes - 1 at
+ * wavenumber k: K and V receive n_modes column-major n x n complex matrices
+ * (n = n_nodes - 1 panels). */
+int bem_axi_operators(bem_ctx ctx, int64_t n_nodes, const double *rho, const double *z, double k, int32_t m0,
+                      int32_t n_modes, double *K, double *V);
+/* bem(body, boundary, k; method=:axisymmetric, incidence_angle, m_max) for a
+ * rigid, pressure-release or fluid-filled body (problem->boundary, k and the
+ * contrasts; the other fields are ignored): solve_axial at incidence_angle
+ * 0 (axisymmetric.jl:683-729,806-857), solve_oblique otherwise
+ * (axisymmetric.jl:995-1057,1185-1300; fluid bodies as the reduced 2n
+ * system in the exterior traces).  The incidence angle is measured from the
+ * axis of symmetry, the azimuth of incidence is 0. */
+int bem_axi_solve(bem_ctx ctx, int64_t n_nodes, const double *rho, const double *z, const bem_problem *problem,
+                  double incidence_angle, int32_t m_max, bem_axi *out);
+/* Panels, modes, largest relative residual ||A x - b|| / ||b|| over the
+ * modes' systems (the reference's solve reports), and timings. */
+int bem_axi_info(bem_axi sol, int32_t *n_panels, int32_t *n_modes, double *max_relative_residual,
+                 double *assembly_ms, double *solve_ms);
+/* Scattered traces of mode index `mode` (p_scat_modes[mode + 1],
+ * dpdn_scat_modes[mode + 1] of the reference), n complex values each. */
+int bem_axi_traces(bem_axi sol, int32_t mode, double *p_scat, double *dpdn_scat);
+/* far_field(ps, p_scat_modes, dpdn_scat_modes, k, theta, phi)
+ * (src/postprocessing/farfield.jl:14-100) at n_dir directions given as
+ * (theta, phi) pairs: theta from the axis of symmetry, phi from the plane of
+ * incidence; backscatter is (pi - incidence_angle, pi). */
+int bem_axi_far_field(bem_axi sol, int64_t n_dir, const double *angles, double *out);
+int bem_axi_destroy(bem_axi sol);
+
 /* ---- test hook: dense LU of a caller matrix ------------------------------
  * Factors the column-major n x n complex matrix A on the device with the
  * library's LU (no equilibration; FP32 when bit 0 of flags is set) and returns
