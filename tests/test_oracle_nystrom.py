@@ -1,0 +1,273 @@
+"""Pins of the curved-element Nystrom oracle (oracle/nystrom.py) by what the
+mathematics and the reference's own goldens fix, independently of the
+oracle's formulas:
+
+* quadrature rules integrate every monomial of their degree exactly;
+* shape functions interpolate (Kronecker property, partition of unity) and
+  their derivatives match finite differences;
+* curved meshes of the sphere converge in area and volume at the
+  interpolation order, with normals along the radius;
+* the regular basis equals j_l(kr) Y_l^m (scipy) up to its stated
+  normalisation, and every flux is the normal derivative of its value;
+* corrected operators satisfy the Green identities for a field outside the
+  interpolation basis, at a rate the uncorrected quadrature does not reach;
+* solutions meet the modal series and the reference's BEM goldens
+  (test/extended/bem/sphere/full3d.jl:209-233,
+  test/extended/bem/spheroid/full3d.jl:10-24) far inside their 1 % bars.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sp
+
+from oracle import meshes, modal
+from oracle import nystrom as NY
+from oracle import oracle as O
+
+
+@pytest.mark.parametrize("qorder", range(1, 9))
+def test_triangle_rules_are_exact_to_their_degree(qorder):
+    bary, w = NY.triangle_rule(qorder)
+    deg = NY._DEGREE[qorder]
+    assert deg >= qorder
+    assert np.all(w > 0) and np.all(bary > 0)
+    assert np.allclose(bary.sum(axis=1), 1.0, atol=1e-15)
+    u, v = bary[:, 1], bary[:, 2]
+    for a in range(deg + 1):
+        for b in range(deg + 1 - a):
+            exact = math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+            assert abs(np.sum(w * u ** a * v ** b) - exact) < 2e-15
+    # one degree higher is not integrated exactly by every rule: the check bites
+    if qorder == 4:
+        bad = max(abs(np.sum(w * u ** a * v ** (5 - a)) - math.factorial(a) * math.factorial(5 - a) / math.factorial(7))
+                  for a in range(6))
+        assert bad > 1e-6
+
+
+@pytest.mark.parametrize("order,npt", [(1, 3), (2, 6), (3, 10)])
+def test_shape_functions_interpolate(order, npt):
+    # Gmsh node positions in (u, v)
+    ref = {1: [(0, 0), (1, 0), (0, 1)],
+           2: [(0, 0), (1, 0), (0, 1), (0.5, 0), (0.5, 0.5), (0, 0.5)],
+           3: [(0, 0), (1, 0), (0, 1), (1 / 3, 0), (2 / 3, 0), (2 / 3, 1 / 3), (1 / 3, 2 / 3),
+               (0, 2 / 3), (0, 1 / 3), (1 / 3, 1 / 3)]}[order]
+    uv = np.array(ref, dtype=float)
+    bary = np.column_stack([1 - uv.sum(axis=1), uv[:, 0], uv[:, 1]])
+    N, Nu, Nv = NY.shape(order, bary)
+    assert N.shape == (npt, npt)
+    assert np.allclose(N, np.eye(npt), atol=1e-14)
+    rng = np.random.default_rng(3)
+    pts = rng.random((20, 2)) * 0.5
+    bary = np.column_stack([1 - pts.sum(axis=1), pts[:, 0], pts[:, 1]])
+    N, Nu, Nv = NY.shape(order, bary)
+    assert np.allclose(N.sum(axis=1), 1.0, atol=1e-14)
+    h = 1e-6
+    for du, dv, D in ((h, 0, Nu), (0, h, Nv)):
+        p = pts + [du, dv]
+        m = pts - [du, dv]
+        Np, _, _ = NY.shape(order, np.column_stack([1 - p.sum(axis=1), p[:, 0], p[:, 1]]))
+        Nm, _, _ = NY.shape(order, np.column_stack([1 - m.sum(axis=1), m[:, 0], m[:, 1]]))
+        assert np.allclose((Np - Nm) / (2 * h), D, atol=1e-8)
+
+
+def test_curved_sphere_geometry_converges():
+    errs = {}
+    for order in (2, 3):
+        for lv in (1, 2):
+            nodes, tri = NY.curved_icosphere(lv, (1.0, 1.0, 1.0), order)
+            assert tri.shape[0] == {2: 6, 3: 10}[order]
+            # every node lies on the unit sphere
+            assert np.allclose(np.linalg.norm(nodes, axis=0), 1.0, atol=1e-15)
+            Q = NY.Quadrature(nodes, tri, 8)
+            area = Q.weights.sum()
+            vol = np.sum(Q.weights * np.sum(Q.coords * Q.normals, axis=1)) / 3
+            errs[(order, lv)] = abs(area - 4 * math.pi)
+            assert abs(vol - 4 * math.pi / 3) < 2 * abs(area - 4 * math.pi)
+            # closed surface: sum w n = 0; normals outward, along the radius
+            assert np.linalg.norm((Q.weights[:, None] * Q.normals).sum(axis=0)) < 1e-12
+            radial = Q.coords / np.linalg.norm(Q.coords, axis=1)[:, None]
+            assert np.min(np.sum(radial * Q.normals, axis=1)) > 0.99
+    flat_nodes, flat_tri = meshes.icosphere(2)
+    flat = NY.Quadrature(flat_nodes, flat_tri, 4)
+    assert abs(flat.weights.sum() - 4 * math.pi) > 100 * errs[(2, 2)]
+    for order in (2, 3):
+        assert errs[(order, 2)] < errs[(order, 1)] / 12   # O(h^4)
+    assert errs[(2, 2)] < 2e-3 and errs[(3, 2)] < errs[(2, 2)] / 2
+
+
+def test_curved_mesh_passes_the_topology_checks():
+    nodes, tri = NY.curved_icosphere(1, (0.02, 0.01, 0.01), 2)
+    code, msg, _ = O.validate(nodes, tri[:3])
+    assert code == 0, msg
+    # every edge node is shared by the two triangles of its edge, in the
+    # reference's conformity sense (src/geometry/surface.jl:406-419)
+    seen = {}
+    for e in range(tri.shape[1]):
+        t = tri[:, e]
+        for j, (u, v) in enumerate(((0, 1), (1, 2), (2, 0))):
+            key = (min(t[u], t[v]), max(t[u], t[v]))
+            mid = t[3 + j]
+            assert seen.setdefault(key, mid) == mid
+
+
+def test_regular_traces_are_bessel_times_harmonics():
+    rng = np.random.default_rng(7)
+    X = rng.normal(size=(12, 3)) * 0.3
+    N = rng.normal(size=(12, 3))
+    N /= np.linalg.norm(N, axis=1)[:, None]
+    center, radius, k, degree = np.array([0.05, -0.02, 0.01]), 0.7, 1.3, 3
+    vals, flux = NY.regular_traces(X, N, center, radius, k, degree)
+    assert vals.shape == (12, (degree + 1) ** 2)
+    d = X - center
+    r = np.linalg.norm(d, axis=1)
+    theta = np.arccos(d[:, 2] / r)
+    phi = np.arctan2(d[:, 1], d[:, 0])
+    col = 0
+    for m in range(degree + 1):
+        for l in range(m, degree + 1):
+            dfact = math.prod(range(1, 2 * l + 2, 2))
+            expected = sp.sph_harm_y(l, m, theta, phi) * sp.spherical_jn(l, k * r) * dfact / (k * radius) ** l
+            assert np.allclose(vals[:, col], expected, rtol=1e-12, atol=1e-14)
+            col += 1
+            if m > 0:
+                assert np.allclose(vals[:, col], np.conj(expected), rtol=1e-12, atol=1e-14)
+                col += 1
+    # flux = n . grad(value), by central differences
+    h = 1e-6
+    vp, _ = NY.regular_traces(X + h * N, N, center, radius, k, degree)
+    vm, _ = NY.regular_traces(X - h * N, N, center, radius, k, degree)
+    assert np.allclose((vp - vm) / (2 * h), flux, rtol=1e-7, atol=1e-8)
+
+
+def test_point_traces_flux_is_the_normal_derivative():
+    rng = np.random.default_rng(11)
+    X = rng.normal(size=(8, 3))
+    N = rng.normal(size=(8, 3))
+    N /= np.linalg.norm(N, axis=1)[:, None]
+    zs = NY.fibonacci_sphere(18, 5.0, np.array([0.1, 0.2, 0.3]))
+    assert np.allclose(np.linalg.norm(zs - [0.1, 0.2, 0.3], axis=1), 5.0)
+    g0, g1 = NY.point_traces(X, N, zs, 2.5)
+    r = np.linalg.norm(X[:, None, :] - zs[None], axis=2)
+    assert np.allclose(g0, np.exp(2.5j * r) / (4 * math.pi * r))
+    h = 1e-6
+    gp, _ = NY.point_traces(X + h * N, N, zs, 2.5)
+    gm, _ = NY.point_traces(X - h * N, N, zs, 2.5)
+    assert np.allclose((gp - gm) / (2 * h), g1, rtol=1e-7, atol=1e-10)
+
+
+def _green_defects(lv, k, regular):
+    """Green-identity residuals for the plane wave e^{ik d.x} (not in either
+    interpolation basis) on the curved unit sphere: corrected and naive."""
+    nodes, tri = NY.curved_icosphere(lv, (1.0, 1.0, 1.0), 2)
+    Q = NY.Quadrature(nodes, tri, 4)
+    d = np.array([0.36, 0.48, 0.8])
+    u, du = Q.incident(k, d)
+    out = {}
+    for deriv in (False, True):
+        A0, B0 = Q.layers(k, deriv)
+        A, B = NY.corrected_layers(Q, k, deriv, regular)
+        trace = du if deriv else u
+        out[deriv] = (np.max(np.abs(A @ du - B @ u - 0.5 * trace)) / np.max(np.abs(trace)),
+                      np.max(np.abs(A0 @ du - B0 @ u - 0.5 * trace)) / np.max(np.abs(trace)))
+    return out
+
+
+@pytest.mark.parametrize("k,regular", [(2.0, False), (0.4, True)])
+def test_corrected_operators_satisfy_green_identities(k, regular):
+    coarse = _green_defects(1, k, regular)
+    fine = _green_defects(2, k, regular)
+    # pressure identity: high order (x10-15 per halving of h); the
+    # hypersingular one converges slowly (own-element interpolation) while the
+    # naive rule diverges
+    c, c0 = coarse[False]
+    f, f0 = fine[False]
+    assert f < c / 8 and f < f0 / 100, (c, f, f0)
+    c, c0 = coarse[True]
+    f, f0 = fine[True]
+    assert f < c / 1.5 and f < f0 / 1000, (c, f, f0)
+    assert fine[False][0] < 5e-5 and fine[True][0] < 2e-3
+
+
+def _sphere_quad(radius, level, order=2):
+    nodes, tri = NY.curved_icosphere(level, (radius, radius, radius), order)
+    return NY.Quadrature(nodes, tri, 4)
+
+
+K38 = 2 * math.pi * 38000.0 / 1477.4
+
+
+@pytest.mark.parametrize("bc", ["rigid", "soft"])
+def test_nystrom_sphere_against_modal(bc):
+    """Config-1 sphere (a = 1 cm, ka = 1.62): 0.1 dB at 480 nodes, 0.01 dB at
+    1920, against the flat scheme's 0.07 dB at 1280 panels."""
+    ref = modal.form_function((bc,), K38, 0.01)
+    errs = []
+    for lv in (1, 2):
+        Q = _sphere_quad(0.01, lv)
+        p, q, diag = NY.solve_boundary(Q, bc, K38, beta=0.0)
+        assert diag["relative_residual"] < 1e-12
+        f = Q.far_field(K38, np.array([-1.0, 0, 0]), p, q)[0]
+        errs.append(abs(f - ref) / abs(ref))
+    assert errs[0] < 1e-2 and errs[1] < 1.5e-3
+    assert errs[1] < errs[0] / 4
+
+
+def test_nystrom_fluid_sphere_against_modal_and_formulations():
+    g, h = 1028.9 / 1026.8, 1480.3 / 1477.4
+    ref = modal.form_function(("fluid", g, h), K38, 0.01)
+    Q = _sphere_quad(0.01, 1)
+    for form in ("muller", "cbie"):
+        p, q, _ = NY.solve_fluid(Q, g, h, K38, beta=0.0, formulation=form)
+        f = Q.far_field(K38, np.array([-1.0, 0, 0]), p, q)[0]
+        assert abs(f - ref) / abs(ref) < 0.02, form
+
+
+def test_matched_medium_is_invisible():
+    Q = _sphere_quad(0.01, 1)
+    p, q, _ = NY.solve_fluid(Q, 1.0, 1.0, K38, beta=0.3)
+    f = Q.far_field(K38, np.array([-1.0, 0, 0]), p, q)[0]
+    assert abs(f) < 1e-12
+
+
+def test_regular_and_calderon_rules_follow_the_quadrature_size():
+    Q = _sphere_quad(1.0, 1)
+    c, radius, rms = NY.quadrature_size(Q)
+    assert np.linalg.norm(c) < 1e-12 and radius <= 1.0 + 1e-12 and 0.99 < rms / radius <= 1.0
+    # 2 k rms <= 1 binds on a sphere (rms ~ radius)
+    assert NY.regular_range(0.499 / rms, Q) and not NY.regular_range(0.501 / rms, Q)
+    assert NY.regular_degree(6) == 2 and NY.regular_degree(12) == 3 and NY.regular_degree(16) == 4
+
+
+@pytest.mark.parametrize("g,h,k,refs", [
+    (0.0012, 0.23, 1.0, (0.08627220103266989 + 0.5718263756162451j, -1.169649453390533 + 0.8439868202224056j,
+                         -0.4123286201575636 + 0.7056764613690221j)),
+    (1000.0, 2.0, 1.0, (-0.4684232462649632 + 0.0118836351691168j, 0.1742206481610343 + 0.08026589273243463j,
+                        -0.2384449751929127 + 0.04496309639283315j)),
+    (0.0012, 0.23, 0.0138, (-0.2603525438230707 + 72.4628313282518j,)),
+])
+def test_reference_sphere_fluid_goldens(g, h, k, refs):
+    """test/extended/bem/sphere/full3d.jl:209-233 at the reference's 0.1 dB and
+    1 % bars, including the resonant gas bubble (|f| ~ 72, regular-wave
+    interpolation and Calderon blocks) the flat-panel scheme misses."""
+    Q = _sphere_quad(1.0, 2)
+    p, q, _ = NY.solve_fluid(Q, g, h, k, beta=0.0)
+    dirs = np.array([[-1.0, 0, 0], [1.0, 0, 0], [0, 1.0, 0]])[:len(refs)]
+    for fi, r in zip(Q.far_field(k, dirs, p, q), refs):
+        assert abs(20 * math.log10(abs(fi)) - 20 * math.log10(abs(r))) < 0.1
+        assert abs(fi - r) / abs(r) < 0.01
+
+
+def test_reference_spheroid_rigid_golden():
+    """test/extended/bem/spheroid/full3d.jl:10-24 (Spheroid(1.5, 1.0), k = 2,
+    beta = pi/3) at its 1 % bar with 480 quadratic-element nodes."""
+    nodes, tri = NY.curved_icosphere(1, (1.5, 1.0, 1.0), 2)
+    Q = NY.Quadrature(nodes, tri, 4)
+    p, q, _ = NY.solve_boundary(Q, "rigid", 2.0, beta=math.pi / 3)
+    refs = (([-0.5, -math.sqrt(3) / 2, 0.0], 0.277466754734215 + 0.478594831065819j),
+            ([0.5, math.sqrt(3) / 2, 0.0], 0.544249737180197 + 0.633566137972720j),
+            ([0.0, 0.0, 1.0], -0.604161429137314 + 0.394521828342496j))
+    for d, r in refs:
+        f = Q.far_field(2.0, np.array(d), p, q)[0]
+        assert abs(20 * math.log10(abs(f)) - 20 * math.log10(abs(r))) < 0.1
+        assert abs(f - r) / abs(r) < 0.01
