@@ -51,6 +51,8 @@ EXPORTED = [
     "bem_hmatrix_destroy", "bem_hmatrix_get_info", "bem_hmatrix_matvec", "bem_hmatrix_apply", "bem_hmatrix_solve",
     "bem_debug_lu", "bem_far_field_noalloc", "bem_pressure", "bem_reduced_defaults", "bem_reduced_sweep",
     "bem_axi_operators", "bem_axi_solve", "bem_axi_info", "bem_axi_traces", "bem_axi_far_field", "bem_axi_destroy",
+    "bem_curved_icosphere_size", "bem_curved_icosphere", "bem_mesh_create_nystrom", "bem_nystrom_quadrature",
+    "bem_mesh_nystrom_info",
 ]
 
 
@@ -198,6 +200,16 @@ def load_library(path=LIB_PATH):
         "bem_revolution_size": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _ip, _ip]),
         "bem_revolution": (ctypes.c_int, [ctypes.c_int32, _dp, _dp, ctypes.c_int32, _dp, _ip]),
         "bem_mesh_create": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, ctypes.c_int64, _ip, ctypes.POINTER(_vp)]),
+        "bem_curved_icosphere_size": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _ip, _ip]),
+        "bem_curved_icosphere": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_int32, _dp, _ip]),
+        "bem_mesh_create_nystrom": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int32, _ip,
+                                                   ctypes.c_int32, ctypes.POINTER(_vp)]),
+        "bem_nystrom_quadrature": (ctypes.c_int, [ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int32, _ip,
+                                                  ctypes.c_int32, _ip, _dp, _dp, _dp, ctypes.c_char_p,
+                                                  ctypes.c_int32]),
+        "bem_mesh_nystrom_info": (ctypes.c_int, [_vp, _ip, ctypes.POINTER(ctypes.c_int32),
+                                                 ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
         "bem_mesh_validate": (ctypes.c_int, [ctypes.c_int64, _dp, ctypes.c_int64, _ip, _dp,
                                              ctypes.c_char_p, ctypes.c_int32]),
         "bem_mesh_destroy": (ctypes.c_int, [_vp]),
@@ -401,6 +413,42 @@ def default_context():
 # ---------------------------------------------------------------------------
 # meshes
 # ---------------------------------------------------------------------------
+def curved_icosphere_arrays(level, ax=1.0, ay=None, az=None, order=2):
+    """Level-`level` icosphere with order-`order` Lagrange triangles (extra nodes
+    on the sphere), mapped by (ax, ay, az): nodes 3xNv, triangles npt x N
+    (1-based, Gmsh order)."""
+    L = load_library()
+    ay = ax if ay is None else ay
+    az = ay if az is None else az
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+    if L.bem_curved_icosphere_size(level, order, ctypes.byref(nv), ctypes.byref(nt)) != BEM_OK:
+        raise ValueError("mesh_order must be 1, 2 or 3")
+    npt = {1: 3, 2: 6, 3: 10}[order]
+    nodes = np.empty((nv.value, 3))
+    tri = np.empty((nt.value, npt), dtype=np.int64)
+    if L.bem_curved_icosphere(level, ax, ay, az, order, _d(nodes), _i(tri)) != BEM_OK:
+        raise ValueError("invalid icosphere")
+    return nodes.T.copy(), tri.T.copy()
+
+
+def nystrom_quadrature(nodes, triangles, qorder=4):
+    """Quadrature nodes of a (curved) triangle mesh on the host: (coords n x 3,
+    normals n x 3, weights n) in the node order of the device unknowns
+    (element by element).  Raises ValueError with the reference's messages."""
+    L = load_library()
+    nodes = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64).T)
+    tri = np.ascontiguousarray(np.asarray(triangles, dtype=np.int64).T)
+    n = ctypes.c_int64()
+    buf = ctypes.create_string_buffer(256)
+    args = (nodes.shape[0], _d(nodes), tri.shape[0], tri.shape[1], _i(tri), int(qorder))
+    if L.bem_nystrom_quadrature(*args, ctypes.byref(n), None, None, None, buf, 256) != BEM_OK:
+        raise ValueError(buf.value.decode())
+    xyz = np.empty((n.value, 3)); nr = np.empty((n.value, 3)); w = np.empty(n.value)
+    if L.bem_nystrom_quadrature(*args, ctypes.byref(n), _d(xyz), _d(nr), _d(w), buf, 256) != BEM_OK:
+        raise ValueError(buf.value.decode())
+    return xyz, nr, w
+
+
 def icosphere_arrays(level, ax=1.0, ay=None, az=None):
     """Level-`level` icosphere mapped by (ax, ay, az): nodes 3xNv, triangles 3xN (1-based)."""
     L = load_library()
@@ -445,27 +493,42 @@ def validate_surface(nodes, triangles):
 
 
 class Mesh:
-    """A closed triangulated surface resident on the device (`Mesh` of src/geometry/mesh.jl:39-44)."""
+    """A closed triangulated surface resident on the device (`Mesh` of src/geometry/mesh.jl:39-44).
 
-    def __init__(self, nodes, triangles, ctx=None, body=None):
+    Flat 3-node triangles with no `qorder` use one unknown per panel
+    (centroid collocation).  6- or 10-node triangles, or any triangles with a
+    `qorder`, use the reference's Nystrom discretisation: an order-`qorder`
+    quadrature on every curved element (default 4), whose nodes are the
+    unknowns (src/geometry/surface.jl:271-298,331)."""
+
+    def __init__(self, nodes, triangles, ctx=None, body=None, qorder=None):
         self.ctx = ctx or default_context()
         nodes = np.asarray(nodes, dtype=np.float64)
         triangles = np.asarray(triangles)
         if nodes.ndim != 2 or nodes.shape[0] != 3:
             raise ValueError("nodes must have three rows")
-        if triangles.ndim != 2 or triangles.shape[0] != 3:
-            raise ValueError("triangles must have 3 rows (flat panels)")
+        if triangles.ndim != 2 or triangles.shape[0] not in (3, 6, 10):
+            raise ValueError("triangles must have 3, 6 or 10 rows in Gmsh ordering")
         if not np.all(np.isfinite(nodes)):
             raise ValueError("node coordinates must be finite")
         self.nodes = nodes
         self.triangles = triangles.astype(np.int64)
         self.body = body
+        self.nystrom = qorder is not None or triangles.shape[0] != 3
+        self.qorder = (4 if qorder is None else int(qorder)) if self.nystrom else None
         h = _vp()
         nd = np.ascontiguousarray(nodes.T)
         tr = np.ascontiguousarray(self.triangles.T)
-        self.ctx.check(self.ctx.lib.bem_mesh_create(self.ctx._h, nd.shape[0], _d(nd), tr.shape[0], _i(tr),
-                                                    ctypes.byref(h)))
+        if self.nystrom:
+            if not 1 <= self.qorder <= 8:
+                raise ValueError("qorder must be between 1 and 8")
+            self.ctx.check(self.ctx.lib.bem_mesh_create_nystrom(self.ctx._h, nd.shape[0], _d(nd), tr.shape[0],
+                                                                tr.shape[1], _i(tr), self.qorder, ctypes.byref(h)))
+        else:
+            self.ctx.check(self.ctx.lib.bem_mesh_create(self.ctx._h, nd.shape[0], _d(nd), tr.shape[0], _i(tr),
+                                                        ctypes.byref(h)))
         self._h = h
+        self.n_elements = tr.shape[0]
         nt, nv = ctypes.c_int64(), ctypes.c_int64()
         res, vol = ctypes.c_double(), ctypes.c_double()
         self.ctx.lib.bem_mesh_info(h, ctypes.byref(nt), ctypes.byref(nv), ctypes.byref(res), ctypes.byref(vol))
@@ -485,8 +548,10 @@ class Mesh:
         return c, nr, a
 
     def quadrature(self):
-        """7-point Gauss nodes: list of (coords, normal, weight), the shape far_field consumes."""
-        xyz = np.empty((7 * self.n, 3)); nr = np.empty((7 * self.n, 3)); w = np.empty(7 * self.n)
+        """(coords, normals, weights), the shape far_field consumes: the 7-point
+        Gauss nodes of every panel, or the Nystrom nodes (the unknowns)."""
+        m = self.n if self.nystrom else 7 * self.n
+        xyz = np.empty((m, 3)); nr = np.empty((m, 3)); w = np.empty(m)
         self.ctx.check(self.ctx.lib.bem_mesh_quadrature(self._h, _d(xyz), _d(nr), _d(w)))
         return xyz, nr, w
 
@@ -496,17 +561,32 @@ class Mesh:
         return c.value
 
 
-def mesh(nodes_or_body, triangles=None, *, level=3, n_phi=64, units="m", ctx=None):
-    """`mesh(nodes, triangles)` for a supplied surface (1-based triangles,
-    src/geometry/surface.jl:271-298), `mesh(body; level)` for a Sphere /
-    Spheroid meshed as a mapped icosphere, or `mesh(cylinder; n_phi)` for a
+def mesh(nodes_or_body, triangles=None, *, level=3, n_phi=64, units="m", ctx=None, mesh_order=None,
+         qorder=None):
+    """`mesh(nodes, triangles; qorder)` for a supplied surface (1-based
+    triangles with 3, 6 or 10 rows, src/geometry/surface.jl:271-298),
+    `mesh(body; level, mesh_order, qorder)` for a Sphere / Spheroid meshed as
+    a mapped icosphere (flat panels, or curved order-`mesh_order` Nystrom
+    elements as gmsh_sphere_mesh / gmsh_spheroid_mesh build,
+    src/geometry/canonical_mesh.jl:3-36), or `mesh(cylinder; n_phi)` for a
     (bent) capped Cylinder with n_phi azimuthal segments."""
     scale = {"m": 1.0, "cm": 1e-2, "mm": 1e-3}.get(units)
     if scale is None:
         raise ValueError("units must be :m, :cm or :mm")
     if triangles is not None:
-        return Mesh(np.asarray(nodes_or_body, dtype=np.float64) * scale, triangles, ctx=ctx)
+        return Mesh(np.asarray(nodes_or_body, dtype=np.float64) * scale, triangles, ctx=ctx, qorder=qorder)
     body = nodes_or_body
+    if mesh_order is not None or qorder is not None:
+        order = 2 if mesh_order is None else mesh_order
+        if order not in (1, 2, 3):
+            raise ValueError("mesh_order must be 1, 2 or 3")
+        if isinstance(body, Sphere):
+            nodes, tri = curved_icosphere_arrays(level, body.radius * scale, order=order)
+        elif isinstance(body, Spheroid):
+            nodes, tri = curved_icosphere_arrays(level, body.a * scale, body.b * scale, body.b * scale, order=order)
+        else:
+            raise ValueError("curved meshes are built for spheres and spheroids")
+        return Mesh(nodes, tri, ctx=ctx, body=body, qorder=4 if qorder is None else qorder)
     if isinstance(body, Sphere):
         nodes, tri = icosphere_arrays(level, body.radius * scale)
     elif isinstance(body, Spheroid):
@@ -895,7 +975,8 @@ def bem(surface, boundary, k, *, incidence_angle=math.pi / 2, incidence_azimuth=
             return _bem_axisymmetric(surface, boundary, k, incidence_angle, n=n, m_max=m_max, ctx=ctx)
         if method != "full" or isinstance(surface, MeridianMesh):
             raise ValueError(f"bem(...) supports method=:axisymmetric or :full, got {method}")
-        surface = mesh(surface, level=kw.pop("level", 3), ctx=kw.pop("ctx", None))
+        surface = mesh(surface, level=kw.pop("level", 3), ctx=kw.pop("ctx", None),
+                       mesh_order=kw.pop("mesh_order", None), qorder=kw.pop("qorder", None))
     else:
         kw.pop("method", None)
     op = Operator(surface, boundary, k, formulation=kw.pop("formulation", None),

@@ -319,6 +319,7 @@ struct bem_mesh_s {
     bem_ctx ctx = nullptr;
     HostMesh h;
     DevMesh d;
+    double* sources = nullptr;  // Nystrom meshes: 3 x ns point sources of the interpolation basis
     std::vector<void*> bufs;
     std::atomic<int> refs{1};
 };
@@ -758,6 +759,85 @@ bem_lu_tune tune_lu(bem_ctx c, int n) {
     return rep;
 }
 
+// ---------------------------------------------------------------------------
+// curved-element Nystrom operators (density interpolation)
+// ---------------------------------------------------------------------------
+// Regular-wave interpolation (src/solvers/bem/full.jl:316-317,
+// quadrature/fluid.jl:9): Mueller bodies whose larger wavenumber satisfies
+// k * radius <= 1 and 2 k * rms <= 1 over the quadrature nodes; every other
+// system uses the point-source basis.
+bool nys_regular(const HostMesh& h, const bem_problem& p) {
+    if (p.boundary != BEM_FLUID || p.formulation != BEM_MULLER) return false;
+    const double kmax = std::max(p.k, p.k / p.soundspeed_contrast);
+    return kmax * h.qradius <= 1.0 && 2.0 * kmax * h.qrms <= 1.0;
+}
+
+NysBasis nys_basis(const bem_mesh m, double k, bool regular) {
+    NysBasis b;
+    b.k = k;
+    if (regular) {
+        b.kind = 1;
+        b.degree = (int)std::floor(std::sqrt(2.0 * m->h.nq)) - 1;  // quadrature/fluid.jl:112-113
+        b.P = (b.degree + 1) * (b.degree + 1);
+        for (int c = 0; c < 3; ++c) b.center[c] = m->h.qcenter[c];
+        b.radius = m->h.qradius;
+    } else {
+        b.kind = 0;
+        b.P = (int)(m->h.sources.size() / 3);
+        b.zs = m->sources;
+    }
+    return b;
+}
+
+// Device bytes of the correction workspace of an assembly (both media, both
+// operator families).
+size_t nys_bytes(const bem_mesh m) {
+    if (!m->h.nq) return 0;
+    const size_t nt = (size_t)m->d.nt;
+    const int P = std::max((int)(m->h.sources.size() / 3), 25);
+    return sizeof(z64) * (4 * (size_t)P * nt + nys_defect_parts((int)nt, P) + 8 * nt * m->h.nq) + 8 * 256;
+}
+
+// Naive Nystrom system of kind ap.kind into o0/o1, then the corrections of
+// S and D (and of K' and H when `derivative`) of each medium added into the
+// element-diagonal blocks.
+template <typename R>
+void nys_assemble(bem_operator op, const AsmParams& ap, C2<R>* o0, C2<R>* o1, bool derivative) {
+    hipStream_t s = op->lane->s;
+    const bem_mesh m = op->mesh;
+    const DevMesh& d = m->d;
+    const int nt = d.nt;
+    const bool two = ap.kind == ASM_MULLER || ap.kind == ASM_FLUID_CBIE;
+    const bool regular = nys_regular(m->h, op->prob);
+    const double evals = (double)nt * nt * (two ? 2.0 : 1.0);
+    prof_begin(&op->lane->pl, s, C_ASSEMBLE);
+    launch_nys_assemble<R>(s, d, ap, o0, o1);
+    prof_end(&op->lane->pl, s, C_ASSEMBLE, evals, (double)nt * nt * sizeof(C2<R>) * (two ? 4.0 : 2.0));
+    prof_begin(&op->lane->pl, s, C_NEAR);
+    const int P = nys_basis(m, ap.k, regular).P;
+    z64* g0 = (z64*)op_alloc(op, sizeof(z64) * P * (size_t)nt);
+    z64* g1 = (z64*)op_alloc(op, sizeof(z64) * P * (size_t)nt);
+    z64* th_sd = (z64*)op_alloc(op, sizeof(z64) * P * (size_t)nt);
+    z64* th_kh = derivative ? (z64*)op_alloc(op, sizeof(z64) * P * (size_t)nt) : nullptr;
+    z64* part = (z64*)op_alloc(op, sizeof(z64) * nys_defect_parts(nt, P));
+    z64* corr[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    for (int med = 0; med < (two ? 2 : 1); ++med) {
+        const NysBasis b = nys_basis(m, med == 0 ? ap.k : ap.kint, regular);
+        launch_nys_basis(s, d, b, g0, g1);
+        launch_nys_defect(s, d, b.k, P, derivative ? 3 : 1, g0, g1, part, th_sd, th_kh);
+        corr[med][0] = (z64*)op_alloc(op, sizeof(z64) * 2 * (size_t)nt * d.nq);
+        if (derivative) corr[med][1] = (z64*)op_alloc(op, sizeof(z64) * 2 * (size_t)nt * d.nq);
+        launch_nys_weights(s, d, P, g0, g1, th_sd, th_kh, corr[med][0], corr[med][1]);
+    }
+    launch_nys_apply<R>(s, d, ap, corr[0][0], corr[0][1], corr[1][0], corr[1][1], o0, o1);
+    prof_end(&op->lane->pl, s, C_NEAR, evals * (derivative ? 2.0 : 1.0) * P, 0.0);
+    HIPCHECK(hipGetLastError());
+}
+
+void require_panels(const bem_mesh m, const char* what) {
+    if (m->h.nq) throw BemError(BEM_ERR_ARGUMENT, std::string(what) + " requires a flat-panel mesh");
+}
+
 // Low-frequency Mueller systems take their derivative blocks from the
 // Calderon identities S K' = D S and S H = D D - I/4 (full.jl:187-207,
 // 319-339): when max(k, k_int) * radius <= pi/2, with radius the largest
@@ -810,8 +890,12 @@ void calderon_blocks(bem_operator op) {
         ap.kint = ap.k;
         ap.g = 1.0;
         ap.ld = nt;
-        launch_assemble<double>(s, op->mesh->d, ap, S, D);
-        launch_near_self<double>(s, op->mesh->d, ap, S, D);
+        if (op->mesh->d.nq) {
+            nys_assemble<double>(op, ap, S, D, false);
+        } else {
+            launch_assemble<double>(s, op->mesh->d, ap, S, D);
+            launch_near_self<double>(s, op->mesh->d, ap, S, D);
+        }
         gemm_assign<double>(s, D, nt, S, nt, M[med], ld2, nt, nt, nt);
         gemm_assign<double>(s, D, nt, D, nt, M[med] + nt, ld2, nt, nt, nt);
         launch_add_diag(s, nt, M[med] + nt, ld2, -0.25);
@@ -1140,16 +1224,23 @@ void operator_create(bem_ctx ctx, Lane* lane, bem_mesh mesh, const bem_problem* 
             // rigid solves out0 (1/2 I - D [- aH]); soft solves out1 (S [+ a(1/2 I + K')])
             if (prob->boundary == BEM_PRESSURE_RELEASE) std::swap(o0, o1);
         }
-        const double evals = (double)op->nt * op->nt * 7.0 * (fluid ? 2.0 : 1.0);
-        prof_begin(&op->lane->pl, s, C_ASSEMBLE);
-        if (op->single) launch_assemble<float>(s, mesh->d, ap, (z32*)o0, (z32*)o1);
-        else launch_assemble<double>(s, mesh->d, ap, (z64*)o0, (z64*)o1);
-        prof_end(&op->lane->pl, s, C_ASSEMBLE, evals, (double)n * n * esz * (fluid ? 1.0 : 2.0));
-        prof_begin(&op->lane->pl, s, C_NEAR);
-        if (op->single) launch_near_self<float>(s, mesh->d, ap, (z32*)o0, (z32*)o1);
-        else launch_near_self<double>(s, mesh->d, ap, (z64*)o0, (z64*)o1);
-        prof_end(&op->lane->pl, s, C_NEAR, (double)mesh->d.nnear * kNearPts * (fluid ? 2.0 : 1.0), 0.0);
         op->calderon = calderon_applies(mesh->h, *prob);
+        if (mesh->d.nq) {
+            // the derivative operators are direct unless the Calderon blocks replace them
+            const bool deriv = ap.kind == ASM_BM || (ap.kind == ASM_MULLER && !op->calderon);
+            if (op->single) nys_assemble<float>(op, ap, (z32*)o0, (z32*)o1, deriv);
+            else nys_assemble<double>(op, ap, (z64*)o0, (z64*)o1, deriv);
+        } else {
+            const double evals = (double)op->nt * op->nt * 7.0 * (fluid ? 2.0 : 1.0);
+            prof_begin(&op->lane->pl, s, C_ASSEMBLE);
+            if (op->single) launch_assemble<float>(s, mesh->d, ap, (z32*)o0, (z32*)o1);
+            else launch_assemble<double>(s, mesh->d, ap, (z64*)o0, (z64*)o1);
+            prof_end(&op->lane->pl, s, C_ASSEMBLE, evals, (double)n * n * esz * (fluid ? 1.0 : 2.0));
+            prof_begin(&op->lane->pl, s, C_NEAR);
+            if (op->single) launch_near_self<float>(s, mesh->d, ap, (z32*)o0, (z32*)o1);
+            else launch_near_self<double>(s, mesh->d, ap, (z64*)o0, (z64*)o1);
+            prof_end(&op->lane->pl, s, C_NEAR, (double)mesh->d.nnear * kNearPts * (fluid ? 2.0 : 1.0), 0.0);
+        }
         if (op->calderon) {
             if (op->single) calderon_blocks<float>(op);
             else calderon_blocks<double>(op);
@@ -1415,6 +1506,7 @@ size_t sweep_need(const bem_mesh m, const bem_problem& p) {
     b += sizeof(z64) * n * 8 + sizeof(double) * n * 3 + sizeof(int) * n * 2;
     b += sizeof(z64) * (2 * nt + kFarBlocks + 4) + 64 * 1024;
     if (calderon_applies(m->h, p)) b += calderon_bytes(m->d.nt);
+    b += nys_bytes(m);
     return b;
 }
 
@@ -3648,6 +3740,136 @@ int bem_mesh_create(bem_ctx ctx, int64_t nv, const double* nodes, int64_t nt, co
     });
 }
 
+int bem_nystrom_quadrature(int64_t nv, const double* nodes, int64_t nt, int32_t npt, const int64_t* tri,
+                           int32_t qorder, int64_t* n_nodes, double* coords, double* normals, double* weights,
+                           char* message, int32_t mlen) {
+    std::string msg;
+    auto fail = [&](const std::string& m) {
+        if (message && mlen > 0) std::snprintf(message, mlen, "%s", m.c_str());
+        return BEM_ERR_ARGUMENT;
+    };
+    if (!nodes || !tri || nv <= 0 || nt <= 0) return fail("surface must contain triangles");
+    try {
+        HostMesh h;
+        h.nv = nv;
+        h.nodes.assign(nodes, nodes + 3 * nv);
+        h.npt = npt;
+        h.qorder = qorder;
+        h.nel = nt;
+        if (npt != 3 && npt != 6 && npt != 10) return fail("triangles must have 3, 6 or 10 rows in Gmsh ordering");
+        h.elem.resize((size_t)npt * nt);
+        for (size_t i = 0; i < h.elem.size(); ++i) h.elem[i] = tri[i] - 1;
+        if (validate_curved(nv, h.nodes.data(), nt, npt, h.elem.data(), &h.resolution, &h.volume, msg))
+            return fail(msg);
+        if (build_nystrom(h, msg)) return fail(msg);
+        if (n_nodes) *n_nodes = h.nt;
+        for (int64_t p = 0; p < h.nt; ++p) {
+            for (int c = 0; c < 3; ++c) {
+                if (coords) coords[3 * p + c] = h.cen[3 * p + c];
+                if (normals) normals[3 * p + c] = h.nrm[3 * p + c];
+            }
+            if (weights) weights[p] = h.area[p];
+        }
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
+    if (message && mlen > 0) message[0] = 0;
+    return BEM_OK;
+}
+
+int bem_mesh_create_nystrom(bem_ctx ctx, int64_t nv, const double* nodes, int64_t nt, int32_t npt,
+                            const int64_t* tri, int32_t qorder, bem_mesh* out) {
+    if (!ctx || !out) return BEM_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        if (!nodes || !tri || nv <= 0 || nt <= 0) throw BemError(BEM_ERR_ARGUMENT, "surface must contain triangles");
+        if (npt != 3 && npt != 6 && npt != 10)
+            throw BemError(BEM_ERR_ARGUMENT, "triangles must have 3, 6 or 10 rows in Gmsh ordering");
+        if (qorder < 1 || qorder > 8) throw BemError(BEM_ERR_ARGUMENT, "qorder must be between 1 and 8");
+        if (nt > (int64_t)INT_MAX / 64) throw BemError(BEM_ERR_ARGUMENT, "too many triangles");
+        auto* m = new bem_mesh_s();
+        m->ctx = ctx;
+        HostMesh& h = m->h;
+        h.nv = nv;
+        h.nodes.assign(nodes, nodes + 3 * nv);
+        h.npt = npt;
+        h.qorder = qorder;
+        h.nel = nt;
+        h.elem.resize((size_t)npt * nt);
+        for (size_t i = 0; i < h.elem.size(); ++i) h.elem[i] = tri[i] - 1;
+        std::string msg;
+        if (validate_curved(nv, h.nodes.data(), nt, npt, h.elem.data(), &h.resolution, &h.volume, msg) ||
+            build_nystrom(h, msg)) {
+            delete m;
+            throw BemError(BEM_ERR_ARGUMENT, msg);
+        }
+        if (h.nt > (int64_t)INT_MAX / 2) {
+            delete m;
+            throw BemError(BEM_ERR_ARGUMENT, "too many quadrature nodes");
+        }
+        try {
+            hipStream_t s = ctx->lane0()->s;
+            DevMesh& d = m->d;
+            d.nt = (int)h.nt;
+            d.nq = h.nq;
+            auto up = [&](const void* src, size_t bytes) {
+                void* p = ctx->alloc(bytes);
+                m->bufs.push_back(p);
+                HIPCHECK(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s));
+                return p;
+            };
+            const int64_t n = h.nt;
+            std::vector<double> cen(3 * n), nrm(3 * n);
+            for (int64_t j = 0; j < n; ++j)
+                for (int c = 0; c < 3; ++c) {
+                    cen[c * n + j] = h.cen[3 * j + c];
+                    nrm[c * n + j] = h.nrm[3 * j + c];
+                }
+            d.cen = (double*)up(cen.data(), sizeof(double) * cen.size());
+            d.nrm = (double*)up(nrm.data(), sizeof(double) * nrm.size());
+            d.area = (double*)up(h.area.data(), sizeof(double) * n);
+            m->sources = (double*)up(h.sources.data(), sizeof(double) * h.sources.size());
+            HIPCHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            hipStreamSynchronize(ctx->lane0()->s);
+            for (void* p : m->bufs) ctx->release(p);
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int bem_mesh_nystrom_info(bem_mesh m, int64_t* n_elements, int32_t* npt, int32_t* nq, int32_t* qorder) {
+    if (!m) return BEM_ERR_ARGUMENT;
+    if (n_elements) *n_elements = m->h.nq ? m->h.nel : m->h.nt;
+    if (npt) *npt = m->h.npt;
+    if (nq) *nq = m->h.nq;
+    if (qorder) *qorder = m->h.qorder;
+    return BEM_OK;
+}
+
+int bem_curved_icosphere_size(int32_t level, int32_t order, int64_t* nv, int64_t* nt) {
+    if (level < 0 || level > 9 || order < 1 || order > 3 || !nv || !nt) return BEM_ERR_ARGUMENT;
+    const int64_t f = 20 * ((int64_t)1 << (2 * level));
+    const int64_t v = 10 * ((int64_t)1 << (2 * level)) + 2;
+    const int64_t e = 3 * f / 2;
+    *nt = f;
+    *nv = order == 1 ? v : order == 2 ? v + e : v + 2 * e + f;
+    return BEM_OK;
+}
+
+int bem_curved_icosphere(int32_t level, double ax, double ay, double az, int32_t order, double* nodes,
+                         int64_t* tri) {
+    if (level < 0 || level > 9 || order < 1 || order > 3 || !nodes || !tri) return BEM_ERR_ARGUMENT;
+    std::vector<double> v;
+    std::vector<int64_t> t;
+    curved_icosphere(level, ax, ay, az, order, v, t);
+    std::memcpy(nodes, v.data(), sizeof(double) * v.size());
+    for (size_t i = 0; i < t.size(); ++i) tri[i] = t[i] + 1;
+    return BEM_OK;
+}
+
 int bem_mesh_destroy(bem_mesh mesh) {
     if (!mesh) return BEM_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> g(mesh->ctx->mu);
@@ -3674,6 +3896,16 @@ int bem_mesh_panels(bem_mesh m, double* c, double* n, double* a) {
 
 int bem_mesh_quadrature(bem_mesh m, double* coords, double* normals, double* weights) {
     if (!m) return BEM_ERR_ARGUMENT;
+    if (m->h.nq) {
+        for (int64_t p = 0; p < m->h.nt; ++p) {
+            for (int c = 0; c < 3; ++c) {
+                if (coords) coords[3 * p + c] = m->h.cen[3 * p + c];
+                if (normals) normals[3 * p + c] = m->h.nrm[3 * p + c];
+            }
+            if (weights) weights[p] = m->h.area[p];
+        }
+        return BEM_OK;
+    }
     double l[7][3], w[7];
     q7_rule(l, w);
     const int64_t nt = m->h.nt;
@@ -3886,6 +4118,7 @@ int bem_pressure(bem_solution sol, int32_t r, int64_t npts, const double* points
     return guarded(ctx, [&] {
         if (field < BEM_FIELD_TOTAL || field > BEM_FIELD_INTERIOR)
             throw BemError(BEM_ERR_ARGUMENT, "field must be :total, :scattered, :incident or :interior");
+        require_panels(sol->mesh, "near-field pressure");
         for (int64_t i = 0; i < 3 * npts; ++i)
             if (!std::isfinite(points[i])) throw BemError(BEM_ERR_ARGUMENT, "point coordinates must be finite");
         FieldParams fp{};
@@ -4358,6 +4591,7 @@ int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, c
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     return guarded(ctx, [&] {
         validate_problem(*problem);
+        require_panels(mesh, "compression=(method=:hmatrix,)");
         const bool fluid = problem->boundary == BEM_FLUID;
         if (problem->boundary != BEM_RIGID && !fluid)
             throw BemError(BEM_ERR_ARGUMENT, "the hierarchical path solves rigid surfaces and fluid bodies");

@@ -50,6 +50,11 @@ struct DevMesh {
     float *cen32 = nullptr, *nrm32 = nullptr, *area32 = nullptr, *gp32 = nullptr;
     int *near_ptr = nullptr, *near_col = nullptr, *near_row = nullptr;
     int64_t nnear = 0;
+    // Curved-element Nystrom meshes: nq > 0 quadrature nodes per element
+    // (element e owns nodes e*nq ...), cen / nrm / area hold the nodes'
+    // coordinates, normals and weights, nt their count; the panel-only
+    // arrays above are null.
+    int nq = 0;
 };
 
 // System kinds written by the assembly kernels.
@@ -141,6 +146,39 @@ void launch_hnear_self(hipStream_t s, const DevMesh& m, int kind, const HPar& p,
 // partial-pivot ACA of the listed low-rank blocks (all when list is null)
 void launch_aca(hipStream_t s, const DevMesh& m, int kind, const HPar& p, const int* perm, int nlist, const int* list,
                 HBlk* blk, z32* uv, int* conv, double tol);
+
+// ---- bem_nystrom.hip: curved-element Nystrom operators ----------------------
+// Interpolation basis of the density-interpolation correction at wavenumber
+// k: point sources G(., z_p) (kind 0) or regular solid harmonics times the
+// normalised spherical Bessel series (kind 1, degree `degree`).
+struct NysBasis {
+    int kind = 0, P = 0, degree = 0;
+    double k = 0;
+    double center[3] = {0, 0, 0}, radius = 1;
+    const double* zs = nullptr;  // device, 3 x P (kind 0)
+};
+// Naive Nystrom entries w_j K(x_i, y_j) (zero at i = j) of the system `p.kind`
+// with its identity terms, row-major like launch_assemble.
+template <typename R>
+void launch_nys_assemble(hipStream_t s, const DevMesh& m, const AsmParams& p, C2<R>* out0, C2<R>* out1);
+// Basis traces at the nodes: g0[p * nt + j] (pressure), g1 (normal flux).
+void launch_nys_basis(hipStream_t s, const DevMesh& m, const NysBasis& b, z64* g0, z64* g1);
+// Green-identity defects of the naive layers at wavenumber k on the basis:
+// fam bit 0: th_sd = -1/2 g0 + S g1 - D g0; bit 1: th_kh = -1/2 g1 + K' g1 - H g0
+// (P x nt each, [p * nt + i]).  `part` holds nys_defect_parts(nt, P) complex.
+size_t nys_defect_parts(int nt, int P);
+void launch_nys_defect(hipStream_t s, const DevMesh& m, double k, int P, int fam, const z64* g0, const z64* g1,
+                       z64* part, z64* th_sd, z64* th_kh);
+// Per element, the scaled minimum-norm fit of each target's defect on the
+// element's basis traces: corr_*[(i * nq + b)] = {dS or dK', dD or dH} of
+// target node i and source node (i / nq) * nq + b.
+void launch_nys_weights(hipStream_t s, const DevMesh& m, int P, const z64* g0, const z64* g1, const z64* th_sd,
+                        const z64* th_kh, z64* corr_sd, z64* corr_kh);
+// Adds the corrections of the exterior (e_*) and interior (i_*; Mueller and
+// fluid CBIE) media into the element-diagonal blocks of the system.
+template <typename R>
+void launch_nys_apply(hipStream_t s, const DevMesh& m, const AsmParams& p, const z64* e_sd, const z64* e_kh,
+                      const z64* i_sd, const z64* i_kh, C2<R>* out0, C2<R>* out1);
 
 // ---- bem_linalg.hip ----
 template <typename R>

@@ -28,7 +28,42 @@ struct HostMesh {
     std::vector<int32_t> near_ptr;  // nt+1 CSR over target rows
     std::vector<int32_t> near_col;  // source panels, ascending per row
     double resolution = 0, volume = 0;
+    // Curved-element Nystrom meshes (nq > 0): the unknowns are the quadrature
+    // nodes, element e owning nodes e*nq .. e*nq+nq-1; cen, nrm and area hold
+    // their coordinates, unit normals and weights, and nt their count.  The
+    // nel elements of npt Gmsh-ordered nodes sit in `elem` (0-based).
+    int npt = 3, nq = 0, qorder = 0;
+    int64_t nel = 0;
+    std::vector<int64_t> elem;
+    double qcenter[3] = {0, 0, 0}, qradius = 0, qrms = 0;  // node mean, max and area-weighted rms distance
+    std::vector<double> sources;                          // 3 x ns point sources of the interpolation basis
 };
+
+// ---- curved-element Nystrom quadrature --------------------------------------
+// Fully symmetric triangle rule exact to degree >= qorder (1..8): nq
+// barycentric triples and weights summing to 1/2, the reference triangle's
+// area.  Returns nq, or 0 for an unsupported order.
+int triangle_rule(int qorder, std::vector<double>& bary, std::vector<double>& w);
+// Lagrange shape functions of Gmsh-ordered triangles (npt = 3, 6, 10) at
+// barycentric l[3]: values N[npt] and derivatives along u = l1, v = l2.
+void lagrange_shape(int npt, const double* l, double* N, double* Nu, double* Nv);
+// Topology checks of curved triangles (src/geometry/surface.jl:341-419):
+// node indices, repeated nodes, the corner checks of validate_topology, and
+// conforming edge nodes.  `tri0` is npt x nt, 0-based.
+int validate_curved(int64_t nv, const double* nodes, int64_t nt, int npt, const int64_t* tri0,
+                    double* resolution, double* volume, std::string& msg);
+// Quadrature nodes of m.elem (m.nodes, m.npt, m.qorder set): fills nq, nt,
+// cen, nrm, area, the size figures and the point sources.  Returns 0, or
+// nonzero with msg set for a singular Jacobian.
+int build_nystrom(HostMesh& m, std::string& msg);
+// Point sources of the general density interpolation: ceil(3 nq) points of a
+// golden-angle spiral on the sphere of radius 1.5 x half the diagonal of the
+// nodes' bounding box, about its centre.
+constexpr double kDimOversample = 3.0, kDimRadius = 1.5;
+// Icosphere of `level` with order-2 or order-3 nodes projected to the unit
+// sphere and scaled per axis (npt x nt elements, 0-based, Gmsh order).
+void curved_icosphere(int level, double ax, double ay, double az, int order, std::vector<double>& nodes,
+                      std::vector<int64_t>& tri0);
 
 // Radon 7-point rule: barycentric coordinates and weights (sum 1).
 void q7_rule(double l[7][3], double w[7]);

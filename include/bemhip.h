@@ -182,6 +182,17 @@ int bem_icosphere(int32_t level, double ax, double ay, double az, double *nodes,
  * (src/geometry/bodies.jl:73-89).  Sizes: nodes = (ns-2)*n_phi + 2,
  * triangles = 2*(ns-2)*n_phi. */
 int bem_revolution_size(int32_t ns, int32_t n_phi, int64_t *n_nodes, int64_t *n_tri);
+/* Curved icosphere: the level-`level` icosphere with order-2 (6-node) or
+ * order-3 (10-node) Lagrange triangles whose extra nodes are projected to the
+ * unit sphere, then mapped like bem_icosphere; the curved counterpart of
+ * gmsh_sphere_mesh / gmsh_spheroid_mesh with mesh_order 2 or 3
+ * (src/geometry/canonical_mesh.jl:3-36).  Sizes: N = 20*4^L triangles, nodes
+ * 10*4^L+2 (order 1), + 30*4^L (order 2), + 60*4^L + N (order 3).  tri is
+ * npt x N, 1-based, Gmsh order (vertices, edge nodes along 1->2, 2->3, 3->1,
+ * interior). */
+int bem_curved_icosphere_size(int32_t level, int32_t order, int64_t *n_nodes, int64_t *n_tri);
+int bem_curved_icosphere(int32_t level, double ax, double ay, double az, int32_t order, double *nodes,
+                         int64_t *tri);
 int bem_revolution(int32_t ns, const double *xs, const double *rs, int32_t n_phi, double *nodes,
                    int64_t *tri);
 
@@ -206,6 +217,34 @@ int bem_mesh_panels(bem_mesh mesh, double *centroids, double *normals, double *a
 int bem_mesh_quadrature(bem_mesh mesh, double *coords, double *normals, double *weights);
 /* Number of near (subdivided-quadrature) panel pairs. */
 int bem_mesh_near_count(bem_mesh mesh, int64_t *count);
+
+/* ---- curved-element Nystrom meshes --------------------------------------
+ * The reference's own full-3D discretisation: mesh(nodes, triangles; qorder)
+ * (src/geometry/surface.jl:271-298) over 3-, 6- or 10-node triangles in Gmsh
+ * order (npt x N, 1-based), with an order-`qorder` quadrature on every curved
+ * element (Inti.Quadrature, surface.jl:331; qorder 1..8, the reference's
+ * default 4 gives 6 nodes per element).  The unknowns are the quadrature
+ * nodes, numbered element by element; the layer operators are Nystrom
+ * matrices corrected by density interpolation (correction=(method=:dim,),
+ * src/solvers/bem/full.jl:89-97, src/solvers/bem/quadrature/fluid.jl:78-163).
+ * The returned mesh works with every operator, solve, sweep and server call;
+ * bem_mesh_info / bem_mesh_panels / bem_mesh_quadrature then report the
+ * quadrature nodes (count, coordinates, normals, weights), which are the
+ * reference's `quad` (full.jl:509-519).  The topology checks add the
+ * reference's curved ones (repeated nodes, conforming edge nodes,
+ * surface.jl:406-419).  Compression and near-field pressure need panel
+ * meshes and return BEM_ERR_ARGUMENT on these. */
+int bem_mesh_create_nystrom(bem_ctx ctx, int64_t n_nodes, const double *nodes, int64_t n_tri,
+                            int32_t nodes_per_tri, const int64_t *tri, int32_t qorder, bem_mesh *out);
+/* Host only: the same validation and quadrature without a device.  With
+ * coords == NULL only *n_quad is set (N times the rule's node count). */
+int bem_nystrom_quadrature(int64_t n_nodes, const double *nodes, int64_t n_tri, int32_t nodes_per_tri,
+                           const int64_t *tri, int32_t qorder, int64_t *n_quad, double *coords,
+                           double *normals, double *weights, char *message, int32_t message_len);
+/* Elements, nodes per element, quadrature nodes per element (0 for flat
+ * panels) and qorder of a mesh. */
+int bem_mesh_nystrom_info(bem_mesh mesh, int64_t *n_elements, int32_t *nodes_per_tri, int32_t *nq,
+                          int32_t *qorder);
 
 /* ---- operators: assemble + factor once, solve many ---------------------
  * Replaces _assemble_full_boundary/_assemble_full_fluid followed by
