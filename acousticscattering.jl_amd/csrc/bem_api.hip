@@ -53,10 +53,11 @@ struct BemError : std::runtime_error {
 // "h_rows" (the event timers do not nest).
 const char* kClassNames[] = {"assemble", "near",      "lu_panel", "lu_gemm", "lu_trsm", "lu_swap",
                              "solve",    "residual",  "far_field", "h_apply", "h_aca",  "h_ortho",
-                             "lu_gemm_fp64", "h_proj", "h_rows", "lu_split", "lu_gemm_bf16", "lu_graph"};
-constexpr int kNumClasses = 18;
+                             "lu_gemm_fp64", "h_proj", "h_rows", "lu_split", "lu_gemm_bf16", "lu_graph",
+                             "nys_assemble", "nys_defect", "nys_fit"};
+constexpr int kNumClasses = 21;
 enum { C_ASSEMBLE = 0, C_NEAR = 1, C_RESIDUAL = 7, C_FAR = 8, C_HAPPLY = 9, C_HACA = 10, C_HORTHO = 11,
-       C_HPROJ = 13, C_HROWS = 14, C_LUGRAPH = 17 };
+       C_HPROJ = 13, C_HROWS = 14, C_LUGRAPH = 17, C_NYS_ASM = 18, C_NYS_DEF = 19, C_NYS_FIT = 20 };
 
 struct ProfRec {
     int cls;
@@ -798,6 +799,13 @@ size_t nys_bytes(const bem_mesh m) {
     return sizeof(z64) * (4 * (size_t)P * nt + nys_defect_parts((int)nt, P) + 8 * nt * m->h.nq) + 8 * 256;
 }
 
+// FP64 operations of one point-kernel evaluation (kpoint, bem_kernels.h),
+// an FMA counted as two and sincos and rsqrt as one each: separation and
+// R^2 (8), 1/R, R, kR and sincos (4), the Green's function (6), D and K'
+// (16 each: projection, factor, complex product into the sum), H (20) and
+// n_x.n_y (5) for S, D, K' and H (Burton-Miller, Mueller); 34 for S and D.
+constexpr double kNysEvalFlops4 = 75.0, kNysEvalFlops2 = 34.0;
+
 // Naive Nystrom system of kind ap.kind into o0/o1, then the corrections of
 // S and D (and of K' and H when `derivative`) of each medium added into the
 // element-diagonal blocks.
@@ -809,11 +817,15 @@ void nys_assemble(bem_operator op, const AsmParams& ap, C2<R>* o0, C2<R>* o1, bo
     const int nt = d.nt;
     const bool two = ap.kind == ASM_MULLER || ap.kind == ASM_FLUID_CBIE;
     const bool regular = nys_regular(m->h, op->prob);
-    const double evals = (double)nt * nt * (two ? 2.0 : 1.0);
-    prof_begin(&op->lane->pl, s, C_ASSEMBLE);
+    // flops: kNysEvalFlops per pair and medium (the point kernels of the
+    // system's operators); bytes: the entries written
+    const double pairs = (double)nt * nt;
+    const int media = two ? 2 : 1;
+    const bool all4 = ap.kind == ASM_BM || ap.kind == ASM_MULLER;
+    prof_begin(&op->lane->pl, s, C_NYS_ASM);
     launch_nys_assemble<R>(s, d, ap, o0, o1);
-    prof_end(&op->lane->pl, s, C_ASSEMBLE, evals, (double)nt * nt * sizeof(C2<R>) * (two ? 4.0 : 2.0));
-    prof_begin(&op->lane->pl, s, C_NEAR);
+    prof_end(&op->lane->pl, s, C_NYS_ASM, pairs * media * (all4 ? kNysEvalFlops4 : kNysEvalFlops2),
+             pairs * sizeof(C2<R>) * (two ? 4.0 : 2.0));
     const int P = nys_basis(m, ap.k, regular).P;
     z64* g0 = (z64*)op_alloc(op, sizeof(z64) * P * (size_t)nt);
     z64* g1 = (z64*)op_alloc(op, sizeof(z64) * P * (size_t)nt);
@@ -821,16 +833,26 @@ void nys_assemble(bem_operator op, const AsmParams& ap, C2<R>* o0, C2<R>* o1, bo
     z64* th_kh = derivative ? (z64*)op_alloc(op, sizeof(z64) * P * (size_t)nt) : nullptr;
     z64* part = (z64*)op_alloc(op, sizeof(z64) * nys_defect_parts(nt, P));
     z64* corr[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    for (int med = 0; med < (two ? 2 : 1); ++med) {
+    const int fams = derivative ? 2 : 1;
+    for (int med = 0; med < media; ++med) {
         const NysBasis b = nys_basis(m, med == 0 ? ap.k : ap.kint, regular);
+        prof_begin(&op->lane->pl, s, C_NYS_FIT);
         launch_nys_basis(s, d, b, g0, g1);
+        prof_end(&op->lane->pl, s, C_NYS_FIT, 0.0, 0.0);
+        // algorithmic flops of the defects: two complex multiply-adds (8 flops
+        // each) per pair, basis function and operator family
+        prof_begin(&op->lane->pl, s, C_NYS_DEF);
         launch_nys_defect(s, d, b.k, P, derivative ? 3 : 1, g0, g1, part, th_sd, th_kh);
+        prof_end(&op->lane->pl, s, C_NYS_DEF, pairs * fams * 16.0 * P, 0.0);
         corr[med][0] = (z64*)op_alloc(op, sizeof(z64) * 2 * (size_t)nt * d.nq);
         if (derivative) corr[med][1] = (z64*)op_alloc(op, sizeof(z64) * 2 * (size_t)nt * d.nq);
+        prof_begin(&op->lane->pl, s, C_NYS_FIT);
         launch_nys_weights(s, d, P, g0, g1, th_sd, th_kh, corr[med][0], corr[med][1]);
+        prof_end(&op->lane->pl, s, C_NYS_FIT, 0.0, 0.0);
     }
+    prof_begin(&op->lane->pl, s, C_NYS_FIT);
     launch_nys_apply<R>(s, d, ap, corr[0][0], corr[0][1], corr[1][0], corr[1][1], o0, o1);
-    prof_end(&op->lane->pl, s, C_NEAR, evals * (derivative ? 2.0 : 1.0) * P, 0.0);
+    prof_end(&op->lane->pl, s, C_NYS_FIT, 0.0, 0.0);
     HIPCHECK(hipGetLastError());
 }
 

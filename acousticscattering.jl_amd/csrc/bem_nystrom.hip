@@ -369,7 +369,9 @@ __global__ __launch_bounds__(64) void k_nys_weights(int nt, int nq, int P, const
     }
     for (int t = lane; t < nc * nc; t += 64) Q[t] = z64{t % (nc + 1) == 0 ? 1.0 : 0.0, 0.0};
     __syncthreads();
-    // one-sided Jacobi: each lane owns row `lane` of B and of Q
+    // one-sided Jacobi: each lane owns row `lane` of B and of Q; a pair is
+    // rotated while |b_p^H b_q| exceeds mr * eps of |b_p| |b_q|
+    const double tol = mr * DBL_EPSILON;
     for (int sweep = 0; sweep < kJacobiSweeps; ++sweep) {
         bool rotated = false;
         for (int p = 0; p < nc - 1; ++p) {
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(64) void k_nys_weights(int nt, int nq, int P, const
                 double gr = bp.x * bq.x + bp.y * bq.y, gi = bp.x * bq.y - bp.y * bq.x;  // conj(bp) bq
                 wave_sum4(al, be, gr, gi);
                 const double g = hypot(gr, gi);
-                if (!(g > 1e-15 * sqrt(al * be)) || !(g > 0)) continue;
+                if (!(g > tol * sqrt(al * be)) || !(g > 0)) continue;
                 rotated = true;
                 const double zeta = (be - al) / (2.0 * g);
                 const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
