@@ -187,7 +187,7 @@ void launch_nys_basis(hipStream_t s, const DevMesh& m, const NysBasis& b, z64* g
 // order), kDefPB basis functions per grid layer.  Source nodes and their
 // basis traces are staged in LDS by chunks and read wave-uniformly.
 // ---------------------------------------------------------------------------
-constexpr int kDefPB = 8;
+constexpr int kDefPB = 18;
 constexpr int kDefChunk = 64;
 constexpr int kDefThreads = 256;
 
@@ -226,12 +226,13 @@ __global__ __launch_bounds__(kDefThreads) void k_nys_defect(DevMesh m, double k,
             sy[t][3] = m.nrm[j]; sy[t][4] = m.nrm[nt + j]; sy[t][5] = m.nrm[2 * nt + j];
             sy[t][6] = m.area[j];
         }
+        // basis values beyond the pass's pb functions are staged as zeros, so
+        // the products below run unpredicated
         for (int t = threadIdx.x; t < kDefPB * kDefChunk; t += kDefThreads) {
             const int q = t / kDefChunk, jj = t % kDefChunk;
-            if (q < pb && jj < cn) {
-                sg0[q][jj] = g0[(size_t)(p0 + q) * nt + c0 + jj];
-                sg1[q][jj] = g1[(size_t)(p0 + q) * nt + c0 + jj];
-            }
+            const bool ok = q < pb && jj < cn;
+            sg0[q][jj] = ok ? g0[(size_t)(p0 + q) * nt + c0 + jj] : z64{0, 0};
+            sg1[q][jj] = ok ? g1[(size_t)(p0 + q) * nt + c0 + jj] : z64{0, 0};
         }
         __syncthreads();
         for (int jj = 0; jj < cn; ++jj) {
@@ -241,18 +242,29 @@ __global__ __launch_bounds__(kDefThreads) void k_nys_defect(DevMesh m, double k,
             const double c = nx0 * ny0 + nx1 * ny1 + nx2 * ny2;
             Acc4<double> v{};
             kpoint<double, NEED>(k, rx, ry, rz, nx0, nx1, nx2, ny0, ny1, ny2, c, sy[jj][6], v);
+            // eight FMAs per basis function and family: += S b - D a (or K' b - H a)
 #pragma unroll
             for (int q = 0; q < kDefPB; ++q) {
-                if (q < pb) {
-                    const z64 a = sg0[q][jj], b = sg1[q][jj];
-                    if (FAM & 1) {
-                        asd[q].x += v.S.x * b.x - v.S.y * b.y - (v.D.x * a.x - v.D.y * a.y);
-                        asd[q].y += v.S.x * b.y + v.S.y * b.x - (v.D.x * a.y + v.D.y * a.x);
-                    }
-                    if (FAM & 2) {
-                        akh[q].x += v.K.x * b.x - v.K.y * b.y - (v.H.x * a.x - v.H.y * a.y);
-                        akh[q].y += v.K.x * b.y + v.K.y * b.x - (v.H.x * a.y + v.H.y * a.x);
-                    }
+                const z64 a = sg0[q][jj], b = sg1[q][jj];
+                if (FAM & 1) {
+                    asd[q].x = fma(v.S.x, b.x, asd[q].x);
+                    asd[q].x = fma(-v.S.y, b.y, asd[q].x);
+                    asd[q].x = fma(-v.D.x, a.x, asd[q].x);
+                    asd[q].x = fma(v.D.y, a.y, asd[q].x);
+                    asd[q].y = fma(v.S.x, b.y, asd[q].y);
+                    asd[q].y = fma(v.S.y, b.x, asd[q].y);
+                    asd[q].y = fma(-v.D.x, a.y, asd[q].y);
+                    asd[q].y = fma(-v.D.y, a.x, asd[q].y);
+                }
+                if (FAM & 2) {
+                    akh[q].x = fma(v.K.x, b.x, akh[q].x);
+                    akh[q].x = fma(-v.K.y, b.y, akh[q].x);
+                    akh[q].x = fma(-v.H.x, a.x, akh[q].x);
+                    akh[q].x = fma(v.H.y, a.y, akh[q].x);
+                    akh[q].y = fma(v.K.x, b.y, akh[q].y);
+                    akh[q].y = fma(v.K.y, b.x, akh[q].y);
+                    akh[q].y = fma(-v.H.x, a.y, akh[q].y);
+                    akh[q].y = fma(-v.H.y, a.x, akh[q].y);
                 }
             }
         }
