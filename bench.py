@@ -74,6 +74,7 @@ def sweep_plan(bh, ctx, n_points, max_level):
 
 TRAFFIC_PROFILE = os.path.join("profiles", "lu_gemm3m_traffic_fluid_spheroid_n20480.md")
 TRAFFIC_PROFILE_BF16 = os.path.join("profiles", "lu_gemm_bf16_traffic_fluid_spheroid_n20480.md")
+TRAFFIC_PROFILE_HM = os.path.join("profiles", "hmatrix_apply_traffic_bent_cylinder_200k.md")
 
 
 def gemm_roofline(fp32, bf16, traffic_note):
@@ -543,7 +544,10 @@ def hmatrix_main(args):
         # 8 TB/s): the products, FP64 on the matrix cores, bound it
         "roofline": {"kernel": "compressed apply (64 right-hand sides)", "bound": "mfma", "achieved": tfs,
                      "peak": MFMA_FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": tfs * 1e12 / MFMA_FP64_PEAK,
-                     "traffic": None, "hbm_gbs": gbs, "hbm_frac": gbs / 8000.0,
+                     "traffic": measured_traffic(TRAFFIC_PROFILE_HM), "traffic_source": TRAFFIC_PROFILE_HM,
+                     "traffic_basis": "HBM bytes per 64-RHS apply (k_hproj + k_hrows), FETCH_SIZE and WRITE_SIZE "
+                                      "passes of this workload",
+                     "hbm_gbs": gbs, "hbm_frac": gbs / 8000.0,
                      "flops_per_byte": ap["flops"] / ap["bytes"] if ap["bytes"] > 0 else None,
                      "launches": ap["launches"], "avg_launch_ms": ap["ms"] / max(1, ap["launches"]),
                      "kernels": {kn: {"avg_launch_ms": stats[kn]["ms"] / max(1, stats[kn]["launches"]),
