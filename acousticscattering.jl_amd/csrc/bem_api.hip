@@ -860,6 +860,110 @@ void require_panels(const bem_mesh m, const char* what) {
     if (m->h.nq) throw BemError(BEM_ERR_ARGUMENT, std::string(what) + " requires a flat-panel mesh");
 }
 
+// Near-field pressure of a Nystrom solution (src/fields/pressure.jl:404-472):
+// regions by winding number, then per region (exterior at k, interior at
+// k_int) and per basis the layer potentials with the nearest element excluded
+// and recovered by density interpolation (maxdist = 2 radius,
+// pressure.jl:441-462).  Points with k |x - c| <= 2 use the regular-wave basis
+// when k is in its range (pressure.jl:446-450).
+void nys_pressure(bem_solution sol, int r, int64_t npts, const double* points, int field, double* out) {
+    bem_ctx ctx = sol->ctx;
+    const bem_mesh m = sol->mesh;
+    const HostMesh& h = m->h;
+    const DevMesh& d = m->d;
+    hipStream_t s = ctx->lane0()->s;
+    FieldParams fp{};
+    fp.k = sol->k;
+    fp.fluid = sol->prob.boundary == BEM_FLUID ? 1 : 0;
+    fp.g = fp.fluid ? sol->prob.density_contrast : 1.0;
+    fp.kint = fp.fluid ? sol->k / sol->prob.soundspeed_contrast : sol->k;
+    fp.field = field;
+    direction(sol->angles[2 * r], sol->angles[2 * r + 1], fp.d);
+    const z64* P = sol->p + (size_t)r * sol->nt;
+    const z64* Q = sol->q + (size_t)r * sol->nt;
+    std::vector<void*> held;
+    auto alloc = [&](size_t bytes) {
+        void* p = ctx->alloc(bytes);
+        held.push_back(p);
+        return p;
+    };
+    try {
+        const int64_t chunk = std::min<int64_t>(npts, 1 << 18);
+        double* dpts = (double*)alloc(sizeof(double) * 3 * chunk);
+        z64* dout = (z64*)alloc(sizeof(z64) * chunk);
+        int* dreg = (int*)alloc(sizeof(int) * chunk);
+        double* dwind = (double*)alloc(sizeof(double) * chunk);
+        int* didx = (int*)alloc(sizeof(int) * chunk);
+        // basis traces and element pseudo-inverses per (region, basis), made once
+        struct Prep { z64 *g0 = nullptr, *g1 = nullptr, *pinv = nullptr; NysBasis b; };
+        Prep prep[2][2];
+        auto prepared = [&](int inside, int regular) -> Prep& {
+            Prep& pr = prep[inside][regular];
+            if (!pr.g0) {
+                pr.b = nys_basis(m, inside ? fp.kint : fp.k, regular != 0);
+                const size_t nb = (size_t)pr.b.P * d.nt;
+                pr.g0 = (z64*)alloc(sizeof(z64) * nb);
+                pr.g1 = (z64*)alloc(sizeof(z64) * nb);
+                pr.pinv = (z64*)alloc(sizeof(z64) * (size_t)(d.nt / d.nq) * 2 * d.nq * pr.b.P);
+                launch_nys_basis(s, d, pr.b, pr.g0, pr.g1);
+                launch_nys_pinv(s, d, pr.b.P, pr.g0, pr.g1, pr.pinv);
+            }
+            return pr;
+        };
+        auto in_range = [&](double k) { return k * h.qradius <= 1.0 && 2.0 * k * h.qrms <= 1.0; };
+        std::vector<int> reg(chunk), idx;
+        for (int64_t i0 = 0; i0 < npts; i0 += chunk) {
+            const int nc = (int)std::min<int64_t>(chunk, npts - i0);
+            HIPCHECK(hipMemcpyAsync(dpts, points + 3 * i0, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, s));
+            launch_nys_winding(s, d, fp, nc, dpts, dreg, dwind);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(reg.data(), dreg, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            for (int i = 0; i < nc; ++i)
+                if (reg[i] == REGION_NONE)
+                    throw BemError(BEM_ERR_ARGUMENT, "the selected field has no acoustic pressure at this point");
+            for (int inside = 0; inside < 2; ++inside)
+                for (int regular = 0; regular < 2; ++regular) {
+                    const double k = inside ? fp.kint : fp.k;
+                    const bool ok = in_range(k);
+                    idx.clear();
+                    for (int i = 0; i < nc; ++i) {
+                        if (reg[i] != (inside ? REGION_INTERIOR : REGION_EXTERIOR)) continue;
+                        const double* x = points + 3 * (i0 + i);
+                        const double dx = x[0] - h.qcenter[0], dy = x[1] - h.qcenter[1], dz = x[2] - h.qcenter[2];
+                        const bool near = ok && k * std::sqrt(dx * dx + dy * dy + dz * dz) <= 2.0;
+                        if (near == (regular != 0)) idx.push_back(i);
+                    }
+                    if (idx.empty()) continue;
+                    Prep& pr = prepared(inside, regular);
+                    NysPointEval ev{};
+                    ev.inside = inside;
+                    ev.k = k;
+                    ev.maxdist = 2.0 * h.qradius;
+                    ev.basis = pr.b;
+                    ev.g0 = pr.g0;
+                    ev.g1 = pr.g1;
+                    ev.pinv = pr.pinv;
+                    ev.g = fp.g;
+                    ev.k0 = fp.k;
+                    for (int c = 0; c < 3; ++c) ev.d[c] = fp.d[c];
+                    ev.add_incident = !inside && field == BEM_FIELD_TOTAL;
+                    HIPCHECK(hipMemcpyAsync(didx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, s));
+                    launch_nys_points(s, d, ev, (int)idx.size(), didx, dpts, P, Q, dout);
+                    HIPCHECK(hipGetLastError());
+                    HIPCHECK(hipStreamSynchronize(s));
+                }
+            HIPCHECK(hipMemcpyAsync(out + 2 * i0, dout, sizeof(z64) * nc, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+        }
+    } catch (...) {
+        hipStreamSynchronize(s);
+        for (void* p : held) ctx->release(p);
+        throw;
+    }
+    for (void* p : held) ctx->release(p);
+}
+
 // Low-frequency Mueller systems take their derivative blocks from the
 // Calderon identities S K' = D S and S H = D D - I/4 (full.jl:187-207,
 // 319-339): when max(k, k_int) * radius <= pi/2, with radius the largest
@@ -3851,6 +3955,10 @@ int bem_mesh_create_nystrom(bem_ctx ctx, int64_t nv, const double* nodes, int64_
             d.nrm = (double*)up(nrm.data(), sizeof(double) * nrm.size());
             d.area = (double*)up(h.area.data(), sizeof(double) * n);
             m->sources = (double*)up(h.sources.data(), sizeof(double) * h.sources.size());
+            std::vector<double> subs;
+            sub_triangles(h, subs);
+            d.vert = (double*)up(subs.data(), sizeof(double) * subs.size());
+            d.nsub = (int)(subs.size() / 9);
             HIPCHECK(hipStreamSynchronize(s));
         } catch (...) {
             hipStreamSynchronize(ctx->lane0()->s);
@@ -4140,9 +4248,12 @@ int bem_pressure(bem_solution sol, int32_t r, int64_t npts, const double* points
     return guarded(ctx, [&] {
         if (field < BEM_FIELD_TOTAL || field > BEM_FIELD_INTERIOR)
             throw BemError(BEM_ERR_ARGUMENT, "field must be :total, :scattered, :incident or :interior");
-        require_panels(sol->mesh, "near-field pressure");
         for (int64_t i = 0; i < 3 * npts; ++i)
             if (!std::isfinite(points[i])) throw BemError(BEM_ERR_ARGUMENT, "point coordinates must be finite");
+        if (sol->mesh->d.nq && field != BEM_FIELD_INCIDENT) {
+            nys_pressure(sol, r, npts, points, field, out);
+            return;
+        }
         FieldParams fp{};
         fp.k = sol->k;
         fp.fluid = sol->prob.boundary == BEM_FLUID ? 1 : 0;

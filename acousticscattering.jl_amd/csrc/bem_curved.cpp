@@ -272,6 +272,24 @@ int build_nystrom(HostMesh& m, std::string& msg) {
     return 0;
 }
 
+void sub_triangles(const HostMesh& m, std::vector<double>& out) {
+    static const int T1[1][3] = {{0, 1, 2}};
+    static const int T2[4][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}, {3, 4, 5}};
+    // the cubic element's nodes on the lattice (i, j), u = i/3, v = j/3:
+    // up triangles (i,j),(i+1,j),(i,j+1) and down ones (i+1,j),(i+1,j+1),(i,j+1)
+    static const int T3[9][3] = {{0, 3, 8}, {3, 9, 8}, {3, 4, 9}, {4, 5, 9}, {4, 1, 5},
+                                 {8, 9, 7}, {9, 6, 7}, {9, 5, 6}, {7, 6, 2}};
+    const int ns = m.npt == 3 ? 1 : m.npt == 6 ? 4 : 9;
+    const int(*T)[3] = m.npt == 3 ? T1 : m.npt == 6 ? T2 : T3;
+    out.assign(9 * (size_t)ns * m.nel, 0.0);
+    for (int64_t e = 0; e < m.nel; ++e) {
+        const int64_t* E = &m.elem[(size_t)m.npt * e];
+        for (int t = 0; t < ns; ++t)
+            for (int v = 0; v < 3; ++v)
+                for (int c = 0; c < 3; ++c) out[9 * ((size_t)e * ns + t) + 3 * v + c] = m.nodes[3 * E[T[t][v]] + c];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // curved icosphere: the flat icosphere's vertices and faces, then per face
 // (in face order) the nodes of its edges (a,b), (b,c), (c,a) at their first

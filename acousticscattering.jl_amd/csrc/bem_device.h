@@ -53,8 +53,10 @@ struct DevMesh {
     // Curved-element Nystrom meshes: nq > 0 quadrature nodes per element
     // (element e owns nodes e*nq ...), cen / nrm / area hold the nodes'
     // coordinates, normals and weights, nt their count; the panel-only
-    // arrays above are null.
+    // arrays above are null except `vert`, which holds nsub flat triangles
+    // through every node of each element (the surface of the winding number).
     int nq = 0;
+    int nsub = 0;
 };
 
 // System kinds written by the assembly kernels.
@@ -174,6 +176,31 @@ void launch_nys_defect(hipStream_t s, const DevMesh& m, double k, int P, int fam
 // target node i and source node (i / nq) * nq + b.
 void launch_nys_weights(hipStream_t s, const DevMesh& m, int P, const z64* g0, const z64* g1, const z64* th_sd,
                         const z64* th_kh, z64* corr_sd, z64* corr_kh);
+// With `pinv` set, launch_nys_weights writes instead each element's scaled
+// pseudo-inverse: pinv[(e * 2nq + c) * P + p], the fit w_c of the unit defect
+// e_p (w = pinv th).
+void launch_nys_pinv(hipStream_t s, const DevMesh& m, int P, const z64* g0, const z64* g1, z64* pinv);
+// Pressure of a Nystrom solution (src/fields/pressure.jl:441-462): winding
+// numbers and regions of the points (as launch_pressure), then, per region
+// and basis, the layer potentials at the points in `idx` with the nearest
+// element excluded and recovered by density interpolation.
+void launch_nys_winding(hipStream_t s, const DevMesh& m, const FieldParams& fp, int npts, const double* pts,
+                        int* region, double* winding);
+struct NysPointEval {
+    int inside;          // 1: S u - D v with multiplier -1; 0: D v - S u, multiplier 0
+    double k;            // wavenumber of the region
+    double maxdist;      // corrections apply when the nearest node is this close
+    NysBasis basis;      // interpolation basis at k (traces g0, g1 at the nodes; pinv per element)
+    const z64* g0;
+    const z64* g1;
+    const z64* pinv;
+    // densities: u multiplies S, v multiplies D.  Exterior: u = q, v = p; interior:
+    // v = p + p_inc, u = g (q + q_inc) with the incident wave of direction d, k0
+    double g, k0, d[3];
+    int add_incident;    // exterior total field
+};
+void launch_nys_points(hipStream_t s, const DevMesh& m, const NysPointEval& ev, int n, const int* idx,
+                       const double* pts, const z64* p, const z64* q, z64* out);
 // Adds the corrections of the exterior (e_*) and interior (i_*; Mueller and
 // fluid CBIE) media into the element-diagonal blocks of the system.
 template <typename R>

@@ -56,6 +56,10 @@ int validate_curved(int64_t nv, const double* nodes, int64_t nt, int npt, const 
 // cen, nrm, area, the size figures and the point sources.  Returns 0, or
 // nonzero with msg set for a singular Jacobian.
 int build_nystrom(HostMesh& m, std::string& msg);
+// Flat triangles through every node of each element (1, 4 or 9 per element,
+// oriented like it), 9 doubles each: the surface whose winding number
+// classifies field points.
+void sub_triangles(const HostMesh& m, std::vector<double>& out);
 // Point sources of the general density interpolation: ceil(3 nq) points of a
 // golden-angle spiral on the sphere of radius 1.5 x half the diagonal of the
 // nodes' bounding box, about its centre.

@@ -118,4 +118,19 @@ __device__ inline void wave_sum_acc(Acc4<double>& a) {
 }
 
 
+// Solid angle of the flat triangle V (9 doubles) about x, from Van Oosterom
+// and Strackee (IEEE Trans. Biomed. Eng. 30 (1983) 125-126):
+// tan(Omega/2) = a.(b x c) / (|a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a|).
+__device__ inline double tri_solid_angle(const double* V, double x0, double x1, double x2) {
+    const double a0 = V[0] - x0, a1 = V[1] - x1, a2 = V[2] - x2;
+    const double b0 = V[3] - x0, b1 = V[4] - x1, b2 = V[5] - x2;
+    const double c0 = V[6] - x0, c1 = V[7] - x1, c2 = V[8] - x2;
+    const double la = sqrt(a0 * a0 + a1 * a1 + a2 * a2), lb = sqrt(b0 * b0 + b1 * b1 + b2 * b2),
+                 lc = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    const double num = a0 * (b1 * c2 - b2 * c1) + a1 * (b2 * c0 - b0 * c2) + a2 * (b0 * c1 - b1 * c0);
+    const double den = la * lb * lc + (a0 * b0 + a1 * b1 + a2 * b2) * lc + (a0 * c0 + a1 * c1 + a2 * c2) * lb +
+                       (b0 * c0 + b1 * c1 + b2 * c2) * la;
+    return 2.0 * atan2(num, den);
+}
+
 }  // namespace bem

@@ -88,13 +88,10 @@ __device__ inline z64 zmul(z64 a, z64 b) { return z64{a.x * b.x - a.y * b.y, a.x
 __device__ inline z64 zadd(z64 a, z64 b) { return z64{a.x + b.x, a.y + b.y}; }
 __device__ inline z64 zsc(z64 a, double s) { return z64{a.x * s, a.y * s}; }
 
-__global__ __launch_bounds__(128) void k_nys_basis(DevMesh m, NysBasis b, z64* __restrict__ g0,
-                                                   z64* __restrict__ g1) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nt = m.nt;
-    if (j >= nt) return;
-    const double x[3] = {m.cen[j], m.cen[nt + j], m.cen[2 * nt + j]};
-    const double n[3] = {m.nrm[j], m.nrm[nt + j], m.nrm[2 * nt + j]};
+// Basis traces at one point x with normal n: g0[p * stride], g1[p * stride]
+// for p < b.P (with n = 0 the fluxes are zero).
+__device__ void basis_point(const NysBasis& b, const double* x, const double* n, z64* g0, z64* g1,
+                            size_t stride) {
     if (b.kind == 0) {
         // G(x, z) = e^{ikR}/(4 pi R) and its derivative along n at x
         for (int p = 0; p < b.P; ++p) {
@@ -107,8 +104,8 @@ __global__ __launch_bounds__(128) void k_nys_basis(DevMesh m, NysBasis b, z64* _
             const double Gx = gr * cs, Gy = gr * sn;
             const double a = (rx * n[0] + ry * n[1] + rz * n[2]) * iR;
             const double tx = -a * iR, ty = b.k * a;
-            g0[(size_t)p * nt + j] = z64{Gx, Gy};
-            g1[(size_t)p * nt + j] = z64{Gx * tx - Gy * ty, Gx * ty + Gy * tx};
+            g0[(size_t)p * stride] = z64{Gx, Gy};
+            g1[(size_t)p * stride] = z64{Gx * tx - Gy * ty, Gx * ty + Gy * tx};
         }
         return;
     }
@@ -165,16 +162,26 @@ __global__ __launch_bounds__(128) void k_nys_basis(DevMesh m, NysBasis b, z64* _
             for (int c = 0; c < 3; ++c)
                 fl = zadd(fl, zsc(zadd(zsc(cg[c], radial), zsc(cur, 2.0 * v[c] * deriv)), n[c]));
             fl = zsc(fl, nrm / b.radius);
-            g0[(size_t)col * nt + j] = val;
-            g1[(size_t)col * nt + j] = fl;
+            g0[(size_t)col * stride] = val;
+            g1[(size_t)col * stride] = fl;
             ++col;
             if (m > 0) {
-                g0[(size_t)col * nt + j] = z64{val.x, -val.y};
-                g1[(size_t)col * nt + j] = z64{fl.x, -fl.y};
+                g0[(size_t)col * stride] = z64{val.x, -val.y};
+                g1[(size_t)col * stride] = z64{fl.x, -fl.y};
                 ++col;
             }
         }
     }
+}
+
+__global__ __launch_bounds__(128) void k_nys_basis(DevMesh m, NysBasis b, z64* __restrict__ g0,
+                                                   z64* __restrict__ g1) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = m.nt;
+    if (j >= nt) return;
+    const double x[3] = {m.cen[j], m.cen[nt + j], m.cen[2 * nt + j]};
+    const double n[3] = {m.nrm[j], m.nrm[nt + j], m.nrm[2 * nt + j]};
+    basis_point(b, x, n, g0 + j, g1 + j, (size_t)nt);
 }
 
 void launch_nys_basis(hipStream_t s, const DevMesh& m, const NysBasis& b, z64* g0, z64* g1) {
@@ -345,7 +352,7 @@ constexpr int kJacobiSweeps = 40;
 __global__ __launch_bounds__(64) void k_nys_weights(int nt, int nq, int P, const z64* __restrict__ g0,
                                                     const z64* __restrict__ g1, const z64* __restrict__ th_sd,
                                                     const z64* __restrict__ th_kh, z64* __restrict__ corr_sd,
-                                                    z64* __restrict__ corr_kh) {
+                                                    z64* __restrict__ corr_kh, z64* __restrict__ pinv) {
     extern __shared__ double lds[];
     const int e = blockIdx.x;
     const int lane = threadIdx.x;
@@ -425,13 +432,15 @@ __global__ __launch_bounds__(64) void k_nys_weights(int nt, int nq, int P, const
     }
     __syncthreads();
     const double cut = 1e-26 * smax;  // singular values above 1e-13 of the largest
-    for (int f = 0; f < 2; ++f) {
+    // the fits: of each target's defect (f = 0, 1), or of the unit defects
+    // e_p, which give the pseudo-inverse's columns (f = 2)
+    for (int f = 0; f < 3; ++f) {
         const z64* thf = f == 0 ? th_sd : th_kh;
         z64* out = f == 0 ? corr_sd : corr_kh;
-        if (!thf) continue;
-        for (int a = 0; a < nq; ++a) {
+        if (f < 2 ? !thf : !pinv) continue;
+        for (int a = 0; a < (f < 2 ? nq : P); ++a) {
             const int i = base + a;
-            if (lane < P) th[lane] = thf[(size_t)lane * nt + i];
+            if (lane < P) th[lane] = f < 2 ? thf[(size_t)lane * nt + i] : z64{lane == a ? 1.0 : 0.0, 0.0};
             __syncthreads();
             // coef_c = (u_c^H th) / sigma_c^2, u_c the c-th column of B (tall)
             // or of Q (wide), both of length P
@@ -451,7 +460,9 @@ __global__ __launch_bounds__(64) void k_nys_weights(int nt, int nq, int P, const
                 wv[lane] = zsc(w, 1.0 / scale[lane]);
             }
             __syncthreads();
-            if (lane < nq) {
+            if (f == 2) {
+                if (lane < C) pinv[((size_t)blockIdx.x * C + lane) * P + a] = wv[lane];
+            } else if (lane < nq) {
                 const z64 d1 = wv[nq + lane], d2 = wv[lane];
                 out[2 * ((size_t)i * nq + lane)] = z64{-d1.x, -d1.y};
                 out[2 * ((size_t)i * nq + lane) + 1] = d2;
@@ -467,7 +478,179 @@ void launch_nys_weights(hipStream_t s, const DevMesh& m, int P, const z64* g0, c
     const int mr = std::max(P, C), nc = std::min(P, C);
     const size_t bytes = sizeof(z64) * ((size_t)mr * nc + (size_t)nc * nc + P + nc + C) + sizeof(double) * (C + nc);
     hipLaunchKernelGGL(k_nys_weights, dim3(m.nt / nq), dim3(64), bytes, s, m.nt, nq, P, g0, g1, th_sd, th_kh,
-                       corr_sd, corr_kh);
+                       corr_sd, corr_kh, (z64*)nullptr);
+}
+
+void launch_nys_pinv(hipStream_t s, const DevMesh& m, int P, const z64* g0, const z64* g1, z64* pinv) {
+    const int nq = m.nq, C = 2 * nq;
+    const int mr = std::max(P, C), nc = std::min(P, C);
+    const size_t bytes = sizeof(z64) * ((size_t)mr * nc + (size_t)nc * nc + P + nc + C) + sizeof(double) * (C + nc);
+    hipLaunchKernelGGL(k_nys_weights, dim3(m.nt / nq), dim3(64), bytes, s, m.nt, nq, P, g0, g1, (const z64*)nullptr,
+                       (const z64*)nullptr, (z64*)nullptr, (z64*)nullptr, pinv);
+}
+
+// ---------------------------------------------------------------------------
+// pressure at points (src/fields/pressure.jl:441-462)
+// ---------------------------------------------------------------------------
+// Winding numbers over the flat sub-triangles and the reference's region
+// rule (pressure.jl:496-501), one wave per point, as the panel path.
+__global__ __launch_bounds__(256) void k_nys_winding(DevMesh m, FieldParams fp, int npts,
+                                                     const double* __restrict__ pts, int* __restrict__ region,
+                                                     double* __restrict__ winding) {
+    const int lane = threadIdx.x & 63;
+    const int pt = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pt >= npts) return;
+    const double x0 = pts[3 * pt], x1 = pts[3 * pt + 1], x2 = pts[3 * pt + 2];
+    double om = 0.0;
+    for (int j = lane; j < m.nsub; j += 64) om += tri_solid_angle(m.vert + 9 * (size_t)j, x0, x1, x2);
+    om = __shfl(wave_sum(om), 0, 64);
+    const double w = om * 0.07957747154594767;  // / (4 pi)
+    const bool outside_or_on = w < 0.75, inside_or_on = w > 0.25;
+    int reg = REGION_NONE;
+    if (fp.field == FIELD_INCIDENT) reg = REGION_EXTERIOR;
+    else if ((fp.field == FIELD_TOTAL || fp.field == FIELD_SCATTERED) && outside_or_on) reg = REGION_EXTERIOR;
+    else if (fp.fluid && (fp.field == FIELD_TOTAL || fp.field == FIELD_INTERIOR) && inside_or_on)
+        reg = REGION_INTERIOR;
+    if (lane == 0) {
+        region[pt] = reg;
+        winding[pt] = w;
+    }
+}
+
+void launch_nys_winding(hipStream_t s, const DevMesh& m, const FieldParams& fp, int npts, const double* pts,
+                        int* region, double* winding) {
+    hipLaunchKernelGGL(k_nys_winding, dim3((npts + 3) / 4), dim3(256), 0, s, m, fp, npts, pts, region, winding);
+}
+
+// One wave per point: the nearest node (lowest index on ties); the layer
+// potentials over every node outside its element, accumulating beside them
+// the identity's defect on the basis (kPtPB functions per pass); then the
+// element's fit from its pseudo-inverse and the corrected contribution of the
+// element's own nodes.
+constexpr int kPtPB = 16;
+
+__device__ inline void node_densities(const DevMesh& m, const NysPointEval& ev, const z64* p, const z64* q, int j,
+                                      z64& u, z64& v) {
+    const int nt = m.nt;
+    u = q[j];
+    v = p[j];
+    if (ev.inside) {
+        const double ph = ev.k0 * (ev.d[0] * m.cen[j] + ev.d[1] * m.cen[nt + j] + ev.d[2] * m.cen[2 * nt + j]);
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        const double dn = ev.k0 * (ev.d[0] * m.nrm[j] + ev.d[1] * m.nrm[nt + j] + ev.d[2] * m.nrm[2 * nt + j]);
+        v = z64{v.x + cs, v.y + sn};                               // p + p_inc
+        u = z64{ev.g * (u.x - dn * sn), ev.g * (u.y + dn * cs)};   // g (q + q_inc)
+    }
+}
+
+__global__ __launch_bounds__(64) void k_nys_points(DevMesh m, NysPointEval ev, int n, const int* __restrict__ idx,
+                                                   const double* __restrict__ pts, const z64* __restrict__ p,
+                                                   const z64* __restrict__ q, z64* __restrict__ out) {
+    __shared__ z64 th[64], at0[64], at1[64], wv[64];
+    const int lane = threadIdx.x;
+    const int pt = idx[blockIdx.x];
+    const int nt = m.nt, nq = m.nq, P = ev.basis.P, C = 2 * nq;
+    const double x[3] = {pts[3 * pt], pts[3 * pt + 1], pts[3 * pt + 2]};
+    // nearest node
+    double best = INFINITY;
+    int bj = nt;
+    for (int j = lane; j < nt; j += 64) {
+        const double r0 = x[0] - m.cen[j], r1 = x[1] - m.cen[nt + j], r2 = x[2] - m.cen[2 * nt + j];
+        const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+        if (d2 < best) { best = d2; bj = j; }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double ob = __shfl_xor(best, off, 64);
+        const int oj = __shfl_xor(bj, off, 64);
+        if (ob < best || (ob == best && oj < bj)) { best = ob; bj = oj; }
+    }
+    const bool corr = sqrt(best) <= ev.maxdist;
+    const int e0 = corr ? (bj / nq) * nq : -1;  // first node of the excluded element
+    const double zero3[3] = {0.0, 0.0, 0.0};
+    if (corr && lane == 0) basis_point(ev.basis, x, zero3, at0, at1, 1);
+    double sx = 0, sy = 0, dx = 0, dy = 0;
+    const int passes = corr ? (P + kPtPB - 1) / kPtPB : 1;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int p0 = pass * kPtPB, pb = corr ? min(kPtPB, P - p0) : 0;
+        z64 acc[kPtPB];
+#pragma unroll
+        for (int c = 0; c < kPtPB; ++c) acc[c] = z64{0, 0};
+        for (int j = lane; j < nt; j += 64) {
+            if (corr && j >= e0 && j < e0 + nq) continue;  // the excluded element
+            const double r0 = x[0] - m.cen[j], r1 = x[1] - m.cen[nt + j], r2 = x[2] - m.cen[2 * nt + j];
+            Acc4<double> a{};
+            kpoint<double, 3>(ev.k, r0, r1, r2, 0.0, 0.0, 0.0, m.nrm[j], m.nrm[nt + j], m.nrm[2 * nt + j], 0.0,
+                              m.area[j], a);
+            if (pass == 0) {
+                z64 u, v;
+                node_densities(m, ev, p, q, j, u, v);
+                sx += a.S.x * u.x - a.S.y * u.y;
+                sy += a.S.x * u.y + a.S.y * u.x;
+                dx += a.D.x * v.x - a.D.y * v.y;
+                dy += a.D.x * v.y + a.D.y * v.x;
+            }
+#pragma unroll
+            for (int c = 0; c < kPtPB; ++c) {
+                if (c < pb) {
+                    const z64 b0 = ev.g0[(size_t)(p0 + c) * nt + j], b1 = ev.g1[(size_t)(p0 + c) * nt + j];
+                    acc[c].x += a.S.x * b1.x - a.S.y * b1.y - (a.D.x * b0.x - a.D.y * b0.y);
+                    acc[c].y += a.S.x * b1.y + a.S.y * b1.x - (a.D.x * b0.y + a.D.y * b0.x);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kPtPB; ++c) {
+            if (c < pb) {
+                const double rx = wave_sum(acc[c].x), ry = wave_sum(acc[c].y);
+                if (lane == 0) th[p0 + c] = z64{rx, ry};
+            }
+        }
+    }
+    sx = wave_sum(sx); sy = wave_sum(sy);
+    dx = wave_sum(dx); dy = wave_sum(dy);
+    z64 val = ev.inside ? z64{sx - dx, sy - dy} : z64{dx - sx, dy - sy};
+    if (corr) {
+        __syncthreads();
+        // the identity's multiplier: -1 inside, 0 outside
+        if (lane < P && ev.inside) th[lane] = z64{th[lane].x - at0[lane].x, th[lane].y - at0[lane].y};
+        __syncthreads();
+        // w = pinv th, lanes over the element's 2 nq trace rows
+        if (lane < C) {
+            const z64* pe = ev.pinv + ((size_t)(e0 / nq) * C + lane) * P;
+            z64 w{0, 0};
+            for (int c = 0; c < P; ++c) w = zadd(w, zmul(pe[c], th[c]));
+            wv[lane] = w;
+        }
+        __syncthreads();
+        // dS_b = -w[nq + b], dD_b = w[b]: D v - S u gains w_b v_b + w_{nq+b} u_b
+        double cx = 0, cy = 0;
+        if (lane < nq) {
+            z64 u, v;
+            node_densities(m, ev, p, q, e0 + lane, u, v);
+            const z64 t = zadd(zmul(wv[lane], v), zmul(wv[nq + lane], u));
+            cx = t.x;
+            cy = t.y;
+        }
+        cx = wave_sum(cx);
+        cy = wave_sum(cy);
+        val = ev.inside ? z64{val.x - cx, val.y - cy} : z64{val.x + cx, val.y + cy};
+    }
+    if (lane == 0) {
+        if (ev.add_incident) {
+            double sn, cs;
+            sincos(ev.k0 * (ev.d[0] * x[0] + ev.d[1] * x[1] + ev.d[2] * x[2]), &sn, &cs);
+            val.x += cs;
+            val.y += sn;
+        }
+        out[pt] = val;
+    }
+}
+
+void launch_nys_points(hipStream_t s, const DevMesh& m, const NysPointEval& ev, int n, const int* idx,
+                       const double* pts, const z64* p, const z64* q, z64* out) {
+    if (n > 0) hipLaunchKernelGGL(k_nys_points, dim3(n), dim3(64), 0, s, m, ev, n, idx, pts, p, q, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -232,8 +232,11 @@ int bem_mesh_near_count(bem_mesh mesh, int64_t *count);
  * quadrature nodes (count, coordinates, normals, weights), which are the
  * reference's `quad` (full.jl:509-519).  The topology checks add the
  * reference's curved ones (repeated nodes, conforming edge nodes,
- * surface.jl:406-419).  Compression and near-field pressure need panel
- * meshes and return BEM_ERR_ARGUMENT on these. */
+ * surface.jl:406-419).  bem_pressure on their solutions follows the
+ * reference's Nystrom field evaluation (the nearest element excluded and
+ * recovered by density interpolation, src/fields/pressure.jl:441-462, with
+ * regions by the winding number of flat triangles through the element
+ * nodes).  Compression needs panel meshes and returns BEM_ERR_ARGUMENT. */
 int bem_mesh_create_nystrom(bem_ctx ctx, int64_t n_nodes, const double *nodes, int64_t n_tri,
                             int32_t nodes_per_tri, const int64_t *tri, int32_t qorder, bem_mesh *out);
 /* Host only: the same validation and quadrature without a device.  With

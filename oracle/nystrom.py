@@ -487,3 +487,124 @@ def curved_icosphere(level, axes=(1.0, 1.0, 1.0), order=2):
     for c in range(3):
         out[c] *= axes[c]
     return out, np.asarray(elems, dtype=np.int64).T.copy()
+
+
+# ---------------------------------------------------------------------------
+# near-field pressure at points (src/fields/pressure.jl:404-472)
+# ---------------------------------------------------------------------------
+def sub_triangles(nodes, tri0):
+    """Flat triangles through every node of each curved element (1, 4 or 9
+    per element), outward like the element: the surface the winding number
+    is taken over.  Returns (n_sub, 3, 3) vertex coordinates."""
+    nodes = np.asarray(nodes, dtype=np.float64).T
+    npt = tri0.shape[0]
+    if npt == 3:
+        local = [(0, 1, 2)]
+    elif npt == 6:
+        local = [(0, 3, 5), (3, 1, 4), (5, 4, 2), (3, 4, 5)]
+    else:
+        # lattice (i, j) of the cubic element: vertices, edge nodes, interior
+        L = {(0, 0): 0, (3, 0): 1, (0, 3): 2, (1, 0): 3, (2, 0): 4, (2, 1): 5, (1, 2): 6,
+             (0, 2): 7, (0, 1): 8, (1, 1): 9}
+        local = []
+        for j in range(3):
+            for i in range(3 - j):
+                local.append((L[(i, j)], L[(i + 1, j)], L[(i, j + 1)]))
+                if i + j < 2:
+                    local.append((L[(i + 1, j)], L[(i + 1, j + 1)], L[(i, j + 1)]))
+    out = []
+    for e in range(tri0.shape[1]):
+        for (a, b, c) in local:
+            out.append((nodes[tri0[a, e]], nodes[tri0[b, e]], nodes[tri0[c, e]]))
+    return np.array(out)
+
+
+def winding(subs, pts):
+    """(1/4 pi) sum of the flat triangles' solid angles about each point
+    (Van Oosterom and Strackee 1983)."""
+    w = np.zeros(len(pts))
+    for i, x in enumerate(pts):
+        a, b, c = subs[:, 0] - x, subs[:, 1] - x, subs[:, 2] - x
+        la, lb, lc = (np.linalg.norm(v, axis=1) for v in (a, b, c))
+        num = np.einsum("ij,ij->i", a, np.cross(b, c))
+        den = la * lb * lc + np.einsum("ij,ij->i", a, b) * lc + np.einsum("ij,ij->i", a, c) * lb \
+            + np.einsum("ij,ij->i", b, c) * la
+        w[i] = np.sum(2 * np.arctan2(num, den)) / (4 * math.pi)
+    return w
+
+
+def layer_values(quad, k, pts, inside, u, v, regular_ok):
+    """S u - D v (inside) or D v - S u (outside) at off-surface points with
+    the nearest element excluded and recovered by density interpolation
+    (exclude_nearest, target_location, maxdist = 2 radius;
+    src/fields/pressure.jl:441-462, src/solvers/bem/quadrature/fluid.jl:78-163).
+    Points with k |x - c| <= 2 use the regular basis when `regular_ok`."""
+    nq = quad.nq
+    center, radius, _ = quadrature_size(quad)
+    maxdist = 2 * radius
+    sigma = -1.0 if inside else 0.0
+    zs = source_points(quad)
+    out = np.zeros(len(pts), dtype=complex)
+    for i, x in enumerate(pts):
+        r = x[None, :] - quad.coords
+        R = np.linalg.norm(r, axis=1)
+        G = np.exp(1j * k * R) / (4 * math.pi * R) * quad.weights
+        Dk = G * (1.0 / R - 1j * k) * np.einsum("jd,jd->j", r, quad.normals) / R
+        near = int(np.argmin(R))
+        e = near // nq
+        cols = slice(e * nq, (e + 1) * nq)
+        corrected = R[near] <= maxdist
+        if corrected:
+            G = G.copy(); Dk = Dk.copy()
+            G[cols] = 0
+            Dk[cols] = 0
+        val = (G @ u - Dk @ v) if inside else (Dk @ v - G @ u)
+        if corrected:
+            use_regular = regular_ok and k * np.linalg.norm(x - center) <= 2
+            if use_regular:
+                pres, flux = regular_traces(quad.coords, quad.normals, center, radius, k, regular_degree(nq))
+                at, _ = regular_traces(x[None, :], np.zeros((1, 3)), center, radius, k, regular_degree(nq))
+            else:
+                pres, flux = point_traces(quad.coords, quad.normals, zs, k)
+                at, _ = point_traces(x[None, :], np.zeros((1, 3)), zs, k)
+            defect = sigma * at[0] + G @ flux - Dk @ pres
+            traces = np.vstack([pres[cols], flux[cols]])
+            scales = np.maximum(np.sqrt(np.sum(np.abs(traces) ** 2, axis=1)), np.finfo(np.float64).eps)
+            U, s, Vh = np.linalg.svd((traces / scales[:, None]).T, full_matrices=False)
+            keep = s > 1e-13 * s[0]
+            w = (Vh[keep].conj().T @ ((U[:, keep].conj().T @ defect) / s[keep])) / scales
+            dS, dD = -w[nq:], w[:nq]
+            corr = (dS @ u[cols] - dD @ v[cols]) if inside else (dD @ v[cols] - dS @ u[cols])
+            val = val + corr
+        out[i] = val
+    return out
+
+
+def pressure(quad, nodes, tri0, k, pts, p, q, beta=0.0, alpha=0.0, field="total", fluid=None):
+    """pressure(solution, points; field) of a Nystrom solution: (values,
+    regions 0 exterior / 1 interior / 2 none, winding numbers)."""
+    pts = np.atleast_2d(np.asarray(pts, dtype=np.float64))
+    d = O.direction(beta, alpha)
+    inc = np.exp(1j * k * (pts @ d))
+    if field == "incident":
+        return inc, np.zeros(len(pts), dtype=int), np.zeros(len(pts))
+    w = winding(sub_triangles(nodes, tri0), pts)
+    reg = np.full(len(pts), 2)
+    out = np.full(len(pts), np.nan + 0j)
+    for i, wi in enumerate(w):
+        if field in ("total", "scattered") and wi < 0.75:
+            reg[i] = 0
+        elif fluid is not None and field in ("total", "interior") and wi > 0.25:
+            reg[i] = 1
+    ext = np.where(reg == 0)[0]
+    if len(ext):
+        out[ext] = layer_values(quad, k, pts[ext], False, q, p, regular_range(k, quad))
+        if field == "total":
+            out[ext] += inc[ext]
+    inn = np.where(reg == 1)[0]
+    if len(inn):
+        g, h = fluid
+        ki = k / h
+        pinc, qinc = quad.incident(k, d)
+        out[inn] = layer_values(quad, ki, pts[inn], True, g * (q + qinc), p + pinc, regular_range(ki, quad))
+    return out, reg, w

@@ -201,6 +201,41 @@ def test_nystrom_mesh_reports_quadrature_and_rejects_panel_only_paths(ctx):
     assert np.array_equal(xyz, x0) and np.array_equal(w, w0)
     with pytest.raises(ValueError, match="flat-panel mesh"):
         bh.HMatrix(gm, bh.Rigid(), 2.0)
-    sol = bh.bem(gm, bh.Rigid(), 2.0)
-    with pytest.raises(ValueError, match="flat-panel mesh"):
-        bh.pressure(sol, np.array([3.0, 0, 0]))
+
+
+def _dirs(n, seed):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    return d / np.linalg.norm(d, axis=1)[:, None]
+
+
+@pytest.mark.parametrize("case", ["rigid38", "fluid38", "gas_regular"])
+def test_pressure_matches_oracle(ctx, case):
+    """Near-field pressure of Nystrom solutions (src/fields/pressure.jl:441-462):
+    the device field from the device traces equals the oracle's field from
+    the same traces to 1e-9, outside from 1.01 radii and inside a fluid body;
+    the low-frequency case mixes regular-wave (k|x| <= 2) and point-source
+    corrections."""
+    say(f"[gpu] nystrom pressure {case}")
+    if case == "gas_regular":
+        a, k, bc = 1.0, 0.3, bh.FluidFilled(0.5, 0.7)
+    else:
+        a, k = 0.01, K38
+        bc = bh.Rigid() if case == "rigid38" else bh.FluidFilled(*FLUID38)
+    gm, om = _pair(ctx, 2, (a, a, a))
+    sol = bh.bem(gm, bc, k, incidence_angle=0.0)
+    p, q = sol.p_scat, sol.dpdn_scat
+    fluid = (bc.density_contrast, bc.soundspeed_contrast) if isinstance(bc, bh.FluidFilled) else None
+    radii = np.array([1.01, 1.05, 1.2, 2.0, 3.0, 6.0, 9.0, 20.0]) * a
+    pts = _dirs(8, 3) * radii[:, None]
+    fields = ["scattered", "total"] + (["interior"] if fluid else [])
+    nodes = gm.nodes
+    for field in fields:
+        x = pts if field != "interior" else _dirs(5, 4) * (np.array([0.2, 0.5, 0.8, 0.95, 0.99]) * a)[:, None]
+        v = bh.pressure(sol, x.T, field=field)
+        ref, reg, _ = NY.pressure(om, nodes, gm.triangles - 1, k, x, p, q, field=field, fluid=fluid)
+        assert np.all(reg == (1 if field == "interior" else 0))
+        assert np.max(np.abs(v - ref)) < 1e-9 * np.max(np.abs(ref)), field
+    if fluid is None:
+        with pytest.raises(ValueError, match="no acoustic pressure"):
+            bh.pressure(sol, np.array([0.0, 0.0, 0.0]))

@@ -271,3 +271,81 @@ def test_reference_spheroid_rigid_golden():
         f = Q.far_field(2.0, np.array(d), p, q)[0]
         assert abs(20 * math.log10(abs(f)) - 20 * math.log10(abs(r))) < 0.1
         assert abs(f - r) / abs(r) < 0.01
+
+
+def _dirs(n, seed):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    return d / np.linalg.norm(d, axis=1)[:, None]
+
+
+@pytest.mark.parametrize("bc", ["rigid", "soft", "fluid"])
+def test_nystrom_pressure_matches_modal_near_field(bc):
+    """The 38 kHz sphere at 1920 nodes: scattered field from 1.01a (a tenth
+    of an element from the surface, where the excluded and interpolated
+    nearest element carries the singular part) to 3a, and the fluid
+    sphere's interior field, against the exact series to 5e-4 of the field's
+    scale (src/fields/pressure.jl:441-462)."""
+    a = 0.01
+    nodes, tri = NY.curved_icosphere(2, (a, a, a), 2)
+    Q = NY.Quadrature(nodes, tri, 4)
+    fl = (1028.9 / 1026.8, 1480.3 / 1477.4)
+    if bc == "fluid":
+        p, q, _ = NY.solve_fluid(Q, *fl, K38, beta=0.0)
+        mbc, fluid = ("fluid",) + fl, fl
+    else:
+        p, q, _ = NY.solve_boundary(Q, bc, K38, beta=0.0)
+        mbc, fluid = (bc,), None
+    dirs = _dirs(8, 1)
+    radii = np.array([1.01, 1.02, 1.05, 1.1, 1.2, 1.5, 2.0, 3.0]) * a
+    pts = dirs * radii[:, None]
+    v, reg, w = NY.pressure(Q, nodes, tri, K38, pts, p, q, field="scattered", fluid=fluid)
+    assert np.all(reg == 0) and np.all(np.abs(w) < 1e-9)
+    ref = np.array([modal.near_field(mbc, K38, a, r, d[0]) for r, d in zip(radii, dirs)])
+    assert np.max(np.abs(v - ref)) < 5e-4 * np.max(np.abs(ref))
+    tot, _, _ = NY.pressure(Q, nodes, tri, K38, pts, p, q, field="total", fluid=fluid)
+    assert np.allclose(tot - v, np.exp(1j * K38 * pts[:, 0]), atol=1e-12)
+    if bc == "fluid":
+        ri = np.array([0.3, 0.6, 0.9, 0.98]) * a
+        pin = _dirs(4, 5) * ri[:, None]
+        vi, regi, wi = NY.pressure(Q, nodes, tri, K38, pin, p, q, field="interior", fluid=fluid)
+        assert np.all(regi == 1) and np.all(np.abs(wi - 1) < 1e-9)
+        refi = np.array([modal.near_field(mbc, K38, a, r, d[0]) for r, d in zip(ri, _dirs(4, 5))])
+        assert np.max(np.abs(vi - refi)) < 5e-4 * np.max(np.abs(refi))
+    else:
+        _, regi, _ = NY.pressure(Q, nodes, tri, K38, np.array([[0.0, 0, 0]]), p, q, field="total")
+        assert regi[0] == 2
+
+
+def test_nystrom_pressure_regular_regime_and_far_limit():
+    """A gas-filled unit sphere at k = 0.3 (both media in the regular-wave
+    range): points with k|x| <= 2 use the regular basis, farther ones the
+    point sources; the series agrees, and far away the field tends to the
+    far-field amplitude times e^{ikr}/r."""
+    nodes, tri = NY.curved_icosphere(2, (1.0, 1.0, 1.0), 2)
+    Q = NY.Quadrature(nodes, tri, 4)
+    g, h, k = 0.5, 0.7, 0.3
+    assert NY.regular_range(k / h, Q)
+    p, q, _ = NY.solve_fluid(Q, g, h, k, beta=0.0)
+    dirs = _dirs(6, 2)
+    radii = np.array([1.02, 1.3, 3.0, 6.0, 9.0, 20.0])
+    pts = dirs * radii[:, None]
+    v, _, _ = NY.pressure(Q, nodes, tri, k, pts, p, q, field="scattered", fluid=(g, h))
+    ref = np.array([modal.near_field(("fluid", g, h), k, 1.0, r, d[0]) for r, d in zip(radii, dirs)])
+    assert np.max(np.abs(v - ref) / np.abs(ref)) < 2e-3
+    far = 2e6
+    f = Q.far_field(k, dirs[:1], p, q)[0]
+    vf, _, _ = NY.pressure(Q, nodes, tri, k, dirs[:1] * far, p, q, field="scattered", fluid=(g, h))
+    assert abs(vf[0] * far * np.exp(-1j * k * far) - f) < 1e-5 * abs(f)
+
+
+def test_nystrom_pressure_matched_medium_and_winding():
+    nodes, tri = NY.curved_icosphere(1, (1.0, 0.8, 0.9), 3)
+    Q = NY.Quadrature(nodes, tri, 4)
+    p, q, _ = NY.solve_fluid(Q, 1.0, 1.0, 2.0, beta=0.4)
+    pts = np.array([[1.5, 0, 0], [0, 0.85, 0], [0.0, 0.0, 0.0], [0.5, 0.2, 0.1]])
+    v, reg, w = NY.pressure(Q, nodes, tri, 2.0, pts, p, q, beta=0.4, field="total", fluid=(1.0, 1.0))
+    assert list(reg) == [0, 0, 1, 1]
+    assert np.allclose(w, [0, 0, 1, 1], atol=1e-9)
+    inc = np.exp(2j * (pts @ O.direction(0.4)))
+    assert np.allclose(v, inc, atol=1e-9)
