@@ -799,6 +799,44 @@ size_t nys_bytes(const bem_mesh m) {
     return sizeof(z64) * (4 * (size_t)P * nt + nys_defect_parts((int)nt, P) + 8 * nt * m->h.nq) + 8 * 256;
 }
 
+// Workspace of the density-interpolation corrections of one mesh: basis
+// traces, defects of both operator families and their per-slice partial sums.
+struct NysWork {
+    int P = 0;
+    z64 *g0 = nullptr, *g1 = nullptr, *th_sd = nullptr, *th_kh = nullptr, *part = nullptr;
+};
+template <typename A>
+NysWork nys_work(A&& alloc, const bem_mesh m, bool regular, bool derivative) {
+    NysWork w;
+    const size_t nt = (size_t)m->d.nt;
+    w.P = nys_basis(m, 1.0, regular).P;
+    w.g0 = (z64*)alloc(sizeof(z64) * w.P * nt);
+    w.g1 = (z64*)alloc(sizeof(z64) * w.P * nt);
+    w.th_sd = (z64*)alloc(sizeof(z64) * w.P * nt);
+    if (derivative) w.th_kh = (z64*)alloc(sizeof(z64) * w.P * nt);
+    w.part = (z64*)alloc(sizeof(z64) * nys_defect_parts((int)nt, w.P));
+    return w;
+}
+// The corrections of S and D (and of K' and H when `derivative`) at
+// wavenumber k: corr_sd / corr_kh in the launch_nys_weights layout.
+void nys_medium(hipStream_t s, ProfLane* pl, const bem_mesh m, const NysWork& w, double k, bool regular,
+                bool derivative, z64* corr_sd, z64* corr_kh) {
+    const DevMesh& d = m->d;
+    const double pairs = (double)d.nt * d.nt;
+    const NysBasis b = nys_basis(m, k, regular);
+    prof_begin(pl, s, C_NYS_FIT);
+    launch_nys_basis(s, d, b, w.g0, w.g1);
+    prof_end(pl, s, C_NYS_FIT, 0.0, 0.0);
+    // algorithmic flops of the defects: two complex multiply-adds (8 flops
+    // each) per pair, basis function and operator family
+    prof_begin(pl, s, C_NYS_DEF);
+    launch_nys_defect(s, d, b.k, w.P, derivative ? 3 : 1, w.g0, w.g1, w.part, w.th_sd, w.th_kh);
+    prof_end(pl, s, C_NYS_DEF, pairs * (derivative ? 2.0 : 1.0) * 16.0 * w.P, 0.0);
+    prof_begin(pl, s, C_NYS_FIT);
+    launch_nys_weights(s, d, w.P, w.g0, w.g1, w.th_sd, derivative ? w.th_kh : nullptr, corr_sd, corr_kh);
+    prof_end(pl, s, C_NYS_FIT, 0.0, 0.0);
+}
+
 // FP64 operations of one point-kernel evaluation (kpoint, bem_kernels.h),
 // an FMA counted as two and sincos and rsqrt as one each: separation and
 // R^2 (8), 1/R, R, kR and sincos (4), the Green's function (6), D and K'
@@ -826,29 +864,13 @@ void nys_assemble(bem_operator op, const AsmParams& ap, C2<R>* o0, C2<R>* o1, bo
     launch_nys_assemble<R>(s, d, ap, o0, o1);
     prof_end(&op->lane->pl, s, C_NYS_ASM, pairs * media * (all4 ? kNysEvalFlops4 : kNysEvalFlops2),
              pairs * sizeof(C2<R>) * (two ? 4.0 : 2.0));
-    const int P = nys_basis(m, ap.k, regular).P;
-    z64* g0 = (z64*)op_alloc(op, sizeof(z64) * P * (size_t)nt);
-    z64* g1 = (z64*)op_alloc(op, sizeof(z64) * P * (size_t)nt);
-    z64* th_sd = (z64*)op_alloc(op, sizeof(z64) * P * (size_t)nt);
-    z64* th_kh = derivative ? (z64*)op_alloc(op, sizeof(z64) * P * (size_t)nt) : nullptr;
-    z64* part = (z64*)op_alloc(op, sizeof(z64) * nys_defect_parts(nt, P));
+    const NysWork w = nys_work([&](size_t b) { return op_alloc(op, b); }, m, regular, derivative);
     z64* corr[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    const int fams = derivative ? 2 : 1;
     for (int med = 0; med < media; ++med) {
-        const NysBasis b = nys_basis(m, med == 0 ? ap.k : ap.kint, regular);
-        prof_begin(&op->lane->pl, s, C_NYS_FIT);
-        launch_nys_basis(s, d, b, g0, g1);
-        prof_end(&op->lane->pl, s, C_NYS_FIT, 0.0, 0.0);
-        // algorithmic flops of the defects: two complex multiply-adds (8 flops
-        // each) per pair, basis function and operator family
-        prof_begin(&op->lane->pl, s, C_NYS_DEF);
-        launch_nys_defect(s, d, b.k, P, derivative ? 3 : 1, g0, g1, part, th_sd, th_kh);
-        prof_end(&op->lane->pl, s, C_NYS_DEF, pairs * fams * 16.0 * P, 0.0);
         corr[med][0] = (z64*)op_alloc(op, sizeof(z64) * 2 * (size_t)nt * d.nq);
         if (derivative) corr[med][1] = (z64*)op_alloc(op, sizeof(z64) * 2 * (size_t)nt * d.nq);
-        prof_begin(&op->lane->pl, s, C_NYS_FIT);
-        launch_nys_weights(s, d, P, g0, g1, th_sd, th_kh, corr[med][0], corr[med][1]);
-        prof_end(&op->lane->pl, s, C_NYS_FIT, 0.0, 0.0);
+        nys_medium(s, &op->lane->pl, m, w, med == 0 ? ap.k : ap.kint, regular, derivative, corr[med][0],
+                   corr[med][1]);
     }
     prof_begin(&op->lane->pl, s, C_NYS_FIT);
     launch_nys_apply<R>(s, d, ap, corr[0][0], corr[0][1], corr[1][0], corr[1][1], o0, o1);
@@ -1997,6 +2019,7 @@ struct bem_hmatrix_s {
     HOp cal[4];                       // S_e, (1/2 I - D_e), S_i, (1/2 I - D_i)
     z64 *minv_e = nullptr, *minv_i = nullptr;  // block Jacobi of S_e, S_i
     z64* lblk = nullptr;              // inverses of the local 2m x 2m Mueller blocks (nleaf x 128 x 128)
+    z64 *nys_sd = nullptr, *nys_kh = nullptr;  // Nystrom meshes: density-interpolation corrections
     int64_t nsys = 0;                 // unknowns of the system (n, or 2N on the panel tree)
     std::vector<int> sysperm;         // unknown of each system row (nsys)
     int* dsysperm = nullptr;
@@ -2060,11 +2083,27 @@ void hm_plan(bem_hmatrix h) {
     const double t0 = now_ms();
     std::vector<double> pts(hm.cen.begin(), hm.cen.end());
     if (nq == 2) pts.insert(pts.end(), hm.cen.begin(), hm.cen.end());
-    // near pairs never fall in low-rank blocks (box distance > 2 hmax)
-    double hmax = 0;
-    for (double v : hm.hmax) hmax = std::max(hmax, v);
+    // near pairs never fall in low-rank blocks (box distance > 2 hmax); on
+    // Nystrom meshes the pairs of nodes of one element (the corrected ones)
+    // never do (box distance > the largest element's node spread)
+    double near = 0;
+    if (hm.nq) {
+        for (int64_t e = 0; e < hm.nel; ++e)
+            for (int a = 0; a < hm.nq; ++a)
+                for (int b = a + 1; b < hm.nq; ++b) {
+                    const double* x = &hm.cen[3 * (e * hm.nq + a)];
+                    const double* y = &hm.cen[3 * (e * hm.nq + b)];
+                    near = std::max(near, std::sqrt((x[0] - y[0]) * (x[0] - y[0]) + (x[1] - y[1]) * (x[1] - y[1]) +
+                                                    (x[2] - y[2]) * (x[2] - y[2])));
+                }
+        near *= 1.0 + 1e-9;
+    } else {
+        double hmax = 0;
+        for (double v : hm.hmax) hmax = std::max(hmax, v);
+        near = kNearEta * hmax * (1.0 + 1e-9);
+    }
     HPlan P;
-    hmatrix_plan(pts.data(), n, o.leaf_size, o.eta, kNearEta * hmax * (1.0 + 1e-9), P);
+    hmatrix_plan(pts.data(), n, o.leaf_size, o.eta, near, P);
     std::vector<int> leaf_of_node(P.nodes.size(), -1);
     std::vector<std::pair<int64_t, int32_t>> lv;
     for (size_t id = 0; id < P.nodes.size(); ++id)
@@ -2100,7 +2139,15 @@ void hm_plan(bem_hmatrix h) {
         const HBlk& b = h->plan_blk[it->second];
         return b.off + (pI - b.row_lo) * b.n + (pJ - b.col_lo);
     };
-    for (int64_t I = 0; I < n; ++I) {
+    for (int64_t I = 0; hm.nq && I < n; ++I) {  // Nystrom: the node pairs of each element
+        const int64_t e0 = (I / hm.nq) * hm.nq;
+        for (int b = 0; b < hm.nq; ++b) {
+            nsi.push_back((int)I);
+            nsj.push_back((int)(e0 + b));
+            sdst.push_back(slot(I, e0 + b));
+        }
+    }
+    for (int64_t I = 0; !hm.nq && I < n; ++I) {
         const int64_t i = I % nt;
         for (int32_t e = hm.near_ptr[i]; e < hm.near_ptr[i + 1]; ++e)
             for (int q = 0; q < nq; ++q) {
@@ -2169,8 +2216,11 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
     op.dblk = hm_upload(h, op.blk);
     op.dval = (z32*)hm_alloc(h, sizeof(z32) * std::max<int64_t>(1, h->dense_count));
     launch_hdense(s, dm, kind, h->par, h->perm, nd, op.dblk, op.dval);
-    launch_hnear_self(s, dm, kind, h->par, h->nnear, h->npi, h->npj, h->ndst, h->nself, h->nsi, h->nsj, h->sdst,
-                      op.dval);
+    if (dm.nq)
+        launch_hnys_self(s, dm, kind, h->par, h->nself, h->nsi, h->nsj, h->sdst, h->nys_sd, h->nys_kh, op.dval);
+    else
+        launch_hnear_self(s, dm, kind, h->par, h->nnear, h->npi, h->npj, h->ndst, h->nself, h->nsi, h->nsj,
+                          h->sdst, op.dval);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s));
     const double t2 = now_ms();
@@ -2354,6 +2404,16 @@ void hm_build(bem_hmatrix h) {
     h->nsys = h->n;
     h->sysperm = h->hperm;
     h->dsysperm = h->perm;
+    if (h->mesh->d.nq) {
+        // Nystrom meshes: the corrections of S and D (and K', H for
+        // Burton-Miller) at k, point-source basis, from the naive defects
+        const size_t nc = sizeof(z64) * 2 * (size_t)h->mesh->d.nt * h->mesh->d.nq;
+        const NysWork w = nys_work([&](size_t b) { return hm_alloc(h, b); }, h->mesh, false, bm);
+        h->nys_sd = (z64*)hm_alloc(h, nc);
+        if (bm) h->nys_kh = (z64*)hm_alloc(h, nc);
+        nys_medium(h->lane->s, &h->lane->pl, h->mesh, w, h->prob.k, false, bm, h->nys_sd, h->nys_kh);
+        HIPCHECK(hipGetLastError());
+    }
     if (h->fluid) {
         // Mueller: the right-hand side is the incident traces themselves
         hm_operator(h, h->A, HK_MULLER);
@@ -3954,6 +4014,12 @@ int bem_mesh_create_nystrom(bem_ctx ctx, int64_t nv, const double* nodes, int64_
             d.cen = (double*)up(cen.data(), sizeof(double) * cen.size());
             d.nrm = (double*)up(nrm.data(), sizeof(double) * nrm.size());
             d.area = (double*)up(h.area.data(), sizeof(double) * n);
+            // FP32 copies for the compressed path's regular entries
+            std::vector<float> cen32(cen.begin(), cen.end()), nrm32(nrm.begin(), nrm.end()),
+                area32(h.area.begin(), h.area.end());
+            d.cen32 = (float*)up(cen32.data(), sizeof(float) * cen32.size());
+            d.nrm32 = (float*)up(nrm32.data(), sizeof(float) * nrm32.size());
+            d.area32 = (float*)up(area32.data(), sizeof(float) * n);
             m->sources = (double*)up(h.sources.data(), sizeof(double) * h.sources.size());
             std::vector<double> subs;
             sub_triangles(h, subs);
@@ -4724,8 +4790,8 @@ int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, c
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     return guarded(ctx, [&] {
         validate_problem(*problem);
-        require_panels(mesh, "compression=(method=:hmatrix,)");
         const bool fluid = problem->boundary == BEM_FLUID;
+        if (fluid) require_panels(mesh, "compressed fluid BEM");
         if (problem->boundary != BEM_RIGID && !fluid)
             throw BemError(BEM_ERR_ARGUMENT, "the hierarchical path solves rigid surfaces and fluid bodies");
         if (fluid) {

@@ -142,6 +142,12 @@ void launch_hdense(hipStream_t s, const DevMesh& m, int kind, const HPar& p, con
                    const HBlk* blk, z32* val);
 // near pairs (unknown indices pi, pj) and pairs of unknowns on one panel (si,
 // sj) into dense-block slots
+// Nystrom meshes (rigid kinds): the pairs of nodes of one element, whose slots
+// take the naive entry (zero on the diagonal) plus the identity terms and the
+// density-interpolation corrections corr_sd / corr_kh (launch_nys_weights
+// layout; corr_kh may be null), in FP64.
+void launch_hnys_self(hipStream_t s, const DevMesh& m, int kind, const HPar& p, int64_t nself, const int* si,
+                      const int* sj, const int64_t* dst, const z64* corr_sd, const z64* corr_kh, z32* val);
 void launch_hnear_self(hipStream_t s, const DevMesh& m, int kind, const HPar& p, int64_t npairs, const int* pi,
                        const int* pj, const int64_t* dst, int64_t nself, const int* si, const int* sj,
                        const int64_t* self_dst, z32* val);
