@@ -2019,7 +2019,9 @@ struct bem_hmatrix_s {
     HOp cal[4];                       // S_e, (1/2 I - D_e), S_i, (1/2 I - D_i)
     z64 *minv_e = nullptr, *minv_i = nullptr;  // block Jacobi of S_e, S_i
     z64* lblk = nullptr;              // inverses of the local 2m x 2m Mueller blocks (nleaf x 128 x 128)
-    z64 *nys_sd = nullptr, *nys_kh = nullptr;  // Nystrom meshes: density-interpolation corrections
+    // Nystrom meshes: density-interpolation corrections of each medium (0
+    // exterior, 1 interior of fluid bodies)
+    z64 *nys_sd[2] = {nullptr, nullptr}, *nys_kh[2] = {nullptr, nullptr};
     int64_t nsys = 0;                 // unknowns of the system (n, or 2N on the panel tree)
     std::vector<int> sysperm;         // unknown of each system row (nsys)
     int* dsysperm = nullptr;
@@ -2139,13 +2141,15 @@ void hm_plan(bem_hmatrix h) {
         const HBlk& b = h->plan_blk[it->second];
         return b.off + (pI - b.row_lo) * b.n + (pJ - b.col_lo);
     };
-    for (int64_t I = 0; hm.nq && I < n; ++I) {  // Nystrom: the node pairs of each element
-        const int64_t e0 = (I / hm.nq) * hm.nq;
-        for (int b = 0; b < hm.nq; ++b) {
-            nsi.push_back((int)I);
-            nsj.push_back((int)(e0 + b));
-            sdst.push_back(slot(I, e0 + b));
-        }
+    for (int64_t I = 0; hm.nq && I < n; ++I) {  // Nystrom: the node pairs of each element (both quadrants)
+        const int64_t e0 = ((I % nt) / hm.nq) * hm.nq;
+        for (int q = 0; q < nq; ++q)
+            for (int b = 0; b < hm.nq; ++b) {
+                const int64_t J = e0 + b + q * nt;
+                nsi.push_back((int)I);
+                nsj.push_back((int)J);
+                sdst.push_back(slot(I, J));
+            }
     }
     for (int64_t I = 0; !hm.nq && I < n; ++I) {
         const int64_t i = I % nt;
@@ -2217,7 +2221,8 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
     op.dval = (z32*)hm_alloc(h, sizeof(z32) * std::max<int64_t>(1, h->dense_count));
     launch_hdense(s, dm, kind, h->par, h->perm, nd, op.dblk, op.dval);
     if (dm.nq)
-        launch_hnys_self(s, dm, kind, h->par, h->nself, h->nsi, h->nsj, h->sdst, h->nys_sd, h->nys_kh, op.dval);
+        launch_hnys_self(s, dm, kind, h->par, h->nself, h->nsi, h->nsj, h->sdst, h->nys_sd[0], h->nys_kh[0],
+                         h->nys_sd[1], h->nys_kh[1], op.dval);
     else
         launch_hnear_self(s, dm, kind, h->par, h->nnear, h->npi, h->npj, h->ndst, h->nself, h->nsi, h->nsj,
                           h->sdst, op.dval);
@@ -2406,12 +2411,18 @@ void hm_build(bem_hmatrix h) {
     h->dsysperm = h->perm;
     if (h->mesh->d.nq) {
         // Nystrom meshes: the corrections of S and D (and K', H for
-        // Burton-Miller) at k, point-source basis, from the naive defects
+        // Burton-Miller and Mueller) of each medium, point-source basis (the
+        // direct Mueller regime is above the regular-wave range), from the
+        // naive defects
+        const bool deriv = bm || h->fluid;
         const size_t nc = sizeof(z64) * 2 * (size_t)h->mesh->d.nt * h->mesh->d.nq;
-        const NysWork w = nys_work([&](size_t b) { return hm_alloc(h, b); }, h->mesh, false, bm);
-        h->nys_sd = (z64*)hm_alloc(h, nc);
-        if (bm) h->nys_kh = (z64*)hm_alloc(h, nc);
-        nys_medium(h->lane->s, &h->lane->pl, h->mesh, w, h->prob.k, false, bm, h->nys_sd, h->nys_kh);
+        const NysWork w = nys_work([&](size_t b) { return hm_alloc(h, b); }, h->mesh, false, deriv);
+        for (int med = 0; med < (h->fluid ? 2 : 1); ++med) {
+            h->nys_sd[med] = (z64*)hm_alloc(h, nc);
+            if (deriv) h->nys_kh[med] = (z64*)hm_alloc(h, nc);
+            nys_medium(h->lane->s, &h->lane->pl, h->mesh, w, med == 0 ? h->par.k : h->par.ki, false, deriv,
+                       h->nys_sd[med], h->nys_kh[med]);
+        }
         HIPCHECK(hipGetLastError());
     }
     if (h->fluid) {
@@ -4791,7 +4802,10 @@ int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, c
     return guarded(ctx, [&] {
         validate_problem(*problem);
         const bool fluid = problem->boundary == BEM_FLUID;
-        if (fluid) require_panels(mesh, "compressed fluid BEM");
+        if (fluid && mesh->h.nq && calderon_applies(mesh->h, *problem))
+            throw BemError(BEM_ERR_ARGUMENT,
+                           "compressed fluid BEM on Nystrom meshes needs max(k, k_int) * radius > pi/2 (below it "
+                           "the Calderon form requires a flat-panel mesh)");
         if (problem->boundary != BEM_RIGID && !fluid)
             throw BemError(BEM_ERR_ARGUMENT, "the hierarchical path solves rigid surfaces and fluid bodies");
         if (fluid) {

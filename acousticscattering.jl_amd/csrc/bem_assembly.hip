@@ -626,44 +626,60 @@ __global__ __launch_bounds__(128) void k_hself(DevMesh m, HPar p, int64_t nself,
                                     : hcombine<KIND>(a, 1.0 / p.k, true);
 }
 
+__device__ inline void add_corr(Acc4<double>& a, const z64* sd, const z64* kh, int64_t t) {
+    const z64 dS = sd[2 * t], dD = sd[2 * t + 1];
+    a.S.x += dS.x; a.S.y += dS.y;
+    a.D.x += dD.x; a.D.y += dD.y;
+    if (kh) {
+        const z64 dK = kh[2 * t], dH = kh[2 * t + 1];
+        a.K.x += dK.x; a.K.y += dK.y;
+        a.H.x += dH.x; a.H.y += dH.y;
+    }
+}
+
 template <int KIND>
 __global__ __launch_bounds__(128) void k_hnys_self(DevMesh m, HPar p, int64_t nself, const int* __restrict__ si,
                                                    const int* __restrict__ sj, const int64_t* __restrict__ dst,
                                                    const z64* __restrict__ corr_sd, const z64* __restrict__ corr_kh,
-                                                   z32* __restrict__ val) {
+                                                   const z64* __restrict__ corr_sd_i,
+                                                   const z64* __restrict__ corr_kh_i, z32* __restrict__ val) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nself) return;
     const int nt = m.nt, nq = m.nq;
-    const int i = si[e], j = sj[e];
-    Acc4<double> a{};
+    const int I = si[e], J = sj[e];
+    const int i = hpanel<KIND>(I, nt), j = hpanel<KIND>(J, nt);
+    Acc4<double> a{}, b{};
     if (i != j) {
         const double nx0 = m.nrm[i], nx1 = m.nrm[nt + i], nx2 = m.nrm[2 * nt + i];
         const double ny0 = m.nrm[j], ny1 = m.nrm[nt + j], ny2 = m.nrm[2 * nt + j];
-        kpoint<double, HNeed<KIND>::v>(p.k, m.cen[i] - m.cen[j], m.cen[nt + i] - m.cen[nt + j],
-                                       m.cen[2 * nt + i] - m.cen[2 * nt + j], nx0, nx1, nx2, ny0, ny1, ny2,
-                                       nx0 * ny0 + nx1 * ny1 + nx2 * ny2, m.area[j], a);
+        const double rx = m.cen[i] - m.cen[j], ry = m.cen[nt + i] - m.cen[nt + j],
+                     rz = m.cen[2 * nt + i] - m.cen[2 * nt + j];
+        const double c = nx0 * ny0 + nx1 * ny1 + nx2 * ny2;
+        kpoint<double, HNeed<KIND>::v>(p.k, rx, ry, rz, nx0, nx1, nx2, ny0, ny1, ny2, c, m.area[j], a);
+        if (KIND == HK_MULLER) kpoint<double, 15>(p.ki, rx, ry, rz, nx0, nx1, nx2, ny0, ny1, ny2, c, m.area[j], b);
     }
     const int64_t t = (int64_t)i * nq + (j - (i / nq) * nq);
-    const z64 dS = corr_sd[2 * t], dD = corr_sd[2 * t + 1];
-    a.S.x += dS.x; a.S.y += dS.y;
-    a.D.x += dD.x; a.D.y += dD.y;
-    if (corr_kh) {
-        const z64 dK = corr_kh[2 * t], dH = corr_kh[2 * t + 1];
-        a.K.x += dK.x; a.K.y += dK.y;
-        a.H.x += dH.x; a.H.y += dH.y;
+    add_corr(a, corr_sd, corr_kh, t);
+    if (KIND == HK_MULLER) {
+        add_corr(b, corr_sd_i, corr_kh_i, t);
+        val[dst[e]] = mcombine(a, b, I >= nt, J >= nt, p.g, I == J);
+    } else {
+        val[dst[e]] = hcombine<KIND>(a, 1.0 / p.k, i == j);
     }
-    val[dst[e]] = hcombine<KIND>(a, 1.0 / p.k, i == j);
 }
 
 void launch_hnys_self(hipStream_t s, const DevMesh& m, int kind, const HPar& p, int64_t nself, const int* si,
-                      const int* sj, const int64_t* dst, const z64* corr_sd, const z64* corr_kh, z32* val) {
+                      const int* sj, const int64_t* dst, const z64* corr_sd, const z64* corr_kh,
+                      const z64* corr_sd_i, const z64* corr_kh_i, z32* val) {
     if (nself <= 0) return;
     const dim3 grid((unsigned)((nself + 127) / 128));
-#define BEM_HNS(K) hipLaunchKernelGGL((k_hnys_self<K>), grid, dim3(128), 0, s, m, p, nself, si, sj, dst, corr_sd, corr_kh, val)
+#define BEM_HNS(K) hipLaunchKernelGGL((k_hnys_self<K>), grid, dim3(128), 0, s, m, p, nself, si, sj, dst, corr_sd, \
+                                      corr_kh, corr_sd_i, corr_kh_i, val)
     switch (kind) {
     case HK_BM: BEM_HNS(HK_BM); break;
     case HK_CBIE: BEM_HNS(HK_CBIE); break;
     case HK_BM_RHS: BEM_HNS(HK_BM_RHS); break;
+    case HK_MULLER: BEM_HNS(HK_MULLER); break;
     default: BEM_HNS(HK_CBIE_RHS); break;
     }
 #undef BEM_HNS

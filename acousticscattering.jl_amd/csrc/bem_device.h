@@ -146,8 +146,11 @@ void launch_hdense(hipStream_t s, const DevMesh& m, int kind, const HPar& p, con
 // take the naive entry (zero on the diagonal) plus the identity terms and the
 // density-interpolation corrections corr_sd / corr_kh (launch_nys_weights
 // layout; corr_kh may be null), in FP64.
+// Mueller (HK_MULLER) takes the interior medium's corrections in corr_sd_i /
+// corr_kh_i and the unknowns' two quadrants (I >= nt: normal derivative).
 void launch_hnys_self(hipStream_t s, const DevMesh& m, int kind, const HPar& p, int64_t nself, const int* si,
-                      const int* sj, const int64_t* dst, const z64* corr_sd, const z64* corr_kh, z32* val);
+                      const int* sj, const int64_t* dst, const z64* corr_sd, const z64* corr_kh,
+                      const z64* corr_sd_i, const z64* corr_kh_i, z32* val);
 void launch_hnear_self(hipStream_t s, const DevMesh& m, int kind, const HPar& p, int64_t npairs, const int* pi,
                        const int* pj, const int64_t* dst, int64_t nself, const int* si, const int* sj,
                        const int64_t* self_dst, z32* val);

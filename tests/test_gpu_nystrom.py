@@ -199,8 +199,38 @@ def test_nystrom_mesh_reports_quadrature_and_rejects_panel_only_paths(ctx):
     xyz, nr, w = gm.quadrature()
     x0, n0, w0 = bh.nystrom_quadrature(nodes, tri, 4)
     assert np.array_equal(xyz, x0) and np.array_equal(w, w0)
-    with pytest.raises(ValueError, match="flat-panel mesh"):
-        bh.HMatrix(gm, bh.FluidFilled(1.2, 0.8), 2.0)
+    # unit sphere at k = 1, k_int = 1.25: the Calderon form's range
+    with pytest.raises(ValueError, match="Calderon form requires a flat-panel mesh"):
+        bh.HMatrix(gm, bh.FluidFilled(1.2, 0.8), 1.0)
+
+
+def test_compressed_nystrom_mueller_matches_dense(ctx):
+    """compression=(method=:hmatrix,) for a fluid body on a Nystrom mesh
+    (src/solvers/bem/compression/fluid.jl:20-106) above the Calderon range:
+    the compressed 2n x 2n Mueller operator equals the dense Nystrom one to the
+    compression accuracy, and GMRES the dense LU solution (the reference's bar
+    is 2e-5, test/basic/bem/formulations.jl:344-345)."""
+    say("[gpu] compressed nystrom mueller")
+    gm, om = _pair(ctx, 2, (1.0, 0.8, 0.9))
+    bc, k = bh.FluidFilled(1.2, 0.8), 4.0
+    _, radius, _ = NY.quadrature_size(om)
+    assert (k / 0.8) * radius > math.pi / 2
+    H = bh.HMatrix(gm, bc, k)
+    info = H.info()
+    assert info["n"] == 2 * gm.n and info["lowrank_blocks"] > 0 and info["unconverged_blocks"] == 0
+    op = bh.Operator(gm, bc, k)
+    A = op.matrix()
+    rng = np.random.default_rng(13)
+    X = rng.standard_normal((2 * gm.n, 3)) + 1j * rng.standard_normal((2 * gm.n, 3))
+    Y = H.matvec(X)
+    assert np.linalg.norm(Y - A @ X) / np.linalg.norm(A @ X) < 2e-5
+    angles = [(0.0, 0.0), (1.1, 0.4)]
+    sol = H.solve(angles)
+    dense = op.solve(angles)
+    for r in range(len(angles)):
+        assert sol.diagnostics(r)["converged"]
+        f, f0 = sol.far_field(r)[0], dense.far_field(r)[0]
+        assert abs(f - f0) / abs(f0) < 2e-5
 
 
 @pytest.mark.parametrize("formulation", ["burton_miller", "cbie"])
