@@ -2022,6 +2022,7 @@ struct bem_hmatrix_s {
     // Nystrom meshes: density-interpolation corrections of each medium (0
     // exterior, 1 interior of fluid bodies)
     z64 *nys_sd[2] = {nullptr, nullptr}, *nys_kh[2] = {nullptr, nullptr};
+    int nys_med = 0;  // medium of the single-medium operator being built (Calderon form)
     int64_t nsys = 0;                 // unknowns of the system (n, or 2N on the panel tree)
     std::vector<int> sysperm;         // unknown of each system row (nsys)
     int* dsysperm = nullptr;
@@ -2221,8 +2222,8 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
     op.dval = (z32*)hm_alloc(h, sizeof(z32) * std::max<int64_t>(1, h->dense_count));
     launch_hdense(s, dm, kind, h->par, h->perm, nd, op.dblk, op.dval);
     if (dm.nq)
-        launch_hnys_self(s, dm, kind, h->par, h->nself, h->nsi, h->nsj, h->sdst, h->nys_sd[0], h->nys_kh[0],
-                         h->nys_sd[1], h->nys_kh[1], op.dval);
+        launch_hnys_self(s, dm, kind, h->par, h->nself, h->nsi, h->nsj, h->sdst, h->nys_sd[h->nys_med],
+                         h->nys_kh[h->nys_med], h->nys_sd[1], h->nys_kh[1], op.dval);
     else
         launch_hnear_self(s, dm, kind, h->par, h->nnear, h->npi, h->npj, h->ndst, h->nself, h->nsi, h->nsj,
                           h->sdst, op.dval);
@@ -2399,6 +2400,23 @@ void hm_coarse(bem_hmatrix h);
 
 void hm_build_calderon(bem_hmatrix h);
 
+// Nystrom meshes: the corrections of S and D (and of K', H when `deriv`) of
+// each medium, from the naive defects over all node pairs; regular-wave
+// basis under the dense path's rule (Mueller at low frequency), else point
+// sources.
+void hm_nys_corrections(bem_hmatrix h, bool deriv) {
+    const bool regular = nys_regular(h->mesh->h, h->prob);
+    const size_t nc = sizeof(z64) * 2 * (size_t)h->mesh->d.nt * h->mesh->d.nq;
+    const NysWork w = nys_work([&](size_t b) { return hm_alloc(h, b); }, h->mesh, regular, deriv);
+    for (int med = 0; med < (h->fluid ? 2 : 1); ++med) {
+        h->nys_sd[med] = (z64*)hm_alloc(h, nc);
+        if (deriv) h->nys_kh[med] = (z64*)hm_alloc(h, nc);
+        nys_medium(h->lane->s, &h->lane->pl, h->mesh, w, med == 0 ? h->par.k : h->par.ki, regular, deriv,
+                   h->nys_sd[med], h->nys_kh[med]);
+    }
+    HIPCHECK(hipGetLastError());
+}
+
 void hm_build(bem_hmatrix h) {
     const bool bm = h->prob.formulation == BEM_BURTON_MILLER;
     if (h->calderon) {
@@ -2409,22 +2427,7 @@ void hm_build(bem_hmatrix h) {
     h->nsys = h->n;
     h->sysperm = h->hperm;
     h->dsysperm = h->perm;
-    if (h->mesh->d.nq) {
-        // Nystrom meshes: the corrections of S and D (and K', H for
-        // Burton-Miller and Mueller) of each medium, point-source basis (the
-        // direct Mueller regime is above the regular-wave range), from the
-        // naive defects
-        const bool deriv = bm || h->fluid;
-        const size_t nc = sizeof(z64) * 2 * (size_t)h->mesh->d.nt * h->mesh->d.nq;
-        const NysWork w = nys_work([&](size_t b) { return hm_alloc(h, b); }, h->mesh, false, deriv);
-        for (int med = 0; med < (h->fluid ? 2 : 1); ++med) {
-            h->nys_sd[med] = (z64*)hm_alloc(h, nc);
-            if (deriv) h->nys_kh[med] = (z64*)hm_alloc(h, nc);
-            nys_medium(h->lane->s, &h->lane->pl, h->mesh, w, med == 0 ? h->par.k : h->par.ki, false, deriv,
-                       h->nys_sd[med], h->nys_kh[med]);
-        }
-        HIPCHECK(hipGetLastError());
-    }
+    if (h->mesh->d.nq) hm_nys_corrections(h, bm || h->fluid);
     if (h->fluid) {
         // Mueller: the right-hand side is the incident traces themselves
         hm_operator(h, h->A, HK_MULLER);
@@ -2483,13 +2486,16 @@ void hm_build_calderon(bem_hmatrix h) {
         h->sysperm[N + pos] = (int)(N + h->hperm[pos]);
     }
     h->dsysperm = hm_upload(h, h->sysperm);
+    if (h->mesh->d.nq) hm_nys_corrections(h, false);  // corrected S and D of both media
     const HPar keep = h->par;
     const double km[2] = {keep.k, keep.ki};
     for (int med = 0; med < 2; ++med) {
         h->par = HPar{km[med], km[med], 1.0};
+        h->nys_med = med;
         hm_operator(h, h->cal[2 * med], HK_CBIE_RHS);  // S
         hm_operator(h, h->cal[2 * med + 1], HK_CBIE);  // 1/2 I - D
     }
+    h->nys_med = 0;
     h->par = keep;
     // block Jacobi of S_e and S_i for the inner solves
     int* st = (int*)hm_alloc(h, sizeof(int));
@@ -4802,10 +4808,7 @@ int bem_hmatrix_create(bem_ctx ctx, bem_mesh mesh, const bem_problem* problem, c
     return guarded(ctx, [&] {
         validate_problem(*problem);
         const bool fluid = problem->boundary == BEM_FLUID;
-        if (fluid && mesh->h.nq && calderon_applies(mesh->h, *problem))
-            throw BemError(BEM_ERR_ARGUMENT,
-                           "compressed fluid BEM on Nystrom meshes needs max(k, k_int) * radius > pi/2 (below it "
-                           "the Calderon form requires a flat-panel mesh)");
+
         if (problem->boundary != BEM_RIGID && !fluid)
             throw BemError(BEM_ERR_ARGUMENT, "the hierarchical path solves rigid surfaces and fluid bodies");
         if (fluid) {

@@ -236,10 +236,9 @@ int bem_mesh_near_count(bem_mesh mesh, int64_t *count);
  * reference's Nystrom field evaluation (the nearest element excluded and
  * recovered by density interpolation, src/fields/pressure.jl:441-462, with
  * regions by the winding number of flat triangles through the element
- * nodes).  bem_hmatrix_create compresses them (the element corrections in
- * the dense leaves) for rigid surfaces and for fluid bodies in the direct
- * Mueller form; below the Calderon limit (max(k, k_int) radius <= pi/2)
- * compressed fluid bodies need panel meshes and return BEM_ERR_ARGUMENT. */
+ * nodes).  bem_hmatrix_create compresses them too (the element corrections
+ * in the dense leaves): rigid surfaces, and fluid bodies in the direct
+ * Mueller form or, below the Calderon limit, the Calderon form. */
 int bem_mesh_create_nystrom(bem_ctx ctx, int64_t n_nodes, const double *nodes, int64_t n_tri,
                             int32_t nodes_per_tri, const int64_t *tri, int32_t qorder, bem_mesh *out);
 /* Host only: the same validation and quadrature without a device.  With

@@ -199,9 +199,32 @@ def test_nystrom_mesh_reports_quadrature_and_rejects_panel_only_paths(ctx):
     xyz, nr, w = gm.quadrature()
     x0, n0, w0 = bh.nystrom_quadrature(nodes, tri, 4)
     assert np.array_equal(xyz, x0) and np.array_equal(w, w0)
-    # unit sphere at k = 1, k_int = 1.25: the Calderon form's range
-    with pytest.raises(ValueError, match="Calderon form requires a flat-panel mesh"):
-        bh.HMatrix(gm, bh.FluidFilled(1.2, 0.8), 1.0)
+
+
+@pytest.mark.parametrize("k", [1.0, 0.3])
+def test_compressed_nystrom_calderon_matches_dense(ctx, k):
+    """Low-frequency fluid bodies on a Nystrom mesh, compressed: K' and H from
+    the Calderon identities over compressed corrected S and D of each medium
+    (src/solvers/bem/compression/fluid.jl:37-41; here S^-1 by inner GMRES),
+    with point-source (k = 1) or regular-wave (k = 0.3) interpolation, against
+    the dense Nystrom solve of the same equations."""
+    say(f"[gpu] compressed nystrom calderon k={k}")
+    gm, om = _pair(ctx, 2, (1.0, 0.8, 0.9))
+    bc = bh.FluidFilled(1.2, 0.8)
+    _, radius, _ = NY.quadrature_size(om)
+    assert (k / 0.8) * radius <= math.pi / 2
+    assert NY.regular_range(k / 0.8, om) == (k < 0.5)
+    angles = [(0.0, 0.0), (1.1, 0.4)]
+    sol = bh.HMatrix(gm, bc, k).solve(angles)
+    dense = bh.Operator(gm, bc, k).solve(angles)
+    for r in range(len(angles)):
+        d = sol.diagnostics(r)
+        assert d["converged"] and d["derivative_evaluation"] == {"exterior": "calderon", "interior": "calderon"}
+        p, _ = sol.traces(r)
+        p0, _ = dense.traces(r)
+        assert np.abs(p - p0).max() / np.abs(p0).max() < 1e-4
+        f, f0 = sol.far_field(r)[0], dense.far_field(r)[0]
+        assert abs(f - f0) / abs(f0) < 1e-4
 
 
 def test_compressed_nystrom_mueller_matches_dense(ctx):
