@@ -1964,12 +1964,14 @@ int srv_guard(bem_server srv, F&& f) {
 struct HOp {
     int kind = HK_BM;
     std::vector<HBlk> blk;  // dense [0, nd), low-rank [nd, nd + nl)
-    int ntiles = 0;
+    int ntiles = 0, nsplit = 0;
     HBlk* dblk = nullptr;
     z32* dval = nullptr;
     z32* uv = nullptr;
     int64_t* toff = nullptr;
     HTile* tiles = nullptr;
+    HSplit* splits = nullptr;
+    z64* P = nullptr;         // segment slabs of the split projection tiles
     int64_t* vrow = nullptr;  // V row offsets of the stacked projection rows
     int64_t v_entries = 0, lowrank_entries = 0;  // sum rank * n and sum rank * (m + n)
     z64* T = nullptr;
@@ -2319,6 +2321,8 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
         for (int b = nd; b < nd + nl; ++b)
             if (op.blk[b].rank > 0) groups[{op.blk[b].col_lo, op.blk[b].n}].push_back(b);
         std::vector<HTile> tiles;
+        std::vector<HSplit> splits;
+        int nparts = 0;
         std::vector<int64_t> vrow, toff(nd + nl, 0);
         int64_t tsum = 0;
         for (const auto& g : groups) {
@@ -2331,8 +2335,21 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
                 op.v_entries += (int64_t)x.rank * x.n;
                 op.lowrank_entries += (int64_t)x.rank * (x.m + x.n);
             }
-            for (int64_t r0 = g0; r0 < tsum; r0 += kHProjRows)
-                tiles.push_back({r0, (int)std::min<int64_t>(kHProjRows, tsum - r0), g.first.first, g.first.second});
+            // clusters longer than kHProjSeg: segments of equal length (a
+            // multiple of 32 columns), each tile's partials summed in order
+            const int ncol = g.first.second;
+            const int nseg = (ncol + kHProjSeg - 1) / kHProjSeg;
+            const int seg = nseg > 1 ? ((ncol + nseg - 1) / nseg + 31) / 32 * 32 : ncol;
+            for (int64_t r0 = g0; r0 < tsum; r0 += kHProjRows) {
+                const int rows = (int)std::min<int64_t>(kHProjRows, tsum - r0);
+                if (nseg == 1) {
+                    tiles.push_back({r0, rows, g.first.first, ncol, 0, -1});
+                    continue;
+                }
+                splits.push_back({r0, rows, nparts, 0});
+                for (int v0 = 0; v0 < ncol; v0 += seg, ++nparts, ++splits.back().nseg)
+                    tiles.push_back({r0, rows, g.first.first + v0, std::min(seg, ncol - v0), v0, nparts});
+            }
         }
         // most work first (columns x rows of the tile), so the long tiles do
         // not trail the launch
@@ -2341,6 +2358,11 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
         });
         op.ntiles = (int)tiles.size();
         op.tiles = hm_upload(h, tiles);
+        op.nsplit = (int)splits.size();
+        if (op.nsplit) {
+            op.splits = hm_upload(h, splits);
+            op.P = (z64*)hm_alloc(h, sizeof(z64) * (size_t)nparts * kHProjRows * kHRhs);
+        }
         op.vrow = hm_upload(h, vrow);
         op.toff = hm_upload(h, toff);
         op.T = (z64*)hm_alloc(h, sizeof(z64) * std::max<int64_t>(1, tsum * kHRhs));
@@ -2711,7 +2733,7 @@ void hm_apply(bem_hmatrix h, HOp& op, const z64* X, int nr, z64* Y) {
     const double pf = 8.0 * nr * (double)op.v_entries;
     const double pb = sizeof(z32) * (double)op.v_entries + (double)h->n * nr * sizeof(z64);
     prof_begin(&h->lane->pl, h->lane->s, C_HPROJ);
-    launch_hproj(h->lane->s, op.ntiles, op.tiles, op.vrow, op.uv, X, nr, op.T);
+    launch_hproj(h->lane->s, op.ntiles, op.tiles, op.vrow, op.uv, X, nr, op.T, op.nsplit, op.splits, op.P);
     prof_end(&h->lane->pl, h->lane->s, C_HPROJ, pf, pb);
     prof_begin(&h->lane->pl, h->lane->s, C_HROWS);
     launch_hrows(h->lane->s, op.L, op.dblk, op.dval, op.uv, op.toff, op.T, X, nr, Y);

@@ -7,13 +7,23 @@ namespace bem {
 constexpr int kHRhs = 64;  // right-hand sides per batched apply (stride of T)
 
 // Tile of the low-rank projection: up to kHProjRows consecutive rows of the
-// stacked V factors of low-rank blocks that share one column cluster
-// [col_lo, col_lo + n).  Row s of the tile is V row vrow[row0 + s] (offset into
-// the factor store) and its product lands in T row row0 + s (stride kHRhs).
+// stacked V factors of low-rank blocks that share one column cluster, over
+// the columns [col_lo, col_lo + n) of the tree order, which are columns voff
+// .. voff + n - 1 of each V row.  Row s of the tile is V row vrow[row0 + s]
+// (offset into the factor store).  With part < 0 the tile covers the whole
+// cluster and its product lands in T row row0 + s (stride kHRhs); clusters
+// longer than kHProjSeg columns are cut into segments, each writing slab
+// `part` of P (kHProjRows x kHRhs), and an HSplit sums the nseg slabs
+// part0 .. part0 + nseg - 1 into T rows row0 .. row0 + rows - 1.
 constexpr int kHProjRows = 64;
+constexpr int kHProjSeg = 4096;
 struct HTile {
     int64_t row0;
-    int rows, col_lo, n;
+    int rows, col_lo, n, voff, part;
+};
+struct HSplit {
+    int64_t row0;
+    int rows, part0, nseg;
 };
 // A low-rank block covering a leaf row cluster, at row offset o in the block.
 struct HRef {
@@ -33,12 +43,13 @@ struct HLeafs {
     const int* order = nullptr;  // leaves by decreasing apply work (launch order)
 };
 
-// Y = A X for nr <= kHRhs right-hand sides (tree-position order) in two
-// launches: the projections T = V X of the low-rank blocks (T: kHRhs complex
-// doubles per stacked rank row), then per leaf row the dense blocks times X
-// plus U T of the low-rank blocks covering it.
+// Y = A X for nr <= kHRhs right-hand sides (tree-position order): the
+// projections T = V X of the low-rank blocks (T: kHRhs complex doubles per
+// stacked rank row; the segments of long clusters through P and the split
+// sums), then per leaf row the dense blocks times X plus U T of the
+// low-rank blocks covering it.
 void launch_hproj(hipStream_t s, int ntiles, const HTile* tiles, const int64_t* vrow, const z32* uv, const z64* X,
-                  int nr, z64* T);
+                  int nr, z64* T, int nsplit, const HSplit* splits, z64* P);
 void launch_hrows(hipStream_t s, const HLeafs& L, const HBlk* blk, const z32* dval, const z32* uv,
                   const int64_t* toff, const z64* T, const z64* X, int nr, z64* Y);
 void launch_hjacobi_inv(hipStream_t s, const HLeafs& L, const HBlk* blk, const z32* dval, z64* minv, int* status);

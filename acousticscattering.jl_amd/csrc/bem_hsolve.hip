@@ -110,31 +110,35 @@ __device__ __forceinline__ void store_colmajor(HStage<KC>& S, int t, const z32 (
     for (int u = 0; u < KC / 4; ++u) S.sA[i][l + u] = a[u];
 }
 
-// acc[rt] += sA[16 rt + i][k] * sB[k][16 w + j] over one staged chunk, for
-// the row tiles rt < NT.
+// acc[rt] += sA[16 rt + i][k] * sB[k][16 w + j] over the first kn (a
+// multiple of 4, <= KC; the same for the whole workgroup) columns of one
+// staged chunk, for the row tiles rt < NT.  Columns beyond the operand's end
+// are staged as zeros, so kn only skips products of zeros.
 template <int NT, int KC>
-__device__ __forceinline__ void chunk_products(const HStage<KC>& S, int w, int li, int lk, dv4 (&cr)[NT],
+__device__ __forceinline__ void chunk_products(const HStage<KC>& S, int kn, int w, int li, int lk, dv4 (&cr)[NT],
                                                dv4 (&ci)[NT]) {
 #pragma unroll 2
     for (int kk = 0; kk < KC; kk += 4) {
-        const int k = kk + lk;
-        const z64 b = S.sB[k][16 * w + li];
-        double ar[NT], ai[NT];
+        if (kk < kn) {
+            const int k = kk + lk;
+            const z64 b = S.sB[k][16 * w + li];
+            double ar[NT], ai[NT];
 #pragma unroll
-        for (int rt = 0; rt < NT; ++rt) {
-            const z32 a = S.sA[16 * rt + li][k];
-            ar[rt] = a.x;
-            ai[rt] = a.y;
-        }
+            for (int rt = 0; rt < NT; ++rt) {
+                const z32 a = S.sA[16 * rt + li][k];
+                ar[rt] = a.x;
+                ai[rt] = a.y;
+            }
 #pragma unroll
-        for (int rt = 0; rt < NT; ++rt) {
-            cr[rt] = mfma64(ar[rt], b.x, cr[rt]);
-            ci[rt] = mfma64(ar[rt], b.y, ci[rt]);
-        }
+            for (int rt = 0; rt < NT; ++rt) {
+                cr[rt] = mfma64(ar[rt], b.x, cr[rt]);
+                ci[rt] = mfma64(ar[rt], b.y, ci[rt]);
+            }
 #pragma unroll
-        for (int rt = 0; rt < NT; ++rt) {
-            cr[rt] = mfma64(-ai[rt], b.y, cr[rt]);
-            ci[rt] = mfma64(ai[rt], b.x, ci[rt]);
+            for (int rt = 0; rt < NT; ++rt) {
+                cr[rt] = mfma64(-ai[rt], b.y, cr[rt]);
+                ci[rt] = mfma64(ai[rt], b.x, ci[rt]);
+            }
         }
     }
 }
@@ -152,22 +156,32 @@ __device__ __forceinline__ void store_tiles(const dv4 (&cr)[NT], const dv4 (&ci)
         }
 }
 
+// multiple of 4 at or above min(KC, kmax - k0)
+template <int KC>
+__device__ __forceinline__ int chunk_width(int k0, int kmax) {
+    return (min(KC, kmax - k0) + 3) & ~3;
+}
+
 // (1) Projection T = V X_cols for the stacked V rows of one tile (up to 64
-// rows of low-rank blocks that share a column cluster): one X chunk in LDS
+// rows of low-rank blocks that share a column cluster) over its column range
+// (the whole cluster, or one segment of a long one): one X chunk in LDS
 // serves every stacked row, so X is read once per tile rather than once per
-// block.  (A variant in which each workgroup streams several tiles, with the
-// next tile's first chunk in flight across the boundary, needed 184 VGPRs,
-// two workgroups per CU instead of three, and ran 8% slower.)
+// block.  A segment writes its partial product to its own slab of P, summed
+// into T afterwards in segment order.  (A variant in which each workgroup
+// streams several tiles, with the next tile's first chunk in flight across
+// the boundary, needed 184 VGPRs, two workgroups per CU instead of three,
+// and ran 8% slower.)
 constexpr int kProjKC = 32;  // K chunk of the projection
 
 template <int NT>
 __device__ void hproj_tile(const HTile& tl, const int64_t* __restrict__ vrow, const z32* __restrict__ uv,
-                           const z64* __restrict__ X, int nr, z64* __restrict__ T, HStage<kProjKC>& S) {
+                           const z64* __restrict__ X, int nr, z64* __restrict__ T, z64* __restrict__ P,
+                           HStage<kProjKC>& S) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int ia = tid >> 2;
     const bool row_ok = ia < tl.rows;
-    const z32* vsrc = row_ok ? uv + vrow[tl.row0 + ia] : uv;
+    const z32* vsrc = row_ok ? uv + vrow[tl.row0 + ia] + tl.voff : uv;
     dv4 cr[NT], ci[NT];
 #pragma unroll
     for (int q = 0; q < NT; ++q) cr[q] = ci[q] = dv4{0, 0, 0, 0};
@@ -183,21 +197,40 @@ __device__ void hproj_tile(const HTile& tl, const int64_t* __restrict__ vrow, co
             load_rowmajor<kProjKC>(vsrc, row_ok, k0 + kProjKC, tl.n, tid, R.a);
             load_vec<kProjKC>(X, tl.col_lo, nr, k0 + kProjKC, tl.n, nr, tid, R.b);
         }
-        chunk_products<NT, kProjKC>(S, w, li, lk, cr, ci);
+        chunk_products<NT, kProjKC>(S, chunk_width<kProjKC>(k0, tl.n), w, li, lk, cr, ci);
     }
-    store_tiles<NT>(cr, ci, T + tl.row0 * kHRhs, kHRhs, tl.rows, nr, w, li, lk);
+    z64* out = tl.part < 0 ? T + tl.row0 * kHRhs : P + (int64_t)tl.part * kHProjRows * kHRhs;
+    store_tiles<NT>(cr, ci, out, kHRhs, tl.rows, nr, w, li, lk);
 }
 
 __global__ __launch_bounds__(256) void k_hproj(const HTile* __restrict__ tiles, const int64_t* __restrict__ vrow,
                                                const z32* __restrict__ uv, const z64* __restrict__ X, int nr,
-                                               z64* __restrict__ T) {
+                                               z64* __restrict__ T, z64* __restrict__ P) {
     const HTile tl = tiles[blockIdx.x];
     __shared__ HStage<kProjKC> S;
     switch ((tl.rows + 15) >> 4) {
-    case 1: hproj_tile<1>(tl, vrow, uv, X, nr, T, S); break;
-    case 2: hproj_tile<2>(tl, vrow, uv, X, nr, T, S); break;
-    case 3: hproj_tile<3>(tl, vrow, uv, X, nr, T, S); break;
-    default: hproj_tile<4>(tl, vrow, uv, X, nr, T, S); break;
+    case 1: hproj_tile<1>(tl, vrow, uv, X, nr, T, P, S); break;
+    case 2: hproj_tile<2>(tl, vrow, uv, X, nr, T, P, S); break;
+    case 3: hproj_tile<3>(tl, vrow, uv, X, nr, T, P, S); break;
+    default: hproj_tile<4>(tl, vrow, uv, X, nr, T, P, S); break;
+    }
+}
+
+// T rows of each split tile = the sum of its segments' slabs, in segment
+// order (fixed summation order: deterministic).
+__global__ __launch_bounds__(256) void k_hproj_sum(const HSplit* __restrict__ splits, const z64* __restrict__ P,
+                                                   int nr, z64* __restrict__ T) {
+    const HSplit sp = splits[blockIdx.x];
+    for (int idx = threadIdx.x; idx < sp.rows * nr; idx += 256) {
+        const int i = idx / nr, c = idx - i * nr;
+        const z64* src = P + ((int64_t)sp.part0 * kHProjRows + i) * kHRhs + c;
+        z64 acc = src[0];
+        for (int q = 1; q < sp.nseg; ++q) {
+            const z64 v = src[(int64_t)q * kHProjRows * kHRhs];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        T[(sp.row0 + i) * kHRhs + c] = acc;
     }
 }
 
@@ -244,6 +277,7 @@ __device__ void hrows_leaf(const HLeafs& L, int leaf, const HBlk* __restrict__ b
         else store_colmajor(S, tid, R.a);
         store_vec(S, tid, R.b);
         __syncthreads();
+        const int kn = chunk_width<kRowsKC>(k0, kmax);
         // next chunk: the rest of this item, else the next item with work
         k0 += kRowsKC;
         if (k0 >= kmax) {
@@ -252,7 +286,7 @@ __device__ void hrows_leaf(const HLeafs& L, int leaf, const HBlk* __restrict__ b
             while (e < nitems && (kmax = kdim(e)) <= 0) ++e;
         }
         if (e < nitems) load(e, k0, R);
-        chunk_products<NT, kRowsKC>(S, w, li, lk, cr, ci);
+        chunk_products<NT, kRowsKC>(S, kn, w, li, lk, cr, ci);
     }
     store_tiles<NT>(cr, ci, Y + (int64_t)lo * nr, nr, m, nr, w, li, lk);
 }
@@ -276,8 +310,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 void launch_hproj(hipStream_t s, int ntiles, const HTile* tiles, const int64_t* vrow, const z32* uv, const z64* X,
-                  int nr, z64* T) {
-    if (ntiles > 0) hipLaunchKernelGGL(k_hproj, dim3(ntiles), dim3(256), 0, s, tiles, vrow, uv, X, nr, T);
+                  int nr, z64* T, int nsplit, const HSplit* splits, z64* P) {
+    if (ntiles > 0) hipLaunchKernelGGL(k_hproj, dim3(ntiles), dim3(256), 0, s, tiles, vrow, uv, X, nr, T, P);
+    if (nsplit > 0) hipLaunchKernelGGL(k_hproj_sum, dim3(nsplit), dim3(256), 0, s, splits, P, nr, T);
 }
 void launch_hrows(hipStream_t s, const HLeafs& L, const HBlk* blk, const z32* dval, const z32* uv,
                   const int64_t* toff, const z64* T, const z64* X, int nr, z64* Y) {

@@ -570,7 +570,7 @@ def hmatrix_main(args):
         "roofline": {"kernel": "compressed apply (64 right-hand sides)", "bound": "mfma", "achieved": tfs,
                      "peak": MFMA_FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": tfs * 1e12 / MFMA_FP64_PEAK,
                      "traffic": measured_traffic(TRAFFIC_PROFILE_HM), "traffic_source": TRAFFIC_PROFILE_HM,
-                     "traffic_basis": "HBM bytes per 64-RHS apply (k_hproj + k_hrows), FETCH_SIZE and WRITE_SIZE "
+                     "traffic_basis": "HBM bytes per 64-RHS apply (k_hproj, k_hproj_sum, k_hrows), FETCH_SIZE and WRITE_SIZE "
                                       "passes of this workload",
                      "hbm_gbs": gbs, "hbm_frac": gbs / 8000.0,
                      "flops_per_byte": ap["flops"] / ap["bytes"] if ap["bytes"] > 0 else None,
