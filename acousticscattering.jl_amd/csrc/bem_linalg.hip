@@ -1333,36 +1333,54 @@ __global__ __launch_bounds__(256) void k_split_cols(const z32* __restrict__ B, i
 
 // C[M x N] -= A B from split planes SA (rows, Mpad) and SB (columns, Npad,
 // starting at column boff, a multiple of 128); K a multiple of 32.  128 x 128
-// tiles, 8 waves as 2 x 4 of 64 x 32 (4 x 2 MFMA blocks), the tile order of
-// k_gemm3m.  Each 32-wide k chunk runs as three phases, one per 3M stream:
-// a phase stages the stream's three pieces of both operands (48 KB) in one
-// half of a double-buffered LDS (rows padded to 80 bytes, so the 16 rows a
-// 16-lane group reads fall on distinct banks) while the previous phase's 48
-// MFMAs per wave run; one barrier per phase.  The padded rows and columns
-// are zero, so only the stores of C test the edges.
-constexpr int kBfPitch = 5;  // LDS row pitch in 16-byte units
-__global__ __launch_bounds__(512, 1) void k_gemm3m_bf(const bf8v* __restrict__ SA, int Mpad,
-                                                      const bf8v* __restrict__ SB, int Npad, int boff,
-                                                      z32* __restrict__ C, int64_t ldc, int M, int N, int K,
-                                                      int group) {
+// tiles in the tile order of k_gemm3m, 4 waves of 64 x 64 (4 x 4 MFMA
+// blocks; one wave per SIMD, its three 3M accumulator sets, 192 values, in
+// the accumulation registers).  Each 32-wide k chunk runs as three phases,
+// one per 3M stream, each the six piece products of the stream in the order
+// mid mid, hi lo, lo hi, hi mid, mid hi, hi hi (small terms first).  The
+// planes go straight from global memory into LDS (global_load_lds_dwordx4:
+// no staging registers, no LDS store instructions).  LDS holds one buffer
+// per stream, each the three pieces of both operands for one chunk as
+// [piece][k / 8][row] in 16-byte units (the 16 lanes of a read pass take 256
+// consecutive bytes, so reads are free of bank conflicts), 144 KB in all.
+// The copy of stream s + 2 is in flight while stream s multiplies, two
+// phases ahead; each wave waits for its own copies (vmcnt) before the
+// barrier that publishes them.  The padded rows and columns are zero, so
+// only the stores of C test the edges.  (An earlier form with 8 waves of
+// 64 x 32, two per SIMD, and register-staged copies one phase ahead ran the
+// updates of a 40960-unknown factorization at 6.02 ms per launch against
+// 5.32.)
+typedef __attribute__((address_space(3))) void lds_void_t;
+struct BfBuf {
+    bf8v a[3][4][kGM];
+    bf8v b[3][4][kGN];
+};
+__device__ __forceinline__ void bf_copy(const bf8v* src, bf8v* lds_base) {
+    __builtin_amdgcn_global_load_lds((const void*)src, (lds_void_t*)lds_base, 16, 0, 0);
+}
+// s_waitcnt immediates (vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt
+// 11:8): vmcnt(12) lgkmcnt(0), and vmcnt(0) lgkmcnt(0)
+constexpr int kWaitVm12 = 0x007C, kWaitVm0 = 0x0070;
+__global__ __launch_bounds__(256, 1) void k_gemm3m_bf(const bf8v* __restrict__ SA, int Mpad,
+                                                       const bf8v* __restrict__ SB, int Npad, int boff,
+                                                       z32* __restrict__ C, int64_t ldc, int M, int N, int K,
+                                                       int group) {
     typedef float v4 __attribute__((ext_vector_type(4)));
-    __shared__ bf8v sA[2][3][kGM][kBfPitch];
-    __shared__ bf8v sB[2][3][kGN][kBfPitch];
+    __shared__ BfBuf buf0, buf1, buf2;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wm = (wv >> 2) * 64, wn = (wv & 3) * 32;
+    const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
     int tm, tn;
     g3_tile(gridDim.x, gridDim.y, group, tm, tn);
     const int m0 = tm * kGM, n0 = tn * kGN;
     const bool full = m0 + kGM <= M && n0 + kGN <= N;
     const int li = lane & 15, lq = lane >> 4;
-    v4 t1[4][2], t2[4][2], t3[4][2];
-    // C - A B: t1 = -Re C + Ar Br, t2 = Ai Bi, t3 = -Re C - Im C + (Ar + Ai)(Br + Bi)
+    v4 t1[4][4], t2[4][4], t3[4][4];
     const z32* cl = C + (int64_t)(4 * lq) * ldc + li;
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
+        for (int y = 0; y < 4; ++y) {
             t2[x][y] = v4{0, 0, 0, 0};
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -1373,38 +1391,34 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m_bf(const bf8v* __restrict__ S
                 t3[x][y][r] = -c.x - c.y;
             }
         }
-    const int lm = tid & 127, lk = tid >> 7;
-    const int KG = K / 8;
-    const int64_t psA = (int64_t)KG * Mpad, psB = (int64_t)KG * Npad;
-    const bf8v* pa = SA + (int64_t)lk * Mpad + m0 + lm;
-    const bf8v* pb = SB + (int64_t)lk * Npad + boff + n0 + lm;
-    bf8v ra[3], rb[3];
-    auto gload = [&](int c, int st) {
+    // copy units of this wave: waves 0, 1 the A planes, waves 2, 3 the B
+    // planes; unit u (0..11) of wave half h is piece p = (12 h + u) / 8,
+    // k group kg = (12 h + u) % 8 / 2, rows 64 ((12 h + u) % 2) .. + 63
+    const bool opA = wv < 2;
+    const int h = wv & 1;
+    const int64_t pitch = opA ? Mpad : Npad;
+    const int64_t plane = (int64_t)(K / 8) * pitch;
+    const bf8v* gbase = (opA ? SA + m0 : SB + boff + n0) + lane;
+    auto copy = [&](int c, int st, BfBuf& B) {
+        const bf8v* g = gbase + (int64_t)(st * 3) * plane + (int64_t)(4 * c) * pitch;
 #pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            ra[p] = pa[(st * 3 + p) * psA + (int64_t)(4 * c) * Mpad];
-            rb[p] = pb[(st * 3 + p) * psB + (int64_t)(4 * c) * Npad];
+        for (int u = 0; u < 12; ++u) {
+            const int v = 12 * h + u, p = v >> 3, kg = (v & 7) >> 1, half = v & 1;
+            bf8v* dst = opA ? &B.a[p][kg][64 * half] : &B.b[p][kg][64 * half];
+            bf_copy(g + p * plane + kg * pitch + 64 * half, dst);
         }
     };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            sA[buf][p][lm][lk] = ra[p];
-            sB[buf][p][lm][lk] = rb[p];
-        }
-    };
-    auto products = [&](int buf, v4 (&acc)[4][2]) {
-        bf8v a[3][4], b[3][2];
+    auto products = [&](const BfBuf& B, v4 (&acc)[4][4]) {
+        bf8v a[3][4], b[3][4];
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
 #pragma unroll
-            for (int x = 0; x < 4; ++x) a[p][x] = sA[buf][p][wm + 16 * x + li][lq];
+            for (int x = 0; x < 4; ++x) a[p][x] = B.a[p][lq][wm + 16 * x + li];
 #pragma unroll
-            for (int y = 0; y < 2; ++y) b[p][y] = sB[buf][p][wn + 16 * y + li][lq];
+            for (int y = 0; y < 4; ++y) b[p][y] = B.b[p][lq][wn + 16 * y + li];
         }
-        // small terms first: mid mid, hi lo, lo hi, hi mid, mid hi, hi hi
-#define BF_TERM(P, Q)                                                                                     \
-    _Pragma("unroll") for (int x = 0; x < 4; ++x) _Pragma("unroll") for (int y = 0; y < 2; ++y) acc[x][y] = \
+#define BF_TERM(P, Q)                                                                                    \
+    _Pragma("unroll") for (int x = 0; x < 4; ++x) _Pragma("unroll") for (int y = 0; y < 4; ++y) acc[x][y] = \
         __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[P][x], b[Q][y], acc[x][y], 0, 0, 0);
         BF_TERM(1, 1)
         BF_TERM(0, 2)
@@ -1414,34 +1428,36 @@ __global__ __launch_bounds__(512, 1) void k_gemm3m_bf(const bf8v* __restrict__ S
         BF_TERM(0, 0)
 #undef BF_TERM
     };
+    // publish: this wave's copies of the buffer about to be read are done
+    // (at most the 12 copies issued after them remain), every wave's LDS
+    // reads of the previous phase are done, then the barrier
+    auto sync = [&](bool later_in_flight) {
+        if (later_in_flight) __builtin_amdgcn_s_waitcnt(kWaitVm12);
+        else __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        __asm__ volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        __asm__ volatile("" ::: "memory");
+    };
     const int nc = K / 32;
-    gload(0, 0);
-    lstore(0);
-    __syncthreads();
-    int buf = 0;
+    copy(0, 0, buf0);
+    copy(0, 1, buf1);
     for (int c = 0; c < nc; ++c) {
-        gload(c, 1);
-        products(buf, t1);
-        lstore(buf ^ 1);
-        __syncthreads();
-        buf ^= 1;
-        gload(c, 2);
-        products(buf, t2);
-        lstore(buf ^ 1);
-        __syncthreads();
-        buf ^= 1;
         const bool more = c + 1 < nc;
-        if (more) gload(c + 1, 0);
-        products(buf, t3);
-        if (more) lstore(buf ^ 1);
-        __syncthreads();
-        buf ^= 1;
+        sync(true);
+        copy(c, 2, buf2);
+        products(buf0, t1);
+        sync(true);
+        if (more) copy(c + 1, 0, buf0);
+        products(buf1, t2);
+        sync(more);
+        if (more) copy(c + 1, 1, buf1);
+        products(buf2, t3);
     }
     z32* cs = C + (int64_t)(4 * lq) * ldc + li;
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
+        for (int y = 0; y < 4; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int rb = m0 + wm + x * 16 + r, cb = n0 + wn + y * 16;
@@ -1582,7 +1598,7 @@ static void launch_update_split(hipStream_t s, z32* A, int64_t lda, int r0, cons
     if (P.M <= 0 || N <= 0) return;
     prof_begin(prof, s, CLS_GEMMBF);
     dim3 grid((N + kGN - 1) / kGN, P.Mpad / kGM);
-    hipLaunchKernelGGL(k_gemm3m_bf, grid, dim3(512), 0, s, P.rows, P.Mpad, P.cols, P.Mpad, c0,
+    hipLaunchKernelGGL(k_gemm3m_bf, grid, dim3(256), 0, s, P.rows, P.Mpad, P.cols, P.Mpad, c0,
                        A + (int64_t)r0 * lda + r0 + c0, lda, P.M, N, P.K, group);
     prof_end(prof, s, CLS_GEMMBF, 8.0 * (double)P.M * N * P.K,
              (2.0 * (double)P.M * N * sizeof(z32) + 9.0 * (double)(P.M + N) * P.K * sizeof(__bf16)));
