@@ -10,16 +10,14 @@ FP64 far field.  |dTS| is checked against the same sweep solved all-FP64.
 
 Prints one JSON line (value = solves/s of the timed sweep).
 
-The sweep runs four lanes (two HIP streams each) under 8 hardware queues
-(GPU_MAX_HW_QUEUES, `--hw-queues`, two per lane): with the runtime's default
-of 4, the lanes' streams share queues and a lane's kernels wait in order
-behind another lane's.  The sweep is bound by its chain of heavy solves, so
-4, 6 and 8 lanes give the same throughput; four give the lowest median
-latency.  The variable is set in this process before the library (and so
-the HIP runtime) is loaded.  One solve with 16384 or more unknowns runs at a
-time (`--heavy-lanes`, the library's default) while the other lanes take the
-smaller ones: the sweep's throughput is within 2 % of letting them run
-together, at a third of the median and half of the tail latency.
+The sweep runs six lanes (`--lanes`; two HIP streams each) under two
+hardware queues per lane (GPU_MAX_HW_QUEUES, `--hw-queues`): with the
+runtime's default of 4, the lanes' streams share queues and a lane's kernels
+wait in order behind another lane's.  The variable is set in this process
+before the library (and so the HIP runtime) is loaded.  Two solves with 16384
+or more unknowns run at a time (`--heavy-lanes`): one heavy solve's panels,
+triangular solves and refinement overlap the other's trailing updates, while
+the other lanes take the smaller solves.
 
 Time budget (`--budget-s`, default 400 s for the whole process): at most one
 warm-up sweep runs; after `--min-steps` timed sweeps (3), further timed
