@@ -2176,7 +2176,10 @@ void hm_plan(bem_hmatrix h) {
     for (int b = 0; b < h->nl; ++b) {
         const HNode &r = P.nodes[P.lowrank[b].row], &c = P.nodes[P.lowrank[b].col];
         HBlk x{(int)r.lo, (int)(r.hi - r.lo), (int)c.lo, (int)(c.hi - c.lo), 0, 0, uoff};
-        x.rmax = std::min({x.m, x.n, cap1});
+        // the largest blocks (long rank chains) start with a larger capacity,
+        // so that few of them need the second pass, which runs them alone
+        const int cap = std::min(x.m, x.n) >= kAcaWide ? std::min(kAcaWideCap, o.max_rank) : cap1;
+        x.rmax = std::min({x.m, x.n, cap});
         if (h->fluid && !h->calderon) {
             double d2 = 0;
             for (int q = 0; q < 3; ++q) {
@@ -2239,13 +2242,21 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
         return (int64_t)op.blk[a].m * op.blk[a].n > (int64_t)op.blk[b].m * op.blk[b].n;
     });
     int* dlist = hm_upload(h, list);
+    // the list's prefix of wide blocks (sorted by m n, so min(m, n) >= kAcaWide
+    // only among the first ones), run on the side stream
+    auto wide_prefix = [&](const std::vector<int>& l) {
+        int nb = 0;
+        while (nb < (int)l.size() && std::min(op.blk[l[nb]].m, op.blk[l[nb]].n) >= kAcaWide) ++nb;
+        return nb;
+    };
     int* conv = (int*)hm_alloc(h, sizeof(int) * (nd + nl + 1));
     z32* uv1 = (z32*)ctx->alloc(sizeof(z32) * std::max<int64_t>(1, h->first_cap));
     z32* uv2 = nullptr;
     std::vector<int> hconv(nd + nl);
     try {
         prof_begin(&h->lane->pl, s, C_HACA);
-        launch_aca(s, dm, kind, h->par, h->perm, nl, dlist, op.dblk, uv1, conv, o.aca_tolerance);
+        launch_aca(s, h->lane->s2, h->lane->ev_ready, h->lane->ev_panel, dm, kind, h->par, h->perm, nl,
+                   wide_prefix(list), dlist, op.dblk, uv1, conv, o.aca_tolerance);
         prof_end(&h->lane->pl, s, C_HACA, 0.0, 0.0);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(op.blk.data(), op.dblk, sizeof(HBlk) * op.blk.size(), hipMemcpyDeviceToHost, s));
@@ -2265,12 +2276,15 @@ void hm_operator(bem_hmatrix h, HOp& op, int kind) {
                 redo.push_back(b);
             }
         }
+        std::stable_sort(redo.begin(), redo.end(), [&](int a, int b) {
+            return (int64_t)op.blk[a].m * op.blk[a].n > (int64_t)op.blk[b].m * op.blk[b].n;
+        });
         if (!redo.empty()) {
             uv2 = (z32*)ctx->alloc(sizeof(z32) * uoff2);
             HIPCHECK(hipMemcpyAsync(op.dblk, op.blk.data(), sizeof(HBlk) * op.blk.size(), hipMemcpyHostToDevice, s));
             int* dredo = hm_upload(h, redo);
-            launch_aca(s, dm, kind, h->par, h->perm, (int)redo.size(), dredo, op.dblk, uv2, conv,
-                       o.aca_tolerance);
+            launch_aca(s, h->lane->s2, h->lane->ev_ready, h->lane->ev_panel, dm, kind, h->par, h->perm,
+                       (int)redo.size(), wide_prefix(redo), dredo, op.dblk, uv2, conv, o.aca_tolerance);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipMemcpyAsync(op.blk.data(), op.dblk, sizeof(HBlk) * op.blk.size(), hipMemcpyDeviceToHost, s));
             HIPCHECK(hipMemcpyAsync(hconv.data(), conv, sizeof(int) * hconv.size(), hipMemcpyDeviceToHost, s));

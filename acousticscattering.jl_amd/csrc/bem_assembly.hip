@@ -724,8 +724,9 @@ void launch_hnear_self(hipStream_t s, const DevMesh& m, int kind, const HPar& p,
     }
 }
 
-// Block-wide reductions of the ACA kernel (256 threads = 4 waves).
+// Block-wide reductions of the ACA kernel (NW waves).
 // argmax of v >= 0 with ties to the smaller index; result in every thread.
+template <int NW>
 __device__ inline void aca_argmax(float& v, int& idx, float* sv, int* si) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -745,14 +746,15 @@ __device__ inline void aca_argmax(float& v, int& idx, float* sv, int* si) {
     v = sv[0];
     idx = si[0];
 #pragma unroll
-    for (int w = 1; w < 4; ++w)
+    for (int w = 1; w < NW; ++w)
         if (sv[w] > v || (sv[w] == v && si[w] < idx)) {
             v = sv[w];
             idx = si[w];
         }
 }
-// sums of NV doubles per thread, fixed order; results in every thread
-template <int NV>
+// sums of NV doubles per thread, fixed order (waves in pairs, then in
+// order); results in every thread
+template <int NW, int NV>
 __device__ inline void aca_sums(double (&x)[NV], double* sd) {
 #pragma unroll
     for (int q = 0; q < NV; ++q)
@@ -764,7 +766,12 @@ __device__ inline void aca_sums(double (&x)[NV], double* sd) {
         for (int q = 0; q < NV; ++q) sd[(threadIdx.x >> 6) * NV + q] = x[q];
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < NV; ++q) x[q] = (sd[q] + sd[NV + q]) + (sd[2 * NV + q] + sd[3 * NV + q]);
+    for (int q = 0; q < NV; ++q) {
+        double t = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w += 2) t += sd[w * NV + q] + sd[(w + 1) * NV + q];
+        x[q] = t;
+    }
 }
 
 // Partial-pivot adaptive cross approximation (Bebendorf, Numer. Math. 86
@@ -773,11 +780,14 @@ __device__ inline void aca_sums(double (&x)[NV], double* sd) {
 // pivot; residual column -> u; next pivot row = the largest |u| off the used
 // row.  Stops when |u| |v| <= tol ||S||_F, with the Frobenius norm of the
 // approximation updated exactly (cross terms included), or at the block's
-// rank capacity (conv = 0).
-template <int KIND>
-__global__ __launch_bounds__(256) void k_aca(DevMesh mesh, HParF p, const int* __restrict__ perm, const int* list,
-                                             HBlk* __restrict__ blks, z32* __restrict__ uv, int* __restrict__ conv,
-                                             float tol) {
+// rank capacity (conv = 0).  NW waves per block: 4, or 8 for the largest
+// blocks, whose rank steps stream every earlier U and V column (8 waves of
+// 194 registers fill a CU; 16 would spill).
+template <int KIND, int NW>
+__global__ __launch_bounds__(64 * NW) void k_aca(DevMesh mesh, HParF p, const int* __restrict__ perm, const int* list,
+                                                 HBlk* __restrict__ blks, z32* __restrict__ uv,
+                                                 int* __restrict__ conv, float tol) {
+    constexpr int NT = 64 * NW;
     const int bid = list ? list[blockIdx.x] : blockIdx.x;
     const HBlk b = blks[bid];
     const MeshView<float> v = view_of(mesh, 0.f);
@@ -785,9 +795,9 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, HParF p, const int* _
     z32* U = uv + b.off;
     z32* V = uv + b.off + (int64_t)b.rmax * b.m;
     __shared__ z32 coef[kAcaCap];
-    __shared__ float sv[4];
-    __shared__ int si[4];
-    __shared__ double sd[4 * 32];
+    __shared__ float sv[NW];
+    __shared__ int si[NW];
+    __shared__ double sd[NW * 32];
     double normS2 = 0;
     int r = 0, istar = 0, retries = 0, ok = 0;
     while (r < b.rmax) {
@@ -797,7 +807,7 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, HParF p, const int* _
         const int gi = perm[b.row_lo + istar];
         float bv = -1.f;
         int bj = 0;
-        for (int j = tid; j < b.n; j += 256) {
+        for (int j = tid; j < b.n; j += NT) {
             z32 a = hentry_aca<KIND>(v, nt, gi, perm[b.col_lo + j], p, b.sig);
             for (int l = 0; l < r; ++l) a = csub(a, cmul(coef[l], V[(int64_t)l * b.n + j]));
             V[(int64_t)r * b.n + j] = a;
@@ -807,7 +817,7 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, HParF p, const int* _
                 bj = j;
             }
         }
-        aca_argmax(bv, bj, sv, si);
+        aca_argmax<NW>(bv, bj, sv, si);
         if (!(bv > 0.f)) {
             // the residual vanishes on this row: try a few other rows first
             if (r == 0 && ++retries < 8 && retries < b.m) {
@@ -826,7 +836,7 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, HParF p, const int* _
         float bu = -1.f;
         int bi = 0;
         double acc[2] = {0.0, 0.0};  // |u|^2, |v|^2
-        for (int i = tid; i < b.m; i += 256) {
+        for (int i = tid; i < b.m; i += NT) {
             z32 a = hentry_aca<KIND>(v, nt, perm[b.row_lo + i], gj, p, b.sig);
             for (int l = 0; l < r; ++l) a = csub(a, cmul(U[(int64_t)l * b.m + i], coef[l]));
             U[(int64_t)r * b.m + i] = a;
@@ -837,20 +847,20 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, HParF p, const int* _
                 bi = i;
             }
         }
-        for (int j = tid; j < b.n; j += 256) {
+        for (int j = tid; j < b.n; j += NT) {
             const z32 a = cmul(V[(int64_t)r * b.n + j], pinv);
             V[(int64_t)r * b.n + j] = a;
             acc[1] += (double)a.x * a.x + (double)a.y * a.y;
         }
         __syncthreads();
-        aca_sums<2>(acc, sd);
+        aca_sums<NW, 2>(acc, sd);
         // cross terms 2 Re sum_{l<r} (u_l^H u_r)(v_l^H v_r), eight l at a time
         double cross = 0;
         for (int l0 = 0; l0 < r; l0 += 8) {
             double d[32];
 #pragma unroll
             for (int q = 0; q < 32; ++q) d[q] = 0;
-            for (int i = tid; i < b.m; i += 256) {
+            for (int i = tid; i < b.m; i += NT) {
                 const z32 ur = U[(int64_t)r * b.m + i];
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
@@ -860,7 +870,7 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, HParF p, const int* _
                         d[4 * q + 1] += (double)ul.x * ur.y - (double)ul.y * ur.x;
                     }
             }
-            for (int j = tid; j < b.n; j += 256) {
+            for (int j = tid; j < b.n; j += NT) {
                 const z32 vr = V[(int64_t)r * b.n + j];
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
@@ -870,7 +880,7 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, HParF p, const int* _
                         d[4 * q + 3] += (double)vl.x * vr.y - (double)vl.y * vr.x;
                     }
             }
-            aca_sums<32>(d, sd);
+            aca_sums<NW, 32>(d, sd);
 #pragma unroll
             for (int q = 0; q < 8; ++q) cross += d[4 * q] * d[4 * q + 2] - d[4 * q + 1] * d[4 * q + 3];
         }
@@ -880,7 +890,7 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, HParF p, const int* _
             ok = 1;
             break;
         }
-        aca_argmax(bu, bi, sv, si);
+        aca_argmax<NW>(bu, bi, sv, si);
         if (!(bu > 0.f)) {
             ok = 1;
             break;
@@ -893,10 +903,28 @@ __global__ __launch_bounds__(256) void k_aca(DevMesh mesh, HParF p, const int* _
     }
 }
 
-void launch_aca(hipStream_t s, const DevMesh& m, int kind, const HPar& p, const int* perm, int nlist, const int* list,
-                HBlk* blk, z32* uv, int* conv, double tol) {
+// Blocks list[0 .. nbig) (the largest; the list is sorted by m n) run with
+// 8 waves per block on stream `s_big`, concurrently with the rest (4 waves
+// per block) on `s`; `fork` and `join` order the two streams.
+void launch_aca(hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join, const DevMesh& m, int kind,
+                const HPar& p, const int* perm, int nlist, int nbig, const int* list, HBlk* blk, z32* uv, int* conv,
+                double tol) {
     if (nlist <= 0) return;
-#define BEM_ACA(K) hipLaunchKernelGGL((k_aca<K>), dim3(nlist), dim3(256), 0, s, m, hpar_f(p), perm, list, blk, uv, conv, (float)tol)
+    if (!s_big) nbig = 0;
+    nbig = std::min(nbig, nlist);
+    if (nbig > 0) {
+        (void)hipEventRecord(fork, s);
+        (void)hipStreamWaitEvent(s_big, fork, 0);
+    }
+#define BEM_ACA(K)                                                                                                 \
+    do {                                                                                                           \
+        if (nbig > 0)                                                                                              \
+            hipLaunchKernelGGL((k_aca<K, 8>), dim3(nbig), dim3(512), 0, s_big, m, hpar_f(p), perm, list, blk,      \
+                               uv, conv, (float)tol);                                                              \
+        if (nlist > nbig)                                                                                          \
+            hipLaunchKernelGGL((k_aca<K, 4>), dim3(nlist - nbig), dim3(256), 0, s, m, hpar_f(p), perm, list + nbig, \
+                               blk, uv, conv, (float)tol);                                                         \
+    } while (0)
     switch (kind) {
     case HK_BM: BEM_ACA(HK_BM); break;
     case HK_CBIE: BEM_ACA(HK_CBIE); break;
@@ -905,6 +933,10 @@ void launch_aca(hipStream_t s, const DevMesh& m, int kind, const HPar& p, const 
     default: BEM_ACA(HK_CBIE_RHS); break;
     }
 #undef BEM_ACA
+    if (nbig > 0) {
+        (void)hipEventRecord(join, s_big);
+        (void)hipStreamWaitEvent(s, join, 0);
+    }
 }
 
 }  // namespace bem

@@ -155,8 +155,14 @@ void launch_hnear_self(hipStream_t s, const DevMesh& m, int kind, const HPar& p,
                        const int* pj, const int64_t* dst, int64_t nself, const int* si, const int* sj,
                        const int64_t* self_dst, z32* val);
 // partial-pivot ACA of the listed low-rank blocks (all when list is null)
-void launch_aca(hipStream_t s, const DevMesh& m, int kind, const HPar& p, const int* perm, int nlist, const int* list,
-                HBlk* blk, z32* uv, int* conv, double tol);
+// Blocks list[0 .. nbig) (the largest) run on s_big with wider workgroups,
+// concurrently with the rest on s (fork / join: events ordering the streams).
+void launch_aca(hipStream_t s, hipStream_t s_big, hipEvent_t fork, hipEvent_t join, const DevMesh& m, int kind,
+                const HPar& p, const int* perm, int nlist, int nbig, const int* list, HBlk* blk, z32* uv, int* conv,
+                double tol);
+// ACA blocks with min(m, n) at or above this run on the wide workgroups and
+// start with rank capacity kAcaWideCap instead of 64.
+constexpr int kAcaWide = 4096, kAcaWideCap = 128;
 
 // ---- bem_nystrom.hip: curved-element Nystrom operators ----------------------
 // Interpolation basis of the density-interpolation correction at wavenumber
